@@ -1,0 +1,135 @@
+/*
+ * phgpu.h -- C-ABI of the MI355X (gfx950) implementation of PHASTA's
+ * compressible implicit hot path: ElmGMRs/AsIGMR/e3 assembly and the
+ * block-diagonal-preconditioned sparse GMRES SolGMRs.
+ *
+ * Every array argument keeps the reference's Fortran layout (column-major,
+ * node index fastest, 1-based connectivity values) so that the Fortran host
+ * can pass its own arrays through iso_c_binding unchanged:
+ *   y, ac, yold, acold (nshg,ndof)  global dof order (u,v,w,p,T)
+ *   res, rmes, Dy      (nshg,5)     equation order (continuity, mom x/y/z,
+ *                                   energy) = local dof order (p,u,v,w,T)
+ *   BDiag              (nshg,5,5)
+ *   lhsK               (25,nnz_tot) 5x5 block, column-major inside a block
+ *   colm (nshg+1), rowp (nnz_tot)   CSR (colm = row pointers, rowp = sorted
+ *                                   column indices; genadj.f:56-65)
+ * All entry points return 0 on success and a nonzero PHG_E* code otherwise
+ * (the reference aborts through error(); the Fortran shim converts a nonzero
+ * status into `call error('SolGMRs ','hip     ',code)`).  The library is not
+ * reentrant: one context per process and GPU, one host thread, one stream.
+ */
+#ifndef PHGPU_H
+#define PHGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PHG_OK = 0,
+  PHG_EINVAL = 1,     /* bad argument / unsupported flag or BC code        */
+  PHG_EHIP = 2,       /* HIP runtime error                                  */
+  PHG_ENOMEM = 3,     /* device or host allocation failed                   */
+  PHG_ENAN = 10,      /* tnanq: NaN or |x| > 1e10 in res or BDiag           */
+  PHG_EDIVERGE = 11,  /* rstatCheck: residual grew by more than 1e4         */
+  PHG_ECOLOR = 12     /* element colouring needs more than 64 colours      */
+};
+
+enum { PHG_FP64 = 0, PHG_MIXED = 1 };
+
+/* COMMON-block scalars read on the path (common.h:121-259); same field
+ * order as the CPU oracle's parameter block. */
+typedef struct phg_params {
+  double Rgas, gamma, gamma1, pr, mu;   /* /mmatpar/, datmat(1,2,1)         */
+  double dtsfct, taucfct, temper;       /* /genpar/, /outpar/               */
+  double almi, alfi, gami, Dtgl;        /* /timdat/                         */
+  double etol, epsM;                    /* epstol(1), /precis/              */
+  int Kspace, nGMRES, minIters;         /* /solpar/, /incomp/               */
+  int lhs, iprec;                       /* itrdrv.f:456-457                 */
+} phg_params;
+
+/* Mesh arrays as the Fortran host holds them (readnblk.f, genadj.f);
+ * copied to the device by phg_init and not retained. */
+typedef struct phg_mesh {
+  long long nshg, nel, nelb, nnz_tot;
+  int ndofBC, reserved;
+  const double* x;     /* (nshg,3)                                         */
+  const int* ien;      /* (nel,4)  interior tets, mien blocks concatenated  */
+  const int* ienb;     /* (nelb,4) wall tets, face nodes 1..3 (mienb)       */
+  const int* iBC;      /* (nshg)                                           */
+  const double* BC;    /* (nshg,ndofBC)                                    */
+  const int* iper;     /* (nshg)                                           */
+  const int* colm;     /* (nshg+1)                                         */
+  const int* rowp;     /* (nnz_tot)                                        */
+} phg_mesh;
+
+/* Solver statistics (the /itrpar/ outputs iKs, lGMRES, ntotGM read by
+ * rstat.f:95-139, plus the rstat/rstatCheck norms). */
+typedef struct phg_stats {
+  int iKs, lGMRES, ntotGM, converged;
+  double unorm0, epsnrm, echeck;
+  double totres[2];      /* rstat: ||res||/nshg, ||rmes||/nshg             */
+  double reslast;        /* rstatCheck memory across calls                 */
+  long long nnan, nlarge;
+  double ms_assembly, ms_solve, ms_total;
+  int refine_cycles;     /* mixed precision: outer refinement cycles       */
+  int reserved;
+} phg_stats;
+
+typedef struct phg_ctx phg_ctx;
+
+/* Set-up: copies mesh and BC data to the GPU, colours the elements, builds
+ * the device block-CSR layout and the element-to-block map that replaces
+ * sparseloc (fillsparse.f:243-270).  precision: PHG_FP64 or PHG_MIXED. */
+int phg_init(const phg_mesh* mesh, const phg_params* prm, int precision,
+             phg_ctx** out);
+int phg_finalize(phg_ctx* ctx);
+int phg_set_params(phg_ctx* ctx, const phg_params* prm);
+const char* phg_last_error(void);
+/* Informational: number of interior / boundary colours, device bytes. */
+int phg_info(phg_ctx* ctx, int* ncolors, int* ncolors_b, long long* dev_bytes);
+
+/* ElmGMRs(y,ac,x,shp,shgl,iBC,BC,shpb,shglb,res,rmes,BDiag,iper,ilwork,
+ *         lhsK,col,row,rerr)                      elmgmr.f:280-285
+ * Host arrays; lhsK may be NULL (it stays resident on the device). */
+int phg_elmgmrs(phg_ctx* ctx, const double* y, const double* ac, double* res,
+                double* rmes, double* BDiag, double* lhsK);
+
+/* SolGMRs(y,ac,yold,acold,x,iBC,BC,col,row,lhsk,res,BDiag,HBrg,eBrg,yBrg,
+ *         Rcos,Rsin,iper,ilwork,shp,shgl,shpb,shglb,Dy,rerr) solgmr.f:368
+ * Host arrays.  BDiag returns LU-factored, lhsK (if non-NULL) scaled, res
+ * preconditioned, rmes raw, Dy the increment consumed by itrCorrect. */
+int phg_solgmrs(phg_ctx* ctx, const double* y, const double* ac,
+                double* res, double* rmes, double* BDiag, double* lhsK,
+                double* Dy, phg_stats* st);
+
+/* Device-resident time stepping around the same call (itrdrv.f:398-599):
+ * the state stays in HBM between calls. */
+int phg_state_upload(phg_ctx* ctx, const double* yold, const double* acold);
+int phg_state_download(phg_ctx* ctx, double* y, double* ac, double* Dy);
+/* itrPredict + itrBC                            itrPC.f:56-121, itrbc.f */
+int phg_predict(phg_ctx* ctx);
+/* SolGMRs + itrCorrect + itrBC on the resident state  itrdrv.f:477-599 */
+int phg_newton_iteration(phg_ctx* ctx, phg_stats* st);
+
+/* Fine-grained entry points (unit parity; host arrays in/out). */
+/* SparseAp(iper,ilwork,iBC,col,row,lhsK,p)          sparseap.f:1     */
+int phg_sparse_ap(phg_ctx* ctx, const double* lhsK, double* p);
+/* i3LU(Diag,r,code), code 'LU_Fact ','forward ','backward' i3lu.f:1   */
+int phg_i3lu(phg_ctx* ctx, double* Diag, double* r, const char* code);
+/* Spsi3pre(BDiag,lhsK,col,row)                      spsi3pre.f:1     */
+int phg_spsi3pre(phg_ctx* ctx, const double* BDiag, double* lhsK);
+/* bc3per(iBC,res,iper,ilwork,nQs)                   bc3per.f:1       */
+int phg_bc3per(phg_ctx* ctx, double* res, int nQs);
+
+/* Kernel timing hook for the benchmark: run `reps` SpMV launches on the
+ * resident lhsK and return the mean launch time (HIP events on the
+ * library stream). */
+int phg_bench_spmv(phg_ctx* ctx, int reps, double* ms_per_launch,
+                   double* bytes_per_launch);
+int phg_bench_assembly(phg_ctx* ctx, int reps, double* ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

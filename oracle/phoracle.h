@@ -1,0 +1,108 @@
+/*
+ * phoracle.h -- CPU oracle for the compressible hot path (test
+ * infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py; never by the product path).
+ *
+ * A plain-C FP64 restatement of PHASTA's ElmGMRs -> AsIGMR -> e3 assembly
+ * and SolGMRs block-diagonal GMRES, following the reference operation order
+ * (file:line citations in phoracle.c).  Arrays use the reference's Fortran
+ * layouts: column-major, node index fastest, 1-based connectivity values.
+ */
+#ifndef PHORACLE_H
+#define PHORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_mesh {
+  long long nshg;     /* nodes incl. periodic slaves (numnp == nshg, p=1) */
+  long long nel;      /* interior tets                                    */
+  long long nelb;     /* boundary (wall) tets                             */
+  long long nnz_tot;  /* 5x5 blocks in lhsK                               */
+  int ndofBC;         /* second dimension of BC                           */
+  int ibksiz;         /* elements per block (input.config:255, 64)        */
+  const double* x;    /* (nshg,3)                                         */
+  const int* ien;     /* (nel,4)  1-based                                 */
+  const int* ienb;    /* (nelb,4) 1-based, face nodes first               */
+  const int* iBC;     /* (nshg)                                           */
+  const double* BC;   /* (nshg,ndofBC)                                    */
+  const int* iper;    /* (nshg) 1-based                                   */
+  const int* colm;    /* (nshg+1) 1-based row pointers                    */
+  const int* rowp;    /* (nnz_tot) 1-based sorted column indices          */
+} orc_mesh;
+
+typedef struct orc_params {
+  double Rgas, gamma, gamma1, pr, mu;   /* /mmatpar/, datmat(1,2,1)        */
+  double dtsfct, taucfct, temper;       /* /genpar/, /outpar/              */
+  double almi, alfi, gami, Dtgl;        /* /timdat/                        */
+  double etol, epsM;                    /* epstol(1), /precis/             */
+  int Kspace, nGMRES, minIters;         /* /solpar/, /incomp/              */
+  int lhs, iprec;                       /* per-call flags (itrdrv.f:456)   */
+} orc_params;
+
+typedef struct orc_stats {
+  int iKs;          /* Krylov vectors used in the last cycle              */
+  int lGMRES;       /* cycles - 1                                         */
+  int ntotGM;       /* running total of Krylov iterations                 */
+  int converged;    /* |e_{iKs+1}| <= epsnrm                              */
+  double unorm0;    /* ||L^-1 r|| at entry                                */
+  double epsnrm;    /* etol * unorm0                                      */
+  double echeck;    /* |e_{iKs+1}| at exit                                */
+  double totres[2]; /* rstat: sqrt(sum res^2)/nshg for res and rmes       */
+  double reslast;   /* rstatCheck memory (input/output)                   */
+  long long nnan;   /* tnanq counts over res and BDiag                    */
+  long long nlarge;
+} orc_stats;
+
+/* quadrature/shape tables for linear tets (symtet.c, symtri.c, genint.f,
+ * genshp.f, genshpb.f, shptet.c).  shp[q][a], shgl[q][j][a], Qwt[q];
+ * boundary: shpb[q][a], shglb[q][j][a], Qwtb[q] (3 points). */
+void orc_quadrature(double shp[4][4], double shgl[4][3][4], double Qwt[4],
+                    double shpb[3][4], double shglb[3][3][4], double Qwtb[3]);
+
+/* e3 for one element; yl/acl local order (p,u,v,w,T).  rl[a][f] and
+ * EGmass[5a+f][5b+g] are overwritten (not accumulated). */
+void orc_e3(const orc_params* pr, const double yl[4][5],
+            const double acl[4][5], const double xl[4][3], double rl[4][5],
+            double EGmass[20][20], int want_lhs);
+
+/* e3b for one wall element (iBCB = 0: floating fluxes). rl overwritten. */
+void orc_e3b(const orc_params* pr, const double yl[4][5],
+             const double xl[4][3], double rl[4][5]);
+
+/* ElmGMRs (elmgmr.f:280-612).  res,rmes (nshg,5); BDiag (nshg,5,5);
+ * lhsK (25,nnz_tot).  lhsK/BDiag touched only when pr->lhs / pr->iprec. */
+int orc_elmgmrs(const orc_mesh* m, const orc_params* pr, const double* y,
+                const double* ac, double* res, double* rmes, double* BDiag,
+                double* lhsK);
+
+/* i3LU (i3lu.f): code 0 = 'LU_Fact ', 1 = 'forward ', 2 = 'backward',
+ * 3 = 'product '. */
+int orc_i3lu(long long nshg, double* Diag, double* r, int code);
+void orc_spsi3pre(long long nshg, const double* BDiag, double* lhsK,
+                  const int* colm, const int* rowp);
+void orc_sparse_ap(long long nshg, const int* iper, const int* colm,
+                   const int* rowp, const double* lhsK, double* p);
+void orc_bc3per(long long nshg, const int* iBC, const int* iper, double* res,
+                int nQs);
+
+/* SolGMRs (solgmr.f:368-744) including ElmGMRs.  Dy (nshg,5) output;
+ * res returns the preconditioned residual, rmes the raw one. */
+int orc_solgmrs(const orc_mesh* m, const orc_params* pr, const double* y,
+                const double* ac, double* res, double* rmes, double* BDiag,
+                double* lhsK, double* Dy, orc_stats* st);
+
+/* itrPredict / itrCorrect / itrBC (itrPC.f:56-148, itrbc.f:1-198);
+ * y, ac, yold, acold are (nshg,5) in the global order (u,v,w,p,T). */
+void orc_itr_predict(long long nshg, const orc_params* pr, const double* yold,
+                     const double* acold, double* y, double* ac);
+void orc_itr_correct(long long nshg, const orc_params* pr, double* y,
+                     double* ac, const double* yold, const double* acold,
+                     const double* Dy);
+int orc_itr_bc(const orc_mesh* m, double* y, double* ac);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,48 @@
+"""In-tree build of the native libraries.
+
+* ``phasta_amd/libphmesh.so``  host-only box-mesh generator (g++/OpenMP)
+* ``phasta_amd/libphgpu.so``   the HIP library behind ``include/phgpu.h``,
+  gfx950 only, packed-FP32 instructions disabled
+"""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "phasta_amd")
+CSRC = os.path.join(PKG, "csrc")
+INC = os.path.join(ROOT, "include")
+
+HIP_SOURCES = ["phg_api.hip", "k_assembly.hip", "k_solver.hip"]
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-fopenmp", f"-I{INC}"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_mesh():
+    _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", f"-I{INC}",
+          os.path.join(CSRC, "boxmesh.cpp"), "-o", os.path.join(PKG, "libphmesh.so")])
+
+
+def build_gpu():
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    out = os.path.join(PKG, "libphgpu.so")
+    tmp = out + ".tmp"
+    _run([hipcc] + HIPCC_FLAGS + [os.path.join(CSRC, s) for s in HIP_SOURCES] + ["-o", tmp])
+    os.replace(tmp, out)
+
+
+def build_all():
+    build_mesh()
+    build_gpu()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "mesh": build_mesh, "gpu": build_gpu}[w]()

@@ -1,0 +1,68 @@
+"""Frozen flag set and synthetic case definitions for the hot path.
+
+Every value that the reference reads from ``solver.inp``/``input.config`` on
+this path is pinned here once (reference defaults unless noted):
+
+* calorically perfect gas, ``ipress = 0`` (getthm.f:106-172); gamma = 1.4;
+  Rgas = 1/(0.79/Rs(N2) + 0.21/Rs(O2)) with Rs = Rh/M (input.f:165-182,
+  common.f:100-112): 288.30... J/(kg K)
+* constant viscosity mu = 1.8e-5 Pa s (matflg(2)=0, getdiff.f:126-128),
+  Stokes bulk viscosity (matflg(3)=0), Prandtl 0.72 (input.config:113)
+* Navier = 1, ipord = 1 linear tets, 4-point interior / 3-point wall rule
+* itau = 0 (Shakib diagonal tau), dtsfct = taucfct = temper = 1
+  (input.config:233-266), iDC = 0, idiff = 0 ("Include Viscous Correction
+  in Stabilization: False"), iLHScond = 0
+* backward Euler ("First Order": almi = alfi = gami = 1, ipred = 1,
+  itrPC.f:16-20); LHSupd = 1 so lhs = iprec = 1 each Newton iteration
+* nGMRES = 1, Kspace 10 (16^3 case) or 20, epstol 1e-3, minIters 0
+  (reference default 10; set explicitly and identically in every arm)
+* wall thermal condition: floating flux (iBCB = 0, e3b.f:149-228)
+* time step: dt = CFL * h / (c0 (1 + M)) with CFL = 1 and h = 1/N
+  (10-11 block-diagonal GMRES iterations per solve on the 16^3 box, the
+  range of the reference's legacy compressible case)
+"""
+import math
+
+RH = 8.31441000000000e+0          # common.f:104
+MSH = (2.8e-2, 3.2e-2)            # common.f:108 (N2, O2)
+XN2, XO2 = 0.79, 0.21             # common.f:106-107
+RS = (RH / MSH[0], RH / MSH[1])
+RGAS = 1.0 / (XN2 / RS[0] + XO2 / RS[1])   # input.f:182
+GAMMA = 1.4
+GAMMA1 = 0.4
+MU = 1.8e-5
+PRANDTL = 0.72
+T0 = 300.0
+C0 = math.sqrt(GAMMA * RGAS * T0)
+EPSM = math.sqrt(2.220446049250313e-16)   # input.f:30 sqrt(epsilon(one))
+CFL = 1.0
+IBKSIZ = 64
+
+MESH_SEED = 20261202
+FIELD_SEED = 7
+
+# Synthetic workloads (unit box, Kuhn 6-tet split, periodic x/z, walls y).
+CASES = {
+    "box16": dict(N=16, mach=0.3, jitter=0.1, grade=1.0, kspace=10,
+                  newton=1, steps=1, precision="fp64"),
+    "box64": dict(N=64, mach=0.5, jitter=0.1, grade=1.0, kspace=20,
+                  newton=3, steps=1, precision="fp64"),
+    "box128": dict(N=128, mach=0.3, jitter=0.1, grade=1.1, kspace=20,
+                   newton=3, steps=1, precision="mixed"),
+    "box256": dict(N=256, mach=0.6, jitter=0.1, grade=1.08, kspace=20,
+                   newton=3, steps=5, precision="mixed"),
+}
+
+
+def time_step(N, mach):
+    return CFL * (1.0 / N) / (C0 * (1.0 + mach))
+
+
+def solver_params(N, mach, kspace, etol=1e-3, min_iters=0):
+    """The COMMON-block scalars the path reads, as a plain dict."""
+    dt = time_step(N, mach)
+    return dict(Rgas=RGAS, gamma=GAMMA, gamma1=GAMMA1, pr=PRANDTL, mu=MU,
+                dtsfct=1.0, taucfct=1.0, temper=1.0,
+                almi=1.0, alfi=1.0, gami=1.0, Dtgl=1.0 / dt,
+                etol=etol, epsM=EPSM, Kspace=kspace, nGMRES=1,
+                minIters=min_iters, lhs=1, iprec=1)

@@ -1,0 +1,674 @@
+// Element assembly on gfx950: residual, 5x5 block tangent and block
+// diagonal for linear tets (the work of ElmGMRs -> AsIGMR -> e3 and
+// AsBMFG -> e3b, elmgmr.f:280-612), plus the nodal Dirichlet/periodic
+// constraints on res and BDiag (bc3res.f, bc3bdg.f).
+//
+// One element per lane.  Elements are processed colour by colour; no two
+// elements of a colour share a node, so every scatter into res, BDiag and
+// the block-CSR values is a plain read-modify-write without atomics and the
+// result is deterministic.  For affine tets the metrics, shape gradients and
+// Y gradients are constant over the element, so they are computed once and
+// only the interpolated state, thermodynamics, A-matrices and tau vary per
+// quadrature point.  The tangent block (a,b) is formed per quadrature point
+// as
+//     WdetJ [ P Q + K_ab ],   P = A(s_a) diag(tau) + N_a I,
+//                             Q = A(s_b) + c N_b A0,   c = almi/(gami alfi) Dtgl
+// with A(v) = sum_i v_i A_i the flux Jacobian along v, s_a = grad N_a and
+// K_ab = sum_ij s_a,i K_ij s_b,j the viscous diffusivity contracted with the
+// shape gradients.  This is the sum of the reference's four contributions
+// (e3conv N_a A_i N_b,i; e3LS time term N_a,i A_i tau A0 N_b; e3massl
+// N_a N_b A0; e3wmlt N_a,i (K_ij + A_i tau A_j) N_b,j), regrouped so that no
+// 15x15 stiffness array is ever formed.
+#include "phg_internal.h"
+
+namespace {
+
+struct Quad {
+  double N[4][4];   // interior shape values N[q][a]
+  double Nb[3][4];  // wall-face shape values
+  double Qwt, Qwtb;
+};
+
+__constant__ Quad cQ;
+
+// Constraint and element data are read through these pointers.
+struct AsmArgs {
+  const int* ien;          // 4 planes of nel (colour order, 0-based)
+  const uint32_t* bmap;    // 16 planes of nel
+  const uint8_t* bcm;
+  const double* y;         // 5 planes, global order (u,v,w,p,T)
+  const double* ac;
+  const double* x;         // 3 planes
+  double* res;             // 5 planes, local order
+  double* bdiag;           // 25 planes
+  void* lhs;               // slot values (T)
+  long long nshg, nel;
+  int p0, p1;
+  int lhs_on, prec_on;
+  PhysParams ph;
+};
+
+__device__ inline double sel4(double v0, double v1, double v2, double v3, int i) {
+  return i == 0 ? v0 : (i == 1 ? v1 : (i == 2 ? v2 : v3));
+}
+__device__ inline int sel4i(int v0, int v1, int v2, int v3, int i) {
+  return i == 0 ? v0 : (i == 1 ? v1 : (i == 2 ? v2 : v3));
+}
+
+struct Thermo {
+  double u[3], rho, drdp, drdT, e1p, e2p, e3p, e4p, tau[3];
+};
+
+__device__ inline void gather_local(const double* g, long long nshg, int n, double* v) {
+  // global (u,v,w,p,T) -> local (p,u,v,w,T)   (localy.f:53-73)
+  v[1] = g[n];
+  v[2] = g[n + nshg];
+  v[3] = g[n + 2 * nshg];
+  v[0] = g[n + 3 * nshg];
+  v[4] = g[n + 4 * nshg];
+}
+
+// A_i v along coordinate direction i (v in local order)
+__device__ inline void Ai_times(const Thermo& t, int i, const double v[5], double r[5]) {
+  const double ui = t.u[i];
+  const double uv = t.u[0] * v[1] + t.u[1] * v[2] + t.u[2] * v[3];
+  r[0] = t.drdp * ui * v[0] + t.rho * v[1 + i] + t.drdT * ui * v[4];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+    r[1 + m] = (t.drdp * t.u[m] * ui + (m == i ? 1.0 : 0.0)) * v[0] +
+               t.rho * (t.u[m] * v[1 + i] + ui * v[1 + m]) + t.drdT * t.u[m] * ui * v[4];
+  r[4] = ui * t.e2p * v[0] + t.e3p * v[1 + i] + t.rho * ui * uv + ui * t.e4p * v[4];
+}
+
+__device__ inline void A0_times(const Thermo& t, const double a[5], double r[5]) {
+  r[0] = t.drdp * a[0] + t.drdT * a[4];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) r[1 + m] = t.drdp * t.u[m] * a[0] + t.rho * a[1 + m] + t.drdT * t.u[m] * a[4];
+  r[4] = t.e1p * a[0] + t.rho * (t.u[0] * a[1] + t.u[1] * a[2] + t.u[2] * a[3]) + t.e4p * a[4];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_assemble(AsmArgs A) {
+  const int p = A.p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.p1) return;
+  const long long nshg = A.nshg, nel = A.nel;
+  int nd[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) nd[a] = A.ien[a * nel + p];
+
+  double yl[4][5], acl[4][5], xl[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    gather_local(A.y, nshg, nd[a], yl[a]);
+    gather_local(A.ac, nshg, nd[a], acl[a]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xl[a][c] = A.x[nd[a] + c * nshg];
+  }
+  const PhysParams& ph = A.ph;
+
+  // ---- metrics: dx/dxi = (x_j - x_4)/2 for j = 1..3 (shgl = dN/2) ----
+  double d[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d[i][j] = 0.5 * (xl[j][i] - xl[3][i]);
+  double ix[3][3];  // dxi/dx
+  ix[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
+  ix[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
+  ix[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  const double det = ix[0][0] * d[0][0] + ix[0][1] * d[1][0] + ix[0][2] * d[2][0];
+  const double idet = 1.0 / det;
+  ix[0][0] *= idet;
+  ix[0][1] *= idet;
+  ix[0][2] *= idet;
+  ix[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * idet;
+  ix[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * idet;
+  ix[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * idet;
+  ix[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * idet;
+  ix[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * idet;
+  ix[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * idet;
+  const double WdetJ = cQ.Qwt * det;
+  // shape gradients: node j<3 -> 0.5 * row j of dxi/dx, node 3 -> -(sum)
+  double shg[4][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    shg[0][k] = 0.5 * ix[0][k];
+    shg[1][k] = 0.5 * ix[1][k];
+    shg[2][k] = 0.5 * ix[2][k];
+    shg[3][k] = -0.5 * (ix[0][k] + ix[1][k] + ix[2][k]);
+  }
+  // metric tensor with the tet invariance correction (e3tau.f:1442-1474)
+  double G[6];
+  {
+    const double c1 = 1.259921049894873e+00, c2 = 6.299605249474365e-01;
+    double t[3][3];  // t[k][j] = sum_l M_kl ix[l][j], M = c2 + (c1-c2) I
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      t[0][j] = c1 * ix[0][j] + c2 * (ix[1][j] + ix[2][j]);
+      t[1][j] = c1 * ix[1][j] + c2 * (ix[0][j] + ix[2][j]);
+      t[2][j] = c1 * ix[2][j] + c2 * (ix[0][j] + ix[1][j]);
+    }
+    G[0] = ix[0][0] * t[0][0] + ix[1][0] * t[1][0] + ix[2][0] * t[2][0];
+    G[1] = ix[0][0] * t[0][1] + ix[1][0] * t[1][1] + ix[2][0] * t[2][1];
+    G[2] = ix[0][1] * t[0][1] + ix[1][1] * t[1][1] + ix[2][1] * t[2][1];
+    G[3] = ix[0][0] * t[0][2] + ix[1][0] * t[1][2] + ix[2][0] * t[2][2];
+    G[4] = ix[0][1] * t[0][2] + ix[1][1] * t[1][2] + ix[2][1] * t[2][2];
+    G[5] = ix[0][2] * t[0][2] + ix[1][2] * t[1][2] + ix[2][2] * t[2][2];
+  }
+  const double trG = G[0] + G[2] + G[5];
+  const double GG = G[0] * G[0] + G[2] * G[2] + G[5] * G[5] +
+                    2.0 * (G[1] * G[1] + G[3] * G[3] + G[4] * G[4]);
+  // gradients of Y, constant over the element
+  double g[3][5];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int f = 0; f < 5; ++f)
+      g[i][f] = shg[0][i] * yl[0][f] + shg[1][i] * yl[1][f] + shg[2][i] * yl[2][f] +
+                shg[3][i] * yl[3][f];
+
+  const double cv = ph.Rgas / ph.gamma1;
+  const double cp = ph.Rgas * ph.gamma / ph.gamma1;
+  const double mu = ph.mu, lam = -(2.0 / 3.0) * mu, kap = mu * cp / ph.pr;
+  const double l2m = lam + 2.0 * mu;
+  const double dts = ph.dtsfct * ph.Dtgl;
+  const double cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
+  const double divu = g[0][1] + g[1][2] + g[2][3];
+
+  Thermo th[4];
+  double rl[4][5];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int f = 0; f < 5; ++f) rl[a][f] = 0.0;
+
+  // ---------------- residual (ires = 1) ----------------
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double* Nq = cQ.N[q];
+    double Y[5], aci[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      Y[f] = Nq[0] * yl[0][f] + Nq[1] * yl[1][f] + Nq[2] * yl[2][f] + Nq[3] * yl[3][f];
+      aci[f] = Nq[0] * acl[0][f] + Nq[1] * acl[1][f] + Nq[2] * acl[2][f] + Nq[3] * acl[3][f];
+    }
+    const double pres = Y[0], Tq = Y[4];
+    Thermo& t = th[q];
+    t.u[0] = Y[1];
+    t.u[1] = Y[2];
+    t.u[2] = Y[3];
+    const double rk = 0.5 * (Y[1] * Y[1] + Y[2] * Y[2] + Y[3] * Y[3]);
+    t.rho = pres / (ph.Rgas * Tq);
+    const double ei = Tq * (ph.Rgas / ph.gamma1);
+    const double h = Tq * (ph.Rgas * ph.gamma / ph.gamma1);
+    const double alfap = 1.0 / Tq, betaT = 1.0 / pres;
+    t.drdp = t.rho * betaT;
+    t.drdT = -t.rho * alfap;
+    t.e1p = t.drdp * (h + rk) - alfap * Tq;
+    t.e2p = t.e1p + 1.0;
+    t.e3p = t.rho * (h + rk);
+    t.e4p = t.drdT * (h + rk) + t.rho * cp;
+    // Shakib diagonal tau (itau = 0, ipord = 1)
+    const double uGu = t.u[0] * (t.u[0] * G[0] + 2.0 * (t.u[1] * G[1] + t.u[2] * G[3])) +
+                       t.u[1] * (t.u[1] * G[2] + 2.0 * t.u[2] * G[4]) + t.u[2] * t.u[2] * G[5];
+    const double tt = t.rho * t.rho * ((2.0 * dts) * (2.0 * dts) + uGu) + 36.0 * (mu * mu) * GG;
+    const double sq = sqrt(tt);
+    t.tau[0] = 0.125 * sq / (t.rho * trG) * ph.taucfct;
+    t.tau[1] = 1.0 / sq;
+    t.tau[2] = t.tau[1] / cv * ph.temper;
+
+    // fluxes: ri[5i+f] = -F_i + F^v_i; ri[15+f] = A0 aci
+    double ri[20];
+    const double E = ei + rk;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double mui = -t.u[i] * t.rho;
+      ri[5 * i + 0] = mui;
+      double vf[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        vf[m] = mu * (g[i][1 + m] + g[m][1 + i]) + (m == i ? lam * divu : 0.0);
+        ri[5 * i + 1 + m] = mui * t.u[m] - (m == i ? pres : 0.0) + vf[m];
+      }
+      ri[5 * i + 4] = mui * E - t.u[i] * pres +
+                      (t.u[0] * vf[0] + t.u[1] * vf[1] + t.u[2] * vf[2] + kap * g[i][4]);
+    }
+    // least-squares residual L = A_i Y,i + A0 Y,t, scaled by tau
+    double L[5], tmp[5];
+    Ai_times(t, 0, g[0], L);
+    Ai_times(t, 1, g[1], tmp);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) L[f] += tmp[f];
+    Ai_times(t, 2, g[2], tmp);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) L[f] += tmp[f];
+    A0_times(t, aci, &ri[15]);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) L[f] += ri[15 + f];
+    L[0] *= t.tau[0];
+    L[1] *= t.tau[1];
+    L[2] *= t.tau[1];
+    L[3] *= t.tau[1];
+    L[4] *= t.tau[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      Ai_times(t, i, L, tmp);
+#pragma unroll
+      for (int f = 0; f < 5; ++f) ri[5 * i + f] += tmp[f];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int f = 0; f < 5; ++f)
+        rl[a][f] += WdetJ * (shg[a][0] * ri[f] + shg[a][1] * ri[5 + f] + shg[a][2] * ri[10 + f]) +
+                    Nq[a] * WdetJ * ri[15 + f];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int f = 0; f < 5; ++f) A.res[nd[a] + f * nshg] += rl[a][f];
+
+  if (!A.lhs_on && !A.prec_on) return;
+
+  // ---------------- tangent blocks ----------------
+  uint8_t msk[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) msk[a] = A.bcm[nd[a]];
+  T* lhs = reinterpret_cast<T*>(A.lhs);
+
+  for (int a = 0; a < 4; ++a) {
+    double s[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = sel4(shg[0][k], shg[1][k], shg[2][k], shg[3][k], a);
+    const int na = sel4i(nd[0], nd[1], nd[2], nd[3], a);
+    const uint8_t ma = (uint8_t)sel4i(msk[0], msk[1], msk[2], msk[3], a);
+    for (int b = 0; b < 4; ++b) {
+      if (!A.lhs_on && a != b) continue;
+      double r[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) r[k] = sel4(shg[0][k], shg[1][k], shg[2][k], shg[3][k], b);
+      const uint8_t mb = (uint8_t)sel4i(msk[0], msk[1], msk[2], msk[3], b);
+      const double st = s[0] * r[0] + s[1] * r[1] + s[2] * r[2];
+      double acc[5][5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const Thermo& t = th[q];
+        const double Na = cQ.N[q][a], Nb = cQ.N[q][b];
+        const double una = t.u[0] * s[0] + t.u[1] * s[1] + t.u[2] * s[2];
+        const double unb = t.u[0] * r[0] + t.u[1] * r[1] + t.u[2] * r[2];
+        const double w = unb + cfac * Nb;
+        double P[5][5], Q[5][5];
+        // P = A(s_a) diag(tau) + N_a I
+        P[0][0] = t.drdp * una * t.tau[0] + Na;
+        P[0][4] = t.drdT * una * t.tau[2];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) P[0][1 + n] = t.rho * s[n] * t.tau[1];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          P[1 + m][0] = (t.drdp * t.u[m] * una + s[m]) * t.tau[0];
+#pragma unroll
+          for (int n = 0; n < 3; ++n)
+            P[1 + m][1 + n] = t.rho * (t.u[m] * s[n] + (m == n ? una : 0.0)) * t.tau[1] + (m == n ? Na : 0.0);
+          P[1 + m][4] = t.drdT * t.u[m] * una * t.tau[2];
+        }
+        P[4][0] = t.e2p * una * t.tau[0];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) P[4][1 + n] = (t.e3p * s[n] + t.rho * t.u[n] * una) * t.tau[1];
+        P[4][4] = t.e4p * una * t.tau[2] + Na;
+        // Q = A(s_b) + c N_b A0
+        Q[0][0] = t.drdp * w;
+        Q[0][4] = t.drdT * w;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) Q[0][1 + n] = t.rho * r[n];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          Q[1 + m][0] = t.drdp * t.u[m] * w + r[m];
+#pragma unroll
+          for (int n = 0; n < 3; ++n) Q[1 + m][1 + n] = t.rho * (t.u[m] * r[n] + (m == n ? w : 0.0));
+          Q[1 + m][4] = t.drdT * t.u[m] * w;
+        }
+        Q[4][0] = t.e2p * unb + cfac * Nb * t.e1p;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) Q[4][1 + n] = t.e3p * r[n] + t.rho * t.u[n] * w;
+        Q[4][4] = t.e4p * w;
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+          for (int j = 0; j < 5; ++j)
+            acc[i][j] += P[i][0] * Q[0][j] + P[i][1] * Q[1][j] + P[i][2] * Q[2][j] +
+                         P[i][3] * Q[3][j] + P[i][4] * Q[4][j];
+        // viscous part K_ab
+        const double us = una, ut = unb;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int n = 0; n < 3; ++n)
+            acc[1 + m][1 + n] += mu * ((m == n ? st : 0.0) + s[n] * r[m]) + lam * s[m] * r[n];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) acc[4][1 + n] += mu * (st * t.u[n] + s[n] * ut) + lam * us * r[n];
+        acc[4][4] += kap * st;
+      }
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) acc[i][j] *= WdetJ;
+      // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103)
+      if (A.prec_on && a == b) {
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+#pragma unroll
+          for (int i = 0; i < 5; ++i) A.bdiag[na + (i + 5 * j) * nshg] += acc[i][j];
+      }
+      if (!A.lhs_on) continue;
+      // bc3LHS: constrained rows of node a, columns of node b, unit diagonal
+      if (ma | mb) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+          for (int j = 0; j < 5; ++j) {
+            if (((ma >> i) & 1) || ((mb >> j) & 1)) acc[i][j] = 0.0;
+            if (a == b && i == j && ((ma >> i) & 1)) acc[i][j] = 1.0;
+          }
+      }
+      const uint32_t qs = A.bmap[(4 * a + b) * nel + p];
+      T* dst = lhs + (size_t)(qs >> 6) * (25 * kGroup) + (qs & 63);
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) dst[(i + 5 * j) * kGroup] += (T)acc[i][j];
+    }
+  }
+}
+
+// Wall faces: floating convective, pressure, viscous and heat fluxes
+// (iBCB = 0), residual only (iLHScond = 0).  Nodes 1..3 lie on the face,
+// ordered so (x2-x1) x (x3-x1) points outward.
+struct WallArgs {
+  const int* ien;   // 4 planes of nelb, colour order
+  const double* y;
+  const double* x;
+  double* res;
+  long long nshg, nelb;
+  int p0, p1;
+  PhysParams ph;
+};
+
+__global__ void __launch_bounds__(64) k_wall(WallArgs A) {
+  const int p = A.p0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= A.p1) return;
+  const long long nshg = A.nshg, nelb = A.nelb;
+  const PhysParams& ph = A.ph;
+  int nd[4];
+  double yl[4][5], xl[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    nd[a] = A.ien[a * nelb + p];
+    gather_local(A.y, nshg, nd[a], yl[a]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xl[a][c] = A.x[nd[a] + c * nshg];
+  }
+  // volume metrics for the gradients
+  double d[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d[i][j] = 0.5 * (xl[j][i] - xl[3][i]);
+  double ix[3][3];
+  ix[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
+  ix[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
+  ix[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  const double idet = 1.0 / (ix[0][0] * d[0][0] + ix[0][1] * d[1][0] + ix[0][2] * d[2][0]);
+  ix[0][0] *= idet;
+  ix[0][1] *= idet;
+  ix[0][2] *= idet;
+  ix[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * idet;
+  ix[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * idet;
+  ix[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * idet;
+  ix[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * idet;
+  ix[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * idet;
+  ix[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * idet;
+  double shg[4][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    shg[0][k] = 0.5 * ix[0][k];
+    shg[1][k] = 0.5 * ix[1][k];
+    shg[2][k] = 0.5 * ix[2][k];
+    shg[3][k] = -0.5 * (ix[0][k] + ix[1][k] + ix[2][k]);
+  }
+  double g[3][5];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int f = 0; f < 5; ++f)
+      g[i][f] = shg[0][i] * yl[0][f] + shg[1][i] * yl[1][f] + shg[2][i] * yl[2][f] +
+                shg[3][i] * yl[3][f];
+  // outward unit normal and weighted area
+  double v1[3], v2[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    v1[c] = xl[1][c] - xl[0][c];
+    v2[c] = xl[2][c] - xl[0][c];
+  }
+  double nrm[3] = {v1[1] * v2[2] - v2[1] * v1[2], v2[0] * v1[2] - v1[0] * v2[2],
+                   v1[0] * v2[1] - v2[0] * v1[1]};
+  const double area2 = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  const double inv = 1.0 / area2;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) nrm[c] *= inv;
+  const double WdetJb = cQ.Qwtb * area2 * 0.25;
+
+  const double cp = ph.Rgas * ph.gamma / ph.gamma1;
+  const double mu = ph.mu, lam = -(2.0 / 3.0) * mu, kap = mu * cp / ph.pr;
+  const double divu = g[0][1] + g[1][2] + g[2][3];
+  // wall traction and heat flux use the volume gradients (constant)
+  double tn[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      acc += nrm[i] * (mu * (g[i][1 + m] + g[m][1 + i]) + (m == i ? lam * divu : 0.0));
+    tn[m] = acc;
+  }
+  const double heat = -kap * (nrm[0] * g[0][4] + nrm[1] * g[1][4] + nrm[2] * g[2][4]);
+
+  double rl[3][5];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int f = 0; f < 5; ++f) rl[a][f] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double* Nq = cQ.Nb[q];
+    double Y[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) Y[f] = Nq[0] * yl[0][f] + Nq[1] * yl[1][f] + Nq[2] * yl[2][f];
+    const double pres = Y[0], u1 = Y[1], u2 = Y[2], u3 = Y[3], Tq = Y[4];
+    const double rk = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+    const double rho = pres / (ph.Rgas * Tq);
+    const double ei = Tq * (ph.Rgas / ph.gamma1);
+    const double un = nrm[0] * u1 + nrm[1] * u2 + nrm[2] * u3;
+    const double rou = rho * un;
+    double F[5];
+    F[0] = rou;
+    F[1] = rou * u1 + nrm[0] * pres - tn[0];
+    F[2] = rou * u2 + nrm[1] * pres - tn[1];
+    F[3] = rou * u3 + nrm[2] * pres - tn[2];
+    F[4] = rou * (ei + rk) + un * pres - (u1 * tn[0] + u2 * tn[1] + u3 * tn[2]) + heat;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int f = 0; f < 5; ++f) rl[a][f] += WdetJb * Nq[a] * F[f];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int f = 0; f < 5; ++f) A.res[nd[a] + f * nshg] += rl[a][f];
+}
+
+// Dirichlet rows of res and BDiag (bc3res.f:25-153, bc3bdg.f:25-330)
+__global__ void k_bc_nodes(double* res, double* bdiag, const uint8_t* bcm, long long nshg,
+                           int prec) {
+  long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= nshg) return;
+  const uint8_t m = bcm[n];
+  if (!m) return;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    if (!((m >> f) & 1)) continue;
+    res[n + f * nshg] = 0.0;
+    if (prec) {
+#pragma unroll
+      for (int o = 0; o < 5; ++o) {
+        bdiag[n + (f + 5 * o) * nshg] = 0.0;
+        bdiag[n + (o + 5 * f) * nshg] = 0.0;
+      }
+      bdiag[n + (f + 5 * f) * nshg] = 1.0;
+    }
+  }
+}
+
+// Periodic fold: res(master) += res(slaves) in ascending slave order, slave
+// rows zeroed; BDiag(master) += BDiag(slaves), then slaves copy the master
+// (bc3res.f:157-163, bc3bdg.f:338-352).
+__global__ void k_fold_res_bdiag(double* res, double* bdiag, const int* master,
+                                 const int* ptr, const int* slave, int nfold,
+                                 long long nshg, int prec) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfold) return;
+  const long long m = master[i];
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const long long s = slave[k];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      res[m + f * nshg] += res[s + f * nshg];
+      res[s + f * nshg] = 0.0;
+    }
+  }
+  if (!prec) return;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const long long s = slave[k];
+#pragma unroll
+    for (int e = 0; e < 25; ++e) bdiag[m + e * nshg] += bdiag[s + e * nshg];
+  }
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const long long s = slave[k];
+#pragma unroll
+    for (int e = 0; e < 25; ++e) bdiag[s + e * nshg] = bdiag[m + e * nshg];
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T* p, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = T(0);
+}
+
+bool g_quad_ready = false;
+
+int upload_quadrature() {
+  if (g_quad_ready) return 0;
+  Quad h{};
+  const double a4 = 0.5854101966249685, b4 = 0.1381966011250150;
+  const double r4[4][3] = {{a4, b4, b4}, {b4, a4, b4}, {b4, b4, a4}, {b4, b4, b4}};
+  for (int q = 0; q < 4; ++q) {
+    h.N[q][0] = r4[q][0];
+    h.N[q][1] = r4[q][1];
+    h.N[q][2] = r4[q][2];
+    h.N[q][3] = 1.0 - r4[q][0] - r4[q][1] - r4[q][2];
+  }
+  const double A = 0.3333333333333330, B = 0.666666666666667, C = 0.166666666666667;
+  const double r3[3][3] = {{B, C, C}, {C, B, C}, {C, C, B}};
+  for (int q = 0; q < 3; ++q) {
+    h.Nb[q][0] = r3[q][0];
+    h.Nb[q][1] = r3[q][1];
+    h.Nb[q][2] = r3[q][2];
+    h.Nb[q][3] = 1.0 - r3[q][0] - r3[q][1] - r3[q][2];
+  }
+  h.Qwt = (4.0 / 3.0) * 0.25;
+  h.Qwtb = 2.0 * A;
+  PHG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cQ), &h, sizeof(Quad)));
+  g_quad_ready = true;
+  return 0;
+}
+
+PhysParams phys_of(const phg_params& p) {
+  return PhysParams{p.Rgas, p.gamma,   p.gamma1, p.pr,   p.mu,   p.dtsfct,
+                    p.taucfct, p.temper, p.almi, p.alfi, p.gami, p.Dtgl};
+}
+
+}  // namespace
+
+int launch_fill(phg_ctx* c, void* p, size_t bytes) {
+  size_t n = bytes / 4;
+  int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+  if (blocks < 1) return 0;
+  k_fill<float><<<blocks, 256, 0, c->stream>>>(reinterpret_cast<float*>(p), n);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
+  PHG_TRY(upload_quadrature());
+  AsmArgs A{};
+  A.ien = c->ienc;
+  A.bmap = c->bmap;
+  A.bcm = c->bcm;
+  A.y = c->y;
+  A.ac = c->ac;
+  A.x = c->x;
+  A.res = c->res;
+  A.bdiag = c->bdiag;
+  A.nshg = c->nshg;
+  A.nel = c->nel;
+  A.lhs_on = lhs;
+  A.prec_on = prec;
+  A.ph = phys_of(c->prm);
+  const bool mixed = c->precision == PHG_MIXED;
+  A.lhs = mixed ? (void*)c->lhs32 : (void*)c->lhs64;
+  for (size_t k = 0; k + 1 < c->col_off.size(); ++k) {
+    A.p0 = c->col_off[k];
+    A.p1 = c->col_off[k + 1];
+    int n = A.p1 - A.p0;
+    if (n <= 0) continue;
+    if (mixed)
+      k_assemble<float><<<(n + 63) / 64, 64, 0, c->stream>>>(A);
+    else
+      k_assemble<double><<<(n + 63) / 64, 64, 0, c->stream>>>(A);
+  }
+  PHG_HIP(hipGetLastError());
+  WallArgs W{};
+  W.ien = c->ienbc;
+  W.y = c->y;
+  W.x = c->x;
+  W.res = c->res;
+  W.nshg = c->nshg;
+  W.nelb = c->nelb;
+  W.ph = A.ph;
+  for (size_t k = 0; k + 1 < c->colb_off.size(); ++k) {
+    W.p0 = c->colb_off[k];
+    W.p1 = c->colb_off[k + 1];
+    int n = W.p1 - W.p0;
+    if (n <= 0) continue;
+    k_wall<<<(n + 63) / 64, 64, 0, c->stream>>>(W);
+  }
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_bc_res_bdiag(phg_ctx* c, bool prec) {
+  long long n = c->nshg;
+  k_bc_nodes<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->res, c->bdiag, c->bcm, n, prec);
+  PHG_HIP(hipGetLastError());
+  if (c->nfold > 0) {
+    k_fold_res_bdiag<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(
+        c->res, c->bdiag, c->fold_master, c->fold_ptr, c->fold_slave, c->nfold, n, prec);
+    PHG_HIP(hipGetLastError());
+  }
+  return 0;
+}

@@ -1,0 +1,523 @@
+// Linear-solver kernels on gfx950 for the block-diagonal-preconditioned
+// GMRES of SolGMRs (solgmr.f:368-744): 5x5 block LU of BDiag (i3lu.f),
+// two-sided block scaling of the matrix (spsi3pre.f), the block-CSR matvec
+// with the periodic gather (sparseap.f), the periodic fold (bc3per.f),
+// Krylov-vector kernels with deterministic wavefront/grid reductions
+// (sumgat, mpitools.f:104-138) and the itrCorrect/itrBC state update
+// (itrPC.f:128-148, itrbc.f).
+//
+// Matrix layout ("slots"): rows are grouped by 64, one wavefront per group.
+// Group g owns slots [grp_slot[g], grp_slot[g+1]); slot s holds one 5x5 block
+// for every row of the group, entry e (reference order f + 5 g) of row r at
+// vals[(s*25 + e)*64 + r].  Each matrix load of a wavefront is therefore one
+// contiguous 512-byte (FP64) or 256-byte (FP32) segment.  Rows shorter than
+// the group's slot count are padded with zero blocks.
+#include "phg_internal.h"
+
+namespace {
+
+constexpr int kRedThreads = 256;
+
+// ---------------- block LU (i3lu.f) ----------------
+// 25 planes, D[(i + 5 j) * n + node] = Diag(node, i+1, j+1)
+__global__ void k_lu_fact(double* D, long long n) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double a[5][5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) a[i][j] = D[(i + 5 * j) * n + k];
+  // Doolittle: unit-lower L, upper U with its diagonal stored inverted
+#pragma unroll
+  for (int p = 0; p < 5; ++p) {
+#pragma unroll
+    for (int j = p; j < 5; ++j) {
+      double s = a[p][j];
+#pragma unroll
+      for (int m = 0; m < p; ++m) s -= a[p][m] * a[m][j];
+      a[p][j] = s;
+    }
+    a[p][p] = 1.0 / a[p][p];
+#pragma unroll
+    for (int i = p + 1; i < 5; ++i) {
+      double s = a[i][p];
+#pragma unroll
+      for (int m = 0; m < p; ++m) s -= a[i][m] * a[m][p];
+      a[i][p] = a[p][p] * s;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) D[(i + 5 * j) * n + k] = a[i][j];
+}
+
+__global__ void k_lu_forward(const double* D, double* r, long long n) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double v[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) v[i] = r[i * n + k];
+#pragma unroll
+  for (int i = 1; i < 5; ++i) {
+    double s = v[i];
+#pragma unroll
+    for (int m = 0; m < i; ++m) s -= D[(i + 5 * m) * n + k] * v[m];
+    v[i] = s;
+  }
+#pragma unroll
+  for (int i = 1; i < 5; ++i) r[i * n + k] = v[i];
+}
+
+__global__ void k_lu_backward(const double* D, double* r, long long n) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double v[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) v[i] = r[i * n + k];
+#pragma unroll
+  for (int i = 4; i >= 0; --i) {
+    double s = v[i];
+#pragma unroll
+    for (int m = 4; m > i; --m) s -= v[m] * D[(i + 5 * m) * n + k];
+    v[i] = D[(i + 5 * i) * n + k] * s;
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) r[i * n + k] = v[i];
+}
+
+// ---------------- two-sided block scaling (spsi3pre.f) ----------------
+// lhsK(i,j) <- L_i^{-1} lhsK(i,j) U_j^{-1} with the factors of BDiag.
+template <typename T>
+__global__ void __launch_bounds__(64) k_spsi3pre(T* vals, const int* scol, const long long* grp_slot,
+                                                 const double* D, long long n) {
+  const long long g = blockIdx.x;
+  const int r = threadIdx.x;
+  const long long row = g * kGroup + r;
+  if (row >= n) return;
+  double lo[5][5];
+#pragma unroll
+  for (int i = 1; i < 5; ++i)
+#pragma unroll
+    for (int m = 0; m < i; ++m) lo[i][m] = D[(i + 5 * m) * n + row];
+  for (long long s = grp_slot[g]; s < grp_slot[g + 1]; ++s) {
+    const long long j = scol[s * kGroup + r];
+    T* blk = vals + s * (25 * kGroup) + r;
+    double b[5][5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+#pragma unroll
+      for (int f = 0; f < 5; ++f) b[f][c] = (double)blk[(f + 5 * c) * kGroup];
+    // left: L_i^{-1} on every column
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+#pragma unroll
+      for (int f = 1; f < 5; ++f) {
+        double acc = b[f][c];
+#pragma unroll
+        for (int m = 0; m < f; ++m) acc -= lo[f][m] * b[m][c];
+        b[f][c] = acc;
+      }
+    // right: U_j^{-1} on every row (columns in order)
+    double up[5][5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+#pragma unroll
+      for (int m = 0; m <= c; ++m) up[m][c] = D[(m + 5 * c) * n + j];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        double acc = b[f][c];
+#pragma unroll
+        for (int m = 0; m < c; ++m) acc -= up[m][c] * b[f][m];
+        b[f][c] = up[c][c] * acc;
+      }
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+#pragma unroll
+      for (int f = 0; f < 5; ++f) blk[(f + 5 * c) * kGroup] = (T)b[f][c];
+  }
+}
+
+// ---------------- block-CSR matvec (sparseap.f) ----------------
+// q = K p(iper(:)); one lane per row, one wavefront per 64-row group.
+template <typename T>
+__global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const int* __restrict__ scolp,
+                                              const long long* __restrict__ grp_slot,
+                                              const double* __restrict__ p, double* __restrict__ q,
+                                              long long n) {
+  const long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long g = row / kGroup;
+  const int r = (int)(row % kGroup);
+  if (row >= n) return;
+  double t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+  const long long s0 = grp_slot[g], s1 = grp_slot[g + 1];
+  for (long long s = s0; s < s1; ++s) {
+    const long long j = scolp[s * kGroup + r];
+    const double p0 = p[j], p1 = p[j + n], p2 = p[j + 2 * n], p3 = p[j + 3 * n], p4 = p[j + 4 * n];
+    const T* blk = vals + s * (25 * kGroup) + r;
+    double a[25];
+#pragma unroll
+    for (int e = 0; e < 25; ++e) a[e] = (double)blk[e * kGroup];
+    t0 += a[0] * p0 + a[5] * p1 + a[10] * p2 + a[15] * p3 + a[20] * p4;
+    t1 += a[1] * p0 + a[6] * p1 + a[11] * p2 + a[16] * p3 + a[21] * p4;
+    t2 += a[2] * p0 + a[7] * p1 + a[12] * p2 + a[17] * p3 + a[22] * p4;
+    t3 += a[3] * p0 + a[8] * p1 + a[13] * p2 + a[18] * p3 + a[23] * p4;
+    t4 += a[4] * p0 + a[9] * p1 + a[14] * p2 + a[19] * p3 + a[24] * p4;
+  }
+  q[row] = t0;
+  q[row + n] = t1;
+  q[row + 2 * n] = t2;
+  q[row + 3 * n] = t3;
+  q[row + 4 * n] = t4;
+}
+
+// bc3per: master += slaves (ascending), slaves zeroed
+__global__ void k_fold_vec(double* v, const int* master, const int* ptr, const int* slave,
+                           int nfold, long long n, int nq) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfold) return;
+  const long long m = master[i];
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const long long s = slave[k];
+    for (int f = 0; f < nq; ++f) {
+      v[m + f * n] += v[s + f * n];
+      v[s + f * n] = 0.0;
+    }
+  }
+}
+
+// ---------------- deterministic reductions ----------------
+// Each block reduces its share (wavefront shuffle, then LDS), writes one
+// partial; the last block to finish sums the partials in block order.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ inline bool block_reduce_and_finish(double v, double* part, unsigned* count,
+                                               double* out) {
+  __shared__ double ws[kRedThreads / 64];
+  __shared__ bool last;
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) ws[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) s += ws[w];
+    part[blockIdx.x] = s;
+    __threadfence();
+    unsigned t = atomicAdd(count, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double s = 0.0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+    s += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // fixed-order combination of the per-thread sums
+  s = wave_sum(s);
+  __syncthreads();
+  if (lane == 0) ws[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) t += ws[w];
+    *out = t;
+    *count = 0u;
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_dot(const double* a, const double* b, long long n,
+                                                     double* part, unsigned* count, double* out) {
+  double v = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    v += a[i] * b[i];
+  block_reduce_and_finish(v, part, count, out);
+}
+
+// MGS step j >= 2: u <- u - beta_{j-1} u_{j-1};  beta_j = (u, u_j)
+__global__ void __launch_bounds__(kRedThreads) k_mgs(double* u, const double* ujm1, const double* uj,
+                                                     const double* beta_prev, long long n,
+                                                     double* part, unsigned* count, double* out) {
+  const double beta = *beta_prev;
+  double v = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double w = u[i] - beta * ujm1[i];
+    u[i] = w;
+    v += w * (uj == u ? w : uj[i]);
+  }
+  block_reduce_and_finish(v, part, count, out);
+}
+
+// u = src / sqrt(*sq); norm_out <- sqrt(*sq) (written by one lane)
+__global__ void k_scale(double* u, const double* src, long long n, const double* sq,
+                        double* norm_out) {
+  const double nrm = sqrt(*sq);
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i == 0 && norm_out) *norm_out = nrm;
+  for (; i < n; i += (long long)gridDim.x * blockDim.x) u[i] = src[i] / nrm;
+}
+
+struct Coef {
+  double c[64];
+};
+
+// dy += sum_k c_k basis_k, k ascending (solgmr.f:690-692)
+__global__ void k_combine(double* dy, const double* basis, int k, Coef cf, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double v = dy[i];
+    for (int j = 0; j < k; ++j) v = v + cf.c[j] * basis[(size_t)j * n + i];
+    dy[i] = v;
+  }
+}
+
+// stats: [sum res^2, sum rmes^2, nan/large counts of res, BDiag]
+__global__ void __launch_bounds__(kRedThreads) k_stat_sumsq(const double* a, long long n, double* part,
+                                                            unsigned* count, double* out) {
+  double v = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    v += a[i] * a[i];
+  block_reduce_and_finish(v, part, count, out);
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_stat_bad(const double* a, long long n, double* part,
+                                                          unsigned* count, double* out) {
+  double v = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double x = a[i];
+    if (x != x || fabs(x) > 1.0e10) v += 1.0;
+  }
+  block_reduce_and_finish(v, part, count, out);
+}
+
+// ---------------- layout conversion ----------------
+template <typename T>
+__global__ void k_csr_to_slots(const double* ref, T* slots, const uint32_t* c2s, long long nnz) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t qs = c2s[k];
+  T* dst = slots + (size_t)(qs >> 6) * (25 * kGroup) + (qs & 63);
+#pragma unroll
+  for (int e = 0; e < 25; ++e) dst[e * kGroup] = (T)ref[k * 25 + e];
+}
+
+template <typename T>
+__global__ void k_slots_to_csr(const T* slots, double* ref, const uint32_t* c2s, long long nnz) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t qs = c2s[k];
+  const T* src = slots + (size_t)(qs >> 6) * (25 * kGroup) + (qs & 63);
+#pragma unroll
+  for (int e = 0; e < 25; ++e) ref[k * 25 + e] = (double)src[e * kGroup];
+}
+
+// ---------------- itrPredict / itrCorrect / itrBC ----------------
+__global__ void k_predict(const double* yold, const double* acold, double* y, double* ac,
+                          long long n5, double fac) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n5) return;
+  y[i] = yold[i];
+  ac[i] = acold[i] * fac;
+}
+
+__global__ void k_correct(double* y, double* ac, const double* yold, const double* acold,
+                          const double* dy, long long n, double fct1, double fct2) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  // y(:,1:3) -= Dy(:,2:4); y(:,4) -= Dy(:,1); y(:,5) -= Dy(:,5)
+  const int from[5] = {1, 2, 3, 0, 4};
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    const double v = y[k + f * n] - dy[k + from[f] * n];
+    y[k + f * n] = v;
+    ac[k + f * n] = acold[k + f * n] * fct1 + (v - yold[k + f * n]) * fct2;
+  }
+}
+
+// Dirichlet values on y (temperature, no-slip velocity, pressure)
+__global__ void k_bc_y(double* y, const uint8_t* bcm, const double* BC, long long n) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint8_t m = bcm[k];
+  if (!m) return;
+  if (m & 16) y[k + 4 * n] = BC[k + n];
+  if (m & 2) {
+    y[k] = BC[k + 2 * n];
+    y[k + n] = BC[k + 3 * n];
+    y[k + 2 * n] = BC[k + 4 * n];
+  }
+  if (m & 1) y[k + 3 * n] = BC[k];
+}
+
+// periodic copy y(slave) = y(master), ac likewise
+__global__ void k_per_copy(double* y, double* ac, const int* master, const int* ptr,
+                           const int* slave, int nfold, long long n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfold) return;
+  const long long m = master[i];
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const long long s = slave[k];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      y[s + f * n] = y[m + f * n];
+      ac[s + f * n] = ac[m + f * n];
+    }
+  }
+}
+
+inline int grid_for(long long n, int threads) {
+  long long b = (n + threads - 1) / threads;
+  return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+int launch_i3lu(phg_ctx* c, double* diag, double* r, int code) {
+  const long long n = c->nshg;
+  const int g = grid_for(n, 256);
+  if (code == 0)
+    k_lu_fact<<<g, 256, 0, c->stream>>>(diag, n);
+  else if (code == 1)
+    k_lu_forward<<<g, 256, 0, c->stream>>>(diag, r, n);
+  else if (code == 2)
+    k_lu_backward<<<g, 256, 0, c->stream>>>(diag, r, n);
+  else {
+    phg_set_error("i3LU: unsupported code %d", code);
+    return PHG_EINVAL;
+  }
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_spsi3pre(phg_ctx* c) {
+  if (c->precision == PHG_MIXED)
+    k_spsi3pre<float><<<(int)c->ngroups, kGroup, 0, c->stream>>>(c->lhs32, c->scol, c->grp_slot,
+                                                                 c->bdiag, c->nshg);
+  else
+    k_spsi3pre<double><<<(int)c->ngroups, kGroup, 0, c->stream>>>(c->lhs64, c->scol, c->grp_slot,
+                                                                  c->bdiag, c->nshg);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_spmv(phg_ctx* c, const double* p, double* q) {
+  const int g = grid_for(c->nshg, 256);
+  if (c->precision == PHG_MIXED)
+    k_spmv<float><<<g, 256, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg);
+  else
+    k_spmv<double><<<g, 256, 0, c->stream>>>(c->lhs64, c->scolp, c->grp_slot, p, q, c->nshg);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_bc3per(phg_ctx* c, double* v, int nq) {
+  if (c->nfold <= 0) return 0;
+  k_fold_vec<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(v, c->fold_master, c->fold_ptr,
+                                                            c->fold_slave, c->nfold, c->nshg, nq);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_dot(phg_ctx* c, const double* a, const double* b, long long n, double* out) {
+  k_dot<<<c->red_blocks, kRedThreads, 0, c->stream>>>(a, b, n, c->red_part, c->red_count, out);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_mgs_step(phg_ctx* c, double* u, const double* ujm1, const double* uj,
+                    const double* beta_prev, long long n, double* out) {
+  k_mgs<<<c->red_blocks, kRedThreads, 0, c->stream>>>(u, ujm1, uj, beta_prev, n, c->red_part,
+                                                      c->red_count, out);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_scale(phg_ctx* c, double* u, const double* src, long long n, const double* sq_ptr,
+                 double* norm_out) {
+  int g = (int)std::min<long long>(grid_for(n, 256), 16384);
+  k_scale<<<g, 256, 0, c->stream>>>(u, src, n, sq_ptr, norm_out);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_combine(phg_ctx* c, double* dy, const double* basis, int k, const double* coef,
+                   long long n) {
+  Coef cf{};
+  if (k > 64) {
+    phg_set_error("combine: Kspace %d > 64", k);
+    return PHG_EINVAL;
+  }
+  for (int j = 0; j < k; ++j) cf.c[j] = coef[j];
+  int g = (int)std::min<long long>(grid_for(n, 256), 16384);
+  k_combine<<<g, 256, 0, c->stream>>>(dy, basis, k, cf, n);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+// out[0] = sum res^2, out[1] = bad entries of res, out[2] = bad entries of BDiag
+int launch_stats(phg_ctx* c, double* out) {
+  const long long n5 = 5 * c->nshg;
+  k_stat_sumsq<<<c->red_blocks, kRedThreads, 0, c->stream>>>(c->res, n5, c->red_part,
+                                                             c->red_count, out);
+  k_stat_bad<<<c->red_blocks, kRedThreads, 0, c->stream>>>(c->res, n5, c->red_part, c->red_count,
+                                                           out + 1);
+  k_stat_bad<<<c->red_blocks, kRedThreads, 0, c->stream>>>(c->bdiag, 5 * n5, c->red_part,
+                                                           c->red_count, out + 2);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_csr_to_slots(phg_ctx* c, const double* ref, double* slots) {
+  k_csr_to_slots<double><<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(ref, slots, c->csr2slot,
+                                                                       c->nnz);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_slots_to_csr(phg_ctx* c, const double* slots, double* ref) {
+  k_slots_to_csr<double><<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(slots, ref, c->csr2slot,
+                                                                       c->nnz);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_predict(phg_ctx* c) {
+  const long long n5 = 5 * c->nshg;
+  const double fac = 1.0 - c->prm.almi / c->prm.gami;
+  k_predict<<<grid_for(n5, 256), 256, 0, c->stream>>>(c->yold, c->acold, c->y, c->ac, n5, fac);
+  PHG_HIP(hipGetLastError());
+  return launch_itrbc(c);
+}
+
+int launch_correct(phg_ctx* c) {
+  const double fct1 = 1.0 - c->prm.almi / c->prm.gami;
+  const double fct2 = c->prm.almi * c->prm.Dtgl / c->prm.gami / c->prm.alfi;
+  k_correct<<<grid_for(c->nshg, 256), 256, 0, c->stream>>>(c->y, c->ac, c->yold, c->acold, c->dy,
+                                                           c->nshg, fct1, fct2);
+  PHG_HIP(hipGetLastError());
+  return launch_itrbc(c);
+}
+
+int launch_itrbc(phg_ctx* c) {
+  k_bc_y<<<grid_for(c->nshg, 256), 256, 0, c->stream>>>(c->y, c->bcm, c->BCv, c->nshg);
+  PHG_HIP(hipGetLastError());
+  if (c->nfold > 0) {
+    k_per_copy<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(c->y, c->ac, c->fold_master,
+                                                              c->fold_ptr, c->fold_slave, c->nfold,
+                                                              c->nshg);
+    PHG_HIP(hipGetLastError());
+  }
+  return 0;
+}

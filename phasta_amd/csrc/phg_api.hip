@@ -1,0 +1,712 @@
+// C-ABI of libphgpu (include/phgpu.h): set-up of the device-resident mesh,
+// matrix layout and colouring, and the host-side sequencing of the
+// SolGMRs/ElmGMRs work on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "phg_internal.h"
+
+namespace {
+
+char g_err[1024] = "";
+
+template <typename T>
+int dev_alloc(phg_ctx* c, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) return 0;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    phg_set_error("hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    return PHG_ENOMEM;
+  }
+  c->dev_bytes += (long long)(count * sizeof(T));
+  return 0;
+}
+
+template <typename T>
+int upload(phg_ctx* c, T* dst, const T* src, size_t count) {
+  if (count == 0) return 0;
+  PHG_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+template <typename T>
+int download(phg_ctx* c, T* dst, const T* src, size_t count) {
+  if (count == 0) return 0;
+  PHG_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+int sync(phg_ctx* c) {
+  PHG_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Greedy first-fit colouring: no two elements of a colour share a node.
+// Returns colour offsets and the element order (ascending inside a colour).
+int colour_elements(long long nnode, long long nel, const int* ien /*1-based (nel,4)*/,
+                    std::vector<int>& offsets, std::vector<int>& order) {
+  std::vector<uint64_t> used(nnode, 0);
+  std::vector<uint8_t> col(nel);
+  int ncol = 0;
+  for (long long e = 0; e < nel; ++e) {
+    uint64_t m = 0;
+    for (int a = 0; a < 4; ++a) m |= used[ien[e + nel * a] - 1];
+    if (~m == 0) return PHG_ECOLOR;
+    int k = __builtin_ctzll(~m);
+    col[e] = (uint8_t)k;
+    for (int a = 0; a < 4; ++a) used[ien[e + nel * a] - 1] |= (1ull << k);
+    ncol = std::max(ncol, k + 1);
+  }
+  offsets.assign(ncol + 1, 0);
+  for (long long e = 0; e < nel; ++e) offsets[col[e] + 1]++;
+  for (int k = 0; k < ncol; ++k) offsets[k + 1] += offsets[k];
+  order.resize(nel);
+  std::vector<int> pos(offsets.begin(), offsets.end() - 1);
+  for (long long e = 0; e < nel; ++e) order[pos[col[e]]++] = (int)e;
+  return 0;
+}
+
+bool bc_code_supported(int ibc) {
+  const int v = (ibc >> 3) & 7;
+  if (ibc & 1) return false;          // density constraint
+  if (ibc & (1 << 11)) return false;  // scaled plane extraction
+  return v == 0 || v == 7;
+}
+
+int setup(phg_ctx* c, const phg_mesh* m) {
+  const long long nshg = m->nshg, nel = m->nel, nelb = m->nelb, nnz = m->nnz_tot;
+  // ---- validation ----
+  for (long long n = 0; n < nshg; ++n) {
+    if (!bc_code_supported(m->iBC[n])) {
+      phg_set_error("iBC(%lld)=%d: only pressure, temperature, no-slip velocity and periodic "
+                    "codes are supported on this path",
+                    n + 1, m->iBC[n]);
+      return PHG_EINVAL;
+    }
+    const int ip = m->iper[n] - 1;
+    if (ip < 0 || ip >= nshg || m->iper[ip] - 1 != ip) {
+      phg_set_error("iper(%lld) must point to a final master", n + 1);
+      return PHG_EINVAL;
+    }
+  }
+  for (long long i = 0; i < 4 * nel; ++i)
+    if (m->ien[i] < 1 || m->ien[i] > nshg) {
+      phg_set_error("ien entry %lld out of range", i);
+      return PHG_EINVAL;
+    }
+  for (long long i = 0; i < 4 * nelb; ++i)
+    if (m->ienb[i] < 1 || m->ienb[i] > nshg) {
+      phg_set_error("ienb entry %lld out of range", i);
+      return PHG_EINVAL;
+    }
+  for (long long k = 0; k < nnz; ++k)
+    if (m->rowp[k] < 1 || m->rowp[k] > nshg) {
+      phg_set_error("rowp entry %lld out of range", k);
+      return PHG_EINVAL;
+    }
+  for (long long r = 0; r < nshg; ++r)
+    if (m->colm[r + 1] < m->colm[r]) {
+      phg_set_error("colm not monotone at %lld", r);
+      return PHG_EINVAL;
+    }
+  if (m->colm[0] != 1 || m->colm[nshg] - 1 != nnz) {
+    phg_set_error("colm inconsistent with nnz_tot");
+    return PHG_EINVAL;
+  }
+  if (nshg >= (1LL << 31) || nnz >= (1LL << 31)) {
+    phg_set_error("mesh too large for 32-bit indices");
+    return PHG_EINVAL;
+  }
+
+  // ---- colouring ----
+  std::vector<int> order, orderb;
+  if (colour_elements(nshg, nel, m->ien, c->col_off, order) ||
+      colour_elements(nshg, nelb, m->ienb, c->colb_off, orderb)) {
+    phg_set_error("element colouring needs more than 64 colours");
+    return PHG_ECOLOR;
+  }
+
+  // ---- slot layout of the block-CSR matrix ----
+  c->ngroups = (nshg + kGroup - 1) / kGroup;
+  c->h_grp_slot.assign(c->ngroups + 1, 0);
+  for (long long g = 0; g < c->ngroups; ++g) {
+    int smax = 0;
+    for (long long r = g * kGroup; r < std::min(nshg, (g + 1) * kGroup); ++r)
+      smax = std::max(smax, m->colm[r + 1] - m->colm[r]);
+    c->h_grp_slot[g + 1] = c->h_grp_slot[g] + smax;
+  }
+  c->nslots = c->h_grp_slot[c->ngroups];
+  if (c->nslots * kGroup >= (1LL << 32)) {
+    phg_set_error("matrix too large for 32-bit slot indices");
+    return PHG_EINVAL;
+  }
+  std::vector<int> scol(c->nslots * kGroup), scolp(c->nslots * kGroup);
+  std::vector<uint32_t> c2s(nnz);
+#pragma omp parallel for schedule(static)
+  for (long long g = 0; g < c->ngroups; ++g) {
+    const long long S = c->h_grp_slot[g + 1] - c->h_grp_slot[g];
+    for (int r = 0; r < kGroup; ++r) {
+      const long long row = g * kGroup + r;
+      const long long b0 = row < nshg ? m->colm[row] - 1 : 0;
+      const long long len = row < nshg ? m->colm[row + 1] - m->colm[row] : 0;
+      for (long long s = 0; s < S; ++s) {
+        const long long q = (c->h_grp_slot[g] + s) * kGroup + r;
+        int col = row < nshg ? (int)row : 0;
+        if (s < len) {
+          col = m->rowp[b0 + s] - 1;
+          c2s[b0 + s] = (uint32_t)q;
+        }
+        scol[q] = col;
+        scolp[q] = m->iper[col] - 1;
+      }
+    }
+  }
+
+  // ---- element -> slot map (replaces the per-entry sparseloc search) ----
+  std::vector<int> ienc(4 * nel);
+  std::vector<uint32_t> bmap(16 * nel);
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (long long p = 0; p < nel; ++p) {
+    const long long e = order[p];
+    int nd[4];
+    for (int a = 0; a < 4; ++a) {
+      nd[a] = m->ien[e + nel * a] - 1;
+      ienc[a * nel + p] = nd[a];
+    }
+    for (int a = 0; a < 4; ++a) {
+      const int* lst = m->rowp + (m->colm[nd[a]] - 1);
+      const int len = m->colm[nd[a] + 1] - m->colm[nd[a]];
+      for (int b = 0; b < 4; ++b) {
+        const int* it = std::lower_bound(lst, lst + len, nd[b] + 1);
+        if (it == lst + len || *it != nd[b] + 1) {
+          bad = 1;
+          continue;
+        }
+        bmap[(4 * a + b) * nel + p] = c2s[(m->colm[nd[a]] - 1) + (it - lst)];
+      }
+    }
+  }
+  if (bad) {
+    phg_set_error("element adjacency missing from colm/rowp");
+    return PHG_EINVAL;
+  }
+  std::vector<int> ienbc(4 * std::max<long long>(nelb, 1));
+  for (long long p = 0; p < nelb; ++p)
+    for (int a = 0; a < 4; ++a) ienbc[a * nelb + p] = m->ienb[orderb[p] + nelb * a] - 1;
+
+  // ---- periodic fold lists ----
+  std::vector<std::vector<int>> sl(nshg);
+  std::vector<int> masters;
+  for (long long j = 0; j < nshg; ++j)
+    if (m->iBC[j] & 1024) {
+      const int i = m->iper[j] - 1;
+      if (i == j) continue;
+      if (sl[i].empty()) masters.push_back(i);
+      sl[i].push_back((int)j);
+    }
+  std::sort(masters.begin(), masters.end());
+  std::vector<int> fptr(1, 0), fsl;
+  for (int mm : masters) {
+    for (int s : sl[mm]) fsl.push_back(s);
+    fptr.push_back((int)fsl.size());
+  }
+  c->nfold = (int)masters.size();
+
+  std::vector<uint8_t> bcm(nshg);
+  std::vector<int> iper0(nshg);
+  for (long long n = 0; n < nshg; ++n) {
+    bcm[n] = constraint_mask(m->iBC[n]);
+    iper0[n] = m->iper[n] - 1;
+  }
+
+  // ---- device allocation ----
+  const int K = std::max(1, c->prm.Kspace);
+  c->kcap = K;
+  const bool mixed = c->precision == PHG_MIXED;
+  PHG_TRY(dev_alloc(c, &c->x, 3 * nshg));
+  PHG_TRY(dev_alloc(c, &c->ienc, 4 * nel));
+  PHG_TRY(dev_alloc(c, &c->bmap, 16 * nel));
+  PHG_TRY(dev_alloc(c, &c->ienbc, ienbc.size()));
+  PHG_TRY(dev_alloc(c, &c->bcm, nshg));
+  PHG_TRY(dev_alloc(c, &c->iper, nshg));
+  PHG_TRY(dev_alloc(c, &c->BCv, (size_t)nshg * m->ndofBC));
+  PHG_TRY(dev_alloc(c, &c->fold_master, std::max(1, c->nfold)));
+  PHG_TRY(dev_alloc(c, &c->fold_ptr, c->nfold + 1));
+  PHG_TRY(dev_alloc(c, &c->fold_slave, std::max<size_t>(1, fsl.size())));
+  PHG_TRY(dev_alloc(c, &c->grp_slot, c->ngroups + 1));
+  PHG_TRY(dev_alloc(c, &c->scol, c->nslots * kGroup));
+  PHG_TRY(dev_alloc(c, &c->scolp, c->nslots * kGroup));
+  PHG_TRY(dev_alloc(c, &c->csr2slot, nnz));
+  PHG_TRY(dev_alloc(c, &c->lhs64, c->nslots * kGroup * 25));
+  if (mixed) PHG_TRY(dev_alloc(c, &c->lhs32, c->nslots * kGroup * 25));
+  PHG_TRY(dev_alloc(c, &c->res, 5 * nshg));
+  PHG_TRY(dev_alloc(c, &c->rmes, 5 * nshg));
+  PHG_TRY(dev_alloc(c, &c->bdiag, 25 * nshg));
+  PHG_TRY(dev_alloc(c, &c->y, 5 * nshg));
+  PHG_TRY(dev_alloc(c, &c->ac, 5 * nshg));
+  PHG_TRY(dev_alloc(c, &c->yold, 5 * nshg));
+  PHG_TRY(dev_alloc(c, &c->acold, 5 * nshg));
+  PHG_TRY(dev_alloc(c, &c->dy, 5 * nshg));
+  PHG_TRY(dev_alloc(c, &c->krylov, (size_t)(K + 1) * 5 * nshg));
+  c->red_blocks = 1024;
+  PHG_TRY(dev_alloc(c, &c->red_part, c->red_blocks));
+  PHG_TRY(dev_alloc(c, &c->red_count, 1));
+  PHG_TRY(dev_alloc(c, &c->scal, 256));
+  PHG_HIP(hipHostMalloc((void**)&c->h_scal, 256 * sizeof(double), hipHostMallocDefault));
+
+  std::vector<double> xs(3 * nshg);
+  std::copy(m->x, m->x + 3 * nshg, xs.begin());
+  PHG_TRY(upload(c, c->x, xs.data(), xs.size()));
+  PHG_TRY(upload(c, c->ienc, ienc.data(), ienc.size()));
+  PHG_TRY(upload(c, c->bmap, bmap.data(), bmap.size()));
+  PHG_TRY(upload(c, c->ienbc, ienbc.data(), ienbc.size()));
+  PHG_TRY(upload(c, c->bcm, bcm.data(), bcm.size()));
+  PHG_TRY(upload(c, c->iper, iper0.data(), iper0.size()));
+  PHG_TRY(upload(c, c->BCv, m->BC, (size_t)nshg * m->ndofBC));
+  if (c->nfold > 0) {
+    PHG_TRY(upload(c, c->fold_master, masters.data(), masters.size()));
+    PHG_TRY(upload(c, c->fold_slave, fsl.data(), fsl.size()));
+  }
+  PHG_TRY(upload(c, c->fold_ptr, fptr.data(), fptr.size()));
+  PHG_TRY(upload(c, c->grp_slot, c->h_grp_slot.data(), c->h_grp_slot.size()));
+  PHG_TRY(upload(c, c->scol, scol.data(), scol.size()));
+  PHG_TRY(upload(c, c->scolp, scolp.data(), scolp.size()));
+  PHG_TRY(upload(c, c->csr2slot, c2s.data(), c2s.size()));
+  PHG_TRY(launch_fill(c, c->red_count, sizeof(unsigned)));
+  PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
+  if (mixed) PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
+  PHG_TRY(launch_fill(c, c->yold, sizeof(double) * 5 * nshg));
+  PHG_TRY(launch_fill(c, c->acold, sizeof(double) * 5 * nshg));
+  PHG_TRY(sync(c));
+  return 0;
+}
+
+int ensure_tmp(phg_ctx* c, size_t bytes) {
+  if (c->tmp_bytes >= bytes) return 0;
+  if (c->tmp) PHG_HIP(hipFree(c->tmp));
+  c->tmp = nullptr;
+  c->tmp_bytes = 0;
+  PHG_HIP(hipMalloc((void**)&c->tmp, bytes));
+  c->tmp_bytes = bytes;
+  return 0;
+}
+
+double* kvec(phg_ctx* c, int k) { return c->krylov + (size_t)k * 5 * c->nshg; }
+
+// ElmGMRs on the resident y/ac: res, rmes(=0), BDiag, lhsK (elmgmr.f:395-610)
+int elm_core(phg_ctx* c) {
+  const long long n = c->nshg;
+  const bool lhs = c->prm.lhs == 1, prec = c->prm.iprec != 0;
+  PHG_TRY(launch_fill(c, c->res, sizeof(double) * 5 * n));
+  if (prec) PHG_TRY(launch_fill(c, c->bdiag, sizeof(double) * 25 * n));
+  if (lhs) {
+    if (c->precision == PHG_MIXED)
+      PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
+    else
+      PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
+  }
+  PHG_TRY(launch_assembly(c, lhs, prec));
+  PHG_TRY(launch_bc_res_bdiag(c, prec));
+  return 0;
+}
+
+// SolGMRs on the resident y/ac (solgmr.f:449-744).  Leaves Dy in c->dy.
+int sol_core(phg_ctx* c, phg_stats* st) {
+  const long long n = c->nshg, V = 5 * n;
+  const int K = c->prm.Kspace;
+  if (K < 1 || K > c->kcap || K > 60) {
+    phg_set_error("Kspace %d outside 1..%d", K, std::min(c->kcap, 60));
+    return PHG_EINVAL;
+  }
+  if (c->prm.nGMRES != 1) {
+    phg_set_error("nGMRES must be 1 (restart branch of solgmr.f:526 is inactive)");
+    return PHG_EINVAL;
+  }
+  hipEvent_t ev[3];
+  for (auto& e : ev) PHG_HIP(hipEventCreate(&e));
+  PHG_HIP(hipEventRecord(ev[0], c->stream));
+  PHG_TRY(elm_core(c));
+  PHG_HIP(hipEventRecord(ev[1], c->stream));
+  PHG_HIP(hipMemcpyAsync(c->rmes, c->res, sizeof(double) * V, hipMemcpyDeviceToDevice, c->stream));
+  double* S = c->scal;  // [0] sum res^2 (after forward), [1],[2] bad counts, [3] unorm, [4] sum rmes^2
+  double* H = c->scal + 16;
+  // tnanq on res and BDiag before factorisation (solgmr.f:459-460)
+  PHG_TRY(launch_stats(c, S));  // S[0] provisional, overwritten below
+  if (c->prm.iprec != 0) PHG_TRY(launch_i3lu(c, c->bdiag, nullptr, 0));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->res, 1));
+  PHG_TRY(launch_dot(c, c->res, c->res, V, S));
+  PHG_TRY(launch_dot(c, c->rmes, c->rmes, V, S + 4));
+  PHG_TRY(launch_fill(c, c->dy, sizeof(double) * V));
+  if (c->prm.lhs == 1) PHG_TRY(launch_spsi3pre(c));
+  PHG_TRY(launch_scale(c, kvec(c, 0), c->res, V, S, S + 3));
+  PHG_TRY(download(c, c->h_scal, S, 8));
+  PHG_TRY(sync(c));
+  const double sumsq = c->h_scal[0];
+  st->nnan = 0;
+  st->nlarge = (long long)(c->h_scal[1] + c->h_scal[2]);
+  if (st->nlarge > 0) {
+    phg_set_error("tnanq: %lld NaN/large entries in res/BDiag", st->nlarge);
+    for (auto& e : ev) hipEventDestroy(e);
+    return PHG_ENAN;
+  }
+  // rstatCheck.f:36-79
+  const double totres_chk = std::sqrt(sumsq / (double)(float)n);
+  if (c->reslast > 0.0 && totres_chk > 10000.0 * c->reslast) {
+    phg_set_error("rstatCheck: residual diverging (%g > 1e4 * %g)", totres_chk, c->reslast);
+    for (auto& e : ev) hipEventDestroy(e);
+    return PHG_EDIVERGE;
+  }
+  c->reslast = totres_chk;
+  double unorm = c->h_scal[3];
+  st->unorm0 = unorm;
+  const double epsnrm = c->prm.etol * unorm;
+  st->epsnrm = epsnrm;
+  int iKs = 0;
+  double echeck = 0.0;
+  std::vector<double> Hh((size_t)(K + 2) * (K + 1), 0.0), e(K + 2, 0.0), Rc(K + 2, 0.0),
+      Rs(K + 2, 0.0), yB(K + 2, 0.0);
+  auto Hm = [&](int a, int b) -> double& { return Hh[(size_t)(a - 1) + (size_t)(K + 1) * (b - 1)]; };
+  if (!(unorm < 100. * c->prm.epsM * c->prm.epsM)) {
+    e[1] = unorm;
+    for (int iK = 1; iK <= K; ++iK) {
+      iKs = iK;
+      double* w = kvec(c, iK);
+      PHG_TRY(launch_spmv(c, kvec(c, iK - 1), w));
+      PHG_TRY(launch_bc3per(c, w, 5));
+      PHG_TRY(launch_dot(c, w, kvec(c, 0), V, H + 0));
+      for (int jK = 2; jK <= iK + 1; ++jK)
+        PHG_TRY(launch_mgs_step(c, w, kvec(c, jK - 2), jK == iK + 1 ? w : kvec(c, jK - 1), H + jK - 2,
+                                V, H + jK - 1));
+      PHG_TRY(launch_scale(c, w, w, V, H + iK, H + iK + 1));
+      PHG_TRY(download(c, c->h_scal + 16, H, iK + 2));
+      PHG_TRY(sync(c));
+      for (int jK = 1; jK <= iK; ++jK) Hm(jK, iK) = c->h_scal[16 + jK - 1];
+      Hm(iK + 1, iK) = c->h_scal[16 + iK + 1];
+      // Givens rotations on the new column (solgmr.f:645-663)
+      for (int jK = 1; jK <= iK - 1; ++jK) {
+        const double t = Rc[jK] * Hm(jK, iK) + Rs[jK] * Hm(jK + 1, iK);
+        Hm(jK + 1, iK) = -Rs[jK] * Hm(jK, iK) + Rc[jK] * Hm(jK + 1, iK);
+        Hm(jK, iK) = t;
+      }
+      const double t = std::sqrt(Hm(iK, iK) * Hm(iK, iK) + Hm(iK + 1, iK) * Hm(iK + 1, iK));
+      Rc[iK] = Hm(iK, iK) / t;
+      Rs[iK] = Hm(iK + 1, iK) / t;
+      Hm(iK, iK) = t;
+      Hm(iK + 1, iK) = 0.0;
+      const double t2 = Rc[iK] * e[iK] + Rs[iK] * e[iK + 1];
+      e[iK + 1] = -Rs[iK] * e[iK] + Rc[iK] * e[iK + 1];
+      e[iK] = t2;
+      c->ntotGM++;
+      echeck = std::fabs(e[iK + 1]);
+      if (echeck <= epsnrm && iK >= c->prm.minIters) break;
+    }
+    for (int jK = iKs; jK >= 1; --jK) {
+      yB[jK] = e[jK] / Hm(jK, jK);
+      for (int lK = 1; lK <= jK - 1; ++lK) e[lK] = e[lK] - yB[jK] * Hm(lK, jK);
+    }
+    PHG_TRY(launch_combine(c, c->dy, c->krylov, iKs, yB.data() + 1, V));
+    echeck = std::fabs(e[iKs + 1]);
+  }
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->dy, 2));
+  PHG_HIP(hipEventRecord(ev[2], c->stream));
+  PHG_TRY(download(c, c->h_scal + 4, S + 4, 1));
+  PHG_TRY(sync(c));
+  // rstat.f:95-116
+  st->totres[0] = std::sqrt(sumsq) / (double)(float)n;
+  st->totres[1] = std::sqrt(c->h_scal[4]) / (double)(float)n;
+  st->iKs = iKs;
+  st->lGMRES = 0;
+  st->ntotGM = c->ntotGM;
+  st->echeck = echeck;
+  st->converged = echeck <= epsnrm;
+  st->reslast = c->reslast;
+  float ms1 = 0, ms2 = 0;
+  hipEventElapsedTime(&ms1, ev[0], ev[1]);
+  hipEventElapsedTime(&ms2, ev[1], ev[2]);
+  st->ms_assembly = ms1;
+  st->ms_solve = ms2;
+  st->ms_total = ms1 + ms2;
+  for (auto& ev1 : ev) hipEventDestroy(ev1);
+  return 0;
+}
+
+int check_ctx(phg_ctx* c) {
+  if (!c) {
+    phg_set_error("null context");
+    return PHG_EINVAL;
+  }
+  return 0;
+}
+
+int require_fp64(phg_ctx* c, const char* who) {
+  if (c->precision != PHG_FP64) {
+    phg_set_error("%s: host-array matrix entry points need a PHG_FP64 context", who);
+    return PHG_EINVAL;
+  }
+  return 0;
+}
+
+}  // namespace
+
+void phg_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char* phg_last_error(void) { return g_err; }
+
+extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precision, phg_ctx** out) {
+  if (!mesh || !prm || !out) {
+    phg_set_error("phg_init: null argument");
+    return PHG_EINVAL;
+  }
+  *out = nullptr;
+  if (precision != PHG_FP64 && precision != PHG_MIXED) {
+    phg_set_error("phg_init: unknown precision %d", precision);
+    return PHG_EINVAL;
+  }
+  if (mesh->nshg <= 0 || mesh->nel <= 0 || mesh->nnz_tot <= 0 || mesh->ndofBC < 5) {
+    phg_set_error("phg_init: bad sizes");
+    return PHG_EINVAL;
+  }
+  phg_ctx* c = new phg_ctx();
+  c->nshg = mesh->nshg;
+  c->nel = mesh->nel;
+  c->nelb = mesh->nelb;
+  c->nnz = mesh->nnz_tot;
+  c->ndofBC = mesh->ndofBC;
+  c->precision = precision;
+  c->prm = *prm;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    phg_set_error("hipStreamCreate: %s", hipGetErrorString(e));
+    delete c;
+    return PHG_EHIP;
+  }
+  int rc = setup(c, mesh);
+  if (rc) {
+    std::string keep = g_err;
+    phg_finalize(c);
+    phg_set_error("%s", keep.c_str());
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" int phg_finalize(phg_ctx* c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->x,        c->ienc,     c->bmap,   c->ienbc,  c->bcm,       c->iper,
+                  c->BCv,      c->fold_master, c->fold_ptr, c->fold_slave, c->grp_slot,
+                  c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,
+                  c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
+                  c->acold,    c->dy,       c->krylov, c->red_part, c->red_count, c->scal,
+                  c->tmp};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (c->h_scal) hipHostFree(c->h_scal);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+extern "C" int phg_set_params(phg_ctx* c, const phg_params* prm) {
+  PHG_TRY(check_ctx(c));
+  if (!prm || prm->Kspace > c->kcap) {
+    phg_set_error("phg_set_params: Kspace may not grow beyond the value given to phg_init");
+    return PHG_EINVAL;
+  }
+  c->prm = *prm;
+  return 0;
+}
+
+extern "C" int phg_info(phg_ctx* c, int* ncolors, int* ncolors_b, long long* dev_bytes) {
+  PHG_TRY(check_ctx(c));
+  if (ncolors) *ncolors = (int)c->col_off.size() - 1;
+  if (ncolors_b) *ncolors_b = (int)c->colb_off.size() - 1;
+  if (dev_bytes) *dev_bytes = c->dev_bytes;
+  return 0;
+}
+
+extern "C" int phg_elmgmrs(phg_ctx* c, const double* y, const double* ac, double* res, double* rmes,
+                           double* BDiag, double* lhsK) {
+  PHG_TRY(check_ctx(c));
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->y, y, 5 * n));
+  PHG_TRY(upload(c, c->ac, ac, 5 * n));
+  PHG_TRY(elm_core(c));
+  PHG_TRY(download(c, res, c->res, 5 * n));
+  if (rmes) std::memset(rmes, 0, sizeof(double) * 5 * n);  // elmgmr.f:400
+  if (BDiag && c->prm.iprec != 0) PHG_TRY(download(c, BDiag, c->bdiag, 25 * n));
+  if (lhsK && c->prm.lhs == 1) {
+    PHG_TRY(require_fp64(c, "phg_elmgmrs"));
+    PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
+    PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
+    PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
+  }
+  return sync(c);
+}
+
+extern "C" int phg_solgmrs(phg_ctx* c, const double* y, const double* ac, double* res, double* rmes,
+                           double* BDiag, double* lhsK, double* Dy, phg_stats* st) {
+  PHG_TRY(check_ctx(c));
+  phg_stats local{};
+  if (!st) st = &local;
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->y, y, 5 * n));
+  PHG_TRY(upload(c, c->ac, ac, 5 * n));
+  PHG_TRY(sol_core(c, st));
+  if (res) PHG_TRY(download(c, res, c->res, 5 * n));
+  if (rmes) PHG_TRY(download(c, rmes, c->rmes, 5 * n));
+  if (BDiag) PHG_TRY(download(c, BDiag, c->bdiag, 25 * n));
+  if (Dy) PHG_TRY(download(c, Dy, c->dy, 5 * n));
+  if (lhsK) {
+    PHG_TRY(require_fp64(c, "phg_solgmrs"));
+    PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
+    PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
+    PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
+  }
+  return sync(c);
+}
+
+extern "C" int phg_state_upload(phg_ctx* c, const double* yold, const double* acold) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(upload(c, c->yold, yold, 5 * c->nshg));
+  PHG_TRY(upload(c, c->acold, acold, 5 * c->nshg));
+  return sync(c);
+}
+
+extern "C" int phg_state_download(phg_ctx* c, double* y, double* ac, double* Dy) {
+  PHG_TRY(check_ctx(c));
+  if (y) PHG_TRY(download(c, y, c->y, 5 * c->nshg));
+  if (ac) PHG_TRY(download(c, ac, c->ac, 5 * c->nshg));
+  if (Dy) PHG_TRY(download(c, Dy, c->dy, 5 * c->nshg));
+  return sync(c);
+}
+
+extern "C" int phg_predict(phg_ctx* c) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(launch_predict(c));
+  return 0;
+}
+
+extern "C" int phg_newton_iteration(phg_ctx* c, phg_stats* st) {
+  PHG_TRY(check_ctx(c));
+  phg_stats local{};
+  if (!st) st = &local;
+  PHG_TRY(sol_core(c, st));
+  PHG_TRY(launch_correct(c));
+  return 0;
+}
+
+extern "C" int phg_sparse_ap(phg_ctx* c, const double* lhsK, double* p) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(require_fp64(c, "phg_sparse_ap"));
+  const long long n = c->nshg;
+  if (lhsK) {
+    PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
+    PHG_TRY(upload(c, c->tmp, lhsK, 25 * c->nnz));
+    PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
+    PHG_TRY(launch_csr_to_slots(c, c->tmp, c->lhs64));
+  }
+  PHG_TRY(upload(c, kvec(c, 0), p, 5 * n));
+  PHG_TRY(launch_spmv(c, kvec(c, 0), kvec(c, 1)));
+  PHG_TRY(download(c, p, kvec(c, 1), 5 * n));
+  return sync(c);
+}
+
+extern "C" int phg_i3lu(phg_ctx* c, double* Diag, double* r, const char* code) {
+  PHG_TRY(check_ctx(c));
+  int k = -1;
+  if (code && !std::strncmp(code, "LU_Fact ", 8)) k = 0;
+  if (code && !std::strncmp(code, "forward ", 8)) k = 1;
+  if (code && !std::strncmp(code, "backward", 8)) k = 2;
+  if (k < 0) {
+    phg_set_error("i3LU: unknown code");  // i3lu.f:175 calls error()
+    return PHG_EINVAL;
+  }
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->bdiag, Diag, 25 * n));
+  if (k > 0) PHG_TRY(upload(c, c->res, r, 5 * n));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->res, k));
+  if (k == 0) PHG_TRY(download(c, Diag, c->bdiag, 25 * n));
+  if (k > 0) PHG_TRY(download(c, r, c->res, 5 * n));
+  return sync(c);
+}
+
+extern "C" int phg_spsi3pre(phg_ctx* c, const double* BDiag, double* lhsK) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(require_fp64(c, "phg_spsi3pre"));
+  PHG_TRY(upload(c, c->bdiag, BDiag, 25 * c->nshg));
+  PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
+  PHG_TRY(upload(c, c->tmp, lhsK, 25 * c->nnz));
+  PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
+  PHG_TRY(launch_csr_to_slots(c, c->tmp, c->lhs64));
+  PHG_TRY(launch_spsi3pre(c));
+  PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
+  PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
+  return sync(c);
+}
+
+extern "C" int phg_bc3per(phg_ctx* c, double* res, int nQs) {
+  PHG_TRY(check_ctx(c));
+  if (nQs < 1 || nQs > 25) {
+    phg_set_error("bc3per: nQs %d", nQs);
+    return PHG_EINVAL;
+  }
+  const long long n = c->nshg;
+  PHG_TRY(ensure_tmp(c, sizeof(double) * nQs * n));
+  PHG_TRY(upload(c, c->tmp, res, nQs * n));
+  PHG_TRY(launch_bc3per(c, c->tmp, nQs));
+  PHG_TRY(download(c, res, c->tmp, nQs * n));
+  return sync(c);
+}
+
+extern "C" int phg_bench_spmv(phg_ctx* c, int reps, double* ms_per_launch, double* bytes) {
+  PHG_TRY(check_ctx(c));
+  hipEvent_t a, b;
+  PHG_HIP(hipEventCreate(&a));
+  PHG_HIP(hipEventCreate(&b));
+  PHG_TRY(launch_spmv(c, kvec(c, 0), kvec(c, 1)));  // warm
+  PHG_HIP(hipEventRecord(a, c->stream));
+  for (int i = 0; i < reps; ++i) PHG_TRY(launch_spmv(c, kvec(c, i & 1), kvec(c, (i + 1) & 1)));
+  PHG_HIP(hipEventRecord(b, c->stream));
+  PHG_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (ms_per_launch) *ms_per_launch = ms / std::max(1, reps);
+  const double sA = c->precision == PHG_MIXED ? 4.0 : 8.0;
+  if (bytes)
+    *bytes = (double)c->nnz * (25.0 * sA + 4.0) + 4.0 * (double)(c->nshg + 1) +
+             2.0 * (double)c->nshg * 5.0 * 8.0;
+  return 0;
+}
+
+extern "C" int phg_bench_assembly(phg_ctx* c, int reps, double* ms_per_launch) {
+  PHG_TRY(check_ctx(c));
+  hipEvent_t a, b;
+  PHG_HIP(hipEventCreate(&a));
+  PHG_HIP(hipEventCreate(&b));
+  PHG_TRY(elm_core(c));
+  PHG_HIP(hipEventRecord(a, c->stream));
+  for (int i = 0; i < reps; ++i) PHG_TRY(elm_core(c));
+  PHG_HIP(hipEventRecord(b, c->stream));
+  PHG_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (ms_per_launch) *ms_per_launch = ms / std::max(1, reps);
+  return 0;
+}

@@ -1,0 +1,138 @@
+// Internal declarations of libphgpu: the device-resident context and the
+// host-side launchers of the HIP kernels.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "phgpu.h"
+
+void phg_set_error(const char* fmt, ...);
+
+#define PHG_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) {                                               \
+      phg_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,            \
+                    hipGetErrorString(e_));                               \
+      return PHG_EHIP;                                                    \
+    }                                                                     \
+  } while (0)
+
+#define PHG_TRY(expr)          \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_) return rc_;       \
+  } while (0)
+
+// Rows of the device block-CSR matrix are grouped by 64 (one wavefront);
+// a group holds S slots, each slot one 5x5 block per row of the group.
+constexpr int kGroup = 64;
+
+// Physical constants and time-integration factors used in the kernels.
+struct PhysParams {
+  double Rgas, gamma, gamma1, pr, mu;
+  double dtsfct, taucfct, temper;
+  double almi, alfi, gami, Dtgl;
+};
+
+// Per-node Dirichlet mask: bit f set = equation/dof f (local order
+// p,u,v,w,T) is constrained (bc3lhs.f, bc3res.f, bc3bdg.f).
+__host__ __device__ inline uint8_t constraint_mask(int ibc) {
+  uint8_t m = 0;
+  if (ibc & 4) m |= 1;                         // pressure
+  if (((ibc >> 3) & 7) == 7) m |= 2 | 4 | 8;   // no-slip velocity
+  if (ibc & 2) m |= 16;                        // temperature
+  return m;
+}
+
+struct phg_ctx {
+  long long nshg = 0, nel = 0, nelb = 0, nnz = 0;
+  int ndofBC = 0;
+  int precision = PHG_FP64;
+  phg_params prm{};
+  hipStream_t stream = nullptr;
+  long long dev_bytes = 0;
+
+  // mesh (device)
+  double* x = nullptr;          // 3 planes of nshg
+  int* ienc = nullptr;          // 4 planes of nel, colour order, 0-based
+  uint32_t* bmap = nullptr;     // 16 planes of nel: slot-row of block (a,b)
+  int* ienbc = nullptr;         // 4 planes of nelb, colour order, 0-based
+  uint8_t* bcm = nullptr;       // constraint mask per node
+  int* iper = nullptr;          // 0-based master
+  double* BCv = nullptr;        // (nshg, ndofBC)
+  std::vector<int> col_off;     // interior colour offsets (host)
+  std::vector<int> colb_off;    // boundary colour offsets (host)
+
+  // periodic fold: masters that own slaves, slaves ascending per master
+  int nfold = 0;
+  int* fold_master = nullptr;
+  int* fold_ptr = nullptr;
+  int* fold_slave = nullptr;
+
+  // block-CSR in groups of 64 rows ("slot" layout)
+  long long ngroups = 0, nslots = 0;
+  std::vector<long long> h_grp_slot;
+  long long* grp_slot = nullptr;   // ngroups+1
+  int* scol = nullptr;             // nslots*64 raw column (0-based)
+  int* scolp = nullptr;            // nslots*64 periodic-master column
+  uint32_t* csr2slot = nullptr;    // nnz -> slot-row index
+  double* lhs64 = nullptr;         // nslots*64*25
+  float* lhs32 = nullptr;          // mixed precision copy
+
+  // nodal vectors (SoA planes of nshg)
+  double* res = nullptr;
+  double* rmes = nullptr;
+  double* bdiag = nullptr;         // 25 planes
+  float* bdiag32 = nullptr;
+  double* y = nullptr;
+  double* ac = nullptr;
+  double* yold = nullptr;
+  double* acold = nullptr;
+  double* dy = nullptr;
+  double* krylov = nullptr;        // (Kspace+1) vectors of 5*nshg
+  int kcap = 0;
+
+  // reductions
+  int red_blocks = 0;
+  double* red_part = nullptr;      // red_blocks * 8
+  unsigned* red_count = nullptr;
+  double* scal = nullptr;          // device scalars (H column etc.)
+  double* h_scal = nullptr;        // pinned mirror
+  double reslast = 0.0;
+  int ntotGM = 0;
+
+  // scratch for host-array entry points
+  double* tmp = nullptr;
+  size_t tmp_bytes = 0;
+};
+
+// ---- launchers (k_assembly.hip) ----
+int launch_fill(phg_ctx* c, void* p, size_t bytes);
+int launch_assembly(phg_ctx* c, bool lhs, bool prec);
+int launch_bc_res_bdiag(phg_ctx* c, bool prec);
+
+// ---- launchers (k_solver.hip) ----
+int launch_i3lu(phg_ctx* c, double* diag, double* r, int code);
+int launch_spsi3pre(phg_ctx* c);
+int launch_spmv(phg_ctx* c, const double* p, double* q);
+int launch_bc3per(phg_ctx* c, double* v, int nq);
+int launch_dot(phg_ctx* c, const double* a, const double* b, long long n,
+               double* out);
+int launch_mgs_step(phg_ctx* c, double* u, const double* ujm1,
+                    const double* uj, const double* beta_prev, long long n,
+                    double* out);
+int launch_scale(phg_ctx* c, double* u, const double* src, long long n,
+                 const double* sq_ptr, double* norm_out);
+int launch_combine(phg_ctx* c, double* dy, const double* basis, int k,
+                   const double* coef, long long n);
+int launch_stats(phg_ctx* c, double* out);
+int launch_csr_to_slots(phg_ctx* c, const double* ref, double* slots);
+int launch_slots_to_csr(phg_ctx* c, const double* slots, double* ref);
+int launch_predict(phg_ctx* c);
+int launch_correct(phg_ctx* c);
+int launch_itrbc(phg_ctx* c);

@@ -1,0 +1,235 @@
+"""Host-side mirror of the reference's solver interface over ``libphgpu.so``.
+
+``GpuSolver`` exposes the reference's entry points with their argument
+meaning and array layouts (Fortran-ordered numpy arrays, 1-based
+connectivity):
+
+* ``elmgmrs(y, ac)``  -> res, rmes, BDiag, lhsK        (ElmGMRs, elmgmr.f:280)
+* ``solgmrs(y, ac)``  -> res, rmes, BDiag, lhsK, Dy    (SolGMRs, solgmr.f:368)
+* ``sparse_ap(lhsK, p)``, ``i3lu(Diag, r, code)``, ``spsi3pre(BDiag, lhsK)``,
+  ``bc3per(res)``                                       (sparseap.f, i3lu.f,
+  spsi3pre.f, bc3per.f)
+* ``predict()``, ``newton_iteration()`` on the device-resident state
+  (itrPredict/itrBC and SolGMRs + itrCorrect/itrBC, itrdrv.f:398-599).
+
+Errors from the library raise ``PhgError`` with the library's message, the
+counterpart of the reference's ``error()`` abort.
+"""
+import ctypes
+
+import numpy as np
+
+from ._native import load
+
+_i64 = ctypes.c_longlong
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+PHG_FP64, PHG_MIXED = 0, 1
+
+
+class PhgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libphgpu error {code}: {msg}")
+        self.code = code
+
+
+class PhgParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
+                 "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
+               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec")]
+
+
+class PhgMesh(ctypes.Structure):
+    _fields_ = [("nshg", _i64), ("nel", _i64), ("nelb", _i64), ("nnz_tot", _i64),
+                ("ndofBC", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("x", _dp), ("ien", _ip), ("ienb", _ip), ("iBC", _ip), ("BC", _dp),
+                ("iper", _ip), ("colm", _ip), ("rowp", _ip)]
+
+
+class PhgStats(ctypes.Structure):
+    _fields_ = [("iKs", ctypes.c_int), ("lGMRES", ctypes.c_int), ("ntotGM", ctypes.c_int),
+                ("converged", ctypes.c_int), ("unorm0", ctypes.c_double),
+                ("epsnrm", ctypes.c_double), ("echeck", ctypes.c_double),
+                ("totres", ctypes.c_double * 2), ("reslast", ctypes.c_double),
+                ("nnan", _i64), ("nlarge", _i64), ("ms_assembly", ctypes.c_double),
+                ("ms_solve", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("refine_cycles", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d["totres"] = tuple(self.totres)
+        d["converged"] = bool(self.converged)
+        return d
+
+
+# Every symbol declared in include/phgpu.h.
+EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_info",
+           "phg_elmgmrs", "phg_solgmrs", "phg_state_upload", "phg_state_download",
+           "phg_predict", "phg_newton_iteration", "phg_sparse_ap", "phg_i3lu",
+           "phg_spsi3pre", "phg_bc3per", "phg_bench_spmv", "phg_bench_assembly")
+
+
+def lib():
+    L = load("libphgpu.so")
+    if not getattr(L, "_typed", False):
+        L.phg_last_error.restype = ctypes.c_char_p
+        L.phg_init.argtypes = [ctypes.POINTER(PhgMesh), ctypes.POINTER(PhgParams),
+                               ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        for name in EXPORTS:
+            getattr(L, name)
+        L._typed = True
+    return L
+
+
+def _p(a, t=_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def _check(rc):
+    if rc:
+        raise PhgError(rc, lib().phg_last_error().decode())
+
+
+def _f64(a, shape):
+    a = np.asarray(a, dtype=np.float64)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return np.asfortranarray(a)
+
+
+def make_params(d):
+    p = PhgParams()
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+class GpuSolver:
+    """One device context for one mesh (phg_init ... phg_finalize)."""
+
+    def __init__(self, case, params=None, precision=PHG_FP64):
+        self.case = case
+        self.params = dict(case.params if params is None else params)
+        m = PhgMesh()
+        m.nshg, m.nel, m.nelb, m.nnz_tot = case.nshg, case.nel, case.nelb, case.nnz_tot
+        m.ndofBC = case.BC.shape[1]
+        self._keep = [np.asfortranarray(a) for a in
+                      (case.x, case.ien, case.ienb, case.iBC, case.BC, case.iper,
+                       case.colm, case.rowp)]
+        x, ien, ienb, iBC, BC, iper, colm, rowp = self._keep
+        m.x, m.ien, m.ienb, m.iBC = _p(x), _p(ien, _ip), _p(ienb, _ip), _p(iBC, _ip)
+        m.BC, m.iper, m.colm, m.rowp = _p(BC), _p(iper, _ip), _p(colm, _ip), _p(rowp, _ip)
+        self._prm = make_params(self.params)
+        h = ctypes.c_void_p()
+        _check(lib().phg_init(ctypes.byref(m), ctypes.byref(self._prm), precision,
+                              ctypes.byref(h)))
+        self.h = h
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().phg_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, **over):
+        self.params.update(over)
+        self._prm = make_params(self.params)
+        _check(lib().phg_set_params(self.h, ctypes.byref(self._prm)))
+
+    def info(self):
+        nc, nb, by = ctypes.c_int(), ctypes.c_int(), _i64()
+        _check(lib().phg_info(self.h, ctypes.byref(nc), ctypes.byref(nb), ctypes.byref(by)))
+        return dict(ncolors=nc.value, ncolors_b=nb.value, dev_bytes=by.value)
+
+    # ---- reference-shaped entry points (host arrays) ----
+    def elmgmrs(self, y, ac, want_lhs=True):
+        n, nnz = self.case.nshg, self.case.nnz_tot
+        y, ac = _f64(y, (n, 5)), _f64(ac, (n, 5))
+        res = np.zeros((n, 5), order="F")
+        rmes = np.zeros((n, 5), order="F")
+        bd = np.zeros((n, 5, 5), order="F")
+        lk = np.zeros((25, nnz), order="F") if want_lhs else None
+        _check(lib().phg_elmgmrs(self.h, _p(y), _p(ac), _p(res), _p(rmes), _p(bd), _p(lk)))
+        return res, rmes, bd, lk
+
+    def solgmrs(self, y, ac, want_lhs=False):
+        n, nnz = self.case.nshg, self.case.nnz_tot
+        y, ac = _f64(y, (n, 5)), _f64(ac, (n, 5))
+        res = np.zeros((n, 5), order="F")
+        rmes = np.zeros((n, 5), order="F")
+        bd = np.zeros((n, 5, 5), order="F")
+        dy = np.zeros((n, 5), order="F")
+        lk = np.zeros((25, nnz), order="F") if want_lhs else None
+        st = PhgStats()
+        _check(lib().phg_solgmrs(self.h, _p(y), _p(ac), _p(res), _p(rmes), _p(bd), _p(lk),
+                                 _p(dy), ctypes.byref(st)))
+        out = st.as_dict()
+        out.update(res=res, rmes=rmes, BDiag=bd, lhsK=lk, Dy=dy)
+        return out
+
+    def sparse_ap(self, lhsK, p):
+        p = _f64(p, (self.case.nshg, 5)).copy(order="F")
+        lk = None if lhsK is None else _f64(lhsK, (25, self.case.nnz_tot))
+        _check(lib().phg_sparse_ap(self.h, _p(lk), _p(p)))
+        return p
+
+    def i3lu(self, diag, r, code):
+        n = self.case.nshg
+        diag = _f64(diag, (n, 5, 5)).copy(order="F")
+        r = None if r is None else _f64(r, (n, 5)).copy(order="F")
+        _check(lib().phg_i3lu(self.h, _p(diag), _p(r), code.encode()))
+        return diag, r
+
+    def spsi3pre(self, bdiag_lu, lhsK):
+        bd = _f64(bdiag_lu, (self.case.nshg, 5, 5))
+        lk = _f64(lhsK, (25, self.case.nnz_tot)).copy(order="F")
+        _check(lib().phg_spsi3pre(self.h, _p(bd), _p(lk)))
+        return lk
+
+    def bc3per(self, res):
+        res = np.asfortranarray(np.array(res, dtype=np.float64))
+        _check(lib().phg_bc3per(self.h, _p(res), ctypes.c_int(res.shape[1])))
+        return res
+
+    # ---- device-resident stepping ----
+    def upload_state(self, yold, acold=None):
+        n = self.case.nshg
+        yold = _f64(yold, (n, 5))
+        acold = np.zeros((n, 5), order="F") if acold is None else _f64(acold, (n, 5))
+        _check(lib().phg_state_upload(self.h, _p(yold), _p(acold)))
+
+    def predict(self):
+        _check(lib().phg_predict(self.h))
+
+    def newton_iteration(self):
+        st = PhgStats()
+        _check(lib().phg_newton_iteration(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def download_state(self):
+        n = self.case.nshg
+        y = np.zeros((n, 5), order="F")
+        ac = np.zeros((n, 5), order="F")
+        dy = np.zeros((n, 5), order="F")
+        _check(lib().phg_state_download(self.h, _p(y), _p(ac), _p(dy)))
+        return y, ac, dy
+
+    def bench_spmv(self, reps=20):
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        _check(lib().phg_bench_spmv(self.h, reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
+    def bench_assembly(self, reps=3):
+        ms = ctypes.c_double()
+        _check(lib().phg_bench_assembly(self.h, reps, ctypes.byref(ms)))
+        return ms.value

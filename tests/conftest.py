@@ -1,0 +1,25 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libphgpu.so)")
+
+
+@pytest.fixture(scope="session")
+def box4():
+    from phasta_amd import mesh
+    return mesh.box_case(4, mach=0.3, kspace=10)
+
+
+@pytest.fixture(scope="session")
+def box16():
+    from phasta_amd import mesh
+    return mesh.box_case(16, mach=0.3, kspace=10)

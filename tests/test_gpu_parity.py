@@ -1,0 +1,97 @@
+"""GPU parity of the FP64 path against the CPU oracle, through the C-ABI.
+
+Bar: FP64 results within 1e-12 normwise relative error of the oracle (the
+reference's operation order; the GPU sums in colour order and regroups the
+tangent algebraically, so agreement is to rounding, not bit-exact), and the
+same Krylov iteration count.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from phasta_amd import mesh, solver
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.fixture(scope="module", params=[4, 16], ids=["box4", "box16"])
+def pair(request):
+    case = mesh.box_case(request.param, mach=0.3, kspace=10)
+    print(f"[parity] box{request.param}: nshg={case.nshg} nel={case.nel}", flush=True)
+    orc = pyoracle.Oracle(case)
+    gpu = solver.GpuSolver(case)
+    yield case, orc, gpu
+    gpu.close()
+
+
+def test_elmgmrs(pair):
+    case, orc, gpu = pair
+    y, ac = orc.itr_predict_bc(case.yold)
+    rng = np.random.default_rng(1)
+    ac = ac + 1e3 * rng.standard_normal(ac.shape)  # exercise the time terms
+    ac = np.asfortranarray(ac)
+    r_res, r_rmes, r_bd, r_lk = orc.elmgmrs(y, ac)
+    g_res, g_rmes, g_bd, g_lk = gpu.elmgmrs(y, ac)
+    assert rel(g_res, r_res) < TOL
+    assert rel(g_bd, r_bd) < TOL
+    assert rel(g_lk, r_lk) < TOL
+    assert not g_rmes.any()
+
+
+def test_solgmrs(pair):
+    case, orc, gpu = pair
+    y, ac = orc.itr_predict_bc(case.yold)
+    ref = orc.solgmrs(y, ac)
+    out = gpu.solgmrs(y, ac, want_lhs=True)
+    assert out["iKs"] == ref["iKs"]
+    for k in ("res", "rmes", "BDiag", "Dy", "lhsK"):
+        assert rel(out[k], ref[k]) < TOL, k
+    assert abs(out["unorm0"] - ref["unorm0"]) <= TOL * ref["unorm0"]
+
+
+def test_sparse_ap_i3lu_spsi3pre_bc3per(pair):
+    case, orc, gpu = pair
+    y, ac = orc.itr_predict_bc(case.yold)
+    _, _, bd, lk = orc.elmgmrs(y, ac)
+    rng = np.random.default_rng(2)
+    p = np.asfortranarray(rng.standard_normal((case.nshg, 5)))
+    ref = p.copy(order="F")
+    orc.sparse_ap(lk, ref)
+    assert rel(gpu.sparse_ap(lk, p), ref) < TOL
+    # i3LU factor / forward / backward
+    d_ref = bd.copy(order="F")
+    orc.i3lu(d_ref, None, "LU_Fact ")
+    d_gpu, _ = gpu.i3lu(bd, None, "LU_Fact ")
+    assert rel(d_gpu, d_ref) < TOL
+    for code in ("forward ", "backward"):
+        r_ref = p.copy(order="F")
+        orc.i3lu(d_ref, r_ref, code)
+        _, r_gpu = gpu.i3lu(d_ref, p, code)
+        assert rel(r_gpu, r_ref) < TOL, code
+    # Spsi3pre
+    lk_ref = lk.copy(order="F")
+    orc.spsi3pre(d_ref, lk_ref)
+    assert rel(gpu.spsi3pre(d_ref, lk), lk_ref) < TOL
+    # bc3per
+    b_ref = p.copy(order="F")
+    orc.bc3per(b_ref)
+    assert np.array_equal(gpu.bc3per(p), b_ref)
+
+
+def test_device_newton_step(pair):
+    case, orc, gpu = pair
+    acold = np.zeros((case.nshg, 5), order="F")
+    y_ref, ac_ref, hist = orc.newton_step(case.yold, acold, newton=2)
+    gpu.upload_state(case.yold, acold)
+    gpu.predict()
+    its = [gpu.newton_iteration() for _ in range(2)]
+    y, ac, _ = gpu.download_state()
+    assert [h["iKs"] for h in its] == [h["iKs"] for h in hist]
+    assert rel(y, y_ref) < TOL
+    assert rel(ac, ac_ref) < 1e-10  # ac = (y - yold)/dt amplifies y's rounding
