@@ -28,11 +28,15 @@ __global__ void k_lu_fact(double* D, long long n) {
   for (int j = 0; j < 5; ++j)
 #pragma unroll
     for (int i = 0; i < 5; ++i) a[i][j] = D[(i + 5 * j) * n + k];
-  // Doolittle: unit-lower L, upper U with its diagonal stored inverted
+  // Doolittle: unit-lower L, upper U with its diagonal stored inverted.
+  // As in the reference (i3lu.f:82-97) the entry U(4,5) is never reduced:
+  // row 4 of U only gets its diagonal, so (4,5) keeps the raw BDiag value
+  // and the forward/backward sweeps use it as such.
 #pragma unroll
   for (int p = 0; p < 5; ++p) {
 #pragma unroll
     for (int j = p; j < 5; ++j) {
+      if (p == 3 && j == 4) continue;
       double s = a[p][j];
 #pragma unroll
       for (int m = 0; m < p; ++m) s -= a[p][m] * a[m][j];
