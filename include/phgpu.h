@@ -72,7 +72,9 @@ typedef struct phg_stats {
   double reslast;        /* rstatCheck memory across calls                 */
   long long nnan, nlarge;
   double ms_assembly, ms_solve, ms_total;
-  int refine_cycles;     /* mixed precision: outer refinement cycles       */
+  double ms_spmv, ms_mgs; /* summed HIP-event times of the matvec and MGS  */
+  int n_spmv, n_mgs;      /* launches behind ms_spmv / ms_mgs              */
+  int refine_cycles;      /* mixed precision: outer refinement cycles      */
   int reserved;
 } phg_stats;
 
@@ -128,6 +130,8 @@ int phg_bc3per(phg_ctx* ctx, double* res, int nQs);
 int phg_bench_spmv(phg_ctx* ctx, int reps, double* ms_per_launch,
                    double* bytes_per_launch);
 int phg_bench_assembly(phg_ctx* ctx, int reps, double* ms_per_launch);
+/* Wait for all work queued on the library stream. */
+int phg_sync(phg_ctx* ctx);
 
 #ifdef __cplusplus
 }

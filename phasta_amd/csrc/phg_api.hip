@@ -372,20 +372,37 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   std::vector<double> Hh((size_t)(K + 2) * (K + 1), 0.0), e(K + 2, 0.0), Rc(K + 2, 0.0),
       Rs(K + 2, 0.0), yB(K + 2, 0.0);
   auto Hm = [&](int a, int b) -> double& { return Hh[(size_t)(a - 1) + (size_t)(K + 1) * (b - 1)]; };
+  hipEvent_t kev[4];
+  for (auto& k1 : kev) PHG_HIP(hipEventCreate(&k1));
+  st->ms_spmv = st->ms_mgs = 0.0;
+  st->n_spmv = st->n_mgs = 0;
   if (!(unorm < 100. * c->prm.epsM * c->prm.epsM)) {
     e[1] = unorm;
     for (int iK = 1; iK <= K; ++iK) {
       iKs = iK;
       double* w = kvec(c, iK);
+      PHG_HIP(hipEventRecord(kev[0], c->stream));
       PHG_TRY(launch_spmv(c, kvec(c, iK - 1), w));
+      PHG_HIP(hipEventRecord(kev[1], c->stream));
       PHG_TRY(launch_bc3per(c, w, 5));
+      PHG_HIP(hipEventRecord(kev[2], c->stream));
       PHG_TRY(launch_dot(c, w, kvec(c, 0), V, H + 0));
       for (int jK = 2; jK <= iK + 1; ++jK)
         PHG_TRY(launch_mgs_step(c, w, kvec(c, jK - 2), jK == iK + 1 ? w : kvec(c, jK - 1), H + jK - 2,
                                 V, H + jK - 1));
+      PHG_HIP(hipEventRecord(kev[3], c->stream));
       PHG_TRY(launch_scale(c, w, w, V, H + iK, H + iK + 1));
       PHG_TRY(download(c, c->h_scal + 16, H, iK + 2));
       PHG_TRY(sync(c));
+      {
+        float a1 = 0, a2 = 0;
+        hipEventElapsedTime(&a1, kev[0], kev[1]);
+        hipEventElapsedTime(&a2, kev[2], kev[3]);
+        st->ms_spmv += a1;
+        st->ms_mgs += a2;
+        st->n_spmv += 1;
+        st->n_mgs += iK + 1;
+      }
       for (int jK = 1; jK <= iK; ++jK) Hm(jK, iK) = c->h_scal[16 + jK - 1];
       Hm(iK + 1, iK) = c->h_scal[16 + iK + 1];
       // Givens rotations on the new column (solgmr.f:645-663)
@@ -433,6 +450,7 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   st->ms_solve = ms2;
   st->ms_total = ms1 + ms2;
   for (auto& ev1 : ev) hipEventDestroy(ev1);
+  for (auto& k1 : kev) hipEventDestroy(k1);
   return 0;
 }
 
@@ -691,6 +709,11 @@ extern "C" int phg_bench_spmv(phg_ctx* c, int reps, double* ms_per_launch, doubl
     *bytes = (double)c->nnz * (25.0 * sA + 4.0) + 4.0 * (double)(c->nshg + 1) +
              2.0 * (double)c->nshg * 5.0 * 8.0;
   return 0;
+}
+
+extern "C" int phg_sync(phg_ctx* c) {
+  PHG_TRY(check_ctx(c));
+  return sync(c);
 }
 
 extern "C" int phg_bench_assembly(phg_ctx* c, int reps, double* ms_per_launch) {

@@ -55,6 +55,8 @@ class PhgStats(ctypes.Structure):
                 ("totres", ctypes.c_double * 2), ("reslast", ctypes.c_double),
                 ("nnan", _i64), ("nlarge", _i64), ("ms_assembly", ctypes.c_double),
                 ("ms_solve", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("ms_spmv", ctypes.c_double), ("ms_mgs", ctypes.c_double),
+                ("n_spmv", ctypes.c_int), ("n_mgs", ctypes.c_int),
                 ("refine_cycles", ctypes.c_int), ("reserved", ctypes.c_int)]
 
     def as_dict(self):
@@ -68,7 +70,8 @@ class PhgStats(ctypes.Structure):
 EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_info",
            "phg_elmgmrs", "phg_solgmrs", "phg_state_upload", "phg_state_download",
            "phg_predict", "phg_newton_iteration", "phg_sparse_ap", "phg_i3lu",
-           "phg_spsi3pre", "phg_bc3per", "phg_bench_spmv", "phg_bench_assembly")
+           "phg_spsi3pre", "phg_bc3per", "phg_bench_spmv", "phg_bench_assembly",
+           "phg_sync")
 
 
 def lib():
@@ -223,6 +226,9 @@ class GpuSolver:
         dy = np.zeros((n, 5), order="F")
         _check(lib().phg_state_download(self.h, _p(y), _p(ac), _p(dy)))
         return y, ac, dy
+
+    def sync(self):
+        _check(lib().phg_sync(self.h))
 
     def bench_spmv(self, reps=20):
         ms, by = ctypes.c_double(), ctypes.c_double()
