@@ -3,7 +3,9 @@
 // AsBMFG -> e3b, elmgmr.f:280-612), plus the nodal Dirichlet/periodic
 // constraints on res and BDiag (bc3res.f, bc3bdg.f).
 //
-// One element per lane.  Elements are processed colour by colour; no two
+// Sixteen lanes per element, one 5x5 tangent block (a,b) per lane; the
+// element's per-quadrature-point state is computed once (lane b computes
+// point b) and shared through LDS.  Elements are processed colour by colour; no two
 // elements of a colour share a node, so every scatter into res, BDiag and
 // the block-CSR values is a plain read-modify-write without atomics and the
 // result is deterministic.  For affine tets the metrics, shape gradients and
@@ -34,14 +36,12 @@ __constant__ Quad cQ;
 // Constraint and element data are read through these pointers.
 struct AsmArgs {
   const int* ien;          // 4 planes of nel (colour order, 0-based)
-  const uint32_t* bmap;    // 16 planes of nel
+  const uint32_t* bmap;    // 16 planes of nel: CSR block index of (a,b)
   const uint8_t* bcm;
-  const double* y;         // 5 planes, global order (u,v,w,p,T)
-  const double* ac;
-  const double* x;         // 3 planes
-  double* res;             // 5 planes, local order
-  double* bdiag;           // 25 planes
-  void* lhs;               // slot values (T)
+  const double* node;      // 16 per node: y (p,u,v,w,T), ac (same order), x
+  double* res;             // 5 per node (accumulation layout)
+  double* bdiag;           // 25 per node (accumulation layout)
+  void* lhs;               // 25 per CSR block (T), reference lhsK order
   long long nshg, nel;
   int p0, p1;
   int lhs_on, prec_on;
@@ -87,22 +87,39 @@ __device__ inline void A0_times(const Thermo& t, const double a[5], double r[5])
   r[4] = t.e1p * a[0] + t.rho * (t.u[0] * a[1] + t.u[1] * a[2] + t.u[2] * a[3]) + t.e4p * a[4];
 }
 
+// Lanes per element: lane l = 4a + b owns the tangent block (a,b) and the
+// residual contribution of quadrature point b to node a.
+constexpr int kLanesPerElem = 16;
+constexpr int kAsmThreads = 256;
+constexpr int kElemsPerBlock = kAsmThreads / kLanesPerElem;
+constexpr int kThermo = 13;
+
 template <typename T>
-__global__ void __launch_bounds__(64) k_assemble(AsmArgs A) {
-  const int p = A.p0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.p1) return;
+__global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
+  // per-element quadrature-point state shared by the element's 16 lanes
+  __shared__ double sth[kElemsPerBlock][4][kThermo];
+  const int tid = threadIdx.x;
+  const int el = tid / kLanesPerElem;
+  const int l = tid % kLanesPerElem, a = l >> 2, b = l & 3;
+  const int p_raw = A.p0 + blockIdx.x * kElemsPerBlock + el;
+  const bool valid = p_raw < A.p1;
+  const int p = valid ? p_raw : A.p0;  // idle lanes load a valid element
   const long long nshg = A.nshg, nel = A.nel;
   int nd[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) nd[a] = A.ien[a * nel + p];
+  for (int k = 0; k < 4; ++k) nd[k] = A.ien[k * nel + p];
 
   double yl[4][5], acl[4][5], xl[4][3];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    gather_local(A.y, nshg, nd[a], yl[a]);
-    gather_local(A.ac, nshg, nd[a], acl[a]);
+  for (int k = 0; k < 4; ++k) {
+    const double* v = A.node + 16 * (size_t)nd[k];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xl[a][c] = A.x[nd[a] + c * nshg];
+    for (int f = 0; f < 5; ++f) {
+      yl[k][f] = v[f];
+      acl[k][f] = v[5 + f];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xl[k][c] = v[10 + c];
   }
   const PhysParams& ph = A.ph;
 
@@ -170,22 +187,24 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs A) {
   const double cv = ph.Rgas / ph.gamma1;
   const double cp = ph.Rgas * ph.gamma / ph.gamma1;
   const double mu = ph.mu, lam = -(2.0 / 3.0) * mu, kap = mu * cp / ph.pr;
-  const double l2m = lam + 2.0 * mu;
   const double dts = ph.dtsfct * ph.Dtgl;
   const double cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   const double divu = g[0][1] + g[1][2] + g[2][3];
 
-  Thermo th[4];
-  double rl[4][5];
+  // shape gradients of this lane's row node a and column node b
+  double s[3], r[3];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int f = 0; f < 5; ++f) rl[a][f] = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    s[k] = sel4(shg[0][k], shg[1][k], shg[2][k], shg[3][k], a);
+    r[k] = sel4(shg[0][k], shg[1][k], shg[2][k], shg[3][k], b);
+  }
 
-  // ---------------- residual (ires = 1) ----------------
+  // ---------------- quadrature point q = b: state and residual ----------------
+  double rl[5];
+  {
+    double Nq[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const double* Nq = cQ.N[q];
+    for (int k = 0; k < 4; ++k) Nq[k] = sel4(cQ.N[0][k], cQ.N[1][k], cQ.N[2][k], cQ.N[3][k], b);
     double Y[5], aci[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
@@ -193,7 +212,7 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs A) {
       aci[f] = Nq[0] * acl[0][f] + Nq[1] * acl[1][f] + Nq[2] * acl[2][f] + Nq[3] * acl[3][f];
     }
     const double pres = Y[0], Tq = Y[4];
-    Thermo& t = th[q];
+    Thermo t;
     t.u[0] = Y[1];
     t.u[1] = Y[2];
     t.u[2] = Y[3];
@@ -216,7 +235,22 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs A) {
     t.tau[0] = 0.125 * sq / (t.rho * trG) * ph.taucfct;
     t.tau[1] = 1.0 / sq;
     t.tau[2] = t.tau[1] / cv * ph.temper;
-
+    if (a == 0) {
+      double* o = sth[el][b];
+      o[0] = t.u[0];
+      o[1] = t.u[1];
+      o[2] = t.u[2];
+      o[3] = t.rho;
+      o[4] = t.drdp;
+      o[5] = t.drdT;
+      o[6] = t.e1p;
+      o[7] = t.e2p;
+      o[8] = t.e3p;
+      o[9] = t.e4p;
+      o[10] = t.tau[0];
+      o[11] = t.tau[1];
+      o[12] = t.tau[2];
+    }
     // fluxes: ri[5i+f] = -F_i + F^v_i; ri[15+f] = A0 aci
     double ri[20];
     const double E = ei + rk;
@@ -256,132 +290,149 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs A) {
 #pragma unroll
       for (int f = 0; f < 5; ++f) ri[5 * i + f] += tmp[f];
     }
+    const double Na = sel4(Nq[0], Nq[1], Nq[2], Nq[3], a);
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int f = 0; f < 5; ++f)
+      rl[f] = WdetJ * (s[0] * ri[f] + s[1] * ri[5 + f] + s[2] * ri[10 + f]) + Na * WdetJ * ri[15 + f];
+    // sum the four quadrature points (lanes b = 0..3 of this node a)
 #pragma unroll
-      for (int f = 0; f < 5; ++f)
-        rl[a][f] += WdetJ * (shg[a][0] * ri[f] + shg[a][1] * ri[5 + f] + shg[a][2] * ri[10 + f]) +
-                    Nq[a] * WdetJ * ri[15 + f];
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int f = 0; f < 5; ++f) A.res[nd[a] + f * nshg] += rl[a][f];
-
-  if (!A.lhs_on && !A.prec_on) return;
-
-  // ---------------- tangent blocks ----------------
-  uint8_t msk[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) msk[a] = A.bcm[nd[a]];
-  T* lhs = reinterpret_cast<T*>(A.lhs);
-
-  for (int a = 0; a < 4; ++a) {
-    double s[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) s[k] = sel4(shg[0][k], shg[1][k], shg[2][k], shg[3][k], a);
-    const int na = sel4i(nd[0], nd[1], nd[2], nd[3], a);
-    const uint8_t ma = (uint8_t)sel4i(msk[0], msk[1], msk[2], msk[3], a);
-    for (int b = 0; b < 4; ++b) {
-      if (!A.lhs_on && a != b) continue;
-      double r[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) r[k] = sel4(shg[0][k], shg[1][k], shg[2][k], shg[3][k], b);
-      const uint8_t mb = (uint8_t)sel4i(msk[0], msk[1], msk[2], msk[3], b);
-      const double st = s[0] * r[0] + s[1] * r[1] + s[2] * r[2];
-      double acc[5][5];
-#pragma unroll
-      for (int i = 0; i < 5; ++i)
-#pragma unroll
-        for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const Thermo& t = th[q];
-        const double Na = cQ.N[q][a], Nb = cQ.N[q][b];
-        const double una = t.u[0] * s[0] + t.u[1] * s[1] + t.u[2] * s[2];
-        const double unb = t.u[0] * r[0] + t.u[1] * r[1] + t.u[2] * r[2];
-        const double w = unb + cfac * Nb;
-        double P[5][5], Q[5][5];
-        // P = A(s_a) diag(tau) + N_a I
-        P[0][0] = t.drdp * una * t.tau[0] + Na;
-        P[0][4] = t.drdT * una * t.tau[2];
-#pragma unroll
-        for (int n = 0; n < 3; ++n) P[0][1 + n] = t.rho * s[n] * t.tau[1];
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          P[1 + m][0] = (t.drdp * t.u[m] * una + s[m]) * t.tau[0];
-#pragma unroll
-          for (int n = 0; n < 3; ++n)
-            P[1 + m][1 + n] = t.rho * (t.u[m] * s[n] + (m == n ? una : 0.0)) * t.tau[1] + (m == n ? Na : 0.0);
-          P[1 + m][4] = t.drdT * t.u[m] * una * t.tau[2];
-        }
-        P[4][0] = t.e2p * una * t.tau[0];
-#pragma unroll
-        for (int n = 0; n < 3; ++n) P[4][1 + n] = (t.e3p * s[n] + t.rho * t.u[n] * una) * t.tau[1];
-        P[4][4] = t.e4p * una * t.tau[2] + Na;
-        // Q = A(s_b) + c N_b A0
-        Q[0][0] = t.drdp * w;
-        Q[0][4] = t.drdT * w;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) Q[0][1 + n] = t.rho * r[n];
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          Q[1 + m][0] = t.drdp * t.u[m] * w + r[m];
-#pragma unroll
-          for (int n = 0; n < 3; ++n) Q[1 + m][1 + n] = t.rho * (t.u[m] * r[n] + (m == n ? w : 0.0));
-          Q[1 + m][4] = t.drdT * t.u[m] * w;
-        }
-        Q[4][0] = t.e2p * unb + cfac * Nb * t.e1p;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) Q[4][1 + n] = t.e3p * r[n] + t.rho * t.u[n] * w;
-        Q[4][4] = t.e4p * w;
-#pragma unroll
-        for (int i = 0; i < 5; ++i)
-#pragma unroll
-          for (int j = 0; j < 5; ++j)
-            acc[i][j] += P[i][0] * Q[0][j] + P[i][1] * Q[1][j] + P[i][2] * Q[2][j] +
-                         P[i][3] * Q[3][j] + P[i][4] * Q[4][j];
-        // viscous part K_ab
-        const double us = una, ut = unb;
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-          for (int n = 0; n < 3; ++n)
-            acc[1 + m][1 + n] += mu * ((m == n ? st : 0.0) + s[n] * r[m]) + lam * s[m] * r[n];
-#pragma unroll
-        for (int n = 0; n < 3; ++n) acc[4][1 + n] += mu * (st * t.u[n] + s[n] * ut) + lam * us * r[n];
-        acc[4][4] += kap * st;
-      }
-#pragma unroll
-      for (int i = 0; i < 5; ++i)
-#pragma unroll
-        for (int j = 0; j < 5; ++j) acc[i][j] *= WdetJ;
-      // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103)
-      if (A.prec_on && a == b) {
-#pragma unroll
-        for (int j = 0; j < 5; ++j)
-#pragma unroll
-          for (int i = 0; i < 5; ++i) A.bdiag[na + (i + 5 * j) * nshg] += acc[i][j];
-      }
-      if (!A.lhs_on) continue;
-      // bc3LHS: constrained rows of node a, columns of node b, unit diagonal
-      if (ma | mb) {
-#pragma unroll
-        for (int i = 0; i < 5; ++i)
-#pragma unroll
-          for (int j = 0; j < 5; ++j) {
-            if (((ma >> i) & 1) || ((mb >> j) & 1)) acc[i][j] = 0.0;
-            if (a == b && i == j && ((ma >> i) & 1)) acc[i][j] = 1.0;
-          }
-      }
-      const uint32_t qs = A.bmap[(4 * a + b) * nel + p];
-      T* dst = lhs + (size_t)(qs >> 6) * (25 * kGroup) + (qs & 63);
-#pragma unroll
-      for (int j = 0; j < 5; ++j)
-#pragma unroll
-        for (int i = 0; i < 5; ++i) dst[(i + 5 * j) * kGroup] += (T)acc[i][j];
+    for (int f = 0; f < 5; ++f) {
+      rl[f] += __shfl_xor(rl[f], 1, 64);
+      rl[f] += __shfl_xor(rl[f], 2, 64);
     }
   }
+  __syncthreads();
+
+  const int na = sel4i(nd[0], nd[1], nd[2], nd[3], a);
+  if (valid && b == 0) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) A.res[5 * (size_t)na + f] += rl[f];
+  }
+  if (!A.lhs_on && !(A.prec_on && a == b)) return;
+
+  // ---------------- tangent block (a,b) ----------------
+  const double st = s[0] * r[0] + s[1] * r[1] + s[2] * r[2];
+  double acc[5][5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
+#pragma unroll 1
+  for (int q = 0; q < 4; ++q) {
+    const double* o = sth[el][q];
+    Thermo t;
+    t.u[0] = o[0];
+    t.u[1] = o[1];
+    t.u[2] = o[2];
+    t.rho = o[3];
+    t.drdp = o[4];
+    t.drdT = o[5];
+    t.e1p = o[6];
+    t.e2p = o[7];
+    t.e3p = o[8];
+    t.e4p = o[9];
+    t.tau[0] = o[10];
+    t.tau[1] = o[11];
+    t.tau[2] = o[12];
+    const double Na = sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
+    const double Nb = sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
+    const double una = t.u[0] * s[0] + t.u[1] * s[1] + t.u[2] * s[2];
+    const double unb = t.u[0] * r[0] + t.u[1] * r[1] + t.u[2] * r[2];
+    const double w = unb + cfac * Nb;
+    // Q = A(s_b) + c N_b A0
+    double Q[5][5];
+    Q[0][0] = t.drdp * w;
+    Q[0][4] = t.drdT * w;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) Q[0][1 + n] = t.rho * r[n];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      Q[1 + m][0] = t.drdp * t.u[m] * w + r[m];
+#pragma unroll
+      for (int n = 0; n < 3; ++n) Q[1 + m][1 + n] = t.rho * (t.u[m] * r[n] + (m == n ? w : 0.0));
+      Q[1 + m][4] = t.drdT * t.u[m] * w;
+    }
+    Q[4][0] = t.e2p * unb + cfac * Nb * t.e1p;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) Q[4][1 + n] = t.e3p * r[n] + t.rho * t.u[n] * w;
+    Q[4][4] = t.e4p * w;
+    // acc += P Q with P = A(s_a) diag(tau) + N_a I, one row of P at a time
+    {
+      double Pr[5];
+      Pr[0] = t.drdp * una * t.tau[0] + Na;
+#pragma unroll
+      for (int n = 0; n < 3; ++n) Pr[1 + n] = t.rho * s[n] * t.tau[1];
+      Pr[4] = t.drdT * una * t.tau[2];
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        acc[0][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
+                     Pr[4] * Q[4][j];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double Pr[5];
+      Pr[0] = (t.drdp * t.u[m] * una + s[m]) * t.tau[0];
+#pragma unroll
+      for (int n = 0; n < 3; ++n)
+        Pr[1 + n] = t.rho * (t.u[m] * s[n] + (m == n ? una : 0.0)) * t.tau[1] + (m == n ? Na : 0.0);
+      Pr[4] = t.drdT * t.u[m] * una * t.tau[2];
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        acc[1 + m][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
+                         Pr[4] * Q[4][j];
+    }
+    {
+      double Pr[5];
+      Pr[0] = t.e2p * una * t.tau[0];
+#pragma unroll
+      for (int n = 0; n < 3; ++n) Pr[1 + n] = (t.e3p * s[n] + t.rho * t.u[n] * una) * t.tau[1];
+      Pr[4] = t.e4p * una * t.tau[2] + Na;
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+        acc[4][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
+                     Pr[4] * Q[4][j];
+    }
+    // viscous part K_ab
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int n = 0; n < 3; ++n)
+        acc[1 + m][1 + n] += mu * ((m == n ? st : 0.0) + s[n] * r[m]) + lam * s[m] * r[n];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) acc[4][1 + n] += mu * (st * t.u[n] + s[n] * unb) + lam * una * r[n];
+    acc[4][4] += kap * st;
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] *= WdetJ;
+  // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103)
+  if (A.prec_on && a == b) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) A.bdiag[25 * (size_t)na + i + 5 * j] += acc[i][j];
+  }
+  if (!A.lhs_on) return;
+  // bc3LHS: constrained rows of node a, columns of node b, unit diagonal
+  const uint8_t ma = A.bcm[na];
+  const uint8_t mb = A.bcm[sel4i(nd[0], nd[1], nd[2], nd[3], b)];
+  if (ma | mb) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        if (((ma >> i) & 1) || ((mb >> j) & 1)) acc[i][j] = 0.0;
+        if (a == b && i == j && ((ma >> i) & 1)) acc[i][j] = 1.0;
+      }
+  }
+  T* lhs = reinterpret_cast<T*>(A.lhs);
+  T* dst = lhs + 25 * (size_t)A.bmap[l * nel + p];
+#pragma unroll
+  for (int j = 0; j < 5; ++j)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) dst[i + 5 * j] += (T)acc[i][j];
 }
 
 // Wall faces: floating convective, pressure, viscous and heat fluxes
@@ -389,9 +440,8 @@ __global__ void __launch_bounds__(64) k_assemble(AsmArgs A) {
 // ordered so (x2-x1) x (x3-x1) points outward.
 struct WallArgs {
   const int* ien;   // 4 planes of nelb, colour order
-  const double* y;
-  const double* x;
-  double* res;
+  const double* node;
+  double* res;      // 5 per node
   long long nshg, nelb;
   int p0, p1;
   PhysParams ph;
@@ -407,9 +457,11 @@ __global__ void __launch_bounds__(64) k_wall(WallArgs A) {
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     nd[a] = A.ien[a * nelb + p];
-    gather_local(A.y, nshg, nd[a], yl[a]);
+    const double* v = A.node + 16 * (size_t)nd[a];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xl[a][c] = A.x[nd[a] + c * nshg];
+    for (int f = 0; f < 5; ++f) yl[a][f] = v[f];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xl[a][c] = v[10 + c];
   }
   // volume metrics for the gradients
   double d[3][3];
@@ -507,29 +559,44 @@ __global__ void __launch_bounds__(64) k_wall(WallArgs A) {
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int f = 0; f < 5; ++f) A.res[nd[a] + f * nshg] += rl[a][f];
+    for (int f = 0; f < 5; ++f) A.res[5 * (size_t)nd[a] + f] += rl[a][f];
 }
 
-// Dirichlet rows of res and BDiag (bc3res.f:25-153, bc3bdg.f:25-330)
-__global__ void k_bc_nodes(double* res, double* bdiag, const uint8_t* bcm, long long nshg,
-                           int prec) {
+// Accumulation layout -> reference planes, with the Dirichlet rows of res
+// and BDiag (bc3res.f:25-153, bc3bdg.f:25-330)
+__global__ void k_bc_nodes(const double* ares, const double* abd, double* res, double* bdiag,
+                           const uint8_t* bcm, long long nshg, int prec) {
   long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= nshg) return;
   const uint8_t m = bcm[n];
-  if (!m) return;
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
-    if (!((m >> f) & 1)) continue;
-    res[n + f * nshg] = 0.0;
-    if (prec) {
+  for (int f = 0; f < 5; ++f) res[n + f * nshg] = ((m >> f) & 1) ? 0.0 : ares[5 * n + f];
+  if (!prec) return;
 #pragma unroll
-      for (int o = 0; o < 5; ++o) {
-        bdiag[n + (f + 5 * o) * nshg] = 0.0;
-        bdiag[n + (o + 5 * f) * nshg] = 0.0;
-      }
-      bdiag[n + (f + 5 * f) * nshg] = 1.0;
+  for (int j = 0; j < 5; ++j)
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      double v = abd[25 * n + i + 5 * j];
+      if (((m >> i) & 1) || ((m >> j) & 1)) v = (i == j) ? 1.0 : 0.0;
+      bdiag[n + (i + 5 * j) * nshg] = v;
     }
-  }
+}
+
+// y, ac (global order, planes) and x -> one 16-double line per node
+__global__ void k_pack_nodes(const double* y, const double* ac, const double* x, double* pack,
+                             long long nshg) {
+  long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= nshg) return;
+  double v[16];
+  gather_local(y, nshg, (int)n, v);
+  gather_local(ac, nshg, (int)n, v + 5);
+  v[10] = x[n];
+  v[11] = x[n + nshg];
+  v[12] = x[n + 2 * nshg];
+  v[13] = v[14] = v[15] = 0.0;
+  double2* d = reinterpret_cast<double2*>(pack + 16 * n);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) d[k] = make_double2(v[2 * k], v[2 * k + 1]);
 }
 
 // Periodic fold: res(master) += res(slaves) in ascending slave order, slave
@@ -613,40 +680,46 @@ int launch_fill(phg_ctx* c, void* p, size_t bytes) {
   return 0;
 }
 
+int launch_pack_nodes(phg_ctx* c) {
+  const long long n = c->nshg;
+  k_pack_nodes<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->y, c->ac, c->x, c->nodepack, n);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   PHG_TRY(upload_quadrature());
+  PHG_TRY(launch_pack_nodes(c));
   AsmArgs A{};
   A.ien = c->ienc;
   A.bmap = c->bmap;
   A.bcm = c->bcm;
-  A.y = c->y;
-  A.ac = c->ac;
-  A.x = c->x;
-  A.res = c->res;
-  A.bdiag = c->bdiag;
+  A.node = c->nodepack;
+  A.res = c->acc_res;
+  A.bdiag = c->acc_bd;
   A.nshg = c->nshg;
   A.nel = c->nel;
   A.lhs_on = lhs;
   A.prec_on = prec;
   A.ph = phys_of(c->prm);
   const bool mixed = c->precision == PHG_MIXED;
-  A.lhs = mixed ? (void*)c->lhs32 : (void*)c->lhs64;
+  A.lhs = mixed ? (void*)c->lhs_aos32 : (void*)c->lhs_aos;
   for (size_t k = 0; k + 1 < c->col_off.size(); ++k) {
     A.p0 = c->col_off[k];
     A.p1 = c->col_off[k + 1];
     int n = A.p1 - A.p0;
     if (n <= 0) continue;
+    const int blocks = (n + kElemsPerBlock - 1) / kElemsPerBlock;
     if (mixed)
-      k_assemble<float><<<(n + 63) / 64, 64, 0, c->stream>>>(A);
+      k_assemble<float><<<blocks, kAsmThreads, 0, c->stream>>>(A);
     else
-      k_assemble<double><<<(n + 63) / 64, 64, 0, c->stream>>>(A);
+      k_assemble<double><<<blocks, kAsmThreads, 0, c->stream>>>(A);
   }
   PHG_HIP(hipGetLastError());
   WallArgs W{};
   W.ien = c->ienbc;
-  W.y = c->y;
-  W.x = c->x;
-  W.res = c->res;
+  W.node = c->nodepack;
+  W.res = c->acc_res;
   W.nshg = c->nshg;
   W.nelb = c->nelb;
   W.ph = A.ph;
@@ -663,7 +736,8 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
 
 int launch_bc_res_bdiag(phg_ctx* c, bool prec) {
   long long n = c->nshg;
-  k_bc_nodes<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->res, c->bdiag, c->bcm, n, prec);
+  k_bc_nodes<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->acc_res, c->acc_bd, c->res, c->bdiag,
+                                                             c->bcm, n, prec);
   PHG_HIP(hipGetLastError());
   if (c->nfold > 0) {
     k_fold_res_bdiag<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(

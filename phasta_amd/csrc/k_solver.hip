@@ -92,10 +92,15 @@ __global__ void k_lu_backward(const double* D, double* r, long long n) {
 }
 
 // ---------------- two-sided block scaling (spsi3pre.f) ----------------
-// lhsK(i,j) <- L_i^{-1} lhsK(i,j) U_j^{-1} with the factors of BDiag.
-template <typename T>
-__global__ void __launch_bounds__(64) k_spsi3pre(T* vals, const int* scol, const long long* grp_slot,
-                                                 const double* D, long long n) {
+// lhsK(i,j) <- L_i^{-1} lhsK(i,j) U_j^{-1} with the factors of BDiag.  Reads
+// the assembled blocks (reference order, 25 contiguous values per block) and
+// writes the 64-row interleaved matvec layout.
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO* __restrict__ vals,
+                                                 const int* __restrict__ s2c,
+                                                 const int* __restrict__ scol,
+                                                 const long long* __restrict__ grp_slot,
+                                                 const double* __restrict__ D, long long n) {
   const long long g = blockIdx.x;
   const int r = threadIdx.x;
   const long long row = g * kGroup + r;
@@ -106,13 +111,15 @@ __global__ void __launch_bounds__(64) k_spsi3pre(T* vals, const int* scol, const
 #pragma unroll
     for (int m = 0; m < i; ++m) lo[i][m] = D[(i + 5 * m) * n + row];
   for (long long s = grp_slot[g]; s < grp_slot[g + 1]; ++s) {
+    const long long k = s2c[s * kGroup + r];
+    if (k < 0) continue;  // padding stays zero
     const long long j = scol[s * kGroup + r];
-    T* blk = vals + s * (25 * kGroup) + r;
+    const TI* src = aos + 25 * k;
     double b[5][5];
 #pragma unroll
     for (int c = 0; c < 5; ++c)
 #pragma unroll
-      for (int f = 0; f < 5; ++f) b[f][c] = (double)blk[(f + 5 * c) * kGroup];
+      for (int f = 0; f < 5; ++f) b[f][c] = (double)src[f + 5 * c];
     // left: L_i^{-1} on every column
 #pragma unroll
     for (int c = 0; c < 5; ++c)
@@ -138,10 +145,11 @@ __global__ void __launch_bounds__(64) k_spsi3pre(T* vals, const int* scol, const
         for (int m = 0; m < c; ++m) acc -= up[m][c] * b[f][m];
         b[f][c] = up[c][c] * acc;
       }
+    TO* blk = vals + s * (25 * kGroup) + r;
 #pragma unroll
     for (int c = 0; c < 5; ++c)
 #pragma unroll
-      for (int f = 0; f < 5; ++f) blk[(f + 5 * c) * kGroup] = (T)b[f][c];
+      for (int f = 0; f < 5; ++f) blk[(f + 5 * c) * kGroup] = (TO)b[f][c];
   }
 }
 
@@ -407,11 +415,11 @@ int launch_i3lu(phg_ctx* c, double* diag, double* r, int code) {
 
 int launch_spsi3pre(phg_ctx* c) {
   if (c->precision == PHG_MIXED)
-    k_spsi3pre<float><<<(int)c->ngroups, kGroup, 0, c->stream>>>(c->lhs32, c->scol, c->grp_slot,
-                                                                 c->bdiag, c->nshg);
+    k_spsi3pre<float, float><<<(int)c->ngroups, kGroup, 0, c->stream>>>(
+        c->lhs_aos32, c->lhs32, c->slot2csr, c->scol, c->grp_slot, c->bdiag, c->nshg);
   else
-    k_spsi3pre<double><<<(int)c->ngroups, kGroup, 0, c->stream>>>(c->lhs64, c->scol, c->grp_slot,
-                                                                  c->bdiag, c->nshg);
+    k_spsi3pre<double, double><<<(int)c->ngroups, kGroup, 0, c->stream>>>(
+        c->lhs_aos, c->lhs64, c->slot2csr, c->scol, c->grp_slot, c->bdiag, c->nshg);
   PHG_HIP(hipGetLastError());
   return 0;
 }

@@ -146,6 +146,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
     return PHG_EINVAL;
   }
   std::vector<int> scol(c->nslots * kGroup), scolp(c->nslots * kGroup);
+  std::vector<int> s2c(c->nslots * kGroup, -1);
   std::vector<uint32_t> c2s(nnz);
 #pragma omp parallel for schedule(static)
   for (long long g = 0; g < c->ngroups; ++g) {
@@ -160,6 +161,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
         if (s < len) {
           col = m->rowp[b0 + s] - 1;
           c2s[b0 + s] = (uint32_t)q;
+          s2c[q] = (int)(b0 + s);
         }
         scol[q] = col;
         scolp[q] = m->iper[col] - 1;
@@ -188,7 +190,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
           bad = 1;
           continue;
         }
-        bmap[(4 * a + b) * nel + p] = c2s[(m->colm[nd[a]] - 1) + (it - lst)];
+        bmap[(4 * a + b) * nel + p] = (uint32_t)((m->colm[nd[a]] - 1) + (it - lst));
       }
     }
   }
@@ -243,6 +245,14 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->scol, c->nslots * kGroup));
   PHG_TRY(dev_alloc(c, &c->scolp, c->nslots * kGroup));
   PHG_TRY(dev_alloc(c, &c->csr2slot, nnz));
+  PHG_TRY(dev_alloc(c, &c->slot2csr, c->nslots * kGroup));
+  if (mixed)
+    PHG_TRY(dev_alloc(c, &c->lhs_aos32, 25 * nnz));
+  else
+    PHG_TRY(dev_alloc(c, &c->lhs_aos, 25 * nnz));
+  PHG_TRY(dev_alloc(c, &c->acc_res, 5 * nshg));
+  PHG_TRY(dev_alloc(c, &c->acc_bd, 25 * nshg));
+  PHG_TRY(dev_alloc(c, &c->nodepack, 16 * nshg));
   PHG_TRY(dev_alloc(c, &c->lhs64, c->nslots * kGroup * 25));
   if (mixed) PHG_TRY(dev_alloc(c, &c->lhs32, c->nslots * kGroup * 25));
   PHG_TRY(dev_alloc(c, &c->res, 5 * nshg));
@@ -278,6 +288,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->scol, scol.data(), scol.size()));
   PHG_TRY(upload(c, c->scolp, scolp.data(), scolp.size()));
   PHG_TRY(upload(c, c->csr2slot, c2s.data(), c2s.size()));
+  PHG_TRY(upload(c, c->slot2csr, s2c.data(), s2c.size()));
   PHG_TRY(launch_fill(c, c->red_count, sizeof(unsigned)));
   PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
   if (mixed) PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
@@ -303,13 +314,13 @@ double* kvec(phg_ctx* c, int k) { return c->krylov + (size_t)k * 5 * c->nshg; }
 int elm_core(phg_ctx* c) {
   const long long n = c->nshg;
   const bool lhs = c->prm.lhs == 1, prec = c->prm.iprec != 0;
-  PHG_TRY(launch_fill(c, c->res, sizeof(double) * 5 * n));
-  if (prec) PHG_TRY(launch_fill(c, c->bdiag, sizeof(double) * 25 * n));
+  PHG_TRY(launch_fill(c, c->acc_res, sizeof(double) * 5 * n));
+  if (prec) PHG_TRY(launch_fill(c, c->acc_bd, sizeof(double) * 25 * n));
   if (lhs) {
     if (c->precision == PHG_MIXED)
-      PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
+      PHG_TRY(launch_fill(c, c->lhs_aos32, sizeof(float) * 25 * c->nnz));
     else
-      PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
+      PHG_TRY(launch_fill(c, c->lhs_aos, sizeof(double) * 25 * c->nnz));
   }
   PHG_TRY(launch_assembly(c, lhs, prec));
   PHG_TRY(launch_bc_res_bdiag(c, prec));
@@ -528,7 +539,8 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
                   c->acold,    c->dy,       c->krylov, c->red_part, c->red_count, c->scal,
-                  c->tmp};
+                  c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
+                  c->nodepack};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->h_scal) hipHostFree(c->h_scal);
@@ -567,9 +579,7 @@ extern "C" int phg_elmgmrs(phg_ctx* c, const double* y, const double* ac, double
   if (BDiag && c->prm.iprec != 0) PHG_TRY(download(c, BDiag, c->bdiag, 25 * n));
   if (lhsK && c->prm.lhs == 1) {
     PHG_TRY(require_fp64(c, "phg_elmgmrs"));
-    PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
-    PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
-    PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
+    PHG_TRY(download(c, lhsK, c->lhs_aos, 25 * c->nnz));
   }
   return sync(c);
 }
@@ -666,9 +676,7 @@ extern "C" int phg_spsi3pre(phg_ctx* c, const double* BDiag, double* lhsK) {
   PHG_TRY(require_fp64(c, "phg_spsi3pre"));
   PHG_TRY(upload(c, c->bdiag, BDiag, 25 * c->nshg));
   PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
-  PHG_TRY(upload(c, c->tmp, lhsK, 25 * c->nnz));
-  PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
-  PHG_TRY(launch_csr_to_slots(c, c->tmp, c->lhs64));
+  PHG_TRY(upload(c, c->lhs_aos, lhsK, 25 * c->nnz));
   PHG_TRY(launch_spsi3pre(c));
   PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
   PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));

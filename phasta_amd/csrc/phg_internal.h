@@ -60,7 +60,7 @@ struct phg_ctx {
   // mesh (device)
   double* x = nullptr;          // 3 planes of nshg
   int* ienc = nullptr;          // 4 planes of nel, colour order, 0-based
-  uint32_t* bmap = nullptr;     // 16 planes of nel: slot-row of block (a,b)
+  uint32_t* bmap = nullptr;     // 16 planes of nel: CSR block index of (a,b)
   int* ienbc = nullptr;         // 4 planes of nelb, colour order, 0-based
   uint8_t* bcm = nullptr;       // constraint mask per node
   int* iper = nullptr;          // 0-based master
@@ -81,8 +81,16 @@ struct phg_ctx {
   int* scol = nullptr;             // nslots*64 raw column (0-based)
   int* scolp = nullptr;            // nslots*64 periodic-master column
   uint32_t* csr2slot = nullptr;    // nnz -> slot-row index
-  double* lhs64 = nullptr;         // nslots*64*25
+  int* slot2csr = nullptr;         // slot-row -> CSR block index, -1 = padding
+  double* lhs64 = nullptr;         // nslots*64*25, scaled matrix for the matvec
   float* lhs32 = nullptr;          // mixed precision copy
+
+  // assembly targets, block/node contiguous (reference lhsK(25,nnz) order)
+  double* lhs_aos = nullptr;       // 25*nnz
+  float* lhs_aos32 = nullptr;      // 25*nnz (mixed)
+  double* acc_res = nullptr;       // 5 per node
+  double* acc_bd = nullptr;        // 25 per node
+  double* nodepack = nullptr;      // 16 per node: y (local order), ac, x
 
   // nodal vectors (SoA planes of nshg)
   double* res = nullptr;
@@ -132,6 +140,7 @@ int launch_combine(phg_ctx* c, double* dy, const double* basis, int k,
                    const double* coef, long long n);
 int launch_stats(phg_ctx* c, double* out);
 int launch_csr_to_slots(phg_ctx* c, const double* ref, double* slots);
+int launch_pack_nodes(phg_ctx* c);
 int launch_slots_to_csr(phg_ctx* c, const double* slots, double* ref);
 int launch_predict(phg_ctx* c);
 int launch_correct(phg_ctx* c);
