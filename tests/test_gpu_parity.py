@@ -5,6 +5,8 @@ reference's operation order; the GPU sums in colour order and regroups the
 tangent algebraically, so agreement is to rounding, not bit-exact), and the
 same Krylov iteration count.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -95,3 +97,23 @@ def test_device_newton_step(pair):
     assert [h["iKs"] for h in its] == [h["iKs"] for h in hist]
     assert rel(y, y_ref) < TOL
     assert rel(ac, ac_ref) < 1e-10  # ac = (y - yold)/dt amplifies y's rounding
+
+
+@pytest.mark.parametrize("N", [2, 3])
+def test_gpu_against_reference_golden(N):
+    """GPU vs the reference's own compiled routines (committed fixtures)."""
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"ref_box{N}.npz")
+    g = np.load(gold)
+    case = mesh.box_case(N, mach=0.3, kspace=10)
+    gpu = solver.GpuSolver(case)
+    try:
+        res, rmes, bd, lk = gpu.elmgmrs(g["y"], g["ac"])
+        assert rel(res, g["res"]) < TOL
+        assert rel(bd, g["BDiag"]) < TOL
+        assert rel(lk, g["lhsK"]) < TOL
+        lu, _ = gpu.i3lu(g["BDiag"], None, "LU_Fact ")
+        assert rel(lu, g["BDiag_lu"]) < TOL
+        assert rel(gpu.sparse_ap(g["lhsK"], g["p"]), g["Ap"]) < TOL
+        assert rel(gpu.spsi3pre(g["BDiag_lu"], g["lhsK"]), g["lhsK_scaled"]) < TOL
+    finally:
+        gpu.close()

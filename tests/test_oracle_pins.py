@@ -1,0 +1,271 @@
+"""Pins of the CPU oracle (oracle/phoracle.c).
+
+1. Bitwise against the reference's own compiled Fortran: committed golden
+   vectors from oracle/_ref (tests/golden/make_golden.py) and, where
+   oracle/_ref is built, live calls of ElmGMRs, e3, i3LU, SparseAp,
+   Spsi3pre, bc3per and itrPredict/itrCorrect/itrBC.
+2. What the mathematics fixes for the parts oracle/_ref does not build
+   (SolGMRs' GMRES loop) and as independent checks of the rest: quadrature
+   rules, a zero residual and an exact tangent (= finite-difference
+   Jacobian) at a uniform state on a fully periodic box, the block-LU
+   factor identity, GMRES = exact solve on the full Krylov space and =
+   Krylov least squares below it, the minIters/etol stopping rule.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+import pyref
+from phasta_amd import config, mesh
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+# ---------------------------------------------------------------- golden
+@pytest.mark.parametrize("N", [2, 3])
+def test_oracle_bitwise_equals_reference_golden(N):
+    g = np.load(os.path.join(GOLD, f"ref_box{N}.npz"))
+    case = mesh.box_case(N, mach=0.3, kspace=10)
+    o = pyoracle.Oracle(case)
+    y, ac = o.itr_predict_bc(case.yold)
+    assert np.array_equal(y, g["y"])
+    res, rmes, bd, lk = o.elmgmrs(g["y"], g["ac"])
+    assert np.array_equal(res, g["res"])
+    assert np.array_equal(bd, g["BDiag"])
+    assert np.array_equal(lk, g["lhsK"])
+    assert not rmes.any()
+    lu = bd.copy(order="F")
+    o.i3lu(lu, None, "LU_Fact ")
+    assert np.array_equal(lu, g["BDiag_lu"])
+    ap = g["p"].copy(order="F")
+    o.sparse_ap(np.asfortranarray(g["lhsK"]), ap)
+    assert np.array_equal(ap, g["Ap"])
+    ls = np.array(g["lhsK"], order="F")
+    o.spsi3pre(np.asfortranarray(g["BDiag_lu"]), ls)
+    assert np.array_equal(ls, g["lhsK_scaled"])
+    y1, ac1 = o.itr_correct_bc(g["y"], g["ac"], case.yold, np.zeros_like(g["ac"]),
+                               np.asfortranarray(g["dy"]))
+    assert np.array_equal(y1, g["y_corr"])
+    assert np.array_equal(ac1, g["ac_corr"])
+
+
+@pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
+def test_oracle_bitwise_equals_live_reference():
+    case = mesh.box_case(4, mach=0.5, kspace=10)
+    o, r = pyoracle.Oracle(case), pyref.Reference(case)
+    y, ac = o.itr_predict_bc(case.yold)
+    y2, ac2 = r.predict_bc(case.yold)
+    assert np.array_equal(y, y2) and np.array_equal(ac, ac2)
+    rng = np.random.default_rng(5)
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    for a, b in zip(o.elmgmrs(y, ac), r.elmgmrs(y, ac)):
+        assert np.array_equal(a, b)
+    # element kernel on perturbed, non-periodic-consistent states
+    for e in rng.choice(case.nel, 8, replace=False):
+        nd = case.ien[e] - 1
+        yl = np.array([[y[g, 3], y[g, 0], y[g, 1], y[g, 2], y[g, 4]] for g in nd])
+        yl *= 1 + 0.05 * rng.standard_normal(yl.shape)
+        acl = 1e3 * rng.standard_normal((4, 5))
+        rl1, eg1 = o.e3(yl, acl, case.x[nd])
+        rl2, eg2 = r.e3(yl, acl, case.x[nd])
+        assert np.array_equal(rl1, rl2) and np.array_equal(eg1, eg2)
+
+
+# ----------------------------------------------------------- quadrature
+def test_quadrature_rules():
+    import ctypes
+    shp = np.zeros((4, 4)); shgl = np.zeros((4, 3, 4)); qwt = np.zeros(4)
+    shpb = np.zeros((3, 4)); shglb = np.zeros((3, 3, 4)); qwtb = np.zeros(3)
+    dp = ctypes.POINTER(ctypes.c_double)
+    pyoracle.lib().orc_quadrature(*(a.ctypes.data_as(dp) for a in
+                                    (shp, shgl, qwt, shpb, shglb, qwtb)))
+    assert abs(qwt.sum() - 4.0 / 3.0) < 1e-15          # reference tet volume 1/6 * 8
+    assert np.allclose(shp.sum(1), 1.0, atol=1e-15)
+    assert np.allclose(shgl.sum(2), 0.0)
+    # the 4-point rule is exact for quadratics: int N_a N_b = (1+d_ab)/20 * V
+    M = (shp.T * qwt) @ shp / qwt.sum()
+    assert np.allclose(M, (np.ones((4, 4)) + np.eye(4)) / 20, atol=1e-15)
+    # wall rule: 3 points on the face (4th shape function ~0), weights 2/3
+    assert np.allclose(shpb[:, 3], 0.0, atol=1e-14)
+    assert np.allclose(qwtb.sum(), 2.0, atol=1e-14)
+
+
+# ---------------------------------------------------- fully periodic box
+def periodic_box(N):
+    c = mesh.box_case(N, jitter=0.1)
+    n = np.arange(c.nshg)
+    i, j, k = n % (N + 1), (n // (N + 1)) % (N + 1), n // (N + 1) ** 2
+    slave = (i == N) | (j == N) | (k == N)
+    c.iBC = np.where(slave, 1024, 0).astype(np.int32)
+    c.iper = ((i % N) + (N + 1) * ((j % N) + (N + 1) * (k % N)) + 1).astype(np.int32)
+    c.ienb = np.zeros((0, 4), dtype=np.int32, order="F")
+    return c
+
+
+def uniform_state(c):
+    y = np.zeros((c.nshg, 5), order="F")
+    y[:, 0], y[:, 1], y[:, 2] = 0.3 * config.C0, 0.05 * config.C0, -0.02 * config.C0
+    y[:, 3], y[:, 4] = 1.0e5, 300.0
+    return y
+
+
+def dense_folded(c, lk):
+    """P^T K P restricted to periodic masters, rows/cols in local dof order."""
+    n = c.nshg
+    K = np.zeros((5 * n, 5 * n))
+    for r in range(n):
+        for k in range(c.colm[r] - 1, c.colm[r + 1] - 1):
+            col = c.rowp[k] - 1
+            K[5 * r:5 * r + 5, 5 * col:5 * col + 5] = lk[:, k].reshape(5, 5, order="F")
+    m = c.iper - 1
+    masters = np.unique(m)
+    idx = {mm: t for t, mm in enumerate(masters)}
+    P = np.zeros((5 * n, 5 * len(masters)))
+    for r in range(n):
+        P[5 * r:5 * r + 5, 5 * idx[m[r]]:5 * idx[m[r]] + 5] = np.eye(5)
+    return P.T @ K @ P, masters
+
+
+def test_uniform_state_residual_vanishes_on_periodic_box():
+    c = periodic_box(3)
+    o = pyoracle.Oracle(c)
+    y = uniform_state(c)
+    res, _, _, _ = o.elmgmrs(y, np.zeros_like(y))
+    scale = 1e5 * (1.0 / 3) ** 2  # pressure x face area
+    assert np.abs(res).max() < 1e-9 * scale
+
+
+def test_tangent_is_fd_jacobian_at_uniform_state():
+    """At a uniform state every term the reference freezes (dA/dY, dtau/dY,
+    the non-conservative Galerkin form) vanishes or cancels over the
+    periodic box, so the assembled tangent must be the Jacobian of the
+    assembled residual with ac = c (y - yold)."""
+    c = periodic_box(2)
+    prm = dict(c.params)
+    o = pyoracle.Oracle(c, prm)
+    y0 = uniform_state(c)
+    cf = prm["almi"] * prm["Dtgl"] / prm["gami"] / prm["alfi"]
+    _, _, _, lk = o.elmgmrs(y0, np.zeros_like(y0))
+    Km, masters = dense_folded(c, lk)
+    loc2glob = [3, 0, 1, 2, 4]
+    hstep = {0: 1.0, 1: 1e-3, 2: 1e-3, 3: 1e-3, 4: 3e-3}   # per local dof
+    m = c.iper - 1
+    J = np.zeros_like(Km)
+    for t, mm in enumerate(masters):
+        for g in range(5):
+            cols = []
+            for sgn in (1, -1):
+                y = y0.copy(order="F")
+                y[m == mm, loc2glob[g]] += sgn * hstep[g]
+                res, _, _, _ = o.elmgmrs(y, np.asfortranarray(cf * (y - y0)))
+                cols.append(res[masters].reshape(-1))
+            J[:, 5 * t + g] = (cols[0] - cols[1]) / (2 * hstep[g])
+    assert rel(J, Km) < 1e-8
+
+
+# ------------------------------------------------------------- block LU
+def test_block_lu_identity_with_unreduced_u45():
+    """i3LU (i3lu.f:41-97): unit-lower L and U with inverted diagonal, but
+    U(4,5) is never reduced, so (L U)(4,5) = A(4,5) + sum_k<4 l4k u5k and
+    every other entry reproduces A."""
+    rng = np.random.default_rng(7)
+    case = mesh.box_case(2)
+    n = case.nshg
+    A = rng.standard_normal((n, 5, 5)) + 6 * np.eye(5)
+    o = pyoracle.Oracle(case)
+    F = np.asfortranarray(A.copy())
+    o.i3lu(F, None, "LU_Fact ")
+    for node in range(n):
+        L = np.tril(F[node], -1) + np.eye(5)
+        U = np.triu(F[node], 1) + np.diag(1.0 / np.diag(F[node]))
+        LU = L @ U
+        extra = L[3, :3] @ U[:3, 4]
+        expect = A[node].copy()
+        expect[3, 4] += extra
+        assert np.allclose(LU, expect, rtol=1e-13, atol=1e-13)
+    # forward then backward applies (L U)^{-1}
+    r = np.asfortranarray(rng.standard_normal((n, 5)))
+    x = r.copy(order="F")
+    o.i3lu(F, x, "forward ")
+    o.i3lu(F, x, "backward")
+    for node in range(n):
+        L = np.tril(F[node], -1) + np.eye(5)
+        U = np.triu(F[node], 1) + np.diag(1.0 / np.diag(F[node]))
+        assert np.allclose(L @ U @ x[node], r[node], rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------- GMRES
+def preconditioned_operator(o, c, y, ac):
+    """Dense A~ (scaled, folded), b~ = L^-1 r, restricted to master dofs."""
+    res, _, bd, lk = o.elmgmrs(y, ac)
+    lu = bd.copy(order="F")
+    o.i3lu(lu, None, "LU_Fact ")
+    b = res.copy(order="F")
+    o.i3lu(lu, b, "forward ")
+    ls = lk.copy(order="F")
+    o.spsi3pre(lu, ls)
+    masters = np.unique(c.iper - 1)
+    cols = []
+    for mm in masters:
+        for f in range(5):
+            e = np.zeros((c.nshg, 5), order="F")
+            e[mm, f] = 1.0
+            o.sparse_ap(ls, e)
+            o.bc3per(e)
+            cols.append(e[masters].reshape(-1))
+    return np.array(cols).T, b[masters].reshape(-1), lu, masters, res
+
+
+def test_gmres_full_krylov_space_is_the_exact_solve():
+    c = mesh.box_case(2, mach=0.3)
+    o = pyoracle.Oracle(c)
+    y, ac = o.itr_predict_bc(c.yold)
+    At, bt, lu, masters, res = preconditioned_operator(o, c, y, ac)
+    nd = At.shape[0]
+    out = o.solgmrs(y, ac, Kspace=nd, etol=1e-13)
+    _, _, _, lk = o.elmgmrs(y, ac)
+    Km, _ = dense_folded(c, lk)
+    x = np.linalg.solve(Km, res[masters].reshape(-1))
+    assert rel(out["Dy"][masters].reshape(-1), x) < 1e-9
+    assert not out["Dy"][np.setdiff1d(np.arange(c.nshg), masters)].any()
+
+
+@pytest.mark.parametrize("k", [3, 7])
+def test_gmres_iterate_is_krylov_least_squares(k):
+    c = mesh.box_case(3, mach=0.3)
+    o = pyoracle.Oracle(c)
+    y, ac = o.itr_predict_bc(c.yold)
+    At, bt, lu, masters, _ = preconditioned_operator(o, c, y, ac)
+    V = np.empty((bt.size, k))
+    v = bt / np.linalg.norm(bt)
+    for j in range(k):                      # orthonormal Krylov basis
+        V[:, j] = v
+        v = At @ v
+        v -= V[:, :j + 1] @ (V[:, :j + 1].T @ v)
+        v -= V[:, :j + 1] @ (V[:, :j + 1].T @ v)
+        v /= np.linalg.norm(v)
+    coef, *_ = np.linalg.lstsq(At @ V, bt, rcond=None)
+    z = np.zeros((c.nshg, 5), order="F")
+    z[masters] = (V @ coef).reshape(-1, 5)
+    o.i3lu(lu, z, "backward")
+    out = o.solgmrs(y, ac, Kspace=k, etol=0.0)
+    assert out["iKs"] == k
+    assert rel(out["Dy"], z) < 1e-9
+
+
+def test_stopping_rule_and_min_iters():
+    c = mesh.box_case(3, mach=0.3)
+    o = pyoracle.Oracle(c)
+    y, ac = o.itr_predict_bc(c.yold)
+    a = o.solgmrs(y, ac, Kspace=20, etol=1e-1, minIters=0)
+    assert a["echeck"] <= a["epsnrm"] and a["iKs"] < 10
+    b = o.solgmrs(y, ac, Kspace=20, etol=1e-1, minIters=10)
+    assert b["iKs"] == 10                 # solgmr.f:669: converged but iKs < minIters
+    assert b["echeck"] <= a["echeck"]
