@@ -109,10 +109,17 @@ int phg_solgmrs(phg_ctx* ctx, const double* y, const double* ac,
  * the state stays in HBM between calls. */
 int phg_state_upload(phg_ctx* ctx, const double* yold, const double* acold);
 int phg_state_download(phg_ctx* ctx, double* y, double* ac, double* Dy);
+int phg_state_download_old(phg_ctx* ctx, double* yold, double* acold);
+/* Current iterate (y, ac) from the host, e.g. to resume inside a step. */
+int phg_state_upload_cur(phg_ctx* ctx, const double* y, const double* ac);
 /* itrPredict + itrBC                            itrPC.f:56-121, itrbc.f */
 int phg_predict(phg_ctx* ctx);
 /* SolGMRs + itrCorrect + itrBC on the resident state  itrdrv.f:477-599 */
 int phg_newton_iteration(phg_ctx* ctx, phg_stats* st);
+
+/* itrUpdate + itrBC on (yold, acold): end of a time step
+ *                                   itrPC.f:176-208, itrdrv.f:651-652 */
+int phg_update(phg_ctx* ctx);
 
 /* Fine-grained entry points (unit parity; host arrays in/out). */
 /* SparseAp(iper,ilwork,iBC,col,row,lhsK,p)          sparseap.f:1     */

@@ -1190,6 +1190,15 @@ void orc_itr_correct(long long nshg, const orc_params* P, double* y,
   for (long long i = 0; i < 5 * nshg; ++i) ac[i] = acold[i] * fct1 + (y[i] - yold[i]) * fct2;
 }
 
+/* itrPC.f:176-208, branch without level-set redistancing (iLset != 2) */
+void orc_itr_update(long long nshg, const orc_params* P, double* yold,
+                    double* acold, const double* y, const double* ac) {
+  double fct2 = ONE / P->almi;
+  double fct3 = ONE / P->alfi;
+  for (long long i = 0; i < 5 * nshg; ++i) acold[i] = acold[i] + (ac[i] - acold[i]) * fct2;
+  for (long long i = 0; i < 5 * nshg; ++i) yold[i] = yold[i] + (y[i] - yold[i]) * fct3;
+}
+
 /* itrbc.f:60-178 (codes used by this path) */
 int orc_itr_bc(const orc_mesh* m, double* y, double* ac) {
   const long long nshg = m->nshg;

@@ -100,6 +100,8 @@ void orc_itr_predict(long long nshg, const orc_params* pr, const double* yold,
 void orc_itr_correct(long long nshg, const orc_params* pr, double* y,
                      double* ac, const double* yold, const double* acold,
                      const double* Dy);
+void orc_itr_update(long long nshg, const orc_params* pr, double* yold,
+                    double* acold, const double* y, const double* ac);
 int orc_itr_bc(const orc_mesh* m, double* y, double* ac);
 
 #ifdef __cplusplus

@@ -151,6 +151,18 @@ class Oracle:
             raise RuntimeError(f"orc_itr_bc rc={rc}")
         return y, ac
 
+    def itr_update_bc(self, yold, acold, y, ac):
+        """itrUpdate + itrBC on (yold, acold) (itrdrv.f:651-652)."""
+        n = self.case.nshg
+        yold, acold = yold.copy(order="F"), acold.copy(order="F")
+        L = lib()
+        L.orc_itr_update(_i64(n), ctypes.byref(self.p()), _p(yold), _p(acold),
+                         _p(np.asfortranarray(y)), _p(np.asfortranarray(ac)))
+        rc = L.orc_itr_bc(ctypes.byref(self.m), _p(yold), _p(acold))
+        if rc:
+            raise RuntimeError(f"orc_itr_bc rc={rc}")
+        return yold, acold
+
     def i3lu(self, diag, r, code):
         codes = {"LU_Fact ": 0, "forward ": 1, "backward": 2, "product ": 3}
         rp = _p(r) if r is not None else None
@@ -182,3 +194,13 @@ class Oracle:
             y, ac = self.itr_correct_bc(y, ac, yold, acold, out["Dy"])
             hist.append(out)
         return y, ac, hist
+
+    def time_steps(self, yold, acold, steps, newton=1):
+        """`steps` time steps of itrdrv.f:398-652: predict, Newton loop,
+        itrUpdate + itrBC.  Returns (yold, acold, history)."""
+        hist = []
+        for _ in range(steps):
+            y, ac, h = self.newton_step(yold, acold, newton)
+            yold, acold = self.itr_update_bc(yold, acold, y, ac)
+            hist += h
+        return yold, acold, hist

@@ -70,6 +70,12 @@ class Reference:
                                _p(np.asfortranarray(acold)), _p(np.asfortranarray(dy)))
         return y, ac
 
+    def update_bc(self, yold, acold, y, ac):
+        yold, acold = yold.copy(order="F"), acold.copy(order="F")
+        lib().phref_update_bc(_p(yold), _p(acold), _p(np.asfortranarray(y)),
+                              _p(np.asfortranarray(ac)))
+        return yold, acold
+
     def elmgmrs(self, y, ac):
         n, nnz = self.case.nshg, self.case.nnz_tot
         res = np.zeros((n, 5), order="F")

@@ -305,6 +305,17 @@ c.... boundary element blocks (genbkb.f)
       call itrBC(y, ac, hiBC, hBC, hiper, hilwork)
       end subroutine
 
+      subroutine phref_update_bc(yold, acold, y, ac)
+     &           bind(C, name='phref_update_bc')
+      use iso_c_binding
+      use phref_data
+      include "common.h"
+      real(c_double) :: yold(nshg,5), acold(nshg,5), y(nshg,5),
+     &                  ac(nshg,5)
+      call itrUpdate(yold, acold, y, ac)
+      call itrBC(yold, acold, hiBC, hBC, hiper, hilwork)
+      end subroutine
+
       subroutine phref_correct_bc(y, ac, yold, acold, Dy)
      &           bind(C, name='phref_correct_bc')
       use iso_c_binding

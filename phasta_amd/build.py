@@ -3,6 +3,9 @@
 * ``phasta_amd/libphmesh.so``  host-only box-mesh generator (g++/OpenMP)
 * ``phasta_amd/libphgpu.so``   the HIP library behind ``include/phgpu.h``,
   gfx950 only, packed-FP32 instructions disabled
+* ``phasta_amd/fortran/phg_boxdrv``  Fortran host (amdflang, iso_c_binding
+  module ``phgpu``) that drives the library in the shape of the
+  reference's time loop
 """
 import os
 import subprocess
@@ -37,12 +40,26 @@ def build_gpu():
     os.replace(tmp, out)
 
 
+def build_fortran():
+    fdir = os.path.join(PKG, "fortran")
+    mdir = os.path.join(fdir, "_build")
+    os.makedirs(mdir, exist_ok=True)
+    flang = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+    out = os.path.join(fdir, "phg_boxdrv")
+    _run([flang, "-O2", "-ffp-contract=off", "-module-dir", mdir,
+          os.path.join(fdir, "phgpu_mod.f90"), os.path.join(fdir, "phg_boxdrv.f90"),
+          f"-L{PKG}", "-lphgpu", "-lphmesh", "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+
+
 def build_all():
     build_mesh()
     build_gpu()
+    build_fortran()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "mesh": build_mesh, "gpu": build_gpu}[w]()
+        {"all": build_all, "mesh": build_mesh, "gpu": build_gpu,
+         "fortran": build_fortran}[w]()

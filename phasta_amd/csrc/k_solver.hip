@@ -389,6 +389,15 @@ __global__ void k_per_copy(double* y, double* ac, const int* master, const int* 
   }
 }
 
+// itrUpdate (itrPC.f:204-207): acold += (ac-acold)/almi, yold += (y-yold)/alfi
+__global__ void k_update(double* yold, double* acold, const double* y, const double* ac,
+                         long long n5, double fct2, double fct3) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n5) return;
+  acold[i] = acold[i] + (ac[i] - acold[i]) * fct2;
+  yold[i] = yold[i] + (y[i] - yold[i]) * fct3;
+}
+
 inline int grid_for(long long n, int threads) {
   long long b = (n + threads - 1) / threads;
   return (int)(b < 1 ? 1 : b);
@@ -510,7 +519,7 @@ int launch_predict(phg_ctx* c) {
   const double fac = 1.0 - c->prm.almi / c->prm.gami;
   k_predict<<<grid_for(n5, 256), 256, 0, c->stream>>>(c->yold, c->acold, c->y, c->ac, n5, fac);
   PHG_HIP(hipGetLastError());
-  return launch_itrbc(c);
+  return launch_itrbc(c, c->y, c->ac);
 }
 
 int launch_correct(phg_ctx* c) {
@@ -519,14 +528,22 @@ int launch_correct(phg_ctx* c) {
   k_correct<<<grid_for(c->nshg, 256), 256, 0, c->stream>>>(c->y, c->ac, c->yold, c->acold, c->dy,
                                                            c->nshg, fct1, fct2);
   PHG_HIP(hipGetLastError());
-  return launch_itrbc(c);
+  return launch_itrbc(c, c->y, c->ac);
 }
 
-int launch_itrbc(phg_ctx* c) {
-  k_bc_y<<<grid_for(c->nshg, 256), 256, 0, c->stream>>>(c->y, c->bcm, c->BCv, c->nshg);
+int launch_update(phg_ctx* c) {
+  const long long n5 = 5 * c->nshg;
+  k_update<<<grid_for(n5, 256), 256, 0, c->stream>>>(c->yold, c->acold, c->y, c->ac, n5,
+                                                     1.0 / c->prm.almi, 1.0 / c->prm.alfi);
+  PHG_HIP(hipGetLastError());
+  return launch_itrbc(c, c->yold, c->acold);
+}
+
+int launch_itrbc(phg_ctx* c, double* y, double* ac) {
+  k_bc_y<<<grid_for(c->nshg, 256), 256, 0, c->stream>>>(y, c->bcm, c->BCv, c->nshg);
   PHG_HIP(hipGetLastError());
   if (c->nfold > 0) {
-    k_per_copy<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(c->y, c->ac, c->fold_master,
+    k_per_copy<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(y, ac, c->fold_master,
                                                               c->fold_ptr, c->fold_slave, c->nfold,
                                                               c->nshg);
     PHG_HIP(hipGetLastError());

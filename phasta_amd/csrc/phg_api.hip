@@ -621,6 +621,20 @@ extern "C" int phg_state_download(phg_ctx* c, double* y, double* ac, double* Dy)
   return sync(c);
 }
 
+extern "C" int phg_state_upload_cur(phg_ctx* c, const double* y, const double* ac) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(upload(c, c->y, y, 5 * c->nshg));
+  PHG_TRY(upload(c, c->ac, ac, 5 * c->nshg));
+  return sync(c);
+}
+
+extern "C" int phg_state_download_old(phg_ctx* c, double* yold, double* acold) {
+  PHG_TRY(check_ctx(c));
+  if (yold) PHG_TRY(download(c, yold, c->yold, 5 * c->nshg));
+  if (acold) PHG_TRY(download(c, acold, c->acold, 5 * c->nshg));
+  return sync(c);
+}
+
 extern "C" int phg_predict(phg_ctx* c) {
   PHG_TRY(check_ctx(c));
   PHG_TRY(launch_predict(c));
@@ -633,6 +647,12 @@ extern "C" int phg_newton_iteration(phg_ctx* c, phg_stats* st) {
   if (!st) st = &local;
   PHG_TRY(sol_core(c, st));
   PHG_TRY(launch_correct(c));
+  return 0;
+}
+
+extern "C" int phg_update(phg_ctx* c) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(launch_update(c));
   return 0;
 }
 

@@ -144,4 +144,5 @@ int launch_pack_nodes(phg_ctx* c);
 int launch_slots_to_csr(phg_ctx* c, const double* slots, double* ref);
 int launch_predict(phg_ctx* c);
 int launch_correct(phg_ctx* c);
-int launch_itrbc(phg_ctx* c);
+int launch_update(phg_ctx* c);
+int launch_itrbc(phg_ctx* c, double* y, double* ac);

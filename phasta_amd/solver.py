@@ -69,9 +69,9 @@ class PhgStats(ctypes.Structure):
 # Every symbol declared in include/phgpu.h.
 EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_info",
            "phg_elmgmrs", "phg_solgmrs", "phg_state_upload", "phg_state_download",
-           "phg_predict", "phg_newton_iteration", "phg_sparse_ap", "phg_i3lu",
-           "phg_spsi3pre", "phg_bc3per", "phg_bench_spmv", "phg_bench_assembly",
-           "phg_sync")
+           "phg_state_download_old", "phg_state_upload_cur", "phg_predict",
+           "phg_newton_iteration", "phg_update", "phg_sparse_ap", "phg_i3lu", "phg_spsi3pre", "phg_bc3per",
+           "phg_bench_spmv", "phg_bench_assembly", "phg_sync")
 
 
 def lib():
@@ -218,6 +218,21 @@ class GpuSolver:
         st = PhgStats()
         _check(lib().phg_newton_iteration(self.h, ctypes.byref(st)))
         return st.as_dict()
+
+    def update(self):
+        """itrUpdate + itrBC on the resident (yold, acold)."""
+        _check(lib().phg_update(self.h))
+
+    def upload_current(self, y, ac):
+        n = self.case.nshg
+        _check(lib().phg_state_upload_cur(self.h, _p(_f64(y, (n, 5))), _p(_f64(ac, (n, 5)))))
+
+    def download_old(self):
+        n = self.case.nshg
+        yold = np.zeros((n, 5), order="F")
+        acold = np.zeros((n, 5), order="F")
+        _check(lib().phg_state_download_old(self.h, _p(yold), _p(acold)))
+        return yold, acold
 
     def download_state(self):
         n = self.case.nshg

@@ -19,6 +19,9 @@ from phasta_amd import mesh  # noqa: E402
 import pyref  # noqa: E402
 
 
+UPDATE_ALPHA = dict(almi=0.75, alfi=0.6, gami=0.65)
+
+
 def make(N, seed):
     case = mesh.box_case(N, mach=0.3, kspace=10)
     ref = pyref.Reference(case)
@@ -32,9 +35,14 @@ def make(N, seed):
     lk_s = ref.spsi3pre(bd_lu, lk)
     dy = np.asfortranarray(1e-3 * rng.standard_normal((case.nshg, 5)))
     y1, ac1 = ref.correct_bc(y, ac, case.yold, np.zeros_like(ac), dy)
+    # itrUpdate with distinct alpha factors, so that a swapped almi/alfi shows
+    acold = np.asfortranarray(1e2 * rng.standard_normal((case.nshg, 5)))
+    upd = dict(case.params, **UPDATE_ALPHA)
+    yold1, acold1 = pyref.Reference(case, params=upd).update_bc(case.yold, acold, y1, ac1)
     np.savez_compressed(os.path.join(HERE, f"ref_box{N}.npz"), N=N, y=y, ac=ac, res=res,
                         BDiag=bd, lhsK=lk, BDiag_lu=bd_lu, p=p, Ap=ap, lhsK_scaled=lk_s,
-                        dy=dy, y_corr=y1, ac_corr=ac1)
+                        dy=dy, y_corr=y1, ac_corr=ac1, acold=acold, yold_upd=yold1,
+                        acold_upd=acold1)
 
 
 if __name__ == "__main__":

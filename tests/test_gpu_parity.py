@@ -99,6 +99,23 @@ def test_device_newton_step(pair):
     assert rel(ac, ac_ref) < 1e-10  # ac = (y - yold)/dt amplifies y's rounding
 
 
+def test_device_time_steps(pair):
+    """Two time steps of predict / 2 Newton iterations / update on HBM."""
+    case, orc, gpu = pair
+    acold = np.zeros((case.nshg, 5), order="F")
+    yo_ref, aco_ref, hist = orc.time_steps(case.yold, acold, steps=2, newton=2)
+    gpu.upload_state(case.yold, acold)
+    its = []
+    for _ in range(2):
+        gpu.predict()
+        its += [gpu.newton_iteration() for _ in range(2)]
+        gpu.update()
+    yo, aco = gpu.download_old()
+    assert [h["iKs"] for h in its] == [h["iKs"] for h in hist]
+    assert rel(yo, yo_ref) < TOL
+    assert rel(aco, aco_ref) < 1e-10
+
+
 @pytest.mark.parametrize("N", [2, 3])
 def test_gpu_against_reference_golden(N):
     """GPU vs the reference's own compiled routines (committed fixtures)."""
@@ -115,5 +132,13 @@ def test_gpu_against_reference_golden(N):
         assert rel(lu, g["BDiag_lu"]) < TOL
         assert rel(gpu.sparse_ap(g["lhsK"], g["p"]), g["Ap"]) < TOL
         assert rel(gpu.spsi3pre(g["BDiag_lu"], g["lhsK"]), g["lhsK_scaled"]) < TOL
+        # itrUpdate with the fixtures' alpha factors
+        gpu.set_params(almi=0.75, alfi=0.6, gami=0.65)
+        gpu.upload_state(case.yold, g["acold"])
+        gpu.upload_current(g["y_corr"], g["ac_corr"])
+        gpu.update()
+        yo, aco = gpu.download_old()
+        assert rel(yo, g["yold_upd"]) < TOL
+        assert rel(aco, g["acold_upd"]) < TOL
     finally:
         gpu.close()

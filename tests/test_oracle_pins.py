@@ -3,7 +3,7 @@
 1. Bitwise against the reference's own compiled Fortran: committed golden
    vectors from oracle/_ref (tests/golden/make_golden.py) and, where
    oracle/_ref is built, live calls of ElmGMRs, e3, i3LU, SparseAp,
-   Spsi3pre, bc3per and itrPredict/itrCorrect/itrBC.
+   Spsi3pre, bc3per and itrPredict/itrCorrect/itrUpdate/itrBC.
 2. What the mathematics fixes for the parts oracle/_ref does not build
    (SolGMRs' GMRES loop) and as independent checks of the rest: quadrature
    rules, a zero residual and an exact tangent (= finite-difference
@@ -21,6 +21,8 @@ import pyref
 from phasta_amd import config, mesh
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+# alpha factors the itrUpdate fixtures were made with (tests/golden/make_golden.py)
+UPDATE_ALPHA = dict(almi=0.75, alfi=0.6, gami=0.65)
 
 
 def rel(a, b):
@@ -54,6 +56,11 @@ def test_oracle_bitwise_equals_reference_golden(N):
                                np.asfortranarray(g["dy"]))
     assert np.array_equal(y1, g["y_corr"])
     assert np.array_equal(ac1, g["ac_corr"])
+    ou = pyoracle.Oracle(case, params=dict(case.params, **UPDATE_ALPHA))
+    yo, aco = ou.itr_update_bc(case.yold, np.asfortranarray(g["acold"]), g["y_corr"],
+                               g["ac_corr"])
+    assert np.array_equal(yo, g["yold_upd"])
+    assert np.array_equal(aco, g["acold_upd"])
 
 
 @pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
@@ -66,6 +73,10 @@ def test_oracle_bitwise_equals_live_reference():
     rng = np.random.default_rng(5)
     ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
     for a, b in zip(o.elmgmrs(y, ac), r.elmgmrs(y, ac)):
+        assert np.array_equal(a, b)
+    acold = np.asfortranarray(1e2 * rng.standard_normal(ac.shape))
+    for a, b in zip(o.itr_update_bc(case.yold, acold, y, ac),
+                    r.update_bc(case.yold, acold, y, ac)):
         assert np.array_equal(a, b)
     # element kernel on perturbed, non-periodic-consistent states
     for e in rng.choice(case.nel, 8, replace=False):
@@ -269,3 +280,23 @@ def test_stopping_rule_and_min_iters():
     b = o.solgmrs(y, ac, Kspace=20, etol=1e-1, minIters=10)
     assert b["iKs"] == 10                 # solgmr.f:669: converged but iKs < minIters
     assert b["echeck"] <= a["echeck"]
+
+
+def test_rstat_norms_and_abort_rules(box4):
+    """rstat.f:95-108 norms, rstatCheck.f:36-79 divergence and tnanq.f:10-28
+    NaN aborts of the oracle's SolGMRs."""
+    o = pyoracle.Oracle(box4)
+    y, ac = o.itr_predict_bc(box4.yold)
+    out = o.solgmrs(y, ac)
+    n = box4.nshg
+    assert out["totres"][0] == pytest.approx(np.linalg.norm(out["res"]) / n, rel=1e-14)
+    assert out["totres"][1] == pytest.approx(np.linalg.norm(out["rmes"]) / n, rel=1e-14)
+    # a residual 1e4 times above the last accepted one aborts (code 11)
+    o.reslast = 1e-4 * np.sqrt(np.sum(out["res"] ** 2) / n) * 0.999
+    with pytest.raises(RuntimeError, match="rc=11"):
+        o.solgmrs(y, ac)
+    o.reslast = 0.0
+    bad = y.copy(order="F")
+    bad[3, 2] = np.nan
+    with pytest.raises(RuntimeError, match="rc=10"):
+        o.solgmrs(bad, ac)
