@@ -311,58 +311,59 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
   if (!A.lhs_on && !(A.prec_on && a == b)) return;
 
   // ---------------- tangent block (a,b) ----------------
-  const double st = s[0] * r[0] + s[1] * r[1] + s[2] * r[2];
-  double acc[5][5];
+  // Arithmetic type of the tangent: double, or float for the mixed-precision
+  // path (the residual above and the nodal BDiag sums stay in double).
+  using R = T;
+  R sR[3], rR[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    sR[k] = (R)s[k];
+    rR[k] = (R)r[k];
+  }
+  const R st = sR[0] * rR[0] + sR[1] * rR[1] + sR[2] * rR[2];
+  const R muR = (R)mu, lamR = (R)lam, kapR = (R)kap, cfacR = (R)cfac;
+  R acc[5][5];
 #pragma unroll
   for (int i = 0; i < 5; ++i)
 #pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] = 0.0;
+    for (int j = 0; j < 5; ++j) acc[i][j] = (R)0;
 #pragma unroll 1
   for (int q = 0; q < 4; ++q) {
     const double* o = sth[el][q];
-    Thermo t;
-    t.u[0] = o[0];
-    t.u[1] = o[1];
-    t.u[2] = o[2];
-    t.rho = o[3];
-    t.drdp = o[4];
-    t.drdT = o[5];
-    t.e1p = o[6];
-    t.e2p = o[7];
-    t.e3p = o[8];
-    t.e4p = o[9];
-    t.tau[0] = o[10];
-    t.tau[1] = o[11];
-    t.tau[2] = o[12];
-    const double Na = sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
-    const double Nb = sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
-    const double una = t.u[0] * s[0] + t.u[1] * s[1] + t.u[2] * s[2];
-    const double unb = t.u[0] * r[0] + t.u[1] * r[1] + t.u[2] * r[2];
-    const double w = unb + cfac * Nb;
+    const R u0 = (R)o[0], u1 = (R)o[1], u2 = (R)o[2];
+    const R uq[3] = {u0, u1, u2};
+    const R rho = (R)o[3], drdp = (R)o[4], drdT = (R)o[5];
+    const R e1p = (R)o[6], e2p = (R)o[7], e3p = (R)o[8], e4p = (R)o[9];
+    const R tau0 = (R)o[10], tau1 = (R)o[11], tau2 = (R)o[12];
+    const R Na = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
+    const R Nb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
+    const R una = u0 * sR[0] + u1 * sR[1] + u2 * sR[2];
+    const R unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
+    const R w = unb + cfacR * Nb;
     // Q = A(s_b) + c N_b A0
-    double Q[5][5];
-    Q[0][0] = t.drdp * w;
-    Q[0][4] = t.drdT * w;
+    R Q[5][5];
+    Q[0][0] = drdp * w;
+    Q[0][4] = drdT * w;
 #pragma unroll
-    for (int n = 0; n < 3; ++n) Q[0][1 + n] = t.rho * r[n];
+    for (int n = 0; n < 3; ++n) Q[0][1 + n] = rho * rR[n];
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      Q[1 + m][0] = t.drdp * t.u[m] * w + r[m];
+      Q[1 + m][0] = drdp * uq[m] * w + rR[m];
 #pragma unroll
-      for (int n = 0; n < 3; ++n) Q[1 + m][1 + n] = t.rho * (t.u[m] * r[n] + (m == n ? w : 0.0));
-      Q[1 + m][4] = t.drdT * t.u[m] * w;
+      for (int n = 0; n < 3; ++n) Q[1 + m][1 + n] = rho * (uq[m] * rR[n] + (m == n ? w : (R)0));
+      Q[1 + m][4] = drdT * uq[m] * w;
     }
-    Q[4][0] = t.e2p * unb + cfac * Nb * t.e1p;
+    Q[4][0] = e2p * unb + cfacR * Nb * e1p;
 #pragma unroll
-    for (int n = 0; n < 3; ++n) Q[4][1 + n] = t.e3p * r[n] + t.rho * t.u[n] * w;
-    Q[4][4] = t.e4p * w;
+    for (int n = 0; n < 3; ++n) Q[4][1 + n] = e3p * rR[n] + rho * uq[n] * w;
+    Q[4][4] = e4p * w;
     // acc += P Q with P = A(s_a) diag(tau) + N_a I, one row of P at a time
     {
-      double Pr[5];
-      Pr[0] = t.drdp * una * t.tau[0] + Na;
+      R Pr[5];
+      Pr[0] = drdp * una * tau0 + Na;
 #pragma unroll
-      for (int n = 0; n < 3; ++n) Pr[1 + n] = t.rho * s[n] * t.tau[1];
-      Pr[4] = t.drdT * una * t.tau[2];
+      for (int n = 0; n < 3; ++n) Pr[1 + n] = rho * sR[n] * tau1;
+      Pr[4] = drdT * una * tau2;
 #pragma unroll
       for (int j = 0; j < 5; ++j)
         acc[0][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
@@ -370,23 +371,23 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
     }
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      double Pr[5];
-      Pr[0] = (t.drdp * t.u[m] * una + s[m]) * t.tau[0];
+      R Pr[5];
+      Pr[0] = (drdp * uq[m] * una + sR[m]) * tau0;
 #pragma unroll
       for (int n = 0; n < 3; ++n)
-        Pr[1 + n] = t.rho * (t.u[m] * s[n] + (m == n ? una : 0.0)) * t.tau[1] + (m == n ? Na : 0.0);
-      Pr[4] = t.drdT * t.u[m] * una * t.tau[2];
+        Pr[1 + n] = rho * (uq[m] * sR[n] + (m == n ? una : (R)0)) * tau1 + (m == n ? Na : (R)0);
+      Pr[4] = drdT * uq[m] * una * tau2;
 #pragma unroll
       for (int j = 0; j < 5; ++j)
         acc[1 + m][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
                          Pr[4] * Q[4][j];
     }
     {
-      double Pr[5];
-      Pr[0] = t.e2p * una * t.tau[0];
+      R Pr[5];
+      Pr[0] = e2p * una * tau0;
 #pragma unroll
-      for (int n = 0; n < 3; ++n) Pr[1 + n] = (t.e3p * s[n] + t.rho * t.u[n] * una) * t.tau[1];
-      Pr[4] = t.e4p * una * t.tau[2] + Na;
+      for (int n = 0; n < 3; ++n) Pr[1 + n] = (e3p * sR[n] + rho * uq[n] * una) * tau1;
+      Pr[4] = e4p * una * tau2 + Na;
 #pragma unroll
       for (int j = 0; j < 5; ++j)
         acc[4][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
@@ -397,22 +398,25 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int n = 0; n < 3; ++n)
-        acc[1 + m][1 + n] += mu * ((m == n ? st : 0.0) + s[n] * r[m]) + lam * s[m] * r[n];
+        acc[1 + m][1 + n] += muR * ((m == n ? st : (R)0) + sR[n] * rR[m]) + lamR * sR[m] * rR[n];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) acc[4][1 + n] += mu * (st * t.u[n] + s[n] * unb) + lam * una * r[n];
-    acc[4][4] += kap * st;
+    for (int n = 0; n < 3; ++n)
+      acc[4][1 + n] += muR * (st * uq[n] + sR[n] * unb) + lamR * una * rR[n];
+    acc[4][4] += kapR * st;
   }
   if (!valid) return;
+  const R WdetJR = (R)WdetJ;
 #pragma unroll
   for (int i = 0; i < 5; ++i)
 #pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] *= WdetJ;
-  // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103)
+    for (int j = 0; j < 5; ++j) acc[i][j] *= WdetJR;
+  // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103),
+  // summed over the node's elements in double
   if (A.prec_on && a == b) {
 #pragma unroll
     for (int j = 0; j < 5; ++j)
 #pragma unroll
-      for (int i = 0; i < 5; ++i) A.bdiag[25 * (size_t)na + i + 5 * j] += acc[i][j];
+      for (int i = 0; i < 5; ++i) A.bdiag[25 * (size_t)na + i + 5 * j] += (double)acc[i][j];
   }
   if (!A.lhs_on) return;
   // bc3LHS: constrained rows of node a, columns of node b, unit diagonal
@@ -423,8 +427,8 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
     for (int i = 0; i < 5; ++i)
 #pragma unroll
       for (int j = 0; j < 5; ++j) {
-        if (((ma >> i) & 1) || ((mb >> j) & 1)) acc[i][j] = 0.0;
-        if (a == b && i == j && ((ma >> i) & 1)) acc[i][j] = 1.0;
+        if (((ma >> i) & 1) || ((mb >> j) & 1)) acc[i][j] = (R)0;
+        if (a == b && i == j && ((ma >> i) & 1)) acc[i][j] = (R)1;
       }
   }
   T* lhs = reinterpret_cast<T*>(A.lhs);

@@ -155,7 +155,9 @@ __global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO*
 
 // ---------------- block-CSR matvec (sparseap.f) ----------------
 // q = K p(iper(:)); one lane per row, one wavefront per 64-row group.
-template <typename T>
+// T: stored value type; Acc: type of the products and row sums (float for
+// the mixed-precision inner matvec, double otherwise).
+template <typename T, typename Acc>
 __global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const int* __restrict__ scolp,
                                               const long long* __restrict__ grp_slot,
                                               const double* __restrict__ p, double* __restrict__ q,
@@ -164,26 +166,41 @@ __global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const 
   const long long g = row / kGroup;
   const int r = (int)(row % kGroup);
   if (row >= n) return;
-  double t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+  Acc t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
   const long long s0 = grp_slot[g], s1 = grp_slot[g + 1];
   for (long long s = s0; s < s1; ++s) {
     const long long j = scolp[s * kGroup + r];
-    const double p0 = p[j], p1 = p[j + n], p2 = p[j + 2 * n], p3 = p[j + 3 * n], p4 = p[j + 4 * n];
+    const Acc p0 = (Acc)p[j], p1 = (Acc)p[j + n], p2 = (Acc)p[j + 2 * n], p3 = (Acc)p[j + 3 * n],
+              p4 = (Acc)p[j + 4 * n];
     const T* blk = vals + s * (25 * kGroup) + r;
-    double a[25];
+    Acc a[25];
 #pragma unroll
-    for (int e = 0; e < 25; ++e) a[e] = (double)blk[e * kGroup];
+    for (int e = 0; e < 25; ++e) a[e] = (Acc)blk[e * kGroup];
     t0 += a[0] * p0 + a[5] * p1 + a[10] * p2 + a[15] * p3 + a[20] * p4;
     t1 += a[1] * p0 + a[6] * p1 + a[11] * p2 + a[16] * p3 + a[21] * p4;
     t2 += a[2] * p0 + a[7] * p1 + a[12] * p2 + a[17] * p3 + a[22] * p4;
     t3 += a[3] * p0 + a[8] * p1 + a[13] * p2 + a[18] * p3 + a[23] * p4;
     t4 += a[4] * p0 + a[9] * p1 + a[14] * p2 + a[19] * p3 + a[24] * p4;
   }
-  q[row] = t0;
-  q[row + n] = t1;
-  q[row + 2 * n] = t2;
-  q[row + 3 * n] = t3;
-  q[row + 4 * n] = t4;
+  q[row] = (double)t0;
+  q[row + n] = (double)t1;
+  q[row + 2 * n] = (double)t2;
+  q[row + 3 * n] = (double)t3;
+  q[row + 4 * n] = (double)t4;
+}
+
+// r = b - q (true residual of the refinement loop)
+__global__ void k_sub(double* r, const double* b, const double* q, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    r[i] = b[i] - q[i];
+}
+
+// float -> double copy
+__global__ void k_widen(double* dst, const float* src, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (double)src[i];
 }
 
 // bc3per: master += slaves (ascending), slaves zeroed
@@ -433,12 +450,29 @@ int launch_spsi3pre(phg_ctx* c) {
   return 0;
 }
 
-int launch_spmv(phg_ctx* c, const double* p, double* q) {
+int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact) {
   const int g = grid_for(c->nshg, 256);
-  if (c->precision == PHG_MIXED)
-    k_spmv<float><<<g, 256, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg);
+  if (c->precision == PHG_MIXED && !exact)
+    k_spmv<float, float><<<g, 256, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg);
+  else if (c->precision == PHG_MIXED)
+    k_spmv<float, double><<<g, 256, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg);
   else
-    k_spmv<double><<<g, 256, 0, c->stream>>>(c->lhs64, c->scolp, c->grp_slot, p, q, c->nshg);
+    k_spmv<double, double><<<g, 256, 0, c->stream>>>(c->lhs64, c->scolp, c->grp_slot, p, q,
+                                                     c->nshg);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n) {
+  int g = (int)std::min<long long>(grid_for(n, 256), 16384);
+  k_sub<<<g, 256, 0, c->stream>>>(r, b, q, n);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_widen(phg_ctx* c, double* dst, const float* src, long long n) {
+  int g = (int)std::min<long long>(grid_for(n, 256), 16384);
+  k_widen<<<g, 256, 0, c->stream>>>(dst, src, n);
   PHG_HIP(hipGetLastError());
   return 0;
 }
@@ -510,6 +544,13 @@ int launch_csr_to_slots(phg_ctx* c, const double* ref, double* slots) {
 int launch_slots_to_csr(phg_ctx* c, const double* slots, double* ref) {
   k_slots_to_csr<double><<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(slots, ref, c->csr2slot,
                                                                        c->nnz);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_slots_to_csr32(phg_ctx* c, const float* slots, double* ref) {
+  k_slots_to_csr<float><<<grid_for(c->nnz, 256), 256, 0, c->stream>>>(slots, ref, c->csr2slot,
+                                                                      c->nnz);
   PHG_HIP(hipGetLastError());
   return 0;
 }

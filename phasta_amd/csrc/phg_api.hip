@@ -253,8 +253,10 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->acc_res, 5 * nshg));
   PHG_TRY(dev_alloc(c, &c->acc_bd, 25 * nshg));
   PHG_TRY(dev_alloc(c, &c->nodepack, 16 * nshg));
-  PHG_TRY(dev_alloc(c, &c->lhs64, c->nslots * kGroup * 25));
-  if (mixed) PHG_TRY(dev_alloc(c, &c->lhs32, c->nslots * kGroup * 25));
+  if (mixed)
+    PHG_TRY(dev_alloc(c, &c->lhs32, c->nslots * kGroup * 25));
+  else
+    PHG_TRY(dev_alloc(c, &c->lhs64, c->nslots * kGroup * 25));
   PHG_TRY(dev_alloc(c, &c->res, 5 * nshg));
   PHG_TRY(dev_alloc(c, &c->rmes, 5 * nshg));
   PHG_TRY(dev_alloc(c, &c->bdiag, 25 * nshg));
@@ -263,7 +265,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->yold, 5 * nshg));
   PHG_TRY(dev_alloc(c, &c->acold, 5 * nshg));
   PHG_TRY(dev_alloc(c, &c->dy, 5 * nshg));
-  PHG_TRY(dev_alloc(c, &c->krylov, (size_t)(K + 1) * 5 * nshg));
+  // K+1 basis vectors; at least 3 (the refinement loop uses 1 and 2 as scratch)
+  PHG_TRY(dev_alloc(c, &c->krylov, (size_t)(std::max(K, 2) + 1) * 5 * nshg));
   c->red_blocks = 1024;
   PHG_TRY(dev_alloc(c, &c->red_part, c->red_blocks));
   PHG_TRY(dev_alloc(c, &c->red_count, 1));
@@ -290,8 +293,10 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->csr2slot, c2s.data(), c2s.size()));
   PHG_TRY(upload(c, c->slot2csr, s2c.data(), s2c.size()));
   PHG_TRY(launch_fill(c, c->red_count, sizeof(unsigned)));
-  PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
-  if (mixed) PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
+  if (mixed)
+    PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
+  else
+    PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
   PHG_TRY(launch_fill(c, c->yold, sizeof(double) * 5 * nshg));
   PHG_TRY(launch_fill(c, c->acold, sizeof(double) * 5 * nshg));
   PHG_TRY(sync(c));
@@ -327,10 +332,94 @@ int elm_core(phg_ctx* c) {
   return 0;
 }
 
+// One GMRES cycle on the scaled system (solgmr.f:522-692): kvec(0) holds the
+// normalised start vector, unorm its norm.  Adds sum_k y_k u_k to c->dy.
+struct CycleOut {
+  int iKs = 0;
+  double echeck = 0.0;
+};
+
+int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut* out) {
+  const long long V = 5 * c->nshg;
+  const int K = c->prm.Kspace;
+  double* H = c->scal + 16;
+  int iKs = 0;
+  double echeck = 0.0;
+  std::vector<double> Hh((size_t)(K + 2) * (K + 1), 0.0), e(K + 2, 0.0), Rc(K + 2, 0.0),
+      Rs(K + 2, 0.0), yB(K + 2, 0.0);
+  auto Hm = [&](int a, int b) -> double& { return Hh[(size_t)(a - 1) + (size_t)(K + 1) * (b - 1)]; };
+  e[1] = unorm;
+  for (int iK = 1; iK <= K; ++iK) {
+    iKs = iK;
+    double* w = kvec(c, iK);
+    PHG_HIP(hipEventRecord(c->kev[0], c->stream));
+    PHG_TRY(launch_spmv(c, kvec(c, iK - 1), w));
+    PHG_HIP(hipEventRecord(c->kev[1], c->stream));
+    PHG_TRY(launch_bc3per(c, w, 5));
+    PHG_HIP(hipEventRecord(c->kev[2], c->stream));
+    PHG_TRY(launch_dot(c, w, kvec(c, 0), V, H + 0));
+    for (int jK = 2; jK <= iK + 1; ++jK)
+      PHG_TRY(launch_mgs_step(c, w, kvec(c, jK - 2), jK == iK + 1 ? w : kvec(c, jK - 1), H + jK - 2,
+                              V, H + jK - 1));
+    PHG_HIP(hipEventRecord(c->kev[3], c->stream));
+    PHG_TRY(launch_scale(c, w, w, V, H + iK, H + iK + 1));
+    PHG_TRY(download(c, c->h_scal + 16, H, iK + 2));
+    PHG_TRY(sync(c));
+    {
+      float a1 = 0, a2 = 0;
+      PHG_HIP(hipEventElapsedTime(&a1, c->kev[0], c->kev[1]));
+      PHG_HIP(hipEventElapsedTime(&a2, c->kev[2], c->kev[3]));
+      st->ms_spmv += a1;
+      st->ms_mgs += a2;
+      st->n_spmv += 1;
+      st->n_mgs += iK + 1;
+    }
+    for (int jK = 1; jK <= iK; ++jK) Hm(jK, iK) = c->h_scal[16 + jK - 1];
+    Hm(iK + 1, iK) = c->h_scal[16 + iK + 1];
+    // Givens rotations on the new column (solgmr.f:645-663)
+    for (int jK = 1; jK <= iK - 1; ++jK) {
+      const double t = Rc[jK] * Hm(jK, iK) + Rs[jK] * Hm(jK + 1, iK);
+      Hm(jK + 1, iK) = -Rs[jK] * Hm(jK, iK) + Rc[jK] * Hm(jK + 1, iK);
+      Hm(jK, iK) = t;
+    }
+    const double t = std::sqrt(Hm(iK, iK) * Hm(iK, iK) + Hm(iK + 1, iK) * Hm(iK + 1, iK));
+    Rc[iK] = Hm(iK, iK) / t;
+    Rs[iK] = Hm(iK + 1, iK) / t;
+    Hm(iK, iK) = t;
+    Hm(iK + 1, iK) = 0.0;
+    const double t2 = Rc[iK] * e[iK] + Rs[iK] * e[iK + 1];
+    e[iK + 1] = -Rs[iK] * e[iK] + Rc[iK] * e[iK + 1];
+    e[iK] = t2;
+    c->ntotGM++;
+    echeck = std::fabs(e[iK + 1]);
+    if (echeck <= epsnrm && iK >= c->prm.minIters) break;
+  }
+  // back substitution (solgmr.f:681-686), Dy += y_k u_k (solgmr.f:690-692)
+  for (int jK = iKs; jK >= 1; --jK) {
+    yB[jK] = e[jK] / Hm(jK, jK);
+    for (int lK = 1; lK <= jK - 1; ++lK) e[lK] = e[lK] - yB[jK] * Hm(lK, jK);
+  }
+  PHG_TRY(launch_combine(c, c->dy, c->krylov, iKs, yB.data() + 1, V));
+  out->iKs = iKs;
+  out->echeck = std::fabs(e[iKs + 1]);
+  return 0;
+}
+
+// Largest number of GMRES cycles of the mixed-precision refinement loop.
+constexpr int kMaxRefine = 4;
+
 // SolGMRs on the resident y/ac (solgmr.f:449-744).  Leaves Dy in c->dy.
+//
+// Mixed precision: the tangent is assembled and stored in float and the
+// GMRES matvec forms its products in float; the Krylov basis, MGS, Givens
+// and norms stay in double.  After each cycle the true residual of the
+// scaled system, b - K z, is formed in double from the stored matrix and
+// its norm is tested against the same epsnrm as the FP64 path; cycles
+// restart from that residual until it passes (at most kMaxRefine cycles).
 int sol_core(phg_ctx* c, phg_stats* st) {
   const long long n = c->nshg, V = 5 * n;
   const int K = c->prm.Kspace;
+  const bool mixed = c->precision == PHG_MIXED;
   if (K < 1 || K > c->kcap || K > 60) {
     phg_set_error("Kspace %d outside 1..%d", K, std::min(c->kcap, 60));
     return PHG_EINVAL;
@@ -339,14 +428,11 @@ int sol_core(phg_ctx* c, phg_stats* st) {
     phg_set_error("nGMRES must be 1 (restart branch of solgmr.f:526 is inactive)");
     return PHG_EINVAL;
   }
-  hipEvent_t ev[3];
-  for (auto& e : ev) PHG_HIP(hipEventCreate(&e));
-  PHG_HIP(hipEventRecord(ev[0], c->stream));
+  PHG_HIP(hipEventRecord(c->ev[0], c->stream));
   PHG_TRY(elm_core(c));
-  PHG_HIP(hipEventRecord(ev[1], c->stream));
+  PHG_HIP(hipEventRecord(c->ev[1], c->stream));
   PHG_HIP(hipMemcpyAsync(c->rmes, c->res, sizeof(double) * V, hipMemcpyDeviceToDevice, c->stream));
   double* S = c->scal;  // [0] sum res^2 (after forward), [1],[2] bad counts, [3] unorm, [4] sum rmes^2
-  double* H = c->scal + 16;
   // tnanq on res and BDiag before factorisation (solgmr.f:459-460)
   PHG_TRY(launch_stats(c, S));  // S[0] provisional, overwritten below
   if (c->prm.iprec != 0) PHG_TRY(launch_i3lu(c, c->bdiag, nullptr, 0));
@@ -363,86 +449,48 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   st->nlarge = (long long)(c->h_scal[1] + c->h_scal[2]);
   if (st->nlarge > 0) {
     phg_set_error("tnanq: %lld NaN/large entries in res/BDiag", st->nlarge);
-    for (auto& e : ev) hipEventDestroy(e);
     return PHG_ENAN;
   }
   // rstatCheck.f:36-79
   const double totres_chk = std::sqrt(sumsq / (double)(float)n);
   if (c->reslast > 0.0 && totres_chk > 10000.0 * c->reslast) {
     phg_set_error("rstatCheck: residual diverging (%g > 1e4 * %g)", totres_chk, c->reslast);
-    for (auto& e : ev) hipEventDestroy(e);
     return PHG_EDIVERGE;
   }
   c->reslast = totres_chk;
-  double unorm = c->h_scal[3];
+  const double unorm = c->h_scal[3];
   st->unorm0 = unorm;
   const double epsnrm = c->prm.etol * unorm;
   st->epsnrm = epsnrm;
-  int iKs = 0;
-  double echeck = 0.0;
-  std::vector<double> Hh((size_t)(K + 2) * (K + 1), 0.0), e(K + 2, 0.0), Rc(K + 2, 0.0),
-      Rs(K + 2, 0.0), yB(K + 2, 0.0);
-  auto Hm = [&](int a, int b) -> double& { return Hh[(size_t)(a - 1) + (size_t)(K + 1) * (b - 1)]; };
-  hipEvent_t kev[4];
-  for (auto& k1 : kev) PHG_HIP(hipEventCreate(&k1));
   st->ms_spmv = st->ms_mgs = 0.0;
   st->n_spmv = st->n_mgs = 0;
+  st->refine_cycles = 0;
+  int iKs = 0;
+  double echeck = 0.0;
   if (!(unorm < 100. * c->prm.epsM * c->prm.epsM)) {
-    e[1] = unorm;
-    for (int iK = 1; iK <= K; ++iK) {
-      iKs = iK;
-      double* w = kvec(c, iK);
-      PHG_HIP(hipEventRecord(kev[0], c->stream));
-      PHG_TRY(launch_spmv(c, kvec(c, iK - 1), w));
-      PHG_HIP(hipEventRecord(kev[1], c->stream));
-      PHG_TRY(launch_bc3per(c, w, 5));
-      PHG_HIP(hipEventRecord(kev[2], c->stream));
-      PHG_TRY(launch_dot(c, w, kvec(c, 0), V, H + 0));
-      for (int jK = 2; jK <= iK + 1; ++jK)
-        PHG_TRY(launch_mgs_step(c, w, kvec(c, jK - 2), jK == iK + 1 ? w : kvec(c, jK - 1), H + jK - 2,
-                                V, H + jK - 1));
-      PHG_HIP(hipEventRecord(kev[3], c->stream));
-      PHG_TRY(launch_scale(c, w, w, V, H + iK, H + iK + 1));
-      PHG_TRY(download(c, c->h_scal + 16, H, iK + 2));
+    double rnorm = unorm;
+    for (int cyc = 0;; ++cyc) {
+      CycleOut co;
+      PHG_TRY(gmres_cycle(c, rnorm, epsnrm, st, &co));
+      iKs += co.iKs;
+      echeck = co.echeck;
+      st->refine_cycles = cyc + 1;
+      if (!mixed) break;
+      // true residual of the scaled system: r = b - K z (double products)
+      PHG_TRY(launch_spmv(c, c->dy, kvec(c, 1), true));
+      PHG_TRY(launch_bc3per(c, kvec(c, 1), 5));
+      PHG_TRY(launch_sub(c, kvec(c, 2), c->res, kvec(c, 1), V));
+      PHG_TRY(launch_dot(c, kvec(c, 2), kvec(c, 2), V, S + 5));
+      PHG_TRY(launch_scale(c, kvec(c, 0), kvec(c, 2), V, S + 5, S + 6));
+      PHG_TRY(download(c, c->h_scal + 5, S + 5, 2));
       PHG_TRY(sync(c));
-      {
-        float a1 = 0, a2 = 0;
-        hipEventElapsedTime(&a1, kev[0], kev[1]);
-        hipEventElapsedTime(&a2, kev[2], kev[3]);
-        st->ms_spmv += a1;
-        st->ms_mgs += a2;
-        st->n_spmv += 1;
-        st->n_mgs += iK + 1;
-      }
-      for (int jK = 1; jK <= iK; ++jK) Hm(jK, iK) = c->h_scal[16 + jK - 1];
-      Hm(iK + 1, iK) = c->h_scal[16 + iK + 1];
-      // Givens rotations on the new column (solgmr.f:645-663)
-      for (int jK = 1; jK <= iK - 1; ++jK) {
-        const double t = Rc[jK] * Hm(jK, iK) + Rs[jK] * Hm(jK + 1, iK);
-        Hm(jK + 1, iK) = -Rs[jK] * Hm(jK, iK) + Rc[jK] * Hm(jK + 1, iK);
-        Hm(jK, iK) = t;
-      }
-      const double t = std::sqrt(Hm(iK, iK) * Hm(iK, iK) + Hm(iK + 1, iK) * Hm(iK + 1, iK));
-      Rc[iK] = Hm(iK, iK) / t;
-      Rs[iK] = Hm(iK + 1, iK) / t;
-      Hm(iK, iK) = t;
-      Hm(iK + 1, iK) = 0.0;
-      const double t2 = Rc[iK] * e[iK] + Rs[iK] * e[iK + 1];
-      e[iK + 1] = -Rs[iK] * e[iK] + Rc[iK] * e[iK + 1];
-      e[iK] = t2;
-      c->ntotGM++;
-      echeck = std::fabs(e[iK + 1]);
-      if (echeck <= epsnrm && iK >= c->prm.minIters) break;
+      rnorm = c->h_scal[6];
+      echeck = rnorm;
+      if (rnorm <= epsnrm || cyc + 1 >= kMaxRefine) break;
     }
-    for (int jK = iKs; jK >= 1; --jK) {
-      yB[jK] = e[jK] / Hm(jK, jK);
-      for (int lK = 1; lK <= jK - 1; ++lK) e[lK] = e[lK] - yB[jK] * Hm(lK, jK);
-    }
-    PHG_TRY(launch_combine(c, c->dy, c->krylov, iKs, yB.data() + 1, V));
-    echeck = std::fabs(e[iKs + 1]);
   }
   PHG_TRY(launch_i3lu(c, c->bdiag, c->dy, 2));
-  PHG_HIP(hipEventRecord(ev[2], c->stream));
+  PHG_HIP(hipEventRecord(c->ev[2], c->stream));
   PHG_TRY(download(c, c->h_scal + 4, S + 4, 1));
   PHG_TRY(sync(c));
   // rstat.f:95-116
@@ -455,13 +503,11 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   st->converged = echeck <= epsnrm;
   st->reslast = c->reslast;
   float ms1 = 0, ms2 = 0;
-  hipEventElapsedTime(&ms1, ev[0], ev[1]);
-  hipEventElapsedTime(&ms2, ev[1], ev[2]);
+  PHG_HIP(hipEventElapsedTime(&ms1, c->ev[0], c->ev[1]));
+  PHG_HIP(hipEventElapsedTime(&ms2, c->ev[1], c->ev[2]));
   st->ms_assembly = ms1;
   st->ms_solve = ms2;
   st->ms_total = ms1 + ms2;
-  for (auto& ev1 : ev) hipEventDestroy(ev1);
-  for (auto& k1 : kev) hipEventDestroy(k1);
   return 0;
 }
 
@@ -520,7 +566,20 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
     delete c;
     return PHG_EHIP;
   }
+  for (auto& ev : c->ev) {
+    if (hipEventCreate(&ev) != hipSuccess) ev = nullptr;
+  }
+  for (auto& ev : c->kev) {
+    if (hipEventCreate(&ev) != hipSuccess) ev = nullptr;
+  }
   int rc = setup(c, mesh);
+  if (rc == 0) {
+    for (auto ev : c->ev)
+      if (!ev) rc = PHG_EHIP;
+    for (auto ev : c->kev)
+      if (!ev) rc = PHG_EHIP;
+    if (rc) phg_set_error("hipEventCreate failed");
+  }
   if (rc) {
     std::string keep = g_err;
     phg_finalize(c);
@@ -533,7 +592,7 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
 
 extern "C" int phg_finalize(phg_ctx* c) {
   if (!c) return 0;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->x,        c->ienc,     c->bmap,   c->ienbc,  c->bcm,       c->iper,
                   c->BCv,      c->fold_master, c->fold_ptr, c->fold_slave, c->grp_slot,
                   c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,
@@ -542,9 +601,13 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
                   c->nodepack};
   for (void* p : ptrs)
-    if (p) hipFree(p);
-  if (c->h_scal) hipHostFree(c->h_scal);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (p) (void)hipFree(p);
+  if (c->h_scal) (void)hipHostFree(c->h_scal);
+  for (auto ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto ev : c->kev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
@@ -578,8 +641,13 @@ extern "C" int phg_elmgmrs(phg_ctx* c, const double* y, const double* ac, double
   if (rmes) std::memset(rmes, 0, sizeof(double) * 5 * n);  // elmgmr.f:400
   if (BDiag && c->prm.iprec != 0) PHG_TRY(download(c, BDiag, c->bdiag, 25 * n));
   if (lhsK && c->prm.lhs == 1) {
-    PHG_TRY(require_fp64(c, "phg_elmgmrs"));
-    PHG_TRY(download(c, lhsK, c->lhs_aos, 25 * c->nnz));
+    if (c->precision == PHG_MIXED) {
+      PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
+      PHG_TRY(launch_widen(c, c->tmp, c->lhs_aos32, 25 * c->nnz));
+      PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
+    } else {
+      PHG_TRY(download(c, lhsK, c->lhs_aos, 25 * c->nnz));
+    }
   }
   return sync(c);
 }
@@ -598,9 +666,11 @@ extern "C" int phg_solgmrs(phg_ctx* c, const double* y, const double* ac, double
   if (BDiag) PHG_TRY(download(c, BDiag, c->bdiag, 25 * n));
   if (Dy) PHG_TRY(download(c, Dy, c->dy, 5 * n));
   if (lhsK) {
-    PHG_TRY(require_fp64(c, "phg_solgmrs"));
     PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
-    PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
+    if (c->precision == PHG_MIXED)
+      PHG_TRY(launch_slots_to_csr32(c, c->lhs32, c->tmp));
+    else
+      PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
     PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
   }
   return sync(c);
@@ -719,19 +789,16 @@ extern "C" int phg_bc3per(phg_ctx* c, double* res, int nQs) {
 
 extern "C" int phg_bench_spmv(phg_ctx* c, int reps, double* ms_per_launch, double* bytes) {
   PHG_TRY(check_ctx(c));
-  hipEvent_t a, b;
-  PHG_HIP(hipEventCreate(&a));
-  PHG_HIP(hipEventCreate(&b));
   PHG_TRY(launch_spmv(c, kvec(c, 0), kvec(c, 1)));  // warm
-  PHG_HIP(hipEventRecord(a, c->stream));
+  PHG_HIP(hipEventRecord(c->ev[0], c->stream));
   for (int i = 0; i < reps; ++i) PHG_TRY(launch_spmv(c, kvec(c, i & 1), kvec(c, (i + 1) & 1)));
-  PHG_HIP(hipEventRecord(b, c->stream));
-  PHG_HIP(hipEventSynchronize(b));
+  PHG_HIP(hipEventRecord(c->ev[1], c->stream));
+  PHG_HIP(hipEventSynchronize(c->ev[1]));
   float ms = 0;
-  hipEventElapsedTime(&ms, a, b);
-  hipEventDestroy(a);
-  hipEventDestroy(b);
+  PHG_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   if (ms_per_launch) *ms_per_launch = ms / std::max(1, reps);
+  // values (float or double) + column index per block, row pointers, and
+  // the double vectors read and written
   const double sA = c->precision == PHG_MIXED ? 4.0 : 8.0;
   if (bytes)
     *bytes = (double)c->nnz * (25.0 * sA + 4.0) + 4.0 * (double)(c->nshg + 1) +
@@ -746,18 +813,13 @@ extern "C" int phg_sync(phg_ctx* c) {
 
 extern "C" int phg_bench_assembly(phg_ctx* c, int reps, double* ms_per_launch) {
   PHG_TRY(check_ctx(c));
-  hipEvent_t a, b;
-  PHG_HIP(hipEventCreate(&a));
-  PHG_HIP(hipEventCreate(&b));
   PHG_TRY(elm_core(c));
-  PHG_HIP(hipEventRecord(a, c->stream));
+  PHG_HIP(hipEventRecord(c->ev[0], c->stream));
   for (int i = 0; i < reps; ++i) PHG_TRY(elm_core(c));
-  PHG_HIP(hipEventRecord(b, c->stream));
-  PHG_HIP(hipEventSynchronize(b));
+  PHG_HIP(hipEventRecord(c->ev[1], c->stream));
+  PHG_HIP(hipEventSynchronize(c->ev[1]));
   float ms = 0;
-  hipEventElapsedTime(&ms, a, b);
-  hipEventDestroy(a);
-  hipEventDestroy(b);
+  PHG_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   if (ms_per_launch) *ms_per_launch = ms / std::max(1, reps);
   return 0;
 }

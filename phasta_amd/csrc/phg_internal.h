@@ -117,6 +117,10 @@ struct phg_ctx {
   // scratch for host-array entry points
   double* tmp = nullptr;
   size_t tmp_bytes = 0;
+
+  // timing events on the library stream, created once
+  hipEvent_t ev[3] = {};
+  hipEvent_t kev[4] = {};
 };
 
 // ---- launchers (k_assembly.hip) ----
@@ -127,7 +131,11 @@ int launch_bc_res_bdiag(phg_ctx* c, bool prec);
 // ---- launchers (k_solver.hip) ----
 int launch_i3lu(phg_ctx* c, double* diag, double* r, int code);
 int launch_spsi3pre(phg_ctx* c);
-int launch_spmv(phg_ctx* c, const double* p, double* q);
+// exact: in the mixed-precision path, form products and row sums in double
+// (true residual of the refinement loop) instead of float
+int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact = false);
+int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n);
+int launch_widen(phg_ctx* c, double* dst, const float* src, long long n);
 int launch_bc3per(phg_ctx* c, double* v, int nq);
 int launch_dot(phg_ctx* c, const double* a, const double* b, long long n,
                double* out);
@@ -142,6 +150,7 @@ int launch_stats(phg_ctx* c, double* out);
 int launch_csr_to_slots(phg_ctx* c, const double* ref, double* slots);
 int launch_pack_nodes(phg_ctx* c);
 int launch_slots_to_csr(phg_ctx* c, const double* slots, double* ref);
+int launch_slots_to_csr32(phg_ctx* c, const float* slots, double* ref);
 int launch_predict(phg_ctx* c);
 int launch_correct(phg_ctx* c);
 int launch_update(phg_ctx* c);

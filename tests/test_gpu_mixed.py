@@ -1,0 +1,86 @@
+"""Mixed-precision path (float tangent and matvec, double residual, Krylov
+basis, MGS, Givens, norms and refinement residual) against the FP64 oracle.
+
+The residual is computed in double in both paths; the tangent, BDiag and
+lhsK carry float rounding.  The bars below are the stated errors of the
+reduced-precision results against FP64.  The update Dy is checked in the
+way the solver's tolerance defines it: on the FP64 system, the
+preconditioned residual ||L^-1 (res - K Dy)|| must meet etol * ||L^-1 res||
+(up to the float rounding of K), as the FP64 GMRES update does.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from phasta_amd import mesh, solver
+
+pytestmark = pytest.mark.gpu
+
+# stated errors of the mixed path against FP64 (normwise relative)
+TOL_RES = 1e-13      # residual: double in both paths
+TOL_TANGENT = 2e-6   # float tangent blocks and BDiag
+SLACK_SOLVE = 1e-3   # tolerance slack of the preconditioned residual
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module", params=[4, 16], ids=["box4", "box16"])
+def trio(request):
+    case = mesh.box_case(request.param, mach=0.3, kspace=10)
+    print(f"[mixed] box{request.param}: nshg={case.nshg}", flush=True)
+    orc = pyoracle.Oracle(case)
+    gpu = solver.GpuSolver(case, precision=solver.PHG_MIXED)
+    yield case, orc, gpu
+    gpu.close()
+
+
+def test_mixed_elmgmrs_error(trio):
+    case, orc, gpu = trio
+    y, ac = orc.itr_predict_bc(case.yold)
+    rng = np.random.default_rng(1)
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    r_res, _, r_bd, r_lk = orc.elmgmrs(y, ac)
+    g_res, _, g_bd, g_lk = gpu.elmgmrs(y, ac)
+    e_res, e_bd, e_lk = rel(g_res, r_res), rel(g_bd, r_bd), rel(g_lk, r_lk)
+    print(f"[mixed] err res {e_res:.2e} BDiag {e_bd:.2e} lhsK {e_lk:.2e}", flush=True)
+    assert e_res < TOL_RES
+    assert e_bd < TOL_TANGENT
+    assert e_lk < TOL_TANGENT
+    assert e_lk > 0.0  # the tangent really is float
+
+
+def test_mixed_solgmrs_meets_fp64_tolerance(trio):
+    case, orc, gpu = trio
+    y, ac = orc.itr_predict_bc(case.yold)
+    ref = orc.solgmrs(y, ac)
+    out = gpu.solgmrs(y, ac)
+    assert out["converged"]
+    assert out["refine_cycles"] >= 1
+    assert abs(out["iKs"] - ref["iKs"]) <= 1
+    # FP64 system: unscaled tangent and factored BDiag from the oracle
+    _, rmes, bd, lk = orc.elmgmrs(y, ac)
+    lu = bd.copy(order="F")
+    orc.i3lu(lu, None, "LU_Fact ")
+
+    def prec_resid(dy):
+        q = np.asfortranarray(dy.copy())
+        orc.sparse_ap(lk, q)
+        orc.bc3per(q)
+        r = np.asfortranarray(rmes - q)
+        orc.i3lu(lu, r, "forward ")
+        return np.linalg.norm(r)
+
+    b = np.asfortranarray(rmes.copy())
+    orc.i3lu(lu, b, "forward ")
+    epsnrm = case.params["etol"] * np.linalg.norm(b)
+    r64 = prec_resid(ref["Dy"])
+    r32 = prec_resid(out["Dy"])
+    e_dy = rel(out["Dy"], ref["Dy"])
+    print(f"[mixed] iKs {out['iKs']} (fp64 {ref['iKs']}), cycles {out['refine_cycles']}, "
+          f"resid/epsnrm mixed {r32 / epsnrm:.4f} fp64 {r64 / epsnrm:.4f}, "
+          f"Dy err {e_dy:.2e}", flush=True)
+    assert r64 <= epsnrm * (1 + 1e-8)
+    assert r32 <= epsnrm * (1 + SLACK_SOLVE)
+    assert rel(out["rmes"], ref["rmes"]) < TOL_RES  # raw residual, double in both
