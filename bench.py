@@ -112,6 +112,11 @@ def main():
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                 "bytes_per_launch": by, "ms_per_launch": round(spmv_avg, 5)}
 
+    # a plain copy of known size, after the timed region: the achievable
+    # stream bandwidth on this box and the calibration kernel of the
+    # profiler's memory counters
+    ms_copy, by_copy = gpu.bench_copy(5)
+
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "elements/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -133,7 +138,9 @@ def main():
                       "ms_spmv_per_step": round(ms_spmv / args.steps, 3),
                       "ms_mgs_per_step": round(ms_mgs / args.steps, 3),
                       "krylov_iters": [s["iKs"] for s in stats],
-                      "assembly_elements_per_s": round(nel / (ms_asm / args.steps / 1e3), 1)},
+                      "assembly_elements_per_s": round(nel / (ms_asm / args.steps / 1e3), 1),
+                      "copy_GBs": round(by_copy / (ms_copy * 1e-3) / 1e9, 1),
+                      "copy_bytes": by_copy},
     }
     if not args.no_cpu:
         dt, ref, _ = cpu_baseline(case)

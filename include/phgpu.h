@@ -74,8 +74,10 @@ typedef struct phg_stats {
   double ms_assembly, ms_solve, ms_total;
   double ms_spmv, ms_mgs; /* summed HIP-event times of the matvec and MGS  */
   int n_spmv, n_mgs;      /* launches behind ms_spmv / ms_mgs              */
-  int refine_cycles;      /* mixed precision: outer refinement cycles      */
-  int reserved;
+  int refine_cycles;      /* GMRES cycles run (1 on the plain FP64 path)   */
+  int pc_fallback;        /* learned PC: 0 its cycle converged (or unused),
+                             1 discarded for block-diagonal GMRES, 2 finished
+                             by block-diagonal cycles                      */
 } phg_stats;
 
 typedef struct phg_ctx phg_ctx;
@@ -137,6 +139,36 @@ int phg_bc3per(phg_ctx* ctx, double* res, int nQs);
 int phg_bench_spmv(phg_ctx* ctx, int reps, double* ms_per_launch,
                    double* bytes_per_launch);
 int phg_bench_assembly(phg_ctx* ctx, int reps, double* ms_per_launch);
+/* Plain device copy inside the Krylov space (no counterpart in the
+ * reference): a kernel of known traffic that calibrates the memory
+ * counters of a profiler run; bytes = read + written bytes per launch. */
+int phg_bench_copy(phg_ctx* ctx, int reps, double* ms_per_launch, double* bytes);
+/* ---- Learned preconditioner (no counterpart in the reference) ----
+ * A 3-layer message-passing network over the periodic node graph (hidden
+ * width 64, bf16 MFMA with float accumulation) that preconditions flexible
+ * GMRES on the block-diagonally scaled system, with block-diagonal GMRES as
+ * the fallback whenever its cycle does not reduce the true residual.
+ * Parameters: phg_mpnn_param_count() floats, W0 (32x64), b0, W1 (320x64),
+ * b1, W2, b2, W3, b3, Wout (64x5), bout, matrices row-major [in][out].    */
+int phg_mpnn_param_count(void);
+/* Builds the node graph from the same mesh arrays as phg_init; dscale: the
+ * 25 divisors of the BDiag inputs. */
+int phg_mpnn_init(phg_ctx* ctx, const phg_mesh* mesh, const float* dscale);
+int phg_mpnn_set_weights(phg_ctx* ctx, const float* w, int count);
+int phg_mpnn_get_weights(phg_ctx* ctx, float* w, int count);
+/* use_pc: FGMRES with the network inside phg_solgmrs/phg_newton_iteration;
+ * keep_pair: keep (BDiag before factorisation, b, z) of each solve. */
+int phg_mpnn_enable(phg_ctx* ctx, int use_pc, int keep_pair);
+/* z = network(v) for host arrays (bdiag_raw (nshg,5,5), v, z (nshg,5)). */
+int phg_mpnn_apply(phg_ctx* ctx, const double* bdiag_raw, const double* v, double* z);
+/* One training step on a pair (v, t): forward, loss, backward; an Adam
+ * step when lr > 0.  grad (nullable) receives the gradient. */
+int phg_mpnn_train_step(phg_ctx* ctx, const double* bdiag_raw, const double* v,
+                        const double* t, float lr, double* loss, float* grad);
+/* Pair of the last solve: BDiag before factorisation, the scaled right-hand
+ * side b = L^-1 res and the scaled solution z (Dy = U^-1 z). */
+int phg_mpnn_last_pair(phg_ctx* ctx, double* bdiag_raw, double* b, double* zhat);
+
 /* Wait for all work queued on the library stream. */
 int phg_sync(phg_ctx* ctx);
 

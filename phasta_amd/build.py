@@ -16,7 +16,7 @@ PKG = os.path.join(ROOT, "phasta_amd")
 CSRC = os.path.join(PKG, "csrc")
 INC = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["phg_api.hip", "k_assembly.hip", "k_solver.hip"]
+HIP_SOURCES = ["phg_api.hip", "k_assembly.hip", "k_solver.hip", "k_mpnn.hip"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                "-fopenmp", f"-I{INC}"]

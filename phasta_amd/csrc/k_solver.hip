@@ -196,6 +196,13 @@ __global__ void k_sub(double* r, const double* b, const double* q, long long n) 
     r[i] = b[i] - q[i];
 }
 
+// plain copy, 16 B per lane (calibration of the memory counters)
+__global__ void k_copy(double2* __restrict__ dst, const double2* __restrict__ src, long long n2) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // float -> double copy
 __global__ void k_widen(double* dst, const float* src, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -466,6 +473,15 @@ int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact) {
 int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n) {
   int g = (int)std::min<long long>(grid_for(n, 256), 16384);
   k_sub<<<g, 256, 0, c->stream>>>(r, b, q, n);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_copy(phg_ctx* c, double* dst, const double* src, long long n) {
+  const long long n2 = n / 2;
+  int g = (int)std::min<long long>(grid_for(n2, 256), 8192);
+  k_copy<<<g, 256, 0, c->stream>>>(reinterpret_cast<double2*>(dst),
+                                   reinterpret_cast<const double2*>(src), n2);
   PHG_HIP(hipGetLastError());
   return 0;
 }

@@ -339,8 +339,12 @@ struct CycleOut {
   double echeck = 0.0;
 };
 
-int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut* out) {
+// flexible: precondition each direction with the network (FGMRES, right
+// preconditioning) and combine the preconditioned directions.
+int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut* out,
+                bool flexible = false) {
   const long long V = 5 * c->nshg;
+  auto zvec = [&](int k) { return c->mp.zbasis + (size_t)k * V; };
   const int K = c->prm.Kspace;
   double* H = c->scal + 16;
   int iKs = 0;
@@ -352,8 +356,13 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
   for (int iK = 1; iK <= K; ++iK) {
     iKs = iK;
     double* w = kvec(c, iK);
+    const double* dir = kvec(c, iK - 1);
+    if (flexible) {
+      PHG_TRY(mp_forward(c, kvec(c, iK - 1), zvec(iK - 1), false));
+      dir = zvec(iK - 1);
+    }
     PHG_HIP(hipEventRecord(c->kev[0], c->stream));
-    PHG_TRY(launch_spmv(c, kvec(c, iK - 1), w));
+    PHG_TRY(launch_spmv(c, dir, w));
     PHG_HIP(hipEventRecord(c->kev[1], c->stream));
     PHG_TRY(launch_bc3per(c, w, 5));
     PHG_HIP(hipEventRecord(c->kev[2], c->stream));
@@ -399,7 +408,7 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
     yB[jK] = e[jK] / Hm(jK, jK);
     for (int lK = 1; lK <= jK - 1; ++lK) e[lK] = e[lK] - yB[jK] * Hm(lK, jK);
   }
-  PHG_TRY(launch_combine(c, c->dy, c->krylov, iKs, yB.data() + 1, V));
+  PHG_TRY(launch_combine(c, c->dy, flexible ? c->mp.zbasis : c->krylov, iKs, yB.data() + 1, V));
   out->iKs = iKs;
   out->echeck = std::fabs(e[iKs + 1]);
   return 0;
@@ -435,6 +444,10 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   double* S = c->scal;  // [0] sum res^2 (after forward), [1],[2] bad counts, [3] unorm, [4] sum rmes^2
   // tnanq on res and BDiag before factorisation (solgmr.f:459-460)
   PHG_TRY(launch_stats(c, S));  // S[0] provisional, overwritten below
+  const bool learned = c->mp.ready && c->mp.use_pc;
+  if ((learned || c->mp.keep_pair) && c->mp.ready)
+    PHG_HIP(hipMemcpyAsync(c->mp.bd_raw, c->bdiag, sizeof(double) * 25 * n,
+                           hipMemcpyDeviceToDevice, c->stream));
   if (c->prm.iprec != 0) PHG_TRY(launch_i3lu(c, c->bdiag, nullptr, 0));
   PHG_TRY(launch_i3lu(c, c->bdiag, c->res, 1));
   PHG_TRY(launch_dot(c, c->res, c->res, V, S));
@@ -467,16 +480,12 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   st->refine_cycles = 0;
   int iKs = 0;
   double echeck = 0.0;
+  st->pc_fallback = 0;
   if (!(unorm < 100. * c->prm.epsM * c->prm.epsM)) {
     double rnorm = unorm;
-    for (int cyc = 0;; ++cyc) {
-      CycleOut co;
-      PHG_TRY(gmres_cycle(c, rnorm, epsnrm, st, &co));
-      iKs += co.iKs;
-      echeck = co.echeck;
-      st->refine_cycles = cyc + 1;
-      if (!mixed) break;
-      // true residual of the scaled system: r = b - K z (double products)
+    // true residual of the scaled system, r = b - K dy (double products),
+    // normalised into kvec(0); returns its norm
+    auto true_residual = [&](double* norm) -> int {
       PHG_TRY(launch_spmv(c, c->dy, kvec(c, 1), true));
       PHG_TRY(launch_bc3per(c, kvec(c, 1), 5));
       PHG_TRY(launch_sub(c, kvec(c, 2), c->res, kvec(c, 1), V));
@@ -484,11 +493,45 @@ int sol_core(phg_ctx* c, phg_stats* st) {
       PHG_TRY(launch_scale(c, kvec(c, 0), kvec(c, 2), V, S + 5, S + 6));
       PHG_TRY(download(c, c->h_scal + 5, S + 5, 2));
       PHG_TRY(sync(c));
-      rnorm = c->h_scal[6];
+      *norm = c->h_scal[6];
+      return 0;
+    };
+    bool plain_from_start = !learned;
+    if (learned) {
+      // FGMRES cycle with the network; block-diagonal GMRES judges it
+      CycleOut co;
+      PHG_TRY(gmres_cycle(c, rnorm, epsnrm, st, &co, true));
+      iKs += co.iKs;
+      st->refine_cycles = 1;
+      PHG_TRY(true_residual(&rnorm));
+      echeck = rnorm;
+      if (!(rnorm < unorm)) {
+        // no reduction: discard and solve with block-diagonal GMRES
+        st->pc_fallback = 1;
+        PHG_TRY(launch_fill(c, c->dy, sizeof(double) * V));
+        PHG_TRY(launch_scale(c, kvec(c, 0), c->res, V, S, S + 3));
+        rnorm = unorm;
+        plain_from_start = true;
+      } else if (rnorm > epsnrm) {
+        st->pc_fallback = 2;  // reduced, finished by block-diagonal cycles
+      }
+    }
+    const bool need_plain = plain_from_start || rnorm > epsnrm;
+    for (int cyc = 0; need_plain; ++cyc) {
+      CycleOut co;
+      PHG_TRY(gmres_cycle(c, rnorm, epsnrm, st, &co));
+      iKs += co.iKs;
+      echeck = co.echeck;
+      st->refine_cycles += 1;
+      if (!mixed && !learned) break;  // the reference's single cycle
+      PHG_TRY(true_residual(&rnorm));
       echeck = rnorm;
       if (rnorm <= epsnrm || cyc + 1 >= kMaxRefine) break;
     }
   }
+  if (c->mp.ready && c->mp.keep_pair)
+    PHG_HIP(hipMemcpyAsync(c->mp.zhat, c->dy, sizeof(double) * V, hipMemcpyDeviceToDevice,
+                           c->stream));
   PHG_TRY(launch_i3lu(c, c->bdiag, c->dy, 2));
   PHG_HIP(hipEventRecord(c->ev[2], c->stream));
   PHG_TRY(download(c, c->h_scal + 4, S + 4, 1));
@@ -593,6 +636,7 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
 extern "C" int phg_finalize(phg_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  mp_free(c);
   void* ptrs[] = {c->x,        c->ienc,     c->bmap,   c->ienbc,  c->bcm,       c->iper,
                   c->BCv,      c->fold_master, c->fold_ptr, c->fold_slave, c->grp_slot,
                   c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,
@@ -804,6 +848,120 @@ extern "C" int phg_bench_spmv(phg_ctx* c, int reps, double* ms_per_launch, doubl
     *bytes = (double)c->nnz * (25.0 * sA + 4.0) + 4.0 * (double)(c->nshg + 1) +
              2.0 * (double)c->nshg * 5.0 * 8.0;
   return 0;
+}
+
+extern "C" int phg_bench_copy(phg_ctx* c, int reps, double* ms_per_launch, double* bytes) {
+  PHG_TRY(check_ctx(c));
+  // first half of the Krylov space -> second half
+  const long long nvec = std::max(c->kcap, 2) + 1;  // vectors allocated by setup
+  const long long half = ((nvec / 2) * 5 * c->nshg) & ~1LL;
+  double* src = c->krylov;
+  double* dst = c->krylov + half;
+  PHG_TRY(launch_copy(c, dst, src, half));  // warm
+  PHG_HIP(hipEventRecord(c->ev[0], c->stream));
+  for (int i = 0; i < reps; ++i) PHG_TRY(launch_copy(c, dst, src, half));
+  PHG_HIP(hipEventRecord(c->ev[1], c->stream));
+  PHG_HIP(hipEventSynchronize(c->ev[1]));
+  float ms = 0;
+  PHG_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  if (ms_per_launch) *ms_per_launch = ms / std::max(1, reps);
+  if (bytes) *bytes = 2.0 * 8.0 * (double)half;
+  return 0;
+}
+
+// ---------------- learned preconditioner ----------------
+
+extern "C" int phg_mpnn_param_count(void) { return mp_param_count(); }
+
+extern "C" int phg_mpnn_init(phg_ctx* c, const phg_mesh* mesh, const float* dscale) {
+  PHG_TRY(check_ctx(c));
+  if (!mesh || !dscale || mesh->nshg != c->nshg || mesh->nnz_tot != c->nnz) {
+    phg_set_error("phg_mpnn_init: mesh does not match the context");
+    return PHG_EINVAL;
+  }
+  return mp_init(c, mesh, dscale);
+}
+
+static int mp_check(phg_ctx* c) {
+  PHG_TRY(check_ctx(c));
+  if (!c->mp.ready) {
+    phg_set_error("learned preconditioner: call phg_mpnn_init first");
+    return PHG_EINVAL;
+  }
+  return 0;
+}
+
+extern "C" int phg_mpnn_set_weights(phg_ctx* c, const float* w, int count) {
+  PHG_TRY(mp_check(c));
+  if (!w || count != mp_param_count()) {
+    phg_set_error("phg_mpnn_set_weights: expected %d floats", mp_param_count());
+    return PHG_EINVAL;
+  }
+  PHG_HIP(hipMemcpyAsync(c->mp.w, w, sizeof(float) * count, hipMemcpyHostToDevice, c->stream));
+  PHG_TRY(mp_pack(c));
+  c->mp.adam_t = 0;
+  if (c->mp.train_ready) {
+    PHG_TRY(launch_fill(c, c->mp.m1, sizeof(float) * count));
+    PHG_TRY(launch_fill(c, c->mp.m2, sizeof(float) * count));
+  }
+  return sync(c);
+}
+
+extern "C" int phg_mpnn_get_weights(phg_ctx* c, float* w, int count) {
+  PHG_TRY(mp_check(c));
+  if (!w || count != mp_param_count()) {
+    phg_set_error("phg_mpnn_get_weights: expected %d floats", mp_param_count());
+    return PHG_EINVAL;
+  }
+  PHG_HIP(hipMemcpyAsync(w, c->mp.w, sizeof(float) * count, hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
+}
+
+extern "C" int phg_mpnn_enable(phg_ctx* c, int use_pc, int keep_pair) {
+  PHG_TRY(mp_check(c));
+  c->mp.use_pc = use_pc != 0;
+  c->mp.keep_pair = keep_pair != 0;
+  return 0;
+}
+
+extern "C" int phg_mpnn_apply(phg_ctx* c, const double* bdiag_raw, const double* v, double* z) {
+  PHG_TRY(mp_check(c));
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->mp.bd_raw, bdiag_raw, 25 * n));
+  PHG_TRY(upload(c, c->mp.vin, v, 5 * n));
+  PHG_TRY(mp_forward(c, c->mp.vin, c->mp.zout, false));
+  PHG_TRY(download(c, z, c->mp.zout, 5 * n));
+  return sync(c);
+}
+
+extern "C" int phg_mpnn_train_step(phg_ctx* c, const double* bdiag_raw, const double* v,
+                                   const double* t, float lr, double* loss, float* grad) {
+  PHG_TRY(mp_check(c));
+  PHG_TRY(mp_alloc_train(c));
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->mp.bd_raw, bdiag_raw, 25 * n));
+  PHG_TRY(upload(c, c->mp.vin, v, 5 * n));
+  PHG_TRY(upload(c, c->mp.zout, t, 5 * n));
+  PHG_TRY(mp_forward(c, c->mp.vin, nullptr, true));
+  PHG_TRY(mp_targets(c, c->mp.zout, c->mp.vin));
+  PHG_TRY(mp_backward(c, lr, loss));
+  if (grad)
+    PHG_HIP(hipMemcpyAsync(grad, c->mp.g, sizeof(float) * mp_param_count(), hipMemcpyDeviceToHost,
+                           c->stream));
+  return sync(c);
+}
+
+extern "C" int phg_mpnn_last_pair(phg_ctx* c, double* bdiag_raw, double* b, double* zhat) {
+  PHG_TRY(mp_check(c));
+  const long long n = c->nshg;
+  if (!c->mp.keep_pair) {
+    phg_set_error("phg_mpnn_last_pair: enable keep_pair before the solve");
+    return PHG_EINVAL;
+  }
+  if (bdiag_raw) PHG_TRY(download(c, bdiag_raw, c->mp.bd_raw, 25 * n));
+  if (b) PHG_TRY(download(c, b, c->res, 5 * n));
+  if (zhat) PHG_TRY(download(c, zhat, c->mp.zhat, 5 * n));
+  return sync(c);
 }
 
 extern "C" int phg_sync(phg_ctx* c) {

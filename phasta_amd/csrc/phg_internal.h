@@ -49,6 +49,49 @@ __host__ __device__ inline uint8_t constraint_mask(int ibc) {
   return m;
 }
 
+// Learned preconditioner state (k_mpnn.hip); allocated by phg_mpnn_init.
+struct MpNet {
+  bool ready = false;
+  bool use_pc = false;       // FGMRES with the network inside SolGMRs
+  bool keep_pair = false;    // keep (raw BDiag, b, z) of the last solve
+  long long n_act = 0, nedge = 0;
+  int* ptr = nullptr;        // periodic node graph, CSR over all nodes
+  int* col = nullptr;
+  int* rev = nullptr;        // index of the reverse edge
+  float4* ev = nullptr;      // e_ij (x, y, z) per edge
+  float4* esum = nullptr;    // sum_j e_ij per node
+  float* inv = nullptr;      // 1/deg (0 for slaves)
+  float* dscale = nullptr;   // 25 input scales of BDiag
+  double* bd_raw = nullptr;  // BDiag before factorisation (25 planes)
+  const double* bd_src = nullptr;
+  double* zhat = nullptr;    // scaled-system solution of the last solve
+  uint16_t* x = nullptr;     // n x 32 inputs (bf16)
+  uint16_t* h[4] = {};       // n x 64 hidden states (bf16)
+  uint16_t* A[3] = {};       // n x 320 aggregated inputs (bf16)
+  float* U[4] = {};          // n x 64 pre-activations (training)
+  float* Y = nullptr;        // n x 5 outputs (training)
+  float* Ystar = nullptr;    // n x 5 targets (training)
+  float* dh = nullptr;       // n x 64
+  float* dU = nullptr;       // n x 64
+  float* dA = nullptr;       // n x 320
+  float* dY = nullptr;       // n x 5
+  float* sq = nullptr;       // n squared errors
+  float* part = nullptr;     // weight-gradient partials
+  int nparts = 0;
+  float* w = nullptr;        // master weights (float)
+  float* g = nullptr;        // gradient
+  float* m1 = nullptr;       // Adam moments
+  float* m2 = nullptr;
+  uint16_t* wt[4] = {};      // bf16 weights, Wt[c][k]
+  uint16_t* wr[4] = {};      // bf16 weights, W[k][c]
+  double* sbuf = nullptr;    // [0] ||v||^2, [1] s, [2] loss
+  double* vin = nullptr;     // 5 planes, host-array entry points
+  double* zout = nullptr;    // 5 planes
+  double* zbasis = nullptr;  // FGMRES preconditioned directions (K vectors)
+  int adam_t = 0;
+  bool train_ready = false;
+};
+
 struct phg_ctx {
   long long nshg = 0, nel = 0, nelb = 0, nnz = 0;
   int ndofBC = 0;
@@ -118,6 +161,8 @@ struct phg_ctx {
   double* tmp = nullptr;
   size_t tmp_bytes = 0;
 
+  MpNet mp;
+
   // timing events on the library stream, created once
   hipEvent_t ev[3] = {};
   hipEvent_t kev[4] = {};
@@ -128,6 +173,20 @@ int launch_fill(phg_ctx* c, void* p, size_t bytes);
 int launch_assembly(phg_ctx* c, bool lhs, bool prec);
 int launch_bc_res_bdiag(phg_ctx* c, bool prec);
 
+// ---- learned preconditioner (k_mpnn.hip) ----
+int mp_param_count();
+int mp_init(phg_ctx* c, const phg_mesh* m, const float* dscale);
+int mp_free(phg_ctx* c);
+int mp_alloc_train(phg_ctx* c);
+int mp_pack(phg_ctx* c);
+// z = v + y(v)/s on 5-plane vectors; train keeps the activations
+int mp_forward(phg_ctx* c, const double* v, double* z, bool train);
+// loss and gradient for targets ystar (n x 5, float, device) of the last
+// train forward; lr > 0 applies an Adam step
+int mp_backward(phg_ctx* c, float lr, double* loss_out);
+// targets (t - v) s of the last forward's v (t, v: 5 planes, device)
+int mp_targets(phg_ctx* c, const double* t, const double* v);
+
 // ---- launchers (k_solver.hip) ----
 int launch_i3lu(phg_ctx* c, double* diag, double* r, int code);
 int launch_spsi3pre(phg_ctx* c);
@@ -136,6 +195,8 @@ int launch_spsi3pre(phg_ctx* c);
 int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact = false);
 int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n);
 int launch_widen(phg_ctx* c, double* dst, const float* src, long long n);
+// n doubles, n even, 16-byte aligned
+int launch_copy(phg_ctx* c, double* dst, const double* src, long long n);
 int launch_bc3per(phg_ctx* c, double* v, int nq);
 int launch_dot(phg_ctx* c, const double* a, const double* b, long long n,
                double* out);

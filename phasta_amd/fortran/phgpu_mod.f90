@@ -35,7 +35,7 @@ module phgpu
     real(c_double) :: ms_assembly, ms_solve, ms_total
     real(c_double) :: ms_spmv, ms_mgs
     integer(c_int) :: n_spmv, n_mgs
-    integer(c_int) :: refine_cycles, reserved
+    integer(c_int) :: refine_cycles, pc_fallback
   end type
 
   interface

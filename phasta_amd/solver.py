@@ -57,7 +57,7 @@ class PhgStats(ctypes.Structure):
                 ("ms_solve", ctypes.c_double), ("ms_total", ctypes.c_double),
                 ("ms_spmv", ctypes.c_double), ("ms_mgs", ctypes.c_double),
                 ("n_spmv", ctypes.c_int), ("n_mgs", ctypes.c_int),
-                ("refine_cycles", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("refine_cycles", ctypes.c_int), ("pc_fallback", ctypes.c_int)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -71,7 +71,10 @@ EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_
            "phg_elmgmrs", "phg_solgmrs", "phg_state_upload", "phg_state_download",
            "phg_state_download_old", "phg_state_upload_cur", "phg_predict",
            "phg_newton_iteration", "phg_update", "phg_sparse_ap", "phg_i3lu", "phg_spsi3pre", "phg_bc3per",
-           "phg_bench_spmv", "phg_bench_assembly", "phg_sync")
+           "phg_bench_spmv", "phg_bench_assembly", "phg_bench_copy", "phg_sync",
+           "phg_mpnn_param_count", "phg_mpnn_init", "phg_mpnn_set_weights",
+           "phg_mpnn_get_weights", "phg_mpnn_enable", "phg_mpnn_apply",
+           "phg_mpnn_train_step", "phg_mpnn_last_pair")
 
 
 def lib():
@@ -131,6 +134,7 @@ class GpuSolver:
         _check(lib().phg_init(ctypes.byref(m), ctypes.byref(self._prm), precision,
                               ctypes.byref(h)))
         self.h = h
+        self._mesh = m
         self.precision = precision
 
     def close(self):
@@ -250,7 +254,79 @@ class GpuSolver:
         _check(lib().phg_bench_spmv(self.h, reps, ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
 
+    # ---- learned preconditioner (k_mpnn.hip) ----
+    def mpnn_init(self, dscale):
+        m = self._mesh
+        ds = np.ascontiguousarray(dscale, dtype=np.float32)
+        if ds.shape != (25,):
+            raise ValueError("dscale must hold 25 floats")
+        _check(lib().phg_mpnn_init(self.h, ctypes.byref(m), ds.ctypes.data_as(ctypes.c_void_p)))
+
+    def mpnn_set_weights(self, flat):
+        w = np.ascontiguousarray(flat, dtype=np.float32)
+        _check(lib().phg_mpnn_set_weights(self.h, w.ctypes.data_as(ctypes.c_void_p), len(w)))
+
+    def mpnn_get_weights(self):
+        w = np.zeros(lib().phg_mpnn_param_count(), dtype=np.float32)
+        _check(lib().phg_mpnn_get_weights(self.h, w.ctypes.data_as(ctypes.c_void_p), len(w)))
+        return w
+
+    def mpnn_enable(self, use_pc=True, keep_pair=False):
+        _check(lib().phg_mpnn_enable(self.h, int(use_pc), int(keep_pair)))
+
+    def mpnn_apply(self, bdiag_raw, v):
+        n = self.case.nshg
+        bd, v = _f64(bdiag_raw, (n, 5, 5)), _f64(v, (n, 5))
+        z = np.zeros((n, 5), order="F")
+        _check(lib().phg_mpnn_apply(self.h, _p(bd), _p(v), _p(z)))
+        return z
+
+    def mpnn_train_step(self, bdiag_raw, v, t, lr=0.0, want_grad=False):
+        n = self.case.nshg
+        bd, v, t = _f64(bdiag_raw, (n, 5, 5)), _f64(v, (n, 5)), _f64(t, (n, 5))
+        loss = ctypes.c_double()
+        g = np.zeros(lib().phg_mpnn_param_count(), dtype=np.float32) if want_grad else None
+        gp = g.ctypes.data_as(ctypes.c_void_p) if want_grad else None
+        _check(lib().phg_mpnn_train_step(self.h, _p(bd), _p(v), _p(t), ctypes.c_float(lr),
+                                         ctypes.byref(loss), gp))
+        return (loss.value, g) if want_grad else loss.value
+
+    def mpnn_last_pair(self):
+        n = self.case.nshg
+        bd = np.zeros((n, 5, 5), order="F")
+        b = np.zeros((n, 5), order="F")
+        z = np.zeros((n, 5), order="F")
+        _check(lib().phg_mpnn_last_pair(self.h, _p(bd), _p(b), _p(z)))
+        return bd, b, z
+
+    def bench_copy(self, reps=10):
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        _check(lib().phg_bench_copy(self.h, reps, ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
     def bench_assembly(self, reps=3):
         ms = ctypes.c_double()
         _check(lib().phg_bench_assembly(self.h, reps, ctypes.byref(ms)))
         return ms.value
+
+
+# Parameter block of the learned preconditioner, in the library's order:
+# W0 (32 x 64), b0, W1 (320 x 64), b1, W2, b2, W3, b3, Wout (64 x 5), bout,
+# every matrix row-major W[k][c] (input k, output c).
+MPNN_SHAPES = (("W0", (32, 64)), ("b0", (64,)), ("W1", (320, 64)), ("b1", (64,)),
+               ("W2", (320, 64)), ("b2", (64,)), ("W3", (320, 64)), ("b3", (64,)),
+               ("Wout", (64, 5)), ("bout", (5,)))
+
+
+def mpnn_flatten(params):
+    return np.concatenate([np.asarray(params[k], dtype=np.float32).ravel()
+                           for k, _ in MPNN_SHAPES])
+
+
+def mpnn_unflatten(flat):
+    out, o = {}, 0
+    for k, shp in MPNN_SHAPES:
+        m = int(np.prod(shp))
+        out[k] = np.asarray(flat[o:o + m], dtype=np.float32).reshape(shp)
+        o += m
+    return out

@@ -60,7 +60,7 @@ def test_mixed_solgmrs_meets_fp64_tolerance(trio):
     assert out["refine_cycles"] >= 1
     assert abs(out["iKs"] - ref["iKs"]) <= 1
     # FP64 system: unscaled tangent and factored BDiag from the oracle
-    _, rmes, bd, lk = orc.elmgmrs(y, ac)
+    rraw, _, bd, lk = orc.elmgmrs(y, ac)  # raw residual (after bc3Res)
     lu = bd.copy(order="F")
     orc.i3lu(lu, None, "LU_Fact ")
 
@@ -68,11 +68,11 @@ def test_mixed_solgmrs_meets_fp64_tolerance(trio):
         q = np.asfortranarray(dy.copy())
         orc.sparse_ap(lk, q)
         orc.bc3per(q)
-        r = np.asfortranarray(rmes - q)
+        r = np.asfortranarray(rraw - q)
         orc.i3lu(lu, r, "forward ")
         return np.linalg.norm(r)
 
-    b = np.asfortranarray(rmes.copy())
+    b = np.asfortranarray(rraw.copy())
     orc.i3lu(lu, b, "forward ")
     epsnrm = case.params["etol"] * np.linalg.norm(b)
     r64 = prec_resid(ref["Dy"])

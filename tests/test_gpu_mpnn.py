@@ -1,0 +1,102 @@
+"""Learned preconditioner kernels (k_mpnn.hip) against the CPU oracle of the
+network (oracle/pympnn.py), and the flexible-GMRES plumbing.
+
+Bars: the forward correction z - v and the gradients agree with the oracle's
+bf16 model to 3e-2 normwise (bf16 roundings of intermediate values can land
+on different sides of a tie when float sums are taken in another order; the
+oracle's own gradients are pinned by finite differences in float64).  A
+zero network makes FGMRES identical to block-diagonal GMRES.
+"""
+import numpy as np
+import pytest
+
+import pympnn as M
+import pyoracle
+from phasta_amd import mesh, solver
+
+pytestmark = pytest.mark.gpu
+TOL_NET = 3e-2
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def net_case():
+    case = mesh.box_case(4, mach=0.3, kspace=10)
+    print(f"[mpnn] box4: nshg={case.nshg}", flush=True)
+    g = M.periodic_graph(case)
+    rng = np.random.default_rng(11)
+    p = M.init_params(5)
+    for k in p:
+        if k.startswith("b"):
+            p[k] = (0.1 * rng.standard_normal(p[k].shape)).astype(np.float32)
+    n = case.nshg
+    bd = np.asfortranarray(rng.standard_normal((n, 5, 5)))
+    v = np.asfortranarray(rng.standard_normal((n, 5)))
+    t = np.asfortranarray(v + 0.3 * rng.standard_normal((n, 5)))
+    dscale = np.full(25, 2.0, dtype=np.float32)
+    gpu = solver.GpuSolver(case)
+    gpu.mpnn_init(dscale)
+    gpu.mpnn_set_weights(solver.mpnn_flatten(p))
+    yield case, g, p, bd, v, t, dscale, gpu
+    gpu.close()
+
+
+def test_graph_and_forward_match_oracle(net_case):
+    case, g, p, bd, v, t, dscale, gpu = net_case
+    z = gpu.mpnn_apply(bd, v)
+    x, s = M.features(bd, v, dscale, g.active)
+    y, _ = M.forward(p, g, x)
+    z_ref = v + y / s
+    z_ref[~g.active] = v[~g.active]
+    e = rel(z - v, z_ref - v)
+    print(f"[mpnn] forward rel err {e:.2e}", flush=True)
+    assert e < TOL_NET
+    assert np.array_equal(z[~g.active], v[~g.active])
+
+
+def test_gradients_match_oracle(net_case):
+    case, g, p, bd, v, t, dscale, gpu = net_case
+    loss, grad = gpu.mpnn_train_step(bd, v, t, lr=0.0, want_grad=True)
+    x, s = M.features(bd, v, dscale, g.active)
+    ystar = ((t - v) * s).astype(np.float32)
+    loss_ref, gr, _ = M.loss_and_grad(p, g, x, ystar)
+    gg = solver.mpnn_unflatten(grad)
+    print(f"[mpnn] loss gpu {loss:.6e} oracle {loss_ref:.6e}", flush=True)
+    assert abs(loss - loss_ref) <= TOL_NET * loss_ref
+    for k, _ in solver.MPNN_SHAPES:
+        e = rel(gg[k], gr[k])
+        print(f"[mpnn] grad {k} rel err {e:.2e}", flush=True)
+        assert e < TOL_NET, k
+
+
+def test_adam_steps_reduce_the_loss(net_case):
+    case, g, p, bd, v, t, dscale, gpu = net_case
+    losses = [gpu.mpnn_train_step(bd, v, t, lr=2e-3) for _ in range(30)]
+    print(f"[mpnn] loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+    assert losses[-1] < 0.7 * losses[0]
+    gpu.mpnn_set_weights(solver.mpnn_flatten(p))  # leave the fixture as it was
+
+
+def test_zero_network_fgmres_equals_block_diagonal_gmres():
+    case = mesh.box_case(4, mach=0.3, kspace=10)
+    orc = pyoracle.Oracle(case)
+    y, ac = orc.itr_predict_bc(case.yold)
+    ref = orc.solgmrs(y, ac)
+    gpu = solver.GpuSolver(case)
+    try:
+        gpu.mpnn_init(np.ones(25, dtype=np.float32))
+        gpu.mpnn_set_weights(np.zeros(solver.lib().phg_mpnn_param_count(), np.float32))
+        gpu.mpnn_enable(use_pc=True, keep_pair=True)
+        out = gpu.solgmrs(y, ac)
+        bd, b, z = gpu.mpnn_last_pair()
+    finally:
+        gpu.close()
+    assert out["iKs"] == ref["iKs"]
+    assert out["pc_fallback"] == 0
+    assert rel(out["Dy"], ref["Dy"]) < 1e-12
+    # the kept pair: b is the preconditioned residual, z solves the scaled system
+    assert rel(b, ref["res"]) < 1e-12
+    assert np.linalg.norm(z) > 0

@@ -1,0 +1,84 @@
+"""Pins of the learned-preconditioner network's CPU oracle (oracle/pympnn.py):
+finite-difference gradients in float64, the adjoint identity of the
+aggregation, the periodic graph's symmetry, and the bfloat16 rounding."""
+import numpy as np
+import pytest
+
+import pympnn as M
+from phasta_amd import mesh
+
+
+@pytest.fixture(scope="module")
+def small():
+    case = mesh.box_case(3, mach=0.3, kspace=10)
+    g = M.periodic_graph(case)
+    rng = np.random.default_rng(3)
+    n = case.nshg
+    bd = rng.standard_normal((n, 5, 5))
+    v = rng.standard_normal((n, 5))
+    x, _ = M.features(bd, v, np.ones(25), g.active)
+    return case, g, x, rng
+
+
+def test_periodic_graph_is_symmetric_with_opposite_edges(small):
+    case, g, _, _ = small
+    per = case.iper - 1
+    slaves = per != np.arange(case.nshg)
+    assert not g.active[slaves].any() and g.active[~slaves].all()
+    pairs = {}
+    for i in range(g.n):
+        for k in range(g.ptr[i], g.ptr[i + 1]):
+            pairs[(i, g.col[k])] = k
+    for (i, j), k in pairs.items():
+        kk = pairs[(j, i)]
+        # same geometric edge seen from both ends, each scaled by its row
+        a, b = g.evec[k], g.evec[kk]
+        assert np.allclose(a / np.linalg.norm(a), -b / np.linalg.norm(b), atol=1e-6)
+    # Kuhn-split nodes have 14 neighbours; wall nodes (y = 0, 1) lose 4
+    deg = np.diff(g.ptr)
+    wall = (np.abs(case.x[:, 1]) < 1e-12) | (np.abs(case.x[:, 1] - 1) < 1e-12)
+    assert set(deg[~slaves & ~wall]) == {14}
+    assert set(deg[~slaves & wall]) == {10}
+
+
+def test_aggregate_adjoint(small):
+    _, g, _, rng = small
+    h = rng.standard_normal((g.n, M.H)).astype(np.float64)
+    dA = rng.standard_normal((g.n, M.NAGG)).astype(np.float64)
+    h[~g.active] = 0.0
+    dA[~g.active] = 0.0
+    lhs = np.sum(M.aggregate(g, h) * dA)
+    rhs = np.sum(h * M.aggregate_T(g, dA))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def test_gradients_match_finite_differences(small):
+    _, g, x, rng = small
+    p = {k: v.astype(np.float64) for k, v in M.init_params(1).items()}
+    for k in p:
+        if k.startswith("b"):
+            p[k] = 0.1 * rng.standard_normal(p[k].shape)
+    ystar = rng.standard_normal((g.n, M.NOUT))
+    loss, gr, _ = M.loss_and_grad(p, g, x, ystar, rounding=False)
+    eps = 1e-6
+    for k in ("W0", "b0", "W1", "b2", "W3", "Wout", "bout"):
+        idx = [tuple(rng.integers(0, s) for s in p[k].shape) for _ in range(4)]
+        for ix in idx:
+            q = {kk: vv.copy() for kk, vv in p.items()}
+            q[k][ix] += eps
+            lp, _, _ = M.loss_and_grad(q, g, x, ystar, rounding=False)
+            q[k][ix] -= 2 * eps
+            lm, _, _ = M.loss_and_grad(q, g, x, ystar, rounding=False)
+            fd = (lp - lm) / (2 * eps)
+            assert abs(fd - gr[k][ix]) <= 1e-5 * max(1e-3, abs(fd)), (k, ix, fd, gr[k][ix])
+
+
+def test_bf16_rounding():
+    a = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 2 ** -7, 1.0 + 3 * 2 ** -8, -2.5, 3.0e38, 1e-40],
+                 dtype=np.float32)
+    b = M.bf(a)
+    # ties go to even mantissas; representable values are kept
+    assert b[0] == 1.0 and b[1] == 1.0 and b[2] == 1.0 + 2 ** -7
+    assert b[3] == 1.0 + 2 ** -6
+    assert b[4] == -2.5
+    assert np.all(b.view(np.uint32) & 0xFFFF == 0)
