@@ -169,6 +169,16 @@ int phg_mpnn_train_step(phg_ctx* ctx, const double* bdiag_raw, const double* v,
  * side b = L^-1 res and the scaled solution z (Dy = U^-1 z). */
 int phg_mpnn_last_pair(phg_ctx* ctx, double* bdiag_raw, double* b, double* zhat);
 
+/* Training over several meshes: dst uses src's parameters and optimiser
+ * state from now on (same device). */
+int phg_mpnn_share(phg_ctx* dst, phg_ctx* src);
+/* Device-resident training set of `capacity` pairs (35 floats per node). */
+int phg_mpnn_dataset(phg_ctx* ctx, int capacity);
+/* Stores the pair of the last solve (keep_pair enabled) in a slot. */
+int phg_mpnn_store_pair(phg_ctx* ctx, int slot);
+/* One training step (Adam when lr > 0) on a stored pair; loss nullable. */
+int phg_mpnn_train_pair(phg_ctx* ctx, int slot, float lr, double* loss);
+
 /* Wait for all work queued on the library stream. */
 int phg_sync(phg_ctx* ctx);
 

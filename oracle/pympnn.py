@@ -6,7 +6,8 @@ finite-difference gradient checks in float64 (tests/test_mpnn_oracle.py).
 
 Network (per node i of the periodic node graph, hidden width H = 64):
 
-    x_i   = [D_i / sD (25), v_i * s (5), 0, 0],  s = sqrt(n_act) / ||v||  (32)
+    x_i   = [Dn_i / sD (25), v_i * s (5), 0, 0],  s = sqrt(n_act) / ||v||  (32)
+            Dn_i = diag(d)^-1 D_i diag(d)^-1,  d_r = sqrt|D_i,rr| (1 where 0)
     h0    = bf(silu(bf(x) W0 + b0))
     for l = 1..3:
         mean_i = (1/deg_i) sum_j h_j
@@ -110,13 +111,17 @@ def periodic_graph(case):
 
 
 def features(bdiag, v, dscale, active):
-    """Node inputs x (n x 32): scaled BDiag (25, column-major block) and the
-    vector scaled to unit RMS; zero rows for inactive nodes."""
+    """Node inputs x (n x 32): BDiag with its diagonal scaled to +-1 (25,
+    column-major block) over dscale, and the vector scaled to unit RMS over
+    the active nodes; zero rows for inactive nodes."""
     n = v.shape[0]
     vn = np.linalg.norm(v)
     s = np.sqrt(float(np.count_nonzero(active))) / vn if vn > 0 else 0.0
     x = np.zeros((n, NIN), dtype=np.float32)
-    x[:, :25] = bdiag.reshape(n, 25, order="F") / dscale[None, :]
+    d = np.sqrt(np.abs(np.einsum("irr->ir", bdiag)))
+    d[d == 0] = 1.0
+    dn = bdiag / (d[:, :, None] * d[:, None, :])
+    x[:, :25] = dn.reshape(n, 25, order="F") / dscale[None, :]
     x[:, 25:30] = v * s
     x[~active] = 0.0
     return x, s

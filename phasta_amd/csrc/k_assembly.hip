@@ -109,15 +109,10 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) nd[k] = A.ien[k * nel + p];
 
-  double yl[4][5], acl[4][5], xl[4][3];
+  double xl[4][3];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const double* v = A.node + 16 * (size_t)nd[k];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      yl[k][f] = v[f];
-      acl[k][f] = v[5 + f];
-    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) xl[k][c] = v[10 + c];
   }
@@ -175,14 +170,32 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
   const double trG = G[0] + G[2] + G[5];
   const double GG = G[0] * G[0] + G[2] * G[2] + G[5] * G[5] +
                     2.0 * (G[1] * G[1] + G[3] * G[3] + G[4] * G[4]);
-  // gradients of Y, constant over the element
-  double g[3][5];
+
+  // shape values of quadrature point b; the nodal y and ac are streamed in
+  // once and folded into the point values and the (element-constant) Y
+  // gradients, so they are never held all at once
+  double Nq[4];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int k = 0; k < 4; ++k) Nq[k] = sel4(cQ.N[0][k], cQ.N[1][k], cQ.N[2][k], cQ.N[3][k], b);
+  double g[3][5], Y[5], aci[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f)
-      g[i][f] = shg[0][i] * yl[0][f] + shg[1][i] * yl[1][f] + shg[2][i] * yl[2][f] +
-                shg[3][i] * yl[3][f];
+  for (int f = 0; f < 5; ++f) {
+    Y[f] = 0.0;
+    aci[f] = 0.0;
+    g[0][f] = g[1][f] = g[2][f] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double* v = A.node + 16 * (size_t)nd[k];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      const double yv = v[f], av = v[5 + f];
+      Y[f] += Nq[k] * yv;
+      aci[f] += Nq[k] * av;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) g[i][f] += shg[k][i] * yv;
+    }
+  }
 
   const double cv = ph.Rgas / ph.gamma1;
   const double cp = ph.Rgas * ph.gamma / ph.gamma1;
@@ -202,15 +215,6 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
   // ---------------- quadrature point q = b: state and residual ----------------
   double rl[5];
   {
-    double Nq[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) Nq[k] = sel4(cQ.N[0][k], cQ.N[1][k], cQ.N[2][k], cQ.N[3][k], b);
-    double Y[5], aci[5];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      Y[f] = Nq[0] * yl[0][f] + Nq[1] * yl[1][f] + Nq[2] * yl[2][f] + Nq[3] * yl[3][f];
-      aci[f] = Nq[0] * acl[0][f] + Nq[1] * acl[1][f] + Nq[2] * acl[2][f] + Nq[3] * acl[3][f];
-    }
     const double pres = Y[0], Tq = Y[4];
     Thermo t;
     t.u[0] = Y[1];
@@ -251,49 +255,47 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
       o[11] = t.tau[1];
       o[12] = t.tau[2];
     }
-    // fluxes: ri[5i+f] = -F_i + F^v_i; ri[15+f] = A0 aci
-    double ri[20];
-    const double E = ei + rk;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const double mui = -t.u[i] * t.rho;
-      ri[5 * i + 0] = mui;
-      double vf[3];
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        vf[m] = mu * (g[i][1 + m] + g[m][1 + i]) + (m == i ? lam * divu : 0.0);
-        ri[5 * i + 1 + m] = mui * t.u[m] - (m == i ? pres : 0.0) + vf[m];
-      }
-      ri[5 * i + 4] = mui * E - t.u[i] * pres +
-                      (t.u[0] * vf[0] + t.u[1] * vf[1] + t.u[2] * vf[2] + kap * g[i][4]);
-    }
-    // least-squares residual L = A_i Y,i + A0 Y,t, scaled by tau
-    double L[5], tmp[5];
+    // time term A0 aci and the least-squares residual L = A_i Y,i + A0 Y,t
+    double m0[5], L[5], tmp[5];
+    A0_times(t, aci, m0);
     Ai_times(t, 0, g[0], L);
-    Ai_times(t, 1, g[1], tmp);
 #pragma unroll
-    for (int f = 0; f < 5; ++f) L[f] += tmp[f];
-    Ai_times(t, 2, g[2], tmp);
+    for (int i = 1; i < 3; ++i) {
+      Ai_times(t, i, g[i], tmp);
 #pragma unroll
-    for (int f = 0; f < 5; ++f) L[f] += tmp[f];
-    A0_times(t, aci, &ri[15]);
+      for (int f = 0; f < 5; ++f) L[f] += tmp[f];
+    }
 #pragma unroll
-    for (int f = 0; f < 5; ++f) L[f] += ri[15 + f];
+    for (int f = 0; f < 5; ++f) L[f] += m0[f];
     L[0] *= t.tau[0];
     L[1] *= t.tau[1];
     L[2] *= t.tau[1];
     L[3] *= t.tau[1];
     L[4] *= t.tau[2];
+    // per direction i: ri = -F_i + F^v_i + A_i tau L, contracted with s_a,i
+    const double E = ei + rk;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) rl[f] = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
+      double ri[5];
+      const double mui = -t.u[i] * t.rho;
+      ri[0] = mui;
+      double vf[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        vf[m] = mu * (g[i][1 + m] + g[m][1 + i]) + (m == i ? lam * divu : 0.0);
+        ri[1 + m] = mui * t.u[m] - (m == i ? pres : 0.0) + vf[m];
+      }
+      ri[4] = mui * E - t.u[i] * pres +
+              (t.u[0] * vf[0] + t.u[1] * vf[1] + t.u[2] * vf[2] + kap * g[i][4]);
       Ai_times(t, i, L, tmp);
 #pragma unroll
-      for (int f = 0; f < 5; ++f) ri[5 * i + f] += tmp[f];
+      for (int f = 0; f < 5; ++f) rl[f] += s[i] * (ri[f] + tmp[f]);
     }
     const double Na = sel4(Nq[0], Nq[1], Nq[2], Nq[3], a);
 #pragma unroll
-    for (int f = 0; f < 5; ++f)
-      rl[f] = WdetJ * (s[0] * ri[f] + s[1] * ri[5 + f] + s[2] * ri[10 + f]) + Na * WdetJ * ri[15 + f];
+    for (int f = 0; f < 5; ++f) rl[f] = WdetJ * rl[f] + Na * WdetJ * m0[f];
     // sum the four quadrature points (lanes b = 0..3 of this node a)
 #pragma unroll
     for (int f = 0; f < 5; ++f) {

@@ -2,7 +2,8 @@
 // the periodic node graph (hidden width 64), bf16 operands on MFMA with
 // float accumulation, and its hand-written backward pass for training.
 //
-//   x_i  = [BDiag_i / dscale (25), v_i * s (5), 0, 0],   s = sqrt(n_act)/||v||
+//   x_i  = [Dn_i / dscale (25), v_i * s (5), 0, 0],   s = sqrt(n_act)/||v||
+//   Dn_i = diag(d)^-1 BDiag_i diag(d)^-1,  d_r = sqrt|BDiag_i,rr|
 //   h0   = bf(silu(x W0 + b0))
 //   A_i  = bf([h_i, mean_j h_j, (1/deg_i) sum_j (h_j - h_i) (x) e_ij])    (320)
 //   h    = bf(h + silu(A W_l + b_l)),  l = 1..3
@@ -57,8 +58,20 @@ __global__ void k_mp_features(const double* __restrict__ bdiag, const double* __
   const bool act = ptr[i + 1] > ptr[i];
   const float s = (float)sptr[0];
   uint16_t* xi = x + (size_t)i * kNin;
+  // symmetric diagonal scaling: Dn = diag(d)^-1 D diag(d)^-1, d_r = sqrt|D_rr|
+  double d[5];
 #pragma unroll
-  for (int e = 0; e < 25; ++e) xi[e] = act ? f2bf((float)bdiag[(size_t)e * n + i] / dscale[e]) : 0;
+  for (int r = 0; r < 5; ++r) {
+    const double t = sqrt(fabs(bdiag[(size_t)(6 * r) * n + i]));
+    d[r] = t > 0.0 ? t : 1.0;
+  }
+#pragma unroll
+  for (int cc = 0; cc < 5; ++cc)
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+      const int e = r + 5 * cc;
+      xi[e] = act ? f2bf((float)(bdiag[(size_t)e * n + i] / (d[r] * d[cc])) / dscale[e]) : 0;
+    }
 #pragma unroll
   for (int f = 0; f < 5; ++f) xi[25 + f] = act ? f2bf((float)v[(size_t)f * n + i] * s) : 0;
   xi[30] = 0;
@@ -453,6 +466,33 @@ __global__ void k_mp_target(const double* __restrict__ t, const double* __restri
     ystar[i * 5 + f] = act ? (float)((t[(size_t)f * n + i] - v[(size_t)f * n + i]) * s) : 0.f;
 }
 
+// pair slot (35 floats per node, node-major planes: 25 BDiag, 5 b, 5 z)
+__global__ void k_mp_store(const double* __restrict__ bd, const double* __restrict__ b,
+                           const double* __restrict__ z, float* __restrict__ slot, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int e = 0; e < 25; ++e) slot[(size_t)e * n + i] = (float)bd[(size_t)e * n + i];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    slot[(size_t)(25 + f) * n + i] = (float)b[(size_t)f * n + i];
+    slot[(size_t)(30 + f) * n + i] = (float)z[(size_t)f * n + i];
+  }
+}
+
+__global__ void k_mp_load(const float* __restrict__ slot, double* __restrict__ bd,
+                          double* __restrict__ b, double* __restrict__ z, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int e = 0; e < 25; ++e) bd[(size_t)e * n + i] = (double)slot[(size_t)e * n + i];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    b[(size_t)f * n + i] = (double)slot[(size_t)(25 + f) * n + i];
+    z[(size_t)f * n + i] = (double)slot[(size_t)(30 + f) * n + i];
+  }
+}
+
 // s = sqrt(n_act / ||v||^2) (0 for a zero vector)
 __global__ void k_mp_scale_from_sq(const double* sq, double n_act, double* s) {
   s[0] = sq[0] > 0.0 ? sqrt(n_act / sq[0]) : 0.0;
@@ -638,12 +678,14 @@ int mp_alloc_train(phg_ctx* c) {
   PHG_TRY(mp_alloc(c, &M.sq, n));
   M.nparts = (int)std::min<long long>(kPartBlocks, (n + 31) / 32);
   PHG_TRY(mp_alloc(c, &M.part, (size_t)M.nparts * kNagg * kH));
-  PHG_TRY(mp_alloc(c, &M.g, Layout::total));
-  PHG_TRY(mp_alloc(c, &M.m1, Layout::total));
-  PHG_TRY(mp_alloc(c, &M.m2, Layout::total));
-  PHG_TRY(launch_fill(c, M.m1, sizeof(float) * Layout::total));
-  PHG_TRY(launch_fill(c, M.m2, sizeof(float) * Layout::total));
-  M.adam_t = 0;
+  if (!M.param_owner) {
+    PHG_TRY(mp_alloc(c, &M.g, Layout::total));
+    PHG_TRY(mp_alloc(c, &M.m1, Layout::total));
+    PHG_TRY(mp_alloc(c, &M.m2, Layout::total));
+    PHG_TRY(launch_fill(c, M.m1, sizeof(float) * Layout::total));
+    PHG_TRY(launch_fill(c, M.m2, sizeof(float) * Layout::total));
+    M.adam_t = 0;
+  }
   M.train_ready = true;
   return 0;
 }
@@ -653,11 +695,85 @@ int mp_free(phg_ctx* c) {
   void* ptrs[] = {M.ptr,  M.col,  M.rev,  M.ev,   M.esum,  M.inv,   M.dscale, M.bd_raw, M.zhat,
                   M.x,    M.h[0], M.h[1], M.h[2], M.h[3],  M.A[0],  M.A[1],   M.A[2],   M.U[0],
                   M.U[1], M.U[2], M.U[3], M.Y,    M.Ystar, M.dh,    M.dU,     M.dA,     M.dY,
-                  M.sq,   M.part, M.w,    M.g,    M.m1,    M.m2,    M.wt[0],  M.wt[1],  M.wt[2],
-                  M.wt[3], M.wr[0], M.wr[1], M.wr[2], M.wr[3], M.sbuf, M.vin,   M.zout,   M.zbasis};
+                  M.sq,   M.part, M.sbuf, M.vin,  M.zout,  M.zbasis, M.ds};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (!M.param_owner) {
+    void* own[] = {M.w,     M.g,     M.m1,    M.m2,    M.wt[0], M.wt[1],
+                   M.wt[2], M.wt[3], M.wr[0], M.wr[1], M.wr[2], M.wr[3]};
+    for (void* p : own)
+      if (p) (void)hipFree(p);
+  }
   M = MpNet{};
+  return 0;
+}
+
+// dst trains and runs with src's parameters (and Adam state); both contexts
+// live on the same device.  dst's own parameter buffers are released.
+int mp_share(phg_ctx* dst, phg_ctx* src) {
+  MpNet& D = dst->mp;
+  MpNet& S = src->mp;
+  if (!D.ready || !S.ready || S.param_owner || dst == src) {
+    phg_set_error("phg_mpnn_share: both contexts need phg_mpnn_init; src must own its parameters");
+    return PHG_EINVAL;
+  }
+  PHG_TRY(mp_alloc_train(src));
+  PHG_TRY(mp_alloc_train(dst));
+  PHG_HIP(hipDeviceSynchronize());
+  if (!D.param_owner) {
+    void* own[] = {D.w,     D.g,     D.m1,    D.m2,    D.wt[0], D.wt[1],
+                   D.wt[2], D.wt[3], D.wr[0], D.wr[1], D.wr[2], D.wr[3]};
+    for (void* p : own)
+      if (p) (void)hipFree(p);
+  }
+  D.w = S.w;
+  D.g = S.g;
+  D.m1 = S.m1;
+  D.m2 = S.m2;
+  for (int l = 0; l < 4; ++l) {
+    D.wt[l] = S.wt[l];
+    D.wr[l] = S.wr[l];
+  }
+  D.param_owner = src;
+  return 0;
+}
+
+int mp_dataset(phg_ctx* c, int capacity) {
+  MpNet& M = c->mp;
+  if (M.ds) {
+    (void)hipFree(M.ds);
+    M.ds = nullptr;
+    M.ds_cap = 0;
+  }
+  if (capacity <= 0) return 0;
+  PHG_TRY(mp_alloc(c, &M.ds, (size_t)capacity * 35 * c->nshg));
+  M.ds_cap = capacity;
+  return 0;
+}
+
+int mp_store_pair(phg_ctx* c, int slot) {
+  MpNet& M = c->mp;
+  if (slot < 0 || slot >= M.ds_cap) {
+    phg_set_error("phg_mpnn_store_pair: slot %d outside 0..%d", slot, M.ds_cap - 1);
+    return PHG_EINVAL;
+  }
+  const long long n = c->nshg;
+  k_mp_store<<<grid_for(n, 256), 256, 0, c->stream>>>(M.bd_raw, c->res, M.zhat,
+                                                      M.ds + (size_t)slot * 35 * n, n);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int mp_load_pair(phg_ctx* c, int slot) {
+  MpNet& M = c->mp;
+  if (slot < 0 || slot >= M.ds_cap) {
+    phg_set_error("phg_mpnn_train_pair: slot %d outside 0..%d", slot, M.ds_cap - 1);
+    return PHG_EINVAL;
+  }
+  const long long n = c->nshg;
+  k_mp_load<<<grid_for(n, 256), 256, 0, c->stream>>>(M.ds + (size_t)slot * 35 * n, M.bd_raw,
+                                                     M.vin, M.zout, n);
+  PHG_HIP(hipGetLastError());
   return 0;
 }
 
@@ -746,10 +862,11 @@ int mp_backward(phg_ctx* c, float lr, double* loss_out) {
     PHG_HIP(hipGetLastError());
   }
   if (lr > 0.f) {
-    M.adam_t += 1;
+    int& step = M.param_owner ? M.param_owner->mp.adam_t : M.adam_t;
+    step += 1;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    const float c1 = 1.f / (1.f - std::pow(b1, (float)M.adam_t));
-    const float c2 = 1.f / (1.f - std::pow(b2, (float)M.adam_t));
+    const float c1 = 1.f / (1.f - std::pow(b1, (float)step));
+    const float c2 = 1.f / (1.f - std::pow(b2, (float)step));
     k_mp_adam<<<grid_for(Layout::total, 256), 256, 0, c->stream>>>(M.w, M.g, M.m1, M.m2,
                                                                     Layout::total, lr, b1, b2, eps,
                                                                     c1, c2);

@@ -155,8 +155,11 @@ __global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO*
 
 // ---------------- block-CSR matvec (sparseap.f) ----------------
 // q = K p(iper(:)); one lane per row, one wavefront per 64-row group.
-// T: stored value type; Acc: type of the products and row sums (float for
-// the mixed-precision inner matvec, double otherwise).
+// T: stored value type; Acc: type of the products and row sums.  The mixed
+// path stores float values and sums rows in double: float row sums of the
+// scaled system lose digits to cancellation (on the 16^3 box the refinement
+// loop then needed 12 instead of 10 Krylov iterations), while the kernel is
+// bound by the matrix bytes, which float storage halves either way.
 template <typename T, typename Acc>
 __global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const int* __restrict__ scolp,
                                               const long long* __restrict__ grp_slot,
@@ -458,10 +461,9 @@ int launch_spsi3pre(phg_ctx* c) {
 }
 
 int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact) {
+  (void)exact;  // both matvecs of the mixed path form double row sums
   const int g = grid_for(c->nshg, 256);
-  if (c->precision == PHG_MIXED && !exact)
-    k_spmv<float, float><<<g, 256, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg);
-  else if (c->precision == PHG_MIXED)
+  if (c->precision == PHG_MIXED)
     k_spmv<float, double><<<g, 256, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg);
   else
     k_spmv<double, double><<<g, 256, 0, c->stream>>>(c->lhs64, c->scolp, c->grp_slot, p, q,

@@ -419,10 +419,9 @@ constexpr int kMaxRefine = 4;
 
 // SolGMRs on the resident y/ac (solgmr.f:449-744).  Leaves Dy in c->dy.
 //
-// Mixed precision: the tangent is assembled and stored in float and the
-// GMRES matvec forms its products in float; the Krylov basis, MGS, Givens
-// and norms stay in double.  After each cycle the true residual of the
-// scaled system, b - K z, is formed in double from the stored matrix and
+// Mixed precision: the tangent is assembled, scaled and stored in float;
+// the Krylov basis, MGS, Givens and norms stay in double.  After each cycle
+// the true residual of the scaled system, b - K z, is formed in double and
 // its norm is tested against the same epsnrm as the FP64 path; cycles
 // restart from that residual until it passes (at most kMaxRefine cycles).
 int sol_core(phg_ctx* c, phg_stats* st) {
@@ -949,6 +948,38 @@ extern "C" int phg_mpnn_train_step(phg_ctx* c, const double* bdiag_raw, const do
     PHG_HIP(hipMemcpyAsync(grad, c->mp.g, sizeof(float) * mp_param_count(), hipMemcpyDeviceToHost,
                            c->stream));
   return sync(c);
+}
+
+extern "C" int phg_mpnn_share(phg_ctx* dst, phg_ctx* src) {
+  PHG_TRY(mp_check(dst));
+  PHG_TRY(mp_check(src));
+  return mp_share(dst, src);
+}
+
+extern "C" int phg_mpnn_dataset(phg_ctx* c, int capacity) {
+  PHG_TRY(mp_check(c));
+  PHG_TRY(mp_dataset(c, capacity));
+  return sync(c);
+}
+
+extern "C" int phg_mpnn_store_pair(phg_ctx* c, int slot) {
+  PHG_TRY(mp_check(c));
+  if (!c->mp.keep_pair) {
+    phg_set_error("phg_mpnn_store_pair: enable keep_pair before the solve");
+    return PHG_EINVAL;
+  }
+  PHG_TRY(mp_store_pair(c, slot));
+  return 0;
+}
+
+extern "C" int phg_mpnn_train_pair(phg_ctx* c, int slot, float lr, double* loss) {
+  PHG_TRY(mp_check(c));
+  PHG_TRY(mp_alloc_train(c));
+  PHG_TRY(mp_load_pair(c, slot));
+  PHG_TRY(mp_forward(c, c->mp.vin, nullptr, true));
+  PHG_TRY(mp_targets(c, c->mp.zout, c->mp.vin));
+  PHG_TRY(mp_backward(c, lr, loss));
+  return 0;
 }
 
 extern "C" int phg_mpnn_last_pair(phg_ctx* c, double* bdiag_raw, double* b, double* zhat) {

@@ -90,6 +90,13 @@ struct MpNet {
   double* zbasis = nullptr;  // FGMRES preconditioned directions (K vectors)
   int adam_t = 0;
   bool train_ready = false;
+  // parameters shared with another context (training over several meshes):
+  // w, g, m1, m2, wt, wr then belong to param_owner
+  phg_ctx* param_owner = nullptr;
+  // device-resident training pairs: per slot 35 floats per node
+  // (BDiag before factorisation 25, b 5, z 5)
+  float* ds = nullptr;
+  int ds_cap = 0;
 };
 
 struct phg_ctx {
@@ -186,12 +193,16 @@ int mp_forward(phg_ctx* c, const double* v, double* z, bool train);
 int mp_backward(phg_ctx* c, float lr, double* loss_out);
 // targets (t - v) s of the last forward's v (t, v: 5 planes, device)
 int mp_targets(phg_ctx* c, const double* t, const double* v);
+int mp_share(phg_ctx* dst, phg_ctx* src);
+int mp_dataset(phg_ctx* c, int capacity);
+int mp_store_pair(phg_ctx* c, int slot);
+// widen a stored pair into bd_raw, vin, zout
+int mp_load_pair(phg_ctx* c, int slot);
 
 // ---- launchers (k_solver.hip) ----
 int launch_i3lu(phg_ctx* c, double* diag, double* r, int code);
 int launch_spsi3pre(phg_ctx* c);
-// exact: in the mixed-precision path, form products and row sums in double
-// (true residual of the refinement loop) instead of float
+// exact: the matvec of a true residual (row sums in double in every path)
 int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact = false);
 int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n);
 int launch_widen(phg_ctx* c, double* dst, const float* src, long long n);

@@ -95,6 +95,15 @@ def box_case(N, mach=0.3, jitter=0.1, grade=1.0, kspace=20, etol=1e-3,
                    iper=iper, colm=colm, rowp=rowp, yold=yold, params=params)
 
 
+def box_state(case, mach, field_seed):
+    """A new synthetic state y(nshg,5) on an existing box mesh."""
+    y = np.zeros((case.nshg, 5), order="F")
+    x = np.asfortranarray(case.x)
+    if _lib().phm_box_state(case.nshg, x, mach, field_seed, config.RGAS, config.GAMMA, y):
+        raise RuntimeError("phm_box_state failed")
+    return y
+
+
 def named_case(name, **over):
     c = dict(config.CASES[name])
     c.update(over)

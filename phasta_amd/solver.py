@@ -74,7 +74,8 @@ EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_
            "phg_bench_spmv", "phg_bench_assembly", "phg_bench_copy", "phg_sync",
            "phg_mpnn_param_count", "phg_mpnn_init", "phg_mpnn_set_weights",
            "phg_mpnn_get_weights", "phg_mpnn_enable", "phg_mpnn_apply",
-           "phg_mpnn_train_step", "phg_mpnn_last_pair")
+           "phg_mpnn_train_step", "phg_mpnn_last_pair", "phg_mpnn_share",
+           "phg_mpnn_dataset", "phg_mpnn_store_pair", "phg_mpnn_train_pair")
 
 
 def lib():
@@ -298,6 +299,21 @@ class GpuSolver:
         z = np.zeros((n, 5), order="F")
         _check(lib().phg_mpnn_last_pair(self.h, _p(bd), _p(b), _p(z)))
         return bd, b, z
+
+    def mpnn_share(self, src):
+        _check(lib().phg_mpnn_share(self.h, src.h))
+
+    def mpnn_dataset(self, capacity):
+        _check(lib().phg_mpnn_dataset(self.h, int(capacity)))
+
+    def mpnn_store_pair(self, slot):
+        _check(lib().phg_mpnn_store_pair(self.h, int(slot)))
+
+    def mpnn_train_pair(self, slot, lr, want_loss=True):
+        loss = ctypes.c_double()
+        _check(lib().phg_mpnn_train_pair(self.h, int(slot), ctypes.c_float(lr),
+                                         ctypes.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
 
     def bench_copy(self, reps=10):
         ms, by = ctypes.c_double(), ctypes.c_double()

@@ -1,0 +1,168 @@
+"""Harvest training pairs with the plain solver and train the learned
+preconditioner on the GPU with the library's own forward/backward kernels.
+
+Pairs: for each of `meshes` 64^3 boxes (jitter ~ U[0, 0.15] h, seeded) and
+`pairs // meshes` states on it (Mach ~ U[0.1, 0.8], seeded field), the FP64
+solver runs a predictor and Newton iterations with block-diagonal GMRES
+(Kspace 40, tolerance 1e-4) and keeps, per Newton iteration, the scaled
+right-hand side b = L^-1 res, the BDiag before factorisation and the scaled
+solution z.  The network learns z from (BDiag, b).
+
+    python -m phasta_amd.train_pc --out phasta_amd/weights/mpnn_box64.npz
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from phasta_amd import config, mesh, solver  # noqa: E402
+
+HARVEST_K = 40
+HARVEST_ETOL = 1e-4
+
+
+def log(msg):
+    print(f"[train_pc {time.strftime('%H:%M:%S')}] {msg}", flush=True)
+
+
+def harvest(N, meshes, pairs, newton, seed):
+    rng = np.random.default_rng(seed)
+    per_mesh = pairs // meshes
+    ctxs, slots = [], []
+    for m in range(meshes):
+        jitter = float(rng.uniform(0.0, 0.15))
+        case = mesh.box_case(N, mach=0.5, jitter=jitter, kspace=HARVEST_K, etol=HARVEST_ETOL,
+                             mesh_seed=config.MESH_SEED + 1000 + m)
+        gpu = solver.GpuSolver(case)
+        gpu.mpnn_init(np.ones(25, dtype=np.float32))
+        if ctxs:
+            gpu.mpnn_share(ctxs[0][1])
+        gpu.mpnn_dataset(per_mesh)
+        gpu.mpnn_enable(use_pc=False, keep_pair=True)
+        k = 0
+        iters = []
+        while k < per_mesh:
+            mach = float(rng.uniform(0.1, 0.8))
+            fseed = int(rng.integers(1, 2 ** 31))
+            gpu.set_params(**config.solver_params(N, mach, HARVEST_K, etol=HARVEST_ETOL))
+            gpu.upload_state(mesh.box_state(case, mach, fseed))
+            gpu.predict()
+            for _ in range(newton):
+                if k >= per_mesh:
+                    break
+                st = gpu.newton_iteration()
+                gpu.mpnn_store_pair(k)
+                iters.append(st["iKs"])
+                slots.append((m, k))
+                k += 1
+        gpu.sync()
+        ctxs.append((case, gpu))
+        log(f"mesh {m}: jitter {jitter:.3f}, {per_mesh} pairs, Krylov its "
+            f"mean {np.mean(iters):.1f} max {max(iters)}")
+    return ctxs, slots
+
+
+def train(ctxs, slots, epochs, lr, seed):
+    rng = np.random.default_rng(seed + 1)
+    total = epochs * len(slots)
+    step = 0
+    history = []
+    for ep in range(epochs):
+        order = rng.permutation(len(slots))
+        losses = []
+        t0 = time.perf_counter()
+        for idx in order:
+            m, k = slots[idx]
+            lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
+            losses.append(ctxs[m][1].mpnn_train_pair(k, lr_t))
+            step += 1
+        dt = time.perf_counter() - t0
+        history.append(float(np.mean(losses)))
+        log(f"epoch {ep}: mean loss {history[-1]:.5f} ({dt:.1f} s, "
+            f"{1e3 * dt / len(slots):.2f} ms per pair)")
+    return history
+
+
+def evaluate(N, weights, seed, states=4):
+    """Plain block-diagonal GMRES vs FGMRES with the network on a mesh and
+    states not used in training (Kspace 20, tolerance 1e-3)."""
+    rng = np.random.default_rng(seed + 7)
+    case = mesh.box_case(N, mach=0.5, jitter=0.1, kspace=20, mesh_seed=config.MESH_SEED + 9999)
+    gpu = solver.GpuSolver(case)
+    gpu.mpnn_init(np.ones(25, dtype=np.float32))
+    gpu.mpnn_set_weights(weights)
+    rows = []
+    for _ in range(states):
+        mach = float(rng.uniform(0.1, 0.8))
+        fseed = int(rng.integers(1, 2 ** 31))
+        gpu.set_params(**config.solver_params(N, mach, 20))
+        res = {}
+        for arm in ("plain", "learned"):
+            gpu.mpnn_enable(use_pc=arm == "learned", keep_pair=False)
+            gpu.upload_state(mesh.box_state(case, mach, fseed))
+            gpu.predict()
+            st = gpu.newton_iteration()
+            res[arm] = (st["iKs"], st["pc_fallback"], st["ms_solve"], bool(st["converged"]))
+        rows.append((mach, res))
+        log(f"eval mach {mach:.2f}: plain its {res['plain'][0]} ({res['plain'][2]:.2f} ms), "
+            f"learned its {res['learned'][0]} fallback {res['learned'][1]} "
+            f"({res['learned'][2]:.2f} ms) converged {res['learned'][3]}")
+    gpu.close()
+    return rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--N", type=int, default=64)
+    ap.add_argument("--meshes", type=int, default=8)
+    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--newton", type=int, default=2)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args()
+    t0 = time.perf_counter()
+    ctxs, slots = harvest(args.N, args.meshes, args.pairs, args.newton, args.seed)
+    log(f"harvested {len(slots)} pairs in {time.perf_counter() - t0:.1f} s")
+    owner = ctxs[0][1]
+    owner.mpnn_set_weights(solver.mpnn_flatten(init_params(args.seed)))
+    hist = train(ctxs, slots, args.epochs, args.lr, args.seed)
+    w = owner.mpnn_get_weights()
+    meta = dict(N=args.N, meshes=args.meshes, pairs=len(slots), newton=args.newton,
+                epochs=args.epochs, lr=args.lr, seed=args.seed, loss=hist,
+                harvest_kspace=HARVEST_K, harvest_etol=HARVEST_ETOL,
+                seconds=time.perf_counter() - t0)
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    np.savez(args.out, weights=w, dscale=np.ones(25, dtype=np.float32),
+             meta=json.dumps(meta))
+    log(f"wrote {args.out}: {json.dumps(meta)}")
+    evaluate(args.N, w, args.seed)
+    # shared parameters belong to ctxs[0]: release the others first
+    for _, g in ctxs[1:]:
+        g.close()
+    owner.close()
+
+
+def init_params(seed):
+    """Initial weights (same scheme as the CPU oracle's init_params)."""
+    rng = np.random.default_rng(seed)
+    p = {"W0": rng.standard_normal((32, 64)) * np.sqrt(2.0 / 32), "b0": np.zeros(64)}
+    for l in (1, 2, 3):
+        p[f"W{l}"] = rng.standard_normal((320, 64)) * (0.5 / np.sqrt(320))
+        p[f"b{l}"] = np.zeros(64)
+    p["Wout"] = np.zeros((64, 5))
+    p["bout"] = np.zeros(5)
+    return p
+
+
+if __name__ == "__main__":
+    main()
