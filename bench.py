@@ -132,7 +132,7 @@ def main():
                                f"to 1e-3 + itrCorrect",
                    "N": spec["N"], "nel": nel, "nshg": case.nshg, "nnz_blocks": case.nnz_tot,
                    "newton_per_time_step": newton, "precision": spec["precision"],
-                   "colors": info["ncolors"]},
+                   "device_bytes": info["dev_bytes"]},
         "roofline": roofline,
         "breakdown": {"ms_assembly_per_step": round(ms_asm / args.steps, 3),
                       "ms_spmv_per_step": round(ms_spmv / args.steps, 3),

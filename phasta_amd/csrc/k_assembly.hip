@@ -3,16 +3,27 @@
 // AsBMFG -> e3b, elmgmr.f:280-612), plus the nodal Dirichlet/periodic
 // constraints on res and BDiag (bc3res.f, bc3bdg.f).
 //
-// Sixteen lanes per element, one 5x5 tangent block (a,b) per lane; the
-// element's per-quadrature-point state is computed once (lane b computes
-// point b) and shared through LDS.  Elements are processed colour by colour; no two
-// elements of a colour share a node, so every scatter into res, BDiag and
-// the block-CSR values is a plain read-modify-write without atomics and the
-// result is deterministic.  For affine tets the metrics, shape gradients and
-// Y gradients are constant over the element, so they are computed once and
-// only the interpolated state, thermodynamics, A-matrices and tau vary per
-// quadrature point.  The tangent block (a,b) is formed per quadrature point
-// as
+// Two passes, every output written exactly once:
+//
+//  k_asm_qp      four lanes per element, lane q = quadrature point q: the
+//                element metrics, the interpolated state, thermodynamics and
+//                tau at q (13 values, kept for the tangent pass) and q's
+//                residual contribution to the element's four nodes, summed
+//                over q across the four lanes and written per (element,
+//                node).
+//  k_asm_blocks  one lane per chunk of (element, a, b) contributions to one
+//                block of the block-CSR matrix ("owner" lanes, lists built
+//                once on the host from the element-to-block map): the lane
+//                forms each 5x5 tangent block from the stored point states,
+//                sums them, the chunks of one block are combined in a fixed
+//                order, Dirichlet rows/columns are applied once per block,
+//                and the block is stored.  The owner of a diagonal block
+//                also sums the node's residual and its BDiag.
+//
+// No read-modify-write of the matrix, no atomics, no element colouring, and
+// the summation order is fixed, so the result is deterministic.
+//
+// The tangent block (a,b) is formed per quadrature point as
 //     WdetJ [ P Q + K_ab ],   P = A(s_a) diag(tau) + N_a I,
 //                             Q = A(s_b) + c N_b A0,   c = almi/(gami alfi) Dtgl
 // with A(v) = sum_i v_i A_i the flux Jacobian along v, s_a = grad N_a and
@@ -33,18 +44,16 @@ struct Quad {
 
 __constant__ Quad cQ;
 
-// Constraint and element data are read through these pointers.
-struct AsmArgs {
-  const int* ien;          // 4 planes of nel (colour order, 0-based)
-  const uint32_t* bmap;    // 16 planes of nel: CSR block index of (a,b)
-  const uint8_t* bcm;
+constexpr int kThermo = 13;   // point state: u(3) rho drdp drdT e1p..e4p tau(3)
+constexpr int kGeo = 13;      // shape gradients (4 x 3) and WdetJ
+
+struct QpArgs {
+  const int* ien;          // 4 planes of nel (0-based)
   const double* node;      // 16 per node: y (p,u,v,w,T), ac (same order), x
-  double* res;             // 5 per node (accumulation layout)
-  double* bdiag;           // 25 per node (accumulation layout)
-  void* lhs;               // 25 per CSR block (T), reference lhsK order
-  long long nshg, nel;
-  int p0, p1;
-  int lhs_on, prec_on;
+  void* qps;               // (nel*4) x 13 point states (S)
+  void* geo;               // nel x 13 (S)
+  double* resid;           // (nel*4) x 5 residual of element e at its node a
+  long long nel;
   PhysParams ph;
 };
 
@@ -87,28 +96,19 @@ __device__ inline void A0_times(const Thermo& t, const double a[5], double r[5])
   r[4] = t.e1p * a[0] + t.rho * (t.u[0] * a[1] + t.u[1] * a[2] + t.u[2] * a[3]) + t.e4p * a[4];
 }
 
-// Lanes per element: lane l = 4a + b owns the tangent block (a,b) and the
-// residual contribution of quadrature point b to node a.
-constexpr int kLanesPerElem = 16;
-constexpr int kAsmThreads = 256;
-constexpr int kElemsPerBlock = kAsmThreads / kLanesPerElem;
-constexpr int kThermo = 13;
-
-template <typename T>
-__global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
-  // per-element quadrature-point state shared by the element's 16 lanes
-  __shared__ double sth[kElemsPerBlock][4][kThermo];
-  const int tid = threadIdx.x;
-  const int el = tid / kLanesPerElem;
-  const int l = tid % kLanesPerElem, a = l >> 2, b = l & 3;
-  const int p_raw = A.p0 + blockIdx.x * kElemsPerBlock + el;
-  const bool valid = p_raw < A.p1;
-  const int p = valid ? p_raw : A.p0;  // idle lanes load a valid element
-  const long long nshg = A.nshg, nel = A.nel;
+// ---------------- pass 1: point states and residual ----------------
+// S: storage type of the point states and geometry (double, or float for
+// the mixed-precision path whose tangent is formed in float).
+template <typename S>
+__global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
+  const long long e_raw = blockIdx.x * 64LL + threadIdx.x / 4;
+  const int q = threadIdx.x & 3;
+  const bool valid = e_raw < A.nel;
+  const long long e = valid ? e_raw : A.nel - 1;  // idle lanes load a valid element
+  const long long nel = A.nel;
   int nd[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) nd[k] = A.ien[k * nel + p];
-
+  for (int k = 0; k < 4; ++k) nd[k] = A.ien[k * nel + e];
   double xl[4][3];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -170,13 +170,20 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
   const double trG = G[0] + G[2] + G[5];
   const double GG = G[0] * G[0] + G[2] * G[2] + G[5] * G[5] +
                     2.0 * (G[1] * G[1] + G[3] * G[3] + G[4] * G[4]);
+  if (valid && q == 0) {
+    S* go = reinterpret_cast<S*>(A.geo) + (size_t)e * kGeo;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) go[3 * k + c] = (S)shg[k][c];
+    go[12] = (S)WdetJ;
+  }
 
-  // shape values of quadrature point b; the nodal y and ac are streamed in
-  // once and folded into the point values and the (element-constant) Y
-  // gradients, so they are never held all at once
+  // nodal y and ac streamed into the point values and the (element-
+  // constant) Y gradients
   double Nq[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) Nq[k] = sel4(cQ.N[0][k], cQ.N[1][k], cQ.N[2][k], cQ.N[3][k], b);
+  for (int k = 0; k < 4; ++k) Nq[k] = sel4(cQ.N[0][k], cQ.N[1][k], cQ.N[2][k], cQ.N[3][k], q);
   double g[3][5], Y[5], aci[5];
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
@@ -201,244 +208,282 @@ __global__ void __launch_bounds__(kAsmThreads) k_assemble(AsmArgs A) {
   const double cp = ph.Rgas * ph.gamma / ph.gamma1;
   const double mu = ph.mu, lam = -(2.0 / 3.0) * mu, kap = mu * cp / ph.pr;
   const double dts = ph.dtsfct * ph.Dtgl;
-  const double cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   const double divu = g[0][1] + g[1][2] + g[2][3];
 
-  // shape gradients of this lane's row node a and column node b
-  double s[3], r[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    s[k] = sel4(shg[0][k], shg[1][k], shg[2][k], shg[3][k], a);
-    r[k] = sel4(shg[0][k], shg[1][k], shg[2][k], shg[3][k], b);
+  const double pres = Y[0], Tq = Y[4];
+  Thermo t;
+  t.u[0] = Y[1];
+  t.u[1] = Y[2];
+  t.u[2] = Y[3];
+  const double rk = 0.5 * (Y[1] * Y[1] + Y[2] * Y[2] + Y[3] * Y[3]);
+  t.rho = pres / (ph.Rgas * Tq);
+  const double ei = Tq * (ph.Rgas / ph.gamma1);
+  const double h = Tq * (ph.Rgas * ph.gamma / ph.gamma1);
+  const double alfap = 1.0 / Tq, betaT = 1.0 / pres;
+  t.drdp = t.rho * betaT;
+  t.drdT = -t.rho * alfap;
+  t.e1p = t.drdp * (h + rk) - alfap * Tq;
+  t.e2p = t.e1p + 1.0;
+  t.e3p = t.rho * (h + rk);
+  t.e4p = t.drdT * (h + rk) + t.rho * cp;
+  // Shakib diagonal tau (itau = 0, ipord = 1)
+  const double uGu = t.u[0] * (t.u[0] * G[0] + 2.0 * (t.u[1] * G[1] + t.u[2] * G[3])) +
+                     t.u[1] * (t.u[1] * G[2] + 2.0 * t.u[2] * G[4]) + t.u[2] * t.u[2] * G[5];
+  const double tt = t.rho * t.rho * ((2.0 * dts) * (2.0 * dts) + uGu) + 36.0 * (mu * mu) * GG;
+  const double sq = sqrt(tt);
+  t.tau[0] = 0.125 * sq / (t.rho * trG) * ph.taucfct;
+  t.tau[1] = 1.0 / sq;
+  t.tau[2] = t.tau[1] / cv * ph.temper;
+  if (valid) {
+    S* o = reinterpret_cast<S*>(A.qps) + ((size_t)e * 4 + q) * kThermo;
+    o[0] = (S)t.u[0];
+    o[1] = (S)t.u[1];
+    o[2] = (S)t.u[2];
+    o[3] = (S)t.rho;
+    o[4] = (S)t.drdp;
+    o[5] = (S)t.drdT;
+    o[6] = (S)t.e1p;
+    o[7] = (S)t.e2p;
+    o[8] = (S)t.e3p;
+    o[9] = (S)t.e4p;
+    o[10] = (S)t.tau[0];
+    o[11] = (S)t.tau[1];
+    o[12] = (S)t.tau[2];
   }
 
-  // ---------------- quadrature point q = b: state and residual ----------------
-  double rl[5];
-  {
-    const double pres = Y[0], Tq = Y[4];
-    Thermo t;
-    t.u[0] = Y[1];
-    t.u[1] = Y[2];
-    t.u[2] = Y[3];
-    const double rk = 0.5 * (Y[1] * Y[1] + Y[2] * Y[2] + Y[3] * Y[3]);
-    t.rho = pres / (ph.Rgas * Tq);
-    const double ei = Tq * (ph.Rgas / ph.gamma1);
-    const double h = Tq * (ph.Rgas * ph.gamma / ph.gamma1);
-    const double alfap = 1.0 / Tq, betaT = 1.0 / pres;
-    t.drdp = t.rho * betaT;
-    t.drdT = -t.rho * alfap;
-    t.e1p = t.drdp * (h + rk) - alfap * Tq;
-    t.e2p = t.e1p + 1.0;
-    t.e3p = t.rho * (h + rk);
-    t.e4p = t.drdT * (h + rk) + t.rho * cp;
-    // Shakib diagonal tau (itau = 0, ipord = 1)
-    const double uGu = t.u[0] * (t.u[0] * G[0] + 2.0 * (t.u[1] * G[1] + t.u[2] * G[3])) +
-                       t.u[1] * (t.u[1] * G[2] + 2.0 * t.u[2] * G[4]) + t.u[2] * t.u[2] * G[5];
-    const double tt = t.rho * t.rho * ((2.0 * dts) * (2.0 * dts) + uGu) + 36.0 * (mu * mu) * GG;
-    const double sq = sqrt(tt);
-    t.tau[0] = 0.125 * sq / (t.rho * trG) * ph.taucfct;
-    t.tau[1] = 1.0 / sq;
-    t.tau[2] = t.tau[1] / cv * ph.temper;
-    if (a == 0) {
-      double* o = sth[el][b];
-      o[0] = t.u[0];
-      o[1] = t.u[1];
-      o[2] = t.u[2];
-      o[3] = t.rho;
-      o[4] = t.drdp;
-      o[5] = t.drdT;
-      o[6] = t.e1p;
-      o[7] = t.e2p;
-      o[8] = t.e3p;
-      o[9] = t.e4p;
-      o[10] = t.tau[0];
-      o[11] = t.tau[1];
-      o[12] = t.tau[2];
+  // time term A0 aci and the least-squares residual L = A_i Y,i + A0 Y,t
+  double m0[5], L[5], tmp[5];
+  A0_times(t, aci, m0);
+  Ai_times(t, 0, g[0], L);
+#pragma unroll
+  for (int i = 1; i < 3; ++i) {
+    Ai_times(t, i, g[i], tmp);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) L[f] += tmp[f];
+  }
+#pragma unroll
+  for (int f = 0; f < 5; ++f) L[f] += m0[f];
+  L[0] *= t.tau[0];
+  L[1] *= t.tau[1];
+  L[2] *= t.tau[1];
+  L[3] *= t.tau[1];
+  L[4] *= t.tau[2];
+  // per direction i: F_i = -F_i(conv) + F^v_i + A_i tau L
+  const double E = ei + rk;
+  double F[3][5];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double mui = -t.u[i] * t.rho;
+    F[i][0] = mui;
+    double vf[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      vf[m] = mu * (g[i][1 + m] + g[m][1 + i]) + (m == i ? lam * divu : 0.0);
+      F[i][1 + m] = mui * t.u[m] - (m == i ? pres : 0.0) + vf[m];
     }
-    // time term A0 aci and the least-squares residual L = A_i Y,i + A0 Y,t
-    double m0[5], L[5], tmp[5];
-    A0_times(t, aci, m0);
-    Ai_times(t, 0, g[0], L);
-#pragma unroll
-    for (int i = 1; i < 3; ++i) {
-      Ai_times(t, i, g[i], tmp);
-#pragma unroll
-      for (int f = 0; f < 5; ++f) L[f] += tmp[f];
-    }
-#pragma unroll
-    for (int f = 0; f < 5; ++f) L[f] += m0[f];
-    L[0] *= t.tau[0];
-    L[1] *= t.tau[1];
-    L[2] *= t.tau[1];
-    L[3] *= t.tau[1];
-    L[4] *= t.tau[2];
-    // per direction i: ri = -F_i + F^v_i + A_i tau L, contracted with s_a,i
-    const double E = ei + rk;
-#pragma unroll
-    for (int f = 0; f < 5; ++f) rl[f] = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      double ri[5];
-      const double mui = -t.u[i] * t.rho;
-      ri[0] = mui;
-      double vf[3];
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        vf[m] = mu * (g[i][1 + m] + g[m][1 + i]) + (m == i ? lam * divu : 0.0);
-        ri[1 + m] = mui * t.u[m] - (m == i ? pres : 0.0) + vf[m];
-      }
-      ri[4] = mui * E - t.u[i] * pres +
+    F[i][4] = mui * E - t.u[i] * pres +
               (t.u[0] * vf[0] + t.u[1] * vf[1] + t.u[2] * vf[2] + kap * g[i][4]);
-      Ai_times(t, i, L, tmp);
+    Ai_times(t, i, L, tmp);
 #pragma unroll
-      for (int f = 0; f < 5; ++f) rl[f] += s[i] * (ri[f] + tmp[f]);
-    }
-    const double Na = sel4(Nq[0], Nq[1], Nq[2], Nq[3], a);
+    for (int f = 0; f < 5; ++f) F[i][f] += tmp[f];
+  }
+  // contributions of point q to node a, summed over the element's four
+  // lanes (q = 0..3); lane q keeps node a = q
+  double mine[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) rl[f] = WdetJ * rl[f] + Na * WdetJ * m0[f];
-    // sum the four quadrature points (lanes b = 0..3 of this node a)
+  for (int a = 0; a < 4; ++a) {
+    double r5[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-      rl[f] += __shfl_xor(rl[f], 1, 64);
-      rl[f] += __shfl_xor(rl[f], 2, 64);
+      r5[f] = WdetJ * (shg[a][0] * F[0][f] + shg[a][1] * F[1][f] + shg[a][2] * F[2][f]) +
+              Nq[a] * WdetJ * m0[f];
+      r5[f] += __shfl_xor(r5[f], 1, 64);
+      r5[f] += __shfl_xor(r5[f], 2, 64);
+      if (a == q) mine[f] = r5[f];
     }
   }
-  __syncthreads();
-
-  const int na = sel4i(nd[0], nd[1], nd[2], nd[3], a);
-  if (valid && b == 0) {
+  if (valid) {
+    double* ro = A.resid + ((size_t)e * 4 + q) * 5;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) A.res[5 * (size_t)na + f] += rl[f];
+    for (int f = 0; f < 5; ++f) ro[f] = mine[f];
   }
-  if (!A.lhs_on && !(A.prec_on && a == b)) return;
+}
 
-  // ---------------- tangent block (a,b) ----------------
-  // Arithmetic type of the tangent: double, or float for the mixed-precision
-  // path (the residual above and the nodal BDiag sums stay in double).
+// ---------------- pass 2: block owners ----------------
+struct BlkArgs {
+  const int4* lanes;       // per lane: block-list offset, CSR entry, count,
+                           // flags (bit 0 leader, bit 1 diagonal, bits 8..15
+                           // chunks of the entry, bits 16.. total count)
+  const int* blocks;       // e * 16 + a * 4 + b, grouped per lane
+  const int* ien;          // 4 planes of nel
+  const uint8_t* bcm;
+  const void* qps;
+  const void* geo;
+  const double* resid;
+  double* res;             // 5 per node
+  double* bdiag;           // 25 per node
+  void* lhs;               // 25 per CSR block (T), reference lhsK order
+  long long nel, nlanes;
+  int lhs_on, prec_on;
+  double mu, lam, kap, cfac;
+};
+
+// T: stored block type and arithmetic type of the per-point tangent (float
+// on the mixed path); S: storage type of the point states.  Sums over
+// points and elements are taken in double.
+template <typename T, typename S>
+__global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
+  __shared__ double stage[64][31];
   using R = T;
-  R sR[3], rR[3];
+  const long long l = blockIdx.x * 64LL + threadIdx.x;
+  int4 L = make_int4(0, 0, 0, 0);
+  if (l < A.nlanes) L = A.lanes[l];
+  const bool diag = (L.w >> 1) & 1;
+  const int cnt = (A.lhs_on || diag) ? L.z : 0;
+  double acc[25];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    sR[k] = (R)s[k];
-    rR[k] = (R)r[k];
-  }
-  const R st = sR[0] * rR[0] + sR[1] * rR[1] + sR[2] * rR[2];
-  const R muR = (R)mu, lamR = (R)lam, kapR = (R)kap, cfacR = (R)cfac;
-  R acc[5][5];
+  for (int i = 0; i < 25; ++i) acc[i] = 0.0;
+  double racc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const S* qps = reinterpret_cast<const S*>(A.qps);
+  const S* geo = reinterpret_cast<const S*>(A.geo);
+  const R muR = (R)A.mu, lamR = (R)A.lam, kapR = (R)A.kap, cfacR = (R)A.cfac;
+  for (int k = 0; k < cnt; ++k) {
+    const int code = A.blocks[L.x + k];
+    const long long e = code >> 4;
+    const int a = (code >> 2) & 3, b = code & 3;
+    const S* go = geo + (size_t)e * kGeo;
+    R sR[3], rR[3];
 #pragma unroll
-  for (int i = 0; i < 5; ++i)
-#pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] = (R)0;
+    for (int c = 0; c < 3; ++c) {
+      sR[c] = (R)go[3 * a + c];
+      rR[c] = (R)go[3 * b + c];
+    }
+    const R W = (R)go[12];
+    const R st = sR[0] * rR[0] + sR[1] * rR[1] + sR[2] * rR[2];
+    // acc += WdetJ [P Q + K_ab] per point; WdetJ is folded into P's rows
 #pragma unroll 1
-  for (int q = 0; q < 4; ++q) {
-    const double* o = sth[el][q];
-    const R u0 = (R)o[0], u1 = (R)o[1], u2 = (R)o[2];
-    const R uq[3] = {u0, u1, u2};
-    const R rho = (R)o[3], drdp = (R)o[4], drdT = (R)o[5];
-    const R e1p = (R)o[6], e2p = (R)o[7], e3p = (R)o[8], e4p = (R)o[9];
-    const R tau0 = (R)o[10], tau1 = (R)o[11], tau2 = (R)o[12];
-    const R Na = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
-    const R Nb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
-    const R una = u0 * sR[0] + u1 * sR[1] + u2 * sR[2];
-    const R unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
-    const R w = unb + cfacR * Nb;
-    // Q = A(s_b) + c N_b A0
-    R Q[5][5];
-    Q[0][0] = drdp * w;
-    Q[0][4] = drdT * w;
+    for (int q = 0; q < 4; ++q) {
+      const S* o = qps + ((size_t)e * 4 + q) * kThermo;
+      const R u0 = (R)o[0], u1 = (R)o[1], u2 = (R)o[2];
+      const R uq[3] = {u0, u1, u2};
+      const R rho = (R)o[3], drdp = (R)o[4], drdT = (R)o[5];
+      const R e1p = (R)o[6], e2p = (R)o[7], e3p = (R)o[8], e4p = (R)o[9];
+      const R tau0 = (R)o[10], tau1 = (R)o[11], tau2 = (R)o[12];
+      const R Na = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
+      const R Nb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
+      const R una = u0 * sR[0] + u1 * sR[1] + u2 * sR[2];
+      const R unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
+      const R w = unb + cfacR * Nb;
+      // Q = A(s_b) + c N_b A0
+      R Q[5][5];
+      Q[0][0] = drdp * w;
+      Q[0][4] = drdT * w;
 #pragma unroll
-    for (int n = 0; n < 3; ++n) Q[0][1 + n] = rho * rR[n];
+      for (int n = 0; n < 3; ++n) Q[0][1 + n] = rho * rR[n];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      Q[1 + m][0] = drdp * uq[m] * w + rR[m];
+      for (int m = 0; m < 3; ++m) {
+        Q[1 + m][0] = drdp * uq[m] * w + rR[m];
 #pragma unroll
-      for (int n = 0; n < 3; ++n) Q[1 + m][1 + n] = rho * (uq[m] * rR[n] + (m == n ? w : (R)0));
-      Q[1 + m][4] = drdT * uq[m] * w;
-    }
-    Q[4][0] = e2p * unb + cfacR * Nb * e1p;
+        for (int n = 0; n < 3; ++n) Q[1 + m][1 + n] = rho * (uq[m] * rR[n] + (m == n ? w : (R)0));
+        Q[1 + m][4] = drdT * uq[m] * w;
+      }
+      Q[4][0] = e2p * unb + cfacR * Nb * e1p;
 #pragma unroll
-    for (int n = 0; n < 3; ++n) Q[4][1 + n] = e3p * rR[n] + rho * uq[n] * w;
-    Q[4][4] = e4p * w;
-    // acc += P Q with P = A(s_a) diag(tau) + N_a I, one row of P at a time
-    {
-      R Pr[5];
-      Pr[0] = drdp * una * tau0 + Na;
+      for (int n = 0; n < 3; ++n) Q[4][1 + n] = e3p * rR[n] + rho * uq[n] * w;
+      Q[4][4] = e4p * w;
+      // rows of P = A(s_a) diag(tau) + N_a I
 #pragma unroll
-      for (int n = 0; n < 3; ++n) Pr[1 + n] = rho * sR[n] * tau1;
-      Pr[4] = drdT * una * tau2;
+      for (int m = 0; m < 5; ++m) {
+        R Pr[5];
+        if (m == 0) {
+          Pr[0] = drdp * una * tau0 + Na;
 #pragma unroll
-      for (int j = 0; j < 5; ++j)
-        acc[0][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
-                     Pr[4] * Q[4][j];
-    }
+          for (int n = 0; n < 3; ++n) Pr[1 + n] = rho * sR[n] * tau1;
+          Pr[4] = drdT * una * tau2;
+        } else if (m < 4) {
+          const int mm = m - 1;
+          Pr[0] = (drdp * uq[mm] * una + sR[mm]) * tau0;
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      R Pr[5];
-      Pr[0] = (drdp * uq[m] * una + sR[m]) * tau0;
+          for (int n = 0; n < 3; ++n)
+            Pr[1 + n] =
+                rho * (uq[mm] * sR[n] + (mm == n ? una : (R)0)) * tau1 + (mm == n ? Na : (R)0);
+          Pr[4] = drdT * uq[mm] * una * tau2;
+        } else {
+          Pr[0] = e2p * una * tau0;
 #pragma unroll
-      for (int n = 0; n < 3; ++n)
-        Pr[1 + n] = rho * (uq[m] * sR[n] + (m == n ? una : (R)0)) * tau1 + (m == n ? Na : (R)0);
-      Pr[4] = drdT * uq[m] * una * tau2;
+          for (int n = 0; n < 3; ++n) Pr[1 + n] = (e3p * sR[n] + rho * uq[n] * una) * tau1;
+          Pr[4] = e4p * una * tau2 + Na;
+        }
 #pragma unroll
-      for (int j = 0; j < 5; ++j)
-        acc[1 + m][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
-                         Pr[4] * Q[4][j];
-    }
-    {
-      R Pr[5];
-      Pr[0] = e2p * una * tau0;
+        for (int n = 0; n < 5; ++n) Pr[n] *= W;
 #pragma unroll
-      for (int n = 0; n < 3; ++n) Pr[1 + n] = (e3p * sR[n] + rho * uq[n] * una) * tau1;
-      Pr[4] = e4p * una * tau2 + Na;
+        for (int j = 0; j < 5; ++j)
+          acc[m + 5 * j] += (double)(Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] +
+                                     Pr[3] * Q[3][j] + Pr[4] * Q[4][j]);
+      }
+      // viscous part K_ab
 #pragma unroll
-      for (int j = 0; j < 5; ++j)
-        acc[4][j] += Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
-                     Pr[4] * Q[4][j];
-    }
-    // viscous part K_ab
+      for (int m = 0; m < 3; ++m)
 #pragma unroll
-    for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n)
+          acc[(1 + m) + 5 * (1 + n)] +=
+              (double)(W * (muR * ((m == n ? st : (R)0) + sR[n] * rR[m]) + lamR * sR[m] * rR[n]));
 #pragma unroll
       for (int n = 0; n < 3; ++n)
-        acc[1 + m][1 + n] += muR * ((m == n ? st : (R)0) + sR[n] * rR[m]) + lamR * sR[m] * rR[n];
+        acc[4 + 5 * (1 + n)] += (double)(W * (muR * (st * uq[n] + sR[n] * unb) + lamR * una * rR[n]));
+      acc[24] += (double)(W * kapR * st);
+    }
+    if (diag) {
+      const double* ro = A.resid + ((size_t)e * 4 + a) * 5;
 #pragma unroll
-    for (int n = 0; n < 3; ++n)
-      acc[4][1 + n] += muR * (st * uq[n] + sR[n] * unb) + lamR * una * rR[n];
-    acc[4][4] += kapR * st;
+      for (int f = 0; f < 5; ++f) racc[f] += ro[f];
+    }
   }
-  if (!valid) return;
-  const R WdetJR = (R)WdetJ;
+  // combine the chunks of one block in lane order
 #pragma unroll
-  for (int i = 0; i < 5; ++i)
+  for (int i = 0; i < 25; ++i) stage[threadIdx.x][i] = acc[i];
 #pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] *= WdetJR;
-  // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103),
-  // summed over the node's elements in double
-  if (A.prec_on && a == b) {
+  for (int f = 0; f < 5; ++f) stage[threadIdx.x][25 + f] = racc[f];
+  __syncthreads();
+  if (!(L.w & 1) || l >= A.nlanes) return;
+  const int nch = (L.w >> 8) & 0xff;
+  const int total = L.w >> 16;
+  for (int c = 1; c < nch; ++c) {
+#pragma unroll
+    for (int i = 0; i < 25; ++i) acc[i] += stage[threadIdx.x + c][i];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) racc[f] += stage[threadIdx.x + c][25 + f];
+  }
+  const int code0 = A.blocks[L.x];
+  const long long e0 = code0 >> 4;
+  const int a0 = (code0 >> 2) & 3, b0 = code0 & 3;
+  const long long nel = A.nel;
+  const int v = A.ien[a0 * nel + e0], wcol = A.ien[b0 * nel + e0];
+  if (diag) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) A.res[5 * (size_t)v + f] = racc[f];
+    // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103)
+    if (A.prec_on) {
+#pragma unroll
+      for (int i = 0; i < 25; ++i) A.bdiag[25 * (size_t)v + i] = acc[i];
+    }
+  }
+  if (!A.lhs_on) return;
+  // bc3LHS on every element block: constrained rows of node a and columns of
+  // node b zeroed, a unit diagonal per element block of the node
+  const uint8_t ma = A.bcm[v], mb = A.bcm[wcol];
+  if (ma | mb) {
 #pragma unroll
     for (int j = 0; j < 5; ++j)
 #pragma unroll
-      for (int i = 0; i < 5; ++i) A.bdiag[25 * (size_t)na + i + 5 * j] += (double)acc[i][j];
-  }
-  if (!A.lhs_on) return;
-  // bc3LHS: constrained rows of node a, columns of node b, unit diagonal
-  const uint8_t ma = A.bcm[na];
-  const uint8_t mb = A.bcm[sel4i(nd[0], nd[1], nd[2], nd[3], b)];
-  if (ma | mb) {
-#pragma unroll
-    for (int i = 0; i < 5; ++i)
-#pragma unroll
-      for (int j = 0; j < 5; ++j) {
-        if (((ma >> i) & 1) || ((mb >> j) & 1)) acc[i][j] = (R)0;
-        if (a == b && i == j && ((ma >> i) & 1)) acc[i][j] = (R)1;
+      for (int i = 0; i < 5; ++i) {
+        if (((ma >> i) & 1) || ((mb >> j) & 1)) acc[i + 5 * j] = 0.0;
+        if (diag && i == j && ((ma >> i) & 1)) acc[i + 5 * j] = (double)total;
       }
   }
-  T* lhs = reinterpret_cast<T*>(A.lhs);
-  T* dst = lhs + 25 * (size_t)A.bmap[l * nel + p];
+  T* dst = reinterpret_cast<T*>(A.lhs) + 25 * (size_t)L.y;
 #pragma unroll
-  for (int j = 0; j < 5; ++j)
-#pragma unroll
-    for (int i = 0; i < 5; ++i) dst[i + 5 * j] += (T)acc[i][j];
+  for (int i = 0; i < 25; ++i) dst[i] = (T)acc[i];
 }
 
 // Wall faces: floating convective, pressure, viscous and heat fluxes
@@ -696,39 +741,58 @@ int launch_pack_nodes(phg_ctx* c) {
 int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   PHG_TRY(upload_quadrature());
   PHG_TRY(launch_pack_nodes(c));
-  AsmArgs A{};
-  A.ien = c->ienc;
-  A.bmap = c->bmap;
-  A.bcm = c->bcm;
-  A.node = c->nodepack;
-  A.res = c->acc_res;
-  A.bdiag = c->acc_bd;
-  A.nshg = c->nshg;
-  A.nel = c->nel;
-  A.lhs_on = lhs;
-  A.prec_on = prec;
-  A.ph = phys_of(c->prm);
   const bool mixed = c->precision == PHG_MIXED;
-  A.lhs = mixed ? (void*)c->lhs_aos32 : (void*)c->lhs_aos;
-  for (size_t k = 0; k + 1 < c->col_off.size(); ++k) {
-    A.p0 = c->col_off[k];
-    A.p1 = c->col_off[k + 1];
-    int n = A.p1 - A.p0;
-    if (n <= 0) continue;
-    const int blocks = (n + kElemsPerBlock - 1) / kElemsPerBlock;
-    if (mixed)
-      k_assemble<float><<<blocks, kAsmThreads, 0, c->stream>>>(A);
-    else
-      k_assemble<double><<<blocks, kAsmThreads, 0, c->stream>>>(A);
-  }
+  const PhysParams ph = phys_of(c->prm);
+  // pass 1: point states and element residuals
+  QpArgs Q{};
+  Q.ien = c->ienc;
+  Q.node = c->nodepack;
+  Q.qps = mixed ? (void*)c->qps32 : (void*)c->qps;
+  Q.geo = mixed ? (void*)c->geo32 : (void*)c->geo;
+  Q.resid = c->resid;
+  Q.nel = c->nel;
+  Q.ph = ph;
+  const int g1 = (int)((c->nel + 63) / 64);
+  if (mixed)
+    k_asm_qp<float><<<g1, 256, 0, c->stream>>>(Q);
+  else
+    k_asm_qp<double><<<g1, 256, 0, c->stream>>>(Q);
   PHG_HIP(hipGetLastError());
+  // pass 2: block owners (res, BDiag, lhsK written once)
+  BlkArgs B{};
+  B.lanes = c->lanes;
+  B.blocks = c->blocks;
+  B.ien = c->ienc;
+  B.bcm = c->bcm;
+  B.qps = Q.qps;
+  B.geo = Q.geo;
+  B.resid = c->resid;
+  B.res = c->acc_res;
+  B.bdiag = c->acc_bd;
+  B.lhs = mixed ? (void*)c->lhs_aos32 : (void*)c->lhs_aos;
+  B.nel = c->nel;
+  B.nlanes = c->nlanes;
+  B.lhs_on = lhs;
+  B.prec_on = prec;
+  const double cp = ph.Rgas * ph.gamma / ph.gamma1;
+  B.mu = ph.mu;
+  B.lam = -(2.0 / 3.0) * ph.mu;
+  B.kap = ph.mu * cp / ph.pr;
+  B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
+  const int g2 = (int)((c->nlanes + 63) / 64);
+  if (mixed)
+    k_asm_blocks<float, float><<<g2, 64, 0, c->stream>>>(B);
+  else
+    k_asm_blocks<double, double><<<g2, 64, 0, c->stream>>>(B);
+  PHG_HIP(hipGetLastError());
+  // wall faces: residual only, added colour by colour
   WallArgs W{};
   W.ien = c->ienbc;
   W.node = c->nodepack;
   W.res = c->acc_res;
   W.nshg = c->nshg;
   W.nelb = c->nelb;
-  W.ph = A.ph;
+  W.ph = ph;
   for (size_t k = 0; k + 1 < c->colb_off.size(); ++k) {
     W.p0 = c->colb_off[k];
     W.p1 = c->colb_off[k + 1];

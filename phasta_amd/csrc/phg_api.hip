@@ -123,11 +123,10 @@ int setup(phg_ctx* c, const phg_mesh* m) {
     return PHG_EINVAL;
   }
 
-  // ---- colouring ----
-  std::vector<int> order, orderb;
-  if (colour_elements(nshg, nel, m->ien, c->col_off, order) ||
-      colour_elements(nshg, nelb, m->ienb, c->colb_off, orderb)) {
-    phg_set_error("element colouring needs more than 64 colours");
+  // ---- colouring of the wall elements (their residual is scattered) ----
+  std::vector<int> orderb;
+  if (colour_elements(nshg, nelb, m->ienb, c->colb_off, orderb)) {
+    phg_set_error("wall element colouring needs more than 64 colours");
     return PHG_ECOLOR;
   }
 
@@ -169,17 +168,16 @@ int setup(phg_ctx* c, const phg_mesh* m) {
     }
   }
 
-  // ---- element -> slot map (replaces the per-entry sparseloc search) ----
+  // ---- element -> block map (replaces the per-entry sparseloc search) ----
   std::vector<int> ienc(4 * nel);
   std::vector<uint32_t> bmap(16 * nel);
   int bad = 0;
 #pragma omp parallel for schedule(static) reduction(| : bad)
-  for (long long p = 0; p < nel; ++p) {
-    const long long e = order[p];
+  for (long long e = 0; e < nel; ++e) {
     int nd[4];
     for (int a = 0; a < 4; ++a) {
       nd[a] = m->ien[e + nel * a] - 1;
-      ienc[a * nel + p] = nd[a];
+      ienc[a * nel + e] = nd[a];
     }
     for (int a = 0; a < 4; ++a) {
       const int* lst = m->rowp + (m->colm[nd[a]] - 1);
@@ -190,7 +188,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
           bad = 1;
           continue;
         }
-        bmap[(4 * a + b) * nel + p] = (uint32_t)((m->colm[nd[a]] - 1) + (it - lst));
+        bmap[(4 * a + b) * nel + e] = (uint32_t)((m->colm[nd[a]] - 1) + (it - lst));
       }
     }
   }
@@ -198,6 +196,54 @@ int setup(phg_ctx* c, const phg_mesh* m) {
     phg_set_error("element adjacency missing from colm/rowp");
     return PHG_EINVAL;
   }
+  if (16 * nel >= (1LL << 31)) {
+    phg_set_error("too many elements for 32-bit block lists");
+    return PHG_EINVAL;
+  }
+  // ---- block owners: the element blocks of every CSR entry, in element
+  // order, cut into chunks of at most kChunk; the chunks of one entry are
+  // consecutive lanes of one 64-lane group ----
+  std::vector<int> bcount(nnz + 1, 0);
+  for (long long i = 0; i < 16 * nel; ++i) bcount[bmap[i] + 1]++;
+  for (long long k = 0; k < nnz; ++k) bcount[k + 1] += bcount[k];
+  std::vector<int> blocks(16 * nel);
+  {
+    std::vector<int> pos(bcount.begin(), bcount.end() - 1);
+    for (long long e = 0; e < nel; ++e)
+      for (int ab = 0; ab < 16; ++ab) blocks[pos[bmap[ab * nel + e]]++] = (int)(e * 16 + ab);
+  }
+  std::vector<int4> lanes;
+  lanes.reserve(nnz + 4 * nshg);
+  std::vector<int> blk_lane;  // blocks in lane order
+  blk_lane.reserve(16 * nel);
+  std::vector<int> rowof(nnz);
+  for (long long r = 0; r < nshg; ++r)
+    for (long long k = m->colm[r] - 1; k < m->colm[r + 1] - 1; ++k) rowof[k] = (int)r;
+  for (long long k = 0; k < nnz; ++k) {
+    const int n = bcount[k + 1] - bcount[k];
+    if (n == 0) {
+      phg_set_error("CSR entry %lld has no element block", k);
+      return PHG_EINVAL;
+    }
+    const int nch = (n + kChunk - 1) / kChunk;
+    if (nch > 64 || n >= (1 << 15)) {
+      phg_set_error("CSR entry %lld has %d element blocks", k, n);
+      return PHG_EINVAL;
+    }
+    if ((long long)(lanes.size() % 64) + nch > 64)
+      while (lanes.size() % 64) lanes.push_back(make_int4(0, 0, 0, 0));
+    const bool dg = (m->rowp[k] - 1) == rowof[k];
+    for (int ch = 0; ch < nch; ++ch) {
+      const int b0 = (int)((long long)n * ch / nch), b1 = (int)((long long)n * (ch + 1) / nch);
+      int flags = (dg ? 2 : 0) | (ch == 0 ? 1 : 0);
+      if (ch == 0) flags |= (nch << 8) | (n << 16);
+      lanes.push_back(make_int4((int)blk_lane.size(), (int)k, b1 - b0, flags));
+      for (int b = b0; b < b1; ++b) blk_lane.push_back(blocks[bcount[k] + b]);
+    }
+  }
+  c->nlanes = (long long)lanes.size();
+  blocks.clear();
+  blocks.shrink_to_fit();
   std::vector<int> ienbc(4 * std::max<long long>(nelb, 1));
   for (long long p = 0; p < nelb; ++p)
     for (int a = 0; a < 4; ++a) ienbc[a * nelb + p] = m->ienb[orderb[p] + nelb * a] - 1;
@@ -233,7 +279,16 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   const bool mixed = c->precision == PHG_MIXED;
   PHG_TRY(dev_alloc(c, &c->x, 3 * nshg));
   PHG_TRY(dev_alloc(c, &c->ienc, 4 * nel));
-  PHG_TRY(dev_alloc(c, &c->bmap, 16 * nel));
+  PHG_TRY(dev_alloc(c, &c->lanes, c->nlanes));
+  PHG_TRY(dev_alloc(c, &c->blocks, 16 * nel));
+  if (mixed) {
+    PHG_TRY(dev_alloc(c, &c->qps32, (size_t)nel * 4 * 13));
+    PHG_TRY(dev_alloc(c, &c->geo32, (size_t)nel * 13));
+  } else {
+    PHG_TRY(dev_alloc(c, &c->qps, (size_t)nel * 4 * 13));
+    PHG_TRY(dev_alloc(c, &c->geo, (size_t)nel * 13));
+  }
+  PHG_TRY(dev_alloc(c, &c->resid, (size_t)nel * 4 * 5));
   PHG_TRY(dev_alloc(c, &c->ienbc, ienbc.size()));
   PHG_TRY(dev_alloc(c, &c->bcm, nshg));
   PHG_TRY(dev_alloc(c, &c->iper, nshg));
@@ -277,7 +332,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   std::copy(m->x, m->x + 3 * nshg, xs.begin());
   PHG_TRY(upload(c, c->x, xs.data(), xs.size()));
   PHG_TRY(upload(c, c->ienc, ienc.data(), ienc.size()));
-  PHG_TRY(upload(c, c->bmap, bmap.data(), bmap.size()));
+  PHG_TRY(upload(c, c->lanes, lanes.data(), lanes.size()));
+  PHG_TRY(upload(c, c->blocks, blk_lane.data(), blk_lane.size()));
   PHG_TRY(upload(c, c->ienbc, ienbc.data(), ienbc.size()));
   PHG_TRY(upload(c, c->bcm, bcm.data(), bcm.size()));
   PHG_TRY(upload(c, c->iper, iper0.data(), iper0.size()));
@@ -319,14 +375,8 @@ double* kvec(phg_ctx* c, int k) { return c->krylov + (size_t)k * 5 * c->nshg; }
 int elm_core(phg_ctx* c) {
   const long long n = c->nshg;
   const bool lhs = c->prm.lhs == 1, prec = c->prm.iprec != 0;
-  PHG_TRY(launch_fill(c, c->acc_res, sizeof(double) * 5 * n));
-  if (prec) PHG_TRY(launch_fill(c, c->acc_bd, sizeof(double) * 25 * n));
-  if (lhs) {
-    if (c->precision == PHG_MIXED)
-      PHG_TRY(launch_fill(c, c->lhs_aos32, sizeof(float) * 25 * c->nnz));
-    else
-      PHG_TRY(launch_fill(c, c->lhs_aos, sizeof(double) * 25 * c->nnz));
-  }
+  // every node owns its diagonal block and every CSR entry has an owner, so
+  // the assembly stores res, BDiag and lhsK completely: no clearing
   PHG_TRY(launch_assembly(c, lhs, prec));
   PHG_TRY(launch_bc_res_bdiag(c, prec));
   return 0;
@@ -636,7 +686,8 @@ extern "C" int phg_finalize(phg_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   mp_free(c);
-  void* ptrs[] = {c->x,        c->ienc,     c->bmap,   c->ienbc,  c->bcm,       c->iper,
+  void* ptrs[] = {c->x,        c->ienc,     c->lanes,  c->blocks, c->qps, c->qps32, c->geo,
+                  c->geo32,    c->resid,    c->ienbc,  c->bcm,       c->iper,
                   c->BCv,      c->fold_master, c->fold_ptr, c->fold_slave, c->grp_slot,
                   c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
@@ -667,7 +718,7 @@ extern "C" int phg_set_params(phg_ctx* c, const phg_params* prm) {
 
 extern "C" int phg_info(phg_ctx* c, int* ncolors, int* ncolors_b, long long* dev_bytes) {
   PHG_TRY(check_ctx(c));
-  if (ncolors) *ncolors = (int)c->col_off.size() - 1;
+  if (ncolors) *ncolors = 0;  // interior elements are not coloured
   if (ncolors_b) *ncolors_b = (int)c->colb_off.size() - 1;
   if (dev_bytes) *dev_bytes = c->dev_bytes;
   return 0;

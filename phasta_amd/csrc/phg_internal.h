@@ -32,6 +32,10 @@ void phg_set_error(const char* fmt, ...);
 // a group holds S slots, each slot one 5x5 block per row of the group.
 constexpr int kGroup = 64;
 
+// Largest number of element blocks one assembly lane sums (the blocks of
+// one CSR entry are split into chunks of at most this size).
+constexpr int kChunk = 6;
+
 // Physical constants and time-integration factors used in the kernels.
 struct PhysParams {
   double Rgas, gamma, gamma1, pr, mu;
@@ -109,14 +113,21 @@ struct phg_ctx {
 
   // mesh (device)
   double* x = nullptr;          // 3 planes of nshg
-  int* ienc = nullptr;          // 4 planes of nel, colour order, 0-based
-  uint32_t* bmap = nullptr;     // 16 planes of nel: CSR block index of (a,b)
+  int* ienc = nullptr;          // 4 planes of nel, 0-based
+  // assembly: point states of the elements and the block-owner lanes
+  long long nlanes = 0;
+  int4* lanes = nullptr;        // block-list offset, CSR entry, count, flags
+  int* blocks = nullptr;        // e*16 + a*4 + b, grouped per owner lane
+  double* qps = nullptr;        // (nel*4) x 13 point states (FP64 path)
+  float* qps32 = nullptr;       // same, mixed path
+  double* geo = nullptr;        // nel x 13: shape gradients, WdetJ
+  float* geo32 = nullptr;
+  double* resid = nullptr;      // (nel*4) x 5 element residual per node
   int* ienbc = nullptr;         // 4 planes of nelb, colour order, 0-based
   uint8_t* bcm = nullptr;       // constraint mask per node
   int* iper = nullptr;          // 0-based master
   double* BCv = nullptr;        // (nshg, ndofBC)
-  std::vector<int> col_off;     // interior colour offsets (host)
-  std::vector<int> colb_off;    // boundary colour offsets (host)
+  std::vector<int> colb_off;    // wall-element colour offsets (host)
 
   // periodic fold: masters that own slaves, slaves ascending per master
   int nfold = 0;
