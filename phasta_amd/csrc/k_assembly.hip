@@ -335,7 +335,11 @@ template <typename T, typename S>
 __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   __shared__ double stage[64][31];
   using R = T;
-  const long long l = blockIdx.x * 64LL + threadIdx.x;
+  // consecutive workgroups run on different XCDs (round robin over 8): give
+  // each XCD a contiguous range of lanes so its L2 sees neighbouring blocks
+  const long long per = (gridDim.x + 7) / 8;
+  const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const long long l = wg * 64 + threadIdx.x;
   int4 L = make_int4(0, 0, 0, 0);
   if (l < A.nlanes) L = A.lanes[l];
   const bool diag = (L.w >> 1) & 1;
@@ -779,7 +783,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.lam = -(2.0 / 3.0) * ph.mu;
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
-  const int g2 = (int)((c->nlanes + 63) / 64);
+  const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
   if (mixed)
     k_asm_blocks<float, float><<<g2, 64, 0, c->stream>>>(B);
   else

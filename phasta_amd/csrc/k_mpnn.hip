@@ -79,43 +79,72 @@ __global__ void k_mp_features(const double* __restrict__ bdiag, const double* __
 }
 
 // ---------------- aggregation ----------------
-// One wavefront per node, lane = channel.
+// Sixteen lanes per node, four channels per lane (8-byte loads), so one
+// neighbour row of 128 bytes is one coalesced load of the node's lanes;
+// four nodes per wavefront.
+__device__ inline void ld4(const uint16_t* p, float v[4]) {
+  const uint2 w = *reinterpret_cast<const uint2*>(p);
+  v[0] = __uint_as_float(w.x << 16);
+  v[1] = __uint_as_float(w.x & 0xffff0000u);
+  v[2] = __uint_as_float(w.y << 16);
+  v[3] = __uint_as_float(w.y & 0xffff0000u);
+}
+__device__ inline void st4(uint16_t* p, const float v[4]) {
+  uint2 w;
+  w.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+  w.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+  *reinterpret_cast<uint2*>(p) = w;
+}
+
 __global__ void __launch_bounds__(256) k_mp_agg(const uint16_t* __restrict__ h,
                                                 const int* __restrict__ ptr,
                                                 const int* __restrict__ col,
                                                 const float4* __restrict__ ev,
                                                 const float* __restrict__ inv,
                                                 uint16_t* __restrict__ A, long long n) {
-  const long long i = blockIdx.x * (long long)(blockDim.x / 64) + threadIdx.x / 64;
-  const int c = threadIdx.x & 63;
+  const long long i = blockIdx.x * 16LL + threadIdx.x / 16;
+  const int c0 = 4 * (threadIdx.x & 15);
   if (i >= n) return;
-  const float hi = bf2f(h[(size_t)i * kH + c]);
-  float m = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
+  float hi[4], m[4] = {0.f, 0.f, 0.f, 0.f}, g0[4] = {0.f, 0.f, 0.f, 0.f},
+               g1[4] = {0.f, 0.f, 0.f, 0.f}, g2[4] = {0.f, 0.f, 0.f, 0.f};
+  ld4(h + (size_t)i * kH + c0, hi);
   const int k0 = ptr[i], k1 = ptr[i + 1];
+#pragma unroll 2
   for (int k = k0; k < k1; ++k) {
     const int j = col[k];
     const float4 e = ev[k];
-    const float hj = bf2f(h[(size_t)j * kH + c]);
-    const float d = hj - hi;
-    m += hj;
-    g0 += d * e.x;
-    g1 += d * e.y;
-    g2 += d * e.z;
+    float hj[4];
+    ld4(h + (size_t)j * kH + c0, hj);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float d = hj[t] - hi[t];
+      m[t] += hj[t];
+      g0[t] += d * e.x;
+      g1[t] += d * e.y;
+      g2[t] += d * e.z;
+    }
   }
   const float w = inv[i];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    m[t] *= w;
+    g0[t] *= w;
+    g1[t] *= w;
+    g2[t] *= w;
+  }
   uint16_t* a = A + (size_t)i * kNagg;
-  a[c] = h[(size_t)i * kH + c];
-  a[kH + c] = f2bf(m * w);
-  a[2 * kH + c] = f2bf(g0 * w);
-  a[3 * kH + c] = f2bf(g1 * w);
-  a[4 * kH + c] = f2bf(g2 * w);
+  *reinterpret_cast<uint2*>(a + c0) = *reinterpret_cast<const uint2*>(h + (size_t)i * kH + c0);
+  st4(a + kH + c0, m);
+  st4(a + 2 * kH + c0, g0);
+  st4(a + 3 * kH + c0, g1);
+  st4(a + 4 * kH + c0, g2);
 }
 
 // Adjoint of k_mp_agg:
 //   dh_m += dA_m[0:64] - inv_m sum_c esum_m[c] dgrad_m[c]
 //           + sum_{j in N(m)} inv_j (dmean_j + sum_c e_jm[c] dgrad_j[c])
 // with e_jm the vector of the reverse edge (rev[k]) and esum_m = sum_j e_mj.
-// Rows without edges get dh = 0.
+// Rows without edges get dh = 0.  Same lane layout as k_mp_agg.
 __global__ void __launch_bounds__(256) k_mp_agg_bwd(const float* __restrict__ dA,
                                                     const int* __restrict__ ptr,
                                                     const int* __restrict__ col,
@@ -124,25 +153,44 @@ __global__ void __launch_bounds__(256) k_mp_agg_bwd(const float* __restrict__ dA
                                                     const float4* __restrict__ esum,
                                                     const float* __restrict__ inv,
                                                     float* __restrict__ dh, long long n) {
-  const long long m = blockIdx.x * (long long)(blockDim.x / 64) + threadIdx.x / 64;
-  const int c = threadIdx.x & 63;
+  const long long m = blockIdx.x * 16LL + threadIdx.x / 16;
+  const int c0 = 4 * (threadIdx.x & 15);
   if (m >= n) return;
+  float4* out = reinterpret_cast<float4*>(dh + (size_t)m * kH + c0);
   const int k0 = ptr[m], k1 = ptr[m + 1];
   if (k1 == k0) {
-    dh[(size_t)m * kH + c] = 0.f;
+    *out = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
   const float* am = dA + (size_t)m * kNagg;
   const float4 es = esum[m];
-  float acc = dh[(size_t)m * kH + c] + am[c] -
-              inv[m] * (es.x * am[2 * kH + c] + es.y * am[3 * kH + c] + es.z * am[4 * kH + c]);
+  const float wm = inv[m];
+  const float4 d0 = *out;
+  const float4 s0 = *reinterpret_cast<const float4*>(am + c0);
+  const float4 a2 = *reinterpret_cast<const float4*>(am + 2 * kH + c0);
+  const float4 a3 = *reinterpret_cast<const float4*>(am + 3 * kH + c0);
+  const float4 a4 = *reinterpret_cast<const float4*>(am + 4 * kH + c0);
+  float acc[4];
+  acc[0] = d0.x + s0.x - wm * (es.x * a2.x + es.y * a3.x + es.z * a4.x);
+  acc[1] = d0.y + s0.y - wm * (es.x * a2.y + es.y * a3.y + es.z * a4.y);
+  acc[2] = d0.z + s0.z - wm * (es.x * a2.z + es.y * a3.z + es.z * a4.z);
+  acc[3] = d0.w + s0.w - wm * (es.x * a2.w + es.y * a3.w + es.z * a4.w);
+#pragma unroll 2
   for (int k = k0; k < k1; ++k) {
     const int j = col[k];
     const float4 e = ev[rev[k]];
+    const float wj = inv[j];
     const float* aj = dA + (size_t)j * kNagg;
-    acc += inv[j] * (aj[kH + c] + e.x * aj[2 * kH + c] + e.y * aj[3 * kH + c] + e.z * aj[4 * kH + c]);
+    const float4 b1 = *reinterpret_cast<const float4*>(aj + kH + c0);
+    const float4 b2 = *reinterpret_cast<const float4*>(aj + 2 * kH + c0);
+    const float4 b3 = *reinterpret_cast<const float4*>(aj + 3 * kH + c0);
+    const float4 b4 = *reinterpret_cast<const float4*>(aj + 4 * kH + c0);
+    acc[0] += wj * (b1.x + e.x * b2.x + e.y * b3.x + e.z * b4.x);
+    acc[1] += wj * (b1.y + e.x * b2.y + e.y * b3.y + e.z * b4.y);
+    acc[2] += wj * (b1.z + e.x * b2.z + e.y * b3.z + e.z * b4.z);
+    acc[3] += wj * (b1.w + e.x * b2.w + e.y * b3.w + e.z * b4.w);
   }
-  dh[(size_t)m * kH + c] = acc;
+  *out = make_float4(acc[0], acc[1], acc[2], acc[3]);
 }
 
 // ---------------- dense layers on MFMA ----------------
@@ -809,7 +857,7 @@ int mp_forward(phg_ctx* c, const double* v, double* z, bool train) {
     uint16_t* hin = M.h[train ? l - 1 : (l - 1) & 1];
     uint16_t* hout = M.h[train ? l : l & 1];
     uint16_t* A = M.A[train ? l - 1 : 0];
-    k_mp_agg<<<grid_for(n, 4), 256, 0, c->stream>>>(hin, M.ptr, M.col, M.ev, M.inv, A, n);
+    k_mp_agg<<<grid_for(n, 16), 256, 0, c->stream>>>(hin, M.ptr, M.col, M.ev, M.inv, A, n);
     PHG_HIP(hipGetLastError());
     k_mp_dense<kNagg><<<gd, 256, 0, c->stream>>>(A, M.wt[l], M.w + kBoff[l], hin, M.ptr, hout,
                                                 train ? M.U[l] : nullptr, 1, n);
@@ -857,7 +905,7 @@ int mp_backward(phg_ctx* c, float lr, double* loss_out) {
     PHG_HIP(hipGetLastError());
     if (l == 0) break;
     k_mp_dA<<<gd, 256, 0, c->stream>>>(M.dU, M.wr[l], M.dA, n);
-    k_mp_agg_bwd<<<grid_for(n, 4), 256, 0, c->stream>>>(M.dA, M.ptr, M.col, M.rev, M.ev, M.esum,
+    k_mp_agg_bwd<<<grid_for(n, 16), 256, 0, c->stream>>>(M.dA, M.ptr, M.col, M.rev, M.ev, M.esum,
                                                         M.inv, M.dh, n);
     PHG_HIP(hipGetLastError());
   }

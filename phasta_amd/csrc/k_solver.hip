@@ -229,18 +229,19 @@ __global__ void k_fold_vec(double* v, const int* master, const int* ptr, const i
 }
 
 // ---------------- deterministic reductions ----------------
-// Each block reduces its share (wavefront shuffle, then LDS), writes one
-// partial; the last block to finish sums the partials in block order.
+// A reduction kernel writes one partial per block (wavefront shuffle, then
+// the block's waves in order) and never synchronises with other blocks;
+// the partials are summed in a fixed order by k_finish or, for the MGS
+// chain, by every block of the next step (same code, same data: the same
+// value in every block).  No atomics, so results do not depend on timing.
 __device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
-__device__ inline bool block_reduce_and_finish(double v, double* part, unsigned* count,
-                                               double* out) {
+__device__ inline void block_partial(double v, double* part) {
   __shared__ double ws[kRedThreads / 64];
-  __shared__ bool last;
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) ws[wid] = v;
@@ -249,52 +250,106 @@ __device__ inline bool block_reduce_and_finish(double v, double* part, unsigned*
     double s = 0.0;
     for (int w = 0; w < kRedThreads / 64; ++w) s += ws[w];
     part[blockIdx.x] = s;
-    __threadfence();
-    unsigned t = atomicAdd(count, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+
+// sum of np partials in a fixed order, by a whole block; every thread
+// returns the total
+__device__ inline double block_sum_partials(const double* part, int np) {
+  __shared__ double ws2[kRedThreads / 64];
+  __shared__ double total;
   double s = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-    s += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  // fixed-order combination of the per-thread sums
+  for (int b = threadIdx.x; b < np; b += blockDim.x) s += part[b];
   s = wave_sum(s);
-  __syncthreads();
-  if (lane == 0) ws[wid] = s;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) ws2[wid] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
-    for (int w = 0; w < kRedThreads / 64; ++w) t += ws[w];
-    *out = t;
-    *count = 0u;
+    for (int w = 0; w < kRedThreads / 64; ++w) t += ws2[w];
+    total = t;
   }
-  return true;
+  __syncthreads();
+  return total;
 }
+
+// out[s] = sum of the partials of channel s (stride between channels):
+// np0 partials in channel 0, np partials in the others
+__global__ void __launch_bounds__(kRedThreads) k_finish(const double* part, int np0, int np,
+                                                        long long stride, int nsums, double* out) {
+  for (int s0 = 0; s0 < nsums; ++s0) {
+    const double t = block_sum_partials(part + (size_t)s0 * stride, s0 == 0 ? np0 : np);
+    if (threadIdx.x == 0) out[s0] = t;
+    __syncthreads();
+  }
+}
+
+// Reductions take one pass of kUnroll independent, coalesced loads per
+// thread (grid sized by red_grid), so every load of a thread is in flight
+// at once; a grid-stride tail covers vectors larger than the grid.
+constexpr int kUnroll = 8;
 
 __global__ void __launch_bounds__(kRedThreads) k_dot(const double* a, const double* b, long long n,
-                                                     double* part, unsigned* count, double* out) {
-  double v = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    v += a[i] * b[i];
-  block_reduce_and_finish(v, part, count, out);
+                                                     double* part) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double v[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) v[u] = 0.0;
+  for (; i + (kUnroll - 1) * stride < n; i += kUnroll * stride) {
+    double x[kUnroll], y[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      x[u] = a[i + u * stride];
+      y[u] = b[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) v[u] += x[u] * y[u];
+  }
+  for (; i < n; i += stride) v[0] += a[i] * b[i];
+  double t = 0.0;
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) t += v[u];
+  block_partial(t, part);
 }
 
-// MGS step j >= 2: u <- u - beta_{j-1} u_{j-1};  beta_j = (u, u_j)
+// MGS step j >= 2: u <- u - beta_{j-1} u_{j-1};  beta_j = (u, u_j).
+// beta_{j-1} is the sum of the previous step's np_prev partials.
 __global__ void __launch_bounds__(kRedThreads) k_mgs(double* u, const double* ujm1, const double* uj,
-                                                     const double* beta_prev, long long n,
-                                                     double* part, unsigned* count, double* out) {
-  const double beta = *beta_prev;
-  double v = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
+                                                     const double* part_prev, int np_prev,
+                                                     long long n, double* part) {
+  const double beta = block_sum_partials(part_prev, np_prev);
+  const bool self = uj == u;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  constexpr int U = kUnroll / 2;
+  double v[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) v[k] = 0.0;
+  for (; i + (U - 1) * stride < n; i += U * stride) {
+    double x[U], y[U], z[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      x[k] = u[i + k * stride];
+      y[k] = ujm1[i + k * stride];
+      z[k] = self ? 0.0 : uj[i + k * stride];
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const double w = x[k] - beta * y[k];
+      u[i + k * stride] = w;
+      v[k] += w * (self ? w : z[k]);
+    }
+  }
+  for (; i < n; i += stride) {
     const double w = u[i] - beta * ujm1[i];
     u[i] = w;
-    v += w * (uj == u ? w : uj[i]);
+    v[0] += w * (self ? w : uj[i]);
   }
-  block_reduce_and_finish(v, part, count, out);
+  double t = 0.0;
+#pragma unroll
+  for (int k = 0; k < U; ++k) t += v[k];
+  block_partial(t, part);
 }
 
 // u = src / sqrt(*sq); norm_out <- sqrt(*sq) (written by one lane)
@@ -321,24 +376,14 @@ __global__ void k_combine(double* dy, const double* basis, int k, Coef cf, long 
 }
 
 // stats: [sum res^2, sum rmes^2, nan/large counts of res, BDiag]
-__global__ void __launch_bounds__(kRedThreads) k_stat_sumsq(const double* a, long long n, double* part,
-                                                            unsigned* count, double* out) {
-  double v = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    v += a[i] * a[i];
-  block_reduce_and_finish(v, part, count, out);
-}
-
-__global__ void __launch_bounds__(kRedThreads) k_stat_bad(const double* a, long long n, double* part,
-                                                          unsigned* count, double* out) {
+__global__ void __launch_bounds__(kRedThreads) k_stat_bad(const double* a, long long n, double* part) {
   double v = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const double x = a[i];
     if (x != x || fabs(x) > 1.0e10) v += 1.0;
   }
-  block_reduce_and_finish(v, part, count, out);
+  block_partial(v, part);
 }
 
 // ---------------- layout conversion ----------------
@@ -503,17 +548,43 @@ int launch_bc3per(phg_ctx* c, double* v, int nq) {
   return 0;
 }
 
-int launch_dot(phg_ctx* c, const double* a, const double* b, long long n, double* out) {
-  k_dot<<<c->red_blocks, kRedThreads, 0, c->stream>>>(a, b, n, c->red_part, c->red_count, out);
+// blocks so that one unrolled pass covers n (at most red_blocks)
+inline int red_grid(const phg_ctx* c, long long n, int per_thread) {
+  const long long b = (n + (long long)kRedThreads * per_thread - 1) / ((long long)kRedThreads * per_thread);
+  return (int)std::max<long long>(1, std::min<long long>(b, c->red_blocks));
+}
+
+// partial channel k of the reduction scratch (red_blocks doubles each)
+inline double* red_chan(phg_ctx* c, int k) { return c->red_part + (size_t)k * c->red_blocks; }
+
+int launch_finish(phg_ctx* c, int chan0, int np0, int np, int nsums, double* out) {
+  k_finish<<<1, kRedThreads, 0, c->stream>>>(red_chan(c, chan0), np0, np, c->red_blocks, nsums,
+                                             out);
   PHG_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_mgs_step(phg_ctx* c, double* u, const double* ujm1, const double* uj,
-                    const double* beta_prev, long long n, double* out) {
-  k_mgs<<<c->red_blocks, kRedThreads, 0, c->stream>>>(u, ujm1, uj, beta_prev, n, c->red_part,
-                                                      c->red_count, out);
+int launch_dot_part(phg_ctx* c, const double* a, const double* b, long long n, int chan, int* np) {
+  const int g = red_grid(c, n, kUnroll);
+  k_dot<<<g, kRedThreads, 0, c->stream>>>(a, b, n, red_chan(c, chan));
   PHG_HIP(hipGetLastError());
+  *np = g;
+  return 0;
+}
+
+int launch_dot(phg_ctx* c, const double* a, const double* b, long long n, double* out) {
+  int np = 0;
+  PHG_TRY(launch_dot_part(c, a, b, n, 0, &np));
+  return launch_finish(c, 0, np, np, 1, out);
+}
+
+int launch_mgs_part(phg_ctx* c, double* u, const double* ujm1, const double* uj, int chan_prev,
+                    int np_prev, long long n, int chan, int* np) {
+  const int g = red_grid(c, n, kUnroll / 2);
+  k_mgs<<<g, kRedThreads, 0, c->stream>>>(u, ujm1, uj, red_chan(c, chan_prev), np_prev, n,
+                                          red_chan(c, chan));
+  PHG_HIP(hipGetLastError());
+  *np = g;
   return 0;
 }
 
@@ -542,14 +613,13 @@ int launch_combine(phg_ctx* c, double* dy, const double* basis, int k, const dou
 // out[0] = sum res^2, out[1] = bad entries of res, out[2] = bad entries of BDiag
 int launch_stats(phg_ctx* c, double* out) {
   const long long n5 = 5 * c->nshg;
-  k_stat_sumsq<<<c->red_blocks, kRedThreads, 0, c->stream>>>(c->res, n5, c->red_part,
-                                                             c->red_count, out);
-  k_stat_bad<<<c->red_blocks, kRedThreads, 0, c->stream>>>(c->res, n5, c->red_part, c->red_count,
-                                                           out + 1);
-  k_stat_bad<<<c->red_blocks, kRedThreads, 0, c->stream>>>(c->bdiag, 5 * n5, c->red_part,
-                                                           c->red_count, out + 2);
+  int np = 0;
+  PHG_TRY(launch_dot_part(c, c->res, c->res, n5, 0, &np));
+  PHG_TRY(launch_finish(c, 0, np, np, 1, out));
+  k_stat_bad<<<c->red_blocks, kRedThreads, 0, c->stream>>>(c->res, n5, red_chan(c, 1));
+  k_stat_bad<<<c->red_blocks, kRedThreads, 0, c->stream>>>(c->bdiag, 5 * n5, red_chan(c, 2));
   PHG_HIP(hipGetLastError());
-  return 0;
+  return launch_finish(c, 1, c->red_blocks, c->red_blocks, 2, out + 1);
 }
 
 int launch_csr_to_slots(phg_ctx* c, const double* ref, double* slots) {

@@ -216,29 +216,57 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   lanes.reserve(nnz + 4 * nshg);
   std::vector<int> blk_lane;  // blocks in lane order
   blk_lane.reserve(16 * nel);
-  std::vector<int> rowof(nnz);
-  for (long long r = 0; r < nshg; ++r)
-    for (long long k = m->colm[r] - 1; k < m->colm[r + 1] - 1; ++k) rowof[k] = (int)r;
-  for (long long k = 0; k < nnz; ++k) {
-    const int n = bcount[k + 1] - bcount[k];
-    if (n == 0) {
-      phg_set_error("CSR entry %lld has no element block", k);
-      return PHG_EINVAL;
+  // rows in Morton order of their node coordinates, so the lanes that run
+  // together touch a compact group of elements (cache reuse of the point
+  // states)
+  std::vector<int> rows(nshg);
+  {
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = hi[d] = m->x[d * nshg];
+      for (long long r = 0; r < nshg; ++r) {
+        lo[d] = std::min(lo[d], m->x[r + d * nshg]);
+        hi[d] = std::max(hi[d], m->x[r + d * nshg]);
+      }
     }
-    const int nch = (n + kChunk - 1) / kChunk;
-    if (nch > 64 || n >= (1 << 15)) {
-      phg_set_error("CSR entry %lld has %d element blocks", k, n);
-      return PHG_EINVAL;
+    std::vector<uint64_t> key(nshg);
+    for (long long r = 0; r < nshg; ++r) {
+      uint64_t k = 0;
+      uint32_t q[3];
+      for (int d = 0; d < 3; ++d) {
+        const double span = hi[d] > lo[d] ? hi[d] - lo[d] : 1.0;
+        q[d] = (uint32_t)std::min(1023.0, std::max(0.0, (m->x[r + d * nshg] - lo[d]) / span * 1023.0));
+      }
+      for (int b = 9; b >= 0; --b)
+        for (int d = 2; d >= 0; --d) k = (k << 1) | ((q[d] >> b) & 1u);
+      key[r] = k;
     }
-    if ((long long)(lanes.size() % 64) + nch > 64)
-      while (lanes.size() % 64) lanes.push_back(make_int4(0, 0, 0, 0));
-    const bool dg = (m->rowp[k] - 1) == rowof[k];
-    for (int ch = 0; ch < nch; ++ch) {
-      const int b0 = (int)((long long)n * ch / nch), b1 = (int)((long long)n * (ch + 1) / nch);
-      int flags = (dg ? 2 : 0) | (ch == 0 ? 1 : 0);
-      if (ch == 0) flags |= (nch << 8) | (n << 16);
-      lanes.push_back(make_int4((int)blk_lane.size(), (int)k, b1 - b0, flags));
-      for (int b = b0; b < b1; ++b) blk_lane.push_back(blocks[bcount[k] + b]);
+    for (long long r = 0; r < nshg; ++r) rows[r] = (int)r;
+    std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return key[a] < key[b]; });
+  }
+  for (long long ri = 0; ri < nshg; ++ri) {
+    const long long row = rows[ri];
+    for (long long k = m->colm[row] - 1; k < m->colm[row + 1] - 1; ++k) {
+      const int n = bcount[k + 1] - bcount[k];
+      if (n == 0) {
+        phg_set_error("CSR entry %lld has no element block", k);
+        return PHG_EINVAL;
+      }
+      const int nch = (n + kChunk - 1) / kChunk;
+      if (nch > 64 || n >= (1 << 15)) {
+        phg_set_error("CSR entry %lld has %d element blocks", k, n);
+        return PHG_EINVAL;
+      }
+      if ((long long)(lanes.size() % 64) + nch > 64)
+        while (lanes.size() % 64) lanes.push_back(make_int4(0, 0, 0, 0));
+      const bool dg = (m->rowp[k] - 1) == row;
+      for (int ch = 0; ch < nch; ++ch) {
+        const int b0 = (int)((long long)n * ch / nch), b1 = (int)((long long)n * (ch + 1) / nch);
+        int flags = (dg ? 2 : 0) | (ch == 0 ? 1 : 0);
+        if (ch == 0) flags |= (nch << 8) | (n << 16);
+        lanes.push_back(make_int4((int)blk_lane.size(), (int)k, b1 - b0, flags));
+        for (int b = b0; b < b1; ++b) blk_lane.push_back(blocks[bcount[k] + b]);
+      }
     }
   }
   c->nlanes = (long long)lanes.size();
@@ -323,8 +351,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   // K+1 basis vectors; at least 3 (the refinement loop uses 1 and 2 as scratch)
   PHG_TRY(dev_alloc(c, &c->krylov, (size_t)(std::max(K, 2) + 1) * 5 * nshg));
   c->red_blocks = 1024;
-  PHG_TRY(dev_alloc(c, &c->red_part, c->red_blocks));
-  PHG_TRY(dev_alloc(c, &c->red_count, 1));
+  PHG_TRY(dev_alloc(c, &c->red_part, (size_t)(K + 3) * c->red_blocks));
   PHG_TRY(dev_alloc(c, &c->scal, 256));
   PHG_HIP(hipHostMalloc((void**)&c->h_scal, 256 * sizeof(double), hipHostMallocDefault));
 
@@ -348,7 +375,6 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->scolp, scolp.data(), scolp.size()));
   PHG_TRY(upload(c, c->csr2slot, c2s.data(), c2s.size()));
   PHG_TRY(upload(c, c->slot2csr, s2c.data(), s2c.size()));
-  PHG_TRY(launch_fill(c, c->red_count, sizeof(unsigned)));
   if (mixed)
     PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
   else
@@ -416,10 +442,16 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
     PHG_HIP(hipEventRecord(c->kev[1], c->stream));
     PHG_TRY(launch_bc3per(c, w, 5));
     PHG_HIP(hipEventRecord(c->kev[2], c->stream));
-    PHG_TRY(launch_dot(c, w, kvec(c, 0), V, H + 0));
-    for (int jK = 2; jK <= iK + 1; ++jK)
-      PHG_TRY(launch_mgs_step(c, w, kvec(c, jK - 2), jK == iK + 1 ? w : kvec(c, jK - 1), H + jK - 2,
-                              V, H + jK - 1));
+    // H(1..iK+1, iK) by MGS: channel j-1 holds the partials of H(j, iK)
+    int np0 = 0, np = 0;
+    PHG_TRY(launch_dot_part(c, w, kvec(c, 0), V, 0, &np0));
+    int np_prev = np0;
+    for (int jK = 2; jK <= iK + 1; ++jK) {
+      PHG_TRY(launch_mgs_part(c, w, kvec(c, jK - 2), jK == iK + 1 ? w : kvec(c, jK - 1), jK - 2,
+                              np_prev, V, jK - 1, &np));
+      np_prev = np;
+    }
+    PHG_TRY(launch_finish(c, 0, np0, np, iK + 1, H));
     PHG_HIP(hipEventRecord(c->kev[3], c->stream));
     PHG_TRY(launch_scale(c, w, w, V, H + iK, H + iK + 1));
     PHG_TRY(download(c, c->h_scal + 16, H, iK + 2));
@@ -691,7 +723,7 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->BCv,      c->fold_master, c->fold_ptr, c->fold_slave, c->grp_slot,
                   c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
-                  c->acold,    c->dy,       c->krylov, c->red_part, c->red_count, c->scal,
+                  c->acold,    c->dy,       c->krylov, c->red_part, c->scal,
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
                   c->nodepack};
   for (void* p : ptrs)

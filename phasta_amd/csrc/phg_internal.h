@@ -168,8 +168,7 @@ struct phg_ctx {
 
   // reductions
   int red_blocks = 0;
-  double* red_part = nullptr;      // red_blocks * 8
-  unsigned* red_count = nullptr;
+  double* red_part = nullptr;      // (kcap + 3) channels of red_blocks partials
   double* scal = nullptr;          // device scalars (H column etc.)
   double* h_scal = nullptr;        // pinned mirror
   double reslast = 0.0;
@@ -220,11 +219,16 @@ int launch_widen(phg_ctx* c, double* dst, const float* src, long long n);
 // n doubles, n even, 16-byte aligned
 int launch_copy(phg_ctx* c, double* dst, const double* src, long long n);
 int launch_bc3per(phg_ctx* c, double* v, int nq);
+// reductions: per-block partials in channels of red_blocks doubles,
+// summed in a fixed order by launch_finish (np0 partials in the first
+// channel, np in the others)
 int launch_dot(phg_ctx* c, const double* a, const double* b, long long n,
                double* out);
-int launch_mgs_step(phg_ctx* c, double* u, const double* ujm1,
-                    const double* uj, const double* beta_prev, long long n,
-                    double* out);
+int launch_dot_part(phg_ctx* c, const double* a, const double* b, long long n,
+                    int chan, int* np);
+int launch_mgs_part(phg_ctx* c, double* u, const double* ujm1, const double* uj,
+                    int chan_prev, int np_prev, long long n, int chan, int* np);
+int launch_finish(phg_ctx* c, int chan0, int np0, int np, int nsums, double* out);
 int launch_scale(phg_ctx* c, double* u, const double* src, long long n,
                  const double* sq_ptr, double* norm_out);
 int launch_combine(phg_ctx* c, double* dy, const double* basis, int k,
