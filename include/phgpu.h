@@ -156,7 +156,9 @@ int phg_mpnn_param_count(void);
 int phg_mpnn_init(phg_ctx* ctx, const phg_mesh* mesh, const float* dscale);
 int phg_mpnn_set_weights(phg_ctx* ctx, const float* w, int count);
 int phg_mpnn_get_weights(phg_ctx* ctx, float* w, int count);
-/* use_pc: FGMRES with the network inside phg_solgmrs/phg_newton_iteration;
+/* use_pc > 0: flexible GMRES inside phg_solgmrs/phg_newton_iteration, the
+ * network preconditioning the first use_pc directions of each cycle (the
+ * block-diagonal preconditioner the others); 0: block-diagonal GMRES.
  * keep_pair: keep (BDiag before factorisation, b, z) of each solve. */
 int phg_mpnn_enable(phg_ctx* ctx, int use_pc, int keep_pair);
 /* z = network(v) for host arrays (bdiag_raw (nshg,5,5), v, z (nshg,5)). */

@@ -344,9 +344,11 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   if (l < A.nlanes) L = A.lanes[l];
   const bool diag = (L.w >> 1) & 1;
   const int cnt = (A.lhs_on || diag) ? L.z : 0;
-  double acc[25];
+  // block sums in the tangent's type (float accumulation on the mixed
+  // path); the chunks of one block are combined in double
+  R acc[25];
 #pragma unroll
-  for (int i = 0; i < 25; ++i) acc[i] = 0.0;
+  for (int i = 0; i < 25; ++i) acc[i] = (R)0;
   double racc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const S* qps = reinterpret_cast<const S*>(A.qps);
   const S* geo = reinterpret_cast<const S*>(A.geo);
@@ -365,6 +367,7 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     const R W = (R)go[12];
     const R st = sR[0] * rR[0] + sR[1] * rR[1] + sR[2] * rR[2];
     // acc += WdetJ [P Q + K_ab] per point; WdetJ is folded into P's rows
+    auto add = [&](int idx, R val) { acc[idx] += val; };
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
       const S* o = qps + ((size_t)e * 4 + q) * kThermo;
@@ -422,21 +425,22 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
         for (int n = 0; n < 5; ++n) Pr[n] *= W;
 #pragma unroll
         for (int j = 0; j < 5; ++j)
-          acc[m + 5 * j] += (double)(Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] +
-                                     Pr[3] * Q[3][j] + Pr[4] * Q[4][j]);
+          add(m + 5 * j, Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
+                             Pr[4] * Q[4][j]);
       }
       // viscous part K_ab
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int n = 0; n < 3; ++n)
-          acc[(1 + m) + 5 * (1 + n)] +=
-              (double)(W * (muR * ((m == n ? st : (R)0) + sR[n] * rR[m]) + lamR * sR[m] * rR[n]));
+          add((1 + m) + 5 * (1 + n),
+              W * (muR * ((m == n ? st : (R)0) + sR[n] * rR[m]) + lamR * sR[m] * rR[n]));
 #pragma unroll
       for (int n = 0; n < 3; ++n)
-        acc[4 + 5 * (1 + n)] += (double)(W * (muR * (st * uq[n] + sR[n] * unb) + lamR * una * rR[n]));
-      acc[24] += (double)(W * kapR * st);
+        add(4 + 5 * (1 + n), W * (muR * (st * uq[n] + sR[n] * unb) + lamR * una * rR[n]));
+      add(24, W * kapR * st);
     }
+
     if (diag) {
       const double* ro = A.resid + ((size_t)e * 4 + a) * 5;
 #pragma unroll
@@ -445,16 +449,19 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   }
   // combine the chunks of one block in lane order
 #pragma unroll
-  for (int i = 0; i < 25; ++i) stage[threadIdx.x][i] = acc[i];
+  for (int i = 0; i < 25; ++i) stage[threadIdx.x][i] = (double)acc[i];
 #pragma unroll
   for (int f = 0; f < 5; ++f) stage[threadIdx.x][25 + f] = racc[f];
   __syncthreads();
   if (!(L.w & 1) || l >= A.nlanes) return;
   const int nch = (L.w >> 8) & 0xff;
   const int total = L.w >> 16;
+  double sum[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) sum[i] = stage[threadIdx.x][i];
   for (int c = 1; c < nch; ++c) {
 #pragma unroll
-    for (int i = 0; i < 25; ++i) acc[i] += stage[threadIdx.x + c][i];
+    for (int i = 0; i < 25; ++i) sum[i] += stage[threadIdx.x + c][i];
 #pragma unroll
     for (int f = 0; f < 5; ++f) racc[f] += stage[threadIdx.x + c][25 + f];
   }
@@ -469,7 +476,7 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103)
     if (A.prec_on) {
 #pragma unroll
-      for (int i = 0; i < 25; ++i) A.bdiag[25 * (size_t)v + i] = acc[i];
+      for (int i = 0; i < 25; ++i) A.bdiag[25 * (size_t)v + i] = sum[i];
     }
   }
   if (!A.lhs_on) return;
@@ -481,13 +488,13 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     for (int j = 0; j < 5; ++j)
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        if (((ma >> i) & 1) || ((mb >> j) & 1)) acc[i + 5 * j] = 0.0;
-        if (diag && i == j && ((ma >> i) & 1)) acc[i + 5 * j] = (double)total;
+        if (((ma >> i) & 1) || ((mb >> j) & 1)) sum[i + 5 * j] = 0.0;
+        if (diag && i == j && ((ma >> i) & 1)) sum[i + 5 * j] = (double)total;
       }
   }
   T* dst = reinterpret_cast<T*>(A.lhs) + 25 * (size_t)L.y;
 #pragma unroll
-  for (int i = 0; i < 25; ++i) dst[i] = (T)acc[i];
+  for (int i = 0; i < 25; ++i) dst[i] = (T)sum[i];
 }
 
 // Wall faces: floating convective, pressure, viscous and heat fluxes

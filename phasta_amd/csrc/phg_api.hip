@@ -434,7 +434,13 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
     double* w = kvec(c, iK);
     const double* dir = kvec(c, iK - 1);
     if (flexible) {
-      PHG_TRY(mp_forward(c, kvec(c, iK - 1), zvec(iK - 1), false));
+      // the network preconditions the first use_pc directions of the cycle,
+      // the block-diagonal preconditioner (identity here) the rest
+      if (iK <= c->mp.use_pc)
+        PHG_TRY(mp_forward(c, kvec(c, iK - 1), zvec(iK - 1), false));
+      else
+        PHG_HIP(hipMemcpyAsync(zvec(iK - 1), kvec(c, iK - 1), sizeof(double) * V,
+                               hipMemcpyDeviceToDevice, c->stream));
       dir = zvec(iK - 1);
     }
     PHG_HIP(hipEventRecord(c->kev[0], c->stream));
@@ -525,7 +531,7 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   double* S = c->scal;  // [0] sum res^2 (after forward), [1],[2] bad counts, [3] unorm, [4] sum rmes^2
   // tnanq on res and BDiag before factorisation (solgmr.f:459-460)
   PHG_TRY(launch_stats(c, S));  // S[0] provisional, overwritten below
-  const bool learned = c->mp.ready && c->mp.use_pc;
+  const bool learned = c->mp.ready && c->mp.use_pc > 0;
   if ((learned || c->mp.keep_pair) && c->mp.ready)
     PHG_HIP(hipMemcpyAsync(c->mp.bd_raw, c->bdiag, sizeof(double) * 25 * n,
                            hipMemcpyDeviceToDevice, c->stream));
@@ -1001,7 +1007,7 @@ extern "C" int phg_mpnn_get_weights(phg_ctx* c, float* w, int count) {
 
 extern "C" int phg_mpnn_enable(phg_ctx* c, int use_pc, int keep_pair) {
   PHG_TRY(mp_check(c));
-  c->mp.use_pc = use_pc != 0;
+  c->mp.use_pc = use_pc < 0 ? 0 : use_pc;
   c->mp.keep_pair = keep_pair != 0;
   return 0;
 }

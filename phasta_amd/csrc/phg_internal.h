@@ -56,7 +56,8 @@ __host__ __device__ inline uint8_t constraint_mask(int ibc) {
 // Learned preconditioner state (k_mpnn.hip); allocated by phg_mpnn_init.
 struct MpNet {
   bool ready = false;
-  bool use_pc = false;       // FGMRES with the network inside SolGMRs
+  int use_pc = 0;            // FGMRES: directions per cycle preconditioned
+                             // by the network (0: block-diagonal GMRES)
   bool keep_pair = false;    // keep (raw BDiag, b, z) of the last solve
   long long n_act = 0, nedge = 0;
   int* ptr = nullptr;        // periodic node graph, CSR over all nodes

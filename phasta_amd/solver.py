@@ -272,7 +272,9 @@ class GpuSolver:
         _check(lib().phg_mpnn_get_weights(self.h, w.ctypes.data_as(ctypes.c_void_p), len(w)))
         return w
 
-    def mpnn_enable(self, use_pc=True, keep_pair=False):
+    def mpnn_enable(self, use_pc=1, keep_pair=False):
+        """use_pc: directions per GMRES cycle preconditioned by the network
+        (0 off; a value >= Kspace: every direction)."""
         _check(lib().phg_mpnn_enable(self.h, int(use_pc), int(keep_pair)))
 
     def mpnn_apply(self, bdiag_raw, v):

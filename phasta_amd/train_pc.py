@@ -9,6 +9,7 @@ right-hand side b = L^-1 res, the BDiag before factorisation and the scaled
 solution z.  The network learns z from (BDiag, b).
 
     python -m phasta_amd.train_pc --out phasta_amd/weights/mpnn_box64.npz
+    python -m phasta_amd.train_pc --eval phasta_amd/weights/mpnn_box64.npz
 """
 import argparse
 import json
@@ -105,16 +106,16 @@ def evaluate(N, weights, seed, states=4):
         fseed = int(rng.integers(1, 2 ** 31))
         gpu.set_params(**config.solver_params(N, mach, 20))
         res = {}
-        for arm in ("plain", "learned"):
-            gpu.mpnn_enable(use_pc=arm == "learned", keep_pair=False)
+        for arm, dirs in (("plain", 0), ("first", 1), ("all", 20)):
+            gpu.mpnn_enable(use_pc=dirs, keep_pair=False)
             gpu.upload_state(mesh.box_state(case, mach, fseed))
             gpu.predict()
             st = gpu.newton_iteration()
             res[arm] = (st["iKs"], st["pc_fallback"], st["ms_solve"], bool(st["converged"]))
         rows.append((mach, res))
-        log(f"eval mach {mach:.2f}: plain its {res['plain'][0]} ({res['plain'][2]:.2f} ms), "
-            f"learned its {res['learned'][0]} fallback {res['learned'][1]} "
-            f"({res['learned'][2]:.2f} ms) converged {res['learned'][3]}")
+        log(f"eval mach {mach:.2f}: " + ", ".join(
+            f"{a} its {r[0]} fallback {r[1]} ({r[2]:.2f} ms, converged {r[3]})"
+            for a, r in res.items()))
     gpu.close()
     return rows
 
@@ -128,8 +129,14 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", help="weights file to write")
+    ap.add_argument("--eval", help="only evaluate the weights in this file")
     args = ap.parse_args()
+    if args.eval:
+        evaluate(args.N, np.load(args.eval)["weights"], args.seed)
+        return
+    if not args.out:
+        ap.error("--out is required for training")
     t0 = time.perf_counter()
     ctxs, slots = harvest(args.N, args.meshes, args.pairs, args.newton, args.seed)
     log(f"harvested {len(slots)} pairs in {time.perf_counter() - t0:.1f} s")

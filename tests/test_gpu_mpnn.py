@@ -89,7 +89,7 @@ def test_zero_network_fgmres_equals_block_diagonal_gmres():
     try:
         gpu.mpnn_init(np.ones(25, dtype=np.float32))
         gpu.mpnn_set_weights(np.zeros(solver.lib().phg_mpnn_param_count(), np.float32))
-        gpu.mpnn_enable(use_pc=True, keep_pair=True)
+        gpu.mpnn_enable(use_pc=10, keep_pair=True)  # every direction of the cycle
         out = gpu.solgmrs(y, ac)
         bd, b, z = gpu.mpnn_last_pair()
     finally:
