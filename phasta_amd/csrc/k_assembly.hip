@@ -44,8 +44,34 @@ struct Quad {
 
 __constant__ Quad cQ;
 
-constexpr int kThermo = 13;   // point state: u(3) rho drdp drdT e1p..e4p tau(3)
-constexpr int kGeo = 13;      // shape gradients (4 x 3) and WdetJ
+// Records of 13 values (point state: u(3) rho drdp drdT e1p..e4p tau(3);
+// geometry: shape gradients (4 x 3) and WdetJ), padded so that a record is
+// read with 16-byte loads: 14 doubles or 16 floats.
+template <typename S>
+constexpr int rec_stride() { return sizeof(S) == 8 ? 14 : 16; }
+
+template <typename S>
+__device__ inline void load_rec(const S* p, S v[16]) {
+  if constexpr (sizeof(S) == 8) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const double2 t = q[i];
+      v[2 * i] = t.x;
+      v[2 * i + 1] = t.y;
+    }
+  } else {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 t = q[i];
+      v[4 * i] = t.x;
+      v[4 * i + 1] = t.y;
+      v[4 * i + 2] = t.z;
+      v[4 * i + 3] = t.w;
+    }
+  }
+}
 
 struct QpArgs {
   const int* ien;          // 4 planes of nel (0-based)
@@ -171,7 +197,7 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   const double GG = G[0] * G[0] + G[2] * G[2] + G[5] * G[5] +
                     2.0 * (G[1] * G[1] + G[3] * G[3] + G[4] * G[4]);
   if (valid && q == 0) {
-    S* go = reinterpret_cast<S*>(A.geo) + (size_t)e * kGeo;
+    S* go = reinterpret_cast<S*>(A.geo) + (size_t)e * rec_stride<S>();
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -235,7 +261,7 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   t.tau[1] = 1.0 / sq;
   t.tau[2] = t.tau[1] / cv * ph.temper;
   if (valid) {
-    S* o = reinterpret_cast<S*>(A.qps) + ((size_t)e * 4 + q) * kThermo;
+    S* o = reinterpret_cast<S*>(A.qps) + ((size_t)e * 4 + q) * rec_stride<S>();
     o[0] = (S)t.u[0];
     o[1] = (S)t.u[1];
     o[2] = (S)t.u[2];
@@ -357,7 +383,8 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     const int code = A.blocks[L.x + k];
     const long long e = code >> 4;
     const int a = (code >> 2) & 3, b = code & 3;
-    const S* go = geo + (size_t)e * kGeo;
+    S go[16];
+    load_rec<S>(geo + (size_t)e * rec_stride<S>(), go);
     R sR[3], rR[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -370,7 +397,8 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     auto add = [&](int idx, R val) { acc[idx] += val; };
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
-      const S* o = qps + ((size_t)e * 4 + q) * kThermo;
+      S o[16];
+      load_rec<S>(qps + ((size_t)e * 4 + q) * rec_stride<S>(), o);
       const R u0 = (R)o[0], u1 = (R)o[1], u2 = (R)o[2];
       const R uq[3] = {u0, u1, u2};
       const R rho = (R)o[3], drdp = (R)o[4], drdT = (R)o[5];

@@ -171,8 +171,7 @@ __global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const 
   if (row >= n) return;
   Acc t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
   const long long s0 = grp_slot[g], s1 = grp_slot[g + 1];
-  for (long long s = s0; s < s1; ++s) {
-    const long long j = scolp[s * kGroup + r];
+  auto slot = [&](long long s, long long j) {
     const Acc p0 = (Acc)p[j], p1 = (Acc)p[j + n], p2 = (Acc)p[j + 2 * n], p3 = (Acc)p[j + 3 * n],
               p4 = (Acc)p[j + 4 * n];
     const T* blk = vals + s * (25 * kGroup) + r;
@@ -184,6 +183,15 @@ __global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const 
     t2 += a[2] * p0 + a[7] * p1 + a[12] * p2 + a[17] * p3 + a[22] * p4;
     t3 += a[3] * p0 + a[8] * p1 + a[13] * p2 + a[18] * p3 + a[23] * p4;
     t4 += a[4] * p0 + a[9] * p1 + a[14] * p2 + a[19] * p3 + a[24] * p4;
+  };
+  // the column of the next slot is loaded one slot ahead, so the gathers of
+  // p never wait on the index load of their own slot
+  long long s = s0;
+  long long jn = s < s1 ? scolp[s * kGroup + r] : 0;
+  for (; s < s1; ++s) {
+    const long long j = jn;
+    if (s + 1 < s1) jn = scolp[(s + 1) * kGroup + r];
+    slot(s, j);
   }
   q[row] = (double)t0;
   q[row + n] = (double)t1;

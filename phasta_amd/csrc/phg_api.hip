@@ -309,12 +309,14 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->ienc, 4 * nel));
   PHG_TRY(dev_alloc(c, &c->lanes, c->nlanes));
   PHG_TRY(dev_alloc(c, &c->blocks, 16 * nel));
+  // point-state and geometry records: 13 values padded to 16 floats or 14
+  // doubles (16-byte loads)
   if (mixed) {
-    PHG_TRY(dev_alloc(c, &c->qps32, (size_t)nel * 4 * 13));
-    PHG_TRY(dev_alloc(c, &c->geo32, (size_t)nel * 13));
+    PHG_TRY(dev_alloc(c, &c->qps32, (size_t)nel * 4 * 16));
+    PHG_TRY(dev_alloc(c, &c->geo32, (size_t)nel * 16));
   } else {
-    PHG_TRY(dev_alloc(c, &c->qps, (size_t)nel * 4 * 13));
-    PHG_TRY(dev_alloc(c, &c->geo, (size_t)nel * 13));
+    PHG_TRY(dev_alloc(c, &c->qps, (size_t)nel * 4 * 14));
+    PHG_TRY(dev_alloc(c, &c->geo, (size_t)nel * 14));
   }
   PHG_TRY(dev_alloc(c, &c->resid, (size_t)nel * 4 * 5));
   PHG_TRY(dev_alloc(c, &c->ienbc, ienbc.size()));

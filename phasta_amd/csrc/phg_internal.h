@@ -119,10 +119,10 @@ struct phg_ctx {
   long long nlanes = 0;
   int4* lanes = nullptr;        // block-list offset, CSR entry, count, flags
   int* blocks = nullptr;        // e*16 + a*4 + b, grouped per owner lane
-  double* qps = nullptr;        // (nel*4) x 13 point states (FP64 path)
-  float* qps32 = nullptr;       // same, mixed path
-  double* geo = nullptr;        // nel x 13: shape gradients, WdetJ
-  float* geo32 = nullptr;
+  double* qps = nullptr;        // (nel*4) x 14 point states (13 + pad, FP64 path)
+  float* qps32 = nullptr;       // (nel*4) x 16, mixed path
+  double* geo = nullptr;        // nel x 14: shape gradients, WdetJ (+ pad)
+  float* geo32 = nullptr;       // nel x 16
   double* resid = nullptr;      // (nel*4) x 5 element residual per node
   int* ienbc = nullptr;         // 4 planes of nelb, colour order, 0-based
   uint8_t* bcm = nullptr;       // constraint mask per node
