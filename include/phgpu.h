@@ -126,6 +126,10 @@ int phg_update(phg_ctx* ctx);
 /* Fine-grained entry points (unit parity; host arrays in/out). */
 /* SparseAp(iper,ilwork,iBC,col,row,lhsK,p)          sparseap.f:1     */
 int phg_sparse_ap(phg_ctx* ctx, const double* lhsK, double* p);
+/* Transpose of SparseAp's operator, p <- K^T p with K p = bc3per(A p(iper))
+ * (no counterpart in the reference: the learned preconditioner's training
+ * differentiates through the matvec). */
+int phg_sparse_ap_t(phg_ctx* ctx, const double* lhsK, double* p);
 /* i3LU(Diag,r,code), code 'LU_Fact ','forward ','backward' i3lu.f:1   */
 int phg_i3lu(phg_ctx* ctx, double* Diag, double* r, const char* code);
 /* Spsi3pre(BDiag,lhsK,col,row)                      spsi3pre.f:1     */
@@ -180,6 +184,11 @@ int phg_mpnn_dataset(phg_ctx* ctx, int capacity);
 int phg_mpnn_store_pair(phg_ctx* ctx, int slot);
 /* One training step (Adam when lr > 0) on a stored pair; loss nullable. */
 int phg_mpnn_train_pair(phg_ctx* ctx, int slot, float lr, double* loss);
+
+/* Training on the residual of preconditioned directions: one Adam step per
+ * Krylov vector v_1..v_nvec of the last solve (keep_pair enabled), loss
+ * ||K z(v) - v||^2 / ||v||^2 with K the scaled matrix of that solve. */
+int phg_mpnn_train_krylov(phg_ctx* ctx, int nvec, float lr, double* mean_loss);
 
 /* Wait for all work queued on the library stream. */
 int phg_sync(phg_ctx* ctx);

@@ -432,6 +432,43 @@ __global__ void k_mp_out_bwd(const float* __restrict__ Y, const float* __restric
   }
 }
 
+// dY = scale * dz / s on active rows (dz: 5 planes), scale = -2 / ||v||^2
+// for the residual loss ||K z - v||^2 / ||v||^2 with dz = K^T (v - K z)
+__global__ void k_mp_dy_from_dz(const double* __restrict__ dz, const double* __restrict__ sbuf,
+                                const int* __restrict__ ptr, float* __restrict__ dY, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool act = ptr[i + 1] > ptr[i];
+  const double vv = sbuf[0], sc = sbuf[1];
+  const double f = (vv > 0.0 && sc > 0.0) ? -2.0 / (vv * sc) : 0.0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) dY[i * 5 + k] = act ? (float)(f * dz[(size_t)k * n + i]) : 0.f;
+}
+
+// dh = dY Wout^T (given dY)
+__global__ void k_mp_dh_from_dy(const float* __restrict__ dY, const int* __restrict__ ptr,
+                                const float* __restrict__ Wout, float* __restrict__ dh, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool act = ptr[i + 1] > ptr[i];
+  float d[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) d[f] = dY[i * 5 + f];
+  for (int k = 0; k < kH; ++k) {
+    float a = 0.f;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) a += d[f] * Wout[k * 5 + f];
+    dh[(size_t)i * kH + k] = act ? a : 0.f;
+  }
+}
+
+// loss = ||r||^2 / ||v||^2 (r: sbuf[3] holds ||r||^2) accumulated into sbuf[4]
+__global__ void k_mp_res_loss(double* sbuf) {
+  const double l = sbuf[0] > 0.0 ? sbuf[3] / sbuf[0] : 0.0;
+  sbuf[2] = l;
+  sbuf[4] += l;
+}
+
 // dWout[k][f] = sum_i h[i][k] dY[i][f]; dbout[f] = sum_i dY[i][f]  (one block
 // per k, k = 64 for the bias)
 __global__ void k_mp_out_grad_w(const uint16_t* __restrict__ h, const float* __restrict__ dY,
@@ -694,6 +731,8 @@ int mp_init(phg_ctx* c, const phg_mesh* m, const float* dscale) {
   PHG_TRY(mp_alloc(c, &M.sbuf, 8));
   PHG_TRY(mp_alloc(c, &M.vin, 5 * n));
   PHG_TRY(mp_alloc(c, &M.zout, 5 * n));
+  PHG_TRY(mp_alloc(c, &M.rbuf, 5 * n));
+  PHG_TRY(mp_alloc(c, &M.gbuf, 5 * n));
   PHG_TRY(mp_alloc(c, &M.zbasis, (size_t)std::max(c->kcap, 1) * 5 * n));
   PHG_HIP(hipMemcpyAsync(M.ptr, ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
   PHG_HIP(hipMemcpyAsync(M.col, col.data(), sizeof(int) * ne, hipMemcpyHostToDevice, c->stream));
@@ -743,7 +782,8 @@ int mp_free(phg_ctx* c) {
   void* ptrs[] = {M.ptr,  M.col,  M.rev,  M.ev,   M.esum,  M.inv,   M.dscale, M.bd_raw, M.zhat,
                   M.x,    M.h[0], M.h[1], M.h[2], M.h[3],  M.A[0],  M.A[1],   M.A[2],   M.U[0],
                   M.U[1], M.U[2], M.U[3], M.Y,    M.Ystar, M.dh,    M.dU,     M.dA,     M.dY,
-                  M.sq,   M.part, M.sbuf, M.vin,  M.zout,  M.zbasis, M.ds};
+                  M.sq,   M.part, M.sbuf, M.vin,  M.zout,  M.zbasis, M.ds,
+                  M.rbuf, M.gbuf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (!M.param_owner) {
@@ -870,23 +910,29 @@ int mp_forward(phg_ctx* c, const double* v, double* z, bool train) {
   return 0;
 }
 
-int mp_backward(phg_ctx* c, float lr, double* loss_out) {
+int mp_backward(phg_ctx* c, float lr, double* loss_out, bool given_dy) {
   MpNet& M = c->mp;
   if (!M.train_ready) {
     phg_set_error("learned preconditioner: training buffers not allocated");
     return PHG_EINVAL;
   }
   const long long n = c->nshg;
-  const float inv_na = 1.0f / (float)std::max<long long>(1, M.n_act);
-  k_mp_out_bwd<<<grid_for(n, 256), 256, 0, c->stream>>>(M.Y, M.Ystar, M.ptr, M.w + Layout::Wo,
-                                                        inv_na, M.dY, M.dh, M.sq, n);
-  PHG_HIP(hipGetLastError());
+  if (given_dy) {
+    k_mp_dh_from_dy<<<grid_for(n, 256), 256, 0, c->stream>>>(M.dY, M.ptr, M.w + Layout::Wo, M.dh,
+                                                             n);
+    PHG_HIP(hipGetLastError());
+  } else {
+    const float inv_na = 1.0f / (float)std::max<long long>(1, M.n_act);
+    k_mp_out_bwd<<<grid_for(n, 256), 256, 0, c->stream>>>(M.Y, M.Ystar, M.ptr, M.w + Layout::Wo,
+                                                          inv_na, M.dY, M.dh, M.sq, n);
+    PHG_HIP(hipGetLastError());
+    // loss: fixed-order float sum of the per-node squared errors
+    k_mp_sum<<<1, 256, 0, c->stream>>>(M.sq, n, M.part);
+    k_mp_loss<<<1, 1, 0, c->stream>>>(M.part, 1.0 / (double)std::max<long long>(1, M.n_act),
+                                      M.sbuf + 2);
+    PHG_HIP(hipGetLastError());
+  }
   k_mp_out_grad_w<<<kH + 1, 256, 0, c->stream>>>(M.h[3], M.dY, n, M.g + Layout::Wo);
-  PHG_HIP(hipGetLastError());
-  // loss: fixed-order float sum of the per-node squared errors
-  k_mp_sum<<<1, 256, 0, c->stream>>>(M.sq, n, M.part);
-  k_mp_loss<<<1, 1, 0, c->stream>>>(M.part, 1.0 / (double)std::max<long long>(1, M.n_act),
-                                    M.sbuf + 2);
   PHG_HIP(hipGetLastError());
   const int gd = grid_for(n, 64);
   const long long nH = n * kH;
@@ -934,4 +980,23 @@ int mp_targets(phg_ctx* c, const double* t, const double* v) {
   k_mp_target<<<grid_for(n, 256), 256, 0, c->stream>>>(t, v, M.sbuf + 1, M.ptr, M.Ystar, n);
   PHG_HIP(hipGetLastError());
   return 0;
+}
+
+// One training step on the residual of the preconditioned direction:
+// z = v + y(v)/s, loss = ||K z - v||^2 / ||v||^2 with K the scaled matrix
+// of the last solve, gradient dz = 2 K^T (K z - v) / ||v||^2.
+int mp_train_residual(phg_ctx* c, const double* v, float lr) {
+  MpNet& M = c->mp;
+  const long long n = c->nshg, V = 5 * n;
+  PHG_TRY(mp_forward(c, v, M.zout, true));              // sbuf[0] = ||v||^2, sbuf[1] = s
+  PHG_TRY(launch_spmv(c, M.zout, M.rbuf, true));
+  PHG_TRY(launch_bc3per(c, M.rbuf, 5));
+  PHG_TRY(launch_sub(c, M.gbuf, v, M.rbuf, V));         // gbuf = v - K z
+  PHG_TRY(launch_dot(c, M.gbuf, M.gbuf, V, M.sbuf + 3));
+  k_mp_res_loss<<<1, 1, 0, c->stream>>>(M.sbuf);
+  PHG_HIP(hipGetLastError());
+  PHG_TRY(launch_spmv_t(c, M.gbuf, M.rbuf));            // rbuf = K^T (v - K z)
+  k_mp_dy_from_dz<<<grid_for(n, 256), 256, 0, c->stream>>>(M.rbuf, M.sbuf, M.ptr, M.dY, n);
+  PHG_HIP(hipGetLastError());
+  return mp_backward(c, lr, nullptr, true);
 }

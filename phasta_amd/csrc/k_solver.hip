@@ -200,6 +200,52 @@ __global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const 
   q[row + 4 * n] = (double)t4;
 }
 
+// Transpose matvec without the periodic maps: q_j = sum_i B_ij^T x_i over
+// the blocks of column j, B_ij read at the slot tslot of (i, j); scol gives
+// the raw columns i of row j (the graph is structurally symmetric).
+template <typename T>
+__global__ void __launch_bounds__(256) k_spmv_t(const T* __restrict__ vals, const int* __restrict__ scol,
+                                                const uint32_t* __restrict__ tslot,
+                                                const long long* __restrict__ grp_slot,
+                                                const double* __restrict__ x, double* __restrict__ q,
+                                                long long n) {
+  const long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long g = row / kGroup;
+  const int r = (int)(row % kGroup);
+  if (row >= n) return;
+  double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (long long s = grp_slot[g]; s < grp_slot[g + 1]; ++s) {
+    const long long i = scol[s * kGroup + r];
+    const uint32_t ts = tslot[s * kGroup + r];
+    const T* blk = vals + (size_t)(ts >> 6) * (25 * kGroup) + (ts & 63);
+    double xi[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) xi[f] = x[i + f * n];
+#pragma unroll
+    for (int gg = 0; gg < 5; ++gg) {
+      double a = 0.0;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) a += (double)blk[(f + 5 * gg) * kGroup] * xi[f];
+      t[gg] += a;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 5; ++f) q[row + f * n] = t[f];
+}
+
+// slaves take their master's value (the transpose of the periodic fold)
+__global__ void k_per_copy_vec(double* v, const int* master, const int* ptr, const int* slave,
+                               int nfold, long long n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfold) return;
+  const long long m = master[i];
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const long long s = slave[k];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) v[s + f * n] = v[m + f * n];
+  }
+}
+
 // r = b - q (true residual of the refinement loop)
 __global__ void k_sub(double* r, const double* b, const double* q, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -523,6 +569,30 @@ int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact) {
                                                      c->nshg);
   PHG_HIP(hipGetLastError());
   return 0;
+}
+
+// q = K^T x for the periodic-folded operator K = F A G of the matvec:
+// K^T = F A^T G (G copies masters to slaves, F folds slaves into masters).
+// x is overwritten by G x.
+int launch_spmv_t(phg_ctx* c, double* x, double* q) {
+  if (!c->tslot) {
+    phg_set_error("transpose matvec: slot map not built");
+    return PHG_EINVAL;
+  }
+  if (c->nfold > 0) {
+    k_per_copy_vec<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(x, c->fold_master, c->fold_ptr,
+                                                                  c->fold_slave, c->nfold, c->nshg);
+    PHG_HIP(hipGetLastError());
+  }
+  const int g = grid_for(c->nshg, 256);
+  if (c->precision == PHG_MIXED)
+    k_spmv_t<float><<<g, 256, 0, c->stream>>>(c->lhs32, c->scol, c->tslot, c->grp_slot, x, q,
+                                              c->nshg);
+  else
+    k_spmv_t<double><<<g, 256, 0, c->stream>>>(c->lhs64, c->scol, c->tslot, c->grp_slot, x, q,
+                                               c->nshg);
+  PHG_HIP(hipGetLastError());
+  return launch_bc3per(c, q, 5);
 }
 
 int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n) {

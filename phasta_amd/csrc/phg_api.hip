@@ -168,6 +168,34 @@ int setup(phg_ctx* c, const phg_mesh* m) {
     }
   }
 
+  // ---- slot of the transposed block (i, j) -> (j, i) of every slot
+  // position (transpose matvec); padding positions point to themselves ----
+  std::vector<uint32_t> tsl(c->nslots * kGroup);
+  for (long long q = 0; q < c->nslots * kGroup; ++q) tsl[q] = (uint32_t)q;
+  {
+    int bad_t = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad_t)
+    for (long long i = 0; i < nshg; ++i) {
+      const long long base = m->colm[i] - 1, len = m->colm[i + 1] - m->colm[i];
+      for (long long t = 0; t < len; ++t) {
+        const long long j = m->rowp[base + t] - 1;
+        const int* lst = m->rowp + (m->colm[j] - 1);
+        const int lenj = m->colm[j + 1] - m->colm[j];
+        const int* f = std::lower_bound(lst, lst + lenj, (int)(i + 1));
+        if (f == lst + lenj || *f != i + 1) {
+          bad_t = 1;
+          continue;
+        }
+        const long long qj = (c->h_grp_slot[j / kGroup] + (f - lst)) * kGroup + j % kGroup;
+        tsl[(c->h_grp_slot[i / kGroup] + t) * kGroup + i % kGroup] = (uint32_t)qj;
+      }
+    }
+    if (bad_t) {
+      phg_set_error("colm/rowp is not structurally symmetric");
+      return PHG_EINVAL;
+    }
+  }
+
   // ---- element -> block map (replaces the per-entry sparseloc search) ----
   std::vector<int> ienc(4 * nel);
   std::vector<uint32_t> bmap(16 * nel);
@@ -329,6 +357,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->grp_slot, c->ngroups + 1));
   PHG_TRY(dev_alloc(c, &c->scol, c->nslots * kGroup));
   PHG_TRY(dev_alloc(c, &c->scolp, c->nslots * kGroup));
+  PHG_TRY(dev_alloc(c, &c->tslot, c->nslots * kGroup));
   PHG_TRY(dev_alloc(c, &c->csr2slot, nnz));
   PHG_TRY(dev_alloc(c, &c->slot2csr, c->nslots * kGroup));
   if (mixed)
@@ -375,6 +404,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->grp_slot, c->h_grp_slot.data(), c->h_grp_slot.size()));
   PHG_TRY(upload(c, c->scol, scol.data(), scol.size()));
   PHG_TRY(upload(c, c->scolp, scolp.data(), scolp.size()));
+  PHG_TRY(upload(c, c->tslot, tsl.data(), tsl.size()));
   PHG_TRY(upload(c, c->csr2slot, c2s.data(), c2s.size()));
   PHG_TRY(upload(c, c->slot2csr, s2c.data(), s2c.size()));
   if (mixed)
@@ -727,7 +757,7 @@ extern "C" int phg_finalize(phg_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   mp_free(c);
   void* ptrs[] = {c->x,        c->ienc,     c->lanes,  c->blocks, c->qps, c->qps32, c->geo,
-                  c->geo32,    c->resid,    c->ienbc,  c->bcm,       c->iper,
+                  c->geo32,    c->resid,    c->tslot,    c->ienbc,  c->bcm,       c->iper,
                   c->BCv,      c->fold_master, c->fold_ptr, c->fold_slave, c->grp_slot,
                   c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
@@ -872,6 +902,22 @@ extern "C" int phg_sparse_ap(phg_ctx* c, const double* lhsK, double* p) {
   }
   PHG_TRY(upload(c, kvec(c, 0), p, 5 * n));
   PHG_TRY(launch_spmv(c, kvec(c, 0), kvec(c, 1)));
+  PHG_TRY(download(c, p, kvec(c, 1), 5 * n));
+  return sync(c);
+}
+
+extern "C" int phg_sparse_ap_t(phg_ctx* c, const double* lhsK, double* p) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(require_fp64(c, "phg_sparse_ap_t"));
+  const long long n = c->nshg;
+  if (lhsK) {
+    PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
+    PHG_TRY(upload(c, c->tmp, lhsK, 25 * c->nnz));
+    PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
+    PHG_TRY(launch_csr_to_slots(c, c->tmp, c->lhs64));
+  }
+  PHG_TRY(upload(c, kvec(c, 0), p, 5 * n));
+  PHG_TRY(launch_spmv_t(c, kvec(c, 0), kvec(c, 1)));
   PHG_TRY(download(c, p, kvec(c, 1), 5 * n));
   return sync(c);
 }
@@ -1034,7 +1080,7 @@ extern "C" int phg_mpnn_train_step(phg_ctx* c, const double* bdiag_raw, const do
   PHG_TRY(upload(c, c->mp.zout, t, 5 * n));
   PHG_TRY(mp_forward(c, c->mp.vin, nullptr, true));
   PHG_TRY(mp_targets(c, c->mp.zout, c->mp.vin));
-  PHG_TRY(mp_backward(c, lr, loss));
+  PHG_TRY(mp_backward(c, lr, loss, false));
   if (grad)
     PHG_HIP(hipMemcpyAsync(grad, c->mp.g, sizeof(float) * mp_param_count(), hipMemcpyDeviceToHost,
                            c->stream));
@@ -1069,8 +1115,30 @@ extern "C" int phg_mpnn_train_pair(phg_ctx* c, int slot, float lr, double* loss)
   PHG_TRY(mp_load_pair(c, slot));
   PHG_TRY(mp_forward(c, c->mp.vin, nullptr, true));
   PHG_TRY(mp_targets(c, c->mp.zout, c->mp.vin));
-  PHG_TRY(mp_backward(c, lr, loss));
+  PHG_TRY(mp_backward(c, lr, loss, false));
   return 0;
+}
+
+extern "C" int phg_mpnn_train_krylov(phg_ctx* c, int nvec, float lr, double* mean_loss) {
+  PHG_TRY(mp_check(c));
+  if (!c->mp.keep_pair) {
+    phg_set_error("phg_mpnn_train_krylov: enable keep_pair before the solve");
+    return PHG_EINVAL;
+  }
+  if (nvec < 1 || nvec > std::max(c->kcap, 2) + 1) {
+    phg_set_error("phg_mpnn_train_krylov: nvec %d outside 1..%d", nvec, std::max(c->kcap, 2) + 1);
+    return PHG_EINVAL;
+  }
+  PHG_TRY(mp_alloc_train(c));
+  PHG_TRY(launch_fill(c, c->mp.sbuf + 4, sizeof(double)));
+  for (int k = 0; k < nvec; ++k) PHG_TRY(mp_train_residual(c, kvec(c, k), lr));
+  if (mean_loss) {
+    double s4 = 0.0;
+    PHG_HIP(hipMemcpyAsync(&s4, c->mp.sbuf + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHG_TRY(sync(c));
+    *mean_loss = s4 / nvec;
+  }
+  return sync(c);
 }
 
 extern "C" int phg_mpnn_last_pair(phg_ctx* c, double* bdiag_raw, double* b, double* zhat) {

@@ -89,9 +89,11 @@ struct MpNet {
   float* m2 = nullptr;
   uint16_t* wt[4] = {};      // bf16 weights, Wt[c][k]
   uint16_t* wr[4] = {};      // bf16 weights, W[k][c]
-  double* sbuf = nullptr;    // [0] ||v||^2, [1] s, [2] loss
+  double* sbuf = nullptr;    // [0] ||v||^2, [1] s, [2] loss, [3] ||r||^2, [4] loss sum
   double* vin = nullptr;     // 5 planes, host-array entry points
   double* zout = nullptr;    // 5 planes
+  double* rbuf = nullptr;    // 5 planes (residual training)
+  double* gbuf = nullptr;    // 5 planes
   double* zbasis = nullptr;  // FGMRES preconditioned directions (K vectors)
   int adam_t = 0;
   bool train_ready = false;
@@ -142,6 +144,8 @@ struct phg_ctx {
   long long* grp_slot = nullptr;   // ngroups+1
   int* scol = nullptr;             // nslots*64 raw column (0-based)
   int* scolp = nullptr;            // nslots*64 periodic-master column
+  uint32_t* tslot = nullptr;       // nslots*64 slot of the transposed block
+                                   // (built on demand for training)
   uint32_t* csr2slot = nullptr;    // nnz -> slot-row index
   int* slot2csr = nullptr;         // slot-row -> CSR block index, -1 = padding
   double* lhs64 = nullptr;         // nslots*64*25, scaled matrix for the matvec
@@ -201,7 +205,9 @@ int mp_pack(phg_ctx* c);
 int mp_forward(phg_ctx* c, const double* v, double* z, bool train);
 // loss and gradient for targets ystar (n x 5, float, device) of the last
 // train forward; lr > 0 applies an Adam step
-int mp_backward(phg_ctx* c, float lr, double* loss_out);
+int mp_backward(phg_ctx* c, float lr, double* loss_out, bool given_dy = false);
+// one step on the residual loss ||K z(v) - v||^2 / ||v||^2 (v: 5 planes)
+int mp_train_residual(phg_ctx* c, const double* v, float lr);
 // targets (t - v) s of the last forward's v (t, v: 5 planes, device)
 int mp_targets(phg_ctx* c, const double* t, const double* v);
 int mp_share(phg_ctx* dst, phg_ctx* src);
@@ -216,6 +222,8 @@ int launch_spsi3pre(phg_ctx* c);
 // exact: the matvec of a true residual (row sums in double in every path)
 int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact = false);
 int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n);
+// q = K^T x (x overwritten by its periodic gather); needs c->tslot
+int launch_spmv_t(phg_ctx* c, double* x, double* q);
 int launch_widen(phg_ctx* c, double* dst, const float* src, long long n);
 // n doubles, n even, 16-byte aligned
 int launch_copy(phg_ctx* c, double* dst, const double* src, long long n);

@@ -70,12 +70,14 @@ class PhgStats(ctypes.Structure):
 EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_info",
            "phg_elmgmrs", "phg_solgmrs", "phg_state_upload", "phg_state_download",
            "phg_state_download_old", "phg_state_upload_cur", "phg_predict",
-           "phg_newton_iteration", "phg_update", "phg_sparse_ap", "phg_i3lu", "phg_spsi3pre", "phg_bc3per",
+           "phg_newton_iteration", "phg_update", "phg_sparse_ap", "phg_sparse_ap_t",
+           "phg_i3lu", "phg_spsi3pre", "phg_bc3per",
            "phg_bench_spmv", "phg_bench_assembly", "phg_bench_copy", "phg_sync",
            "phg_mpnn_param_count", "phg_mpnn_init", "phg_mpnn_set_weights",
            "phg_mpnn_get_weights", "phg_mpnn_enable", "phg_mpnn_apply",
            "phg_mpnn_train_step", "phg_mpnn_last_pair", "phg_mpnn_share",
-           "phg_mpnn_dataset", "phg_mpnn_store_pair", "phg_mpnn_train_pair")
+           "phg_mpnn_dataset", "phg_mpnn_store_pair", "phg_mpnn_train_pair",
+           "phg_mpnn_train_krylov")
 
 
 def lib():
@@ -189,6 +191,12 @@ class GpuSolver:
         p = _f64(p, (self.case.nshg, 5)).copy(order="F")
         lk = None if lhsK is None else _f64(lhsK, (25, self.case.nnz_tot))
         _check(lib().phg_sparse_ap(self.h, _p(lk), _p(p)))
+        return p
+
+    def sparse_ap_t(self, lhsK, p):
+        p = _f64(p, (self.case.nshg, 5)).copy(order="F")
+        lk = None if lhsK is None else _f64(lhsK, (25, self.case.nnz_tot))
+        _check(lib().phg_sparse_ap_t(self.h, _p(lk), _p(p)))
         return p
 
     def i3lu(self, diag, r, code):
@@ -316,6 +324,12 @@ class GpuSolver:
         _check(lib().phg_mpnn_train_pair(self.h, int(slot), ctypes.c_float(lr),
                                          ctypes.byref(loss) if want_loss else None))
         return loss.value if want_loss else None
+
+    def mpnn_train_krylov(self, nvec, lr):
+        loss = ctypes.c_double()
+        _check(lib().phg_mpnn_train_krylov(self.h, int(nvec), ctypes.c_float(lr),
+                                           ctypes.byref(loss)))
+        return loss.value
 
     def bench_copy(self, reps=10):
         ms, by = ctypes.c_double(), ctypes.c_double()

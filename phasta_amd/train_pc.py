@@ -6,7 +6,9 @@ Pairs: for each of `meshes` 64^3 boxes (jitter ~ U[0, 0.15] h, seeded) and
 solver runs a predictor and Newton iterations with block-diagonal GMRES
 (Kspace 40, tolerance 1e-4) and keeps, per Newton iteration, the scaled
 right-hand side b = L^-1 res, the BDiag before factorisation and the scaled
-solution z.  The network learns z from (BDiag, b).
+solution z.  The network first learns z from (BDiag, b); then it is tuned on
+the residual ||K z(v) - v||^2 / ||v||^2 of the Krylov vectors v of fresh
+block-diagonal solves, the inputs it meets inside flexible GMRES.
 
     python -m phasta_amd.train_pc --out phasta_amd/weights/mpnn_box64.npz
     python -m phasta_amd.train_pc --eval phasta_amd/weights/mpnn_box64.npz
@@ -92,6 +94,32 @@ def train(ctxs, slots, epochs, lr, seed):
     return history
 
 
+def train_residual(ctxs, N, rounds, states, inner, lr, seed):
+    """Fine-tuning on the residual loss ||K z(v) - v||^2 / ||v||^2 over the
+    Krylov vectors of fresh block-diagonal solves (the directions flexible
+    GMRES hands to the network), K the scaled matrix of each solve."""
+    rng = np.random.default_rng(seed + 3)
+    hist = []
+    for r in range(rounds):
+        losses = []
+        t0 = time.perf_counter()
+        for m, (case, gpu) in enumerate(ctxs):
+            gpu.mpnn_enable(use_pc=0, keep_pair=True)
+            for _ in range(states):
+                mach = float(rng.uniform(0.1, 0.8))
+                fseed = int(rng.integers(1, 2 ** 31))
+                gpu.set_params(**config.solver_params(N, mach, 20))
+                gpu.upload_state(mesh.box_state(case, mach, fseed))
+                gpu.predict()
+                st = gpu.newton_iteration()
+                lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * r / max(1, rounds)))
+                for _ in range(inner):
+                    losses.append(gpu.mpnn_train_krylov(st["iKs"] + 1, lr_t))
+        hist.append(float(np.mean(losses)))
+        log(f"residual round {r}: mean loss {hist[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+    return hist
+
+
 def evaluate(N, weights, seed, states=4):
     """Plain block-diagonal GMRES vs FGMRES with the network on a mesh and
     states not used in training (Kspace 20, tolerance 1e-3)."""
@@ -129,6 +157,12 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--mode", choices=["supervised", "residual", "both"], default="both",
+                    help="supervised on (b, z) pairs, residual loss on Krylov vectors, "
+                         "or the first then the second")
+    ap.add_argument("--rounds", type=int, default=6)
+    ap.add_argument("--states", type=int, default=8)
+    ap.add_argument("--inner", type=int, default=2)
     ap.add_argument("--out", help="weights file to write")
     ap.add_argument("--eval", help="only evaluate the weights in this file")
     args = ap.parse_args()
@@ -142,10 +176,18 @@ def main():
     log(f"harvested {len(slots)} pairs in {time.perf_counter() - t0:.1f} s")
     owner = ctxs[0][1]
     owner.mpnn_set_weights(solver.mpnn_flatten(init_params(args.seed)))
-    hist = train(ctxs, slots, args.epochs, args.lr, args.seed)
+    hist = []
+    if args.mode in ("supervised", "both"):
+        hist = train(ctxs, slots, args.epochs, args.lr, args.seed)
+    rhist = []
+    if args.mode in ("residual", "both"):
+        rhist = train_residual(ctxs, args.N, args.rounds, args.states, args.inner, args.lr,
+                               args.seed)
     w = owner.mpnn_get_weights()
     meta = dict(N=args.N, meshes=args.meshes, pairs=len(slots), newton=args.newton,
                 epochs=args.epochs, lr=args.lr, seed=args.seed, loss=hist,
+                mode=args.mode, residual_loss=rhist, rounds=args.rounds, states=args.states,
+                inner=args.inner,
                 harvest_kspace=HARVEST_K, harvest_etol=HARVEST_ETOL,
                 seconds=time.perf_counter() - t0)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)

@@ -100,3 +100,38 @@ def test_zero_network_fgmres_equals_block_diagonal_gmres():
     # the kept pair: b is the preconditioned residual, z solves the scaled system
     assert rel(b, ref["res"]) < 1e-12
     assert np.linalg.norm(z) > 0
+
+
+def test_residual_loss_matches_host_and_training_reduces_it():
+    """The residual loss ||K z(v) - v||^2 / ||v||^2 of the first Krylov vector
+    v = b / ||b||, formed on the device, equals the same quantity from the
+    host-array entry points; Adam steps on it reduce it."""
+    case = mesh.box_case(4, mach=0.3, kspace=10)
+    orc = pyoracle.Oracle(case)
+    y, ac = orc.itr_predict_bc(case.yold)
+    p = M.init_params(7)
+    p["Wout"] = (0.3 * np.random.default_rng(3).standard_normal(p["Wout"].shape)).astype(np.float32)
+    gpu = solver.GpuSolver(case)
+    try:
+        gpu.mpnn_init(np.ones(25, dtype=np.float32))
+        gpu.mpnn_set_weights(solver.mpnn_flatten(p))
+        gpu.mpnn_enable(use_pc=0, keep_pair=True)
+        out = gpu.solgmrs(y, ac, want_lhs=True)      # lhsK: the scaled matrix K
+        bd, b, _ = gpu.mpnn_last_pair()
+        loss_dev = gpu.mpnn_train_krylov(1, lr=0.0)
+        v = b / np.linalg.norm(b)
+        z = gpu.mpnn_apply(bd, v)
+        kz = gpu.sparse_ap(out["lhsK"], z)
+        loss_host = np.sum((kz - v) ** 2)
+        print(f"[mpnn] residual loss device {loss_dev:.6e} host {loss_host:.6e}", flush=True)
+        assert abs(loss_dev - loss_host) <= 1e-6 * loss_host
+        # training on the Krylov vectors of this solve
+        gpu.solgmrs(y, ac)
+        l0 = gpu.mpnn_train_krylov(out["iKs"] + 1, lr=0.0)
+        for _ in range(20):
+            gpu.mpnn_train_krylov(out["iKs"] + 1, lr=2e-3)
+        l1 = gpu.mpnn_train_krylov(out["iKs"] + 1, lr=0.0)
+        print(f"[mpnn] residual training {l0:.4e} -> {l1:.4e}", flush=True)
+        assert l1 < l0
+    finally:
+        gpu.close()

@@ -1,9 +1,11 @@
 """GPU parity of the FP64 path against the CPU oracle, through the C-ABI.
 
 Bar: FP64 results within 1e-12 normwise relative error of the oracle (the
-reference's operation order; the GPU sums in colour order and regroups the
-tangent algebraically, so agreement is to rounding, not bit-exact), and the
-same Krylov iteration count.
+reference's operation order; the GPU sums element contributions per block in
+element order and regroups the tangent algebraically, so agreement is to
+rounding, not bit-exact), and the same Krylov iteration count.  Cases: 4^3
+and 16^3 boxes, and an 8^3 box y-graded 1.1 from the walls (the stretching
+of the 128^3 workload).
 """
 import os
 
@@ -22,10 +24,12 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-@pytest.fixture(scope="module", params=[4, 16], ids=["box4", "box16"])
+@pytest.fixture(scope="module", params=[(4, 1.0), (16, 1.0), (8, 1.1)],
+                ids=["box4", "box16", "box8_graded"])
 def pair(request):
-    case = mesh.box_case(request.param, mach=0.3, kspace=10)
-    print(f"[parity] box{request.param}: nshg={case.nshg} nel={case.nel}", flush=True)
+    N, grade = request.param
+    case = mesh.box_case(N, mach=0.3, kspace=10, grade=grade)
+    print(f"[parity] box{N} grade {grade}: nshg={case.nshg} nel={case.nel}", flush=True)
     orc = pyoracle.Oracle(case)
     gpu = solver.GpuSolver(case)
     yield case, orc, gpu
@@ -66,6 +70,12 @@ def test_sparse_ap_i3lu_spsi3pre_bc3per(pair):
     ref = p.copy(order="F")
     orc.sparse_ap(lk, ref)
     assert rel(gpu.sparse_ap(lk, p), ref) < TOL
+    # transpose: (q, K p) = (K^T q, p) for the periodic-folded operator
+    q = np.asfortranarray(rng.standard_normal((case.nshg, 5)))
+    kp = gpu.sparse_ap(lk, p)
+    ktq = gpu.sparse_ap_t(lk, q)
+    lhs_ip, rhs_ip = np.sum(q * kp), np.sum(ktq * p)
+    assert abs(lhs_ip - rhs_ip) <= 1e-12 * (np.linalg.norm(q) * np.linalg.norm(kp))
     # i3LU factor / forward / backward
     d_ref = bd.copy(order="F")
     orc.i3lu(d_ref, None, "LU_Fact ")
