@@ -121,7 +121,7 @@ def test_residual_loss_matches_host_and_training_reduces_it():
         loss_dev = gpu.mpnn_train_krylov(1, lr=0.0)
         v = b / np.linalg.norm(b)
         z = gpu.mpnn_apply(bd, v)
-        kz = gpu.sparse_ap(out["lhsK"], z)
+        kz = gpu.bc3per(gpu.sparse_ap(out["lhsK"], z))  # SparseAp, then bc3per
         loss_host = np.sum((kz - v) ** 2)
         print(f"[mpnn] residual loss device {loss_dev:.6e} host {loss_host:.6e}", flush=True)
         assert abs(loss_dev - loss_host) <= 1e-6 * loss_host

@@ -70,9 +70,10 @@ def test_sparse_ap_i3lu_spsi3pre_bc3per(pair):
     ref = p.copy(order="F")
     orc.sparse_ap(lk, ref)
     assert rel(gpu.sparse_ap(lk, p), ref) < TOL
-    # transpose: (q, K p) = (K^T q, p) for the periodic-folded operator
+    # transpose: (q, K p) = (K^T q, p) for the operator of the GMRES
+    # matvec, K p = bc3per(SparseAp(p))
     q = np.asfortranarray(rng.standard_normal((case.nshg, 5)))
-    kp = gpu.sparse_ap(lk, p)
+    kp = gpu.bc3per(gpu.sparse_ap(lk, p))
     ktq = gpu.sparse_ap_t(lk, q)
     lhs_ip, rhs_ip = np.sum(q * kp), np.sum(ktq * p)
     assert abs(lhs_ip - rhs_ip) <= 1e-12 * (np.linalg.norm(q) * np.linalg.norm(kp))
