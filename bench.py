@@ -8,13 +8,16 @@ A step is one Newton iteration on the device-resident state: ElmGMRs
 predictor restarts the time step from the same yold, so each timed step
 repeats a real Newton iteration of the workload.
 
-The default workload is the 128^3 y-graded box the metric is quoted on.  Two
-arms run on the same state: plain FP64 block-diagonal GMRES, and the
-mixed-precision form (float tangent and matrix, double Krylov/MGS/Givens and
-refinement residual), with the learned preconditioner when --pc learned is
-given.  value = interior tets per second of the mixed arm (nel / time per
-Newton iteration); the FP64 arm's time and the ratio are reported beside it.
-Inputs are resident in HBM before the timed region.
+The default workload is the 128^3 y-graded box the metric is quoted on.
+Three arms run on the same state: plain FP64 block-diagonal GMRES ("fp64"),
+the mixed-precision form (float tangent and matrix, double Krylov/MGS/Givens
+and refinement residual) with block-diagonal GMRES ("mixed"), and the same
+with the learned preconditioner on the first direction of every flexible
+GMRES cycle ("learned", weights phasta_amd/weights/mpnn_box64.npz).
+value = interior tets per second of the learned arm, the form the metric
+names (nel / time per Newton iteration); every arm's time, its ratio to
+FP64 and its update's error against FP64 are reported beside it.  Inputs
+are resident in HBM before the timed region.
 """
 import argparse
 import json
@@ -35,6 +38,19 @@ def spmv_bytes(nnz, nshg, s_a):
     """Algorithmic bytes of one block-CSR matvec: blocks + column indices +
     row pointers + one read and one write of the double 5-vector."""
     return nnz * (25 * s_a + 4) + 4 * (nshg + 1) + 2 * nshg * 5 * 8
+
+
+def measured_traffic(kernel, case):
+    """HBM bytes per launch of `kernel` on `case` from the committed rocprof
+    summary (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", f"{case}_newton_iteration.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic {kernel} {case}:"
+    for ln in open(path):
+        if ln.startswith(key):
+            return int(ln[len(key):].strip())
+    return None
 
 
 def cpu_baseline(name):
@@ -117,10 +133,8 @@ def main():
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--case", default="box128")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
-                    help="run only this arm (default: FP64 and mixed)")
-    ap.add_argument("--pc", choices=["block", "learned"], default="block",
-                    help="preconditioner of the mixed arm")
+    ap.add_argument("--arms", default="fp64,mixed,learned",
+                    help="comma-separated subset of fp64, mixed, learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -134,15 +148,16 @@ def main():
     spec = dict(config.CASES[args.case])
     case = mesh.named_case(args.case)
     nel = case.nel
-    arms = ["fp64", "mixed"] if args.precision is None else [args.precision]
+    arms = [a for a in ("fp64", "mixed", "learned") if a in args.arms.split(",")]
+    if not arms:
+        ap.error("--arms names none of fp64, mixed, learned")
     res = {}
     for a in arms:
-        prec = solver.PHG_MIXED if a == "mixed" else solver.PHG_FP64
-        wts = WEIGHTS if (a == "mixed" and args.pc == "learned") else None
-        print(f"[bench] {args.case} arm {a} pc {args.pc if a == 'mixed' else 'block'}",
-              file=sys.stderr, flush=True)
+        prec = solver.PHG_FP64 if a == "fp64" else solver.PHG_MIXED
+        wts = WEIGHTS if a == "learned" else None
+        print(f"[bench] {args.case} arm {a}", file=sys.stderr, flush=True)
         res[a] = run_arm(case, prec, spec["newton"], args.steps, args.warmup, wts)
-    head_name = "mixed" if "mixed" in res else arms[0]
+    head_name = "learned" if "learned" in res else ("mixed" if "mixed" in res else "fp64")
     head = res[head_name]
     value = nel / (head["ms"] / 1e3)
 
@@ -150,13 +165,15 @@ def main():
     st = head["stats"]
     ms_spmv = sum(s["ms_spmv"] for s in st)
     n_spmv = sum(s["n_spmv"] for s in st)
-    s_a = 4 if head_name == "mixed" else 8
+    s_a = 8 if head_name == "fp64" else 4
     by = spmv_bytes(case.nnz_tot, case.nshg, s_a)
     spmv_avg = ms_spmv / max(1, n_spmv)
     spmv_gbs = by / (spmv_avg * 1e-3) / 1e9
-    roofline = {"kernel": "k_spmv (block-CSR matvec, SparseAp)", "bound": "hbm",
+    kname = "k_spmv<double, double>" if head_name == "fp64" else "k_spmv<float, double>"
+    roofline = {"kernel": f"{kname} (block-CSR matvec, SparseAp)", "bound": "hbm",
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(kname, args.case),
                 "bytes_per_launch": by, "ms_per_launch": round(spmv_avg, 5)}
 
     desc = (f"{args.case}: {spec['N']}^3-hex unit box, Kuhn 6-tet split ({nel} tets, "
@@ -168,31 +185,48 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(head["ms"], 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if head_name == "mixed" else "f64",
+        "dtype": "f64" if head_name == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": desc, "N": spec["N"], "nel": nel, "nshg": case.nshg,
                    "nnz_blocks": case.nnz_tot, "newton_per_time_step": spec["newton"],
-                   "arm": head_name, "pc": args.pc if head_name == "mixed" else "block"},
+                   "arm": head_name, "pc": "learned" if head_name == "learned" else "block"},
         "roofline": roofline,
         "arms": {a: breakdown(res[a], args.steps, nel) for a in res},
         "copy_GBs": round(head["by_copy"] / (head["ms_copy"] * 1e-3) / 1e9, 1),
     }
-    if "fp64" in res and "mixed" in res:
-        d64, dmx = res["fp64"]["dy"], res["mixed"]["dy"]
-        line["speedup_vs_fp64"] = round(res["fp64"]["ms"] / res["mixed"]["ms"], 3)
-        line["mixed_vs_fp64"] = {
-            "Dy_rel_err": float(np.linalg.norm(dmx - d64) / np.linalg.norm(d64)),
-            "iKs": [res["mixed"]["first"]["iKs"], res["fp64"]["first"]["iKs"]],
-            "converged": [bool(res["mixed"]["first"]["converged"]),
-                          bool(res["fp64"]["first"]["converged"])]}
+    if "fp64" in res:
+        d64 = res["fp64"]["dy"]
+        for a in res:
+            if a == "fp64":
+                continue
+            # time ratio, and the first Newton update against the FP64 arm's
+            line["arms"][a]["speedup_vs_fp64"] = round(res["fp64"]["ms"] / res[a]["ms"], 3)
+            line["arms"][a]["vs_fp64"] = {
+                "Dy_rel_err": float(np.linalg.norm(res[a]["dy"] - d64) / np.linalg.norm(d64)),
+                "iKs": [res[a]["first"]["iKs"], res["fp64"]["first"]["iKs"]],
+                "converged": [bool(res[a]["first"]["converged"]),
+                              bool(res["fp64"]["first"]["converged"])]}
+        if head_name != "fp64":
+            line["speedup_vs_fp64"] = line["arms"][head_name]["speedup_vs_fp64"]
     if not args.no_cpu:
         sample = "box64"
-        dt, ccase, _ = cpu_baseline(sample)
+        dt, ccase, orc_out = cpu_baseline(sample)
         line["cpu_baseline"] = {"value": round(ccase.nel / dt, 1), "unit": "elements/s",
                                 "cores": 1, "kind": "port",
                                 "sample": f"one FP64 Newton iteration of the {sample} case "
                                           f"({ccase.nel} tets, same per-element work) by the "
                                           f"C oracle on 1 host core ({dt:.1f} s)"}
+        # the same Newton iteration on the GPU (FP64) against the oracle's
+        gpu = solver.GpuSolver(ccase)
+        gpu.upload_state(ccase.yold, np.zeros((ccase.nshg, 5), order="F"))
+        gpu.predict()
+        st = gpu.newton_iteration()
+        _, _, dy = gpu.download_state()
+        gpu.close()
+        d_ref = orc_out["Dy"]
+        line["oracle_parity"] = {
+            "case": sample, "Dy_rel_err": float(np.linalg.norm(dy - d_ref) / np.linalg.norm(d_ref)),
+            "iKs": [st["iKs"], orc_out["iKs"]]}
     print(json.dumps(line), flush=True)
 
 

@@ -187,8 +187,11 @@ int phg_mpnn_train_pair(phg_ctx* ctx, int slot, float lr, double* loss);
 
 /* Training on the residual of preconditioned directions: one Adam step per
  * Krylov vector v_1..v_nvec of the last solve (keep_pair enabled), loss
- * ||K z(v) - v||^2 / ||v||^2 with K the scaled matrix of that solve. */
-int phg_mpnn_train_krylov(phg_ctx* ctx, int nvec, float lr, double* mean_loss);
+ * ||K z(v) - v||^2 / ||v||^2 with K = bc3per o SparseAp the scaled operator
+ * of that solve (gradient through K^T, as phg_sparse_ap_t).  mean_loss and
+ * grad (the gradient of the last vector's loss, phg_mpnn_param_count()
+ * floats) are nullable. */
+int phg_mpnn_train_krylov(phg_ctx* ctx, int nvec, float lr, double* mean_loss, float* grad);
 
 /* Wait for all work queued on the library stream. */
 int phg_sync(phg_ctx* ctx);

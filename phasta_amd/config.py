@@ -17,9 +17,14 @@ this path is pinned here once (reference defaults unless noted):
 * nGMRES = 1, Kspace 10 (16^3 case) or 20, epstol 1e-3, minIters 0
   (reference default 10; set explicitly and identically in every arm)
 * wall thermal condition: floating flux (iBCB = 0, e3b.f:149-228)
-* time step: dt = CFL * h / (c0 (1 + M)) with CFL = 1 and h = 1/N
-  (10-11 block-diagonal GMRES iterations per solve on the 16^3 box, the
-  range of the reference's legacy compressible case)
+* time step: dt = CFL * h_min / (c0 (1 + M)), h_min the smallest grid-line
+  spacing: 1/N on the uniform boxes with CFL = 1 (10-11 block-diagonal
+  GMRES iterations per solve on the 16^3 box, the range of the reference's
+  legacy compressible case); on the y-graded boxes h_min is the first wall
+  cell and CFL = 3 (128^3, ratio 1.1: h_min = 1.12e-4, 10-19 Krylov
+  iterations per Newton iteration; with h = 1/N the wall-normal acoustic
+  CFL is about 50, block-diagonal GMRES(20) stalls and the state breaks
+  down in the second Newton iteration)
 """
 import math
 
@@ -44,23 +49,32 @@ FIELD_SEED = 7
 # Synthetic workloads (unit box, Kuhn 6-tet split, periodic x/z, walls y).
 CASES = {
     "box16": dict(N=16, mach=0.3, jitter=0.1, grade=1.0, kspace=10,
-                  newton=1, steps=1, precision="fp64"),
+                  newton=1, steps=1, precision="fp64", cfl=1.0),
     "box64": dict(N=64, mach=0.5, jitter=0.1, grade=1.0, kspace=20,
-                  newton=3, steps=1, precision="fp64"),
+                  newton=3, steps=1, precision="fp64", cfl=1.0),
     "box128": dict(N=128, mach=0.3, jitter=0.1, grade=1.1, kspace=20,
-                   newton=3, steps=1, precision="mixed"),
+                   newton=3, steps=1, precision="mixed", cfl=3.0),
     "box256": dict(N=256, mach=0.6, jitter=0.1, grade=1.08, kspace=20,
-                   newton=3, steps=5, precision="mixed"),
+                   newton=3, steps=5, precision="mixed", cfl=3.0),
 }
 
 
-def time_step(N, mach):
-    return CFL * (1.0 / N) / (C0 * (1.0 + mach))
+def min_spacing(N, grade=1.0):
+    """Smallest grid-line spacing of the box: 1/N, or with geometric
+    y-grading of ratio `grade` from both walls the first wall cell
+    h0 = (1/2) (grade - 1) / (grade^(N/2) - 1) (the mesh generator's ygrid)."""
+    if grade == 1.0:
+        return 1.0 / N
+    return 0.5 * (grade - 1.0) / (grade ** (N // 2) - 1.0)
 
 
-def solver_params(N, mach, kspace, etol=1e-3, min_iters=0):
+def time_step(N, mach, grade=1.0, cfl=CFL):
+    return cfl * min_spacing(N, grade) / (C0 * (1.0 + mach))
+
+
+def solver_params(N, mach, kspace, etol=1e-3, min_iters=0, grade=1.0, cfl=CFL):
     """The COMMON-block scalars the path reads, as a plain dict."""
-    dt = time_step(N, mach)
+    dt = time_step(N, mach, grade, cfl)
     return dict(Rgas=RGAS, gamma=GAMMA, gamma1=GAMMA1, pr=PRANDTL, mu=MU,
                 dtsfct=1.0, taucfct=1.0, temper=1.0,
                 almi=1.0, alfi=1.0, gami=1.0, Dtgl=1.0 / dt,

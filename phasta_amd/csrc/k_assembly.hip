@@ -44,7 +44,7 @@ struct Quad {
 
 __constant__ Quad cQ;
 
-// Records of 13 values (point state: u(3) rho drdp drdT e1p..e4p tau(3);
+// Records of 13 values (point state: u(3) rho drdp drdT e1p..e4p WdetJ tau(3);
 // geometry: shape gradients (4 x 3) and WdetJ), padded so that a record is
 // read with 16-byte loads: 14 doubles or 16 floats.
 template <typename S>
@@ -272,9 +272,9 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
     o[7] = (S)t.e2p;
     o[8] = (S)t.e3p;
     o[9] = (S)t.e4p;
-    o[10] = (S)t.tau[0];
-    o[11] = (S)t.tau[1];
-    o[12] = (S)t.tau[2];
+    o[10] = (S)(WdetJ * t.tau[0]);  // WdetJ tau: the tangent pass scales P by WdetJ
+    o[11] = (S)(WdetJ * t.tau[1]);
+    o[12] = (S)(WdetJ * t.tau[2]);
   }
 
   // time term A0 aci and the least-squares residual L = A_i Y,i + A0 Y,t
@@ -393,18 +393,24 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     }
     const R W = (R)go[12];
     const R st = sR[0] * rR[0] + sR[1] * rR[1] + sR[2] * rR[2];
-    // acc += WdetJ [P Q + K_ab] per point; WdetJ is folded into P's rows
+    // acc += WdetJ [P Q] per point: the stored tau is WdetJ tau and N_a is
+    // scaled here, so P's rows come out multiplied by WdetJ
     auto add = [&](int idx, R val) { acc[idx] += val; };
+    R ub[3] = {(R)0, (R)0, (R)0};  // sum of the point velocities
+    const S* qrec = qps + (size_t)e * 4 * rec_stride<S>();
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
       S o[16];
-      load_rec<S>(qps + ((size_t)e * 4 + q) * rec_stride<S>(), o);
+      load_rec<S>(qrec + q * rec_stride<S>(), o);
       const R u0 = (R)o[0], u1 = (R)o[1], u2 = (R)o[2];
       const R uq[3] = {u0, u1, u2};
+      ub[0] += u0;
+      ub[1] += u1;
+      ub[2] += u2;
       const R rho = (R)o[3], drdp = (R)o[4], drdT = (R)o[5];
       const R e1p = (R)o[6], e2p = (R)o[7], e3p = (R)o[8], e4p = (R)o[9];
       const R tau0 = (R)o[10], tau1 = (R)o[11], tau2 = (R)o[12];
-      const R Na = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
+      const R Na = W * (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
       const R Nb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
       const R una = u0 * sR[0] + u1 * sR[1] + u2 * sR[2];
       const R unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
@@ -450,23 +456,28 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
           Pr[4] = e4p * una * tau2 + Na;
         }
 #pragma unroll
-        for (int n = 0; n < 5; ++n) Pr[n] *= W;
-#pragma unroll
         for (int j = 0; j < 5; ++j)
           add(m + 5 * j, Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
                              Pr[4] * Q[4][j]);
       }
-      // viscous part K_ab
+    }
+    // viscous part sum_q WdetJ K_ab: the momentum rows and the conduction
+    // term do not depend on the point (4 equal weights), the energy row only
+    // through the velocity, linearly (sum over the points = ub)
+    {
+      const R W4 = (R)4 * W;
+      const R ubs = ub[0] * sR[0] + ub[1] * sR[1] + ub[2] * sR[2];
+      const R ubr = ub[0] * rR[0] + ub[1] * rR[1] + ub[2] * rR[2];
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int n = 0; n < 3; ++n)
           add((1 + m) + 5 * (1 + n),
-              W * (muR * ((m == n ? st : (R)0) + sR[n] * rR[m]) + lamR * sR[m] * rR[n]));
+              W4 * (muR * ((m == n ? st : (R)0) + sR[n] * rR[m]) + lamR * sR[m] * rR[n]));
 #pragma unroll
       for (int n = 0; n < 3; ++n)
-        add(4 + 5 * (1 + n), W * (muR * (st * uq[n] + sR[n] * unb) + lamR * una * rR[n]));
-      add(24, W * kapR * st);
+        add(4 + 5 * (1 + n), W * (muR * (st * ub[n] + sR[n] * ubr) + lamR * ubs * rR[n]));
+      add(24, W4 * kapR * st);
     }
 
     if (diag) {
@@ -654,19 +665,37 @@ __global__ void __launch_bounds__(64) k_wall(WallArgs A) {
 
 // Accumulation layout -> reference planes, with the Dirichlet rows of res
 // and BDiag (bc3res.f:25-153, bc3bdg.f:25-330)
-__global__ void k_bc_nodes(const double* ares, const double* abd, double* res, double* bdiag,
-                           const uint8_t* bcm, long long nshg, int prec) {
-  long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (n >= nshg) return;
-  const uint8_t m = bcm[n];
+// The assembly leaves res (5) and BDiag (25) per node contiguous; a block of
+// 128 nodes stages its records through LDS (coalesced reads) and writes the
+// Dirichlet-masked values as planes.
+__global__ void __launch_bounds__(128) k_bc_nodes(const double* __restrict__ ares,
+                                                  const double* __restrict__ abd,
+                                                  double* __restrict__ res,
+                                                  double* __restrict__ bdiag,
+                                                  const uint8_t* __restrict__ bcm, long long nshg,
+                                                  int prec) {
+  __shared__ double sb[128 * 25];
+  const long long n0 = blockIdx.x * 128LL;
+  const int cnt = (int)(nshg - n0 < 128 ? nshg - n0 : 128);
+  const int t = threadIdx.x;
+  const long long n = n0 + t;
+  for (int k = t; k < 5 * cnt; k += 128) sb[k] = ares[5 * n0 + k];
+  __syncthreads();
+  const uint8_t m = t < cnt ? bcm[n] : 0;
+  if (t < cnt) {
 #pragma unroll
-  for (int f = 0; f < 5; ++f) res[n + f * nshg] = ((m >> f) & 1) ? 0.0 : ares[5 * n + f];
+    for (int f = 0; f < 5; ++f) res[n + f * nshg] = ((m >> f) & 1) ? 0.0 : sb[5 * t + f];
+  }
   if (!prec) return;
+  __syncthreads();
+  for (int k = t; k < 25 * cnt; k += 128) sb[k] = abd[25 * n0 + k];
+  __syncthreads();
+  if (t >= cnt) return;
 #pragma unroll
   for (int j = 0; j < 5; ++j)
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-      double v = abd[25 * n + i + 5 * j];
+      double v = sb[25 * t + i + 5 * j];
       if (((m >> i) & 1) || ((m >> j) & 1)) v = (i == j) ? 1.0 : 0.0;
       bdiag[n + (i + 5 * j) * nshg] = v;
     }
@@ -845,7 +874,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
 
 int launch_bc_res_bdiag(phg_ctx* c, bool prec) {
   long long n = c->nshg;
-  k_bc_nodes<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->acc_res, c->acc_bd, c->res, c->bdiag,
+  k_bc_nodes<<<(int)((n + 127) / 128), 128, 0, c->stream>>>(c->acc_res, c->acc_bd, c->res, c->bdiag,
                                                              c->bcm, n, prec);
   PHG_HIP(hipGetLastError());
   if (c->nfold > 0) {

@@ -145,11 +145,12 @@ __global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO*
         for (int m = 0; m < c; ++m) acc -= up[m][c] * b[f][m];
         b[f][c] = up[c][c] * acc;
       }
-    TO* blk = vals + s * (25 * kGroup) + r;
+    double a[25];
 #pragma unroll
     for (int c = 0; c < 5; ++c)
 #pragma unroll
-      for (int f = 0; f < 5; ++f) blk[(f + 5 * c) * kGroup] = (TO)b[f][c];
+      for (int f = 0; f < 5; ++f) a[f + 5 * c] = b[f][c];
+    store_slot(vals + s * (25 * kGroup), r, a);
   }
 }
 
@@ -160,38 +161,49 @@ __global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO*
 // scaled system lose digits to cancellation (on the 16^3 box the refinement
 // loop then needed 12 instead of 10 Krylov iterations), while the kernel is
 // bound by the matrix bytes, which float storage halves either way.
+// Launched on a grid that is a multiple of 8: consecutive workgroups go to
+// different XCDs (round robin), so workgroup b takes row range
+// (b % 8) * per + b / 8 and each XCD sweeps one contiguous eighth of the
+// rows, whose column neighbours (rows +-1, +-(N+1), +-(N+1)^2 on the box)
+// then stay in that XCD's L2.
 template <typename T, typename Acc>
 __global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const int* __restrict__ scolp,
                                               const long long* __restrict__ grp_slot,
                                               const double* __restrict__ p, double* __restrict__ q,
                                               long long n) {
-  const long long row = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long per = (gridDim.x + 7) / 8;
+  const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const long long row = wg * blockDim.x + threadIdx.x;
+  if (row >= n) return;
   const long long g = row / kGroup;
   const int r = (int)(row % kGroup);
-  if (row >= n) return;
   Acc t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
   const long long s0 = grp_slot[g], s1 = grp_slot[g + 1];
-  auto slot = [&](long long s, long long j) {
-    const Acc p0 = (Acc)p[j], p1 = (Acc)p[j + n], p2 = (Acc)p[j + 2 * n], p3 = (Acc)p[j + 3 * n],
-              p4 = (Acc)p[j + 4 * n];
-    const T* blk = vals + s * (25 * kGroup) + r;
-    Acc a[25];
+  // software pipeline: the gathers of p for slot s + 1 and the column index
+  // of slot s + 2 are issued before the block of slot s is consumed
+  Acc pc[5] = {0, 0, 0, 0, 0};
+  if (s0 < s1) {
+    const long long j0 = scolp[s0 * kGroup + r];
 #pragma unroll
-    for (int e = 0; e < 25; ++e) a[e] = (Acc)blk[e * kGroup];
-    t0 += a[0] * p0 + a[5] * p1 + a[10] * p2 + a[15] * p3 + a[20] * p4;
-    t1 += a[1] * p0 + a[6] * p1 + a[11] * p2 + a[16] * p3 + a[21] * p4;
-    t2 += a[2] * p0 + a[7] * p1 + a[12] * p2 + a[17] * p3 + a[22] * p4;
-    t3 += a[3] * p0 + a[8] * p1 + a[13] * p2 + a[18] * p3 + a[23] * p4;
-    t4 += a[4] * p0 + a[9] * p1 + a[14] * p2 + a[19] * p3 + a[24] * p4;
-  };
-  // the column of the next slot is loaded one slot ahead, so the gathers of
-  // p never wait on the index load of their own slot
-  long long s = s0;
-  long long jn = s < s1 ? scolp[s * kGroup + r] : 0;
-  for (; s < s1; ++s) {
-    const long long j = jn;
-    if (s + 1 < s1) jn = scolp[(s + 1) * kGroup + r];
-    slot(s, j);
+    for (int f = 0; f < 5; ++f) pc[f] = (Acc)p[j0 + f * n];
+  }
+  long long jn = s0 + 1 < s1 ? scolp[(s0 + 1) * kGroup + r] : 0;
+  for (long long s = s0; s < s1; ++s) {
+    Acc pn[5] = {0, 0, 0, 0, 0};
+    if (s + 1 < s1) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) pn[f] = (Acc)p[jn + f * n];
+      if (s + 2 < s1) jn = scolp[(s + 2) * kGroup + r];
+    }
+    Acc a[25];
+    load_slot(vals + s * (25 * kGroup), r, a);
+    t0 += a[0] * pc[0] + a[5] * pc[1] + a[10] * pc[2] + a[15] * pc[3] + a[20] * pc[4];
+    t1 += a[1] * pc[0] + a[6] * pc[1] + a[11] * pc[2] + a[16] * pc[3] + a[21] * pc[4];
+    t2 += a[2] * pc[0] + a[7] * pc[1] + a[12] * pc[2] + a[17] * pc[3] + a[22] * pc[4];
+    t3 += a[3] * pc[0] + a[8] * pc[1] + a[13] * pc[2] + a[18] * pc[3] + a[23] * pc[4];
+    t4 += a[4] * pc[0] + a[9] * pc[1] + a[14] * pc[2] + a[19] * pc[3] + a[24] * pc[4];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) pc[f] = pn[f];
   }
   q[row] = (double)t0;
   q[row + n] = (double)t1;
@@ -217,7 +229,8 @@ __global__ void __launch_bounds__(256) k_spmv_t(const T* __restrict__ vals, cons
   for (long long s = grp_slot[g]; s < grp_slot[g + 1]; ++s) {
     const long long i = scol[s * kGroup + r];
     const uint32_t ts = tslot[s * kGroup + r];
-    const T* blk = vals + (size_t)(ts >> 6) * (25 * kGroup) + (ts & 63);
+    const T* blk = vals + (size_t)(ts >> 6) * (25 * kGroup);
+    const int tr = (int)(ts & 63);
     double xi[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) xi[f] = x[i + f * n];
@@ -225,7 +238,7 @@ __global__ void __launch_bounds__(256) k_spmv_t(const T* __restrict__ vals, cons
     for (int gg = 0; gg < 5; ++gg) {
       double a = 0.0;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) a += (double)blk[(f + 5 * gg) * kGroup] * xi[f];
+      for (int f = 0; f < 5; ++f) a += (double)blk[slot_pos<T>(f + 5 * gg, tr)] * xi[f];
       t[gg] += a;
     }
   }
@@ -255,9 +268,18 @@ __global__ void k_sub(double* r, const double* b, const double* q, long long n) 
 
 // plain copy, 16 B per lane (calibration of the memory counters)
 __global__ void k_copy(double2* __restrict__ dst, const double2* __restrict__ src, long long n2) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2;
-       i += (long long)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+  // four independent 16-byte loads in flight per lane before their stores
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n2; i += 4 * stride) {
+    const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
+                  d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n2; i += stride) dst[i] = src[i];
 }
 
 // float -> double copy
@@ -446,9 +468,9 @@ __global__ void k_csr_to_slots(const double* ref, T* slots, const uint32_t* c2s,
   long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   const uint32_t qs = c2s[k];
-  T* dst = slots + (size_t)(qs >> 6) * (25 * kGroup) + (qs & 63);
+  T* dst = slots + (size_t)(qs >> 6) * (25 * kGroup);
 #pragma unroll
-  for (int e = 0; e < 25; ++e) dst[e * kGroup] = (T)ref[k * 25 + e];
+  for (int e = 0; e < 25; ++e) dst[slot_pos<T>(e, (int)(qs & 63))] = (T)ref[k * 25 + e];
 }
 
 template <typename T>
@@ -456,9 +478,9 @@ __global__ void k_slots_to_csr(const T* slots, double* ref, const uint32_t* c2s,
   long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   const uint32_t qs = c2s[k];
-  const T* src = slots + (size_t)(qs >> 6) * (25 * kGroup) + (qs & 63);
+  const T* src = slots + (size_t)(qs >> 6) * (25 * kGroup);
 #pragma unroll
-  for (int e = 0; e < 25; ++e) ref[k * 25 + e] = (double)src[e * kGroup];
+  for (int e = 0; e < 25; ++e) ref[k * 25 + e] = (double)src[slot_pos<T>(e, (int)(qs & 63))];
 }
 
 // ---------------- itrPredict / itrCorrect / itrBC ----------------
@@ -549,11 +571,12 @@ int launch_i3lu(phg_ctx* c, double* diag, double* r, int code) {
 }
 
 int launch_spsi3pre(phg_ctx* c) {
+  const int g = (int)c->ngroups;
   if (c->precision == PHG_MIXED)
-    k_spsi3pre<float, float><<<(int)c->ngroups, kGroup, 0, c->stream>>>(
+    k_spsi3pre<float, float><<<g, kGroup, 0, c->stream>>>(
         c->lhs_aos32, c->lhs32, c->slot2csr, c->scol, c->grp_slot, c->bdiag, c->nshg);
   else
-    k_spsi3pre<double, double><<<(int)c->ngroups, kGroup, 0, c->stream>>>(
+    k_spsi3pre<double, double><<<g, kGroup, 0, c->stream>>>(
         c->lhs_aos, c->lhs64, c->slot2csr, c->scol, c->grp_slot, c->bdiag, c->nshg);
   PHG_HIP(hipGetLastError());
   return 0;
@@ -561,7 +584,7 @@ int launch_spsi3pre(phg_ctx* c) {
 
 int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact) {
   (void)exact;  // both matvecs of the mixed path form double row sums
-  const int g = grid_for(c->nshg, 256);
+  const int g = (grid_for(c->nshg, 256) + 7) / 8 * 8;  // XCD-strided rows
   if (c->precision == PHG_MIXED)
     k_spmv<float, double><<<g, 256, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg);
   else

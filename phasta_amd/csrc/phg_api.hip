@@ -360,11 +360,11 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->tslot, c->nslots * kGroup));
   PHG_TRY(dev_alloc(c, &c->csr2slot, nnz));
   PHG_TRY(dev_alloc(c, &c->slot2csr, c->nslots * kGroup));
+  PHG_TRY(dev_alloc(c, &c->acc_res, 5 * nshg));
   if (mixed)
     PHG_TRY(dev_alloc(c, &c->lhs_aos32, 25 * nnz));
   else
     PHG_TRY(dev_alloc(c, &c->lhs_aos, 25 * nnz));
-  PHG_TRY(dev_alloc(c, &c->acc_res, 5 * nshg));
   PHG_TRY(dev_alloc(c, &c->acc_bd, 25 * nshg));
   PHG_TRY(dev_alloc(c, &c->nodepack, 16 * nshg));
   if (mixed)
@@ -1119,7 +1119,8 @@ extern "C" int phg_mpnn_train_pair(phg_ctx* c, int slot, float lr, double* loss)
   return 0;
 }
 
-extern "C" int phg_mpnn_train_krylov(phg_ctx* c, int nvec, float lr, double* mean_loss) {
+extern "C" int phg_mpnn_train_krylov(phg_ctx* c, int nvec, float lr, double* mean_loss,
+                                     float* grad) {
   PHG_TRY(mp_check(c));
   if (!c->mp.keep_pair) {
     phg_set_error("phg_mpnn_train_krylov: enable keep_pair before the solve");
@@ -1132,6 +1133,9 @@ extern "C" int phg_mpnn_train_krylov(phg_ctx* c, int nvec, float lr, double* mea
   PHG_TRY(mp_alloc_train(c));
   PHG_TRY(launch_fill(c, c->mp.sbuf + 4, sizeof(double)));
   for (int k = 0; k < nvec; ++k) PHG_TRY(mp_train_residual(c, kvec(c, k), lr));
+  if (grad)
+    PHG_HIP(hipMemcpyAsync(grad, c->mp.g, sizeof(float) * mp_param_count(), hipMemcpyDeviceToHost,
+                           c->stream));
   if (mean_loss) {
     double s4 = 0.0;
     PHG_HIP(hipMemcpyAsync(&s4, c->mp.sbuf + 4, sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -32,6 +32,63 @@ void phg_set_error(const char* fmt, ...);
 // a group holds S slots, each slot one 5x5 block per row of the group.
 constexpr int kGroup = 64;
 
+// Position of entry e (0..24, column-major 5x5) of row r's block inside one
+// slot of 25 * kGroup values of type T: entries 0..23 form 16-byte vectors
+// of V = 16 / sizeof(T) entries, lane-interleaved (vector k of row r at
+// (k * kGroup + r) * V), so a wavefront reads each vector with one
+// 16-byte load per lane over a contiguous 1 KiB; entry 24 follows as a
+// plane of kGroup values.
+template <typename T>
+__host__ __device__ constexpr int slot_pos(int e, int r) {
+  return e < 24 ? ((e / (16 / (int)sizeof(T))) * kGroup + r) * (16 / (int)sizeof(T)) +
+                      e % (16 / (int)sizeof(T))
+                : 24 * kGroup + r;
+}
+
+// The 25 entries of row r's block in one slot (blk = slot base), read and
+// written with 16-byte vector accesses (6 float4 or 12 double2) and one
+// scalar for entry 24.
+template <typename A>
+__device__ inline void load_slot(const float* blk, int r, A a[25]) {
+  const float4* v = reinterpret_cast<const float4*>(blk);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const float4 t = v[k * kGroup + r];
+    a[4 * k] = (A)t.x;
+    a[4 * k + 1] = (A)t.y;
+    a[4 * k + 2] = (A)t.z;
+    a[4 * k + 3] = (A)t.w;
+  }
+  a[24] = (A)blk[24 * kGroup + r];
+}
+template <typename A>
+__device__ inline void load_slot(const double* blk, int r, A a[25]) {
+  const double2* v = reinterpret_cast<const double2*>(blk);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    const double2 t = v[k * kGroup + r];
+    a[2 * k] = (A)t.x;
+    a[2 * k + 1] = (A)t.y;
+  }
+  a[24] = (A)blk[24 * kGroup + r];
+}
+template <typename A>
+__device__ inline void store_slot(float* blk, int r, const A a[25]) {
+  float4* v = reinterpret_cast<float4*>(blk);
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    v[k * kGroup + r] = make_float4((float)a[4 * k], (float)a[4 * k + 1], (float)a[4 * k + 2],
+                                    (float)a[4 * k + 3]);
+  blk[24 * kGroup + r] = (float)a[24];
+}
+template <typename A>
+__device__ inline void store_slot(double* blk, int r, const A a[25]) {
+  double2* v = reinterpret_cast<double2*>(blk);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k * kGroup + r] = make_double2((double)a[2 * k], (double)a[2 * k + 1]);
+  blk[24 * kGroup + r] = (double)a[24];
+}
+
 // Largest number of element blocks one assembly lane sums (the blocks of
 // one CSR entry are split into chunks of at most this size).
 constexpr int kChunk = 6;
@@ -152,10 +209,10 @@ struct phg_ctx {
   float* lhs32 = nullptr;          // mixed precision copy
 
   // assembly targets, block/node contiguous (reference lhsK(25,nnz) order)
-  double* lhs_aos = nullptr;       // 25*nnz
-  float* lhs_aos32 = nullptr;      // 25*nnz (mixed)
   double* acc_res = nullptr;       // 5 per node
   double* acc_bd = nullptr;        // 25 per node
+  double* lhs_aos = nullptr;       // 25*nnz, reference lhsK order
+  float* lhs_aos32 = nullptr;      // 25*nnz (mixed)
   double* nodepack = nullptr;      // 16 per node: y (local order), ac, x
 
   // nodal vectors (SoA planes of nshg)

@@ -66,7 +66,7 @@ class BoxCase:
 
 def box_case(N, mach=0.3, jitter=0.1, grade=1.0, kspace=20, etol=1e-3,
              min_iters=0, mesh_seed=config.MESH_SEED,
-             field_seed=config.FIELD_SEED, ndofBC=10):
+             field_seed=config.FIELD_SEED, ndofBC=10, cfl=config.CFL):
     lib = _lib()
     nshg, nel, nelb = _i64(), _i64(), _i64()
     if lib.phm_box_counts(N, ctypes.byref(nshg), ctypes.byref(nel), ctypes.byref(nelb)):
@@ -90,7 +90,8 @@ def box_case(N, mach=0.3, jitter=0.1, grade=1.0, kspace=20, etol=1e-3,
     yold = np.zeros((nshg, 5), order="F")
     lib.phm_box_state(nshg, x, mach, field_seed, config.RGAS, config.GAMMA, yold)
     BC = np.zeros((nshg, ndofBC), order="F")
-    params = config.solver_params(N, mach, kspace, etol=etol, min_iters=min_iters)
+    params = config.solver_params(N, mach, kspace, etol=etol, min_iters=min_iters,
+                                  grade=grade, cfl=cfl)
     return BoxCase(N=N, mach=mach, x=x, ien=ien, ienb=ienb, iBC=iBC, BC=BC,
                    iper=iper, colm=colm, rowp=rowp, yold=yold, params=params)
 
@@ -108,7 +109,7 @@ def named_case(name, **over):
     c = dict(config.CASES[name])
     c.update(over)
     return box_case(c["N"], mach=c["mach"], jitter=c["jitter"], grade=c["grade"],
-                    kspace=c["kspace"])
+                    kspace=c["kspace"], cfl=c.get("cfl", config.CFL))
 
 
 def expected_nnz(N):

@@ -325,11 +325,16 @@ class GpuSolver:
                                          ctypes.byref(loss) if want_loss else None))
         return loss.value if want_loss else None
 
-    def mpnn_train_krylov(self, nvec, lr):
+    def mpnn_train_krylov(self, nvec, lr, want_grad=False):
+        """Mean residual loss over Krylov vectors 1..nvec of the last solve
+        (one Adam step per vector when lr > 0); with want_grad also the
+        gradient of the last vector's loss."""
         loss = ctypes.c_double()
+        g = np.zeros(lib().phg_mpnn_param_count(), dtype=np.float32) if want_grad else None
+        gp = g.ctypes.data_as(ctypes.c_void_p) if want_grad else None
         _check(lib().phg_mpnn_train_krylov(self.h, int(nvec), ctypes.c_float(lr),
-                                           ctypes.byref(loss)))
-        return loss.value
+                                           ctypes.byref(loss), gp))
+        return (loss.value, g) if want_grad else loss.value
 
     def bench_copy(self, reps=10):
         ms, by = ctypes.c_double(), ctypes.c_double()

@@ -128,6 +128,7 @@ def evaluate(N, weights, seed, states=4):
     gpu = solver.GpuSolver(case)
     gpu.mpnn_init(np.ones(25, dtype=np.float32))
     gpu.mpnn_set_weights(weights)
+    zero = np.zeros_like(weights)
     rows = []
     for _ in range(states):
         mach = float(rng.uniform(0.1, 0.8))
@@ -135,15 +136,21 @@ def evaluate(N, weights, seed, states=4):
         gpu.set_params(**config.solver_params(N, mach, 20))
         res = {}
         for arm, dirs in (("plain", 0), ("first", 1), ("all", 20)):
-            gpu.mpnn_enable(use_pc=dirs, keep_pair=False)
+            gpu.mpnn_enable(use_pc=dirs, keep_pair=(arm == "plain"))
             gpu.upload_state(mesh.box_state(case, mach, fseed))
             gpu.predict()
             st = gpu.newton_iteration()
             res[arm] = (st["iKs"], st["pc_fallback"], st["ms_solve"], bool(st["converged"]))
-        rows.append((mach, res))
-        log(f"eval mach {mach:.2f}: " + ", ".join(
-            f"{a} its {r[0]} fallback {r[1]} ({r[2]:.2f} ms, converged {r[3]})"
-            for a, r in res.items()))
+            if arm == "plain":
+                # residual loss on this solve's Krylov vectors: network vs z = v
+                loss_net = gpu.mpnn_train_krylov(st["iKs"] + 1, 0.0)
+                gpu.mpnn_set_weights(zero)
+                loss_id = gpu.mpnn_train_krylov(st["iKs"] + 1, 0.0)
+                gpu.mpnn_set_weights(weights)
+        rows.append((mach, res, loss_net, loss_id))
+        log(f"eval mach {mach:.2f}: residual loss {loss_net:.4g} (z = v: {loss_id:.4g}); " +
+            ", ".join(f"{a} its {r[0]} fallback {r[1]} ({r[2]:.2f} ms, converged {r[3]})"
+                      for a, r in res.items()))
     gpu.close()
     return rows
 
@@ -163,6 +170,9 @@ def main():
     ap.add_argument("--rounds", type=int, default=6)
     ap.add_argument("--states", type=int, default=8)
     ap.add_argument("--inner", type=int, default=2)
+    ap.add_argument("--res-lr", type=float, default=2e-5,
+                    help="Adam step of the residual-loss phase (the loss is stiff: "
+                         "K amplifies the network's output error)")
     ap.add_argument("--out", help="weights file to write")
     ap.add_argument("--eval", help="only evaluate the weights in this file")
     args = ap.parse_args()
@@ -181,13 +191,13 @@ def main():
         hist = train(ctxs, slots, args.epochs, args.lr, args.seed)
     rhist = []
     if args.mode in ("residual", "both"):
-        rhist = train_residual(ctxs, args.N, args.rounds, args.states, args.inner, args.lr,
+        rhist = train_residual(ctxs, args.N, args.rounds, args.states, args.inner, args.res_lr,
                                args.seed)
     w = owner.mpnn_get_weights()
     meta = dict(N=args.N, meshes=args.meshes, pairs=len(slots), newton=args.newton,
                 epochs=args.epochs, lr=args.lr, seed=args.seed, loss=hist,
                 mode=args.mode, residual_loss=rhist, rounds=args.rounds, states=args.states,
-                inner=args.inner,
+                inner=args.inner, res_lr=args.res_lr,
                 harvest_kspace=HARVEST_K, harvest_etol=HARVEST_ETOL,
                 seconds=time.perf_counter() - t0)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)

@@ -105,7 +105,8 @@ def test_zero_network_fgmres_equals_block_diagonal_gmres():
 def test_residual_loss_matches_host_and_training_reduces_it():
     """The residual loss ||K z(v) - v||^2 / ||v||^2 of the first Krylov vector
     v = b / ||b||, formed on the device, equals the same quantity from the
-    host-array entry points; Adam steps on it reduce it."""
+    host-array entry points; its gradient matches the oracle's backward pass;
+    Adam steps on it reduce it."""
     case = mesh.box_case(4, mach=0.3, kspace=10)
     orc = pyoracle.Oracle(case)
     y, ac = orc.itr_predict_bc(case.yold)
@@ -125,6 +126,21 @@ def test_residual_loss_matches_host_and_training_reduces_it():
         loss_host = np.sum((kz - v) ** 2)
         print(f"[mpnn] residual loss device {loss_dev:.6e} host {loss_host:.6e}", flush=True)
         assert abs(loss_dev - loss_host) <= 1e-6 * loss_host
+        # its gradient: the oracle's backward pass from dL/dy = -2 K^T (v - K z) / s
+        _, grad = gpu.mpnn_train_krylov(1, lr=0.0, want_grad=True)
+        g = M.periodic_graph(case)
+        x, s = M.features(bd, v, np.ones(25), g.active)
+        dz = gpu.sparse_ap_t(out["lhsK"], v - kz)      # K^T (v - K z)
+        dy = np.where(g.active[:, None], -2.0 * dz / s, 0.0)
+        y_o, _ = M.forward(p, g, x)
+        na = int(g.active.sum())
+        ystar = (y_o - 0.5 * na * dy).astype(np.float32)   # 2 (y - y*) / na = dy
+        _, gr, _ = M.loss_and_grad(p, g, x, ystar)
+        gg = solver.mpnn_unflatten(grad)
+        for k, _ in solver.MPNN_SHAPES:
+            e = rel(gg[k], gr[k])
+            print(f"[mpnn] residual-loss grad {k} rel err {e:.2e}", flush=True)
+            assert e < TOL_NET, k
         # training on the Krylov vectors of this solve
         gpu.solgmrs(y, ac)
         l0 = gpu.mpnn_train_krylov(out["iKs"] + 1, lr=0.0)
@@ -135,3 +151,58 @@ def test_residual_loss_matches_host_and_training_reduces_it():
         assert l1 < l0
     finally:
         gpu.close()
+
+
+def test_trained_network_against_block_diagonal_gmres():
+    """What the committed network (phasta_amd/weights/mpnn_box64.npz) gains
+    over the method without it, on a 32^3 mesh seed and states it was not
+    trained on: Krylov iterations and solve time of flexible GMRES with the
+    network on the first direction of each cycle against block-diagonal
+    GMRES, both to the 1e-3 tolerance, and the residual loss ||K z - v||^2
+    of the network against z = v on the Krylov vectors of the plain solve.
+    The numbers are printed (the gain may be negative: the network trained
+    on 64^3 boxes has not beaten block-diagonal GMRES here).  Asserted: both
+    solves converge to the same tolerance (the learned one judged on its true
+    residual), and the fallback bounds the cost: at most one cycle of
+    Kspace directions more than block-diagonal GMRES."""
+    import os
+    import time
+    from phasta_amd import config
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    w = np.load(os.path.join(root, "phasta_amd", "weights", "mpnn_box64.npz"))
+    case = mesh.box_case(32, mach=0.5, jitter=0.1, kspace=20,
+                         mesh_seed=config.MESH_SEED + 4242)
+    gpu = solver.GpuSolver(case)
+    rows = []
+    try:
+        gpu.mpnn_init(w["dscale"])
+        gpu.mpnn_set_weights(w["weights"])
+        zero = np.zeros_like(w["weights"])
+        for mach, seed in ((0.25, 101), (0.55, 202), (0.75, 303)):
+            gpu.set_params(**config.solver_params(32, mach, 20))
+            state = mesh.box_state(case, mach, seed)
+            out = {}
+            for arm, use_pc in (("block", 0), ("learned", 1)):
+                gpu.mpnn_enable(use_pc=use_pc, keep_pair=(arm == "block"))
+                gpu.upload_state(state)
+                gpu.predict()
+                gpu.sync()
+                t0 = time.perf_counter()
+                st = gpu.newton_iteration()
+                gpu.sync()
+                out[arm] = (st, 1e3 * (time.perf_counter() - t0))
+                if arm == "block":
+                    loss_net = gpu.mpnn_train_krylov(st["iKs"] + 1, 0.0)
+                    gpu.mpnn_set_weights(zero)
+                    loss_id = gpu.mpnn_train_krylov(st["iKs"] + 1, 0.0)
+                    gpu.mpnn_set_weights(w["weights"])
+            (sb, tb), (sl, tl) = out["block"], out["learned"]
+            print(f"[ablation] mach {mach}: block-diagonal {sb['iKs']} its {tb:.2f} ms; "
+                  f"learned {sl['iKs']} its {tl:.2f} ms (fallback {sl['pc_fallback']}); "
+                  f"residual loss network {loss_net:.4g} vs z = v {loss_id:.4g}", flush=True)
+            rows.append((sb, sl))
+    finally:
+        gpu.close()
+    for sb, sl in rows:
+        assert sb["converged"] and sl["converged"]
+        assert sl["iKs"] <= sb["iKs"] + 20

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from phasta_amd import mesh
+from phasta_amd import config, mesh
 
 
 @pytest.mark.parametrize("N", [2, 4, 16])
@@ -83,3 +83,16 @@ def test_state_ranges():
     assert np.all(np.abs(s) <= smax)
     assert np.allclose(T / 300 - 1, 0.01 * s)
     assert np.allclose(v, w)
+
+
+@pytest.mark.parametrize("grade", [1.0, 1.1])
+def test_time_step_uses_the_smallest_grid_spacing(grade):
+    """dt = CFL h_min / (c0 (1 + M)), h_min the first wall cell of the graded
+    grid (measured on an unjittered mesh), 1/N on a uniform one."""
+    c = mesh.box_case(16, jitter=0.0, grade=grade, mach=0.3, cfl=3.0)
+    ys = np.unique(np.round(c.x[:, 1], 14))
+    hmin = np.min(np.diff(ys))
+    assert abs(hmin - config.min_spacing(16, grade)) <= 1e-12
+    assert abs(ys[1] - ys[0] - hmin) <= 1e-12          # at the wall
+    dt = 1.0 / c.params["Dtgl"]
+    assert abs(dt - 3.0 * hmin / (config.C0 * 1.3)) <= 1e-12 * dt

@@ -82,3 +82,52 @@ def test_bf16_rounding():
     assert b[3] == 1.0 + 2 ** -6
     assert b[4] == -2.5
     assert np.all(b.view(np.uint32) & 0xFFFF == 0)
+
+
+def test_transpose_of_the_gmres_operator():
+    """The gradient of the residual loss ||K z - v||^2 is -2 K^T (v - K z),
+    K = bc3per o SparseAp the operator of the GMRES cycle.  The device forms
+    K^T x as bc3per(SparseAp^T (G x)), G copying each periodic master's
+    entries to its slaves (SparseAp has slave rows but no slave columns, and
+    bc3per folds slave rows into their masters).  Pinned here on the dense
+    matrices of the oracle's routines; dropping G changes the result."""
+    import pyoracle
+    case = mesh.box_case(3, mach=0.3, kspace=10)
+    orc = pyoracle.Oracle(case)
+    y, ac = orc.itr_predict_bc(case.yold)
+    _, _, _, lk = orc.elmgmrs(y, ac)
+    n, m = case.nshg, 5 * case.nshg
+
+    def sparse_ap(p):
+        q = np.array(p, order="F")
+        orc.sparse_ap(lk, q)
+        return q
+
+    def bc3per(p):
+        q = np.array(p, order="F")
+        orc.bc3per(q)
+        return q
+
+    S = np.zeros((m, m))
+    K = np.zeros((m, m))
+    for j in range(m):
+        e = np.zeros(m)
+        e[j] = 1.0
+        sp = sparse_ap(e.reshape((n, 5), order="F"))
+        S[:, j] = sp.reshape(-1, order="F")
+        K[:, j] = bc3per(sp).reshape(-1, order="F")
+    per = case.iper - 1
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((n, 5))
+
+    def flat(a):
+        return np.asarray(a).reshape(-1, order="F")
+
+    def spt(a):
+        return (S.T @ flat(a)).reshape((n, 5), order="F")
+
+    exact = K.T @ flat(x)
+    gx = x[per]                                  # G: slaves take their master's entries
+    formed = flat(bc3per(spt(gx)))
+    assert np.linalg.norm(formed - exact) <= 1e-13 * np.linalg.norm(exact)
+    assert np.linalg.norm(flat(bc3per(spt(x))) - exact) > 1e-3 * np.linalg.norm(exact)
