@@ -348,7 +348,7 @@ struct BlkArgs {
   const double* resid;
   double* res;             // 5 per node
   double* bdiag;           // 25 per node
-  void* lhs;               // 25 per CSR block (T), reference lhsK order
+  void* lhs;               // one padded block per CSR entry (T), lhsK order
   long long nel, nlanes;
   int lhs_on, prec_on;
   double mu, lam, kap, cfac;
@@ -531,9 +531,20 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
         if (diag && i == j && ((ma >> i) & 1)) sum[i + 5 * j] = (double)total;
       }
   }
-  T* dst = reinterpret_cast<T*>(A.lhs) + 25 * (size_t)L.y;
+  T* dst = reinterpret_cast<T*>(A.lhs) + (size_t)aos_stride<T>() * L.y;
+  if constexpr (sizeof(T) == 4) {
+    float4* v = reinterpret_cast<float4*>(dst);
 #pragma unroll
-  for (int i = 0; i < 25; ++i) dst[i] = (T)sum[i];
+    for (int k = 0; k < 6; ++k)
+      v[k] = make_float4((float)sum[4 * k], (float)sum[4 * k + 1], (float)sum[4 * k + 2],
+                         (float)sum[4 * k + 3]);
+    v[6] = make_float4((float)sum[24], 0.f, 0.f, 0.f);
+  } else {
+    double2* v = reinterpret_cast<double2*>(dst);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = make_double2(sum[2 * k], sum[2 * k + 1]);
+    v[12] = make_double2(sum[24], 0.0);
+  }
 }
 
 // Wall faces: floating convective, pressure, viscous and heat fluxes

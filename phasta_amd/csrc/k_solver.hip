@@ -93,16 +93,20 @@ __global__ void k_lu_backward(const double* D, double* r, long long n) {
 
 // ---------------- two-sided block scaling (spsi3pre.f) ----------------
 // lhsK(i,j) <- L_i^{-1} lhsK(i,j) U_j^{-1} with the factors of BDiag.  Reads
-// the assembled blocks (reference order, 25 contiguous values per block) and
-// writes the 64-row interleaved matvec layout.
+// the assembled blocks (reference order, one padded record per block, 16-byte
+// loads) and writes the 64-row interleaved matvec layout.
+// One workgroup of kPreWaves wavefronts per 64-row group; wavefront w takes
+// the group's slots w, w + kPreWaves, ... (more independent loads in flight
+// than one wavefront walking all slots of its group).
+constexpr int kPreWaves = 4;
+
 template <typename TI, typename TO>
-__global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO* __restrict__ vals,
-                                                 const int* __restrict__ s2c,
-                                                 const int* __restrict__ scol,
-                                                 const long long* __restrict__ grp_slot,
-                                                 const double* __restrict__ D, long long n) {
+__global__ void __launch_bounds__(kPreWaves * kGroup) k_spsi3pre(
+    const TI* __restrict__ aos, TO* __restrict__ vals, const int* __restrict__ s2c,
+    const int* __restrict__ scol, const long long* __restrict__ grp_slot,
+    const double* __restrict__ D, long long n) {
   const long long g = blockIdx.x;
-  const int r = threadIdx.x;
+  const int w = threadIdx.x / kGroup, r = threadIdx.x % kGroup;
   const long long row = g * kGroup + r;
   if (row >= n) return;
   double lo[5][5];
@@ -110,16 +114,35 @@ __global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO*
   for (int i = 1; i < 5; ++i)
 #pragma unroll
     for (int m = 0; m < i; ++m) lo[i][m] = D[(i + 5 * m) * n + row];
-  for (long long s = grp_slot[g]; s < grp_slot[g + 1]; ++s) {
+  for (long long s = grp_slot[g] + w; s < grp_slot[g + 1]; s += kPreWaves) {
     const long long k = s2c[s * kGroup + r];
     if (k < 0) continue;  // padding stays zero
     const long long j = scol[s * kGroup + r];
-    const TI* src = aos + 25 * k;
+    double a[28];
+    if constexpr (sizeof(TI) == 4) {
+      const float4* v = reinterpret_cast<const float4*>(aos + (size_t)aos_stride<TI>() * k);
+#pragma unroll
+      for (int t = 0; t < 7; ++t) {
+        const float4 x = v[t];
+        a[4 * t] = x.x;
+        a[4 * t + 1] = x.y;
+        a[4 * t + 2] = x.z;
+        a[4 * t + 3] = x.w;
+      }
+    } else {
+      const double2* v = reinterpret_cast<const double2*>(aos + (size_t)aos_stride<TI>() * k);
+#pragma unroll
+      for (int t = 0; t < 13; ++t) {
+        const double2 x = v[t];
+        a[2 * t] = x.x;
+        a[2 * t + 1] = x.y;
+      }
+    }
     double b[5][5];
 #pragma unroll
     for (int c = 0; c < 5; ++c)
 #pragma unroll
-      for (int f = 0; f < 5; ++f) b[f][c] = (double)src[f + 5 * c];
+      for (int f = 0; f < 5; ++f) b[f][c] = a[f + 5 * c];
     // left: L_i^{-1} on every column
 #pragma unroll
     for (int c = 0; c < 5; ++c)
@@ -145,7 +168,6 @@ __global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO*
         for (int m = 0; m < c; ++m) acc -= up[m][c] * b[f][m];
         b[f][c] = up[c][c] * acc;
       }
-    double a[25];
 #pragma unroll
     for (int c = 0; c < 5; ++c)
 #pragma unroll
@@ -155,45 +177,53 @@ __global__ void __launch_bounds__(64) k_spsi3pre(const TI* __restrict__ aos, TO*
 }
 
 // ---------------- block-CSR matvec (sparseap.f) ----------------
-// q = K p(iper(:)); one lane per row, one wavefront per 64-row group.
+// q = K p(iper(:)); one lane per row of a 64-row group in each wavefront.
 // T: stored value type; Acc: type of the products and row sums.  The mixed
 // path stores float values and sums rows in double: float row sums of the
 // scaled system lose digits to cancellation (on the 16^3 box the refinement
 // loop then needed 12 instead of 10 Krylov iterations), while the kernel is
 // bound by the matrix bytes, which float storage halves either way.
-// Launched on a grid that is a multiple of 8: consecutive workgroups go to
-// different XCDs (round robin), so workgroup b takes row range
-// (b % 8) * per + b / 8 and each XCD sweeps one contiguous eighth of the
-// rows, whose column neighbours (rows +-1, +-(N+1), +-(N+1)^2 on the box)
-// then stay in that XCD's L2.
+// One workgroup of kSpmvWaves wavefronts per 64-row group: wavefront w takes
+// the group's slots w, w + kSpmvWaves, ..., so the wavefronts of a workgroup
+// stream adjacent slots together and each runs a short loop; the partial row
+// sums are added in wavefront order through LDS (fixed order, deterministic).
+// The grid is a multiple of 8: consecutive workgroups go to different XCDs
+// (round robin), so workgroup b takes group (b % 8) * per + b / 8 and each
+// XCD sweeps one contiguous eighth of the rows, whose column neighbours
+// (rows +-1, +-(N+1), +-(N+1)^2 on the box) then stay in that XCD's L2.
+constexpr int kSpmvWaves = 4;
+
 template <typename T, typename Acc>
-__global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const int* __restrict__ scolp,
-                                              const long long* __restrict__ grp_slot,
-                                              const double* __restrict__ p, double* __restrict__ q,
-                                              long long n) {
+__global__ void __launch_bounds__(kSpmvWaves * kGroup) k_spmv(
+    const T* __restrict__ vals, const int* __restrict__ scolp,
+    const long long* __restrict__ grp_slot, const double* __restrict__ p, double* __restrict__ q,
+    long long n, long long ngroups) {
+  __shared__ Acc part[kSpmvWaves][5][kGroup];
   const long long per = (gridDim.x + 7) / 8;
-  const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  const long long row = wg * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  const long long g = row / kGroup;
-  const int r = (int)(row % kGroup);
+  const long long g = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (g >= ngroups) return;  // whole workgroup
+  const int w = threadIdx.x / kGroup, r = threadIdx.x % kGroup;
+  const long long row = g * kGroup + r;
+  const long long s0 = grp_slot[g] + w, s1 = grp_slot[g + 1];
   Acc t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
-  const long long s0 = grp_slot[g], s1 = grp_slot[g + 1];
-  // software pipeline: the gathers of p for slot s + 1 and the column index
-  // of slot s + 2 are issued before the block of slot s is consumed
+  // software pipeline: the gathers of p for the next slot and the column
+  // index of the one after are issued before the block of this slot is
+  // consumed (rows past n in the last group read padding: column 0, zero
+  // blocks)
+  constexpr int W = kSpmvWaves;
   Acc pc[5] = {0, 0, 0, 0, 0};
   if (s0 < s1) {
     const long long j0 = scolp[s0 * kGroup + r];
 #pragma unroll
     for (int f = 0; f < 5; ++f) pc[f] = (Acc)p[j0 + f * n];
   }
-  long long jn = s0 + 1 < s1 ? scolp[(s0 + 1) * kGroup + r] : 0;
-  for (long long s = s0; s < s1; ++s) {
+  long long jn = s0 + W < s1 ? scolp[(s0 + W) * kGroup + r] : 0;
+  for (long long s = s0; s < s1; s += W) {
     Acc pn[5] = {0, 0, 0, 0, 0};
-    if (s + 1 < s1) {
+    if (s + W < s1) {
 #pragma unroll
       for (int f = 0; f < 5; ++f) pn[f] = (Acc)p[jn + f * n];
-      if (s + 2 < s1) jn = scolp[(s + 2) * kGroup + r];
+      if (s + 2 * W < s1) jn = scolp[(s + 2 * W) * kGroup + r];
     }
     Acc a[25];
     load_slot(vals + s * (25 * kGroup), r, a);
@@ -205,11 +235,20 @@ __global__ void __launch_bounds__(256) k_spmv(const T* __restrict__ vals, const 
 #pragma unroll
     for (int f = 0; f < 5; ++f) pc[f] = pn[f];
   }
-  q[row] = (double)t0;
-  q[row + n] = (double)t1;
-  q[row + 2 * n] = (double)t2;
-  q[row + 3 * n] = (double)t3;
-  q[row + 4 * n] = (double)t4;
+  part[w][0][r] = t0;
+  part[w][1][r] = t1;
+  part[w][2][r] = t2;
+  part[w][3][r] = t3;
+  part[w][4][r] = t4;
+  __syncthreads();
+  if (w != 0 || row >= n) return;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    Acc v = part[0][f][r];
+#pragma unroll
+    for (int k = 1; k < W; ++k) v += part[k][f][r];
+    q[row + f * n] = (double)v;
+  }
 }
 
 // Transpose matvec without the periodic maps: q_j = sum_i B_ij^T x_i over
@@ -282,12 +321,22 @@ __global__ void k_copy(double2* __restrict__ dst, const double2* __restrict__ sr
   for (; i < n2; i += stride) dst[i] = src[i];
 }
 
-// float -> double copy
-__global__ void k_widen(double* dst, const float* src, long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    dst[i] = (double)src[i];
+// padded assembled blocks -> lhsK(25,nnz) (double) and back
+template <typename T>
+__global__ void k_aos_to_ref(const T* __restrict__ aos, double* __restrict__ ref, long long nnz) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= 25 * nnz) return;
+  const long long k = i / 25;
+  ref[i] = (double)aos[(size_t)aos_stride<T>() * k + (i - 25 * k)];
 }
+__global__ void k_ref_to_aos(const double* __restrict__ ref, double* __restrict__ aos,
+                             long long nnz) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= (long long)aos_stride<double>() * nnz) return;
+  const long long k = i / aos_stride<double>(), e = i - (long long)aos_stride<double>() * k;
+  aos[i] = e < 25 ? ref[25 * k + e] : 0.0;
+}
+
 
 // bc3per: master += slaves (ascending), slaves zeroed
 __global__ void k_fold_vec(double* v, const int* master, const int* ptr, const int* slave,
@@ -571,12 +620,12 @@ int launch_i3lu(phg_ctx* c, double* diag, double* r, int code) {
 }
 
 int launch_spsi3pre(phg_ctx* c) {
-  const int g = (int)c->ngroups;
+  const int g = (int)c->ngroups, nt = kPreWaves * kGroup;
   if (c->precision == PHG_MIXED)
-    k_spsi3pre<float, float><<<g, kGroup, 0, c->stream>>>(
+    k_spsi3pre<float, float><<<g, nt, 0, c->stream>>>(
         c->lhs_aos32, c->lhs32, c->slot2csr, c->scol, c->grp_slot, c->bdiag, c->nshg);
   else
-    k_spsi3pre<double, double><<<g, kGroup, 0, c->stream>>>(
+    k_spsi3pre<double, double><<<g, nt, 0, c->stream>>>(
         c->lhs_aos, c->lhs64, c->slot2csr, c->scol, c->grp_slot, c->bdiag, c->nshg);
   PHG_HIP(hipGetLastError());
   return 0;
@@ -584,12 +633,14 @@ int launch_spsi3pre(phg_ctx* c) {
 
 int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact) {
   (void)exact;  // both matvecs of the mixed path form double row sums
-  const int g = (grid_for(c->nshg, 256) + 7) / 8 * 8;  // XCD-strided rows
+  const int g = (int)((c->ngroups + 7) / 8 * 8);  // one workgroup per group, XCD-strided
+  const int nt = kSpmvWaves * kGroup;
   if (c->precision == PHG_MIXED)
-    k_spmv<float, double><<<g, 256, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg);
+    k_spmv<float, double><<<g, nt, 0, c->stream>>>(c->lhs32, c->scolp, c->grp_slot, p, q, c->nshg,
+                                                   c->ngroups);
   else
-    k_spmv<double, double><<<g, 256, 0, c->stream>>>(c->lhs64, c->scolp, c->grp_slot, p, q,
-                                                     c->nshg);
+    k_spmv<double, double><<<g, nt, 0, c->stream>>>(c->lhs64, c->scolp, c->grp_slot, p, q,
+                                                    c->nshg, c->ngroups);
   PHG_HIP(hipGetLastError());
   return 0;
 }
@@ -634,9 +685,19 @@ int launch_copy(phg_ctx* c, double* dst, const double* src, long long n) {
   return 0;
 }
 
-int launch_widen(phg_ctx* c, double* dst, const float* src, long long n) {
-  int g = (int)std::min<long long>(grid_for(n, 256), 16384);
-  k_widen<<<g, 256, 0, c->stream>>>(dst, src, n);
+int launch_aos_to_ref(phg_ctx* c, double* ref) {
+  const long long n = 25 * c->nnz;
+  if (c->precision == PHG_MIXED)
+    k_aos_to_ref<float><<<grid_for(n, 256), 256, 0, c->stream>>>(c->lhs_aos32, ref, c->nnz);
+  else
+    k_aos_to_ref<double><<<grid_for(n, 256), 256, 0, c->stream>>>(c->lhs_aos, ref, c->nnz);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_ref_to_aos(phg_ctx* c, const double* ref) {
+  const long long n = (long long)aos_stride<double>() * c->nnz;
+  k_ref_to_aos<<<grid_for(n, 256), 256, 0, c->stream>>>(ref, c->lhs_aos, c->nnz);
   PHG_HIP(hipGetLastError());
   return 0;
 }

@@ -362,9 +362,9 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->slot2csr, c->nslots * kGroup));
   PHG_TRY(dev_alloc(c, &c->acc_res, 5 * nshg));
   if (mixed)
-    PHG_TRY(dev_alloc(c, &c->lhs_aos32, 25 * nnz));
+    PHG_TRY(dev_alloc(c, &c->lhs_aos32, (size_t)aos_stride<float>() * nnz));
   else
-    PHG_TRY(dev_alloc(c, &c->lhs_aos, 25 * nnz));
+    PHG_TRY(dev_alloc(c, &c->lhs_aos, (size_t)aos_stride<double>() * nnz));
   PHG_TRY(dev_alloc(c, &c->acc_bd, 25 * nshg));
   PHG_TRY(dev_alloc(c, &c->nodepack, 16 * nshg));
   if (mixed)
@@ -805,13 +805,9 @@ extern "C" int phg_elmgmrs(phg_ctx* c, const double* y, const double* ac, double
   if (rmes) std::memset(rmes, 0, sizeof(double) * 5 * n);  // elmgmr.f:400
   if (BDiag && c->prm.iprec != 0) PHG_TRY(download(c, BDiag, c->bdiag, 25 * n));
   if (lhsK && c->prm.lhs == 1) {
-    if (c->precision == PHG_MIXED) {
-      PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
-      PHG_TRY(launch_widen(c, c->tmp, c->lhs_aos32, 25 * c->nnz));
-      PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
-    } else {
-      PHG_TRY(download(c, lhsK, c->lhs_aos, 25 * c->nnz));
-    }
+    PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
+    PHG_TRY(launch_aos_to_ref(c, c->tmp));
+    PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
   }
   return sync(c);
 }
@@ -946,7 +942,8 @@ extern "C" int phg_spsi3pre(phg_ctx* c, const double* BDiag, double* lhsK) {
   PHG_TRY(require_fp64(c, "phg_spsi3pre"));
   PHG_TRY(upload(c, c->bdiag, BDiag, 25 * c->nshg));
   PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
-  PHG_TRY(upload(c, c->lhs_aos, lhsK, 25 * c->nnz));
+  PHG_TRY(upload(c, c->tmp, lhsK, 25 * c->nnz));
+  PHG_TRY(launch_ref_to_aos(c, c->tmp));
   PHG_TRY(launch_spsi3pre(c));
   PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
   PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));

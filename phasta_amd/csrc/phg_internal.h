@@ -45,32 +45,42 @@ __host__ __device__ constexpr int slot_pos(int e, int r) {
                 : 24 * kGroup + r;
 }
 
+// Assembled blocks in CSR order (the reference's lhsK(25,nnz)), each padded
+// to a multiple of 16 bytes (28 floats, 26 doubles) so that one lane moves
+// its block with 16-byte vector accesses; the pad entries are zero.
+template <typename T>
+__host__ __device__ constexpr int aos_stride() { return sizeof(T) == 4 ? 28 : 26; }
+
 // The 25 entries of row r's block in one slot (blk = slot base), read and
 // written with 16-byte vector accesses (6 float4 or 12 double2) and one
-// scalar for entry 24.
+// scalar for entry 24.  The matvec streams the matrix once per product and
+// it does not fit in the caches, so the reads are non-temporal: they do not
+// displace the gathered vector from L2 and the MALL.
+typedef float phg_f4 __attribute__((ext_vector_type(4)));
+typedef double phg_d2 __attribute__((ext_vector_type(2)));
 template <typename A>
 __device__ inline void load_slot(const float* blk, int r, A a[25]) {
-  const float4* v = reinterpret_cast<const float4*>(blk);
+  const phg_f4* v = reinterpret_cast<const phg_f4*>(blk);
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    const float4 t = v[k * kGroup + r];
+    const phg_f4 t = __builtin_nontemporal_load(v + k * kGroup + r);
     a[4 * k] = (A)t.x;
     a[4 * k + 1] = (A)t.y;
     a[4 * k + 2] = (A)t.z;
     a[4 * k + 3] = (A)t.w;
   }
-  a[24] = (A)blk[24 * kGroup + r];
+  a[24] = (A)__builtin_nontemporal_load(blk + 24 * kGroup + r);
 }
 template <typename A>
 __device__ inline void load_slot(const double* blk, int r, A a[25]) {
-  const double2* v = reinterpret_cast<const double2*>(blk);
+  const phg_d2* v = reinterpret_cast<const phg_d2*>(blk);
 #pragma unroll
   for (int k = 0; k < 12; ++k) {
-    const double2 t = v[k * kGroup + r];
+    const phg_d2 t = __builtin_nontemporal_load(v + k * kGroup + r);
     a[2 * k] = (A)t.x;
     a[2 * k + 1] = (A)t.y;
   }
-  a[24] = (A)blk[24 * kGroup + r];
+  a[24] = (A)__builtin_nontemporal_load(blk + 24 * kGroup + r);
 }
 template <typename A>
 __device__ inline void store_slot(float* blk, int r, const A a[25]) {
@@ -211,8 +221,8 @@ struct phg_ctx {
   // assembly targets, block/node contiguous (reference lhsK(25,nnz) order)
   double* acc_res = nullptr;       // 5 per node
   double* acc_bd = nullptr;        // 25 per node
-  double* lhs_aos = nullptr;       // 25*nnz, reference lhsK order
-  float* lhs_aos32 = nullptr;      // 25*nnz (mixed)
+  double* lhs_aos = nullptr;       // aos_stride<double>()*nnz, reference lhsK order
+  float* lhs_aos32 = nullptr;      // aos_stride<float>()*nnz (mixed)
   double* nodepack = nullptr;      // 16 per node: y (local order), ac, x
 
   // nodal vectors (SoA planes of nshg)
@@ -281,7 +291,9 @@ int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact = false);
 int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n);
 // q = K^T x (x overwritten by its periodic gather); needs c->tslot
 int launch_spmv_t(phg_ctx* c, double* x, double* q);
-int launch_widen(phg_ctx* c, double* dst, const float* src, long long n);
+// padded assembled blocks <-> lhsK(25,nnz) in double
+int launch_aos_to_ref(phg_ctx* c, double* ref);
+int launch_ref_to_aos(phg_ctx* c, const double* ref);
 // n doubles, n even, 16-byte aligned
 int launch_copy(phg_ctx* c, double* dst, const double* src, long long n);
 int launch_bc3per(phg_ctx* c, double* v, int nq);

@@ -36,14 +36,24 @@ def log(msg):
     print(f"[train_pc {time.strftime('%H:%M:%S')}] {msg}", flush=True)
 
 
-def harvest(N, meshes, pairs, newton, seed):
+def step_params(N, mach, kspace, grade, etol=1e-3):
+    """Solver scalars of a box: CFL 1 on h = 1/N, or CFL 3 on the first wall
+    cell of a graded box (the rule of the graded workloads)."""
+    return config.solver_params(N, mach, kspace, etol=etol, grade=grade,
+                                cfl=3.0 if grade > 1.0 else 1.0)
+
+
+def harvest(N, meshes, pairs, newton, seed, grade_max=1.0):
     rng = np.random.default_rng(seed)
     per_mesh = pairs // meshes
     ctxs, slots = [], []
     for m in range(meshes):
         jitter = float(rng.uniform(0.0, 0.15))
-        case = mesh.box_case(N, mach=0.5, jitter=jitter, kspace=HARVEST_K, etol=HARVEST_ETOL,
-                             mesh_seed=config.MESH_SEED + 1000 + m)
+        grade = float(rng.uniform(1.0, grade_max)) if grade_max > 1.0 else 1.0
+        case = mesh.box_case(N, mach=0.5, jitter=jitter, grade=grade, kspace=HARVEST_K,
+                             etol=HARVEST_ETOL, mesh_seed=config.MESH_SEED + 1000 + m,
+                             cfl=3.0 if grade > 1.0 else 1.0)
+        case.grade = grade
         gpu = solver.GpuSolver(case)
         gpu.mpnn_init(np.ones(25, dtype=np.float32))
         if ctxs:
@@ -55,7 +65,7 @@ def harvest(N, meshes, pairs, newton, seed):
         while k < per_mesh:
             mach = float(rng.uniform(0.1, 0.8))
             fseed = int(rng.integers(1, 2 ** 31))
-            gpu.set_params(**config.solver_params(N, mach, HARVEST_K, etol=HARVEST_ETOL))
+            gpu.set_params(**step_params(N, mach, HARVEST_K, grade, etol=HARVEST_ETOL))
             gpu.upload_state(mesh.box_state(case, mach, fseed))
             gpu.predict()
             for _ in range(newton):
@@ -68,7 +78,7 @@ def harvest(N, meshes, pairs, newton, seed):
                 k += 1
         gpu.sync()
         ctxs.append((case, gpu))
-        log(f"mesh {m}: jitter {jitter:.3f}, {per_mesh} pairs, Krylov its "
+        log(f"mesh {m}: jitter {jitter:.3f}, grade {grade:.3f}, {per_mesh} pairs, Krylov its "
             f"mean {np.mean(iters):.1f} max {max(iters)}")
     return ctxs, slots
 
@@ -108,7 +118,7 @@ def train_residual(ctxs, N, rounds, states, inner, lr, seed):
             for _ in range(states):
                 mach = float(rng.uniform(0.1, 0.8))
                 fseed = int(rng.integers(1, 2 ** 31))
-                gpu.set_params(**config.solver_params(N, mach, 20))
+                gpu.set_params(**step_params(N, mach, 20, case.grade))
                 gpu.upload_state(mesh.box_state(case, mach, fseed))
                 gpu.predict()
                 st = gpu.newton_iteration()
@@ -120,11 +130,12 @@ def train_residual(ctxs, N, rounds, states, inner, lr, seed):
     return hist
 
 
-def evaluate(N, weights, seed, states=4):
+def evaluate(N, weights, seed, states=4, grade=1.0):
     """Plain block-diagonal GMRES vs FGMRES with the network on a mesh and
     states not used in training (Kspace 20, tolerance 1e-3)."""
     rng = np.random.default_rng(seed + 7)
-    case = mesh.box_case(N, mach=0.5, jitter=0.1, kspace=20, mesh_seed=config.MESH_SEED + 9999)
+    case = mesh.box_case(N, mach=0.5, jitter=0.1, grade=grade, kspace=20,
+                         mesh_seed=config.MESH_SEED + 9999, cfl=3.0 if grade > 1.0 else 1.0)
     gpu = solver.GpuSolver(case)
     gpu.mpnn_init(np.ones(25, dtype=np.float32))
     gpu.mpnn_set_weights(weights)
@@ -133,7 +144,7 @@ def evaluate(N, weights, seed, states=4):
     for _ in range(states):
         mach = float(rng.uniform(0.1, 0.8))
         fseed = int(rng.integers(1, 2 ** 31))
-        gpu.set_params(**config.solver_params(N, mach, 20))
+        gpu.set_params(**step_params(N, mach, 20, grade))
         res = {}
         for arm, dirs in (("plain", 0), ("first", 1), ("all", 20)):
             gpu.mpnn_enable(use_pc=dirs, keep_pair=(arm == "plain"))
@@ -173,16 +184,22 @@ def main():
     ap.add_argument("--res-lr", type=float, default=2e-5,
                     help="Adam step of the residual-loss phase (the loss is stiff: "
                          "K amplifies the network's output error)")
+    ap.add_argument("--grade-max", type=float, default=1.0,
+                    help="y-grading ratio of the training meshes ~ U[1, grade-max]")
+    ap.add_argument("--eval-N", type=int, default=None, help="box size of the evaluation")
+    ap.add_argument("--eval-grade", type=float, default=1.0, help="y-grading of the evaluation box")
     ap.add_argument("--out", help="weights file to write")
     ap.add_argument("--eval", help="only evaluate the weights in this file")
     args = ap.parse_args()
+    eval_n = args.eval_N or args.N
     if args.eval:
-        evaluate(args.N, np.load(args.eval)["weights"], args.seed)
+        evaluate(eval_n, np.load(args.eval)["weights"], args.seed, grade=args.eval_grade)
         return
     if not args.out:
         ap.error("--out is required for training")
     t0 = time.perf_counter()
-    ctxs, slots = harvest(args.N, args.meshes, args.pairs, args.newton, args.seed)
+    ctxs, slots = harvest(args.N, args.meshes, args.pairs, args.newton, args.seed,
+                          args.grade_max)
     log(f"harvested {len(slots)} pairs in {time.perf_counter() - t0:.1f} s")
     owner = ctxs[0][1]
     owner.mpnn_set_weights(solver.mpnn_flatten(init_params(args.seed)))
@@ -197,14 +214,14 @@ def main():
     meta = dict(N=args.N, meshes=args.meshes, pairs=len(slots), newton=args.newton,
                 epochs=args.epochs, lr=args.lr, seed=args.seed, loss=hist,
                 mode=args.mode, residual_loss=rhist, rounds=args.rounds, states=args.states,
-                inner=args.inner, res_lr=args.res_lr,
+                inner=args.inner, res_lr=args.res_lr, grade_max=args.grade_max,
                 harvest_kspace=HARVEST_K, harvest_etol=HARVEST_ETOL,
                 seconds=time.perf_counter() - t0)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     np.savez(args.out, weights=w, dscale=np.ones(25, dtype=np.float32),
              meta=json.dumps(meta))
     log(f"wrote {args.out}: {json.dumps(meta)}")
-    evaluate(args.N, w, args.seed)
+    evaluate(eval_n, w, args.seed, grade=args.eval_grade)
     # shared parameters belong to ctxs[0]: release the others first
     for _, g in ctxs[1:]:
         g.close()
