@@ -617,7 +617,7 @@ int sol_core(phg_ctx* c, phg_stats* st) {
     };
     bool plain_from_start = !learned;
     if (learned) {
-      // FGMRES cycle with the network; block-diagonal GMRES judges it
+      // FGMRES cycle with the network; block-diagonal GMRES is its fallback and yardstick
       CycleOut co;
       PHG_TRY(gmres_cycle(c, rnorm, epsnrm, st, &co, true));
       iKs += co.iKs;

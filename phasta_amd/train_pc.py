@@ -104,10 +104,11 @@ def train(ctxs, slots, epochs, lr, seed):
     return history
 
 
-def train_residual(ctxs, N, rounds, states, inner, lr, seed):
+def train_residual(ctxs, N, rounds, states, inner, lr, seed, newton=1):
     """Fine-tuning on the residual loss ||K z(v) - v||^2 / ||v||^2 over the
     Krylov vectors of fresh block-diagonal solves (the directions flexible
-    GMRES hands to the network), K the scaled matrix of each solve."""
+    GMRES hands to the network), K the scaled matrix of each solve; the
+    first `newton` Newton iterations of every state are used."""
     rng = np.random.default_rng(seed + 3)
     hist = []
     for r in range(rounds):
@@ -121,18 +122,20 @@ def train_residual(ctxs, N, rounds, states, inner, lr, seed):
                 gpu.set_params(**step_params(N, mach, 20, case.grade))
                 gpu.upload_state(mesh.box_state(case, mach, fseed))
                 gpu.predict()
-                st = gpu.newton_iteration()
                 lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * r / max(1, rounds)))
-                for _ in range(inner):
-                    losses.append(gpu.mpnn_train_krylov(st["iKs"] + 1, lr_t))
+                for _ in range(newton):
+                    st = gpu.newton_iteration()
+                    for _ in range(inner):
+                        losses.append(gpu.mpnn_train_krylov(st["iKs"] + 1, lr_t))
         hist.append(float(np.mean(losses)))
         log(f"residual round {r}: mean loss {hist[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
     return hist
 
 
-def evaluate(N, weights, seed, states=4, grade=1.0):
-    """Plain block-diagonal GMRES vs FGMRES with the network on a mesh and
-    states not used in training (Kspace 20, tolerance 1e-3)."""
+def evaluate(N, weights, seed, states=4, grade=1.0, newton=3):
+    """Plain block-diagonal GMRES vs FGMRES with the network on the first
+    direction of every cycle, on a mesh and states not used in training
+    (Kspace 20, tolerance 1e-3, `newton` Newton iterations per state)."""
     rng = np.random.default_rng(seed + 7)
     case = mesh.box_case(N, mach=0.5, jitter=0.1, grade=grade, kspace=20,
                          mesh_seed=config.MESH_SEED + 9999, cfl=3.0 if grade > 1.0 else 1.0)
@@ -146,21 +149,27 @@ def evaluate(N, weights, seed, states=4, grade=1.0):
         fseed = int(rng.integers(1, 2 ** 31))
         gpu.set_params(**step_params(N, mach, 20, grade))
         res = {}
-        for arm, dirs in (("plain", 0), ("first", 1), ("all", 20)):
+        for arm, dirs in (("plain", 0), ("first", 1)):
             gpu.mpnn_enable(use_pc=dirs, keep_pair=(arm == "plain"))
             gpu.upload_state(mesh.box_state(case, mach, fseed))
             gpu.predict()
-            st = gpu.newton_iteration()
-            res[arm] = (st["iKs"], st["pc_fallback"], st["ms_solve"], bool(st["converged"]))
-            if arm == "plain":
-                # residual loss on this solve's Krylov vectors: network vs z = v
-                loss_net = gpu.mpnn_train_krylov(st["iKs"] + 1, 0.0)
-                gpu.mpnn_set_weights(zero)
-                loss_id = gpu.mpnn_train_krylov(st["iKs"] + 1, 0.0)
-                gpu.mpnn_set_weights(weights)
+            its, fb, ms, conv = [], [], 0.0, True
+            for k in range(newton):
+                st = gpu.newton_iteration()
+                its.append(st["iKs"])
+                fb.append(st["pc_fallback"])
+                ms += st["ms_solve"]
+                conv = conv and bool(st["converged"])
+                if arm == "plain" and k == 0:
+                    # residual loss on this solve's Krylov vectors: network vs z = v
+                    loss_net = gpu.mpnn_train_krylov(st["iKs"] + 1, 0.0)
+                    gpu.mpnn_set_weights(zero)
+                    loss_id = gpu.mpnn_train_krylov(st["iKs"] + 1, 0.0)
+                    gpu.mpnn_set_weights(weights)
+            res[arm] = (its, fb, ms, conv)
         rows.append((mach, res, loss_net, loss_id))
         log(f"eval mach {mach:.2f}: residual loss {loss_net:.4g} (z = v: {loss_id:.4g}); " +
-            ", ".join(f"{a} its {r[0]} fallback {r[1]} ({r[2]:.2f} ms, converged {r[3]})"
+            ", ".join(f"{a} its {r[0]} fallback {r[1]} ({r[2]:.2f} ms solve, converged {r[3]})"
                       for a, r in res.items()))
     gpu.close()
     return rows
@@ -181,6 +190,8 @@ def main():
     ap.add_argument("--rounds", type=int, default=6)
     ap.add_argument("--states", type=int, default=8)
     ap.add_argument("--inner", type=int, default=2)
+    ap.add_argument("--res-newton", type=int, default=1,
+                    help="Newton iterations per state in the residual phase")
     ap.add_argument("--res-lr", type=float, default=2e-5,
                     help="Adam step of the residual-loss phase (the loss is stiff: "
                          "K amplifies the network's output error)")
@@ -209,12 +220,13 @@ def main():
     rhist = []
     if args.mode in ("residual", "both"):
         rhist = train_residual(ctxs, args.N, args.rounds, args.states, args.inner, args.res_lr,
-                               args.seed)
+                               args.seed, args.res_newton)
     w = owner.mpnn_get_weights()
     meta = dict(N=args.N, meshes=args.meshes, pairs=len(slots), newton=args.newton,
                 epochs=args.epochs, lr=args.lr, seed=args.seed, loss=hist,
                 mode=args.mode, residual_loss=rhist, rounds=args.rounds, states=args.states,
-                inner=args.inner, res_lr=args.res_lr, grade_max=args.grade_max,
+                inner=args.inner, res_lr=args.res_lr, res_newton=args.res_newton,
+                grade_max=args.grade_max,
                 harvest_kspace=HARVEST_K, harvest_etol=HARVEST_ETOL,
                 seconds=time.perf_counter() - t0)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)

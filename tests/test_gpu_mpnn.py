@@ -162,7 +162,7 @@ def test_trained_network_against_block_diagonal_gmres():
     of the network against z = v on the Krylov vectors of the plain solve.
     The numbers are printed (the gain may be negative: the network trained
     on 64^3 boxes has not beaten block-diagonal GMRES here).  Asserted: both
-    solves converge to the same tolerance (the learned one judged on its true
+    solves converge to the same tolerance (the learned one tested on its true
     residual), and the fallback bounds the cost: at most one cycle of
     Kspace directions more than block-diagonal GMRES."""
     import os

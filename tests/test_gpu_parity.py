@@ -153,3 +153,34 @@ def test_gpu_against_reference_golden(N):
         assert rel(aco, g["acold_upd"]) < TOL
     finally:
         gpu.close()
+
+
+def test_box64_newton_iteration_against_oracle():
+    """The 64^3 workload (1.57 M tets, Mach 0.5, GMRES(20)): one FP64 Newton
+    iteration on the GPU against the oracle's, to the same 1e-12 bar and
+    Krylov count; the mixed-precision update against the FP64 one."""
+    case = mesh.named_case("box64")
+    print(f"[parity] box64: nshg={case.nshg} nel={case.nel}", flush=True)
+    orc = pyoracle.Oracle(case)
+    acold = np.zeros((case.nshg, 5), order="F")
+    y, ac = orc.itr_predict_bc(case.yold, acold)
+    ref = orc.solgmrs(y, ac)
+    out = {}
+    for prec in (solver.PHG_FP64, solver.PHG_MIXED):
+        gpu = solver.GpuSolver(case, precision=prec)
+        try:
+            gpu.upload_state(case.yold, acold)
+            gpu.predict()
+            st = gpu.newton_iteration()
+            _, _, dy = gpu.download_state()
+        finally:
+            gpu.close()
+        out[prec] = (st, dy)
+    st64, dy64 = out[solver.PHG_FP64]
+    stmx, dymx = out[solver.PHG_MIXED]
+    e64, emx = rel(dy64, ref["Dy"]), rel(dymx, dy64)
+    print(f"[parity] box64 FP64 Dy rel err {e64:.2e} (iKs {st64['iKs']} vs {ref['iKs']}); "
+          f"mixed vs FP64 {emx:.2e} (iKs {stmx['iKs']})", flush=True)
+    assert st64["iKs"] == ref["iKs"]
+    assert e64 < TOL
+    assert stmx["converged"] and emx < 1e-5
