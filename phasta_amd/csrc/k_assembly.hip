@@ -138,9 +138,12 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   double xl[4][3];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const double* v = A.node + 16 * (size_t)nd[k];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) xl[k][c] = v[10 + c];
+    // node lines are 128-byte records: 16-byte loads
+    const double2* v = reinterpret_cast<const double2*>(A.node + 16 * (size_t)nd[k]);
+    const double2 x01 = v[5], x2 = v[6];
+    xl[k][0] = x01.x;
+    xl[k][1] = x01.y;
+    xl[k][2] = x2.x;
   }
   const PhysParams& ph = A.ph;
 
@@ -219,7 +222,14 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const double* v = A.node + 16 * (size_t)nd[k];
+    const double2* vp = reinterpret_cast<const double2*>(A.node + 16 * (size_t)nd[k]);
+    double v[10];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      const double2 u = vp[t];
+      v[2 * t] = u.x;
+      v[2 * t + 1] = u.y;
+    }
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
       const double yv = v[f], av = v[5 + f];

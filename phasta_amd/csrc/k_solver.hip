@@ -307,18 +307,10 @@ __global__ void k_sub(double* r, const double* b, const double* q, long long n) 
 
 // plain copy, 16 B per lane (calibration of the memory counters)
 __global__ void k_copy(double2* __restrict__ dst, const double2* __restrict__ src, long long n2) {
-  // four independent 16-byte loads in flight per lane before their stores
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n2; i += 4 * stride) {
-    const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride],
-                  d = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
-  }
-  for (; i < n2; i += stride) dst[i] = src[i];
+  // one 16-byte element per lane, no loop: the fastest plain copy measured
+  // on MI355X (6.2 TB/s for 859 MB; grid-stride loops reached 4.6-5.2)
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n2) dst[i] = src[i];
 }
 
 // padded assembled blocks -> lhsK(25,nnz) (double) and back
@@ -677,8 +669,12 @@ int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long lon
 }
 
 int launch_copy(phg_ctx* c, double* dst, const double* src, long long n) {
+  if (n & 1) {  // copied as 16-byte pairs
+    phg_set_error("copy: odd length %lld", n);
+    return PHG_EINVAL;
+  }
   const long long n2 = n / 2;
-  int g = (int)std::min<long long>(grid_for(n2, 256), 8192);
+  const int g = grid_for(n2, 256);
   k_copy<<<g, 256, 0, c->stream>>>(reinterpret_cast<double2*>(dst),
                                    reinterpret_cast<const double2*>(src), n2);
   PHG_HIP(hipGetLastError());
