@@ -57,7 +57,6 @@ __global__ void k_mp_features(const double* __restrict__ bdiag, const double* __
   if (i >= n) return;
   const bool act = ptr[i + 1] > ptr[i];
   const float s = (float)sptr[0];
-  uint16_t* xi = x + (size_t)i * kNin;
   // symmetric diagonal scaling: Dn = diag(d)^-1 D diag(d)^-1, d_r = sqrt|D_rr|
   double d[5];
 #pragma unroll
@@ -65,17 +64,28 @@ __global__ void k_mp_features(const double* __restrict__ bdiag, const double* __
     const double t = sqrt(fabs(bdiag[(size_t)(6 * r) * n + i]));
     d[r] = t > 0.0 ? t : 1.0;
   }
+  uint16_t xv[kNin];
 #pragma unroll
   for (int cc = 0; cc < 5; ++cc)
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
       const int e = r + 5 * cc;
-      xi[e] = act ? f2bf((float)(bdiag[(size_t)e * n + i] / (d[r] * d[cc])) / dscale[e]) : 0;
+      xv[e] = act ? f2bf((float)(bdiag[(size_t)e * n + i] / (d[r] * d[cc])) / dscale[e]) : 0;
     }
 #pragma unroll
-  for (int f = 0; f < 5; ++f) xi[25 + f] = act ? f2bf((float)v[(size_t)f * n + i] * s) : 0;
-  xi[30] = 0;
-  xi[31] = 0;
+  for (int f = 0; f < 5; ++f) xv[25 + f] = act ? f2bf((float)v[(size_t)f * n + i] * s) : 0;
+  xv[30] = 0;
+  xv[31] = 0;
+  // the node's 64-byte input row as four 16-byte stores
+  uint4* xo = reinterpret_cast<uint4*>(x + (size_t)i * kNin);
+#pragma unroll
+  for (int t = 0; t < kNin / 8; ++t) {
+    uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      w[q] = (uint32_t)xv[8 * t + 2 * q] | ((uint32_t)xv[8 * t + 2 * q + 1] << 16);
+    xo[t] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
 }
 
 // ---------------- aggregation ----------------
@@ -138,6 +148,110 @@ __global__ void __launch_bounds__(256) k_mp_agg(const uint16_t* __restrict__ h,
   st4(a + 2 * kH + c0, g0);
   st4(a + 3 * kH + c0, g1);
   st4(a + 4 * kH + c0, g2);
+}
+
+// Fused inference layer: the aggregation of a 64-row tile goes to LDS and
+// h_out = silu(A W + b) + h_in is formed on MFMA from there, so A never
+// travels through HBM (training keeps the separate k_mp_agg / k_mp_dense
+// pair, whose A and pre-activations the backward pass reads).  All four
+// wavefronts aggregate (16 lanes per node, 4 channels per lane, 16 nodes
+// per pass, as k_mp_agg); wavefront w then forms output columns 16w..16w+15
+// of the 64 rows with its slice of W held in registers, and the output tile
+// is staged in LDS and written as full 128-byte rows.  Same arithmetic, in
+// the same order, as k_mp_agg followed by k_mp_dense<kNagg>.
+constexpr int kLayerRows = 64;
+constexpr int kLdsRow = kNagg + 8;  // padded bf16 row of the A tile: 656 bytes
+
+__global__ void __launch_bounds__(256) k_mp_layer(const uint16_t* __restrict__ h,
+                                                  const int* __restrict__ ptr,
+                                                  const int* __restrict__ col,
+                                                  const float4* __restrict__ ev,
+                                                  const float* __restrict__ inv,
+                                                  const uint16_t* __restrict__ Wt,
+                                                  const float* __restrict__ bias,
+                                                  uint16_t* __restrict__ hout, long long n) {
+  __shared__ __align__(16) uint16_t At[kLayerRows * kLdsRow];
+  __shared__ __align__(16) uint16_t Ot[kLayerRows * kH];
+  const long long r0 = blockIdx.x * (long long)kLayerRows;
+  const int c0 = 4 * (threadIdx.x & 15);
+  for (int pass = 0; pass < kLayerRows / 16; ++pass) {
+    const int lr = pass * 16 + (int)(threadIdx.x >> 4);
+    const long long i = r0 + lr;
+    uint16_t* a = At + lr * kLdsRow;
+    float hi[4] = {0.f, 0.f, 0.f, 0.f}, m[4] = {0.f, 0.f, 0.f, 0.f},
+          g0[4] = {0.f, 0.f, 0.f, 0.f}, g1[4] = {0.f, 0.f, 0.f, 0.f},
+          g2[4] = {0.f, 0.f, 0.f, 0.f};
+    if (i < n) {
+      ld4(h + (size_t)i * kH + c0, hi);
+      const int k0 = ptr[i], k1 = ptr[i + 1];
+#pragma unroll 2
+      for (int k = k0; k < k1; ++k) {
+        const int j = col[k];
+        const float4 e = ev[k];
+        float hj[4];
+        ld4(h + (size_t)j * kH + c0, hj);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const float d = hj[t] - hi[t];
+          m[t] += hj[t];
+          g0[t] += d * e.x;
+          g1[t] += d * e.y;
+          g2[t] += d * e.z;
+        }
+      }
+      const float w = inv[i];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        m[t] *= w;
+        g0[t] *= w;
+        g1[t] *= w;
+        g2[t] *= w;
+      }
+      *reinterpret_cast<uint2*>(a + c0) = *reinterpret_cast<const uint2*>(h + (size_t)i * kH + c0);
+    } else {
+      *reinterpret_cast<uint2*>(a + c0) = make_uint2(0u, 0u);
+    }
+    st4(a + kH + c0, m);
+    st4(a + 2 * kH + c0, g0);
+    st4(a + 3 * kH + c0, g1);
+    st4(a + 4 * kH + c0, g2);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lrow = lane & 15, lk = (lane >> 4) * 8;
+  bf16x8 b[kNagg / 32];
+#pragma unroll
+  for (int ks = 0; ks < kNagg / 32; ++ks)
+    b[ks] = *reinterpret_cast<const bf16x8*>(Wt + (size_t)(16 * wave + lrow) * kNagg + ks * 32 + lk);
+  const int cc = 16 * wave + lrow;
+  const float bb = bias[cc];
+#pragma unroll
+  for (int rt = 0; rt < kLayerRows / 16; ++rt) {
+    f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kNagg / 32; ++ks) {
+      const bf16x8 av =
+          *reinterpret_cast<const bf16x8*>(At + (rt * 16 + lrow) * kLdsRow + ks * 32 + lk);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[ks], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int lr2 = rt * 16 + (lane >> 4) * 4 + q;
+      const long long row = r0 + lr2;
+      const float u = acc[q] + bb;
+      const float hv = silu(u) + bf2f(At[lr2 * kLdsRow + cc]);  // A[., 0:64] is h_in
+      const bool act = row < n && ptr[row + 1] > ptr[row];
+      Ot[lr2 * kH + cc] = act ? f2bf(hv) : (uint16_t)0;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kLayerRows * kH / 8; t += blockDim.x) {
+    const int lr2 = t >> 3, ch = t & 7;
+    const long long row = r0 + lr2;
+    if (row < n)
+      *reinterpret_cast<uint4*>(hout + (size_t)row * kH + ch * 8) =
+          *reinterpret_cast<const uint4*>(Ot + lr2 * kH + ch * 8);
+  }
 }
 
 // Adjoint of k_mp_agg:
@@ -384,11 +498,20 @@ __global__ void __launch_bounds__(256) k_mp_out(const uint16_t* __restrict__ h,
   float y[5];
 #pragma unroll
   for (int f = 0; f < 5; ++f) y[f] = bout[f];
-  const uint16_t* hi = h + (size_t)i * kH;
-  for (int k = 0; k < kH; ++k) {
-    const float hv = bf2f(hi[k]);
+  // the node's 128-byte hidden row as eight 16-byte loads, k ascending
+  const uint4* hp = reinterpret_cast<const uint4*>(h + (size_t)i * kH);
 #pragma unroll
-    for (int f = 0; f < 5; ++f) y[f] += hv * Wout[k * 5 + f];
+  for (int t = 0; t < kH / 8; ++t) {
+    const uint4 u = hp[t];
+    const uint32_t ws[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int k = 8 * t + q;
+      const uint32_t wq = ws[q >> 1];
+      const float hv = __uint_as_float((q & 1) ? (wq & 0xffff0000u) : (wq << 16));
+#pragma unroll
+      for (int f = 0; f < 5; ++f) y[f] += hv * Wout[k * 5 + f];
+    }
   }
   if (!act) {
 #pragma unroll
@@ -722,7 +845,6 @@ int mp_init(phg_ctx* c, const phg_mesh* m, const float* dscale) {
   PHG_TRY(mp_alloc(c, &M.x, (size_t)kNin * n));
   PHG_TRY(mp_alloc(c, &M.h[0], (size_t)kH * n));
   PHG_TRY(mp_alloc(c, &M.h[1], (size_t)kH * n));
-  PHG_TRY(mp_alloc(c, &M.A[0], (size_t)kNagg * n));
   PHG_TRY(mp_alloc(c, &M.w, Layout::total));
   for (int l = 0; l < 4; ++l) {
     PHG_TRY(mp_alloc(c, &M.wt[l], (size_t)kK[l] * kH));
@@ -754,7 +876,7 @@ int mp_alloc_train(phg_ctx* c) {
   if (M.train_ready) return 0;
   const long long n = c->nshg;
   for (int l = 2; l < 4; ++l) PHG_TRY(mp_alloc(c, &M.h[l], (size_t)kH * n));
-  for (int l = 1; l < 3; ++l) PHG_TRY(mp_alloc(c, &M.A[l], (size_t)kNagg * n));
+  for (int l = 0; l < 3; ++l) PHG_TRY(mp_alloc(c, &M.A[l], (size_t)kNagg * n));
   for (int l = 0; l < 4; ++l) PHG_TRY(mp_alloc(c, &M.U[l], (size_t)kH * n));
   PHG_TRY(mp_alloc(c, &M.Y, 5 * n));
   PHG_TRY(mp_alloc(c, &M.Ystar, 5 * n));
@@ -896,11 +1018,17 @@ int mp_forward(phg_ctx* c, const double* v, double* z, bool train) {
     // training keeps h0..h3 and A1..A3; inference ping-pongs h[0], h[1]
     uint16_t* hin = M.h[train ? l - 1 : (l - 1) & 1];
     uint16_t* hout = M.h[train ? l : l & 1];
-    uint16_t* A = M.A[train ? l - 1 : 0];
+    if (!train) {
+      k_mp_layer<<<grid_for(n, kLayerRows), 256, 0, c->stream>>>(hin, M.ptr, M.col, M.ev, M.inv,
+                                                                 M.wt[l], M.w + kBoff[l], hout, n);
+      PHG_HIP(hipGetLastError());
+      continue;
+    }
+    uint16_t* A = M.A[l - 1];
     k_mp_agg<<<grid_for(n, 16), 256, 0, c->stream>>>(hin, M.ptr, M.col, M.ev, M.inv, A, n);
     PHG_HIP(hipGetLastError());
     k_mp_dense<kNagg><<<gd, 256, 0, c->stream>>>(A, M.wt[l], M.w + kBoff[l], hin, M.ptr, hout,
-                                                train ? M.U[l] : nullptr, 1, n);
+                                                M.U[l], 1, n);
     PHG_HIP(hipGetLastError());
   }
   const uint16_t* h3 = M.h[train ? 3 : 1];
