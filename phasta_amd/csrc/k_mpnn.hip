@@ -172,7 +172,11 @@ __global__ void __launch_bounds__(256) k_mp_layer(const uint16_t* __restrict__ h
                                                   uint16_t* __restrict__ hout, long long n) {
   __shared__ __align__(16) uint16_t At[kLayerRows * kLdsRow];
   __shared__ __align__(16) uint16_t Ot[kLayerRows * kH];
-  const long long r0 = blockIdx.x * (long long)kLayerRows;
+  // XCD-strided tiles (grid a multiple of 8): each XCD sweeps a contiguous
+  // range of rows, so the neighbour rows it gathers stay in its L2
+  const long long per = (gridDim.x + 7) / 8;
+  const long long r0 = ((blockIdx.x % 8) * per + blockIdx.x / 8) * (long long)kLayerRows;
+  if (r0 >= n) return;  // whole workgroup
   const int c0 = 4 * (threadIdx.x & 15);
   for (int pass = 0; pass < kLayerRows / 16; ++pass) {
     const int lr = pass * 16 + (int)(threadIdx.x >> 4);
@@ -184,12 +188,7 @@ __global__ void __launch_bounds__(256) k_mp_layer(const uint16_t* __restrict__ h
     if (i < n) {
       ld4(h + (size_t)i * kH + c0, hi);
       const int k0 = ptr[i], k1 = ptr[i + 1];
-#pragma unroll 2
-      for (int k = k0; k < k1; ++k) {
-        const int j = col[k];
-        const float4 e = ev[k];
-        float hj[4];
-        ld4(h + (size_t)j * kH + c0, hj);
+      auto acc_nb = [&](const float hj[4], const float4 e) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const float d = hj[t] - hi[t];
@@ -198,6 +197,24 @@ __global__ void __launch_bounds__(256) k_mp_layer(const uint16_t* __restrict__ h
           g1[t] += d * e.y;
           g2[t] += d * e.z;
         }
+      };
+      int k = k0;
+      // four neighbour rows in flight, accumulated in edge order
+      for (; k + 4 <= k1; k += 4) {
+        float hj[4][4];
+        float4 e[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          e[u] = ev[k + u];
+          ld4(h + (size_t)col[k + u] * kH + c0, hj[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc_nb(hj[u], e[u]);
+      }
+      for (; k < k1; ++k) {
+        float hj[4];
+        ld4(h + (size_t)col[k] * kH + c0, hj);
+        acc_nb(hj, ev[k]);
       }
       const float w = inv[i];
 #pragma unroll
@@ -1019,8 +1036,9 @@ int mp_forward(phg_ctx* c, const double* v, double* z, bool train) {
     uint16_t* hin = M.h[train ? l - 1 : (l - 1) & 1];
     uint16_t* hout = M.h[train ? l : l & 1];
     if (!train) {
-      k_mp_layer<<<grid_for(n, kLayerRows), 256, 0, c->stream>>>(hin, M.ptr, M.col, M.ev, M.inv,
-                                                                 M.wt[l], M.w + kBoff[l], hout, n);
+      const int gl = (grid_for(n, kLayerRows) + 7) / 8 * 8;
+      k_mp_layer<<<gl, 256, 0, c->stream>>>(hin, M.ptr, M.col, M.ev, M.inv, M.wt[l],
+                                            M.w + kBoff[l], hout, n);
       PHG_HIP(hipGetLastError());
       continue;
     }
