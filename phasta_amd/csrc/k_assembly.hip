@@ -465,10 +465,12 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
           for (int n = 0; n < 3; ++n) Pr[1 + n] = (e3p * sR[n] + rho * uq[n] * una) * tau1;
           Pr[4] = e4p * una * tau2 + Na;
         }
+        // accumulated term by term: five fused multiply-adds per entry, the
+        // five entries of the row interleaved (independent chains)
 #pragma unroll
-        for (int j = 0; j < 5; ++j)
-          add(m + 5 * j, Pr[0] * Q[0][j] + Pr[1] * Q[1][j] + Pr[2] * Q[2][j] + Pr[3] * Q[3][j] +
-                             Pr[4] * Q[4][j]);
+        for (int k = 0; k < 5; ++k)
+#pragma unroll
+          for (int j = 0; j < 5; ++j) acc[m + 5 * j] += Pr[k] * Q[k][j];
       }
     }
     // viscous part sum_q WdetJ K_ab: the momentum rows and the conduction
