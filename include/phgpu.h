@@ -77,7 +77,10 @@ typedef struct phg_stats {
   int refine_cycles;      /* GMRES cycles run (1 on the plain FP64 path)   */
   int pc_fallback;        /* learned PC: 0 its cycle converged (or unused),
                              1 discarded for block-diagonal GMRES, 2 finished
-                             by block-diagonal cycles                      */
+                             by block-diagonal cycles, 3 its first direction
+                             left a larger one-step residual than the
+                             block-diagonal direction and the cycle ran
+                             without the network                           */
 } phg_stats;
 
 typedef struct phg_ctx phg_ctx;

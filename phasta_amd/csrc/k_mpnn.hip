@@ -867,12 +867,13 @@ int mp_init(phg_ctx* c, const phg_mesh* m, const float* dscale) {
     PHG_TRY(mp_alloc(c, &M.wt[l], (size_t)kK[l] * kH));
     PHG_TRY(mp_alloc(c, &M.wr[l], (size_t)kK[l] * kH));
   }
-  PHG_TRY(mp_alloc(c, &M.sbuf, 8));
+  PHG_TRY(mp_alloc(c, &M.sbuf, 12));
   PHG_TRY(mp_alloc(c, &M.vin, 5 * n));
   PHG_TRY(mp_alloc(c, &M.zout, 5 * n));
   PHG_TRY(mp_alloc(c, &M.rbuf, 5 * n));
   PHG_TRY(mp_alloc(c, &M.gbuf, 5 * n));
   PHG_TRY(mp_alloc(c, &M.zbasis, (size_t)std::max(c->kcap, 1) * 5 * n));
+  PHG_TRY(mp_alloc(c, &M.wplain, 5 * n));
   PHG_HIP(hipMemcpyAsync(M.ptr, ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, c->stream));
   PHG_HIP(hipMemcpyAsync(M.col, col.data(), sizeof(int) * ne, hipMemcpyHostToDevice, c->stream));
   PHG_HIP(hipMemcpyAsync(M.rev, rev.data(), sizeof(int) * ne, hipMemcpyHostToDevice, c->stream));
@@ -921,7 +922,7 @@ int mp_free(phg_ctx* c) {
   void* ptrs[] = {M.ptr,  M.col,  M.rev,  M.ev,   M.esum,  M.inv,   M.dscale, M.bd_raw, M.zhat,
                   M.x,    M.h[0], M.h[1], M.h[2], M.h[3],  M.A[0],  M.A[1],   M.A[2],   M.U[0],
                   M.U[1], M.U[2], M.U[3], M.Y,    M.Ystar, M.dh,    M.dU,     M.dA,     M.dY,
-                  M.sq,   M.part, M.sbuf, M.vin,  M.zout,  M.zbasis, M.ds,
+                  M.sq,   M.part, M.sbuf, M.vin,  M.zout,  M.zbasis, M.wplain, M.ds,
                   M.rbuf, M.gbuf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

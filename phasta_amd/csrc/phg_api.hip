@@ -445,6 +445,7 @@ int elm_core(phg_ctx* c) {
 struct CycleOut {
   int iKs = 0;
   double echeck = 0.0;
+  bool rejected = false;  // the network's first direction lost the acceptance test
 };
 
 // flexible: precondition each direction with the network (FGMRES, right
@@ -461,24 +462,65 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
       Rs(K + 2, 0.0), yB(K + 2, 0.0);
   auto Hm = [&](int a, int b) -> double& { return Hh[(size_t)(a - 1) + (size_t)(K + 1) * (b - 1)]; };
   e[1] = unorm;
+  bool use_net = flexible && c->mp.use_pc >= 1;
   for (int iK = 1; iK <= K; ++iK) {
     iKs = iK;
     double* w = kvec(c, iK);
     const double* dir = kvec(c, iK - 1);
+    bool have_w = false;
     if (flexible) {
       // the network preconditions the first use_pc directions of the cycle,
       // the block-diagonal preconditioner (identity here) the rest
-      if (iK <= c->mp.use_pc)
+      if (use_net && iK <= c->mp.use_pc) {
         PHG_TRY(mp_forward(c, kvec(c, iK - 1), zvec(iK - 1), false));
-      else
+        if (iK == 1) {
+          // acceptance test, block-diagonal GMRES as the yardstick: the
+          // one-step residual min_a ||u_1 - a K z||^2 = 1 - (u_1.Kz)^2/|Kz|^2
+          // of the network's direction z = N(u_1) against that of z = u_1;
+          // the network's direction is kept unless it is larger
+          double* wp = c->mp.wplain;
+          double* sb = c->mp.sbuf + 8;
+          PHG_HIP(hipEventRecord(c->kev[0], c->stream));
+          PHG_TRY(launch_spmv(c, zvec(0), w));
+          PHG_TRY(launch_spmv(c, kvec(c, 0), wp));
+          PHG_HIP(hipEventRecord(c->kev[1], c->stream));
+          PHG_TRY(launch_bc3per(c, w, 5));
+          PHG_TRY(launch_bc3per(c, wp, 5));
+          PHG_TRY(launch_dot(c, kvec(c, 0), w, V, sb));
+          PHG_TRY(launch_dot(c, w, w, V, sb + 1));
+          PHG_TRY(launch_dot(c, kvec(c, 0), wp, V, sb + 2));
+          PHG_TRY(launch_dot(c, wp, wp, V, sb + 3));
+          double d[4] = {0.0, 0.0, 0.0, 0.0};
+          PHG_HIP(hipMemcpyAsync(d, sb, sizeof(d), hipMemcpyDeviceToHost, c->stream));
+          PHG_TRY(sync(c));
+          float a1 = 0;
+          PHG_HIP(hipEventElapsedTime(&a1, c->kev[0], c->kev[1]));
+          st->ms_spmv += a1;
+          st->n_spmv += 2;
+          const double r_net = d[1] > 0.0 ? 1.0 - d[0] * d[0] / d[1] : 1.0;
+          const double r_blk = d[3] > 0.0 ? 1.0 - d[2] * d[2] / d[3] : 1.0;
+          if (!(r_net <= r_blk)) {
+            use_net = false;
+            out->rejected = true;
+            PHG_HIP(hipMemcpyAsync(zvec(0), kvec(c, 0), sizeof(double) * V,
+                                   hipMemcpyDeviceToDevice, c->stream));
+            PHG_HIP(hipMemcpyAsync(w, wp, sizeof(double) * V, hipMemcpyDeviceToDevice,
+                                   c->stream));
+          }
+          have_w = true;
+        }
+      } else {
         PHG_HIP(hipMemcpyAsync(zvec(iK - 1), kvec(c, iK - 1), sizeof(double) * V,
                                hipMemcpyDeviceToDevice, c->stream));
+      }
       dir = zvec(iK - 1);
     }
-    PHG_HIP(hipEventRecord(c->kev[0], c->stream));
-    PHG_TRY(launch_spmv(c, dir, w));
-    PHG_HIP(hipEventRecord(c->kev[1], c->stream));
-    PHG_TRY(launch_bc3per(c, w, 5));
+    if (!have_w) {
+      PHG_HIP(hipEventRecord(c->kev[0], c->stream));
+      PHG_TRY(launch_spmv(c, dir, w));
+      PHG_HIP(hipEventRecord(c->kev[1], c->stream));
+      PHG_TRY(launch_bc3per(c, w, 5));
+    }
     PHG_HIP(hipEventRecord(c->kev[2], c->stream));
     // H(1..iK+1, iK) by MGS: channel j-1 holds the partials of H(j, iK)
     int np0 = 0, np = 0;
@@ -496,11 +538,11 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
     PHG_TRY(sync(c));
     {
       float a1 = 0, a2 = 0;
-      PHG_HIP(hipEventElapsedTime(&a1, c->kev[0], c->kev[1]));
+      if (!have_w) PHG_HIP(hipEventElapsedTime(&a1, c->kev[0], c->kev[1]));
       PHG_HIP(hipEventElapsedTime(&a2, c->kev[2], c->kev[3]));
       st->ms_spmv += a1;
       st->ms_mgs += a2;
-      st->n_spmv += 1;
+      if (!have_w) st->n_spmv += 1;
       st->n_mgs += iK + 1;
     }
     for (int jK = 1; jK <= iK; ++jK) Hm(jK, iK) = c->h_scal[16 + jK - 1];
@@ -624,7 +666,9 @@ int sol_core(phg_ctx* c, phg_stats* st) {
       st->refine_cycles = 1;
       PHG_TRY(true_residual(&rnorm));
       echeck = rnorm;
-      if (!(rnorm < unorm)) {
+      if (co.rejected) {
+        st->pc_fallback = 3;  // the cycle ran as block-diagonal GMRES
+      } else if (!(rnorm < unorm)) {
         // no reduction: discard and solve with block-diagonal GMRES
         st->pc_fallback = 1;
         PHG_TRY(launch_fill(c, c->dy, sizeof(double) * V));

@@ -156,12 +156,14 @@ struct MpNet {
   float* m2 = nullptr;
   uint16_t* wt[4] = {};      // bf16 weights, Wt[c][k]
   uint16_t* wr[4] = {};      // bf16 weights, W[k][c]
-  double* sbuf = nullptr;    // [0] ||v||^2, [1] s, [2] loss, [3] ||r||^2, [4] loss sum
+  double* sbuf = nullptr;    // [0] ||v||^2, [1] s, [2] loss, [3] ||r||^2, [4] loss sum,
+                             // [8..11] acceptance-test dot products
   double* vin = nullptr;     // 5 planes, host-array entry points
   double* zout = nullptr;    // 5 planes
   double* rbuf = nullptr;    // 5 planes (residual training)
   double* gbuf = nullptr;    // 5 planes
   double* zbasis = nullptr;  // FGMRES preconditioned directions (K vectors)
+  double* wplain = nullptr;  // K v_1 of the block-diagonal first direction (acceptance test)
   int adam_t = 0;
   bool train_ready = false;
   // parameters shared with another context (training over several meshes):

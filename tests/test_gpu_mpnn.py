@@ -102,6 +102,30 @@ def test_zero_network_fgmres_equals_block_diagonal_gmres():
     assert np.linalg.norm(z) > 0
 
 
+
+def test_bad_network_direction_is_rejected():
+    """Block-diagonal GMRES is the yardstick of the network's first
+    direction: a network whose direction leaves a larger one-step residual
+    than the block-diagonal one is set aside for the cycle (pc_fallback 3),
+    and the solve is then the block-diagonal solve of the oracle."""
+    case = mesh.box_case(4, mach=0.3, kspace=10)
+    orc = pyoracle.Oracle(case)
+    y, ac = orc.itr_predict_bc(case.yold)
+    ref = orc.solgmrs(y, ac)
+    p = M.init_params(9)
+    p["Wout"] = (10.0 * np.random.default_rng(4).standard_normal(p["Wout"].shape)).astype(np.float32)
+    gpu = solver.GpuSolver(case)
+    try:
+        gpu.mpnn_init(np.ones(25, dtype=np.float32))
+        gpu.mpnn_set_weights(solver.mpnn_flatten(p))
+        gpu.mpnn_enable(use_pc=1)
+        out = gpu.solgmrs(y, ac)
+    finally:
+        gpu.close()
+    assert out["pc_fallback"] == 3
+    assert out["iKs"] == ref["iKs"]
+    assert rel(out["Dy"], ref["Dy"]) < 1e-12
+
 def test_residual_loss_matches_host_and_training_reduces_it():
     """The residual loss ||K z(v) - v||^2 / ||v||^2 of the first Krylov vector
     v = b / ||b||, formed on the device, equals the same quantity from the
