@@ -150,16 +150,16 @@ __global__ void __launch_bounds__(256) k_mp_agg(const uint16_t* __restrict__ h,
   st4(a + 4 * kH + c0, g2);
 }
 
-// Fused inference layer: the aggregation of a 64-row tile goes to LDS and
+// Fused inference layer: the aggregation of a 32-row tile goes to LDS and
 // h_out = silu(A W + b) + h_in is formed on MFMA from there, so A never
 // travels through HBM (training keeps the separate k_mp_agg / k_mp_dense
 // pair, whose A and pre-activations the backward pass reads).  All four
 // wavefronts aggregate (16 lanes per node, 4 channels per lane, 16 nodes
 // per pass, as k_mp_agg); wavefront w then forms output columns 16w..16w+15
-// of the 64 rows with its slice of W held in registers, and the output tile
+// of the 32 rows with its slice of W held in registers, and the output tile
 // is staged in LDS and written as full 128-byte rows.  Same arithmetic, in
 // the same order, as k_mp_agg followed by k_mp_dense<kNagg>.
-constexpr int kLayerRows = 64;
+constexpr int kLayerRows = 32;  // 25 KB of LDS per workgroup (64 rows: 1.25x slower)
 constexpr int kLdsRow = kNagg + 8;  // padded bf16 row of the A tile: 656 bytes
 
 __global__ void __launch_bounds__(256) k_mp_layer(const uint16_t* __restrict__ h,
