@@ -393,8 +393,8 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     const int code = A.blocks[L.x + k];
     const long long e = code >> 4;
     const int a = (code >> 2) & 3, b = code & 3;
-    S go[16];
-    load_rec<S>(geo + (size_t)e * rec_stride<S>(), go);
+    // the two shape gradients and WdetJ of the element's geometry record
+    const S* go = geo + (size_t)e * rec_stride<S>();
     R sR[3], rR[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -425,23 +425,19 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
       const R una = u0 * sR[0] + u1 * sR[1] + u2 * sR[2];
       const R unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
       const R w = unb + cfacR * Nb;
-      // Q = A(s_b) + c N_b A0
-      R Q[5][5];
-      Q[0][0] = drdp * w;
-      Q[0][4] = drdT * w;
+      // Q = A(s_b) + c N_b A0, held in factored form: its rows 1..3 are
+      // drdp w u_k + r_k (column 0), rho (u_k r^T + w e_k^T) (columns 1..3)
+      // and drdT w u_k (column 4), so with pu = sum_k P(m,k) u_k they enter
+      // the product as q00 pu + sum_k P(m,k) r_k, rho r_n pu + rho w P(m,1+n)
+      // and q04 pu; only rows 0 and 4 and rho w are formed
+      const R q00 = drdp * w, q04 = drdT * w, rw = rho * w;
+      R q0[3], q4[3];
 #pragma unroll
-      for (int n = 0; n < 3; ++n) Q[0][1 + n] = rho * rR[n];
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        Q[1 + m][0] = drdp * uq[m] * w + rR[m];
-#pragma unroll
-        for (int n = 0; n < 3; ++n) Q[1 + m][1 + n] = rho * (uq[m] * rR[n] + (m == n ? w : (R)0));
-        Q[1 + m][4] = drdT * uq[m] * w;
+      for (int n = 0; n < 3; ++n) {
+        q0[n] = rho * rR[n];
+        q4[n] = e3p * rR[n] + rw * uq[n];
       }
-      Q[4][0] = e2p * unb + cfacR * Nb * e1p;
-#pragma unroll
-      for (int n = 0; n < 3; ++n) Q[4][1 + n] = e3p * rR[n] + rho * uq[n] * w;
-      Q[4][4] = e4p * w;
+      const R q40 = e2p * unb + cfacR * Nb * e1p, q44 = e4p * w;
       // rows of P = A(s_a) diag(tau) + N_a I
 #pragma unroll
       for (int m = 0; m < 5; ++m) {
@@ -465,12 +461,26 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
           for (int n = 0; n < 3; ++n) Pr[1 + n] = (e3p * sR[n] + rho * uq[n] * una) * tau1;
           Pr[4] = e4p * una * tau2 + Na;
         }
-        // accumulated term by term: five fused multiply-adds per entry, the
-        // five entries of the row interleaved (independent chains)
+        const R pu = Pr[1] * u0 + Pr[2] * u1 + Pr[3] * u2;
+        R& c0 = acc[m];
+        c0 += Pr[0] * q00;
+        c0 += pu * q00;
+        c0 += Pr[1] * rR[0];
+        c0 += Pr[2] * rR[1];
+        c0 += Pr[3] * rR[2];
+        c0 += Pr[4] * q40;
 #pragma unroll
-        for (int k = 0; k < 5; ++k)
-#pragma unroll
-          for (int j = 0; j < 5; ++j) acc[m + 5 * j] += Pr[k] * Q[k][j];
+        for (int n = 0; n < 3; ++n) {
+          R& cn = acc[m + 5 * (1 + n)];
+          cn += Pr[0] * q0[n];
+          cn += pu * q0[n];
+          cn += rw * Pr[1 + n];
+          cn += Pr[4] * q4[n];
+        }
+        R& c4 = acc[m + 20];
+        c4 += Pr[0] * q04;
+        c4 += pu * q04;
+        c4 += Pr[4] * q44;
       }
     }
     // viscous part sum_q WdetJ K_ab: the momentum rows and the conduction
@@ -492,8 +502,12 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
       add(24, W4 * kapR * st);
     }
 
-    if (diag) {
-      const double* ro = A.resid + ((size_t)e * 4 + a) * 5;
+  }
+  // the node's residual: element residuals of the diagonal blocks' elements
+  if (diag) {
+    for (int k = 0; k < cnt; ++k) {
+      const int code = A.blocks[L.x + k];
+      const double* ro = A.resid + ((size_t)(code >> 4) * 4 + ((code >> 2) & 3)) * 5;
 #pragma unroll
       for (int f = 0; f < 5; ++f) racc[f] += ro[f];
     }
