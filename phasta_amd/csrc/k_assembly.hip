@@ -369,8 +369,13 @@ struct BlkArgs {
 // points and elements are taken in double.
 template <typename T, typename S>
 __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
-  __shared__ double stage[64][31];
+  // chunk partials of the 64 lanes: tangent sums in the tangent's type;
+  // residual sums (double) in their own array on the float path, through
+  // the same array after the tangent on the FP64 path (6.4 + 2.5 KB or
+  // 12.8 KB of LDS per workgroup, so LDS does not cap the occupancy)
   using R = T;
+  __shared__ R stage[64][25];
+  __shared__ double rstage[sizeof(R) == 4 ? 64 : 1][5];
   // consecutive workgroups run on different XCDs (round robin over 8): give
   // each XCD a contiguous range of lanes so its L2 sees neighbouring blocks
   const long long per = (gridDim.x + 7) / 8;
@@ -513,22 +518,36 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     }
   }
   // combine the chunks of one block in lane order
-#pragma unroll
-  for (int i = 0; i < 25; ++i) stage[threadIdx.x][i] = (double)acc[i];
-#pragma unroll
-  for (int f = 0; f < 5; ++f) stage[threadIdx.x][25 + f] = racc[f];
-  __syncthreads();
-  if (!(L.w & 1) || l >= A.nlanes) return;
+  const bool leader = (L.w & 1) && l < A.nlanes;
   const int nch = (L.w >> 8) & 0xff;
   const int total = L.w >> 16;
-  double sum[25];
 #pragma unroll
-  for (int i = 0; i < 25; ++i) sum[i] = stage[threadIdx.x][i];
+  for (int i = 0; i < 25; ++i) stage[threadIdx.x][i] = acc[i];
+  if constexpr (sizeof(R) == 4) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) rstage[threadIdx.x][f] = racc[f];
+  }
+  __syncthreads();
+  double sum[25];
+  if (leader) {
+#pragma unroll
+    for (int i = 0; i < 25; ++i) sum[i] = (double)stage[threadIdx.x][i];
+    for (int c = 1; c < nch; ++c) {
+#pragma unroll
+      for (int i = 0; i < 25; ++i) sum[i] += (double)stage[threadIdx.x + c][i];
+    }
+  }
+  if constexpr (sizeof(R) == 8) {
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 5; ++f) stage[threadIdx.x][f] = racc[f];
+    __syncthreads();
+  }
+  if (!leader) return;
   for (int c = 1; c < nch; ++c) {
 #pragma unroll
-    for (int i = 0; i < 25; ++i) sum[i] += stage[threadIdx.x + c][i];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) racc[f] += stage[threadIdx.x + c][25 + f];
+    for (int f = 0; f < 5; ++f)
+      racc[f] += sizeof(R) == 4 ? rstage[threadIdx.x + c][f] : (double)stage[threadIdx.x + c][f];
   }
   const int code0 = A.blocks[L.x];
   const long long e0 = code0 >> 4;
