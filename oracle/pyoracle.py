@@ -119,12 +119,19 @@ class Oracle:
             raise RuntimeError(f"orc_elmgmrs rc={rc}")
         return res, rmes, bd, lk
 
-    def solgmrs(self, y, ac, **over):
+    def solgmrs(self, y, ac, bd=None, lk=None, **over):
+        """SolGMRs; bd and lk, when given, are the caller's persistent BDiag
+        and lhsK (Fortran order, updated in place): with lhs = iprec = 0 the
+        reference reuses the factored BDiag and scaled lhsK of the last call
+        that formed them (itrdrv.f:456-457, solgmr.f:471,495)."""
         n, nnz = self.case.nshg, self.case.nnz_tot
         res = np.zeros((n, 5), order="F")
         rmes = np.zeros((n, 5), order="F")
-        bd = np.zeros((n, 5, 5), order="F")
-        lk = np.zeros((25, nnz), order="F")
+        if bd is None:
+            bd = np.zeros((n, 5, 5), order="F")
+        if lk is None:
+            lk = np.zeros((25, nnz), order="F")
+        assert bd.flags.f_contiguous and lk.flags.f_contiguous
         dy = np.zeros((n, 5), order="F")
         st = OrcStats()
         st.reslast = self.reslast

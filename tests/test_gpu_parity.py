@@ -97,6 +97,36 @@ def test_sparse_ap_i3lu_spsi3pre_bc3per(pair):
     assert np.array_equal(gpu.bc3per(p), b_ref)
 
 
+
+def test_lhs_reuse_across_calls(pair):
+    """LHSupd > 1 (itrdrv.f:456-457): a call with lhs = iprec = 1 forms and
+    keeps the factored BDiag and scaled lhsK; the next call, lhs = iprec = 0,
+    assembles the residual only and solves with the kept ones
+    (solgmr.f:471,495).  Same 1e-12 bar against the oracle given the same
+    persistent arrays."""
+    case, orc, gpu = pair
+    n, nnz = case.nshg, case.nnz_tot
+    acold = np.zeros((n, 5), order="F")
+    y, ac = orc.itr_predict_bc(case.yold, acold)
+    bd = np.zeros((n, 5, 5), order="F")
+    lk = np.zeros((25, nnz), order="F")
+    ref1 = orc.solgmrs(y, ac, bd=bd, lk=lk, lhs=1, iprec=1)
+    gpu.set_params(lhs=1, iprec=1)
+    out1 = gpu.solgmrs(y, ac)
+    assert rel(out1["Dy"], ref1["Dy"]) < TOL
+    y2, ac2 = orc.itr_correct_bc(y, ac, case.yold, acold, ref1["Dy"])
+    ref2 = orc.solgmrs(y2, ac2, bd=bd, lk=lk, lhs=0, iprec=0)
+    try:
+        gpu.set_params(lhs=0, iprec=0)
+        out2 = gpu.solgmrs(y2, ac2, want_lhs=True)
+    finally:
+        gpu.set_params(lhs=1, iprec=1)
+    assert out2["iKs"] == ref2["iKs"]
+    for k in ("res", "Dy"):
+        assert rel(out2[k], ref2[k]) < TOL, k
+    assert rel(out2["BDiag"], bd) < TOL     # the factored BDiag of call 1
+    assert rel(out2["lhsK"], lk) < TOL      # the scaled lhsK of call 1
+
 def test_device_newton_step(pair):
     case, orc, gpu = pair
     acold = np.zeros((case.nshg, 5), order="F")
