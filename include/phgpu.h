@@ -46,6 +46,7 @@ typedef struct phg_params {
   double etol, epsM;                    /* epstol(1), /precis/              */
   int Kspace, nGMRES, minIters;         /* /solpar/, /incomp/               */
   int lhs, iprec;                       /* itrdrv.f:456-457                 */
+  int ipred;                            /* /genpar/ predictor; 1 only      */
 } phg_params;
 
 /* Mesh arrays as the Fortran host holds them (readnblk.f, genadj.f);
@@ -117,7 +118,12 @@ int phg_state_download(phg_ctx* ctx, double* y, double* ac, double* Dy);
 int phg_state_download_old(phg_ctx* ctx, double* yold, double* acold);
 /* Current iterate (y, ac) from the host, e.g. to resume inside a step. */
 int phg_state_upload_cur(phg_ctx* ctx, const double* y, const double* ac);
-/* itrPredict + itrBC                            itrPC.f:56-121, itrbc.f */
+/* itrPredict + itrBC (itrdrv.f:398-399)        itrPC.f:56-121, itrbc.f
+ * Any generalized-alpha almi, alfi, gami (itrPC.f:16-25) with the
+ * same-velocity predictor ipred = 1.  PHG_EINVAL for ipred 2-4: the
+ * reference's itrPredict passes undeclared (implicitly typed, unset)
+ * scalars for iBC, BC, iper and ilwork to itrBC there (itrPC.f:88,101,
+ * 114), so those predictors have no defined result to reproduce. */
 int phg_predict(phg_ctx* ctx);
 /* SolGMRs + itrCorrect + itrBC on the resident state  itrdrv.f:477-599 */
 int phg_newton_iteration(phg_ctx* ctx, phg_stats* st);

@@ -1165,13 +1165,17 @@ int orc_solgmrs(const orc_mesh* m, const orc_params* P, const double* y,
   return 0;
 }
 
-/* itrPC.f:56-73 (ipred = 1) */
-void orc_itr_predict(long long nshg, const orc_params* P, const double* yold,
-                     const double* acold, double* y, double* ac) {
+/* itrPC.f:56-73, the same-velocity predictor.  ipred 2-4 (itrPC.f:75-119)
+ * pass undeclared, implicitly typed scalars for iBC, BC, iper and ilwork to
+ * itrBC, so the reference defines no result for them: rejected (1). */
+int orc_itr_predict(long long nshg, const orc_params* P, const double* yold,
+                    const double* acold, double* y, double* ac) {
+  if (P->ipred != 1) return 1;
   for (long long i = 0; i < 5 * nshg; ++i) {
     y[i] = yold[i];
     ac[i] = acold[i] * (ONE - P->almi / P->gami);
   }
+  return 0;
 }
 
 /* itrPC.f:128-148 */

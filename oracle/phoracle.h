@@ -39,6 +39,7 @@ typedef struct orc_params {
   double etol, epsM;                    /* epstol(1), /precis/             */
   int Kspace, nGMRES, minIters;         /* /solpar/, /incomp/              */
   int lhs, iprec;                       /* per-call flags (itrdrv.f:456)   */
+  int ipred;                            /* predictor 1-4 (itrPC.f:67-119)  */
 } orc_params;
 
 typedef struct orc_stats {
@@ -94,9 +95,10 @@ int orc_solgmrs(const orc_mesh* m, const orc_params* pr, const double* y,
                 double* lhsK, double* Dy, orc_stats* st);
 
 /* itrPredict / itrCorrect / itrBC (itrPC.f:56-148, itrbc.f:1-198);
- * y, ac, yold, acold are (nshg,5) in the global order (u,v,w,p,T). */
-void orc_itr_predict(long long nshg, const orc_params* pr, const double* yold,
-                     const double* acold, double* y, double* ac);
+ * y, ac, yold, acold are (nshg,5) in the global order (u,v,w,p,T).
+ * orc_itr_predict returns 1 for ipred != 1 (no defined reference result). */
+int orc_itr_predict(long long nshg, const orc_params* pr, const double* yold,
+                    const double* acold, double* y, double* ac);
 void orc_itr_correct(long long nshg, const orc_params* pr, double* y,
                      double* ac, const double* yold, const double* acold,
                      const double* Dy);

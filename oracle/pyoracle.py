@@ -26,7 +26,7 @@ class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
-               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec")]
+               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred")]
 
 
 class OrcStats(ctypes.Structure):
@@ -96,12 +96,16 @@ class Oracle:
 
     # --- nodal/global level -------------------------------------------
     def itr_predict_bc(self, yold, acold=None):
+        """itrPredict + itrBC (itrdrv.f:398-399)."""
         n = self.case.nshg
         acold = np.zeros((n, 5), order="F") if acold is None else acold
         y = np.zeros((n, 5), order="F")
         ac = np.zeros((n, 5), order="F")
         L = lib()
-        L.orc_itr_predict(_i64(n), ctypes.byref(self.p()), _p(yold), _p(acold), _p(y), _p(ac))
+        rc = L.orc_itr_predict(_i64(n), ctypes.byref(self.p()), _p(yold), _p(acold), _p(y),
+                               _p(ac))
+        if rc:
+            raise RuntimeError(f"orc_itr_predict rc={rc}")
         rc = L.orc_itr_bc(ctypes.byref(self.m), _p(y), _p(ac))
         if rc:
             raise RuntimeError(f"orc_itr_bc rc={rc}")

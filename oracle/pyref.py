@@ -19,7 +19,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 PARAM_ORDER = ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                "almi", "alfi", "gami", "Dtgl", "etol", "epsM")
-IPARAM_ORDER = ("Kspace", "nGMRES", "minIters", "lhs", "iprec")
+IPARAM_ORDER = ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred")
 
 
 def available():
@@ -63,6 +63,16 @@ class Reference:
         ac = np.zeros((n, 5), order="F")
         lib().phref_predict_bc(_p(yold), _p(acold), _p(y), _p(ac))
         return y, ac
+
+    def itr_setup(self, rhoinf, ipred, y, acold):
+        """itrSetup (itrPC.f:6-37): ((almi, alfi, gami), ipred, y after
+        the start-up shift)."""
+        y = np.array(y, order="F")
+        fac = np.zeros(3)
+        ip = ctypes.c_int()
+        lib().phref_itrsetup(ctypes.c_double(rhoinf), ctypes.c_int(ipred), _p(y),
+                             _p(np.asfortranarray(acold)), _p(fac), ctypes.byref(ip))
+        return tuple(fac), ip.value, y
 
     def correct_bc(self, y, ac, yold, acold, dy):
         y, ac = y.copy(order="F"), ac.copy(order="F")

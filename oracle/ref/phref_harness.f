@@ -32,7 +32,7 @@ c-----------------------------------------------------------------------
       real(c_double) :: x_(nshg_,3), BC_(nshg_,ndofBC_), dprm(14)
       integer(c_int) :: ien_(nel_,4), ienb_(nelb_,4), iBC_(nshg_),
      &                  iper_(nshg_), colm_(nshg_+1), rowp_(nnz_),
-     &                  iprm(5)
+     &                  iprm(6)
       integer :: iblk, iel0, nprob, k, q, a, j
       real*8 :: qa4, qb4, qA, qB, qC, rr(4,3), r3(3,3), el(4)
       interface
@@ -86,7 +86,7 @@ c.... physics (/mmatpar/, /matdat/, /genpar/, /outpar/, /timdat/)
       lhs = iprm(4)
       iprec = iprm(5)
       LHSupd(1) = 1
-      ipred = 1
+      ipred = iprm(6)
       rhoinf(1) = -1.0d0
       Jactyp = 0
       itseq = 1
@@ -303,6 +303,42 @@ c.... boundary element blocks (genbkb.f)
      &                  ac(nshg,5)
       call itrPredict(yold, acold, y, ac)
       call itrBC(y, ac, hiBC, hBC, hiper, hilwork)
+      end subroutine
+
+c.... itrSetup for spectral radius rinf and predictor ipin: returns
+c     almi, alfi, gami, the predictor it settles on, and y after its
+c     ipred = 4 start-up shift; the COMMON values it changes are restored
+      subroutine phref_itrsetup(rinf, ipin, y, acold, fac, ipout)
+     &           bind(C, name='phref_itrsetup')
+      use iso_c_binding
+      use phref_data
+      include "common.h"
+      real(c_double), value :: rinf
+      integer(c_int), value :: ipin
+      real(c_double) :: y(nshg,5), acold(nshg,5), fac(3)
+      integer(c_int) :: ipout
+      real*8 :: sav(7)
+      integer :: isav(2)
+      sav = (/ almi, alfi, gami, Dtgl, CFLfld, CFLsld, rhoinf(1) /)
+      isav = (/ ipred, Jactyp /)
+      rhoinf(1) = rinf
+      ipred = ipin
+      itseq = 1
+      Delt(1) = one / Dtgl
+      call itrSetup(y, acold)
+      fac(1) = almi
+      fac(2) = alfi
+      fac(3) = gami
+      ipout = ipred
+      almi = sav(1)
+      alfi = sav(2)
+      gami = sav(3)
+      Dtgl = sav(4)
+      CFLfld = sav(5)
+      CFLsld = sav(6)
+      rhoinf(1) = sav(7)
+      ipred = isav(1)
+      Jactyp = isav(2)
       end subroutine
 
       subroutine phref_update_bc(yold, acold, y, ac)

@@ -13,7 +13,9 @@ this path is pinned here once (reference defaults unless noted):
   (input.config:233-266), iDC = 0, idiff = 0 ("Include Viscous Correction
   in Stabilization: False"), iLHScond = 0
 * backward Euler ("First Order": almi = alfi = gami = 1, ipred = 1,
-  itrPC.f:16-20); LHSupd = 1 so lhs = iprec = 1 each Newton iteration
+  itrPC.f:16-20) on the workloads; generalized alpha (rho_inf in [0, 1])
+  and the four predictors through time_integration; LHSupd = 1 so
+  lhs = iprec = 1 each Newton iteration
 * nGMRES = 1, Kspace 10 (16^3 case) or 20, epstol 1e-3, minIters 0
   (reference default 10; set explicitly and identically in every arm)
 * wall thermal condition: floating flux (iBCB = 0, e3b.f:149-228)
@@ -72,11 +74,35 @@ def time_step(N, mach, grade=1.0, cfl=CFL):
     return cfl * min_spacing(N, grade) / (C0 * (1.0 + mach))
 
 
-def solver_params(N, mach, kspace, etol=1e-3, min_iters=0, grade=1.0, cfl=CFL):
+# "Predictor at Start of Step" values (input_fform.cc:636-643)
+PREDICTORS = {"Same Velocity": 1, "Zero Acceleration": 2, "Same Acceleration": 3,
+              "Same Delta": 4}
+
+
+def time_integration(rhoinf=-1.0, ipred=1):
+    """itrSetup (itrPC.f:6-37): the generalized-alpha factors of the
+    Hulbert family for a spectral radius rho_inf in [0, 1] (second order),
+    backward Euler with the same-velocity predictor otherwise ("First
+    Order" sets rho_inf = -1, input_fform.cc:633-635); the same-delta
+    predictor falls back to same acceleration at rho_inf = 0.  The
+    start-up shift y -= (1 - alfi) Delt CFLfl acold of ipred = 4
+    (itrPC.f:31-32) is zero in the reference, which never assigns CFLfl,
+    so the first step starts from y = yold (itrdrv.f:148)."""
+    if rhoinf < 0.0 or rhoinf > 1.0:
+        return dict(almi=1.0, alfi=1.0, gami=1.0, ipred=1)
+    almi = (3.0 - rhoinf) / (1.0 + rhoinf) / 2.0
+    alfi = 1.0 / (1.0 + rhoinf)
+    gami = 0.5 + almi - alfi
+    if ipred == 4 and rhoinf == 0.0:
+        ipred = 3
+    return dict(almi=almi, alfi=alfi, gami=gami, ipred=ipred)
+
+
+def solver_params(N, mach, kspace, etol=1e-3, min_iters=0, grade=1.0, cfl=CFL,
+                  rhoinf=-1.0, ipred=1):
     """The COMMON-block scalars the path reads, as a plain dict."""
     dt = time_step(N, mach, grade, cfl)
     return dict(Rgas=RGAS, gamma=GAMMA, gamma1=GAMMA1, pr=PRANDTL, mu=MU,
-                dtsfct=1.0, taucfct=1.0, temper=1.0,
-                almi=1.0, alfi=1.0, gami=1.0, Dtgl=1.0 / dt,
+                dtsfct=1.0, taucfct=1.0, temper=1.0, Dtgl=1.0 / dt,
                 etol=etol, epsM=EPSM, Kspace=kspace, nGMRES=1,
-                minIters=min_iters, lhs=1, iprec=1)
+                minIters=min_iters, lhs=1, iprec=1, **time_integration(rhoinf, ipred))

@@ -525,6 +525,8 @@ __global__ void k_slots_to_csr(const T* slots, double* ref, const uint32_t* c2s,
 }
 
 // ---------------- itrPredict / itrCorrect / itrBC ----------------
+// itrPredict, same-velocity predictor (itrPC.f:67-73): y = yold,
+// ac = acold (1 - almi/gami)
 __global__ void k_predict(const double* yold, const double* acold, double* y, double* ac,
                           long long n5, double fac) {
   long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -802,6 +804,15 @@ int launch_slots_to_csr32(phg_ctx* c, const float* slots, double* ref) {
 }
 
 int launch_predict(phg_ctx* c) {
+  // The reference's other predictors (ipred 2-4, itrPC.f:75-119) call
+  // itrBC with iBC, BC, iper and ilwork undeclared in itrPredict, so they
+  // are implicitly typed uninitialised scalars passed where arrays belong:
+  // the reference defines no result for them.
+  if (c->prm.ipred != 1) {
+    phg_set_error("phg_predict: ipred %d; only the same-velocity predictor (ipred 1, "
+                  "itrPC.f:67-73) has a defined result in the reference", c->prm.ipred);
+    return PHG_EINVAL;
+  }
   const long long n5 = 5 * c->nshg;
   const double fac = 1.0 - c->prm.almi / c->prm.gami;
   k_predict<<<grid_for(n5, 256), 256, 0, c->stream>>>(c->yold, c->acold, c->y, c->ac, n5, fac);

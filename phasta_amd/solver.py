@@ -38,7 +38,7 @@ class PhgParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
-               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec")]
+               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred")]
 
 
 class PhgMesh(ctypes.Structure):

@@ -45,7 +45,43 @@ def make(N, seed):
                         acold_upd=acold1)
 
 
+# spectral radii for itrSetup, and the one the time-step fixtures use
+RHOINF = (-1.0, 0.0, 0.25, 0.5, 0.75, 1.0, 1.5)
+RHOINF_STEP = 0.5
+
+
+def make_alpha(N, seed):
+    """itrSetup's factors (almi, alfi, gami) and settled predictor for each
+    RHOINF, asked for ipred 1 and 4, with y after its ipred = 4 start-up
+    shift; then itrPredict + itrBC and itrCorrect + itrBC at RHOINF_STEP
+    from a nonzero acold."""
+    case = mesh.box_case(N, mach=0.3, kspace=10)
+    rng = np.random.default_rng(seed)
+    n = case.nshg
+    acold = np.asfortranarray(1e3 * rng.standard_normal((n, 5)))
+    ref = pyref.Reference(case)
+    fac, ipo, shift = [], [], []
+    for r in RHOINF:
+        for ip in (1, 4):
+            f, ipout, ys = ref.itr_setup(r, ip, case.yold, acold)
+            fac.append(f)
+            ipo.append(ipout)
+            shift.append(np.abs(ys - case.yold).max())
+    f = fac[2 * RHOINF.index(RHOINF_STEP)]
+    prm = dict(case.params, almi=f[0], alfi=f[1], gami=f[2], ipred=1)
+    r2 = pyref.Reference(case, params=prm)
+    y, ac = r2.predict_bc(case.yold, acold)
+    dy = np.asfortranarray(1e-3 * rng.standard_normal((n, 5)))
+    y1, ac1 = r2.correct_bc(y, ac, case.yold, acold, dy)
+    np.savez_compressed(os.path.join(HERE, f"ref_alpha_box{N}.npz"), N=N,
+                        rhoinf=np.array(RHOINF), factors=np.array(fac).reshape(-1, 2, 3),
+                        ipred=np.array(ipo).reshape(-1, 2), shift=np.array(shift).reshape(-1, 2),
+                        acold=acold, y=y, ac=ac, dy=dy, y_corr=y1, ac_corr=ac1)
+
+
 if __name__ == "__main__":
     for N, seed in ((2, 11), (3, 12)):
         make(N, seed)
         print("wrote", f"ref_box{N}.npz")
+    make_alpha(3, 13)
+    print("wrote ref_alpha_box3.npz")

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import pyoracle
-from phasta_amd import mesh, solver
+from phasta_amd import config, mesh, solver
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -155,6 +155,54 @@ def test_device_time_steps(pair):
     assert [h["iKs"] for h in its] == [h["iKs"] for h in hist]
     assert rel(yo, yo_ref) < TOL
     assert rel(aco, aco_ref) < 1e-10
+
+
+def test_device_time_steps_generalized_alpha(pair):
+    """Two time steps at rho_inf = 0.5 (itrSetup's almi, alfi, gami,
+    itrPC.f:21-25): predictor, the tangent's time factor almi/(gami alfi),
+    corrector and update against the oracle with the same factors, the
+    second step from a nonzero acold.  The predictors the reference leaves
+    undefined (ipred 2-4) are refused."""
+    case, orc, gpu = pair
+    alpha = config.time_integration(0.5)
+    o = pyoracle.Oracle(case, params=dict(case.params, **alpha))
+    acold = np.zeros((case.nshg, 5), order="F")
+    yo_ref, aco_ref, hist = o.time_steps(case.yold, acold, steps=2, newton=2)
+    try:
+        gpu.set_params(**alpha)
+        gpu.upload_state(case.yold, acold)
+        its = []
+        for _ in range(2):
+            gpu.predict()
+            its += [gpu.newton_iteration() for _ in range(2)]
+            gpu.update()
+        yo, aco = gpu.download_old()
+        for ip in (2, 3, 4):
+            gpu.set_params(ipred=ip)
+            with pytest.raises(solver.PhgError):
+                gpu.predict()
+    finally:
+        gpu.set_params(**config.time_integration(-1.0))
+    assert [h["iKs"] for h in its] == [h["iKs"] for h in hist]
+    assert rel(yo, yo_ref) < TOL
+    assert rel(aco, aco_ref) < 1e-10
+
+
+def test_gpu_generalized_alpha_predictor_against_reference_golden():
+    """itrPredict + itrBC at rho_inf = 0.5 from a nonzero acold against the
+    reference's compiled routines (tests/golden/ref_alpha_box3.npz)."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "ref_alpha_box3.npz"))
+    case = mesh.box_case(3, mach=0.3, kspace=10)
+    gpu = solver.GpuSolver(case, params=dict(case.params, **config.time_integration(0.5)))
+    try:
+        gpu.upload_state(case.yold, g["acold"])
+        gpu.predict()
+        y, ac, _ = gpu.download_state()
+    finally:
+        gpu.close()
+    assert rel(y, g["y"]) < TOL
+    assert rel(ac, g["ac"]) < TOL
 
 
 @pytest.mark.parametrize("N", [2, 3])

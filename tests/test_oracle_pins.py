@@ -63,6 +63,68 @@ def test_oracle_bitwise_equals_reference_golden(N):
     assert np.array_equal(aco, g["acold_upd"])
 
 
+# ------------------------------------------------- generalized alpha
+def test_time_integration_equals_reference_itrsetup():
+    """config.time_integration against the reference's compiled itrSetup
+    (fixture tests/golden/ref_alpha_box3.npz): the factors bitwise, the
+    predictor it settles on, and a zero ipred = 4 start-up shift."""
+    g = np.load(os.path.join(GOLD, "ref_alpha_box3.npz"))
+    for r, fac, ipo in zip(g["rhoinf"], g["factors"], g["ipred"]):
+        for k, ip in enumerate((1, 4)):
+            t = config.time_integration(float(r), ip)
+            assert (t["almi"], t["alfi"], t["gami"]) == tuple(fac[k]), (r, ip)
+            assert t["ipred"] == ipo[k], (r, ip)
+    assert not g["shift"].any()
+
+
+def _amplification(rhoinf, lam_dt):
+    """Amplification matrix of the generalized-alpha scheme on y' = lam y
+    in the variables (y, y') (Jansen, Whiting and Hulbert, CMAME 190, 2000):
+    ydot_{n+am} = lam y_{n+af},  y_{n+1} = y_n + dt ydot_n + gam dt (ydot_{n+1} - ydot_n)."""
+    t = config.time_integration(rhoinf)
+    am, af, gm = t["almi"], t["alfi"], t["gami"]
+    # unknowns (y1, v1) with v = dt y'; rows: the two relations above
+    A = np.array([[-lam_dt * af, am], [1.0, -gm]])
+    B = np.array([[lam_dt * (1 - af), -(1 - am)], [1.0, 1.0 - gm]])
+    return np.linalg.solve(A, B)
+
+
+@pytest.mark.parametrize("rhoinf", [0.0, 0.25, 0.5, 1.0])
+def test_generalized_alpha_spectral_radius_and_order(rhoinf):
+    """What the mathematics fixes for the factors: the spectral radius of
+    the amplification matrix at lam dt -> -inf is rho_inf, the scheme is
+    second-order accurate (one-step error against exp(lam dt) falls 8x per
+    halving of dt) and A-stable on the negative real axis."""
+    # (at rho_inf = 0 the double eigenvalue 0 is a Jordan block: the radius
+    # falls as |lam dt|^-1/2, 3e-5 here)
+    G = _amplification(rhoinf, -1e9)
+    assert abs(max(abs(np.linalg.eigvals(G))) - rhoinf) < 1e-4
+    errs = []
+    for h in (0.1, 0.05, 0.025):
+        G = _amplification(rhoinf, -h)
+        errs.append(min(abs(np.linalg.eigvals(G) - np.exp(-h))))
+    assert 7.0 < errs[0] / errs[1] < 9.0 and 7.0 < errs[1] / errs[2] < 9.0
+    for ld in (-1e-2, -1.0, -1e2, -1e4):
+        assert max(abs(np.linalg.eigvals(_amplification(rhoinf, ld)))) <= 1.0 + 1e-12
+
+
+def test_generalized_alpha_step_equals_reference_golden():
+    """itrPredict + itrBC and itrCorrect + itrBC at rho_inf = 0.5 from a
+    nonzero acold, bitwise against the reference's compiled routines; the
+    oracle refuses the predictors the reference leaves undefined."""
+    g = np.load(os.path.join(GOLD, "ref_alpha_box3.npz"))
+    case = mesh.box_case(3, mach=0.3, kspace=10)
+    o = pyoracle.Oracle(case, params=dict(case.params, **config.time_integration(0.5)))
+    acold = np.asfortranarray(g["acold"])
+    y, ac = o.itr_predict_bc(case.yold, acold)
+    assert np.array_equal(y, g["y"]) and np.array_equal(ac, g["ac"])
+    y1, ac1 = o.itr_correct_bc(y, ac, case.yold, acold, np.asfortranarray(g["dy"]))
+    assert np.array_equal(y1, g["y_corr"]) and np.array_equal(ac1, g["ac_corr"])
+    for ip in (2, 3, 4):
+        with pytest.raises(RuntimeError):
+            pyoracle.Oracle(case, params=dict(case.params, ipred=ip)).itr_predict_bc(case.yold)
+
+
 @pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
 def test_oracle_bitwise_equals_live_reference():
     case = mesh.box_case(4, mach=0.5, kspace=10)
