@@ -70,12 +70,12 @@ def cpu_baseline(name):
     return dt, case, out
 
 
-def run_arm(case, precision, newton, steps, warmup, weights=None):
+def run_arm(case, precision, newton, steps, warmup, weights=None, idiff=0):
     """Warm-up, then `steps` timed Newton iterations; returns timings, the
     per-step statistics and the update of the first Newton iteration."""
     import numpy as np
     from phasta_amd import solver
-    gpu = solver.GpuSolver(case, precision=precision)
+    gpu = solver.GpuSolver(case, params=dict(case.params, idiff=idiff), precision=precision)
     if weights is not None:
         w = np.load(weights)
         gpu.mpnn_init(w["dscale"])
@@ -136,6 +136,9 @@ def main():
     ap.add_argument("--arms", default="fp64,mixed,learned",
                     help="comma-separated subset of fp64, mixed, learned")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--idiff", type=int, default=0, choices=(0, 1, 3),
+                    help="global reconstruction of the diffusive flux (0: off, the "
+                         "workload's flag set)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; sharding over several "
@@ -156,7 +159,7 @@ def main():
         prec = solver.PHG_FP64 if a == "fp64" else solver.PHG_MIXED
         wts = WEIGHTS if a == "learned" else None
         print(f"[bench] {args.case} arm {a}", file=sys.stderr, flush=True)
-        res[a] = run_arm(case, prec, spec["newton"], args.steps, args.warmup, wts)
+        res[a] = run_arm(case, prec, spec["newton"], args.steps, args.warmup, wts, args.idiff)
     head_name = "learned" if "learned" in res else ("mixed" if "mixed" in res else "fp64")
     head = res[head_name]
     value = nel / (head["ms"] / 1e3)
@@ -189,7 +192,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "N": spec["N"], "nel": nel, "nshg": case.nshg,
                    "nnz_blocks": case.nnz_tot, "newton_per_time_step": spec["newton"],
-                   "arm": head_name, "pc": "learned" if head_name == "learned" else "block"},
+                   "arm": head_name, "pc": "learned" if head_name == "learned" else "block",
+                   "idiff": args.idiff},
         "roofline": roofline,
         "arms": {a: breakdown(res[a], args.steps, nel) for a in res},
         "copy_GBs": round(head["by_copy"] / (head["ms_copy"] * 1e-3) / 1e9, 1),

@@ -47,6 +47,7 @@ typedef struct phg_params {
   int Kspace, nGMRES, minIters;         /* /solpar/, /incomp/               */
   int lhs, iprec;                       /* itrdrv.f:456-457                 */
   int ipred;                            /* /genpar/ predictor; 1 only      */
+  int idiff;                            /* /genpar/ 0, 1 or 3 (elmgmr.f:361) */
 } phg_params;
 
 /* Mesh arrays as the Fortran host holds them (readnblk.f, genadj.f);
@@ -133,6 +134,11 @@ int phg_newton_iteration(phg_ctx* ctx, phg_stats* st);
 int phg_update(phg_ctx* ctx);
 
 /* Fine-grained entry points (unit parity; host arrays in/out). */
+/* AsIq over the interior elements + qpbc, the global reconstruction of the
+ * diffusive flux for idiff 1 or 3          elmgmr.f:361-391, asiq.f:1,
+ * qpbc.f:1.  qres (nshg,12): the flux, direction-major (x1: momentum x,
+ * y, z, energy; then x2, x3); rmass (nshg): the inverted lumped mass. */
+int phg_asiq(phg_ctx* ctx, const double* y, double* qres, double* rmass);
 /* SparseAp(iper,ilwork,iBC,col,row,lhsK,p)          sparseap.f:1     */
 int phg_sparse_ap(phg_ctx* ctx, const double* lhsK, double* p);
 /* Transpose of SparseAp's operator, p <- K^T p with K p = bc3per(A p(iper))

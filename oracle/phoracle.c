@@ -24,6 +24,8 @@
  *   phSolver/compressible/e3ls.f:1-771       e3LS
  *   phSolver/compressible/e3tau.f:1-283,1397-1507  e3tau (itau=0), e3gijd
  *   phSolver/compressible/e3massr.f, e3massl.f, e3wmlt.f
+ *   phSolver/compressible/asiq.f, e3q.f, e3qvar.f, phSolver/common/qpbc.f
+ *                                            diffusive flux (idiff 1, 3)
  *   phSolver/compressible/bc3lhs.f, bc3res.f, bc3bdg.f, bc3per.f
  *   phSolver/common/fillsparse.f:68-129,234-270  fillsparseC, sparseloc
  *   phSolver/compressible/asbmfg.f, e3b.f, e3bvar.f  wall-face fluxes
@@ -162,6 +164,13 @@ static inline double rowdot(const double M[5][5], int r, const double v[5]) {
 void orc_e3(const orc_params* P, const double yl[4][5],
             const double acl[4][5], const double xl[4][3], double rl[4][5],
             double EG[20][20], int want_lhs) {
+  orc_e3_q(P, yl, acl, xl, NULL, rl, EG, want_lhs);
+}
+
+void orc_e3_q(const orc_params* P, const double yl[4][5],
+              const double acl[4][5], const double xl[4][3],
+              const double ql[4][12], double rl[4][5], double EG[20][20],
+              int want_lhs) {
   const quad_t* Q = quad();
   const int lhs = want_lhs;
   memset(rl, 0, sizeof(double) * 20);
@@ -433,6 +442,19 @@ void orc_e3(const orc_params* P, const double yl[4][5],
     for (int i = 0; i < 5; ++i)
       rLyi[i] = rLyi[i] + A0[i][0] * aci[0] + A0[i][1] * aci[1] +
                 A0[i][2] * aci[2] + A0[i][3] * aci[3] + A0[i][4] * aci[4];
+    /* ---- e3ivar.f:361-395 divergence of the reconstructed diffusive flux,
+     * e3ls.f:148-155 subtracted from the LS residual (isurf = 0) ---- */
+    if (ql) {
+      double divqi[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int n = 0; n < 4; ++n)
+        for (int k = 0; k < 4; ++k)
+          divqi[k] = divqi[k] + shg[n][0] * ql[n][k] + shg[n][1] * ql[n][4 + k] +
+                     shg[n][2] * ql[n][8 + k];
+      rLyi[1] = rLyi[1] - divqi[0];
+      rLyi[2] = rLyi[2] - divqi[1];
+      rLyi[3] = rLyi[3] - divqi[2];
+      rLyi[4] = rLyi[4] - divqi[3];
+    }
     /* ---- e3tau.f:1-283 (itau = 0, ipord = 1) ---- */
     double gijd[6];
     e3gijd(dxidx, gijd);
@@ -561,6 +583,150 @@ void orc_e3(const orc_params* P, const double yl[4][5],
 }
 
 /* e3b.f:111-300 with e3bvar.f:86-375, iBCB = 0 (floating fluxes) */
+/* e3q.f:1-246 with e3qvar.f (idiff = 1 or 3, isurf = 0): one element's
+ * diffusive flux residual ql[a][k] and lumped mass rmassl[a] */
+static void e3q(const orc_params* P, const double yl[4][5], const double xl[4][3],
+                double ql[4][12], double rmassl[4]) {
+  const quad_t* Q = quad();
+  const double Rgas = P->Rgas, gamma = P->gamma, gamma1 = P->gamma1;
+  memset(ql, 0, sizeof(double) * 48);
+  memset(rmassl, 0, sizeof(double) * 4);
+  double alph1 = 0.0, alph2 = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    if (Q->Qwt[q] == 0.0) continue;
+    double shape[4], sg[3][4];
+    for (int a = 0; a < 4; ++a) {
+      shape[a] = Q->shp[q][a];
+      for (int j = 0; j < 3; ++j) sg[j][a] = Q->shgl[q][j][a];
+    }
+    /* e3qvar.f: primitive variables, getthm (ithm = 7), metrics, gradients */
+    double pres = 0, u1 = 0, u2 = 0, u3 = 0, T = 0;
+    for (int n = 0; n < 4; ++n) {
+      pres = pres + shape[n] * yl[n][0];
+      u1 = u1 + shape[n] * yl[n][1];
+      u2 = u2 + shape[n] * yl[n][2];
+      u3 = u3 + shape[n] * yl[n][3];
+      T = T + shape[n] * yl[n][4];
+    }
+    double cp = Rgas * gamma / gamma1;
+    double dxidx[3][3], shg[4][3], WdetJ;
+    e3metric(xl, sg, Q->Qwt[q], dxidx, shg, &WdetJ);
+    double g1[5], g2[5], g3[5];
+    for (int f = 0; f < 5; ++f) {
+      g1[f] = 0.0;
+      g2[f] = 0.0;
+      g3[f] = 0.0;
+    }
+    for (int n = 0; n < 4; ++n)
+      for (int f = 0; f < 5; ++f) {
+        g1[f] = g1[f] + shg[n][0] * yl[n][f];
+        g2[f] = g2[f] + shg[n][1] * yl[n][f];
+        g3[f] = g3[f] + shg[n][2] * yl[n][f];
+      }
+    /* getdiff.f (constant mu, Stokes lambda, DNS) */
+    double rmu = P->mu;
+    double rlm = -PT66 * rmu;
+    double con = rmu * cp / P->pr;
+    double xmut = 0.0;
+    rlm = rlm - PT66 * xmut;
+    rmu = rmu + xmut;
+    double rlm2mu = rlm + TWO * rmu;
+    con = con + xmut * cp / P->pr;
+    /* e3q.f:109-163 diffusive flux at the point */
+    double qdi[12];
+    qdi[0] = rlm2mu * g1[1] + rlm * g2[2] + rlm * g3[3];
+    qdi[1] = rmu * g1[2] + rmu * g2[1];
+    qdi[2] = rmu * g1[3] + rmu * g3[1];
+    qdi[3] = rlm2mu * u1 * g1[1] + rmu * u2 * g1[2] + rmu * u3 * g1[3] + rmu * u2 * g2[1] +
+             rlm * u1 * g2[2] + rmu * u3 * g3[1] + rlm * u1 * g3[3] + con * g1[4];
+    qdi[4] = rmu * g1[2] + rmu * g2[1];
+    qdi[5] = rlm * g1[1] + rlm2mu * g2[2] + rlm * g3[3];
+    qdi[6] = rmu * g2[3] + rmu * g3[2];
+    qdi[7] = rlm * u2 * g1[1] + rmu * u1 * g1[2] + rmu * u1 * g2[1] + rlm2mu * u2 * g2[2] +
+             rmu * u3 * g2[3] + rmu * u3 * g3[2] + rlm * u2 * g3[3] + con * g2[4];
+    qdi[8] = rmu * g1[3] + rmu * g3[1];
+    qdi[9] = rmu * g2[3] + rmu * g3[2];
+    qdi[10] = rlm * g1[1] + rlm * g2[2] + rlm2mu * g3[3];
+    qdi[11] = rlm * u3 * g1[1] + rmu * u1 * g1[3] + rlm * u3 * g2[2] + rmu * u2 * g2[3] +
+              rmu * u1 * g3[1] + rmu * u2 * g3[2] + rlm2mu * u3 * g3[3] + con * g3[4];
+    /* e3q.f:166-180 */
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 12; ++k) ql[i][k] = ql[i][k] + shape[i] * WdetJ * qdi[k];
+    /* e3q.f:185-205 row-sum (idiff 1) or special lumping (idiff 3) */
+    if (P->idiff == 1)
+      for (int i = 0; i < 4; ++i) rmassl[i] = rmassl[i] + shape[i] * WdetJ;
+    if (P->idiff == 3) {
+      double shpsum = 0.0;
+      for (int i = 0; i < 4; ++i) {
+        shpsum = shpsum + shape[i] * shape[i];
+        rmassl[i] = rmassl[i] + shape[i] * shape[i] * WdetJ;
+      }
+      alph1 = alph1 + WdetJ;
+      alph2 = alph2 + shpsum * WdetJ;
+    }
+  }
+  /* e3q.f:236-240 */
+  if (P->idiff == 3)
+    for (int i = 0; i < 4; ++i) rmassl[i] = rmassl[i] * alph1 / alph2;
+}
+
+int orc_asiq(const orc_mesh* m, const orc_params* P, const double* y,
+             double* qres, double* rmass) {
+  if (P->idiff != 1 && P->idiff != 3) return 1;
+  const long long nshg = m->nshg, nel = m->nel;
+  memset(qres, 0, sizeof(double) * 12 * nshg);
+  memset(rmass, 0, sizeof(double) * nshg);
+  const int bk = m->ibksiz > 0 ? m->ibksiz : 64;
+  double(*ql)[4][12] = malloc(sizeof(double) * 48 * bk);
+  double(*rml)[4] = malloc(sizeof(double) * 4 * bk);
+  int(*nod)[4] = malloc(sizeof(int) * 4 * bk);
+  if (!ql || !rml || !nod) return 4;
+  for (long long e0 = 0; e0 < nel; e0 += bk) {
+    int npro = (int)((nel - e0) < bk ? (nel - e0) : bk);
+    for (int e = 0; e < npro; ++e) {
+      double yl[4][5], xl[4][3];
+      for (int a = 0; a < 4; ++a) {
+        long long g = m->ien[(e0 + e) + nel * a] - 1;
+        nod[e][a] = (int)g;
+        yl[a][1] = y[g];
+        yl[a][2] = y[g + nshg];
+        yl[a][3] = y[g + 2 * nshg];
+        yl[a][0] = y[g + 3 * nshg];
+        yl[a][4] = y[g + 4 * nshg];
+        for (int c = 0; c < 3; ++c) xl[a][c] = m->x[g + nshg * c];
+      }
+      e3q(P, (const double(*)[5])yl, (const double(*)[3])xl, ql[e], rml[e]);
+    }
+    /* asiq.f:62-63, local.f:67-74 scatter order */
+    for (int k = 0; k < 12; ++k)
+      for (int i = 0; i < 4; ++i)
+        for (int e = 0; e < npro; ++e)
+          qres[nod[e][i] + nshg * k] = qres[nod[e][i] + nshg * k] + ql[e][i][k];
+    for (int i = 0; i < 4; ++i)
+      for (int e = 0; e < npro; ++e) rmass[nod[e][i]] = rmass[nod[e][i]] + rml[e][i];
+  }
+  free(ql);
+  free(rml);
+  free(nod);
+  /* qpbc.f: periodic fold, copy back, invert the mass, scale */
+  for (long long j = 0; j < nshg; ++j)
+    if (m->iBC[j] & 1024) {
+      long long i = m->iper[j] - 1;
+      rmass[i] = rmass[i] + rmass[j];
+      for (int k = 0; k < 12; ++k) qres[i + nshg * k] = qres[i + nshg * k] + qres[j + nshg * k];
+    }
+  for (long long j = 0; j < nshg; ++j)
+    if (m->iBC[j] & 1024) {
+      long long i = m->iper[j] - 1;
+      rmass[j] = rmass[i];
+      for (int k = 0; k < 12; ++k) qres[j + nshg * k] = qres[i + nshg * k];
+    }
+  for (long long n = 0; n < nshg; ++n) rmass[n] = ONE / rmass[n];
+  for (int k = 0; k < 12; ++k)
+    for (long long n = 0; n < nshg; ++n) qres[n + nshg * k] = rmass[n] * qres[n + nshg * k];
+  return 0;
+}
+
 void orc_e3b(const orc_params* P, const double yl[4][5], const double xl[4][3],
              double rl[4][5]) {
   const quad_t* Q = quad();
@@ -723,6 +889,18 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
   const long long nshg = m->nshg, nel = m->nel, nelb = m->nelb;
   for (long long n = 0; n < nshg; ++n)
     if (!bc_supported(m->iBC[n])) return 3;
+  if (P->idiff != 0 && P->idiff != 1 && P->idiff != 3) return 5;
+  /* elmgmr.f:361-391 global reconstruction of the diffusive flux */
+  double* qres = NULL;
+  if (P->idiff != 0) {
+    qres = malloc(sizeof(double) * 13 * nshg);
+    if (!qres) return 4;
+    int rc = orc_asiq(m, P, y, qres, qres + 12 * nshg);
+    if (rc) {
+      free(qres);
+      return rc;
+    }
+  }
   /* elmgmr.f:399-403 */
   memset(res, 0, sizeof(double) * nshg * 5);
   if (rmes) memset(rmes, 0, sizeof(double) * nshg * 5);
@@ -755,8 +933,13 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
         acl[a][4] = ac[g + 4 * nshg];
         for (int c = 0; c < 3; ++c) xl[a][c] = m->x[g + nshg * c];
       }
-      orc_e3(P, (const double(*)[5])yl, (const double(*)[5])acl,
-             (const double(*)[3])xl, rl[e], EGb[e], P->lhs == 1);
+      double qll[4][12];
+      if (qres) /* asigmr.f:55 */
+        for (int a = 0; a < 4; ++a)
+          for (int k = 0; k < 12; ++k) qll[a][k] = qres[nod[e][a] + nshg * k];
+      orc_e3_q(P, (const double(*)[5])yl, (const double(*)[5])acl,
+               (const double(*)[3])xl, qres ? (const double(*)[12])qll : NULL, rl[e], EGb[e],
+               P->lhs == 1);
       /* asigmr.f:92-103 BDiagl(:,i,j,k) = EGmass(:,5(i-1)+j,5(i-1)+k) */
       if (P->iprec != 0)
         for (int i = 0; i < 4; ++i)
@@ -817,6 +1000,7 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
   free(EGb);
   free(BDl);
   free(nod);
+  free(qres);
 
   /* bc3res.f:25-163 */
   for (long long n = 0; n < nshg; ++n) {

@@ -40,6 +40,7 @@ typedef struct orc_params {
   int Kspace, nGMRES, minIters;         /* /solpar/, /incomp/              */
   int lhs, iprec;                       /* per-call flags (itrdrv.f:456)   */
   int ipred;                            /* predictor 1-4 (itrPC.f:67-119)  */
+  int idiff;                            /* diffusive flux reconstruction   */
 } orc_params;
 
 typedef struct orc_stats {
@@ -68,12 +69,30 @@ void orc_e3(const orc_params* pr, const double yl[4][5],
             const double acl[4][5], const double xl[4][3], double rl[4][5],
             double EGmass[20][20], int want_lhs);
 
+/* e3 with the element's nodal reconstructed diffusive flux ql[a][0..11]
+ * (qres gathered, asigmr.f:55): its divergence is subtracted from the
+ * least-squares residual (e3ivar.f:361-395, e3ls.f:148-155).  ql = NULL
+ * is orc_e3 (idiff = 0). */
+void orc_e3_q(const orc_params* pr, const double yl[4][5],
+              const double acl[4][5], const double xl[4][3],
+              const double ql[4][12], double rl[4][5], double EGmass[20][20],
+              int want_lhs);
+
+/* AsIq over all interior elements + qpbc (elmgmr.f:361-391, asiq.f,
+ * e3q.f, e3qvar.f, qpbc.f): qres (nshg,12) the reconstructed diffusive
+ * flux, rmass (nshg) the inverted lumped mass.  idiff 1 (row-sum lumping)
+ * or 3 (special lumping); returns 1 for other idiff. */
+int orc_asiq(const orc_mesh* m, const orc_params* pr, const double* y,
+             double* qres, double* rmass);
+
 /* e3b for one wall element (iBCB = 0: floating fluxes). rl overwritten. */
 void orc_e3b(const orc_params* pr, const double yl[4][5],
              const double xl[4][3], double rl[4][5]);
 
 /* ElmGMRs (elmgmr.f:280-612).  res,rmes (nshg,5); BDiag (nshg,5,5);
- * lhsK (25,nnz_tot).  lhsK/BDiag touched only when pr->lhs / pr->iprec. */
+ * lhsK (25,nnz_tot).  lhsK/BDiag touched only when pr->lhs / pr->iprec.
+ * idiff 1 or 3 runs orc_asiq first; idiff 2 (qres never formed in the
+ * reference) and other values return 5. */
 int orc_elmgmrs(const orc_mesh* m, const orc_params* pr, const double* y,
                 const double* ac, double* res, double* rmes, double* BDiag,
                 double* lhsK);

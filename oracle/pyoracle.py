@@ -26,7 +26,8 @@ class OrcParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
-               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred")]
+               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred",
+                                               "idiff")]
 
 
 class OrcStats(ctypes.Structure):
@@ -110,6 +111,17 @@ class Oracle:
         if rc:
             raise RuntimeError(f"orc_itr_bc rc={rc}")
         return y, ac
+
+    def asiq(self, y):
+        """AsIq + qpbc (elmgmr.f:361-391): (qres (nshg,12), 1/rmass)."""
+        n = self.case.nshg
+        qres = np.zeros((n, 12), order="F")
+        rmass = np.zeros(n)
+        rc = lib().orc_asiq(ctypes.byref(self.m), ctypes.byref(self.p()),
+                            _p(np.asfortranarray(y)), _p(qres), _p(rmass))
+        if rc:
+            raise RuntimeError(f"orc_asiq rc={rc}")
+        return qres, rmass
 
     def elmgmrs(self, y, ac):
         n, nnz = self.case.nshg, self.case.nnz_tot

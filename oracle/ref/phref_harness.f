@@ -32,7 +32,7 @@ c-----------------------------------------------------------------------
       real(c_double) :: x_(nshg_,3), BC_(nshg_,ndofBC_), dprm(14)
       integer(c_int) :: ien_(nel_,4), ienb_(nelb_,4), iBC_(nshg_),
      &                  iper_(nshg_), colm_(nshg_+1), rowp_(nnz_),
-     &                  iprm(6)
+     &                  iprm(7)
       integer :: iblk, iel0, nprob, k, q, a, j
       real*8 :: qa4, qb4, qA, qB, qC, rr(4,3), r3(3,3), el(4)
       interface
@@ -87,6 +87,10 @@ c.... physics (/mmatpar/, /matdat/, /genpar/, /outpar/, /timdat/)
       iprec = iprm(5)
       LHSupd(1) = 1
       ipred = iprm(6)
+      idiff = iprm(7)
+      idflx = 0
+      if (idiff >= 1) idflx = (nflow-1) * nsd
+      isurf = 0
       rhoinf(1) = -1.0d0
       Jactyp = 0
       itseq = 1

@@ -40,6 +40,13 @@ struct Quad {
   double N[4][4];   // interior shape values N[q][a]
   double Nb[3][4];  // wall-face shape values
   double Qwt, Qwtb;
+  // diffusive flux reconstruction: with M_ab = sum_q N_a N_b, Md[a] =
+  // M_aa - Mo[a] and Mo[a] the mean of M_ab over b != a (the four-point
+  // rule makes the off-diagonal entries equal), so that sum_b M_ab u_b =
+  // Md[a] u_a + Mo[a] sum_b u_b; sN[a] = sum_q N_a; lump3[a] = sum_q N_a^2
+  // * 4 / sum_q sum_b N_b^2 (special lumping, e3q.f:193-205, the equal
+  // point weights cancelled)
+  double Md[4], Mo[4], sN[4], lump3[4];
 };
 
 __constant__ Quad cQ;
@@ -76,6 +83,7 @@ __device__ inline void load_rec(const S* p, S v[16]) {
 struct QpArgs {
   const int* ien;          // 4 planes of nel (0-based)
   const double* node;      // 16 per node: y (p,u,v,w,T), ac (same order), x
+  const double* qnode;     // 16 per node: reconstructed diffusive flux (12)
   void* qps;               // (nel*4) x 13 point states (S)
   void* geo;               // nel x 13 (S)
   double* resid;           // (nel*4) x 5 residual of element e at its node a
@@ -124,8 +132,10 @@ __device__ inline void A0_times(const Thermo& t, const double a[5], double r[5])
 
 // ---------------- pass 1: point states and residual ----------------
 // S: storage type of the point states and geometry (double, or float for
-// the mixed-precision path whose tangent is formed in float).
-template <typename S>
+// the mixed-precision path whose tangent is formed in float).  kDiff: the
+// divergence of the reconstructed diffusive flux is subtracted from the
+// least-squares residual (idiff 1 or 3; e3ivar.f:361-395, e3ls.f:148-155).
+template <typename S, bool kDiff>
 __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   const long long e_raw = blockIdx.x * 64LL + threadIdx.x / 4;
   const int q = threadIdx.x & 3;
@@ -299,6 +309,29 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   }
 #pragma unroll
   for (int f = 0; f < 5; ++f) L[f] += m0[f];
+  if constexpr (kDiff) {
+    // div q = sum_n shg_n,i q_n(i-th direction block) is constant on the
+    // element: lane q takes node q's term, the four lanes are summed
+    const int nq = sel4i(nd[0], nd[1], nd[2], nd[3], q);
+    const double2* qv = reinterpret_cast<const double2*>(A.qnode + 16 * (size_t)nq);
+    double qq[12];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double2 t2 = qv[k];
+      qq[2 * k] = t2.x;
+      qq[2 * k + 1] = t2.y;
+    }
+    const double s0 = sel4(shg[0][0], shg[1][0], shg[2][0], shg[3][0], q);
+    const double s1 = sel4(shg[0][1], shg[1][1], shg[2][1], shg[3][1], q);
+    const double s2 = sel4(shg[0][2], shg[1][2], shg[2][2], shg[3][2], q);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double dq = s0 * qq[m] + s1 * qq[4 + m] + s2 * qq[8 + m];
+      dq += __shfl_xor(dq, 1, 64);
+      dq += __shfl_xor(dq, 2, 64);
+      L[1 + m] -= dq;
+    }
+  }
   L[0] *= t.tau[0];
   L[1] *= t.tau[1];
   L[2] *= t.tau[1];
@@ -804,6 +837,187 @@ __global__ void k_fold_res_bdiag(double* res, double* bdiag, const int* master,
   }
 }
 
+// ---------------- global reconstruction of the diffusive flux ----------------
+// (AsIq/e3q/e3qvar + qpbc, elmgmr.f:361-391, idiff 1 or 3.)  For linear tets
+// the velocity and temperature gradients are constant on an element, so the
+// point flux qdi (e3q.f:109-163) is, per direction i, the viscous stress row
+// tau_i (momentum) and u(x_q) . tau_i + k T,i (energy).  Its nodal residual
+// ql(a) = sum_q N_a(x_q) WdetJ qdi(x_q) is therefore sN_a W tau_i and
+// (Md_a u_a + Mo_a sum_b u_b) . W tau_i + sN_a W k T,i: k_q_elem stores per
+// element W tau (6), W k grad T (3), (sum_b u_b) . W tau_i (3) and W, and
+// k_q_nodes sums over the node's elements in element order with its own
+// velocity u_a.  k_q_fold and k_q_scale apply qpbc (periodic fold, copy
+// back, q / lumped mass).
+constexpr int kQrec = 16;  // element record (13 values + pad): one 128-byte line
+
+__global__ void __launch_bounds__(256) k_q_elem(const int* __restrict__ ien,
+                                                const double* __restrict__ node,
+                                                double* __restrict__ qrec, long long nel, double mu,
+                                                double lam, double kap) {
+  // contiguous element ranges per XCD (workgroups go round robin over 8;
+  // the grid is a multiple of 8), so neighbouring elements share an L2
+  const long long per = gridDim.x / 8;
+  const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const long long e = wg * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  double xl[4][3], vl[4][4];  // coordinates; u, v, w, T
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2* v = reinterpret_cast<const double2*>(node + 16 * (size_t)ien[k * nel + e]);
+    const double2 pu = v[0], vw = v[1], tp = v[2], x01 = v[5], x2 = v[6];
+    vl[k][0] = pu.y;
+    vl[k][1] = vw.x;
+    vl[k][2] = vw.y;
+    vl[k][3] = tp.x;
+    xl[k][0] = x01.x;
+    xl[k][1] = x01.y;
+    xl[k][2] = x2.x;
+  }
+  double d[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d[i][j] = 0.5 * (xl[j][i] - xl[3][i]);
+  double ix[3][3];
+  ix[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
+  ix[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
+  ix[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  const double det = ix[0][0] * d[0][0] + ix[0][1] * d[1][0] + ix[0][2] * d[2][0];
+  const double idet = 1.0 / det;
+  ix[0][0] *= idet;
+  ix[0][1] *= idet;
+  ix[0][2] *= idet;
+  ix[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * idet;
+  ix[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * idet;
+  ix[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * idet;
+  ix[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * idet;
+  ix[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * idet;
+  ix[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * idet;
+  const double W = cQ.Qwt * det;
+  // g[i][f]: d/dx_i of (u, v, w, T)
+  double g[3][4];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double s0 = 0.5 * ix[0][i], s1 = 0.5 * ix[1][i], s2 = 0.5 * ix[2][i];
+    const double s3 = -(s0 + s1 + s2);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) g[i][f] = s0 * vl[0][f] + s1 * vl[1][f] + s2 * vl[2][f] + s3 * vl[3][f];
+  }
+  const double l2m = lam + 2.0 * mu;
+  double r[kQrec];
+  r[0] = W * (l2m * g[0][0] + lam * g[1][1] + lam * g[2][2]);  // tau_11
+  r[1] = W * (mu * (g[0][1] + g[1][0]));                       // tau_12
+  r[2] = W * (mu * (g[0][2] + g[2][0]));                       // tau_13
+  r[3] = W * (lam * g[0][0] + l2m * g[1][1] + lam * g[2][2]);  // tau_22
+  r[4] = W * (mu * (g[1][2] + g[2][1]));                       // tau_23
+  r[5] = W * (lam * g[0][0] + lam * g[1][1] + l2m * g[2][2]);  // tau_33
+  r[6] = W * (kap * g[0][3]);
+  r[7] = W * (kap * g[1][3]);
+  r[8] = W * (kap * g[2][3]);
+  const double su[3] = {vl[0][0] + vl[1][0] + vl[2][0] + vl[3][0],
+                        vl[0][1] + vl[1][1] + vl[2][1] + vl[3][1],
+                        vl[0][2] + vl[1][2] + vl[2][2] + vl[3][2]};
+  r[9] = su[0] * r[0] + su[1] * r[1] + su[2] * r[2];
+  r[10] = su[0] * r[1] + su[1] * r[3] + su[2] * r[4];
+  r[11] = su[0] * r[2] + su[1] * r[4] + su[2] * r[5];
+  r[12] = W;
+  r[13] = r[14] = r[15] = 0.0;
+  double2* o = reinterpret_cast<double2*>(qrec + (size_t)kQrec * e);
+#pragma unroll
+  for (int k = 0; k < kQrec / 2; ++k) o[k] = make_double2(r[2 * k], r[2 * k + 1]);
+}
+
+// one lane per node, 64-node groups: the node's elements in element order,
+// stored sliced (entry j of lane l of group g at inc[goff[g] + 64 j + l],
+// -1 past the node's count) so that the lanes' reads coalesce
+__global__ void __launch_bounds__(256) k_q_nodes(const long long* __restrict__ goff,
+                                                 const int* __restrict__ inc,
+                                                 const double* __restrict__ node,
+                                                 const double* __restrict__ qrec,
+                                                 double* __restrict__ qnode, long long nshg,
+                                                 int idiff) {
+  // contiguous node ranges per XCD (grid a multiple of 8): the records of an
+  // element are read by its four nodes through one L2
+  const long long per = gridDim.x / 8;
+  const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const long long n = wg * blockDim.x + threadIdx.x;
+  if (n >= nshg) return;
+  const long long g = n >> 6;
+  const int lane = (int)(n & 63);
+  const long long o0 = goff[g];
+  const int deg = (int)((goff[g + 1] - o0) >> 6);
+  const double2* nv = reinterpret_cast<const double2*>(node + 16 * (size_t)n);
+  const double2 pu = nv[0], vw = nv[1];
+  const double un[3] = {pu.y, vw.x, vw.y};
+  double acc[13];
+#pragma unroll
+  for (int k = 0; k < 13; ++k) acc[k] = 0.0;
+  for (int j = 0; j < deg; ++j) {
+    const int code = inc[o0 + 64LL * j + lane];
+    if (code < 0) continue;
+    const long long e = code >> 2;
+    const int a = code & 3;
+    const double2* rv = reinterpret_cast<const double2*>(qrec + (size_t)kQrec * e);
+    double r[14];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const double2 t2 = rv[k];
+      r[2 * k] = t2.x;
+      r[2 * k + 1] = t2.y;
+    }
+    const double sN = sel4(cQ.sN[0], cQ.sN[1], cQ.sN[2], cQ.sN[3], a);
+    const double md = sel4(cQ.Md[0], cQ.Md[1], cQ.Md[2], cQ.Md[3], a);
+    const double mo = sel4(cQ.Mo[0], cQ.Mo[1], cQ.Mo[2], cQ.Mo[3], a);
+    // stress rows: direction 1 (11, 12, 13), 2 (12, 22, 23), 3 (13, 23, 33)
+    const double t1[3] = {r[0], r[1], r[2]}, t2[3] = {r[1], r[3], r[4]}, t3[3] = {r[2], r[4], r[5]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      acc[k] += sN * t1[k];
+      acc[4 + k] += sN * t2[k];
+      acc[8 + k] += sN * t3[k];
+    }
+    acc[3] += md * (un[0] * t1[0] + un[1] * t1[1] + un[2] * t1[2]) + mo * r[9] + sN * r[6];
+    acc[7] += md * (un[0] * t2[0] + un[1] * t2[1] + un[2] * t2[2]) + mo * r[10] + sN * r[7];
+    acc[11] += md * (un[0] * t3[0] + un[1] * t3[1] + un[2] * t3[2]) + mo * r[11] + sN * r[8];
+    const double lump = idiff == 3 ? sel4(cQ.lump3[0], cQ.lump3[1], cQ.lump3[2], cQ.lump3[3], a) : sN;
+    acc[12] += lump * r[12];
+  }
+  double2* o = reinterpret_cast<double2*>(qnode + 16 * (size_t)n);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) o[k] = make_double2(acc[2 * k], acc[2 * k + 1]);
+  o[6] = make_double2(acc[12], 0.0);
+  o[7] = make_double2(0.0, 0.0);
+}
+
+// qpbc.f: master += slaves (ascending), then the slaves copy the master
+__global__ void k_q_fold(double* qnode, const int* master, const int* ptr, const int* slave,
+                         int nfold) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nfold) return;
+  double* qm = qnode + 16 * (size_t)master[i];
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const double* qs = qnode + 16 * (size_t)slave[k];
+#pragma unroll
+    for (int f = 0; f < 13; ++f) qm[f] += qs[f];
+  }
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    double* qs = qnode + 16 * (size_t)slave[k];
+#pragma unroll
+    for (int f = 0; f < 13; ++f) qs[f] = qm[f];
+  }
+}
+
+// qpbc.f: rmass = 1/rmass, q = rmass q
+__global__ void k_q_scale(double* qnode, long long nshg) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= nshg) return;
+  double* q = qnode + 16 * (size_t)n;
+  const double r = 1.0 / q[12];
+#pragma unroll
+  for (int f = 0; f < 12; ++f) q[f] = r * q[f];
+  q[12] = r;
+}
+
 template <typename T>
 __global__ void k_fill(T* p, size_t n) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -834,6 +1048,26 @@ int upload_quadrature() {
   }
   h.Qwt = (4.0 / 3.0) * 0.25;
   h.Qwtb = 2.0 * A;
+  double ssq = 0.0;
+  for (int a = 0; a < 4; ++a) {
+    double M[4];
+    for (int b = 0; b < 4; ++b) {
+      M[b] = 0.0;
+      for (int q = 0; q < 4; ++q) M[b] += h.N[q][a] * h.N[q][b];
+    }
+    h.Mo[a] = 0.0;
+    for (int b = 0; b < 4; ++b)
+      if (b != a) h.Mo[a] += M[b] / 3.0;
+    h.Md[a] = M[a] - h.Mo[a];
+    h.sN[a] = 0.0;
+    h.lump3[a] = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      h.sN[a] += h.N[q][a];
+      h.lump3[a] += h.N[q][a] * h.N[q][a];
+    }
+    ssq += h.lump3[a];
+  }
+  for (int a = 0; a < 4; ++a) h.lump3[a] *= 4.0 / ssq;
   PHG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cQ), &h, sizeof(Quad)));
   g_quad_ready = true;
   return 0;
@@ -862,25 +1096,58 @@ int launch_pack_nodes(phg_ctx* c) {
   return 0;
 }
 
+int launch_diffusive_flux(phg_ctx* c) {
+  if (!c->qnode || !c->qrec) {
+    phg_set_error("diffusive flux buffers not allocated");
+    return PHG_EINVAL;
+  }
+  PHG_TRY(upload_quadrature());
+  const PhysParams ph = phys_of(c->prm);
+  const double cp = ph.Rgas * ph.gamma / ph.gamma1;
+  const long long nel = c->nel, n = c->nshg;
+  auto xcd_grid = [](long long items) { return (int)(((items + 255) / 256 + 7) / 8 * 8); };
+  k_q_elem<<<xcd_grid(nel), 256, 0, c->stream>>>(c->ienc, c->nodepack, c->qrec, nel, ph.mu,
+                                                 -(2.0 / 3.0) * ph.mu, ph.mu * cp / ph.pr);
+  PHG_HIP(hipGetLastError());
+  k_q_nodes<<<xcd_grid(n), 256, 0, c->stream>>>(c->inc_goff, c->inc, c->nodepack, c->qrec,
+                                                c->qnode, n, c->prm.idiff);
+  PHG_HIP(hipGetLastError());
+  if (c->nfold > 0) {
+    k_q_fold<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(c->qnode, c->fold_master, c->fold_ptr,
+                                                           c->fold_slave, c->nfold);
+    PHG_HIP(hipGetLastError());
+  }
+  k_q_scale<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->qnode, n);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   PHG_TRY(upload_quadrature());
   PHG_TRY(launch_pack_nodes(c));
   const bool mixed = c->precision == PHG_MIXED;
+  const bool diff = c->prm.idiff != 0;
+  if (diff) PHG_TRY(launch_diffusive_flux(c));
   const PhysParams ph = phys_of(c->prm);
   // pass 1: point states and element residuals
   QpArgs Q{};
   Q.ien = c->ienc;
   Q.node = c->nodepack;
+  Q.qnode = c->qnode;
   Q.qps = mixed ? (void*)c->qps32 : (void*)c->qps;
   Q.geo = mixed ? (void*)c->geo32 : (void*)c->geo;
   Q.resid = c->resid;
   Q.nel = c->nel;
   Q.ph = ph;
   const int g1 = (int)((c->nel + 63) / 64);
-  if (mixed)
-    k_asm_qp<float><<<g1, 256, 0, c->stream>>>(Q);
+  if (mixed && diff)
+    k_asm_qp<float, true><<<g1, 256, 0, c->stream>>>(Q);
+  else if (mixed)
+    k_asm_qp<float, false><<<g1, 256, 0, c->stream>>>(Q);
+  else if (diff)
+    k_asm_qp<double, true><<<g1, 256, 0, c->stream>>>(Q);
   else
-    k_asm_qp<double><<<g1, 256, 0, c->stream>>>(Q);
+    k_asm_qp<double, false><<<g1, 256, 0, c->stream>>>(Q);
   PHG_HIP(hipGetLastError());
   // pass 2: block owners (res, BDiag, lhsK written once)
   BlkArgs B{};

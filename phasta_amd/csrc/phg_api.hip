@@ -300,6 +300,31 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   c->nlanes = (long long)lanes.size();
   blocks.clear();
   blocks.shrink_to_fit();
+  // ---- node -> (element, local node) incidence, element order per node
+  // (the scatter of AsIq's element flux residuals, asiq.f:62-63), sliced
+  // by 64-node groups: entry j of node 64 g + l at goff[g] + 64 j + l ----
+  const long long ngq = (nshg + 63) / 64;
+  std::vector<long long> goff(ngq + 1, 0);
+  std::vector<int> incs;
+  {
+    std::vector<int> incp(nshg + 1, 0), incl(4 * nel);
+    for (long long i = 0; i < 4 * nel; ++i) incp[ienc[i] + 1]++;
+    for (long long n = 0; n < nshg; ++n) incp[n + 1] += incp[n];
+    std::vector<int> pos(incp.begin(), incp.end() - 1);
+    for (long long e = 0; e < nel; ++e)
+      for (int a = 0; a < 4; ++a) incl[pos[ienc[a * nel + e]]++] = (int)(e * 4 + a);
+    for (long long g = 0; g < ngq; ++g) {
+      int deg = 0;
+      for (long long n = 64 * g; n < std::min(nshg, 64 * g + 64); ++n)
+        deg = std::max(deg, incp[n + 1] - incp[n]);
+      goff[g + 1] = goff[g] + 64LL * deg;
+    }
+    incs.assign(goff[ngq], -1);
+    for (long long n = 0; n < nshg; ++n) {
+      const long long g = n / 64, l = n % 64;
+      for (int j = 0; j < incp[n + 1] - incp[n]; ++j) incs[goff[g] + 64LL * j + l] = incl[incp[n] + j];
+    }
+  }
   std::vector<int> ienbc(4 * std::max<long long>(nelb, 1));
   for (long long p = 0; p < nelb; ++p)
     for (int a = 0; a < 4; ++a) ienbc[a * nelb + p] = m->ienb[orderb[p] + nelb * a] - 1;
@@ -367,6 +392,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
     PHG_TRY(dev_alloc(c, &c->lhs_aos, (size_t)aos_stride<double>() * nnz));
   PHG_TRY(dev_alloc(c, &c->acc_bd, 25 * nshg));
   PHG_TRY(dev_alloc(c, &c->nodepack, 16 * nshg));
+  PHG_TRY(dev_alloc(c, &c->inc_goff, goff.size()));
+  PHG_TRY(dev_alloc(c, &c->inc, incs.size()));
   if (mixed)
     PHG_TRY(dev_alloc(c, &c->lhs32, c->nslots * kGroup * 25));
   else
@@ -390,6 +417,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   std::copy(m->x, m->x + 3 * nshg, xs.begin());
   PHG_TRY(upload(c, c->x, xs.data(), xs.size()));
   PHG_TRY(upload(c, c->ienc, ienc.data(), ienc.size()));
+  PHG_TRY(upload(c, c->inc_goff, goff.data(), goff.size()));
+  PHG_TRY(upload(c, c->inc, incs.data(), incs.size()));
   PHG_TRY(upload(c, c->lanes, lanes.data(), lanes.size()));
   PHG_TRY(upload(c, c->blocks, blk_lane.data(), blk_lane.size()));
   PHG_TRY(upload(c, c->ienbc, ienbc.data(), ienbc.size()));
@@ -430,9 +459,25 @@ int ensure_tmp(phg_ctx* c, size_t bytes) {
 double* kvec(phg_ctx* c, int k) { return c->krylov + (size_t)k * 5 * c->nshg; }
 
 // ElmGMRs on the resident y/ac: res, rmes(=0), BDiag, lhsK (elmgmr.f:395-610)
+// idiff 0, 1 or 3; the flux buffers are allocated when first needed
+int check_idiff(phg_ctx* c) {
+  const int d = c->prm.idiff;
+  if (d != 0 && d != 1 && d != 3) {
+    phg_set_error("idiff %d: 0, 1 (row-sum) or 3 (special lumping) supported; with 2 the "
+                  "reference never forms qres (elmgmr.f:361)", d);
+    return PHG_EINVAL;
+  }
+  if (d != 0 && !c->qnode) {
+    PHG_TRY(dev_alloc(c, &c->qrec, (size_t)c->nel * 16));
+    PHG_TRY(dev_alloc(c, &c->qnode, (size_t)c->nshg * 16));
+  }
+  return 0;
+}
+
 int elm_core(phg_ctx* c) {
   const long long n = c->nshg;
   const bool lhs = c->prm.lhs == 1, prec = c->prm.iprec != 0;
+  PHG_TRY(check_idiff(c));
   // every node owns its diagonal block and every CSR entry has an owner, so
   // the assembly stores res, BDiag and lhsK completely: no clearing
   PHG_TRY(launch_assembly(c, lhs, prec));
@@ -807,7 +852,7 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
                   c->acold,    c->dy,       c->krylov, c->red_part, c->scal,
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
-                  c->nodepack};
+                  c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -835,6 +880,27 @@ extern "C" int phg_info(phg_ctx* c, int* ncolors, int* ncolors_b, long long* dev
   if (ncolors) *ncolors = 0;  // interior elements are not coloured
   if (ncolors_b) *ncolors_b = (int)c->colb_off.size() - 1;
   if (dev_bytes) *dev_bytes = c->dev_bytes;
+  return 0;
+}
+
+extern "C" int phg_asiq(phg_ctx* c, const double* y, double* qres, double* rmass) {
+  PHG_TRY(check_ctx(c));
+  if (c->prm.idiff != 1 && c->prm.idiff != 3) {
+    phg_set_error("phg_asiq: idiff must be 1 or 3");
+    return PHG_EINVAL;
+  }
+  PHG_TRY(check_idiff(c));
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->y, y, 5 * n));
+  PHG_TRY(launch_pack_nodes(c));
+  PHG_TRY(launch_diffusive_flux(c));
+  std::vector<double> h((size_t)16 * n);
+  PHG_TRY(download(c, h.data(), c->qnode, h.size()));
+  PHG_TRY(sync(c));
+  for (long long i = 0; i < n; ++i) {
+    for (int k = 0; k < 12; ++k) qres[i + n * k] = h[16 * i + k];
+    rmass[i] = h[16 * i + 12];
+  }
   return 0;
 }
 

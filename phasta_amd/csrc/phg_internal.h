@@ -226,6 +226,12 @@ struct phg_ctx {
   double* lhs_aos = nullptr;       // aos_stride<double>()*nnz, reference lhsK order
   float* lhs_aos32 = nullptr;      // aos_stride<float>()*nnz (mixed)
   double* nodepack = nullptr;      // 16 per node: y (local order), ac, x
+  // global reconstruction of the diffusive flux (idiff 1, 3)
+  long long* inc_goff = nullptr;   // per 64-node group: offset of its sliced entries
+  int* inc = nullptr;              // e*4 + a per (node, incident element), sliced by
+                                   // 64-node groups, element order per node, -1 padding
+  double* qrec = nullptr;          // nel x 16 element records (allocated on first use)
+  double* qnode = nullptr;         // 16 per node: q (12), 1/lumped mass
 
   // nodal vectors (SoA planes of nshg)
   double* res = nullptr;
@@ -262,6 +268,7 @@ struct phg_ctx {
 // ---- launchers (k_assembly.hip) ----
 int launch_fill(phg_ctx* c, void* p, size_t bytes);
 int launch_assembly(phg_ctx* c, bool lhs, bool prec);
+int launch_diffusive_flux(phg_ctx* c);
 int launch_bc_res_bdiag(phg_ctx* c, bool prec);
 
 // ---- learned preconditioner (k_mpnn.hip) ----

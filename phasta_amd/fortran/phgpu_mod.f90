@@ -18,7 +18,7 @@ module phgpu
     real(c_double) :: etol, epsM
     integer(c_int) :: Kspace, nGMRES, minIters
     integer(c_int) :: lhs, iprec
-    integer(c_int) :: ipred
+    integer(c_int) :: ipred, idiff
   end type
 
   type, bind(C) :: phg_mesh

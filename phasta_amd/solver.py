@@ -38,7 +38,8 @@ class PhgParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
-               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred")]
+               [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred",
+                                               "idiff")]
 
 
 class PhgMesh(ctypes.Structure):
@@ -70,7 +71,7 @@ class PhgStats(ctypes.Structure):
 EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_info",
            "phg_elmgmrs", "phg_solgmrs", "phg_state_upload", "phg_state_download",
            "phg_state_download_old", "phg_state_upload_cur", "phg_predict",
-           "phg_newton_iteration", "phg_update", "phg_sparse_ap", "phg_sparse_ap_t",
+           "phg_newton_iteration", "phg_update", "phg_asiq", "phg_sparse_ap", "phg_sparse_ap_t",
            "phg_i3lu", "phg_spsi3pre", "phg_bc3per",
            "phg_bench_spmv", "phg_bench_assembly", "phg_bench_copy", "phg_sync",
            "phg_mpnn_param_count", "phg_mpnn_init", "phg_mpnn_set_weights",
@@ -186,6 +187,14 @@ class GpuSolver:
         out = st.as_dict()
         out.update(res=res, rmes=rmes, BDiag=bd, lhsK=lk, Dy=dy)
         return out
+
+    def asiq(self, y):
+        """AsIq + qpbc (idiff 1 or 3): (qres (nshg,12), 1/lumped mass)."""
+        n = self.case.nshg
+        qres = np.zeros((n, 12), order="F")
+        rmass = np.zeros(n)
+        _check(lib().phg_asiq(self.h, _p(_f64(y, (n, 5))), _p(qres), _p(rmass)))
+        return qres, rmass
 
     def sparse_ap(self, lhsK, p):
         p = _f64(p, (self.case.nshg, 5)).copy(order="F")

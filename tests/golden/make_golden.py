@@ -79,9 +79,27 @@ def make_alpha(N, seed):
                         acold=acold, y=y, ac=ac, dy=dy, y_corr=y1, ac_corr=ac1)
 
 
+def make_idiff(N, seed):
+    """ElmGMRs with the global reconstruction of the diffusive flux
+    (idiff 1 and 3) on a jittered box at a perturbed state."""
+    case = mesh.box_case(N, mach=0.5, kspace=10, jitter=0.1)
+    rng = np.random.default_rng(seed)
+    y, ac = pyref.Reference(case).predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.02 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    out = {}
+    for idiff in (1, 3):
+        res, rmes, bd, lk = pyref.Reference(case, params=dict(case.params, idiff=idiff)).elmgmrs(y, ac)
+        out[f"res{idiff}"] = res
+    out["BDiag"], out["lhsK"] = bd, lk
+    np.savez_compressed(os.path.join(HERE, f"ref_idiff_box{N}.npz"), N=N, y=y, ac=ac, **out)
+
+
 if __name__ == "__main__":
     for N, seed in ((2, 11), (3, 12)):
         make(N, seed)
         print("wrote", f"ref_box{N}.npz")
     make_alpha(3, 13)
     print("wrote ref_alpha_box3.npz")
+    make_idiff(3, 14)
+    print("wrote ref_idiff_box3.npz")

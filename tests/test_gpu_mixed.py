@@ -52,6 +52,25 @@ def test_mixed_elmgmrs_error(trio):
     assert e_lk > 0.0  # the tangent really is float
 
 
+def test_mixed_elmgmrs_with_flux_reconstruction(trio):
+    """idiff 1: the residual with the divergence of the reconstructed
+    diffusive flux is double in the mixed path too (same bar)."""
+    case, orc, gpu = trio
+    o = pyoracle.Oracle(case, params=dict(case.params, idiff=1))
+    y, ac = o.itr_predict_bc(case.yold)
+    rng = np.random.default_rng(2)
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    r_res = o.elmgmrs(y, ac)[0]
+    try:
+        gpu.set_params(idiff=1)
+        g_res = gpu.elmgmrs(y, ac)[0]
+    finally:
+        gpu.set_params(idiff=0)
+    e = rel(g_res, r_res)
+    print(f"[mixed] idiff 1 err res {e:.2e}", flush=True)
+    assert e < TOL_RES
+
+
 def test_mixed_solgmrs_meets_fp64_tolerance(trio):
     case, orc, gpu = trio
     y, ac = orc.itr_predict_bc(case.yold)

@@ -157,6 +157,60 @@ def test_device_time_steps(pair):
     assert rel(aco, aco_ref) < 1e-10
 
 
+@pytest.mark.parametrize("idiff", [1, 3])
+def test_flux_reconstruction(pair, idiff):
+    """idiff 1 / 3 (elmgmr.f:361-391): the reconstructed diffusive flux and
+    inverted lumped mass (phg_asiq: AsIq + qpbc), ElmGMRs's outputs with the
+    flux divergence in the least-squares residual, and two device Newton
+    iterations, against the oracle with the same flag."""
+    case, orc, gpu = pair
+    o = pyoracle.Oracle(case, params=dict(case.params, idiff=idiff))
+    y, ac = o.itr_predict_bc(case.yold)
+    rng = np.random.default_rng(6)
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    q_ref, rm_ref = o.asiq(y)
+    r_res, _, r_bd, r_lk = o.elmgmrs(y, ac)
+    acold = np.zeros((case.nshg, 5), order="F")
+    y_ref, _, hist = o.newton_step(case.yold, acold, newton=2)
+    try:
+        gpu.set_params(idiff=idiff)
+        q, rm = gpu.asiq(y)
+        g_res, _, g_bd, g_lk = gpu.elmgmrs(y, ac)
+        gpu.upload_state(case.yold, acold)
+        gpu.predict()
+        its = [gpu.newton_iteration() for _ in range(2)]
+        y_g, _, _ = gpu.download_state()
+    finally:
+        gpu.set_params(idiff=0)
+    print(f"[parity] idiff {idiff}: q {rel(q, q_ref):.2e} rmass {rel(rm, rm_ref):.2e} "
+          f"res {rel(g_res, r_res):.2e}", flush=True)
+    assert rel(q, q_ref) < TOL and rel(rm, rm_ref) < TOL
+    assert rel(g_res, r_res) < TOL
+    assert rel(g_bd, r_bd) < TOL and rel(g_lk, r_lk) < TOL
+    assert [h["iKs"] for h in its] == [h["iKs"] for h in hist]
+    assert rel(y_g, y_ref) < TOL
+
+
+def test_gpu_flux_reconstruction_against_reference_golden():
+    """ElmGMRs's residual with idiff 1 and 3 against the reference's
+    compiled routines (tests/golden/ref_idiff_box3.npz); idiff 2 refused."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "ref_idiff_box3.npz"))
+    case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+    gpu = solver.GpuSolver(case)
+    try:
+        for idiff in (1, 3):
+            gpu.set_params(idiff=idiff)
+            res, _, bd, lk = gpu.elmgmrs(g["y"], g["ac"])
+            assert rel(res, g[f"res{idiff}"]) < TOL, idiff
+            assert rel(bd, g["BDiag"]) < TOL and rel(lk, g["lhsK"]) < TOL
+        gpu.set_params(idiff=2)
+        with pytest.raises(solver.PhgError):
+            gpu.elmgmrs(g["y"], g["ac"])
+    finally:
+        gpu.close()
+
+
 def test_device_time_steps_generalized_alpha(pair):
     """Two time steps at rho_inf = 0.5 (itrSetup's almi, alfi, gami,
     itrPC.f:21-25): predictor, the tangent's time factor almi/(gami alfi),

@@ -125,6 +125,69 @@ def test_generalized_alpha_step_equals_reference_golden():
             pyoracle.Oracle(case, params=dict(case.params, ipred=ip)).itr_predict_bc(case.yold)
 
 
+# ------------------------------ global reconstruction of the diffusive flux
+def test_idiff_elmgmrs_equals_reference_golden():
+    """ElmGMRs with idiff 1 and 3 (AsIq, e3q, qpbc and the divergence term
+    of e3ivar/e3LS) bitwise against the reference's compiled routines
+    (fixture tests/golden/ref_idiff_box3.npz); the tangent does not change."""
+    g = np.load(os.path.join(GOLD, "ref_idiff_box3.npz"))
+    case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+    y, ac = np.asfortranarray(g["y"]), np.asfortranarray(g["ac"])
+    for idiff in (1, 3):
+        o = pyoracle.Oracle(case, params=dict(case.params, idiff=idiff))
+        res, rmes, bd, lk = o.elmgmrs(y, ac)
+        assert np.array_equal(res, g[f"res{idiff}"]), idiff
+        assert np.array_equal(bd, g["BDiag"]) and np.array_equal(lk, g["lhsK"])
+    res0 = pyoracle.Oracle(case).elmgmrs(y, ac)[0]
+    assert rel(g["res1"], res0) > 1e-9      # the term is there
+    with pytest.raises(RuntimeError):
+        pyoracle.Oracle(case, params=dict(case.params, idiff=2)).elmgmrs(y, ac)
+
+
+@pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
+def test_idiff_elmgmrs_bitwise_equals_live_reference():
+    case = mesh.box_case(4, mach=0.3, kspace=10, jitter=0.1, grade=1.1)
+    rng = np.random.default_rng(8)
+    o0 = pyoracle.Oracle(case)
+    y, ac = o0.itr_predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.02 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    for idiff in (1, 3):
+        prm = dict(case.params, idiff=idiff)
+        for a, b in zip(pyoracle.Oracle(case, params=prm).elmgmrs(y, ac),
+                        pyref.Reference(case, params=prm).elmgmrs(y, ac)):
+            assert np.array_equal(a, b), idiff
+
+
+def test_asiq_reconstructs_linear_fields_exactly():
+    """What the mathematics fixes for AsIq + qpbc: on a uniform box (every
+    node patch point-symmetric) the lumped L2 projection of a piecewise
+    linear flux is exact at the nodes off the walls, so the shear flow
+    u = a y + b, T = T0 + c y gives tau_12 = tau_21 = mu a and the y
+    energy flux u tau_21 + k T,y = (a y + b) mu a + k c, every other
+    component zero; the lumped masses sum to the box volume."""
+    case = mesh.box_case(4, jitter=0.0)
+    a, b, c = 37.0, 11.0, 25.0
+    x = case.x
+    y = np.zeros((case.nshg, 5), order="F")
+    y[:, 0] = a * x[:, 1] + b
+    y[:, 3] = 1e5
+    y[:, 4] = 300.0 + c * x[:, 1]
+    for idiff in (1, 3):
+        o = pyoracle.Oracle(case, params=dict(case.params, idiff=idiff))
+        qres, rmass = o.asiq(y)
+        mu = config.MU
+        kap = mu * (config.RGAS * config.GAMMA / config.GAMMA1) / config.PRANDTL
+        exact = np.zeros_like(qres)
+        exact[:, 1] = exact[:, 4] = mu * a
+        exact[:, 7] = (a * x[:, 1] + b) * mu * a + kap * c
+        inner = (x[:, 1] > 1e-9) & (x[:, 1] < 1 - 1e-9)
+        err = np.abs(qres[inner] - exact[inner]).max() / np.abs(exact).max()
+        assert err < 1e-12, (idiff, err)
+        slave = (case.iBC & 1024) != 0
+        assert abs(np.sum(1.0 / rmass[~slave]) - 1.0) < 1e-12
+
+
 @pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
 def test_oracle_bitwise_equals_live_reference():
     case = mesh.box_case(4, mach=0.5, kspace=10)
