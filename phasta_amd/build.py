@@ -1,6 +1,7 @@
 """In-tree build of the native libraries.
 
 * ``phasta_amd/libphmesh.so``  host-only box-mesh generator (g++/OpenMP)
+  and the phastaIO file format (include/phastaio.h)
 * ``phasta_amd/libphgpu.so``   the HIP library behind ``include/phgpu.h``,
   gfx950 only, packed-FP32 instructions disabled
 * ``phasta_amd/fortran/phg_boxdrv``  Fortran host (amdflang, iso_c_binding
@@ -29,7 +30,8 @@ def _run(cmd):
 
 def build_mesh():
     _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", f"-I{INC}",
-          os.path.join(CSRC, "boxmesh.cpp"), "-o", os.path.join(PKG, "libphmesh.so")])
+          os.path.join(CSRC, "boxmesh.cpp"), os.path.join(CSRC, "phastaio.cpp"),
+          "-o", os.path.join(PKG, "libphmesh.so")])
 
 
 def build_gpu():

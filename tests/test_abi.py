@@ -8,9 +8,9 @@ from phasta_amd import _native, solver
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared(header):
+def declared(header, prefix=r"ph[gm]_"):
     txt = open(os.path.join(ROOT, "include", header)).read()
-    return set(re.findall(r"^\s*(?:int|const char\*)\s+(ph[gm]_\w+)\(", txt, re.M))
+    return set(re.findall(r"^\s*(?:int|void|const char\*)\s+(" + prefix + r"\w+)\(", txt, re.M))
 
 
 def test_phgpu_exports_every_declared_symbol():
@@ -25,6 +25,19 @@ def test_phmesh_exports_every_declared_symbol():
     lib = _native.load("libphmesh.so")
     for n in declared("phmesh.h"):
         assert hasattr(lib, n), n
+
+
+def test_phastaio_exports_every_declared_symbol():
+    names = declared("phastaio.h", "phastaio_")
+    assert len(names) == 11
+    lib = _native.load("libphmesh.so")
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_every_header_is_covered():
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["phastaio.h", "phgpu.h",
+                                                                 "phmesh.h"]
 
 
 def test_last_error_is_callable_without_gpu():
