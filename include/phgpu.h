@@ -82,7 +82,9 @@ typedef struct phg_stats {
                              by block-diagonal cycles, 3 its first direction
                              left a larger one-step residual than the
                              block-diagonal direction and the cycle ran
-                             without the network                           */
+                             without the network, 4 set aside because it
+                             lost that test earlier in the same time step
+                             (resident stepping: phg_predict ... phg_update) */
 } phg_stats;
 
 typedef struct phg_ctx phg_ctx;

@@ -650,7 +650,8 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   double* S = c->scal;  // [0] sum res^2 (after forward), [1],[2] bad counts, [3] unorm, [4] sum rmes^2
   // tnanq on res and BDiag before factorisation (solgmr.f:459-460)
   PHG_TRY(launch_stats(c, S));  // S[0] provisional, overwritten below
-  const bool learned = c->mp.ready && c->mp.use_pc > 0;
+  const bool set_aside = c->mp.in_step && c->mp.step_off;
+  const bool learned = c->mp.ready && c->mp.use_pc > 0 && !set_aside;
   if ((learned || c->mp.keep_pair) && c->mp.ready)
     PHG_HIP(hipMemcpyAsync(c->mp.bd_raw, c->bdiag, sizeof(double) * 25 * n,
                            hipMemcpyDeviceToDevice, c->stream));
@@ -686,7 +687,7 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   st->refine_cycles = 0;
   int iKs = 0;
   double echeck = 0.0;
-  st->pc_fallback = 0;
+  st->pc_fallback = set_aside && c->mp.ready && c->mp.use_pc > 0 ? 4 : 0;
   if (!(unorm < 100. * c->prm.epsM * c->prm.epsM)) {
     double rnorm = unorm;
     // true residual of the scaled system, r = b - K dy (double products),
@@ -713,6 +714,7 @@ int sol_core(phg_ctx* c, phg_stats* st) {
       echeck = rnorm;
       if (co.rejected) {
         st->pc_fallback = 3;  // the cycle ran as block-diagonal GMRES
+        if (c->mp.in_step) c->mp.step_off = true;
       } else if (!(rnorm < unorm)) {
         // no reduction: discard and solve with block-diagonal GMRES
         st->pc_fallback = 1;
@@ -978,6 +980,8 @@ extern "C" int phg_state_download_old(phg_ctx* c, double* yold, double* acold) {
 extern "C" int phg_predict(phg_ctx* c) {
   PHG_TRY(check_ctx(c));
   PHG_TRY(launch_predict(c));
+  c->mp.in_step = true;  // a new time step: the network is tried again
+  c->mp.step_off = false;
   return 0;
 }
 
@@ -993,6 +997,8 @@ extern "C" int phg_newton_iteration(phg_ctx* c, phg_stats* st) {
 extern "C" int phg_update(phg_ctx* c) {
   PHG_TRY(check_ctx(c));
   PHG_TRY(launch_update(c));
+  c->mp.in_step = false;
+  c->mp.step_off = false;
   return 0;
 }
 
@@ -1163,6 +1169,7 @@ extern "C" int phg_mpnn_get_weights(phg_ctx* c, float* w, int count) {
 extern "C" int phg_mpnn_enable(phg_ctx* c, int use_pc, int keep_pair) {
   PHG_TRY(mp_check(c));
   c->mp.use_pc = use_pc < 0 ? 0 : use_pc;
+  c->mp.step_off = false;
   c->mp.keep_pair = keep_pair != 0;
   return 0;
 }

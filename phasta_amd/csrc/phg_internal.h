@@ -126,6 +126,10 @@ struct MpNet {
   int use_pc = 0;            // FGMRES: directions per cycle preconditioned
                              // by the network (0: block-diagonal GMRES)
   bool keep_pair = false;    // keep (raw BDiag, b, z) of the last solve
+  // resident time stepping: between phg_predict and phg_update a rejected
+  // network stays set aside for the remaining Newton iterations of the step
+  bool in_step = false;
+  bool step_off = false;
   long long n_act = 0, nedge = 0;
   int* ptr = nullptr;        // periodic node graph, CSR over all nodes
   int* col = nullptr;

@@ -126,6 +126,36 @@ def test_bad_network_direction_is_rejected():
     assert out["iKs"] == ref["iKs"]
     assert rel(out["Dy"], ref["Dy"]) < 1e-12
 
+def test_rejected_network_is_set_aside_for_the_time_step():
+    """Resident time stepping: a network that loses the acceptance test in
+    a time step's first Newton iteration (pc_fallback 3) is not run again
+    in that step (4) and is tried again after the next predictor; the
+    steps are the oracle's block-diagonal ones."""
+    case = mesh.box_case(4, mach=0.3, kspace=10)
+    orc = pyoracle.Oracle(case)
+    acold = np.zeros((case.nshg, 5), order="F")
+    yo_ref, aco_ref, hist = orc.time_steps(case.yold, acold, steps=2, newton=2)
+    p = M.init_params(9)
+    p["Wout"] = (10.0 * np.random.default_rng(4).standard_normal(p["Wout"].shape)).astype(np.float32)
+    gpu = solver.GpuSolver(case)
+    try:
+        gpu.mpnn_init(np.ones(25, dtype=np.float32))
+        gpu.mpnn_set_weights(solver.mpnn_flatten(p))
+        gpu.mpnn_enable(use_pc=1)
+        gpu.upload_state(case.yold, acold)
+        its = []
+        for _ in range(2):
+            gpu.predict()
+            its += [gpu.newton_iteration() for _ in range(2)]
+            gpu.update()
+        yo, _ = gpu.download_old()
+    finally:
+        gpu.close()
+    assert [s["pc_fallback"] for s in its] == [3, 4, 3, 4]
+    assert [s["iKs"] for s in its] == [h["iKs"] for h in hist]
+    assert rel(yo, yo_ref) < 1e-12
+
+
 def test_residual_loss_matches_host_and_training_reduces_it():
     """The residual loss ||K z(v) - v||^2 / ||v||^2 of the first Krylov vector
     v = b / ||b||, formed on the device, equals the same quantity from the
