@@ -8,9 +8,14 @@
   readnblk.f:114-376, genblk.f and genbkb.f read for a linear-tet mesh:
   sizes, co-ordinates, interior and boundary connectivity (with the wall
   elements' nbc codes and values), bc mapping / codes / condition arrays
-  and periodic masters.  Turning the condition array into the solver's
-  BC(nshg, ndofBC) is genBC (genbc.f, wall normals included) and is not
-  done here; ``read_geombc`` returns the raw fields.
+  and periodic masters; ``read_geombc`` returns the raw fields.
+* ``gen_bc`` / ``case_from_geombc``: genBC1's conversion (genbc1.f) of
+  the condition array into BC(nshg, ndofBC) for the boundary codes this
+  path supports (temperature, pressure, all three velocity components,
+  periodic), and a solver case (mesh arrays, CSR graph as genadj builds it,
+  the state of an optional restart) from a geombc file.  genBC's wall-normal
+  and slip-wall handling (genwnm, genotwn, slpwBC in genbc.f) serve codes
+  the solver refuses and are not restated.
 * ``compare_restarts``: checkphasta's test (checkphasta.cpp:96-132), the
   largest absolute difference of the "solution" fields against a tolerance.
 """
@@ -141,14 +146,14 @@ def read_restart(path, iotype="binary"):
 
 def write_geombc(path, case, ndof=5, iotype="binary"):
     """The geombc fields of a linear-tet mesh as readnblk.f, genblk.f and
-    genbkb.f read them.  Nodes with nonzero iBC get a condition record;
-    their condition values are zero (no-slip walls, periodic pairs)."""
+    genbkb.f read them.  Nodes with nonzero iBC get a condition record
+    holding the case's BC values in the input form gen_bc reads."""
     nshg, nel, nelb = case.nshg, case.nel, case.nelb
     ibc = np.asarray(case.iBC, dtype=np.int32)
     cons = np.nonzero(ibc)[0]
     nbc = np.zeros(nshg, dtype=np.int32)
     nbc[cons] = np.arange(1, cons.size + 1, dtype=np.int32)
-    bcinp = np.zeros((cons.size, ndof + 7))
+    bcinp = _bc_to_input(ibc[cons], np.asarray(case.BC)[cons], ndof)
     with File(path, "write", iotype) as f:
         f.write("byteorder magic number", [1], [MAGIC])
         for key, v in (("number of nodes", nshg), ("number of modes", nshg),
@@ -207,6 +212,83 @@ def read_geombc(path, iotype="binary"):
     out["iBC"] = ibc
     out["nBC"] = nbc
     return out
+
+
+def _check_codes(ibc):
+    vel = (ibc >> 3) & 7
+    bad = (ibc & 1) | ((vel != 0) & (vel != 7)) | (ibc & ~(2 | 4 | 56 | 1024))
+    if np.any(bad):
+        n = int(np.nonzero(bad)[0][0])
+        raise PhastaIOError(f"boundary code {int(ibc[n])} at node {n + 1} is not supported "
+                            "(temperature, pressure, all three velocity components, periodic)")
+
+
+def _bc_to_input(ibc, bc, ndof):
+    """Condition records (numpbc, ndof+7) that gen_bc turns back into bc:
+    column 2 temperature, 3 pressure, 4-6 velocity direction, 7 its
+    magnitude (the columns genbc1.f reads)."""
+    _check_codes(ibc)
+    out = np.zeros((ibc.size, ndof + 7))
+    if ibc.size == 0:
+        return out
+    out[:, 1] = np.where(ibc & 2, bc[:, 1], 0.0)
+    out[:, 2] = np.where(ibc & 4, bc[:, 0], 0.0)
+    v = bc[:, 2:5]
+    mag = np.sqrt(np.sum(v * v, axis=1))
+    v7 = (((ibc >> 3) & 7) == 7) & (mag > 0)
+    out[:, 3] = 1.0
+    out[v7, 3:6] = v[v7] / mag[v7, None]
+    out[:, 6] = np.where(v7, mag, 0.0)
+    return out
+
+
+def gen_bc(ibc, nbc, bcinp, ndofBC=10):
+    """genBC1 (genbc1.f) for the supported codes: BCtmp(n,:) =
+    BCinp(nBC(n),:) (genbc.f:12-16); temperature BC(:,2) = BCtmp(:,2),
+    pressure BC(:,1) = BCtmp(:,3), velocity code 7 BC(:,3:5) = BCtmp(:,7)
+    BCtmp(:,4:6).  Other codes raise (the solver refuses them too)."""
+    ibc = np.asarray(ibc, dtype=np.int32)
+    _check_codes(ibc)
+    nshg = ibc.size
+    tmp = np.zeros((nshg, bcinp.shape[1] if bcinp.ndim == 2 else 12))
+    m = nbc > 0
+    if np.any(m):
+        tmp[m] = bcinp[nbc[m] - 1]
+    bc = np.zeros((nshg, ndofBC), order="F")
+    t = (ibc & 2) != 0
+    bc[t, 1] = tmp[t, 1]
+    p = (ibc & 4) != 0
+    bc[p, 0] = tmp[p, 2]
+    v7 = ((ibc >> 3) & 7) == 7
+    bc[v7, 2:5] = tmp[v7, 6:7] * tmp[v7, 3:6]
+    return bc
+
+
+def case_from_geombc(path, params, restart=None, ndofBC=10, iotype="binary"):
+    """A solver case from a geombc file: mesh arrays, BC by gen_bc, the
+    CSR graph of the interior connectivity (genadj.f), and yold from the
+    "solution" field of a restart file (zeros without one)."""
+    from . import mesh
+    g = read_geombc(path, iotype)
+    nshg = g["x"].shape[0]
+    ien = np.asfortranarray(g["ien"], dtype=np.int32)
+    lib = mesh._lib()
+    colm = np.zeros(nshg + 1, dtype=np.int32)
+    nnz = ctypes.c_longlong()
+    if lib.phm_csr_count(nshg, ien.shape[0], ien, colm, ctypes.byref(nnz)):
+        raise PhastaIOError("phm_csr_count failed")
+    rowp = np.zeros(nnz.value, dtype=np.int32)
+    if lib.phm_csr_fill(nshg, ien.shape[0], ien, colm, rowp):
+        raise PhastaIOError("phm_csr_fill failed")
+    bc = gen_bc(g["iBC"], g["nBC"], g["BCinp"], ndofBC)
+    yold = np.zeros((nshg, 5), order="F")
+    if restart is not None:
+        q, _, _ = read_restart(restart, iotype)
+        yold = np.asfortranarray(q[:, :5])
+    return mesh.BoxCase(N=0, mach=0.0, x=np.asfortranarray(g["x"]), ien=ien,
+                        ienb=np.asfortranarray(g["ienb"], dtype=np.int32),
+                        iBC=g["iBC"], BC=bc, iper=g["iper"], colm=colm, rowp=rowp,
+                        yold=yold, params=dict(params))
 
 
 def compare_restarts(left, right, tolerance=1e-6):

@@ -13,6 +13,7 @@ import struct
 import numpy as np
 import pytest
 
+import pyoracle
 from phasta_amd import mesh, phastaio as pio
 
 MAGIC = b"byteorder magic number : < 5 > 1 \n"
@@ -121,3 +122,45 @@ def test_compare_restarts_tolerance(tmp_path):
     assert ok and abs(d - 5e-7) < 1e-12
     d, ok = pio.compare_restarts([a], [b], tolerance=1e-7)
     assert not ok
+
+
+def test_gen_bc_formulas():
+    """genBC1 (genbc1.f) on hand-made condition records: temperature from
+    column 2, pressure from column 3, velocity code 7 = magnitude (column
+    7) times direction (columns 4-6); unsupported codes raise."""
+    ibc = np.array([0, 2, 4, 56, 56 | 1024 | 2, 1024], dtype=np.int32)
+    nbc = np.array([0, 1, 2, 3, 4, 5], dtype=np.int32)
+    rec = np.zeros((5, 12))
+    rec[0, 1] = 310.0
+    rec[1, 2] = 1.2e5
+    rec[2, 3:7] = [0.6, 0.8, 0.0, 5.0]
+    rec[3, 1] = 290.0
+    rec[3, 3:7] = [0.0, 0.0, 1.0, 2.0]
+    bc = pio.gen_bc(ibc, nbc, rec)
+    assert bc[1, 1] == 310.0 and bc[2, 0] == 1.2e5
+    assert np.allclose(bc[3, 2:5], [3.0, 4.0, 0.0], rtol=0, atol=1e-15)
+    assert bc[4, 1] == 290.0 and np.array_equal(bc[4, 2:5], [0.0, 0.0, 2.0])
+    assert not bc[0].any() and not bc[5].any()
+    with pytest.raises(pio.PhastaIOError):
+        pio.gen_bc(np.array([8], dtype=np.int32), np.array([1], dtype=np.int32), rec[:1])
+
+
+def test_case_from_geombc_reproduces_the_case(tmp_path):
+    """geombc + restart written from a box case (top wall moving at
+    10 m/s) and read back into a solver case: the same arrays, and the
+    oracle's Newton step on it equals the one on the original, bit for bit."""
+    case = mesh.box_case(4, jitter=0.1, kspace=10)
+    top = np.isclose(case.x[:, 1], 1.0)
+    case.BC[top, 2] = 10.0
+    g = tmp_path / pio.geombc_name()
+    r = tmp_path / pio.restart_name(0)
+    pio.write_geombc(g, case)
+    pio.write_restart(r, case.yold, np.zeros_like(case.yold), 0)
+    c2 = pio.case_from_geombc(g, case.params, restart=r)
+    for k in ("x", "ien", "ienb", "iBC", "iper", "colm", "rowp", "BC", "yold"):
+        assert np.array_equal(getattr(c2, k), getattr(case, k)), k
+    acold = np.zeros((case.nshg, 5), order="F")
+    y1, ac1, h1 = pyoracle.Oracle(case).newton_step(case.yold, acold)
+    y2, ac2, h2 = pyoracle.Oracle(c2).newton_step(c2.yold, acold)
+    assert np.array_equal(y1, y2) and h1[0]["iKs"] == h2[0]["iKs"]
+    assert np.allclose(y1[top, 0], 10.0)
