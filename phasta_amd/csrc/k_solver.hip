@@ -469,6 +469,66 @@ __global__ void __launch_bounds__(kRedThreads) k_mgs(double* u, const double* uj
   block_partial(t, part);
 }
 
+// Classical Gram-Schmidt for the mixed-precision path: all dot products of
+// u with the basis in one pass, then the update u -= sum_j h_j u_j fused
+// with (u, u).  Up to kCgsMax basis vectors; partial sums per workgroup in
+// channels 0..nv-1 (dots) and nv (norm), summed in a fixed order.
+constexpr int kCgsMax = 32;
+
+__device__ inline void block_partials(double* v, int nv, double* part, long long chan_stride) {
+  __shared__ double ws[kCgsMax][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // compile-time indices keep v in registers
+#pragma unroll
+  for (int j = 0; j < kCgsMax; ++j) {
+    if (j < nv) {
+      const double t = wave_sum(v[j]);
+      if (lane == 0) ws[j][wid] = t;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    double s = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) s += ws[threadIdx.x][w];
+    part[(size_t)threadIdx.x * chan_stride + blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_cgs_dots(const double* __restrict__ u,
+                                                          const double* __restrict__ basis, int nv,
+                                                          long long n, double* part,
+                                                          long long chan_stride) {
+  double v[kCgsMax];
+#pragma unroll
+  for (int j = 0; j < kCgsMax; ++j) v[j] = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double x = u[i];
+#pragma unroll
+    for (int j = 0; j < kCgsMax; ++j)
+      if (j < nv) v[j] += x * basis[(size_t)j * n + i];
+  }
+  block_partials(v, nv, part, chan_stride);
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_cgs_update(double* __restrict__ u,
+                                                            const double* __restrict__ basis,
+                                                            int nv, const double* __restrict__ h,
+                                                            long long n, double* part) {
+  __shared__ double hs[kCgsMax];
+  if ((int)threadIdx.x < nv) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double x = u[i];
+    for (int j = 0; j < nv; ++j) x -= hs[j] * basis[(size_t)j * n + i];
+    u[i] = x;
+    acc += x * x;
+  }
+  block_partial(acc, part);
+}
+
 // u = src / sqrt(*sq); norm_out <- sqrt(*sq) (written by one lane)
 __global__ void k_scale(double* u, const double* src, long long n, const double* sq,
                         double* norm_out) {
@@ -746,6 +806,20 @@ int launch_mgs_part(phg_ctx* c, double* u, const double* ujm1, const double* uj,
   PHG_HIP(hipGetLastError());
   *np = g;
   return 0;
+}
+
+int launch_cgs(phg_ctx* c, double* u, const double* basis, int nv, long long n, double* h) {
+  if (nv < 1 || nv > kCgsMax) {
+    phg_set_error("classical Gram-Schmidt: %d basis vectors (1..%d)", nv, kCgsMax);
+    return PHG_EINVAL;
+  }
+  const int g = red_grid(c, n, 4);
+  k_cgs_dots<<<g, kRedThreads, 0, c->stream>>>(u, basis, nv, n, red_chan(c, 0), c->red_blocks);
+  PHG_HIP(hipGetLastError());
+  PHG_TRY(launch_finish(c, 0, g, g, nv, h));
+  k_cgs_update<<<g, kRedThreads, 0, c->stream>>>(u, basis, nv, h, n, red_chan(c, nv));
+  PHG_HIP(hipGetLastError());
+  return launch_finish(c, nv, g, g, 1, h + nv);
 }
 
 int launch_scale(phg_ctx* c, double* u, const double* src, long long n, const double* sq_ptr,

@@ -495,6 +495,9 @@ struct CycleOut {
 
 // flexible: precondition each direction with the network (FGMRES, right
 // preconditioning) and combine the preconditioned directions.
+// basis sizes the classical Gram-Schmidt kernels take (launch_cgs)
+constexpr int kCgsVectors = 32;
+
 int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut* out,
                 bool flexible = false) {
   const long long V = 5 * c->nshg;
@@ -567,16 +570,23 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
       PHG_TRY(launch_bc3per(c, w, 5));
     }
     PHG_HIP(hipEventRecord(c->kev[2], c->stream));
-    // H(1..iK+1, iK) by MGS: channel j-1 holds the partials of H(j, iK)
-    int np0 = 0, np = 0;
-    PHG_TRY(launch_dot_part(c, w, kvec(c, 0), V, 0, &np0));
-    int np_prev = np0;
-    for (int jK = 2; jK <= iK + 1; ++jK) {
-      PHG_TRY(launch_mgs_part(c, w, kvec(c, jK - 2), jK == iK + 1 ? w : kvec(c, jK - 1), jK - 2,
-                              np_prev, V, jK - 1, &np));
-      np_prev = np;
+    if (c->precision == PHG_MIXED && iK <= kCgsVectors) {
+      // the mixed path orthogonalises by classical Gram-Schmidt: two passes
+      // over the basis instead of iK + 1 (its error is stated against FP64)
+      PHG_TRY(launch_cgs(c, w, kvec(c, 0), iK, V, H));
+    } else {
+      // H(1..iK+1, iK) by MGS (solgmr.f:598-618): channel j-1 holds the
+      // partials of H(j, iK)
+      int np0 = 0, np = 0;
+      PHG_TRY(launch_dot_part(c, w, kvec(c, 0), V, 0, &np0));
+      int np_prev = np0;
+      for (int jK = 2; jK <= iK + 1; ++jK) {
+        PHG_TRY(launch_mgs_part(c, w, kvec(c, jK - 2), jK == iK + 1 ? w : kvec(c, jK - 1), jK - 2,
+                                np_prev, V, jK - 1, &np));
+        np_prev = np;
+      }
+      PHG_TRY(launch_finish(c, 0, np0, np, iK + 1, H));
     }
-    PHG_TRY(launch_finish(c, 0, np0, np, iK + 1, H));
     PHG_HIP(hipEventRecord(c->kev[3], c->stream));
     PHG_TRY(launch_scale(c, w, w, V, H + iK, H + iK + 1));
     PHG_TRY(download(c, c->h_scal + 16, H, iK + 2));

@@ -320,6 +320,10 @@ int launch_dot_part(phg_ctx* c, const double* a, const double* b, long long n,
 int launch_mgs_part(phg_ctx* c, double* u, const double* ujm1, const double* uj,
                     int chan_prev, int np_prev, long long n, int chan, int* np);
 int launch_finish(phg_ctx* c, int chan0, int np0, int np, int nsums, double* out);
+// classical Gram-Schmidt of u against nv contiguous basis vectors:
+// h[0..nv-1] = (u, basis_j) of the incoming u, u -= sum h_j basis_j,
+// h[nv] = (u, u) afterwards
+int launch_cgs(phg_ctx* c, double* u, const double* basis, int nv, long long n, double* h);
 int launch_scale(phg_ctx* c, double* u, const double* src, long long n,
                  const double* sq_ptr, double* norm_out);
 int launch_combine(phg_ctx* c, double* dy, const double* basis, int k,
