@@ -164,3 +164,31 @@ def test_case_from_geombc_reproduces_the_case(tmp_path):
     y2, ac2, h2 = pyoracle.Oracle(c2).newton_step(c2.yold, acold)
     assert np.array_equal(y1, y2) and h1[0]["iKs"] == h2[0]["iKs"]
     assert np.allclose(y1[top, 0], 10.0)
+
+
+def test_truncated_and_missing_files_raise(tmp_path):
+    p = tmp_path / "restart.1.1"
+    y = np.ones((50, 5))
+    pio.write_restart(p, y, y, 1)
+    data = p.read_bytes()
+    p.write_bytes(data[: len(data) // 2])          # cut inside the solution block
+    with pytest.raises(pio.PhastaIOError):
+        pio.read_restart(p)
+    with pytest.raises(pio.PhastaIOError):
+        pio.read_restart(tmp_path / "restart.9.1")  # no such file
+    with pytest.raises(pio.PhastaIOError):
+        pio.File(tmp_path / "x", "modify")           # unknown mode
+    (tmp_path / "empty").write_bytes(b"")
+    with pio.File(tmp_path / "empty", "read") as f:
+        assert f.header("solution", 3, "double") is None
+
+
+def test_ascii_geombc_round_trip(tmp_path):
+    case = mesh.box_case(2, jitter=0.0)
+    p = tmp_path / "geombc.dat.1"
+    pio.write_geombc(p, case, iotype="ascii")
+    g = pio.read_geombc(p, iotype="ascii")
+    # ASCII doubles carry six decimals ("%f", as the reference writes them)
+    assert np.allclose(g["x"], case.x, atol=5e-7)
+    assert np.array_equal(g["ien"], case.ien) and np.array_equal(g["iper"], case.iper)
+    assert np.array_equal(g["iBC"], case.iBC)
