@@ -350,3 +350,28 @@ def test_box64_newton_iteration_against_oracle():
     assert st64["iKs"] == ref["iKs"]
     assert e64 < TOL
     assert stmx["converged"] and emx < 1e-5
+
+
+def test_box64_newton_iteration_with_flux_reconstruction():
+    """idiff = 1 at the 64^3 workload's size (1.57 M tets): one FP64
+    Newton iteration on the GPU against the oracle's, 1e-12 and the same
+    Krylov count."""
+    case = mesh.named_case("box64")
+    prm = dict(case.params, idiff=1)
+    print(f"[parity] box64 idiff 1: nshg={case.nshg}", flush=True)
+    orc = pyoracle.Oracle(case, params=prm)
+    acold = np.zeros((case.nshg, 5), order="F")
+    y, ac = orc.itr_predict_bc(case.yold, acold)
+    ref = orc.solgmrs(y, ac)
+    gpu = solver.GpuSolver(case, params=prm)
+    try:
+        gpu.upload_state(case.yold, acold)
+        gpu.predict()
+        st = gpu.newton_iteration()
+        _, _, dy = gpu.download_state()
+    finally:
+        gpu.close()
+    e = rel(dy, ref["Dy"])
+    print(f"[parity] box64 idiff 1 Dy rel err {e:.2e} (iKs {st['iKs']} vs {ref['iKs']})", flush=True)
+    assert st["iKs"] == ref["iKs"]
+    assert e < TOL
