@@ -112,6 +112,14 @@ def run_arm(case, precision, newton, steps, warmup, weights=None, idiff=0):
                 ms_copy=ms_copy, by_copy=by_copy, info=info)
 
 
+def pc_accepted(stats):
+    """Fraction of Newton iterations whose GMRES used the network's
+    direction (pc_fallback 0: its cycle converged, 1: its cycle was
+    discarded, 2: finished by block-diagonal cycles; 3 and 4: the network's
+    direction lost the acceptance test or was set aside)."""
+    return sum(1 for s in stats if s["pc_fallback"] in (0, 1, 2)) / max(1, len(stats))
+
+
 def breakdown(arm, steps, nel):
     st = arm["stats"]
     ms_asm = sum(s["ms_assembly"] for s in st) / steps
@@ -179,6 +187,13 @@ def main():
                 "traffic": measured_traffic(kname, args.case),
                 "bytes_per_launch": by, "ms_per_launch": round(spmv_avg, 5)}
 
+    if head_name == "learned":
+        acc = pc_accepted(head["stats"])
+        pc_label = ("learned" if acc > 0 else
+                    "block-diagonal (the network's direction was rejected in every timed "
+                    "Newton iteration)")
+    else:
+        pc_label = "block-diagonal"
     desc = (f"{args.case}: {spec['N']}^3-hex unit box, Kuhn 6-tet split ({nel} tets, "
             f"{case.nshg} nodes, {case.nnz_tot} 5x5 blocks), Mach {spec['mach']}, jitter "
             f"{spec['jitter']}h, y-grading {spec['grade']}, periodic x/z, no-slip walls y; "
@@ -192,12 +207,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "N": spec["N"], "nel": nel, "nshg": case.nshg,
                    "nnz_blocks": case.nnz_tot, "newton_per_time_step": spec["newton"],
-                   "arm": head_name, "pc": "learned" if head_name == "learned" else "block",
-                   "idiff": args.idiff},
+                   "arm": head_name, "pc": pc_label, "idiff": args.idiff},
         "roofline": roofline,
         "arms": {a: breakdown(res[a], args.steps, nel) for a in res},
         "copy_GBs": round(head["by_copy"] / (head["ms_copy"] * 1e-3) / 1e9, 1),
     }
+    if "learned" in res:
+        line["arms"]["learned"]["pc_accepted_fraction"] = round(
+            pc_accepted(res["learned"]["stats"]), 3)
     if "fp64" in res:
         d64 = res["fp64"]["dy"]
         for a in res:

@@ -48,6 +48,14 @@ typedef struct phg_params {
   int lhs, iprec;                       /* itrdrv.f:456-457                 */
   int ipred;                            /* /genpar/ predictor; 1 only      */
   int idiff;                            /* /genpar/ 0, 1 or 3 (elmgmr.f:361) */
+  int iter;                             /* /timdat/ Newton iteration of the
+                                           time step (itrdrv.f:409,434); the
+                                           rstatCheck divergence abort applies
+                                           when iter > 1 (rstatCheck.f:73).
+                                           phg_predict sets 0 and
+                                           phg_newton_iteration adds 1 before
+                                           its solve; phg_solgmrs takes the
+                                           caller's value.                  */
 } phg_params;
 
 /* Mesh arrays as the Fortran host holds them (readnblk.f, genadj.f);
@@ -85,6 +93,9 @@ typedef struct phg_stats {
                              without the network, 4 set aside because it
                              lost that test earlier in the same time step
                              (resident stepping: phg_predict ... phg_update) */
+  double orth_loss;       /* max over the solve's cycles of
+                             max_ij |u_i . u_j - delta_ij| of the Krylov basis
+                             (phg_diagnostics flag PHG_DIAG_ORTH; else 0)   */
 } phg_stats;
 
 typedef struct phg_ctx phg_ctx;
@@ -143,6 +154,13 @@ int phg_update(phg_ctx* ctx);
 int phg_asiq(phg_ctx* ctx, const double* y, double* qres, double* rmass);
 /* SparseAp(iper,ilwork,iBC,col,row,lhsK,p)          sparseap.f:1     */
 int phg_sparse_ap(phg_ctx* ctx, const double* lhsK, double* p);
+/* p <- bc3per(A p(iper)) with A the FP64 assembled tangent of state y
+ * (EGmass of every element after bc3LHS, elmgmr.f:441-459), applied element
+ * by element without forming or storing lhsK: the unscaled operator of
+ * SparseAp + bc3per.  The mixed-precision path forms its refinement
+ * residual with it.  Matrix-free SolMFG (solmfg.f, au1mfg.f:56-78) instead
+ * differences the residual at y + eps u; this is the exact product. */
+int phg_tangent_ap(phg_ctx* ctx, const double* y, double* p);
 /* Transpose of SparseAp's operator, p <- K^T p with K p = bc3per(A p(iper))
  * (no counterpart in the reference: the learned preconditioner's training
  * differentiates through the matvec). */
@@ -209,6 +227,13 @@ int phg_mpnn_train_pair(phg_ctx* ctx, int slot, float lr, double* loss);
  * grad (the gradient of the last vector's loss, phg_mpnn_param_count()
  * floats) are nullable. */
 int phg_mpnn_train_krylov(phg_ctx* ctx, int nvec, float lr, double* mean_loss, float* grad);
+
+/* Diagnostics (no counterpart in the reference): PHG_DIAG_ORTH measures
+ * the orthogonality of every GMRES cycle's Krylov basis into
+ * phg_stats.orth_loss, at the cost of (iKs+1)(iKs+2)/2 dot products per
+ * cycle; 0 turns diagnostics off. */
+enum { PHG_DIAG_ORTH = 1 };
+int phg_diagnostics(phg_ctx* ctx, int flags);
 
 /* Wait for all work queued on the library stream. */
 int phg_sync(phg_ctx* ctx);

@@ -40,6 +40,12 @@ int phm_box_counts(int N, long long* nshg, long long* nel, long long* nelb);
 /* Build geometry, connectivity and BC codes.  Returns 0 on success. */
 int phm_box_mesh(int N, double jitter, double grade, unsigned long long seed,
                  double* x, int* ien, int* ienb, int* iBC, int* iper);
+/* As phm_box_mesh, with a capped boundary layer when first > 0: the cell
+ * at each wall has height first/N, each next cell grade times the last,
+ * until the uniform core height that fills the half channel (N even). */
+int phm_box_mesh_layer(int N, double jitter, double grade, double first,
+                       unsigned long long seed, double* x, int* ien, int* ienb,
+                       int* iBC, int* iper);
 
 /* CSR graph of the element adjacency (genadj.f:1-78, asadj.f:1-59):
  * first call fills colm and returns nnz_tot, second fills sorted rowp. */

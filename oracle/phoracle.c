@@ -42,6 +42,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define PT125 0.1250000000000000000000000000000
 #define PT5 0.5000000000000000000000000000000
@@ -907,18 +910,27 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
   if (P->lhs == 1) memset(lhsK, 0, sizeof(double) * 25 * m->nnz_tot);
   if (P->iprec != 0) memset(BDiag, 0, sizeof(double) * 25 * nshg);
   const int bk = m->ibksiz > 0 ? m->ibksiz : 64;
-  double(*rl)[4][5] = malloc(sizeof(double) * 20 * bk);
-  double(*EGb)[20][20] = malloc(sizeof(double) * 400 * bk);
-  double(*BDl)[4][25] = malloc(sizeof(double) * 100 * bk);
-  int(*nod)[4] = malloc(sizeof(int) * 4 * bk);
-  if (!rl || !EGb || !BDl || !nod) return 4;
-  /* element blocks of ibksiz (genblk.f:111-114) */
-  for (long long e0 = 0; e0 < nel; e0 += bk) {
-    int npro = (int)((nel - e0) < bk ? (nel - e0) : bk);
-    for (int e = 0; e < npro; ++e) {
+  /* Threads (OpenMP) evaluate e3 for kSuper element blocks at a time into
+   * per-element buffers; the scatter then runs in the reference's order
+   * (block by block, local.f:67-74 and fillsparse.f:101-126), each thread
+   * applying only the contributions to its own range of rows, so every
+   * entry of res, BDiag and lhsK receives its terms in the same sequence as
+   * a serial run: the result is bitwise independent of the thread count. */
+  enum { kSuper = 64 };
+  const long long cap = (long long)kSuper * bk;
+  double(*rl)[4][5] = malloc(sizeof(double) * 20 * cap);
+  double(*EGb)[20][20] = malloc(sizeof(double) * 400 * cap);
+  double(*BDl)[4][25] = malloc(sizeof(double) * 100 * cap);
+  int(*nod)[4] = malloc(sizeof(int) * 4 * cap);
+  long long* kloc = P->lhs == 1 ? malloc(sizeof(long long) * 16 * cap) : NULL;
+  if (!rl || !EGb || !BDl || !nod || (P->lhs == 1 && !kloc)) return 4;
+  for (long long s0 = 0; s0 < nel; s0 += cap) {
+    const long long ns = (nel - s0) < cap ? (nel - s0) : cap;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long e = 0; e < ns; ++e) {
       double yl[4][5], acl[4][5], xl[4][3];
       for (int a = 0; a < 4; ++a) {
-        long long g = m->ien[(e0 + e) + nel * a] - 1;
+        long long g = m->ien[(s0 + e) + nel * a] - 1;
         nod[e][a] = (int)g;
         /* localy.f:53-73: (u,v,w,p,T) -> (p,u,v,w,T) */
         yl[a][1] = y[g];
@@ -945,35 +957,58 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
         for (int i = 0; i < 4; ++i)
           for (int k = 0; k < 5; ++k)
             for (int j = 0; j < 5; ++j) BDl[e][i][j + 5 * k] = EGb[e][5 * i + j][5 * i + k];
-    }
-    /* local.f:67-74 scatter order: dof, local node, element */
-    for (int f = 0; f < 5; ++f)
-      for (int i = 0; i < 4; ++i)
-        for (int e = 0; e < npro; ++e)
-          res[nod[e][i] + nshg * f] = res[nod[e][i] + nshg * f] + rl[e][i][f];
-    if (P->iprec != 0)
-      for (int jk = 0; jk < 25; ++jk)
-        for (int i = 0; i < 4; ++i)
-          for (int e = 0; e < npro; ++e)
-            BDiag[nod[e][i] + nshg * jk] = BDiag[nod[e][i] + nshg * jk] + BDl[e][i][jk];
-    if (P->lhs == 1) {
-      for (int e = 0; e < npro; ++e) bc3lhs_elem(m->iBC, nod[e], EGb[e]);
-      /* fillsparse.f:101-126 */
-      for (int e = 0; e < npro; ++e)
+      if (P->lhs == 1) {
+        bc3lhs_elem(m->iBC, nod[e], EGb[e]);
+        /* sparseloc (fillsparse.f:243-270) of the element's 16 blocks */
         for (int aa = 0; aa < 4; ++aa) {
-          int i = nod[e][aa];
-          int c = m->colm[i];
-          int n = m->colm[i + 1] - c;
-          for (int b = 0; b < 4; ++b) {
-            long long k = sparseloc(m->rowp + (c - 1), n, nod[e][b] + 1) + c - 1;
-            double* blk = lhsK + 25 * (k - 1);
-            for (int g = 0; g < 5; ++g)
-              for (int f = 0; f < 5; ++f)
-                blk[5 * g + f] = blk[5 * g + f] + EGb[e][5 * aa + f][5 * b + g];
-          }
+          const int i = nod[e][aa];
+          const int c = m->colm[i];
+          const int n = m->colm[i + 1] - c;
+          for (int b = 0; b < 4; ++b)
+            kloc[16 * e + 4 * aa + b] = sparseloc(m->rowp + (c - 1), n, nod[e][b] + 1) + c - 1;
         }
+      }
+    }
+#pragma omp parallel
+    {
+      int nt = 1, t = 0;
+#ifdef _OPENMP
+      nt = omp_get_num_threads();
+      t = omp_get_thread_num();
+#endif
+      const long long r0 = nshg * t / nt, r1 = nshg * (t + 1) / nt;
+#define MINE(node) ((node) >= r0 && (node) < r1)
+      for (long long e0 = 0; e0 < ns; e0 += bk) {
+        const int npro = (int)((ns - e0) < bk ? (ns - e0) : bk);
+        /* local.f:67-74 scatter order: dof, local node, element */
+        for (int f = 0; f < 5; ++f)
+          for (int i = 0; i < 4; ++i)
+            for (int e = (int)e0; e < e0 + npro; ++e)
+              if (MINE(nod[e][i]))
+                res[nod[e][i] + nshg * f] = res[nod[e][i] + nshg * f] + rl[e][i][f];
+        if (P->iprec != 0)
+          for (int jk = 0; jk < 25; ++jk)
+            for (int i = 0; i < 4; ++i)
+              for (int e = (int)e0; e < e0 + npro; ++e)
+                if (MINE(nod[e][i]))
+                  BDiag[nod[e][i] + nshg * jk] = BDiag[nod[e][i] + nshg * jk] + BDl[e][i][jk];
+        if (P->lhs == 1)
+          /* fillsparse.f:101-126 */
+          for (int e = (int)e0; e < e0 + npro; ++e)
+            for (int aa = 0; aa < 4; ++aa) {
+              if (!MINE(nod[e][aa])) continue;
+              for (int b = 0; b < 4; ++b) {
+                double* blk = lhsK + 25 * (kloc[16 * e + 4 * aa + b] - 1);
+                for (int g = 0; g < 5; ++g)
+                  for (int f = 0; f < 5; ++f)
+                    blk[5 * g + f] = blk[5 * g + f] + EGb[e][5 * aa + f][5 * b + g];
+              }
+            }
+      }
+#undef MINE
     }
   }
+  free(kloc);
   /* boundary elements: asbmfg.f:39-56, blocks of ibksiz (genbkb.f) */
   for (long long e0 = 0; e0 < nelb; e0 += bk) {
     int npro = (int)((nelb - e0) < bk ? (nelb - e0) : bk);
@@ -1056,6 +1091,7 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
 #define RR(a) r[n + nshg * (a - 1)]
 int orc_i3lu(long long nshg, double* Diag, double* r, int code) {
   if (code == 0) {
+#pragma omp parallel for schedule(static)
     for (long long n = 0; n < nshg; ++n) {
       DG(1, 1) = ONE / DG(1, 1);
       DG(2, 1) = DG(1, 1) * DG(2, 1);
@@ -1087,6 +1123,7 @@ int orc_i3lu(long long nshg, double* Diag, double* r, int code) {
     return 0;
   }
   if (code == 1) {
+#pragma omp parallel for schedule(static)
     for (long long n = 0; n < nshg; ++n) {
       RR(2) = RR(2) - DG(2, 1) * RR(1);
       RR(3) = RR(3) - DG(3, 1) * RR(1) - DG(3, 2) * RR(2);
@@ -1096,6 +1133,7 @@ int orc_i3lu(long long nshg, double* Diag, double* r, int code) {
     return 0;
   }
   if (code == 2) {
+#pragma omp parallel for schedule(static)
     for (long long n = 0; n < nshg; ++n) {
       RR(5) = DG(5, 5) * RR(5);
       RR(4) = DG(4, 4) * (RR(4) - RR(5) * DG(4, 5));
@@ -1126,6 +1164,9 @@ int orc_i3lu(long long nshg, double* Diag, double* r, int code) {
 #define LK(e) L[(e)-1]
 void orc_spsi3pre(long long nshg, const double* BDiag, double* lhsK,
                   const int* colm, const int* rowp) {
+  /* rows are independent: threads over rows leave every value bitwise
+   * equal to a serial run */
+#pragma omp parallel for schedule(static)
   for (long long i = 0; i < nshg; ++i)
     for (long long k = colm[i] - 1; k < colm[i + 1] - 1; ++k) {
       double* L = lhsK + 25 * k;
@@ -1139,6 +1180,7 @@ void orc_spsi3pre(long long nshg, const double* BDiag, double* lhsK,
                     BD(i, 5, 3) * LK(o + 3) - BD(i, 5, 4) * LK(o + 4);
       }
     }
+#pragma omp parallel for schedule(static)
   for (long long i = 0; i < nshg; ++i)
     for (long long k = colm[i] - 1; k < colm[i + 1] - 1; ++k) {
       long long j = rowp[k] - 1;
@@ -1165,6 +1207,7 @@ void orc_sparse_ap(long long nshg, const int* iper, const int* colm,
   double* q = malloc(sizeof(double) * 5 * nshg);
   for (int f = 0; f < 5; ++f)
     for (long long n = 0; n < nshg; ++n) pp[n + nshg * f] = p[(iper[n] - 1) + nshg * f];
+#pragma omp parallel for schedule(static)
   for (long long i = 0; i < nshg; ++i) {
     double t1 = 0., t2 = 0., t3 = 0., t4 = 0., t5 = 0.;
     for (long long k = colm[i] - 1; k < colm[i + 1] - 1; ++k) {
@@ -1243,7 +1286,9 @@ int orc_solgmrs(const orc_mesh* m, const orc_params* P, const double* y,
       rtot = rtot + r2;
     }
     double totres = sqrt(rtot / (double)(float)nshg);
-    if (st->reslast > 0.0 && totres > 10000.0 * st->reslast) return 11;
+    /* the divergence test skips the first solve of a time step (iter is
+     * reset at itrdrv.f:409 and incremented before each solve, :434) */
+    if (P->iter > 1 && totres > 10000.0 * st->reslast) return 11;
     st->reslast = totres;
   }
   memset(Dy, 0, sizeof(double) * V);
@@ -1269,6 +1314,7 @@ int orc_solgmrs(const orc_mesh* m, const orc_params* P, const double* y,
        * which this routine does not update; nGMRES = 1 is assumed. */
       memset(e, 0, sizeof(double) * (K + 1));
       e[0] = unorm;
+#pragma omp parallel for schedule(static)
       for (long long i = 0; i < V; ++i) UB(1)[i] = UB(1)[i] / unorm;
       for (int iK = 1; iK <= K; ++iK) {
         iKs = iK;
@@ -1282,6 +1328,7 @@ int orc_solgmrs(const orc_mesh* m, const orc_params* P, const double* y,
             beta = sumsq_prod(V, w, UB(1));
           } else {
             const double* um = UB(jK - 1);
+#pragma omp parallel for schedule(static)
             for (long long i = 0; i < V; ++i) w[i] = w[i] - beta * um[i];
             beta = sumsq_prod(V, w, UB(jK));
           }
@@ -1289,6 +1336,7 @@ int orc_solgmrs(const orc_mesh* m, const orc_params* P, const double* y,
         }
         unorm = sqrt(beta);
         HB(iK + 1, iK) = unorm;
+#pragma omp parallel for schedule(static)
         for (long long i = 0; i < V; ++i) w[i] = w[i] / unorm;
         for (int jK = 1; jK <= iK - 1; ++jK) {
           double tmp = Rc[jK] * HB(jK, iK) + Rs[jK] * HB(jK + 1, iK);
@@ -1313,6 +1361,7 @@ int orc_solgmrs(const orc_mesh* m, const orc_params* P, const double* y,
       }
       for (int jK = 1; jK <= iKs; ++jK) {
         const double* uj = UB(jK);
+#pragma omp parallel for schedule(static)
         for (long long i = 0; i < V; ++i) Dy[i] = Dy[i] + yB[jK] * uj[i];
       }
       echeck = fabs(e[iKs]);

@@ -41,6 +41,8 @@ typedef struct orc_params {
   int lhs, iprec;                       /* per-call flags (itrdrv.f:456)   */
   int ipred;                            /* predictor 1-4 (itrPC.f:67-119)  */
   int idiff;                            /* diffusive flux reconstruction   */
+  int iter;                             /* /timdat/ Newton iteration of the
+                                           time step (itrdrv.f:409,434)    */
 } orc_params;
 
 typedef struct orc_stats {

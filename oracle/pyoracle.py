@@ -27,7 +27,7 @@ class OrcParams(ctypes.Structure):
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
                [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred",
-                                               "idiff")]
+                                               "idiff", "iter")]
 
 
 class OrcStats(ctypes.Structure):
@@ -212,8 +212,8 @@ class Oracle:
         """One time step: predictor + BC, then `newton` solve/correct pairs."""
         y, ac = self.itr_predict_bc(yold, acold)
         hist = []
-        for _ in range(newton):
-            out = self.solgmrs(y, ac)
+        for k in range(newton):
+            out = self.solgmrs(y, ac, iter=k + 1)  # itrdrv.f:409,434
             y, ac = self.itr_correct_bc(y, ac, yold, acold, out["Dy"])
             hist.append(out)
         return y, ac, hist

@@ -56,22 +56,28 @@ CASES = {
                   newton=3, steps=1, precision="fp64", cfl=1.0),
     "box128": dict(N=128, mach=0.3, jitter=0.1, grade=1.1, kspace=20,
                    newton=3, steps=1, precision="mixed", cfl=3.0),
-    "box256": dict(N=256, mach=0.6, jitter=0.1, grade=1.08, kspace=20,
+    # boundary layer: first wall cell h/100 (aspect ratio 100), growing by
+    # 1.08 per cell up to the uniform core (67 graded cells per wall, core
+    # cells 1.75 h tall)
+    "box256": dict(N=256, mach=0.6, jitter=0.1, grade=1.08, first=0.01, kspace=20,
                    newton=3, steps=5, precision="mixed", cfl=3.0),
 }
 
 
-def min_spacing(N, grade=1.0):
-    """Smallest grid-line spacing of the box: 1/N, or with geometric
+def min_spacing(N, grade=1.0, first=0.0):
+    """Smallest grid-line spacing of the box: 1/N; with geometric
     y-grading of ratio `grade` from both walls the first wall cell
-    h0 = (1/2) (grade - 1) / (grade^(N/2) - 1) (the mesh generator's ygrid)."""
+    h0 = (1/2) (grade - 1) / (grade^(N/2) - 1) (the mesh generator's ygrid);
+    with a capped boundary layer (first > 0) the first cell first/N."""
     if grade == 1.0:
         return 1.0 / N
+    if first > 0.0:
+        return first / N
     return 0.5 * (grade - 1.0) / (grade ** (N // 2) - 1.0)
 
 
-def time_step(N, mach, grade=1.0, cfl=CFL):
-    return cfl * min_spacing(N, grade) / (C0 * (1.0 + mach))
+def time_step(N, mach, grade=1.0, cfl=CFL, first=0.0):
+    return cfl * min_spacing(N, grade, first) / (C0 * (1.0 + mach))
 
 
 # "Predictor at Start of Step" values (input_fform.cc:636-643)
@@ -99,13 +105,13 @@ def time_integration(rhoinf=-1.0, ipred=1):
 
 
 def solver_params(N, mach, kspace, etol=1e-3, min_iters=0, grade=1.0, cfl=CFL,
-                  rhoinf=-1.0, ipred=1, idiff=0):
+                  rhoinf=-1.0, ipred=1, idiff=0, first=0.0):
     """The COMMON-block scalars the path reads, as a plain dict.  idiff:
     0, 1 (what "Include Viscous Correction in Stabilization: True" sets for
     linear elements, input_fform.cc:706-711) or 3: global reconstruction of
     the diffusive flux with row-sum or special lumping (elmgmr.f:361-391,
     e3q.f:185-205)."""
-    dt = time_step(N, mach, grade, cfl)
+    dt = time_step(N, mach, grade, cfl, first)
     return dict(Rgas=RGAS, gamma=GAMMA, gamma1=GAMMA1, pr=PRANDTL, mu=MU,
                 dtsfct=1.0, taucfct=1.0, temper=1.0, Dtgl=1.0 / dt,
                 etol=etol, epsM=EPSM, Kspace=kspace, nGMRES=1,

@@ -42,6 +42,37 @@ double ygrid(int N, double grade, int j) {
   return 1.0 - h0 * (std::pow(grade, N - j) - 1.0) / (grade - 1.0);
 }
 
+// y coordinates of the grid lines with a capped geometric boundary layer at
+// both walls: cell k from a wall (k = 0 at the wall) has height
+// min(d0 grade^k, dc) with d0 = first / N, and the core height dc is found by
+// bisection so that each half of the N cells spans 1/2.  Returns false if
+// the layer cannot fill the half (grade^k growth too slow).
+bool ygrid_layer(int N, double grade, double first, std::vector<double>& yg) {
+  const int half = N / 2;
+  const double d0 = first / N;
+  auto span = [&](double dc) {
+    double s = 0.0, d = d0;
+    for (int k = 0; k < half; ++k, d *= grade) s += std::min(d, dc);
+    return s;
+  };
+  if (span(0.5) < 0.5 || half * d0 > 0.5) return false;
+  double lo = d0, hi = 0.5;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    (span(mid) < 0.5 ? lo : hi) = mid;
+  }
+  const double dc = 0.5 * (lo + hi);
+  yg.assign(N + 1, 0.0);
+  double d = d0;
+  for (int k = 0; k < half; ++k, d *= grade) {
+    yg[k + 1] = yg[k] + std::min(d, dc);
+    yg[N - k - 1] = 1.0 - yg[k + 1];
+  }
+  yg[half] = 0.5;
+  yg[N] = 1.0;
+  return true;
+}
+
 const int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2},
                          {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
@@ -80,13 +111,23 @@ extern "C" int phm_box_counts(int N, long long* nshg, long long* nel,
 extern "C" int phm_box_mesh(int N, double jitter, double grade,
                             unsigned long long seed, double* x, int* ien,
                             int* ienb, int* iBC, int* iper) {
-  if (N < 2 || (grade != 1.0 && (N % 2) != 0) || grade < 1.0) return 1;
+  return phm_box_mesh_layer(N, jitter, grade, 0.0, seed, x, ien, ienb, iBC, iper);
+}
+
+extern "C" int phm_box_mesh_layer(int N, double jitter, double grade, double first,
+                                  unsigned long long seed, double* x, int* ien,
+                                  int* ienb, int* iBC, int* iper) {
+  if (N < 2 || (grade != 1.0 && (N % 2) != 0) || grade < 1.0 || first < 0.0) return 1;
   Box b{N, N + 1LL};
   const long long nshg = b.np1 * b.np1 * b.np1;
   const long long nel = 6LL * N * N * N;
   const long long nelb = 4LL * N * N;
   std::vector<double> yg(N + 1);
-  for (int j = 0; j <= N; ++j) yg[j] = ygrid(N, grade, j);
+  if (first > 0.0 && grade != 1.0) {
+    if (!ygrid_layer(N, grade, first, yg)) return 1;
+  } else {
+    for (int j = 0; j <= N; ++j) yg[j] = ygrid(N, grade, j);
+  }
   const double h = 1.0 / N;
 
 #pragma omp parallel for schedule(static)

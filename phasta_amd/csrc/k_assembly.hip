@@ -97,6 +97,10 @@ __device__ inline double sel4(double v0, double v1, double v2, double v3, int i)
 __device__ inline int sel4i(int v0, int v1, int v2, int v3, int i) {
   return i == 0 ? v0 : (i == 1 ? v1 : (i == 2 ? v2 : v3));
 }
+template <typename T>
+__device__ inline T sel4t(T v0, T v1, T v2, T v3, int i) {
+  return i == 0 ? v0 : (i == 1 ? v1 : (i == 2 ? v2 : v3));
+}
 
 struct Thermo {
   double u[3], rho, drdp, drdT, e1p, e2p, e3p, e4p, tau[3];
@@ -130,6 +134,98 @@ __device__ inline void A0_times(const Thermo& t, const double a[5], double r[5])
   r[4] = t.e1p * a[0] + t.rho * (t.u[0] * a[1] + t.u[1] * a[2] + t.u[2] * a[3]) + t.e4p * a[4];
 }
 
+// Inverse Jacobian of a linear tet from its corner coordinates: dx/dxi =
+// (x_j - x_4)/2 for j = 1..3 (the shape-function derivatives are halved,
+// genshp.f:34-35) inverted by cofactors (e3metric.f:22-63); det is
+// det(dx/dxi).
+__device__ inline void tet_inverse(const double xl[4][3], double ix[3][3], double& det) {
+  double d[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d[i][j] = 0.5 * (xl[j][i] - xl[3][i]);
+  ix[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
+  ix[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
+  ix[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  det = ix[0][0] * d[0][0] + ix[0][1] * d[1][0] + ix[0][2] * d[2][0];
+  const double idet = 1.0 / det;
+  ix[0][0] *= idet;
+  ix[0][1] *= idet;
+  ix[0][2] *= idet;
+  ix[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * idet;
+  ix[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * idet;
+  ix[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * idet;
+  ix[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * idet;
+  ix[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * idet;
+  ix[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * idet;
+}
+
+// shape gradients: node j<3 -> 0.5 * row j of dxi/dx, node 3 -> -(sum)
+__device__ inline void tet_shg(const double ix[3][3], double shg[4][3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    shg[0][k] = 0.5 * ix[0][k];
+    shg[1][k] = 0.5 * ix[1][k];
+    shg[2][k] = 0.5 * ix[2][k];
+    shg[3][k] = -0.5 * (ix[0][k] + ix[1][k] + ix[2][k]);
+  }
+}
+
+// metric tensor G = dxi/dx^T M dxi/dx with the tet invariance correction
+// (e3tau.f:1442-1474), its trace and G:G
+__device__ inline void tet_metric_tensor(const double ix[3][3], double G[6], double& trG,
+                                         double& GG) {
+  const double c1 = 1.259921049894873e+00, c2 = 6.299605249474365e-01;
+  double t[3][3];  // t[k][j] = sum_l M_kl ix[l][j], M = c2 + (c1-c2) I
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    t[0][j] = c1 * ix[0][j] + c2 * (ix[1][j] + ix[2][j]);
+    t[1][j] = c1 * ix[1][j] + c2 * (ix[0][j] + ix[2][j]);
+    t[2][j] = c1 * ix[2][j] + c2 * (ix[0][j] + ix[1][j]);
+  }
+  G[0] = ix[0][0] * t[0][0] + ix[1][0] * t[1][0] + ix[2][0] * t[2][0];
+  G[1] = ix[0][0] * t[0][1] + ix[1][0] * t[1][1] + ix[2][0] * t[2][1];
+  G[2] = ix[0][1] * t[0][1] + ix[1][1] * t[1][1] + ix[2][1] * t[2][1];
+  G[3] = ix[0][0] * t[0][2] + ix[1][0] * t[1][2] + ix[2][0] * t[2][2];
+  G[4] = ix[0][1] * t[0][2] + ix[1][1] * t[1][2] + ix[2][1] * t[2][2];
+  G[5] = ix[0][2] * t[0][2] + ix[1][2] * t[1][2] + ix[2][2] * t[2][2];
+  trG = G[0] + G[2] + G[5];
+  GG = G[0] * G[0] + G[2] * G[2] + G[5] * G[5] + 2.0 * (G[1] * G[1] + G[3] * G[3] + G[4] * G[4]);
+}
+
+// Thermodynamic state at a point from Y = (p, u, v, w, T) (getthm.f:106-172
+// with ipress = 0; e3mtrx.f's A0/A_i entries) and Shakib's diagonal tau
+// (itau = 0, e3tau.f:140-176).  ei: internal energy, rk: kinetic energy.
+__device__ inline void point_thermo(const double Y[5], const PhysParams& ph, const double G[6],
+                                    double trG, double GG, Thermo& t, double& ei, double& rk) {
+  const double cv = ph.Rgas / ph.gamma1;
+  const double cp = ph.Rgas * ph.gamma / ph.gamma1;
+  const double mu = ph.mu;
+  const double dts = ph.dtsfct * ph.Dtgl;
+  const double pres = Y[0], Tq = Y[4];
+  t.u[0] = Y[1];
+  t.u[1] = Y[2];
+  t.u[2] = Y[3];
+  rk = 0.5 * (Y[1] * Y[1] + Y[2] * Y[2] + Y[3] * Y[3]);
+  t.rho = pres / (ph.Rgas * Tq);
+  ei = Tq * (ph.Rgas / ph.gamma1);
+  const double h = Tq * (ph.Rgas * ph.gamma / ph.gamma1);
+  const double alfap = 1.0 / Tq, betaT = 1.0 / pres;
+  t.drdp = t.rho * betaT;
+  t.drdT = -t.rho * alfap;
+  t.e1p = t.drdp * (h + rk) - alfap * Tq;
+  t.e2p = t.e1p + 1.0;
+  t.e3p = t.rho * (h + rk);
+  t.e4p = t.drdT * (h + rk) + t.rho * cp;
+  const double uGu = t.u[0] * (t.u[0] * G[0] + 2.0 * (t.u[1] * G[1] + t.u[2] * G[3])) +
+                     t.u[1] * (t.u[1] * G[2] + 2.0 * t.u[2] * G[4]) + t.u[2] * t.u[2] * G[5];
+  const double tt = t.rho * t.rho * ((2.0 * dts) * (2.0 * dts) + uGu) + 36.0 * (mu * mu) * GG;
+  const double sq = sqrt(tt);
+  t.tau[0] = 0.125 * sq / (t.rho * trG) * ph.taucfct;
+  t.tau[1] = 1.0 / sq;
+  t.tau[2] = t.tau[1] / cv * ph.temper;
+}
+
 // ---------------- pass 1: point states and residual ----------------
 // S: storage type of the point states and geometry (double, or float for
 // the mixed-precision path whose tangent is formed in float).  kDiff: the
@@ -157,58 +253,14 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   }
   const PhysParams& ph = A.ph;
 
-  // ---- metrics: dx/dxi = (x_j - x_4)/2 for j = 1..3 (shgl = dN/2) ----
-  double d[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) d[i][j] = 0.5 * (xl[j][i] - xl[3][i]);
-  double ix[3][3];  // dxi/dx
-  ix[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
-  ix[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
-  ix[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
-  const double det = ix[0][0] * d[0][0] + ix[0][1] * d[1][0] + ix[0][2] * d[2][0];
-  const double idet = 1.0 / det;
-  ix[0][0] *= idet;
-  ix[0][1] *= idet;
-  ix[0][2] *= idet;
-  ix[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * idet;
-  ix[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * idet;
-  ix[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * idet;
-  ix[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * idet;
-  ix[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * idet;
-  ix[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * idet;
+  // ---- metrics, shape gradients and the tau metric tensor ----
+  double ix[3][3], det;
+  tet_inverse(xl, ix, det);
   const double WdetJ = cQ.Qwt * det;
-  // shape gradients: node j<3 -> 0.5 * row j of dxi/dx, node 3 -> -(sum)
   double shg[4][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    shg[0][k] = 0.5 * ix[0][k];
-    shg[1][k] = 0.5 * ix[1][k];
-    shg[2][k] = 0.5 * ix[2][k];
-    shg[3][k] = -0.5 * (ix[0][k] + ix[1][k] + ix[2][k]);
-  }
-  // metric tensor with the tet invariance correction (e3tau.f:1442-1474)
-  double G[6];
-  {
-    const double c1 = 1.259921049894873e+00, c2 = 6.299605249474365e-01;
-    double t[3][3];  // t[k][j] = sum_l M_kl ix[l][j], M = c2 + (c1-c2) I
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      t[0][j] = c1 * ix[0][j] + c2 * (ix[1][j] + ix[2][j]);
-      t[1][j] = c1 * ix[1][j] + c2 * (ix[0][j] + ix[2][j]);
-      t[2][j] = c1 * ix[2][j] + c2 * (ix[0][j] + ix[1][j]);
-    }
-    G[0] = ix[0][0] * t[0][0] + ix[1][0] * t[1][0] + ix[2][0] * t[2][0];
-    G[1] = ix[0][0] * t[0][1] + ix[1][0] * t[1][1] + ix[2][0] * t[2][1];
-    G[2] = ix[0][1] * t[0][1] + ix[1][1] * t[1][1] + ix[2][1] * t[2][1];
-    G[3] = ix[0][0] * t[0][2] + ix[1][0] * t[1][2] + ix[2][0] * t[2][2];
-    G[4] = ix[0][1] * t[0][2] + ix[1][1] * t[1][2] + ix[2][1] * t[2][2];
-    G[5] = ix[0][2] * t[0][2] + ix[1][2] * t[1][2] + ix[2][2] * t[2][2];
-  }
-  const double trG = G[0] + G[2] + G[5];
-  const double GG = G[0] * G[0] + G[2] * G[2] + G[5] * G[5] +
-                    2.0 * (G[1] * G[1] + G[3] * G[3] + G[4] * G[4]);
+  tet_shg(ix, shg);
+  double G[6], trG, GG;
+  tet_metric_tensor(ix, G, trG, GG);
   if (valid && q == 0) {
     S* go = reinterpret_cast<S*>(A.geo) + (size_t)e * rec_stride<S>();
 #pragma unroll
@@ -250,36 +302,13 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
     }
   }
 
-  const double cv = ph.Rgas / ph.gamma1;
   const double cp = ph.Rgas * ph.gamma / ph.gamma1;
   const double mu = ph.mu, lam = -(2.0 / 3.0) * mu, kap = mu * cp / ph.pr;
-  const double dts = ph.dtsfct * ph.Dtgl;
   const double divu = g[0][1] + g[1][2] + g[2][3];
-
-  const double pres = Y[0], Tq = Y[4];
+  const double pres = Y[0];
   Thermo t;
-  t.u[0] = Y[1];
-  t.u[1] = Y[2];
-  t.u[2] = Y[3];
-  const double rk = 0.5 * (Y[1] * Y[1] + Y[2] * Y[2] + Y[3] * Y[3]);
-  t.rho = pres / (ph.Rgas * Tq);
-  const double ei = Tq * (ph.Rgas / ph.gamma1);
-  const double h = Tq * (ph.Rgas * ph.gamma / ph.gamma1);
-  const double alfap = 1.0 / Tq, betaT = 1.0 / pres;
-  t.drdp = t.rho * betaT;
-  t.drdT = -t.rho * alfap;
-  t.e1p = t.drdp * (h + rk) - alfap * Tq;
-  t.e2p = t.e1p + 1.0;
-  t.e3p = t.rho * (h + rk);
-  t.e4p = t.drdT * (h + rk) + t.rho * cp;
-  // Shakib diagonal tau (itau = 0, ipord = 1)
-  const double uGu = t.u[0] * (t.u[0] * G[0] + 2.0 * (t.u[1] * G[1] + t.u[2] * G[3])) +
-                     t.u[1] * (t.u[1] * G[2] + 2.0 * t.u[2] * G[4]) + t.u[2] * t.u[2] * G[5];
-  const double tt = t.rho * t.rho * ((2.0 * dts) * (2.0 * dts) + uGu) + 36.0 * (mu * mu) * GG;
-  const double sq = sqrt(tt);
-  t.tau[0] = 0.125 * sq / (t.rho * trG) * ph.taucfct;
-  t.tau[1] = 1.0 / sq;
-  t.tau[2] = t.tau[1] / cv * ph.temper;
+  double ei, rk;
+  point_thermo(Y, ph, G, trG, GG, t, ei, rk);
   if (valid) {
     S* o = reinterpret_cast<S*>(A.qps) + ((size_t)e * 4 + q) * rec_stride<S>();
     o[0] = (S)t.u[0];
@@ -376,6 +405,174 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
 #pragma unroll
     for (int f = 0; f < 5; ++f) ro[f] = mine[f];
   }
+}
+
+// ---------------- FP64 product with the assembled tangent ----------------
+// w_a = sum_b EGmass(a,b) v_b for every element, without forming EGmass:
+// with EGmass(a,b) = sum_q WdetJ [P_a Q_b] + K_ab, P_a = A(s_a) diag(tau) +
+// N_a I and Q_b = A(s_b) + c N_b A0 (the blocks the assembly forms),
+//     sum_b Q_b v_b = sum_i A_i (grad v)_i + c A0 v(x_q) =: L_v
+// and sum_b K_ab v_b is the viscous flux of v with the velocity frozen at
+// the point.  So lane q forms L_v, tau L_v, the fluxes A_i tau L_v plus the
+// viscous flux, and its contribution WdetJ [sum_i s_a,i F_i + N_a L_v] to
+// the four nodes, summed over the element's four lanes as the residual is.
+// The point states are recomputed in double from the node lines, so the
+// product is the FP64 tangent whatever precision the stored matrix has.
+// bc3LHS: v's constrained components at an element node are not seen (zero
+// columns) and constrained rows are zeroed here; k_jv_nodes puts the unit
+// diagonals of the constrained rows back.  v is read at the periodic master
+// of each node (the p(iper) gather of sparseap.f:33-35).
+struct JvArgs {
+  const int* ien;          // 4 planes of nel
+  const double* node;      // 16 per node: y (local order), ac, x
+  const double* v;         // 5 planes, local order
+  const int* iper;         // 0-based master
+  const uint8_t* bcm;
+  double* out;             // (nel*4) x 5: element e's product at its node a
+  long long nel, nshg;
+  PhysParams ph;
+  double cfac;             // almi/(gami alfi) Dtgl
+};
+
+__global__ void __launch_bounds__(256) k_jv(JvArgs A) {
+  const long long e_raw = blockIdx.x * 64LL + threadIdx.x / 4;
+  const int q = threadIdx.x & 3;
+  const bool valid = e_raw < A.nel;
+  const long long e = valid ? e_raw : A.nel - 1;
+  const long long nel = A.nel, nshg = A.nshg;
+  const PhysParams& ph = A.ph;
+  int nd[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) nd[k] = A.ien[k * nel + e];
+  double xl[4][3], yl[4][5];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2* v = reinterpret_cast<const double2*>(A.node + 16 * (size_t)nd[k]);
+    const double2 y01 = v[0], y23 = v[1], y4a = v[2], x01 = v[5], x2 = v[6];
+    yl[k][0] = y01.x;
+    yl[k][1] = y01.y;
+    yl[k][2] = y23.x;
+    yl[k][3] = y23.y;
+    yl[k][4] = y4a.x;
+    xl[k][0] = x01.x;
+    xl[k][1] = x01.y;
+    xl[k][2] = x2.x;
+  }
+  double ix[3][3], det;
+  tet_inverse(xl, ix, det);
+  const double W = cQ.Qwt * det;
+  double shg[4][3];
+  tet_shg(ix, shg);
+  double G[6], trG, GG;
+  tet_metric_tensor(ix, G, trG, GG);
+  double Nq[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) Nq[k] = sel4(cQ.N[0][k], cQ.N[1][k], cQ.N[2][k], cQ.N[3][k], q);
+  double Y[5], gv[3][5], vq[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) Y[f] = vq[f] = gv[0][f] = gv[1][f] = gv[2][f] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const long long m = A.iper[nd[k]];
+    const uint8_t cm = A.bcm[nd[k]];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      Y[f] += Nq[k] * yl[k][f];
+      const double vv = ((cm >> f) & 1) ? 0.0 : A.v[m + f * nshg];
+      vq[f] += Nq[k] * vv;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) gv[i][f] += shg[k][i] * vv;
+    }
+  }
+  Thermo t;
+  double ei, rk;
+  point_thermo(Y, ph, G, trG, GG, t, ei, rk);
+  const double cp = ph.Rgas * ph.gamma / ph.gamma1;
+  const double mu = ph.mu, lam = -(2.0 / 3.0) * mu, kap = mu * cp / ph.pr;
+  // L_v = sum_i A_i (grad v)_i + c A0 v(x_q)
+  double L[5], tmp[5], m0[5];
+  A0_times(t, vq, m0);
+  Ai_times(t, 0, gv[0], L);
+#pragma unroll
+  for (int i = 1; i < 3; ++i) {
+    Ai_times(t, i, gv[i], tmp);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) L[f] += tmp[f];
+  }
+#pragma unroll
+  for (int f = 0; f < 5; ++f) L[f] += A.cfac * m0[f];
+  double TL[5] = {t.tau[0] * L[0], t.tau[1] * L[1], t.tau[1] * L[2], t.tau[1] * L[3],
+                  t.tau[2] * L[4]};
+  const double divv = gv[0][1] + gv[1][2] + gv[2][3];
+  double F[3][5];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Ai_times(t, i, TL, F[i]);
+    double tv[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      tv[m] = mu * (gv[i][1 + m] + gv[m][1 + i]) + (m == i ? lam * divv : 0.0);
+      F[i][1 + m] += tv[m];
+    }
+    F[i][4] += t.u[0] * tv[0] + t.u[1] * tv[1] + t.u[2] * tv[2] + kap * gv[i][4];
+  }
+  double mine[5];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      double r = W * (shg[a][0] * F[0][f] + shg[a][1] * F[1][f] + shg[a][2] * F[2][f]) +
+                 Nq[a] * W * L[f];
+      r += __shfl_xor(r, 1, 64);
+      r += __shfl_xor(r, 2, 64);
+      if (a == q) mine[f] = r;
+    }
+  }
+  if (valid) {
+    const int nq = sel4i(nd[0], nd[1], nd[2], nd[3], q);
+    const uint8_t rm = A.bcm[nq];
+    double* o = A.out + ((size_t)e * 4 + q) * 5;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) o[f] = ((rm >> f) & 1) ? 0.0 : mine[f];
+  }
+}
+
+// w_n = sum of the node's element products in element order (the node's
+// incidence list, sliced by 64-node groups); a constrained dof f of node n
+// takes the unit diagonals of its element blocks, cnt(n) v(iper(n))_f
+// (bc3lhs.f:55-235: one per element containing the node).
+__global__ void __launch_bounds__(256) k_jv_nodes(const long long* __restrict__ goff,
+                                                  const int* __restrict__ inc,
+                                                  const double* __restrict__ out,
+                                                  const double* __restrict__ v,
+                                                  const int* __restrict__ iper,
+                                                  const uint8_t* __restrict__ bcm,
+                                                  double* __restrict__ w, long long nshg) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= nshg) return;
+  const long long g = n >> 6;
+  const int lane = (int)(n & 63);
+  const long long o0 = goff[g];
+  const int deg = (int)((goff[g + 1] - o0) >> 6);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int cnt = 0;
+  for (int j = 0; j < deg; ++j) {
+    const int code = inc[o0 + 64LL * j + lane];
+    if (code < 0) continue;
+    ++cnt;
+    const double* r = out + (size_t)code * 5;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) acc[f] += r[f];
+  }
+  const uint8_t m = bcm[n];
+  if (m) {
+    const long long mm = iper[n];
+#pragma unroll
+    for (int f = 0; f < 5; ++f)
+      if ((m >> f) & 1) acc[f] = (double)cnt * v[mm + f * nshg];
+  }
+#pragma unroll
+  for (int f = 0; f < 5; ++f) w[n + f * nshg] = acc[f];
 }
 
 // ---------------- pass 2: block owners ----------------
@@ -622,6 +819,285 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = make_double2(sum[2 * k], sum[2 * k + 1]);
     v[12] = make_double2(sum[24], 0.0);
+  }
+}
+
+// ---------------- pass 2: row lanes ----------------
+// One lane per (element e, local node a): the lane forms row a of the
+// element's tangent, the four blocks (a, b), b = 0..3, summed over the four
+// points, from the stored point states: P_a = A(s_a) diag(tau) + N_a I is
+// formed once per point and shared by the four blocks (Q_b = A(s_b) + c N_b
+// A0 enters in factored form, as its rows 0 and 4 and rho w).  The lanes of
+// one node (one matrix row) lie in one workgroup, in element order, so the
+// blocks of the row's CSR entries are summed inside the workgroup: in round
+// b every lane puts its block (a, b) in LDS and owner lanes add the
+// contributions listed for their chunk of a CSR entry (built on the host in
+// element order); a leader combines the chunks of an entry in double,
+// applies bc3LHS and stores the block once.  The owner of a diagonal entry
+// also sums the node's residual and its BDiag.  No atomics: every output is
+// written once, in a fixed order.
+//
+// T: tangent arithmetic and stored block type; BH: blocks formed per pass
+// (4 in float; 2 in double, two passes, to keep the accumulators in
+// registers).
+struct RowArgs {
+  const int* code;          // kRowLanes per workgroup: e * 4 + a, -1 idle
+  const int4* task;         // kRowLanes per workgroup: contribution-list
+                            // offset, CSR entry, count (0: no task), flags
+                            // (bit 0 leader, bit 1 diagonal, bits 2..7 chunks
+                            // of the entry, bits 8.. element count of the row)
+  const uint16_t* list;     // (b << 8) | lane of each contribution
+  const int* ien;           // 4 planes of nel
+  const uint8_t* bcm;
+  const void* qps;
+  const void* geo;
+  const double* resid;      // (nel * 4) x 5
+  double* res;              // 5 per node
+  double* bdiag;            // 25 per node
+  void* lhs;                // one padded block per CSR entry
+  long long nel, nwg;
+  int lhs_on, prec_on;
+  double mu, lam, kap, cfac;
+};
+
+template <typename T, int BH>
+__global__ void __launch_bounds__(kRowLanes) k_asm_rows(RowArgs A) {
+  constexpr int SS = aos_stride<T>();  // 16-byte aligned block slots
+  __shared__ T slot[kRowLanes * SS];
+  __shared__ double rst[kRowLanes * 5];
+  // consecutive workgroups go round robin over the 8 XCDs: give each XCD a
+  // contiguous range of rows (their elements' point states share its L2)
+  const long long per = (gridDim.x + 7) / 8;
+  const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (wg >= A.nwg) return;  // the whole workgroup
+  const int t = threadIdx.x;
+  const long long base = wg * kRowLanes;
+  const int code = A.code[base + t];
+  const int4 tk = A.task[base + t];
+  const bool act = code >= 0;
+  const long long e = act ? (code >> 2) : 0;
+  const int a = act ? (code & 3) : 0;
+  using S = T;  // point states are stored in the tangent's type
+  const S* go = reinterpret_cast<const S*>(A.geo) + (size_t)e * rec_stride<S>();
+  const S* qrec = reinterpret_cast<const S*>(A.qps) + (size_t)e * 4 * rec_stride<S>();
+  T sg[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sg[k][c] = go[3 * k + c];
+  const T W = go[12];
+  const T sA[3] = {sel4t(sg[0][0], sg[1][0], sg[2][0], sg[3][0], a),
+                   sel4t(sg[0][1], sg[1][1], sg[2][1], sg[3][1], a),
+                   sel4t(sg[0][2], sg[1][2], sg[2][2], sg[3][2], a)};
+  const T muR = (T)A.mu, lamR = (T)A.lam, kapR = (T)A.kap, cfacR = (T)A.cfac;
+  T own[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) own[i] = (T)0;
+#pragma unroll 1
+  for (int h = 0; h < 4 / BH; ++h) {
+    T acc[BH][25];
+#pragma unroll
+    for (int bb = 0; bb < BH; ++bb)
+#pragma unroll
+      for (int i = 0; i < 25; ++i) acc[bb][i] = (T)0;
+    T ub[3] = {(T)0, (T)0, (T)0};
+    if (act) {
+#pragma unroll 1
+      for (int q = 0; q < 4; ++q) {
+        S o[16];
+        load_rec<S>(qrec + q * rec_stride<S>(), o);
+        const T u0 = o[0], u1 = o[1], u2 = o[2];
+        const T uq[3] = {u0, u1, u2};
+        ub[0] += u0;
+        ub[1] += u1;
+        ub[2] += u2;
+        const T rho = o[3], drdp = o[4], drdT = o[5];
+        const T e1p = o[6], e2p = o[7], e3p = o[8], e4p = o[9];
+        const T tau0 = o[10], tau1 = o[11], tau2 = o[12];
+        const T Na = W * (T)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
+        const T una = u0 * sA[0] + u1 * sA[1] + u2 * sA[2];
+        // P = A(s_a) diag(tau) + N_a I (rows m, columns k); the stored tau
+        // is WdetJ tau and N_a carries WdetJ, so P comes out times WdetJ
+        T P[5][5];
+        P[0][0] = drdp * una * tau0 + Na;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) P[0][1 + n] = rho * sA[n] * tau1;
+        P[0][4] = drdT * una * tau2;
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) {
+          P[1 + mm][0] = (drdp * uq[mm] * una + sA[mm]) * tau0;
+#pragma unroll
+          for (int n = 0; n < 3; ++n)
+            P[1 + mm][1 + n] =
+                rho * (uq[mm] * sA[n] + (mm == n ? una : (T)0)) * tau1 + (mm == n ? Na : (T)0);
+          P[1 + mm][4] = drdT * uq[mm] * una * tau2;
+        }
+        P[4][0] = e2p * una * tau0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) P[4][1 + n] = (e3p * sA[n] + rho * uq[n] * una) * tau1;
+        P[4][4] = e4p * una * tau2 + Na;
+        // t_m = P(m,0) + sum_k P(m,1+k) u_k: the part of row m shared by
+        // Q's columns through its rank-one terms
+        T tm[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) tm[m] = P[m][0] + (P[m][1] * u0 + P[m][2] * u1 + P[m][3] * u2);
+#pragma unroll
+        for (int bb = 0; bb < BH; ++bb) {
+          const int b = h * BH + bb;
+          const T rR[3] = {sel4t(sg[0][0], sg[1][0], sg[2][0], sg[3][0], b),
+                           sel4t(sg[0][1], sg[1][1], sg[2][1], sg[3][1], b),
+                           sel4t(sg[0][2], sg[1][2], sg[2][2], sg[3][2], b)};
+          const T Nb = (T)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
+          const T unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
+          const T w = unb + cfacR * Nb;
+          // Q = A(s_b) + c N_b A0: column 0 = drdp w u + r, columns 1..3
+          // = rho (u r^T + w I) on the momentum rows, column 4 = drdT w u
+          const T q00 = drdp * w, q04 = drdT * w, rw = rho * w;
+          T q0[3], q4[3];
+#pragma unroll
+          for (int n = 0; n < 3; ++n) {
+            q0[n] = rho * rR[n];
+            q4[n] = e3p * rR[n] + rw * uq[n];
+          }
+          const T q40 = e2p * unb + cfacR * Nb * e1p, q44 = e4p * w;
+#pragma unroll
+          for (int m = 0; m < 5; ++m) {
+            T& c0 = acc[bb][m];
+            c0 += tm[m] * q00;
+            c0 += P[m][1] * rR[0];
+            c0 += P[m][2] * rR[1];
+            c0 += P[m][3] * rR[2];
+            c0 += P[m][4] * q40;
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+              T& cn = acc[bb][m + 5 * (1 + n)];
+              cn += tm[m] * q0[n];
+              cn += rw * P[m][1 + n];
+              cn += P[m][4] * q4[n];
+            }
+            T& c4 = acc[bb][m + 20];
+            c4 += tm[m] * q04;
+            c4 += P[m][4] * q44;
+          }
+        }
+      }
+      // viscous part sum_q WdetJ K_ab: the momentum rows and the conduction
+      // term do not depend on the point, the energy row only through the
+      // velocity, linearly (sum over the points = ub)
+#pragma unroll
+      for (int bb = 0; bb < BH; ++bb) {
+        const int b = h * BH + bb;
+        const T rR[3] = {sel4t(sg[0][0], sg[1][0], sg[2][0], sg[3][0], b),
+                         sel4t(sg[0][1], sg[1][1], sg[2][1], sg[3][1], b),
+                         sel4t(sg[0][2], sg[1][2], sg[2][2], sg[3][2], b)};
+        const T st = sA[0] * rR[0] + sA[1] * rR[1] + sA[2] * rR[2];
+        const T W4 = (T)4 * W;
+        const T ubs = ub[0] * sA[0] + ub[1] * sA[1] + ub[2] * sA[2];
+        const T ubr = ub[0] * rR[0] + ub[1] * rR[1] + ub[2] * rR[2];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int n = 0; n < 3; ++n)
+            acc[bb][(1 + m) + 5 * (1 + n)] +=
+                W4 * (muR * ((m == n ? st : (T)0) + sA[n] * rR[m]) + lamR * sA[m] * rR[n]);
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+          acc[bb][4 + 5 * (1 + n)] += W * (muR * (st * ub[n] + sA[n] * ubr) + lamR * ubs * rR[n]);
+        acc[bb][24] += W4 * kapR * st;
+      }
+    }
+    // rounds: block b of every lane through LDS to the owners of its entry
+#pragma unroll
+    for (int bb = 0; bb < BH; ++bb) {
+      const int b = h * BH + bb;
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 25; ++i) slot[t * SS + i] = acc[bb][i];
+      __syncthreads();
+      for (int c = 0; c < tk.z; ++c) {
+        const int v = A.list[tk.x + c];
+        if ((v >> 8) != b) continue;
+        const T* src = slot + (v & 255) * SS;
+#pragma unroll
+        for (int i = 0; i < 25; ++i) own[i] += src[i];
+      }
+    }
+  }
+  // the node's residual: the element residuals of the diagonal entry's
+  // contributions (lane m's own element and node)
+  const bool diag = (tk.w >> 1) & 1;
+  double rown[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (diag) {
+    for (int c = 0; c < tk.z; ++c) {
+      const int mcode = A.code[base + (A.list[tk.x + c] & 255)];
+      const double* ro = A.resid + (size_t)mcode * 5;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) rown[f] += ro[f];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 25; ++i) slot[t * SS + i] = own[i];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) rst[t * 5 + f] = rown[f];
+  __syncthreads();
+  if (!(tk.w & 1) || tk.z == 0) return;
+  // leader: the chunks of the entry are the next nch lanes' tasks
+  const int nch = (tk.w >> 2) & 63;
+  const int total = tk.w >> 8;
+  double sum[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) sum[i] = (double)slot[t * SS + i];
+  for (int cc = 1; cc < nch; ++cc)
+#pragma unroll
+    for (int i = 0; i < 25; ++i) sum[i] += (double)slot[(t + cc) * SS + i];
+  const long long k = tk.y;
+  const int mcode0 = A.code[base + (A.list[tk.x] & 255)];
+  const long long e0 = mcode0 >> 2;
+  const int a0 = mcode0 & 3, b0 = (A.list[tk.x] >> 8) & 3;
+  const long long nel = A.nel;
+  const int v = A.ien[a0 * nel + e0], wcol = A.ien[b0 * nel + e0];
+  if (diag) {
+    double racc[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) racc[f] = rst[t * 5 + f];
+    for (int cc = 1; cc < nch; ++cc)
+#pragma unroll
+      for (int f = 0; f < 5; ++f) racc[f] += rst[(t + cc) * 5 + f];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) A.res[5 * (size_t)v + f] = racc[f];
+    // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103)
+    if (A.prec_on) {
+#pragma unroll
+      for (int i = 0; i < 25; ++i) A.bdiag[25 * (size_t)v + i] = sum[i];
+    }
+  }
+  if (!A.lhs_on) return;
+  // bc3LHS on every element block: constrained rows of node a and columns of
+  // node b zeroed, a unit diagonal per element block of the node
+  const uint8_t ma = A.bcm[v], mb = A.bcm[wcol];
+  if (ma | mb) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        if (((ma >> i) & 1) || ((mb >> j) & 1)) sum[i + 5 * j] = 0.0;
+        if (diag && i == j && ((ma >> i) & 1)) sum[i + 5 * j] = (double)total;
+      }
+  }
+  T* dst = reinterpret_cast<T*>(A.lhs) + (size_t)aos_stride<T>() * k;
+  if constexpr (sizeof(T) == 4) {
+    float4* vv = reinterpret_cast<float4*>(dst);
+#pragma unroll
+    for (int kk = 0; kk < 6; ++kk)
+      vv[kk] = make_float4((float)sum[4 * kk], (float)sum[4 * kk + 1], (float)sum[4 * kk + 2],
+                           (float)sum[4 * kk + 3]);
+    vv[6] = make_float4((float)sum[24], 0.f, 0.f, 0.f);
+  } else {
+    double2* vv = reinterpret_cast<double2*>(dst);
+#pragma unroll
+    for (int kk = 0; kk < 12; ++kk) vv[kk] = make_double2(sum[2 * kk], sum[2 * kk + 1]);
+    vv[12] = make_double2(sum[24], 0.0);
   }
 }
 
@@ -1170,11 +1646,39 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.lam = -(2.0 / 3.0) * ph.mu;
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
-  const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
-  if (mixed)
-    k_asm_blocks<float, float><<<g2, 64, 0, c->stream>>>(B);
-  else
-    k_asm_blocks<double, double><<<g2, 64, 0, c->stream>>>(B);
+  if (c->rows_pass) {
+    RowArgs R{};
+    R.code = c->rl_code;
+    R.task = c->rl_task;
+    R.list = c->rl_list;
+    R.ien = c->ienc;
+    R.bcm = c->bcm;
+    R.qps = Q.qps;
+    R.geo = Q.geo;
+    R.resid = c->resid;
+    R.res = c->acc_res;
+    R.bdiag = c->acc_bd;
+    R.lhs = B.lhs;
+    R.nel = c->nel;
+    R.nwg = c->rl_nwg;
+    R.lhs_on = lhs;
+    R.prec_on = prec;
+    R.mu = B.mu;
+    R.lam = B.lam;
+    R.kap = B.kap;
+    R.cfac = B.cfac;
+    const int g3 = (int)((c->rl_nwg + 7) / 8 * 8);  // whole XCD rounds
+    if (mixed)
+      k_asm_rows<float, 4><<<g3, kRowLanes, 0, c->stream>>>(R);
+    else
+      k_asm_rows<double, 1><<<g3, kRowLanes, 0, c->stream>>>(R);
+  } else {
+    const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
+    if (mixed)
+      k_asm_blocks<float, float><<<g2, 64, 0, c->stream>>>(B);
+    else
+      k_asm_blocks<double, double><<<g2, 64, 0, c->stream>>>(B);
+  }
   PHG_HIP(hipGetLastError());
   // wall faces: residual only, added colour by colour
   WallArgs W{};
@@ -1193,6 +1697,29 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   }
   PHG_HIP(hipGetLastError());
   return 0;
+}
+
+int launch_jv(phg_ctx* c, const double* v, double* w) {
+  PHG_TRY(upload_quadrature());
+  const PhysParams ph = phys_of(c->prm);
+  JvArgs J{};
+  J.ien = c->ienc;
+  J.node = c->nodepack;
+  J.v = v;
+  J.iper = c->iper;
+  J.bcm = c->bcm;
+  J.out = c->resid;
+  J.nel = c->nel;
+  J.nshg = c->nshg;
+  J.ph = ph;
+  J.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
+  k_jv<<<(int)((c->nel + 63) / 64), 256, 0, c->stream>>>(J);
+  PHG_HIP(hipGetLastError());
+  const long long n = c->nshg;
+  k_jv_nodes<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->inc_goff, c->inc, c->resid, v,
+                                                            c->iper, c->bcm, w, n);
+  PHG_HIP(hipGetLastError());
+  return launch_bc3per(c, w, 5);
 }
 
 int launch_bc_res_bdiag(phg_ctx* c, bool prec) {

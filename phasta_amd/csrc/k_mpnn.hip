@@ -1136,7 +1136,7 @@ int mp_train_residual(phg_ctx* c, const double* v, float lr) {
   MpNet& M = c->mp;
   const long long n = c->nshg, V = 5 * n;
   PHG_TRY(mp_forward(c, v, M.zout, true));              // sbuf[0] = ||v||^2, sbuf[1] = s
-  PHG_TRY(launch_spmv(c, M.zout, M.rbuf, true));
+  PHG_TRY(launch_spmv(c, M.zout, M.rbuf));
   PHG_TRY(launch_bc3per(c, M.rbuf, 5));
   PHG_TRY(launch_sub(c, M.gbuf, v, M.rbuf, V));         // gbuf = v - K z
   PHG_TRY(launch_dot(c, M.gbuf, M.gbuf, V, M.sbuf + 3));

@@ -685,8 +685,7 @@ int launch_spsi3pre(phg_ctx* c) {
   return 0;
 }
 
-int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact) {
-  (void)exact;  // both matvecs of the mixed path form double row sums
+int launch_spmv(phg_ctx* c, const double* p, double* q) {
   const int g = (int)((c->ngroups + 7) / 8 * 8);  // one workgroup per group, XCD-strided
   const int nt = kSpmvWaves * kGroup;
   if (c->precision == PHG_MIXED)

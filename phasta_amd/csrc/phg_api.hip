@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -76,6 +77,140 @@ bool bc_code_supported(int ibc) {
   if (ibc & 1) return false;          // density constraint
   if (ibc & (1 << 11)) return false;  // scaled plane extraction
   return v == 0 || v == 7;
+}
+
+// Row lanes of the block pass (k_asm_rows): one lane per (element, local
+// node) incidence; the lanes of a node (one matrix row) are consecutive, in
+// element order, and never split across workgroups of kRowLanes lanes.
+// Per workgroup the owner tasks of its rows' CSR entries: each entry's
+// contributions (lane, b) in element order, cut into chunks (tasks) of at
+// most `chunk` contributions, the chunks of one entry on consecutive lanes.
+// The chunk size doubles for a workgroup whose tasks would outnumber its
+// lanes.
+struct RowLayout {
+  long long nwg = 0;
+  std::vector<int> code;       // nwg * kRowLanes: e * 4 + a, -1 idle
+  std::vector<int4> task;      // nwg * kRowLanes
+  std::vector<uint16_t> list;  // (b << 8) | lane
+};
+
+int build_row_layout(long long nshg, long long nel, const std::vector<int>& rows,
+                     const std::vector<int>& incp, const std::vector<int>& incl,
+                     const std::vector<uint32_t>& bmap, const int* colm, const int* rowp,
+                     RowLayout& L) {
+  // pack rows (in the given order) into workgroups
+  std::vector<long long> wg_row0(1, 0);
+  {
+    int used = 0;
+    for (long long ri = 0; ri < nshg; ++ri) {
+      const int n = incp[rows[ri] + 1] - incp[rows[ri]];
+      if (n > kRowLanes || n == 0) {
+        phg_set_error("node %d belongs to %d elements (1..%d supported)", rows[ri] + 1, n,
+                      kRowLanes);
+        return PHG_EINVAL;
+      }
+      if (used + n > kRowLanes) {
+        wg_row0.push_back(ri);
+        used = 0;
+      }
+      used += n;
+    }
+    wg_row0.push_back(nshg);
+  }
+  const long long nwg = (long long)wg_row0.size() - 1;
+  L.nwg = nwg;
+  L.code.assign(nwg * kRowLanes, -1);
+  L.task.assign(nwg * kRowLanes, make_int4(0, 0, 0, 0));
+  std::vector<std::vector<uint16_t>> lists(nwg);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
+  for (long long g = 0; g < nwg; ++g) {
+    int* code = L.code.data() + g * kRowLanes;
+    int4* task = L.task.data() + g * kRowLanes;
+    int m = 0;
+    for (long long ri = wg_row0[g]; ri < wg_row0[g + 1]; ++ri)
+      for (int j = incp[rows[ri]]; j < incp[rows[ri] + 1]; ++j) code[m++] = incl[j];
+    // contributions of every entry of the workgroup's rows, element order
+    std::vector<std::vector<uint16_t>> contrib;
+    std::vector<long long> ent;  // CSR entry of each contrib list
+    std::vector<int> ent_row;    // node of the entry's row
+    m = 0;
+    for (long long ri = wg_row0[g]; ri < wg_row0[g + 1]; ++ri) {
+      const int row = rows[ri];
+      const long long k0 = colm[row] - 1, nk = colm[row + 1] - colm[row];
+      const size_t first = contrib.size();
+      for (long long k = 0; k < nk; ++k) {
+        contrib.emplace_back();
+        ent.push_back(k0 + k);
+        ent_row.push_back(row);
+      }
+      for (int j = incp[row]; j < incp[row + 1]; ++j, ++m) {
+        const long long e = code[m] >> 2;
+        const int a = code[m] & 3;
+        for (int b = 0; b < 4; ++b) {
+          const long long kk = bmap[(4 * a + b) * nel + e];
+          if (kk < k0 || kk >= k0 + nk) {
+            bad = 1;
+            continue;
+          }
+          contrib[first + (kk - k0)].push_back((uint16_t)((b << 8) | m));
+        }
+      }
+    }
+    for (const auto& cl : contrib)
+      if (cl.empty()) bad = 1;
+    if (bad) continue;
+    // chunk the entries; double the chunk size until the tasks fit the lanes
+    size_t maxc = 0;
+    for (const auto& cl : contrib) maxc = std::max(maxc, cl.size());
+    for (int chunk = 4;; chunk *= 2) {
+      long long ntask = 0;
+      for (const auto& cl : contrib) ntask += ((long long)cl.size() + chunk - 1) / chunk;
+      if (ntask > kRowLanes) {
+        if ((size_t)chunk >= maxc) {  // one task per entry does not fit either
+          bad = 2;
+          break;
+        }
+        continue;
+      }
+      int t = 0;
+      auto& lst = lists[g];
+      for (size_t q = 0; q < contrib.size(); ++q) {
+        const auto& cl = contrib[q];
+        const int n = (int)cl.size();
+        const int nch = (n + chunk - 1) / chunk;
+        const bool dg = (rowp[ent[q]] - 1) == ent_row[q];
+        for (int ch = 0; ch < nch; ++ch, ++t) {
+          const int c0 = n * ch / nch, c1 = n * (ch + 1) / nch;
+          int flags = (dg ? 2 : 0) | (ch == 0 ? 1 : 0);
+          if (ch == 0) flags |= (nch << 2) | (n << 8);
+          task[t] = make_int4((int)lst.size(), (int)ent[q], c1 - c0, flags);
+          for (int c = c0; c < c1; ++c) lst.push_back(cl[c]);
+        }
+      }
+      break;
+    }
+  }
+  if (bad) {
+    phg_set_error(bad & 1 ? "element adjacency missing from colm/rowp (row-lane layout)"
+                          : "more CSR entries than lanes in a row-lane workgroup");
+    return PHG_EINVAL;
+  }
+  // concatenate the per-workgroup lists
+  std::vector<long long> off(nwg + 1, 0);
+  for (long long g = 0; g < nwg; ++g) off[g + 1] = off[g] + (long long)lists[g].size();
+  if (off[nwg] >= (1LL << 31)) {
+    phg_set_error("row-lane contribution list too long");
+    return PHG_EINVAL;
+  }
+  L.list.resize(off[nwg]);
+#pragma omp parallel for schedule(static)
+  for (long long g = 0; g < nwg; ++g) {
+    std::copy(lists[g].begin(), lists[g].end(), L.list.begin() + off[g]);
+    for (int t = 0; t < kRowLanes; ++t)
+      if (L.task[g * kRowLanes + t].z > 0) L.task[g * kRowLanes + t].x += (int)off[g];
+  }
+  return 0;
 }
 
 int setup(phg_ctx* c, const phg_mesh* m) {
@@ -306,8 +441,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   const long long ngq = (nshg + 63) / 64;
   std::vector<long long> goff(ngq + 1, 0);
   std::vector<int> incs;
+  std::vector<int> incp(nshg + 1, 0), incl(4 * nel);
   {
-    std::vector<int> incp(nshg + 1, 0), incl(4 * nel);
     for (long long i = 0; i < 4 * nel; ++i) incp[ienc[i] + 1]++;
     for (long long n = 0; n < nshg; ++n) incp[n + 1] += incp[n];
     std::vector<int> pos(incp.begin(), incp.end() - 1);
@@ -325,6 +460,11 @@ int setup(phg_ctx* c, const phg_mesh* m) {
       for (int j = 0; j < incp[n + 1] - incp[n]; ++j) incs[goff[g] + 64LL * j + l] = incl[incp[n] + j];
     }
   }
+  RowLayout rl;
+  PHG_TRY(build_row_layout(nshg, nel, rows, incp, incl, bmap, m->colm, m->rowp, rl));
+  c->rl_nwg = rl.nwg;
+  incl.clear();
+  incl.shrink_to_fit();
   std::vector<int> ienbc(4 * std::max<long long>(nelb, 1));
   for (long long p = 0; p < nelb; ++p)
     for (int a = 0; a < 4; ++a) ienbc[a * nelb + p] = m->ienb[orderb[p] + nelb * a] - 1;
@@ -362,6 +502,9 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->ienc, 4 * nel));
   PHG_TRY(dev_alloc(c, &c->lanes, c->nlanes));
   PHG_TRY(dev_alloc(c, &c->blocks, 16 * nel));
+  PHG_TRY(dev_alloc(c, &c->rl_code, rl.code.size()));
+  PHG_TRY(dev_alloc(c, &c->rl_task, rl.task.size()));
+  PHG_TRY(dev_alloc(c, &c->rl_list, rl.list.size()));
   // point-state and geometry records: 13 values padded to 16 floats or 14
   // doubles (16-byte loads)
   if (mixed) {
@@ -421,6 +564,9 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->inc, incs.data(), incs.size()));
   PHG_TRY(upload(c, c->lanes, lanes.data(), lanes.size()));
   PHG_TRY(upload(c, c->blocks, blk_lane.data(), blk_lane.size()));
+  PHG_TRY(upload(c, c->rl_code, rl.code.data(), rl.code.size()));
+  PHG_TRY(upload(c, c->rl_task, rl.task.data(), rl.task.size()));
+  PHG_TRY(upload(c, c->rl_list, rl.list.data(), rl.list.size()));
   PHG_TRY(upload(c, c->ienbc, ienbc.data(), ienbc.size()));
   PHG_TRY(upload(c, c->bcm, bcm.data(), bcm.size()));
   PHG_TRY(upload(c, c->iper, iper0.data(), iper0.size()));
@@ -620,6 +766,19 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
     echeck = std::fabs(e[iK + 1]);
     if (echeck <= epsnrm && iK >= c->prm.minIters) break;
   }
+  if (c->diag_flags & PHG_DIAG_ORTH) {
+    // orthogonality of the cycle's basis u_0..u_iKs: max |u_i . u_j - delta_ij|
+    double* d = c->scal + 128;
+    double loss = 0.0;
+    for (int i = 0; i <= iKs; ++i) {
+      for (int j = 0; j <= i; ++j) PHG_TRY(launch_dot(c, kvec(c, i), kvec(c, j), V, d + j));
+      PHG_TRY(download(c, c->h_scal + 128, d, i + 1));
+      PHG_TRY(sync(c));
+      for (int j = 0; j <= i; ++j)
+        loss = std::max(loss, std::fabs(c->h_scal[128 + j] - (i == j ? 1.0 : 0.0)));
+    }
+    st->orth_loss = std::max(st->orth_loss, loss);
+  }
   // back substitution (solgmr.f:681-686), Dy += y_k u_k (solgmr.f:690-692)
   for (int jK = iKs; jK >= 1; --jK) {
     yB[jK] = e[jK] / Hm(jK, jK);
@@ -637,10 +796,11 @@ constexpr int kMaxRefine = 4;
 // SolGMRs on the resident y/ac (solgmr.f:449-744).  Leaves Dy in c->dy.
 //
 // Mixed precision: the tangent is assembled, scaled and stored in float;
-// the Krylov basis, MGS, Givens and norms stay in double.  After each cycle
-// the true residual of the scaled system, b - K z, is formed in double and
-// its norm is tested against the same epsnrm as the FP64 path; cycles
-// restart from that residual until it passes (at most kMaxRefine cycles).
+// the Krylov basis, Gram-Schmidt, Givens and norms stay in double.  After
+// each cycle the true residual of the scaled system, b - K z, is formed with
+// the FP64 tangent (launch_jv) and its norm is tested against the same
+// epsnrm as the FP64 path; cycles restart from that residual until it
+// passes (at most kMaxRefine cycles).
 int sol_core(phg_ctx* c, phg_stats* st) {
   const long long n = c->nshg, V = 5 * n;
   const int K = c->prm.Kspace;
@@ -662,9 +822,19 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   PHG_TRY(launch_stats(c, S));  // S[0] provisional, overwritten below
   const bool set_aside = c->mp.in_step && c->mp.step_off;
   const bool learned = c->mp.ready && c->mp.use_pc > 0 && !set_aside;
-  if ((learned || c->mp.keep_pair) && c->mp.ready)
-    PHG_HIP(hipMemcpyAsync(c->mp.bd_raw, c->bdiag, sizeof(double) * 25 * n,
-                           hipMemcpyDeviceToDevice, c->stream));
+  if ((learned || c->mp.keep_pair) && c->mp.ready) {
+    // the network's BDiag input is the block diagonal before factorisation:
+    // a call with iprec = 0 reuses the factored BDiag of an earlier call
+    // (solgmr.f:471), so the raw copy of that call is kept
+    if (c->prm.iprec != 0) {
+      PHG_HIP(hipMemcpyAsync(c->mp.bd_raw, c->bdiag, sizeof(double) * 25 * n,
+                             hipMemcpyDeviceToDevice, c->stream));
+      c->mp.bd_raw_ok = true;
+    } else if (!c->mp.bd_raw_ok) {
+      phg_set_error("learned preconditioner with iprec = 0: no earlier call formed BDiag");
+      return PHG_EINVAL;
+    }
+  }
   if (c->prm.iprec != 0) PHG_TRY(launch_i3lu(c, c->bdiag, nullptr, 0));
   PHG_TRY(launch_i3lu(c, c->bdiag, c->res, 1));
   PHG_TRY(launch_dot(c, c->res, c->res, V, S));
@@ -681,9 +851,9 @@ int sol_core(phg_ctx* c, phg_stats* st) {
     phg_set_error("tnanq: %lld NaN/large entries in res/BDiag", st->nlarge);
     return PHG_ENAN;
   }
-  // rstatCheck.f:36-79
+  // rstatCheck.f:36-79: from the second Newton iteration of a time step on
   const double totres_chk = std::sqrt(sumsq / (double)(float)n);
-  if (c->reslast > 0.0 && totres_chk > 10000.0 * c->reslast) {
+  if (c->prm.iter > 1 && totres_chk > 10000.0 * c->reslast) {
     phg_set_error("rstatCheck: residual diverging (%g > 1e4 * %g)", totres_chk, c->reslast);
     return PHG_EDIVERGE;
   }
@@ -694,18 +864,31 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   st->epsnrm = epsnrm;
   st->ms_spmv = st->ms_mgs = 0.0;
   st->n_spmv = st->n_mgs = 0;
+  st->orth_loss = 0.0;
   st->refine_cycles = 0;
   int iKs = 0;
   double echeck = 0.0;
   st->pc_fallback = set_aside && c->mp.ready && c->mp.use_pc > 0 ? 4 : 0;
   if (!(unorm < 100. * c->prm.epsM * c->prm.epsM)) {
     double rnorm = unorm;
-    // true residual of the scaled system, r = b - K dy (double products),
-    // normalised into kvec(0); returns its norm
+    // true residual of the scaled system, r = b - K dy, normalised into
+    // kvec(0); returns its norm.  FP64 path: the stored (FP64) matrix.
+    // Mixed path: the FP64 tangent without the stored float matrix,
+    // r = L^-1 (rmes - A U^-1 dy) with A applied element by element in
+    // double (launch_jv), so the refinement converges on the FP64 system.
     auto true_residual = [&](double* norm) -> int {
-      PHG_TRY(launch_spmv(c, c->dy, kvec(c, 1), true));
-      PHG_TRY(launch_bc3per(c, kvec(c, 1), 5));
-      PHG_TRY(launch_sub(c, kvec(c, 2), c->res, kvec(c, 1), V));
+      if (mixed) {
+        PHG_HIP(hipMemcpyAsync(kvec(c, 1), c->dy, sizeof(double) * V, hipMemcpyDeviceToDevice,
+                               c->stream));
+        PHG_TRY(launch_i3lu(c, c->bdiag, kvec(c, 1), 2));
+        PHG_TRY(launch_jv(c, kvec(c, 1), kvec(c, 2)));
+        PHG_TRY(launch_sub(c, kvec(c, 2), c->rmes, kvec(c, 2), V));
+        PHG_TRY(launch_i3lu(c, c->bdiag, kvec(c, 2), 1));
+      } else {
+        PHG_TRY(launch_spmv(c, c->dy, kvec(c, 1)));
+        PHG_TRY(launch_bc3per(c, kvec(c, 1), 5));
+        PHG_TRY(launch_sub(c, kvec(c, 2), c->res, kvec(c, 1), V));
+      }
       PHG_TRY(launch_dot(c, kvec(c, 2), kvec(c, 2), V, S + 5));
       PHG_TRY(launch_scale(c, kvec(c, 0), kvec(c, 2), V, S + 5, S + 6));
       PHG_TRY(download(c, c->h_scal + 5, S + 5, 2));
@@ -823,6 +1006,7 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
   c->ndofBC = mesh->ndofBC;
   c->precision = precision;
   c->prm = *prm;
+  c->rows_pass = std::getenv("PHG_OWNER_LANES") == nullptr;  // TEMP A/B
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     phg_set_error("hipStreamCreate: %s", hipGetErrorString(e));
@@ -864,7 +1048,8 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
                   c->acold,    c->dy,       c->krylov, c->red_part, c->scal,
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
-                  c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode};
+                  c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode,
+                  c->rl_code,  c->rl_task,  c->rl_list};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -990,6 +1175,7 @@ extern "C" int phg_state_download_old(phg_ctx* c, double* yold, double* acold) {
 extern "C" int phg_predict(phg_ctx* c) {
   PHG_TRY(check_ctx(c));
   PHG_TRY(launch_predict(c));
+  c->prm.iter = 0;       // itrdrv.f:409
   c->mp.in_step = true;  // a new time step: the network is tried again
   c->mp.step_off = false;
   return 0;
@@ -999,6 +1185,7 @@ extern "C" int phg_newton_iteration(phg_ctx* c, phg_stats* st) {
   PHG_TRY(check_ctx(c));
   phg_stats local{};
   if (!st) st = &local;
+  c->prm.iter += 1;  // itrdrv.f:434
   PHG_TRY(sol_core(c, st));
   PHG_TRY(launch_correct(c));
   return 0;
@@ -1024,6 +1211,17 @@ extern "C" int phg_sparse_ap(phg_ctx* c, const double* lhsK, double* p) {
   }
   PHG_TRY(upload(c, kvec(c, 0), p, 5 * n));
   PHG_TRY(launch_spmv(c, kvec(c, 0), kvec(c, 1)));
+  PHG_TRY(download(c, p, kvec(c, 1), 5 * n));
+  return sync(c);
+}
+
+extern "C" int phg_tangent_ap(phg_ctx* c, const double* y, double* p) {
+  PHG_TRY(check_ctx(c));
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->y, y, 5 * n));
+  PHG_TRY(launch_pack_nodes(c));
+  PHG_TRY(upload(c, kvec(c, 0), p, 5 * n));
+  PHG_TRY(launch_jv(c, kvec(c, 0), kvec(c, 1)));
   PHG_TRY(download(c, p, kvec(c, 1), 5 * n));
   return sync(c);
 }
@@ -1280,6 +1478,20 @@ extern "C" int phg_mpnn_last_pair(phg_ctx* c, double* bdiag_raw, double* b, doub
   if (b) PHG_TRY(download(c, b, c->res, 5 * n));
   if (zhat) PHG_TRY(download(c, zhat, c->mp.zhat, 5 * n));
   return sync(c);
+}
+
+extern "C" int phg_diagnostics(phg_ctx* c, int flags) {
+  PHG_TRY(check_ctx(c));
+  if (flags & ~PHG_DIAG_ORTH) {
+    phg_set_error("phg_diagnostics: unknown flags %d", flags);
+    return PHG_EINVAL;
+  }
+  if ((flags & PHG_DIAG_ORTH) && c->kcap > 127) {
+    phg_set_error("phg_diagnostics: Kspace %d too large for the orthogonality check", c->kcap);
+    return PHG_EINVAL;
+  }
+  c->diag_flags = flags;
+  return 0;
 }
 
 extern "C" int phg_sync(phg_ctx* c) {

@@ -102,6 +102,9 @@ __device__ inline void store_slot(double* blk, int r, const A a[25]) {
 // Largest number of element blocks one assembly lane sums (the blocks of
 // one CSR entry are split into chunks of at most this size).
 constexpr int kChunk = 6;
+// Lanes per workgroup of the row-lane block pass (three wavefronts: eight
+// interior nodes of 24 elements each).
+constexpr int kRowLanes = 192;
 
 // Physical constants and time-integration factors used in the kernels.
 struct PhysParams {
@@ -139,6 +142,7 @@ struct MpNet {
   float* inv = nullptr;      // 1/deg (0 for slaves)
   float* dscale = nullptr;   // 25 input scales of BDiag
   double* bd_raw = nullptr;  // BDiag before factorisation (25 planes)
+  bool bd_raw_ok = false;    // bd_raw holds the BDiag of a solve (iprec != 0)
   const double* bd_src = nullptr;
   double* zhat = nullptr;    // scaled-system solution of the last solve
   uint16_t* x = nullptr;     // n x 32 inputs (bf16)
@@ -194,6 +198,12 @@ struct phg_ctx {
   long long nlanes = 0;
   int4* lanes = nullptr;        // block-list offset, CSR entry, count, flags
   int* blocks = nullptr;        // e*16 + a*4 + b, grouped per owner lane
+  // row lanes of the block pass (k_asm_rows), kRowLanes per workgroup
+  long long rl_nwg = 0;
+  int* rl_code = nullptr;       // e * 4 + a per lane, -1 idle
+  int4* rl_task = nullptr;      // owner task per lane
+  uint16_t* rl_list = nullptr;  // (b << 8) | lane per contribution
+  bool rows_pass = true;        // block pass: row lanes (else owner lanes)
   double* qps = nullptr;        // (nel*4) x 14 point states (13 + pad, FP64 path)
   float* qps32 = nullptr;       // (nel*4) x 16, mixed path
   double* geo = nullptr;        // nel x 14: shape gradients, WdetJ (+ pad)
@@ -257,6 +267,7 @@ struct phg_ctx {
   double* h_scal = nullptr;        // pinned mirror
   double reslast = 0.0;
   int ntotGM = 0;
+  int diag_flags = 0;              // phg_diagnostics
 
   // scratch for host-array entry points
   double* tmp = nullptr;
@@ -274,6 +285,11 @@ int launch_fill(phg_ctx* c, void* p, size_t bytes);
 int launch_assembly(phg_ctx* c, bool lhs, bool prec);
 int launch_diffusive_flux(phg_ctx* c);
 int launch_bc_res_bdiag(phg_ctx* c, bool prec);
+// w = A v with A the FP64 assembled tangent (after bc3LHS) of the state of
+// the last assembly, v read at the periodic masters and w folded (bc3per):
+// the operator of SparseAp + bc3per before the two-sided scaling.  Uses
+// c->resid as scratch, so it runs after the assembly's block pass.
+int launch_jv(phg_ctx* c, const double* v, double* w);
 
 // ---- learned preconditioner (k_mpnn.hip) ----
 int mp_param_count();
@@ -299,8 +315,7 @@ int mp_load_pair(phg_ctx* c, int slot);
 // ---- launchers (k_solver.hip) ----
 int launch_i3lu(phg_ctx* c, double* diag, double* r, int code);
 int launch_spsi3pre(phg_ctx* c);
-// exact: the matvec of a true residual (row sums in double in every path)
-int launch_spmv(phg_ctx* c, const double* p, double* q, bool exact = false);
+int launch_spmv(phg_ctx* c, const double* p, double* q);
 int launch_sub(phg_ctx* c, double* r, const double* b, const double* q, long long n);
 // q = K^T x (x overwritten by its periodic gather); needs c->tslot
 int launch_spmv_t(phg_ctx* c, double* x, double* q);

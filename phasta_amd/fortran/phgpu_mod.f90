@@ -18,7 +18,7 @@ module phgpu
     real(c_double) :: etol, epsM
     integer(c_int) :: Kspace, nGMRES, minIters
     integer(c_int) :: lhs, iprec
-    integer(c_int) :: ipred, idiff
+    integer(c_int) :: ipred, idiff, iter
   end type
 
   type, bind(C) :: phg_mesh
@@ -37,6 +37,7 @@ module phgpu
     real(c_double) :: ms_spmv, ms_mgs
     integer(c_int) :: n_spmv, n_mgs
     integer(c_int) :: refine_cycles, pc_fallback
+    real(c_double) :: orth_loss
   end type
 
   interface

@@ -24,6 +24,9 @@ def _lib():
                                        ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
         lib.phm_box_mesh.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_ulonglong, _dp, _ip, _ip, _ip, _ip]
+        lib.phm_box_mesh_layer.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_ulonglong, _dp, _ip, _ip,
+                                           _ip, _ip]
         lib.phm_csr_count.argtypes = [_i64, _i64, _ip, _ip, ctypes.POINTER(_i64)]
         lib.phm_csr_fill.argtypes = [_i64, _i64, _ip, _ip, _ip]
         lib.phm_box_state.argtypes = [_i64, _dp, ctypes.c_double, ctypes.c_ulonglong,
@@ -66,7 +69,10 @@ class BoxCase:
 
 def box_case(N, mach=0.3, jitter=0.1, grade=1.0, kspace=20, etol=1e-3,
              min_iters=0, mesh_seed=config.MESH_SEED,
-             field_seed=config.FIELD_SEED, ndofBC=10, cfl=config.CFL):
+             field_seed=config.FIELD_SEED, ndofBC=10, cfl=config.CFL, first=0.0):
+    """first > 0: capped boundary-layer grading (first wall cell first/N,
+    growing by `grade` until the uniform core); 0: geometric grading over
+    each half channel."""
     lib = _lib()
     nshg, nel, nelb = _i64(), _i64(), _i64()
     if lib.phm_box_counts(N, ctypes.byref(nshg), ctypes.byref(nel), ctypes.byref(nelb)):
@@ -77,7 +83,7 @@ def box_case(N, mach=0.3, jitter=0.1, grade=1.0, kspace=20, etol=1e-3,
     ienb = np.zeros((nelb, 4), dtype=np.int32, order="F")
     iBC = np.zeros(nshg, dtype=np.int32)
     iper = np.zeros(nshg, dtype=np.int32)
-    rc = lib.phm_box_mesh(N, jitter, grade, mesh_seed, x, ien, ienb, iBC, iper)
+    rc = lib.phm_box_mesh_layer(N, jitter, grade, first, mesh_seed, x, ien, ienb, iBC, iper)
     if rc:
         raise RuntimeError(f"phm_box_mesh failed ({rc})")
     colm = np.zeros(nshg + 1, dtype=np.int32)
@@ -91,7 +97,7 @@ def box_case(N, mach=0.3, jitter=0.1, grade=1.0, kspace=20, etol=1e-3,
     lib.phm_box_state(nshg, x, mach, field_seed, config.RGAS, config.GAMMA, yold)
     BC = np.zeros((nshg, ndofBC), order="F")
     params = config.solver_params(N, mach, kspace, etol=etol, min_iters=min_iters,
-                                  grade=grade, cfl=cfl)
+                                  grade=grade, cfl=cfl, first=first)
     return BoxCase(N=N, mach=mach, x=x, ien=ien, ienb=ienb, iBC=iBC, BC=BC,
                    iper=iper, colm=colm, rowp=rowp, yold=yold, params=params)
 
@@ -109,7 +115,8 @@ def named_case(name, **over):
     c = dict(config.CASES[name])
     c.update(over)
     return box_case(c["N"], mach=c["mach"], jitter=c["jitter"], grade=c["grade"],
-                    kspace=c["kspace"], cfl=c.get("cfl", config.CFL))
+                    kspace=c["kspace"], cfl=c.get("cfl", config.CFL),
+                    first=c.get("first", 0.0))
 
 
 def expected_nnz(N):

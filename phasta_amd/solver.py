@@ -39,7 +39,7 @@ class PhgParams(ctypes.Structure):
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
                [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred",
-                                               "idiff")]
+                                               "idiff", "iter")]
 
 
 class PhgMesh(ctypes.Structure):
@@ -58,7 +58,8 @@ class PhgStats(ctypes.Structure):
                 ("ms_solve", ctypes.c_double), ("ms_total", ctypes.c_double),
                 ("ms_spmv", ctypes.c_double), ("ms_mgs", ctypes.c_double),
                 ("n_spmv", ctypes.c_int), ("n_mgs", ctypes.c_int),
-                ("refine_cycles", ctypes.c_int), ("pc_fallback", ctypes.c_int)]
+                ("refine_cycles", ctypes.c_int), ("pc_fallback", ctypes.c_int),
+                ("orth_loss", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -72,8 +73,10 @@ EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_
            "phg_elmgmrs", "phg_solgmrs", "phg_state_upload", "phg_state_download",
            "phg_state_download_old", "phg_state_upload_cur", "phg_predict",
            "phg_newton_iteration", "phg_update", "phg_asiq", "phg_sparse_ap", "phg_sparse_ap_t",
+           "phg_tangent_ap",
            "phg_i3lu", "phg_spsi3pre", "phg_bc3per",
            "phg_bench_spmv", "phg_bench_assembly", "phg_bench_copy", "phg_sync",
+           "phg_diagnostics",
            "phg_mpnn_param_count", "phg_mpnn_init", "phg_mpnn_set_weights",
            "phg_mpnn_get_weights", "phg_mpnn_enable", "phg_mpnn_apply",
            "phg_mpnn_train_step", "phg_mpnn_last_pair", "phg_mpnn_share",
@@ -173,7 +176,12 @@ class GpuSolver:
         _check(lib().phg_elmgmrs(self.h, _p(y), _p(ac), _p(res), _p(rmes), _p(bd), _p(lk)))
         return res, rmes, bd, lk
 
-    def solgmrs(self, y, ac, want_lhs=False):
+    def solgmrs(self, y, ac, want_lhs=False, iter=None):
+        """SolGMRs on host arrays.  iter: the /timdat/ Newton iteration of
+        the time step (the divergence abort applies from iter 2 on); None
+        keeps the context's value."""
+        if iter is not None:
+            self.set_params(iter=int(iter))
         n, nnz = self.case.nshg, self.case.nnz_tot
         y, ac = _f64(y, (n, 5)), _f64(ac, (n, 5))
         res = np.zeros((n, 5), order="F")
@@ -200,6 +208,14 @@ class GpuSolver:
         p = _f64(p, (self.case.nshg, 5)).copy(order="F")
         lk = None if lhsK is None else _f64(lhsK, (25, self.case.nnz_tot))
         _check(lib().phg_sparse_ap(self.h, _p(lk), _p(p)))
+        return p
+
+    def tangent_ap(self, y, p):
+        """bc3per(A p(iper)) with the FP64 tangent A of state y, formed
+        element by element (no stored matrix)."""
+        n = self.case.nshg
+        p = _f64(p, (n, 5)).copy(order="F")
+        _check(lib().phg_tangent_ap(self.h, _p(_f64(y, (n, 5))), _p(p)))
         return p
 
     def sparse_ap_t(self, lhsK, p):
@@ -266,6 +282,11 @@ class GpuSolver:
 
     def sync(self):
         _check(lib().phg_sync(self.h))
+
+    def diagnostics(self, orth=False):
+        """orth: measure each GMRES cycle's basis orthogonality into
+        the stats' orth_loss."""
+        _check(lib().phg_diagnostics(self.h, 1 if orth else 0))
 
     def bench_spmv(self, reps=20):
         ms, by = ctypes.c_double(), ctypes.c_double()

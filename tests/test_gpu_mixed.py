@@ -1,13 +1,17 @@
 """Mixed-precision path (float tangent arithmetic and float matrix storage;
-double residual, matvec row sums, Krylov basis, MGS, Givens, norms and
-refinement residual) against the FP64 oracle.
+double residual, matvec row sums, Krylov basis, Gram-Schmidt, Givens, norms,
+and a refinement residual formed with the FP64 tangent) against the FP64
+oracle.
 
 The residual is computed in double in both paths; the tangent, BDiag and
 lhsK carry float rounding.  The bars below are the stated errors of the
 reduced-precision results against FP64.  The update Dy is checked in the
 way the solver's tolerance defines it: on the FP64 system, the
-preconditioned residual ||L^-1 (res - K Dy)|| must meet etol * ||L^-1 res||
-(up to the float rounding of K), as the FP64 GMRES update does.
+preconditioned residual ||L^-1 (res - K Dy)|| must meet etol * ||L^-1 res||,
+as the FP64 GMRES update does.  The mixed path tests its refinement
+residual with the FP64 tangent but with its own block-diagonal factors
+(BDiag summed from float blocks), so the bar allows their difference
+(about 1e-7 relative) and no more.
 """
 import numpy as np
 import pytest
@@ -20,7 +24,7 @@ pytestmark = pytest.mark.gpu
 # stated errors of the mixed path against FP64 (normwise relative)
 TOL_RES = 1e-13      # residual: double in both paths
 TOL_TANGENT = 2e-6   # float tangent blocks and BDiag
-SLACK_SOLVE = 1e-3   # tolerance slack of the preconditioned residual
+SLACK_SOLVE = 1e-5   # preconditioned norm with the mixed path's own BDiag factors
 
 
 def rel(a, b):

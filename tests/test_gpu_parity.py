@@ -321,34 +321,46 @@ def test_gpu_against_reference_golden(N):
         gpu.close()
 
 
-def test_box64_newton_iteration_against_oracle():
-    """The 64^3 workload (1.57 M tets, Mach 0.5, GMRES(20)): one FP64 Newton
-    iteration on the GPU against the oracle's, to the same 1e-12 bar and
-    Krylov count; the mixed-precision update against the FP64 one."""
+def test_box64_newton_iterations_against_oracle():
+    """The 64^3 workload (1.57 M tets, Mach 0.5, GMRES(20)) for its three
+    Newton iterations: each FP64 update on the GPU against the oracle's,
+    to the same 1e-12 bar and Krylov counts, then the corrected state; the
+    first mixed-precision update against the FP64 one."""
     case = mesh.named_case("box64")
-    print(f"[parity] box64: nshg={case.nshg} nel={case.nel}", flush=True)
+    newton = config.CASES["box64"]["newton"]
+    print(f"[parity] box64: nshg={case.nshg} nel={case.nel}, {newton} Newton iterations",
+          flush=True)
     orc = pyoracle.Oracle(case)
     acold = np.zeros((case.nshg, 5), order="F")
-    y, ac = orc.itr_predict_bc(case.yold, acold)
-    ref = orc.solgmrs(y, ac)
-    out = {}
-    for prec in (solver.PHG_FP64, solver.PHG_MIXED):
-        gpu = solver.GpuSolver(case, precision=prec)
-        try:
-            gpu.upload_state(case.yold, acold)
-            gpu.predict()
+    y_ref, _, hist = orc.newton_step(case.yold, acold, newton=newton)
+    gpu = solver.GpuSolver(case)
+    try:
+        gpu.upload_state(case.yold, acold)
+        gpu.predict()
+        for k in range(newton):
             st = gpu.newton_iteration()
-            _, _, dy = gpu.download_state()
-        finally:
-            gpu.close()
-        out[prec] = (st, dy)
-    st64, dy64 = out[solver.PHG_FP64]
-    stmx, dymx = out[solver.PHG_MIXED]
-    e64, emx = rel(dy64, ref["Dy"]), rel(dymx, dy64)
-    print(f"[parity] box64 FP64 Dy rel err {e64:.2e} (iKs {st64['iKs']} vs {ref['iKs']}); "
-          f"mixed vs FP64 {emx:.2e} (iKs {stmx['iKs']})", flush=True)
-    assert st64["iKs"] == ref["iKs"]
-    assert e64 < TOL
+            y, _, dy = gpu.download_state()
+            e = rel(dy, hist[k]["Dy"])
+            print(f"[parity] box64 iteration {k + 1}: Dy rel err {e:.2e} "
+                  f"(iKs {st['iKs']} vs {hist[k]['iKs']})", flush=True)
+            assert st["iKs"] == hist[k]["iKs"]
+            assert st["converged"] == hist[k]["converged"]
+            assert e < TOL
+            if k == 0:
+                dy64 = dy
+        assert rel(y, y_ref) < TOL
+    finally:
+        gpu.close()
+    gpu = solver.GpuSolver(case, precision=solver.PHG_MIXED)
+    try:
+        gpu.upload_state(case.yold, acold)
+        gpu.predict()
+        stmx = gpu.newton_iteration()
+        _, _, dymx = gpu.download_state()
+    finally:
+        gpu.close()
+    emx = rel(dymx, dy64)
+    print(f"[parity] box64 mixed vs FP64 first update {emx:.2e} (iKs {stmx['iKs']})", flush=True)
     assert stmx["converged"] and emx < 1e-5
 
 

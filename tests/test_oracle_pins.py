@@ -416,12 +416,43 @@ def test_rstat_norms_and_abort_rules(box4):
     n = box4.nshg
     assert out["totres"][0] == pytest.approx(np.linalg.norm(out["res"]) / n, rel=1e-14)
     assert out["totres"][1] == pytest.approx(np.linalg.norm(out["rmes"]) / n, rel=1e-14)
-    # a residual 1e4 times above the last accepted one aborts (code 11)
+    # from the second Newton iteration of a time step on, a residual 1e4
+    # times above the last one aborts (code 11); the first is not tested
     o.reslast = 1e-4 * np.sqrt(np.sum(out["res"] ** 2) / n) * 0.999
     with pytest.raises(RuntimeError, match="rc=11"):
-        o.solgmrs(y, ac)
+        o.solgmrs(y, ac, iter=2)
+    o.solgmrs(y, ac, iter=1)
+    o.reslast = 1e-4 * np.sqrt(np.sum(out["res"] ** 2) / n) * 1.001
+    o.solgmrs(y, ac, iter=2)  # just below the bound
     o.reslast = 0.0
     bad = y.copy(order="F")
     bad[3, 2] = np.nan
     with pytest.raises(RuntimeError, match="rc=10"):
         o.solgmrs(bad, ac)
+
+
+def test_divergence_check_skips_the_first_iteration_of_a_time_step(box4):
+    """rstatCheck.f:73 tests iter > 1 and itrdrv.f:409 resets iter at each
+    time step: a converged step (here a resting uniform state, residual at
+    rounding level) followed by a new step whose first residual is far more
+    than 1e4 times larger does not abort; the same residual as the step's
+    second Newton iteration does."""
+    o = pyoracle.Oracle(box4)
+    rest = np.zeros((box4.nshg, 5), order="F")
+    rest[:, 3], rest[:, 4] = 1.0e5, 300.0
+    acold = np.zeros((box4.nshg, 5), order="F")
+    y0, ac0 = o.itr_predict_bc(rest)
+    o.solgmrs(y0, ac0, iter=1)
+    quiet = o.reslast
+    y, ac = o.itr_predict_bc(box4.yold, acold)
+    out = o.solgmrs(y, ac, iter=1)
+    assert o.reslast > 1e4 * quiet
+    assert out["totres"][0] > 0.0
+    o.reslast = quiet
+    with pytest.raises(RuntimeError, match="rc=11"):
+        o.solgmrs(y, ac, iter=2)
+    # the time-step driver numbers the Newton iterations from 1 per step
+    o2 = pyoracle.Oracle(box4)
+    o2.reslast = quiet
+    yo, aco, hist = o2.time_steps(box4.yold, acold, steps=1, newton=2)
+    assert len(hist) == 2
