@@ -1,0 +1,545 @@
+/*
+ * phoracle_e3.c -- element routines of the CPU oracle (test infrastructure,
+ * not the product): quadrature, e3metric, e3gijd and the interior element
+ * residual and tangent e3.  Included by phoracle.c; kept in a file of its own
+ * so a flop-counting build can compile it with an instrumented number type.
+ */
+
+#define PT125 0.1250000000000000000000000000000
+#define PT5 0.5000000000000000000000000000000
+#define PT66 0.6666666666666666666666666666667
+#define ONE 1.0
+#define TWO 2.0
+
+
+/* quadrature: symtet.c:26-27,76-78 (4-point rule), genint.f:74 (4/3),
+ * symtri.c:22-24,55-56 (3-point rule), genint.f:75 (x2), shape functions
+ * uniformP.c:16-33, gradients halved genshp.f:34-35 / genshpb.f:33-34 */
+void orc_quadrature(double shp[4][4], double shgl[4][3][4], double Qwt[4],
+                    double shpb[3][4], double shglb[3][3][4], double Qwtb[3]) {
+  const double a4 = 0.5854101966249685, b4 = 0.1381966011250150;
+  const double rstw4[4][4] = {{a4, b4, b4, b4}, {b4, a4, b4, b4},
+                              {b4, b4, a4, b4}, {b4, b4, b4, a4}};
+  const double A = 0.3333333333333330000000000000000000;
+  const double B = 0.666666666666667, C = 0.166666666666667;
+  const double rst3[3][3] = {{B, C, C}, {C, B, C}, {C, C, B}};
+  double dN[4][3];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 3; ++j) dN[i][j] = (i == 3) ? -1.0e0 : (i == j ? 1.0e0 : 0.0e0);
+  for (int q = 0; q < 4; ++q) {
+    double L[4];
+    L[0] = rstw4[q][0];
+    L[1] = rstw4[q][1];
+    L[2] = rstw4[q][2];
+    L[3] = 1.0e0 - rstw4[q][0] - rstw4[q][1] - rstw4[q][2];
+    for (int a = 0; a < 4; ++a) {
+      if (shp) shp[q][a] = L[a];
+      for (int j = 0; j < 3; ++j)
+        if (shgl) shgl[q][j][a] = dN[a][j] / TWO;
+    }
+    if (Qwt) Qwt[q] = (4.0 / 3.0) * 0.25;
+  }
+  for (int q = 0; q < 3; ++q) {
+    double L[4];
+    L[0] = rst3[q][0];
+    L[1] = rst3[q][1];
+    L[2] = rst3[q][2];
+    L[3] = 1.0e0 - rst3[q][0] - rst3[q][1] - rst3[q][2];
+    for (int a = 0; a < 4; ++a) {
+      if (shpb) shpb[q][a] = L[a];
+      for (int j = 0; j < 3; ++j)
+        if (shglb) shglb[q][j][a] = dN[a][j] / TWO;
+    }
+    if (Qwtb) Qwtb[q] = TWO * A;
+  }
+}
+
+/* e3metric.f:22-77 */
+static void e3metric(const double xl[4][3], const double sg[3][4], double Qw,
+                     double dxidx[3][3], double shg[4][3], double* WdetJ) {
+  double d[3][3] = {{0}};
+  for (int n = 0; n < 4; ++n)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) d[i][j] = d[i][j] + xl[n][i] * sg[j][n];
+  dxidx[0][0] = d[1][1] * d[2][2] - d[2][1] * d[1][2];
+  dxidx[0][1] = d[2][1] * d[0][2] - d[0][1] * d[2][2];
+  dxidx[0][2] = d[0][1] * d[1][2] - d[0][2] * d[1][1];
+  double tmp = ONE / (dxidx[0][0] * d[0][0] + dxidx[0][1] * d[1][0] +
+                      dxidx[0][2] * d[2][0]);
+  dxidx[0][0] = dxidx[0][0] * tmp;
+  dxidx[0][1] = dxidx[0][1] * tmp;
+  dxidx[0][2] = dxidx[0][2] * tmp;
+  dxidx[1][0] = (d[1][2] * d[2][0] - d[1][0] * d[2][2]) * tmp;
+  dxidx[1][1] = (d[0][0] * d[2][2] - d[2][0] * d[0][2]) * tmp;
+  dxidx[1][2] = (d[1][0] * d[0][2] - d[0][0] * d[1][2]) * tmp;
+  dxidx[2][0] = (d[1][0] * d[2][1] - d[1][1] * d[2][0]) * tmp;
+  dxidx[2][1] = (d[2][0] * d[0][1] - d[0][0] * d[2][1]) * tmp;
+  dxidx[2][2] = (d[0][0] * d[1][1] - d[0][1] * d[1][0]) * tmp;
+  *WdetJ = Qw / tmp;
+  for (int n = 0; n < 4; ++n)
+    for (int k = 0; k < 3; ++k)
+      shg[n][k] = sg[0][n] * dxidx[0][k] + sg[1][n] * dxidx[1][k] + sg[2][n] * dxidx[2][k];
+}
+
+/* e3tau.f:1397-1507, tet branch (lcsyst == 1) */
+static void e3gijd(const double dx[3][3], double g[6]) {
+  const double c1 = 1.259921049894873e+00, c2 = 6.299605249474365e-01;
+  double t1, t2, t3;
+  t1 = c1 * dx[0][0] + c2 * (dx[1][0] + dx[2][0]);
+  t2 = c1 * dx[1][0] + c2 * (dx[0][0] + dx[2][0]);
+  t3 = c1 * dx[2][0] + c2 * (dx[0][0] + dx[1][0]);
+  g[0] = dx[0][0] * t1 + dx[1][0] * t2 + dx[2][0] * t3;
+  t1 = c1 * dx[0][1] + c2 * (dx[1][1] + dx[2][1]);
+  t2 = c1 * dx[1][1] + c2 * (dx[0][1] + dx[2][1]);
+  t3 = c1 * dx[2][1] + c2 * (dx[0][1] + dx[1][1]);
+  g[1] = dx[0][0] * t1 + dx[1][0] * t2 + dx[2][0] * t3;
+  g[2] = dx[0][1] * t1 + dx[1][1] * t2 + dx[2][1] * t3;
+  t1 = c1 * dx[0][2] + c2 * (dx[1][2] + dx[2][2]);
+  t2 = c1 * dx[1][2] + c2 * (dx[0][2] + dx[2][2]);
+  t3 = c1 * dx[2][2] + c2 * (dx[0][2] + dx[1][2]);
+  g[3] = dx[0][0] * t1 + dx[1][0] * t2 + dx[2][0] * t3;
+  g[4] = dx[0][1] * t1 + dx[1][1] * t2 + dx[2][1] * t3;
+  g[5] = dx[0][2] * t1 + dx[1][2] * t2 + dx[2][2] * t3;
+}
+
+typedef struct {
+  double shp[4][4], shgl[4][3][4], Qwt[4];
+  double shpb[3][4], shglb[3][3][4], Qwtb[3];
+} quad_t;
+
+static const quad_t* quad(void) {
+  static quad_t q;
+  static int init = 0;
+  if (!init) {
+    orc_quadrature(q.shp, q.shgl, q.Qwt, q.shpb, q.shglb, q.Qwtb);
+    init = 1;
+  }
+  return &q;
+}
+
+/* dot of row r of M with vector v, columns in ascending order */
+static inline double rowdot(const double M[5][5], int r, const double v[5]) {
+  return M[r][0] * v[0] + M[r][1] * v[1] + M[r][2] * v[2] + M[r][3] * v[3] +
+         M[r][4] * v[4];
+}
+
+void orc_e3_q(const orc_params* P, const double yl[4][5],
+              const double acl[4][5], const double xl[4][3],
+              const double ql[4][12], double rl[4][5], double EG[20][20],
+              int want_lhs) {
+  const quad_t* Q = quad();
+  const int lhs = want_lhs;
+  memset(rl, 0, sizeof(double) * 20);
+  if (lhs) memset(EG, 0, sizeof(double) * 400);
+  const double Rgas = P->Rgas, gamma = P->gamma, gamma1 = P->gamma1;
+  for (int q = 0; q < 4; ++q) {
+    if (Q->Qwt[q] == 0.0) continue; /* e3.f:112 */
+    /* getshp: hierarchic.f:30-57 (ipord = 1) */
+    double shape[4], sg[3][4];
+    for (int a = 0; a < 4; ++a) {
+      shape[a] = Q->shp[q][a];
+      for (int j = 0; j < 3; ++j) sg[j][a] = Q->shgl[q][j][a];
+    }
+    double ri[20], rmi[20], stiff[15][15];
+    memset(ri, 0, sizeof ri);
+    memset(rmi, 0, sizeof rmi);
+    if (lhs) memset(stiff, 0, sizeof stiff);
+
+    /* ---- e3ivar.f:142-156,186-194 primitive variables, acceleration ---- */
+    double pres = 0, u1 = 0, u2 = 0, u3 = 0, T = 0, aci[5] = {0, 0, 0, 0, 0};
+    for (int n = 0; n < 4; ++n) {
+      pres = pres + shape[n] * yl[n][0];
+      u1 = u1 + shape[n] * yl[n][1];
+      u2 = u2 + shape[n] * yl[n][2];
+      u3 = u3 + shape[n] * yl[n][3];
+      T = T + shape[n] * yl[n][4];
+    }
+    for (int n = 0; n < 4; ++n)
+      for (int f = 0; f < 5; ++f) aci[f] = aci[f] + shape[n] * acl[n][f];
+    /* e3ivar.f:203 */
+    double rk = PT5 * (u1 * u1 + u2 * u2 + u3 * u3);
+    /* getthm.f:130-160, ithm = 7 */
+    double rho = pres / (Rgas * T);
+    double ei = T * (Rgas / gamma1);
+    double h = T * (Rgas * gamma / gamma1);
+    double cv = Rgas / gamma1;
+    double cp = Rgas * gamma / gamma1;
+    double alfap = ONE / T;
+    double betaT = ONE / pres;
+    /* getdiff.f:126-128,163-171,263 (constant mu, Stokes lambda, DNS) */
+    double rmu = P->mu;
+    double rlm = -PT66 * rmu;
+    double con = rmu * cp / P->pr;
+    double xmut = 0.0;
+    rlm = rlm - PT66 * xmut;
+    rmu = rmu + xmut;
+    double rlm2mu = rlm + TWO * rmu;
+    con = con + xmut * cp / P->pr;
+    /* e3metric */
+    double dxidx[3][3], shg[4][3], WdetJ;
+    e3metric(xl, sg, Q->Qwt[q], dxidx, shg, &WdetJ);
+    /* e3ivar.f:259-334 global gradients (nshl == 4 branch) */
+    double g1[5], g2[5], g3[5];
+    for (int f = 0; f < 5; ++f) {
+      g1[f] = 0.0;
+      g2[f] = 0.0;
+      g3[f] = 0.0;
+      g1[f] = g1[f] + shg[0][0] * yl[0][f] + shg[1][0] * yl[1][f] +
+              shg[2][0] * yl[2][f] + shg[3][0] * yl[3][f];
+      g2[f] = g2[f] + shg[0][1] * yl[0][f] + shg[1][1] * yl[1][f] +
+              shg[2][1] * yl[2][f] + shg[3][1] * yl[3][f];
+      g3[f] = g3[f] + shg[0][2] * yl[0][f] + shg[1][2] * yl[1][f] +
+              shg[2][2] * yl[2][f] + shg[3][2] * yl[3][f];
+    }
+
+    /* ---- e3mtrx.f:87-224 ---- */
+    double A0[5][5], A1[5][5], A2[5][5], A3[5][5];
+    memset(A0, 0, sizeof A0);
+    memset(A1, 0, sizeof A1);
+    memset(A2, 0, sizeof A2);
+    memset(A3, 0, sizeof A3);
+    double drdp = rho * betaT;
+    double drdT = -rho * alfap;
+    A0[4][0] = drdp * (h + rk) - alfap * T;
+    double e2p = A0[4][0] + ONE;
+    double e3p = rho * (h + rk);
+    double e4p = drdT * (h + rk) + rho * cp;
+    A0[0][0] = drdp;
+    A0[0][4] = drdT;
+    A0[1][0] = drdp * u1;
+    A0[1][1] = rho;
+    A0[1][4] = drdT * u1;
+    A0[2][0] = drdp * u2;
+    A0[2][2] = rho;
+    A0[2][4] = drdT * u2;
+    A0[3][0] = drdp * u3;
+    A0[3][3] = rho;
+    A0[3][4] = drdT * u3;
+    A0[4][1] = rho * u1;
+    A0[4][2] = rho * u2;
+    A0[4][3] = rho * u3;
+    A0[4][4] = e4p;
+
+    A1[0][0] = drdp * u1;
+    A1[0][1] = rho;
+    A1[0][4] = drdT * u1;
+    A1[1][0] = drdp * u1 * u1 + 1;
+    A1[1][1] = TWO * rho * u1;
+    A1[1][4] = drdT * u1 * u1;
+    A1[2][0] = drdp * u1 * u2;
+    A1[2][1] = rho * u2;
+    A1[2][2] = rho * u1;
+    A1[2][4] = drdT * u1 * u2;
+    A1[3][0] = drdp * u1 * u3;
+    A1[3][1] = rho * u3;
+    A1[3][3] = rho * u1;
+    A1[3][4] = drdT * u1 * u3;
+    A1[4][0] = u1 * e2p;
+    A1[4][1] = e3p + rho * u1 * u1;
+    A1[4][2] = rho * u1 * u2;
+    A1[4][3] = rho * u1 * u3;
+    A1[4][4] = u1 * e4p;
+
+    A2[0][0] = drdp * u2;
+    A2[0][2] = rho;
+    A2[0][4] = drdT * u2;
+    A2[1][0] = drdp * u1 * u2;
+    A2[1][1] = rho * u2;
+    A2[1][2] = rho * u1;
+    A2[1][4] = drdT * u1 * u2;
+    A2[2][0] = drdp * u2 * u2 + 1;
+    A2[2][2] = TWO * rho * u2;
+    A2[2][4] = drdT * u2 * u2;
+    A2[3][0] = drdp * u2 * u3;
+    A2[3][2] = rho * u3;
+    A2[3][3] = rho * u2;
+    A2[3][4] = drdT * u2 * u3;
+    A2[4][0] = u2 * e2p;
+    A2[4][1] = rho * u1 * u2;
+    A2[4][2] = e3p + rho * u2 * u2;
+    A2[4][3] = rho * u2 * u3;
+    A2[4][4] = u2 * e4p;
+
+    A3[0][0] = drdp * u3;
+    A3[0][3] = rho;
+    A3[0][4] = drdT * u3;
+    A3[1][0] = drdp * u1 * u3;
+    A3[1][1] = rho * u3;
+    A3[1][3] = rho * u1;
+    A3[1][4] = drdT * u1 * u3;
+    A3[2][0] = drdp * u3 * u2;
+    A3[2][2] = rho * u3;
+    A3[2][3] = rho * u2;
+    A3[2][4] = drdT * u3 * u2;
+    A3[3][0] = drdp * u3 * u3 + 1;
+    A3[3][3] = TWO * rho * u3;
+    A3[3][4] = drdT * u3 * u3;
+    A3[4][0] = u3 * e2p;
+    A3[4][1] = rho * u1 * u3;
+    A3[4][2] = rho * u2 * u3;
+    A3[4][3] = e3p + rho * u3 * u3;
+    A3[4][4] = u3 * e4p;
+
+    /* ---- e3conv.f:72-96 Euler flux (integrated by parts) ---- */
+    ri[0] = (-u1) * rho;
+    ri[1] = (-u1) * rho * u1 - pres;
+    ri[2] = (-u1) * rho * u2;
+    ri[3] = (-u1) * rho * u3;
+    ri[4] = (-u1) * rho * (ei + rk) - u1 * pres;
+    ri[5] = (-u2) * rho;
+    ri[6] = (-u2) * rho * u1;
+    ri[7] = (-u2) * rho * u2 - pres;
+    ri[8] = (-u2) * rho * u3;
+    ri[9] = (-u2) * rho * (ei + rk) - u2 * pres;
+    ri[10] = (-u3) * rho;
+    ri[11] = (-u3) * rho * u1;
+    ri[12] = (-u3) * rho * u2;
+    ri[13] = (-u3) * rho * u3 - pres;
+    ri[14] = (-u3) * rho * (ei + rk) - u3 * pres;
+    /* e3conv.f:100-179 rLyi = A_i Y,i */
+    double rLyi[5];
+    for (int i = 0; i < 5; ++i)
+      rLyi[i] = A1[i][0] * g1[0] + A1[i][1] * g1[1] + A1[i][2] * g1[2] +
+                A1[i][3] * g1[3] + A1[i][4] * g1[4] + A2[i][0] * g2[0] +
+                A2[i][1] * g2[1] + A2[i][2] * g2[2] + A2[i][3] * g2[3] +
+                A2[i][4] * g2[4] + A3[i][0] * g3[0] + A3[i][1] * g3[1] +
+                A3[i][2] * g3[2] + A3[i][3] * g3[3] + A3[i][4] * g3[4];
+    /* e3conv.f:188-345 LHS: EGmass(a,b) += N_a (WdetJ N_b,i A_i) */
+    if (lhs) {
+      for (int j = 0; j < 4; ++j) {
+        int j0 = 5 * j;
+        double f1 = WdetJ * shg[j][0], f2 = WdetJ * shg[j][1], f3 = WdetJ * shg[j][2];
+        double AiNbi[5][5];
+        for (int r = 0; r < 5; ++r)
+          for (int c = 0; c < 5; ++c)
+            AiNbi[r][c] = f1 * A1[r][c] + f2 * A2[r][c] + f3 * A3[r][c];
+        for (int i = 0; i < 4; ++i) {
+          int i0 = 5 * i;
+          for (int jd = 0; jd < 5; ++jd)
+            for (int r = 0; r < 5; ++r)
+              EG[i0 + r][j0 + jd] = EG[i0 + r][j0 + jd] + shape[i] * AiNbi[r][jd];
+        }
+      }
+    }
+
+    /* ---- e3visc.f:63-141 diffusivity K_ij (stiff) ---- */
+    if (lhs) {
+      stiff[1][1] = rlm2mu;
+      stiff[2][2] = rmu;
+      stiff[3][3] = rmu;
+      stiff[4][1] = rlm2mu * u1;
+      stiff[4][2] = rmu * u2;
+      stiff[4][3] = rmu * u3;
+      stiff[4][4] = con;
+      stiff[1][7] = rlm;
+      stiff[2][6] = rmu;
+      stiff[4][6] = rmu * u2;
+      stiff[4][7] = rlm * u1;
+      stiff[1][13] = rlm;
+      stiff[3][11] = rmu;
+      stiff[4][11] = rmu * u3;
+      stiff[4][13] = rlm * u1;
+      stiff[6][2] = rmu;
+      stiff[7][1] = rlm;
+      stiff[9][1] = rlm * u2;
+      stiff[9][2] = rmu * u1;
+      stiff[6][6] = rmu;
+      stiff[7][7] = rlm2mu;
+      stiff[8][8] = rmu;
+      stiff[9][6] = rmu * u1;
+      stiff[9][7] = rlm2mu * u2;
+      stiff[9][8] = rmu * u3;
+      stiff[9][9] = con;
+      stiff[7][13] = rlm;
+      stiff[8][12] = rmu;
+      stiff[9][12] = rmu * u3;
+      stiff[9][13] = rlm * u2;
+      stiff[11][3] = rmu;
+      stiff[13][1] = rlm;
+      stiff[14][1] = rlm * u3;
+      stiff[14][3] = rmu * u1;
+      stiff[12][8] = rmu;
+      stiff[13][7] = rlm;
+      stiff[14][7] = rlm * u3;
+      stiff[14][8] = rmu * u2;
+      stiff[11][11] = rmu;
+      stiff[12][12] = rmu;
+      stiff[13][13] = rlm2mu;
+      stiff[14][11] = rmu * u1;
+      stiff[14][12] = rmu * u2;
+      stiff[14][13] = rlm2mu * u3;
+      stiff[14][14] = con;
+    }
+    /* e3visc.f:278-343 viscous and heat fluxes (rlsli = 0) */
+    const double rls = 0.0;
+    rmi[1] = rlm2mu * g1[1] + rlm * g2[2] + rlm * g3[3] - rls;
+    rmi[2] = rmu * g1[2] + rmu * g2[1] - rls;
+    rmi[3] = rmu * g1[3] + rmu * g3[1] - rls;
+    rmi[4] = rlm2mu * u1 * g1[1] + rmu * u2 * g1[2] + rmu * u3 * g1[3] +
+             rmu * u2 * g2[1] + rlm * u1 * g2[2] + rmu * u3 * g3[1] +
+             rlm * u1 * g3[3] + con * g1[4];
+    for (int k = 1; k <= 4; ++k) ri[k] = ri[k] + rmi[k];
+    rmi[6] = rmu * g1[2] + rmu * g2[1] - rls;
+    rmi[7] = rlm * g1[1] + rlm2mu * g2[2] + rlm * g3[3] - rls;
+    rmi[8] = rmu * g2[3] + rmu * g3[2] - rls;
+    rmi[9] = rlm * u2 * g1[1] + rmu * u1 * g1[2] + rmu * u1 * g2[1] +
+             rlm2mu * u2 * g2[2] + rmu * u3 * g2[3] + rmu * u3 * g3[2] +
+             rlm * u2 * g3[3] + con * g2[4];
+    for (int k = 6; k <= 9; ++k) ri[k] = ri[k] + rmi[k];
+    rmi[11] = rmu * g1[3] + rmu * g3[1] - rls;
+    rmi[12] = rmu * g2[3] + rmu * g3[2] - rls;
+    rmi[13] = rlm * g1[1] + rlm * g2[2] + rlm2mu * g3[3] - rls;
+    rmi[14] = rlm * u3 * g1[1] + rmu * u1 * g1[3] + rlm * u3 * g2[2] +
+              rmu * u2 * g2[3] + rmu * u1 * g3[1] + rmu * u2 * g3[2] +
+              rlm2mu * u3 * g3[3] + con * g3[4];
+    for (int k = 11; k <= 14; ++k) ri[k] = ri[k] + rmi[k];
+
+    /* ---- e3ls.f:105-143 add the time term to the LS residual ---- */
+    for (int i = 0; i < 5; ++i)
+      rLyi[i] = rLyi[i] + A0[i][0] * aci[0] + A0[i][1] * aci[1] +
+                A0[i][2] * aci[2] + A0[i][3] * aci[3] + A0[i][4] * aci[4];
+    /* ---- e3ivar.f:361-395 divergence of the reconstructed diffusive flux,
+     * e3ls.f:148-155 subtracted from the LS residual (isurf = 0) ---- */
+    if (ql) {
+      double divqi[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int n = 0; n < 4; ++n)
+        for (int k = 0; k < 4; ++k)
+          divqi[k] = divqi[k] + shg[n][0] * ql[n][k] + shg[n][1] * ql[n][4 + k] +
+                     shg[n][2] * ql[n][8 + k];
+      rLyi[1] = rLyi[1] - divqi[0];
+      rLyi[2] = rLyi[2] - divqi[1];
+      rLyi[3] = rLyi[3] - divqi[2];
+      rLyi[4] = rLyi[4] - divqi[3];
+    }
+    /* ---- e3tau.f:1-283 (itau = 0, ipord = 1) ---- */
+    double gijd[6];
+    e3gijd(dxidx, gijd);
+    const double fff = 36.0;
+    double dts = P->dtsfct * P->Dtgl;
+    double tau[3];
+    double t2 = rho * rho *
+                    ((TWO * dts) * (TWO * dts) +
+                     (u1 * (u1 * gijd[0] + TWO * (u2 * gijd[1] + u3 * gijd[3])) +
+                      u2 * (u2 * gijd[2] + TWO * u3 * gijd[4]) + u3 * u3 * gijd[5])) +
+                fff * (rmu * rmu) *
+                    (gijd[0] * gijd[0] + gijd[2] * gijd[2] + gijd[5] * gijd[5] +
+                     TWO * (gijd[1] * gijd[1] + gijd[3] * gijd[3] + gijd[4] * gijd[4]));
+    double fct = sqrt(t2);
+    tau[0] = PT125 * fct / (rho * (gijd[0] + gijd[2] + gijd[5])) * P->taucfct;
+    tau[1] = ONE / fct;
+    tau[2] = tau[1] / cv * P->temper;
+    rLyi[0] = tau[0] * rLyi[0];
+    rLyi[1] = tau[1] * rLyi[1];
+    rLyi[2] = tau[1] * rLyi[2];
+    rLyi[3] = tau[1] * rLyi[3];
+    rLyi[4] = tau[2] * rLyi[4];
+    /* e3ls.f:351-458 RHS ri += A_i (tau L) */
+    for (int i = 0; i < 5; ++i) ri[i] = rowdot(A1, i, rLyi) + ri[i];
+    for (int i = 0; i < 5; ++i) ri[5 + i] = rowdot(A2, i, rLyi) + ri[5 + i];
+    for (int i = 0; i < 5; ++i) ri[10 + i] = rowdot(A3, i, rLyi) + ri[10 + i];
+    /* e3ls.f:464-766 LHS */
+    if (lhs) {
+      const double* Ai[3] = {&A1[0][0], &A2[0][0], &A3[0][0]};
+      double AitauA0[3][5][5];
+      for (int k = 0; k < 3; ++k) {
+        const double (*Ak)[5] = (const double (*)[5])Ai[k];
+        double Atau[5][5];
+        for (int i = 0; i < 5; ++i) {
+          Atau[i][0] = Ak[i][0] * tau[0];
+          Atau[i][1] = Ak[i][1] * tau[1];
+          Atau[i][2] = Ak[i][2] * tau[1];
+          Atau[i][3] = Ak[i][3] * tau[1];
+          Atau[i][4] = Ak[i][4] * tau[2];
+        }
+        for (int j = 0; j < 5; ++j)
+          for (int i = 0; i < 5; ++i)
+            AitauA0[k][i][j] = Atau[i][0] * A0[0][j] + Atau[i][1] * A0[1][j] +
+                               Atau[i][2] * A0[2][j] + Atau[i][3] * A0[3][j] +
+                               Atau[i][4] * A0[4][j];
+        for (int l = 0; l < 3; ++l) {
+          const double (*Al)[5] = (const double (*)[5])Ai[l];
+          for (int j = 0; j < 5; ++j)
+            for (int i = 0; i < 5; ++i)
+              stiff[i + 5 * k][j + 5 * l] =
+                  stiff[i + 5 * k][j + 5 * l] +
+                  (Atau[i][0] * Al[0][j] + Atau[i][1] * Al[1][j] + Atau[i][2] * Al[2][j] +
+                   Atau[i][3] * Al[3][j] + Atau[i][4] * Al[4][j]);
+        }
+      }
+      /* e3ls.f:713-760 least-squares time term */
+      for (int i = 0; i < 4; ++i) {
+        int i0 = 5 * i;
+        double At[5][5];
+        for (int id = 0; id < 5; ++id)
+          for (int jd = 0; jd < 5; ++jd)
+            At[id][jd] = shg[i][0] * AitauA0[0][id][jd] + shg[i][1] * AitauA0[1][id][jd] +
+                         shg[i][2] * AitauA0[2][id][jd];
+        for (int j = 0; j < 4; ++j) {
+          int j0 = 5 * j;
+          double fa = shape[j] * WdetJ * P->almi / P->gami / P->alfi * P->Dtgl;
+          for (int id = 0; id < 5; ++id)
+            for (int jd = 0; jd < 5; ++jd)
+              EG[i0 + id][j0 + jd] = EG[i0 + id][j0 + jd] + fa * At[id][jd];
+        }
+      }
+    }
+    /* ---- e3massr.f:34-72 ---- */
+    for (int i = 0; i < 5; ++i)
+      ri[15 + i] = ri[15 + i] + A0[i][0] * aci[0] + A0[i][1] * aci[1] +
+                   A0[i][2] * aci[2] + A0[i][3] * aci[3] + A0[i][4] * aci[4];
+    /* ---- e3massl.f:29-71 (bcool = 0) ---- */
+    if (lhs) {
+      double temp = WdetJ * P->almi / P->gami / P->alfi * P->Dtgl;
+      double bcool = WdetJ * 0.0;
+      const double sponge = 0.0;
+      for (int j = 0; j < 4; ++j) {
+        int j0 = 5 * j;
+        for (int i = 0; i < 4; ++i) {
+          int i0 = 5 * i;
+          double shpij = shape[i] * shape[j];
+          double fa = shpij * (temp + bcool * sponge);
+          for (int jd = 0; jd < 5; ++jd)
+            for (int f = 0; f < 5; ++f)
+              EG[i0 + f][j0 + jd] = EG[i0 + f][j0 + jd] + fa * A0[f][jd];
+        }
+      }
+    }
+    /* ---- e3wmlt.f:72-143 RHS ---- */
+    for (int i = 0; i < 4; ++i)
+      for (int f = 0; f < 5; ++f)
+        rl[i][f] = rl[i][f] + WdetJ * (shg[i][0] * ri[f] + shg[i][1] * ri[5 + f] +
+                                       shg[i][2] * ri[10 + f]);
+    for (int i = 0; i < 4; ++i)
+      for (int f = 0; f < 5; ++f)
+        rl[i][f] = rl[i][f] + shape[i] * WdetJ * ri[15 + f];
+    /* ---- e3wmlt.f:150-227 LHS stiffness ---- */
+    if (lhs) {
+      for (int j = 0; j < 4; ++j) {
+        int j0 = 5 * j;
+        double s1 = WdetJ * shg[j][0], s2 = WdetJ * shg[j][1], s3 = WdetJ * shg[j][2];
+        double st1[5][5], st2[5][5], st3[5][5];
+        for (int jd = 0; jd < 5; ++jd)
+          for (int id = 0; id < 5; ++id) {
+            st1[id][jd] = s1 * stiff[id][jd] + s2 * stiff[id][jd + 5] + s3 * stiff[id][jd + 10];
+            st2[id][jd] = s1 * stiff[id + 5][jd] + s2 * stiff[id + 5][jd + 5] +
+                          s3 * stiff[id + 5][jd + 10];
+            st3[id][jd] = s1 * stiff[id + 10][jd] + s2 * stiff[id + 10][jd + 5] +
+                          s3 * stiff[id + 10][jd + 10];
+          }
+        for (int i = 0; i < 4; ++i) {
+          int i0 = 5 * i;
+          for (int jd = 0; jd < 5; ++jd)
+            for (int r = 0; r < 5; ++r)
+              EG[i0 + r][j0 + jd] = EG[i0 + r][j0 + jd] + shg[i][0] * st1[r][jd] +
+                                    shg[i][1] * st2[r][jd] + shg[i][2] * st3[r][jd];
+        }
+      }
+    }
+  }
+}
+
+void orc_e3(const orc_params* P, const double yl[4][5],
+            const double acl[4][5], const double xl[4][3], double rl[4][5],
+            double EG[20][20], int want_lhs) {
+  orc_e3_q(P, yl, acl, xl, NULL, rl, EG, want_lhs);
+}
