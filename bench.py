@@ -31,6 +31,11 @@ sys.path.insert(0, ROOT)
 METRIC = ("time/Newton iter to 1e-3 tol: FP32+learned PC vs plain FP64; "
           "elements/s")
 HBM_PEAK_GBS = 8000.0
+# MI355X dense peaks (TFLOP/s): fp32 157.3 (packed / matrix), fp64 78.6
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
+# e3 at four points, residual + tangent, one element: FP operations with no
+# structurally zero operand (oracle/count_flops.cpp, tests/test_flop_count.py)
+E3_FLOPS = 38268
 WEIGHTS = os.path.join(ROOT, "phasta_amd", "weights", "mpnn_box64.npz")
 
 
@@ -53,13 +58,26 @@ def measured_traffic(kernel, case):
     return None
 
 
-def cpu_baseline(name):
-    """One oracle Newton iteration (1 core) of a case (bounded sample)."""
+def host_cpu():
+    """CPU model and the cores this process may run on."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_newton(case, threads):
+    """One oracle Newton iteration (ElmGMRs + SolGMRs + itrCorrect/itrBC)
+    of `case` on `threads` OpenMP threads; returns seconds and the update."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
-    from phasta_amd import mesh
-    case = mesh.named_case(name)
+    used = pyoracle.set_threads(threads)
     orc = pyoracle.Oracle(case)
     acold = np.zeros((case.nshg, 5), order="F")
     y, ac = orc.itr_predict_bc(case.yold, acold)
@@ -67,7 +85,7 @@ def cpu_baseline(name):
     out = orc.solgmrs(y, ac)
     orc.itr_correct_bc(y, ac, case.yold, acold, out["Dy"])
     dt = time.perf_counter() - t0
-    return dt, case, out
+    return dt, used, out
 
 
 def run_arm(case, precision, newton, steps, warmup, weights=None, idiff=0):
@@ -127,6 +145,8 @@ def breakdown(arm, steps, nel):
             "ms_assembly_per_step": round(ms_asm, 3),
             "ms_spmv_per_step": round(sum(s["ms_spmv"] for s in st) / steps, 3),
             "ms_mgs_per_step": round(sum(s["ms_mgs"] for s in st) / steps, 3),
+            "ms_point_pass": round(sum(s["ms_point"] for s in st) / steps, 3),
+            "ms_block_pass": round(sum(s["ms_blocks"] for s in st) / steps, 3),
             "krylov_iters": [s["iKs"] for s in st],
             "gmres_cycles": [s["refine_cycles"] for s in st],
             "pc_fallback": [s["pc_fallback"] for s in st],
@@ -172,8 +192,26 @@ def main():
     head = res[head_name]
     value = nel / (head["ms"] / 1e3)
 
-    # live matvec timing of the headline arm (HIP events on the library stream)
+    # live assembly timing of the headline arm (HIP events on the library
+    # stream around the point pass and the block pass): FLOP rate of the
+    # frozen e3 count against the chip's peak in the arm's tangent type
     st = head["stats"]
+    ms_pt = sum(s["ms_point"] for s in st) / len(st)
+    ms_bl = sum(s["ms_blocks"] for s in st) / len(st)
+    asm_tf = E3_FLOPS * nel / ((ms_pt + ms_bl) * 1e-3) / 1e12
+    tdt = "f64" if head_name == "fp64" else "f32"
+    kasm = (f"k_asm_qp<{'double' if tdt == 'f64' else 'float'}, false> + "
+            f"k_asm_rows<{'double, 1' if tdt == 'f64' else 'float, 4'}>")
+    roof_asm = {"kernel": f"{kasm} (ElmGMRs interior assembly: e3 residual + tangent, "
+                          f"res/BDiag/lhsK written once)",
+                "bound": "mfma", "compute": f"VALU {tdt} (no MFMA in this kernel; the peak "
+                "is the chip's dense fp32/fp64 figure, packed FP32 is off in this build)",
+                "achieved": round(asm_tf, 2), "peak": PEAK_TFLOPS[tdt], "unit": "TFLOP/s",
+                "frac": round(asm_tf / PEAK_TFLOPS[tdt], 4),
+                "traffic": measured_traffic(kasm, args.case),
+                "flop_per_element": E3_FLOPS, "elements_per_launch": nel,
+                "ms_point_pass": round(ms_pt, 4), "ms_block_pass": round(ms_bl, 4)}
+    # live matvec timing of the headline arm (HIP events on the library stream)
     ms_spmv = sum(s["ms_spmv"] for s in st)
     n_spmv = sum(s["n_spmv"] for s in st)
     s_a = 8 if head_name == "fp64" else 4
@@ -181,7 +219,7 @@ def main():
     spmv_avg = ms_spmv / max(1, n_spmv)
     spmv_gbs = by / (spmv_avg * 1e-3) / 1e9
     kname = "k_spmv<double, double>" if head_name == "fp64" else "k_spmv<float, double>"
-    roofline = {"kernel": f"{kname} (block-CSR matvec, SparseAp)", "bound": "hbm",
+    roof_mv = {"kernel": f"{kname} (block-CSR matvec, SparseAp)", "bound": "hbm",
                 "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(kname, args.case),
@@ -208,7 +246,8 @@ def main():
         "config": {"workload": desc, "N": spec["N"], "nel": nel, "nshg": case.nshg,
                    "nnz_blocks": case.nnz_tot, "newton_per_time_step": spec["newton"],
                    "arm": head_name, "pc": pc_label, "idiff": args.idiff},
-        "roofline": roofline,
+        "roofline": roof_asm,
+        "roofline_matvec": roof_mv,
         "arms": {a: breakdown(res[a], args.steps, nel) for a in res},
         "copy_GBs": round(head["by_copy"] / (head["ms_copy"] * 1e-3) / 1e9, 1),
     }
@@ -230,24 +269,36 @@ def main():
         if head_name != "fp64":
             line["speedup_vs_fp64"] = line["arms"][head_name]["speedup_vs_fp64"]
     if not args.no_cpu:
-        sample = "box64"
-        dt, ccase, orc_out = cpu_baseline(sample)
-        line["cpu_baseline"] = {"value": round(ccase.nel / dt, 1), "unit": "elements/s",
-                                "cores": 1, "kind": "port",
-                                "sample": f"one FP64 Newton iteration of the {sample} case "
-                                          f"({ccase.nel} tets, same per-element work) by the "
-                                          f"C oracle on 1 host core ({dt:.1f} s)"}
+        # the C oracle (FP64, the reference's operation order) on the host,
+        # on a bounded sample of the workload: one Newton iteration of the
+        # 64^3 box (the same per-element and per-row work, 1/8 of the
+        # elements), on all the cores this process may use and on one core
+        model, ncores = host_cpu()
+        threads = int(os.environ.get("OMP_NUM_THREADS", ncores))
+        small = mesh.named_case("box64")
+        dt_all, used, orc64 = cpu_newton(small, threads)
+        dt_one, _, _ = cpu_newton(small, 1)
+        line["cpu_baseline"] = {
+            "value": round(small.nel / dt_all, 1), "unit": "elements/s", "cores": used,
+            "kind": "port", "cpu": model, "cores_available": ncores,
+            "sample": f"one FP64 Newton iteration (ElmGMRs + SolGMRs + itrCorrect, "
+                      f"{orc64['iKs']} Krylov iterations) of the 64^3 box ({small.nel} tets, "
+                      f"the workload's per-element work on 1/8 of its elements) by the C "
+                      f"oracle on {used} OpenMP threads: {dt_all:.1f} s",
+            "one_core": {"value": round(small.nel / dt_one, 1), "unit": "elements/s",
+                         "seconds": round(dt_one, 2)},
+            "gpu_vs_all_cores": round(value / (small.nel / dt_all), 1)}
         # the same Newton iteration on the GPU (FP64) against the oracle's
-        gpu = solver.GpuSolver(ccase)
-        gpu.upload_state(ccase.yold, np.zeros((ccase.nshg, 5), order="F"))
+        gpu = solver.GpuSolver(small)
+        gpu.upload_state(small.yold, np.zeros((small.nshg, 5), order="F"))
         gpu.predict()
         st = gpu.newton_iteration()
         _, _, dy = gpu.download_state()
         gpu.close()
-        d_ref = orc_out["Dy"]
+        d_ref = orc64["Dy"]
         line["oracle_parity"] = {
-            "case": sample, "Dy_rel_err": float(np.linalg.norm(dy - d_ref) / np.linalg.norm(d_ref)),
-            "iKs": [st["iKs"], orc_out["iKs"]]}
+            "case": "box64", "Dy_rel_err": float(np.linalg.norm(dy - d_ref) / np.linalg.norm(d_ref)),
+            "iKs": [st["iKs"], orc64["iKs"]]}
     print(json.dumps(line), flush=True)
 
 

@@ -56,6 +56,9 @@ typedef struct phg_params {
                                            phg_newton_iteration adds 1 before
                                            its solve; phg_solgmrs takes the
                                            caller's value.                  */
+  int idc;                              /* /solpar/ iDC: 0 off, 1 DC-mallet,
+                                           2 DC-quadratic, 3 DC-minimum
+                                           (input_fform.cc:683-686, e3dc.f) */
 } phg_params;
 
 /* Mesh arrays as the Fortran host holds them (readnblk.f, genadj.f);
@@ -96,6 +99,8 @@ typedef struct phg_stats {
   double orth_loss;       /* max over the solve's cycles of
                              max_ij |u_i . u_j - delta_ij| of the Krylov basis
                              (phg_diagnostics flag PHG_DIAG_ORTH; else 0)   */
+  double ms_point;        /* HIP-event time of the assembly's point pass     */
+  double ms_blocks;       /* HIP-event time of its block (tangent) pass      */
 } phg_stats;
 
 typedef struct phg_ctx phg_ctx;

@@ -357,6 +357,7 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
   for (long long n = 0; n < nshg; ++n)
     if (!bc_supported(m->iBC[n])) return 3;
   if (P->idiff != 0 && P->idiff != 1 && P->idiff != 3) return 5;
+  if (P->idc < 0 || P->idc > 3) return 5; /* input_fform.cc:683-686 */
   /* elmgmr.f:361-391 global reconstruction of the diffusive flux */
   double* qres = NULL;
   if (P->idiff != 0) {
@@ -922,4 +923,14 @@ int orc_itr_bc(const orc_mesh* m, double* y, double* ac) {
       if (ac) ac[n + nshg * f] = ac[i + nshg * f];
     }
   return 0;
+}
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
 }

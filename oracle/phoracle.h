@@ -43,6 +43,7 @@ typedef struct orc_params {
   int idiff;                            /* diffusive flux reconstruction   */
   int iter;                             /* /timdat/ Newton iteration of the
                                            time step (itrdrv.f:409,434)    */
+  int idc;                              /* /solpar/ iDC 0-3 (e3dc.f)       */
 } orc_params;
 
 typedef struct orc_stats {
@@ -126,6 +127,9 @@ void orc_itr_correct(long long nshg, const orc_params* pr, double* y,
 void orc_itr_update(long long nshg, const orc_params* pr, double* yold,
                     double* acold, const double* y, const double* ac);
 int orc_itr_bc(const orc_mesh* m, double* y, double* ac);
+/* OpenMP threads of the parallel loops (1: the serial oracle); returns the
+ * count in effect, 1 when built without OpenMP. */
+int orc_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -279,6 +279,60 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
     A3[4][2] = rho * u2 * u3;
     A3[4][3] = e3p + rho * u3 * u3;
     A3[4][4] = u3 * e4p;
+    /* e3mtrx.f:232-298 (idc != 0): A0DC (the DC norm's metric), A0inv
+     * (symmetric storage, 15) and dV/dY (entropy variables, 15); cv is
+     * recomputed locally there from cp, alfap, T, rho, betaT */
+    double A0DC[4] = {0, 0, 0, 0}, A0inv[15] = {0}, dVdY[15] = {0};
+    if (P->idc != 0) {
+      double s1 = ONE / (rho * rho * betaT * T);
+      double cvl = cp - (alfap * alfap * T / rho / betaT);
+      A0DC[0] = (rho * betaT) * (rho * betaT) * s1;
+      A0DC[1] = -rho * alfap * rho * betaT * s1;
+      A0DC[2] = rho / T;
+      A0DC[3] = (-rho * alfap) * (-rho * alfap) * s1 + (rho * cvl / (T * T));
+      double fact1 = ONE / (rho * cvl * (T * T));
+      double d = alfap * T / rho / betaT;
+      double e1bar = h - rk;
+      double e2bar = e1bar - d;
+      double e3bar = e2bar - cvl * T;
+      double e5bar = e1bar * e1bar - 2.0 * e1bar * d + 2.0 * rk * cvl * T + cp * T / rho / betaT;
+      double c1bar = u1 * u1 + cvl * T;
+      double c2bar = u2 * u2 + cvl * T;
+      double c3bar = u3 * u3 + cvl * T;
+      double u12 = u1 * u2, u31 = u3 * u1, u23 = u2 * u3;
+      A0inv[0] = e5bar * fact1;
+      A0inv[1] = c1bar * fact1;
+      A0inv[2] = c2bar * fact1;
+      A0inv[3] = c3bar * fact1;
+      A0inv[4] = 1.0 * fact1;
+      A0inv[5] = u1 * e3bar * fact1;
+      A0inv[6] = u2 * e3bar * fact1;
+      A0inv[7] = u3 * e3bar * fact1;
+      A0inv[8] = -e2bar * fact1;
+      A0inv[9] = u12 * fact1;
+      A0inv[10] = u31 * fact1;
+      A0inv[11] = -u1 * fact1;
+      A0inv[12] = u23 * fact1;
+      A0inv[13] = -u2 * fact1;
+      A0inv[14] = -u3 * fact1;
+      double f1 = 1.0 / T;
+      double f2 = f1 / T;
+      dVdY[0] = f1 / rho;
+      dVdY[1] = -f1 * u1;
+      dVdY[2] = f1;
+      dVdY[3] = -f1 * u2;
+      dVdY[4] = 0.0;
+      dVdY[5] = f1;
+      dVdY[6] = -f1 * u3;
+      dVdY[7] = 0.0;
+      dVdY[8] = 0.0;
+      dVdY[9] = f1;
+      dVdY[10] = -(h - rk) * f2;
+      dVdY[11] = -f2 * u1;
+      dVdY[12] = -f2 * u2;
+      dVdY[13] = -f2 * u3;
+      dVdY[14] = f2;
+    }
 
     /* ---- e3conv.f:72-96 Euler flux (integrated by parts) ---- */
     ri[0] = (-u1) * rho;
@@ -428,11 +482,44 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
     tau[0] = PT125 * fct / (rho * (gijd[0] + gijd[2] + gijd[5])) * P->taucfct;
     tau[1] = ONE / fct;
     tau[2] = tau[1] / cv * P->temper;
+    double Lt[5]; /* e3tau.f:188: rLyitemp, the LS residual before tau */
+    for (int f = 0; f < 5; ++f) Lt[f] = rLyi[f];
     rLyi[0] = tau[0] * rLyi[0];
     rLyi[1] = tau[1] * rLyi[1];
     rLyi[2] = tau[1] * rLyi[2];
     rLyi[3] = tau[1] * rLyi[3];
     rLyi[4] = tau[2] * rLyi[4];
+    /* e3tau.f:198-268 (iDC != 0): rTLS = L.(dV/dY tau L), raLS = L.A0inv L
+     * and giju, the inverse of the metric tensor */
+    double rTLS = 0.0, raLS = 0.0, giju[6] = {0, 0, 0, 0, 0, 0};
+    if (P->idc != 0) {
+      const double* L = Lt;
+      const double* V = dVdY;
+      const double* r = rLyi;
+      rTLS = L[0] * (r[0] * V[0] + V[1] * r[1] + V[3] * r[2] + r[3] * V[6] + V[10] * r[4]) +
+             L[1] * (r[1] * V[2] + r[2] * V[4] + V[7] * r[3] + r[4] * V[11]) +
+             L[2] * (r[2] * V[5] + V[8] * r[3] + V[12] * r[4]) +
+             L[3] * (r[3] * V[9] + V[13] * r[4]) + L[4] * (V[14] * r[4]);
+      const double* Ai = A0inv;
+      raLS = TWO * L[3] * L[4] * Ai[14] + TWO * L[2] * L[4] * Ai[13] +
+             TWO * L[0] * L[1] * Ai[5] + TWO * L[1] * L[2] * Ai[9] +
+             TWO * L[1] * L[3] * Ai[10] + TWO * L[0] * L[2] * Ai[6] +
+             TWO * L[2] * L[3] * Ai[12] + TWO * L[1] * L[4] * Ai[11] +
+             TWO * L[0] * L[3] * Ai[7] + TWO * L[0] * L[4] * Ai[8] + L[0] * L[0] * Ai[0] +
+             L[1] * L[1] * Ai[1] + L[2] * L[2] * Ai[2] + L[3] * L[3] * Ai[3] +
+             L[4] * L[4] * Ai[4];
+      /* gijdu: 11, 22, 33, 12, 13, 23 */
+      const double m1 = gijd[0], m2 = gijd[2], m3 = gijd[5], m4 = gijd[1], m5 = gijd[3],
+                   m6 = gijd[4];
+      double detI = ONE / (m1 * m2 * m3 - m1 * m6 * m6 - m4 * m4 * m3 + m4 * m5 * m6 * TWO -
+                           m5 * m5 * m2);
+      giju[0] = detI * (m2 * m3 - m6 * m6);
+      giju[1] = detI * (m1 * m3 - m5 * m5);
+      giju[2] = detI * (m1 * m2 - m4 * m4);
+      giju[3] = detI * (m5 * m6 - m4 * m3);
+      giju[4] = detI * (m4 * m6 - m5 * m2);
+      giju[5] = detI * (m4 * m5 - m1 * m6);
+    }
     /* e3ls.f:351-458 RHS ri += A_i (tau L) */
     for (int i = 0; i < 5; ++i) ri[i] = rowdot(A1, i, rLyi) + ri[i];
     for (int i = 0; i < 5; ++i) ri[5 + i] = rowdot(A2, i, rLyi) + ri[5 + i];
@@ -481,6 +568,67 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
             for (int jd = 0; jd < 5; ++jd)
               EG[i0 + id][j0 + jd] = EG[i0 + id][j0 + jd] + fa * At[id][jd];
         }
+      }
+    }
+    /* ---- e3dc.f:44-451 discontinuity capturing (e3.f:217-222) ---- */
+    if (P->idc != 0) {
+      const double* gy[3] = {g1, g2, g3};
+      double A0g[15];
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 5; ++i)
+          A0g[5 * j + i] = A0[i][0] * gy[j][0] + A0[i][1] * gy[j][1] + A0[i][2] * gy[j][2] +
+                           A0[i][3] * gy[j][3] + A0[i][4] * gy[j][4];
+      /* giju rows: (1 4 5), (4 2 6), (5 6 3) */
+      static const int gr[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+      double gA[15];
+      for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 5; ++i)
+          gA[5 * j + i] = giju[gr[j][0]] * A0g[i] + giju[gr[j][1]] * A0g[5 + i] +
+                          giju[gr[j][2]] * A0g[10 + i];
+      double yAy[6];
+      for (int j = 0; j < 3; ++j) {
+        const double* g = gy[j];
+        yAy[j] = A0DC[0] * (g[0] * g[0]) + TWO * g[0] * A0DC[1] * g[4] +
+                 A0DC[2] * (g[1] * g[1]) + A0DC[2] * (g[2] * g[2]) + A0DC[2] * (g[3] * g[3]) +
+                 A0DC[3] * (g[4] * g[4]);
+      }
+      static const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+      for (int k = 0; k < 3; ++k) {
+        const double* a = gy[pr[k][0]];
+        const double* b = gy[pr[k][1]];
+        yAy[3 + k] = a[0] * A0DC[0] * b[0] + a[0] * A0DC[1] * b[4] + a[1] * A0DC[2] * b[1] +
+                     a[2] * A0DC[2] * b[2] + a[3] * A0DC[2] * b[3] + a[4] * A0DC[1] * b[0] +
+                     a[4] * A0DC[3] * b[4];
+      }
+      double gnorm = ONE / (giju[0] * yAy[0] + TWO * giju[3] * yAy[3] + TWO * giju[4] * yAy[4] +
+                            giju[1] * yAy[1] + TWO * giju[5] * yAy[5] + giju[2] * yAy[2] + P->epsM);
+      double DC = 0.0;
+      if (P->idc == 1) {
+        const double fact = ONE; /* ipord = 1 */
+        DC = fmax(0.0, (fact * sqrt(raLS * gnorm)) - (rTLS * gnorm));
+      } else if (P->idc == 2) {
+        DC = TWO * rTLS * gnorm;
+      } else if (P->idc == 3) {
+        const double fact = ONE;
+        DC = fmin(fmax(0.0, fact * sqrt(raLS * gnorm) - rTLS * gnorm), TWO * rTLS * gnorm);
+      }
+      /* ires = 1 */
+      for (int k = 0; k < 15; ++k) ri[k] = ri[k] + DC * gA[k];
+      /* e3dc.f:410-446 (iprec = 1): stiff[I,J] += DC A0 giju(IJ) */
+      if (lhs && P->iprec == 1) {
+        for (int j = 0; j < 5; ++j)
+          for (int i = 0; i < 5; ++i) {
+            double dtmp = A0[i][j] * DC;
+            stiff[i][j] = stiff[i][j] + dtmp * giju[0];
+            stiff[i][j + 5] = stiff[i][j + 5] + dtmp * giju[3];
+            stiff[i][j + 10] = stiff[i][j + 10] + dtmp * giju[4];
+            stiff[i + 5][j] = stiff[i + 5][j] + dtmp * giju[3];
+            stiff[i + 5][j + 5] = stiff[i + 5][j + 5] + dtmp * giju[1];
+            stiff[i + 5][j + 10] = stiff[i + 5][j + 10] + dtmp * giju[5];
+            stiff[i + 10][j] = stiff[i + 10][j] + dtmp * giju[4];
+            stiff[i + 10][j + 5] = stiff[i + 10][j + 5] + dtmp * giju[5];
+            stiff[i + 10][j + 10] = stiff[i + 10][j + 10] + dtmp * giju[2];
+          }
       }
     }
     /* ---- e3massr.f:34-72 ---- */

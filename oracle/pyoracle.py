@@ -27,7 +27,7 @@ class OrcParams(ctypes.Structure):
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
                [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred",
-                                               "idiff", "iter")]
+                                               "idiff", "iter", "idc")]
 
 
 class OrcStats(ctypes.Structure):
@@ -46,6 +46,12 @@ def lib():
             raise RuntimeError("oracle not built: run `make -C oracle`")
         _LIB = ctypes.CDLL(path)
     return _LIB
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (results do not depend on it);
+    returns the count in effect."""
+    return lib().orc_set_threads(int(n))
 
 
 def _p(a, t=_dp):

@@ -35,10 +35,25 @@ def build_mesh():
 
 
 def build_gpu():
+    """Each source compiled to an object in parallel (no device code
+    crosses files: every kernel is launched from its own file), then one
+    link."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    odir = os.path.join(PKG, "_build")
+    os.makedirs(odir, exist_ok=True)
+    objs, procs = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(odir, src.replace(".hip", ".o"))
+        cmd = [hipcc] + [f for f in HIPCC_FLAGS if f != "-shared"] + \
+              ["-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        objs.append(obj)
+    if any(p.wait() != 0 for p in procs):
+        raise RuntimeError("hipcc failed")
     out = os.path.join(PKG, "libphgpu.so")
     tmp = out + ".tmp"
-    _run([hipcc] + HIPCC_FLAGS + [os.path.join(CSRC, s) for s in HIP_SOURCES] + ["-o", tmp])
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fopenmp"] + objs + ["-o", tmp])
     os.replace(tmp, out)
 
 

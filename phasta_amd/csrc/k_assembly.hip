@@ -1616,6 +1616,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   Q.nel = c->nel;
   Q.ph = ph;
   const int g1 = (int)((c->nel + 63) / 64);
+  PHG_HIP(hipEventRecord(c->aev[0], c->stream));
   if (mixed && diff)
     k_asm_qp<float, true><<<g1, 256, 0, c->stream>>>(Q);
   else if (mixed)
@@ -1625,6 +1626,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   else
     k_asm_qp<double, false><<<g1, 256, 0, c->stream>>>(Q);
   PHG_HIP(hipGetLastError());
+  PHG_HIP(hipEventRecord(c->aev[1], c->stream));
   // pass 2: block owners (res, BDiag, lhsK written once)
   BlkArgs B{};
   B.lanes = c->lanes;
@@ -1680,6 +1682,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
       k_asm_blocks<double, double><<<g2, 64, 0, c->stream>>>(B);
   }
   PHG_HIP(hipGetLastError());
+  PHG_HIP(hipEventRecord(c->aev[2], c->stream));
   // wall faces: residual only, added colour by colour
   WallArgs W{};
   W.ien = c->ienbc;

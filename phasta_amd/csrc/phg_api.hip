@@ -613,6 +613,10 @@ int check_idiff(phg_ctx* c) {
                   "reference never forms qres (elmgmr.f:361)", d);
     return PHG_EINVAL;
   }
+  if (c->prm.idc != 0) {
+    phg_set_error("iDC %d: discontinuity capturing is not built on the GPU path", c->prm.idc);
+    return PHG_EINVAL;
+  }
   if (d != 0 && !c->qnode) {
     PHG_TRY(dev_alloc(c, &c->qrec, (size_t)c->nel * 16));
     PHG_TRY(dev_alloc(c, &c->qnode, (size_t)c->nshg * 16));
@@ -953,6 +957,11 @@ int sol_core(phg_ctx* c, phg_stats* st) {
   PHG_HIP(hipEventElapsedTime(&ms2, c->ev[1], c->ev[2]));
   st->ms_assembly = ms1;
   st->ms_solve = ms2;
+  float ma = 0, mb = 0;
+  PHG_HIP(hipEventElapsedTime(&ma, c->aev[0], c->aev[1]));
+  PHG_HIP(hipEventElapsedTime(&mb, c->aev[1], c->aev[2]));
+  st->ms_point = ma;
+  st->ms_blocks = mb;
   st->ms_total = ms1 + ms2;
   return 0;
 }
@@ -1019,11 +1028,16 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
   for (auto& ev : c->kev) {
     if (hipEventCreate(&ev) != hipSuccess) ev = nullptr;
   }
+  for (auto& ev : c->aev) {
+    if (hipEventCreate(&ev) != hipSuccess) ev = nullptr;
+  }
   int rc = setup(c, mesh);
   if (rc == 0) {
     for (auto ev : c->ev)
       if (!ev) rc = PHG_EHIP;
     for (auto ev : c->kev)
+      if (!ev) rc = PHG_EHIP;
+    for (auto ev : c->aev)
       if (!ev) rc = PHG_EHIP;
     if (rc) phg_set_error("hipEventCreate failed");
   }
@@ -1056,6 +1070,8 @@ extern "C" int phg_finalize(phg_ctx* c) {
   for (auto ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto ev : c->kev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto ev : c->aev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

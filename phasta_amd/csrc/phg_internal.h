@@ -278,6 +278,7 @@ struct phg_ctx {
   // timing events on the library stream, created once
   hipEvent_t ev[3] = {};
   hipEvent_t kev[4] = {};
+  hipEvent_t aev[3] = {};  // assembly: before pass 1, after pass 1, after pass 2
 };
 
 // ---- launchers (k_assembly.hip) ----

@@ -18,7 +18,7 @@ module phgpu
     real(c_double) :: etol, epsM
     integer(c_int) :: Kspace, nGMRES, minIters
     integer(c_int) :: lhs, iprec
-    integer(c_int) :: ipred, idiff, iter
+    integer(c_int) :: ipred, idiff, iter, idc
   end type
 
   type, bind(C) :: phg_mesh
@@ -38,6 +38,7 @@ module phgpu
     integer(c_int) :: n_spmv, n_mgs
     integer(c_int) :: refine_cycles, pc_fallback
     real(c_double) :: orth_loss
+    real(c_double) :: ms_point, ms_blocks
   end type
 
   interface

@@ -39,7 +39,7 @@ class PhgParams(ctypes.Structure):
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
                [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred",
-                                               "idiff", "iter")]
+                                               "idiff", "iter", "idc")]
 
 
 class PhgMesh(ctypes.Structure):
@@ -59,7 +59,8 @@ class PhgStats(ctypes.Structure):
                 ("ms_spmv", ctypes.c_double), ("ms_mgs", ctypes.c_double),
                 ("n_spmv", ctypes.c_int), ("n_mgs", ctypes.c_int),
                 ("refine_cycles", ctypes.c_int), ("pc_fallback", ctypes.c_int),
-                ("orth_loss", ctypes.c_double)]
+                ("orth_loss", ctypes.c_double), ("ms_point", ctypes.c_double),
+                ("ms_blocks", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -95,6 +95,23 @@ def make_idiff(N, seed):
     np.savez_compressed(os.path.join(HERE, f"ref_idiff_box{N}.npz"), N=N, y=y, ac=ac, **out)
 
 
+def make_dc(N, seed):
+    """ElmGMRs with discontinuity capturing (iDC 1, 2, 3: e3dc.f with the
+    A0DC, A0inv, dV/dY of e3mtrx.f and the rTLS, raLS, giju of e3tau.f) on
+    a jittered box at a strongly perturbed state, so the DC factor is
+    active at most points."""
+    case = mesh.box_case(N, mach=0.5, kspace=10, jitter=0.1)
+    rng = np.random.default_rng(seed)
+    y, ac = pyref.Reference(case).predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    out = {}
+    for idc in (1, 2, 3):
+        res, rmes, bd, lk = pyref.Reference(case, params=dict(case.params, idc=idc)).elmgmrs(y, ac)
+        out[f"res{idc}"], out[f"BDiag{idc}"], out[f"lhsK{idc}"] = res, bd, lk
+    np.savez_compressed(os.path.join(HERE, f"ref_dc_box{N}.npz"), N=N, y=y, ac=ac, **out)
+
+
 if __name__ == "__main__":
     for N, seed in ((2, 11), (3, 12)):
         make(N, seed)
@@ -103,3 +120,5 @@ if __name__ == "__main__":
     print("wrote ref_alpha_box3.npz")
     make_idiff(3, 14)
     print("wrote ref_idiff_box3.npz")
+    make_dc(3, 15)
+    print("wrote ref_dc_box3.npz")

@@ -159,6 +159,45 @@ def test_idiff_elmgmrs_bitwise_equals_live_reference():
             assert np.array_equal(a, b), idiff
 
 
+# ------------------------------------------------ discontinuity capturing
+def dc_case():
+    return mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+
+
+def test_dc_elmgmrs_equals_reference_golden():
+    """ElmGMRs with iDC 1, 2, 3 (e3dc.f, with e3mtrx.f's A0DC/A0inv/dVdY and
+    e3tau.f's rTLS/raLS/giju) bitwise against the reference's compiled
+    routines (fixture tests/golden/ref_dc_box3.npz); DC changes the residual
+    and the tangent, and the three variants differ."""
+    g = np.load(os.path.join(GOLD, "ref_dc_box3.npz"))
+    case = dc_case()
+    y, ac = np.asfortranarray(g["y"]), np.asfortranarray(g["ac"])
+    base = pyoracle.Oracle(case).elmgmrs(y, ac)
+    for idc in (1, 2, 3):
+        res, rmes, bd, lk = pyoracle.Oracle(case, params=dict(case.params, idc=idc)).elmgmrs(y, ac)
+        assert np.array_equal(res, g[f"res{idc}"]), idc
+        assert np.array_equal(bd, g[f"BDiag{idc}"]), idc
+        assert np.array_equal(lk, g[f"lhsK{idc}"]), idc
+        assert rel(res, base[0]) > 1e-6 and rel(lk, base[3]) > 1e-6, idc
+    assert rel(g["res1"], g["res2"]) > 1e-9 and rel(g["res1"], g["res3"]) > 1e-9
+    with pytest.raises(RuntimeError):
+        pyoracle.Oracle(case, params=dict(case.params, idc=4)).elmgmrs(y, ac)
+
+
+@pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
+def test_dc_elmgmrs_bitwise_equals_live_reference():
+    case = mesh.box_case(4, mach=0.3, kspace=10, jitter=0.1, grade=1.1)
+    rng = np.random.default_rng(9)
+    y, ac = pyoracle.Oracle(case).itr_predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    for idc in (1, 2, 3):
+        prm = dict(case.params, idc=idc)
+        for a, b in zip(pyoracle.Oracle(case, params=prm).elmgmrs(y, ac),
+                        pyref.Reference(case, params=prm).elmgmrs(y, ac)):
+            assert np.array_equal(a, b), idc
+
+
 def test_asiq_reconstructs_linear_fields_exactly():
     """What the mathematics fixes for AsIq + qpbc: on a uniform box (every
     node patch point-symmetric) the lumped L2 projection of a piecewise
