@@ -98,22 +98,28 @@ int build_row_layout(long long nshg, long long nel, const std::vector<int>& rows
                      const std::vector<int>& incp, const std::vector<int>& incl,
                      const std::vector<uint32_t>& bmap, const int* colm, const int* rowp,
                      RowLayout& L) {
-  // pack rows (in the given order) into workgroups
+  // pack rows (in the given order) into workgroups: a workgroup holds at
+  // most kRowLanes incidences (its lanes) and at most kRowLanes CSR entries
+  // (so one owner task per entry always fits).  Rows of periodic edge and
+  // corner nodes have more entries than incident elements, so the second
+  // bound is needed where such rows cluster.
   std::vector<long long> wg_row0(1, 0);
   {
-    int used = 0;
+    int used = 0, used_ent = 0;
     for (long long ri = 0; ri < nshg; ++ri) {
       const int n = incp[rows[ri] + 1] - incp[rows[ri]];
-      if (n > kRowLanes || n == 0) {
-        phg_set_error("node %d belongs to %d elements (1..%d supported)", rows[ri] + 1, n,
-                      kRowLanes);
+      const int ne = colm[rows[ri] + 1] - colm[rows[ri]];
+      if (n > kRowLanes || n == 0 || ne > kRowLanes) {
+        phg_set_error("node %d belongs to %d elements and %d CSR entries (1..%d supported)",
+                      rows[ri] + 1, n, ne, kRowLanes);
         return PHG_EINVAL;
       }
-      if (used + n > kRowLanes) {
+      if (used + n > kRowLanes || used_ent + ne > kRowLanes) {
         wg_row0.push_back(ri);
-        used = 0;
+        used = used_ent = 0;
       }
       used += n;
+      used_ent += ne;
     }
     wg_row0.push_back(nshg);
   }

@@ -68,8 +68,12 @@ def test_nan_in_the_state_returns_enan(case):
         with pytest.raises(solver.PhgError) as ei:
             gpu.solgmrs(bad, ac, iter=1)
         assert ei.value.code == 10
-        out = gpu.solgmrs(y, ac, iter=1)  # the context stays usable
-        assert out["converged"]
+        # the context stays usable: the next call is the oracle's solve
+        # (box16 at K = 10 stops unconverged in the reference too)
+        ref = orc.solgmrs(y, ac, iter=1)
+        out = gpu.solgmrs(y, ac, iter=1)
+        assert out["iKs"] == ref["iKs"] and out["converged"] == ref["converged"]
+        assert rel(out["Dy"], ref["Dy"]) < TOL
     finally:
         gpu.close()
 
@@ -95,6 +99,9 @@ def test_divergence_abort_from_the_second_newton_iteration(case):
 def test_resident_stepping_resets_the_iteration_count(case):
     """phg_predict starts a time step (iter = 0): its first Newton
     iteration is not tested against the previous step's residual."""
+    orc = pyoracle.Oracle(case)
+    y, ac = orc.itr_predict_bc(case.yold)
+    ref = orc.solgmrs(y, ac, iter=1)
     gpu = solver.GpuSolver(case)
     try:
         gpu.upload_state(rest_state(case))
@@ -104,7 +111,9 @@ def test_resident_stepping_resets_the_iteration_count(case):
         gpu.upload_state(case.yold)
         gpu.predict()
         st = gpu.newton_iteration()
-        assert st["reslast"] > 1e4 * quiet and st["converged"]
+        # not aborted, and the same solve as the oracle's first iteration
+        assert st["reslast"] > 1e4 * quiet
+        assert st["iKs"] == ref["iKs"] and st["converged"] == ref["converged"]
     finally:
         gpu.close()
 
