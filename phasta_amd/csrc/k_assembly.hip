@@ -822,285 +822,6 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   }
 }
 
-// ---------------- pass 2: row lanes ----------------
-// One lane per (element e, local node a): the lane forms row a of the
-// element's tangent, the four blocks (a, b), b = 0..3, summed over the four
-// points, from the stored point states: P_a = A(s_a) diag(tau) + N_a I is
-// formed once per point and shared by the four blocks (Q_b = A(s_b) + c N_b
-// A0 enters in factored form, as its rows 0 and 4 and rho w).  The lanes of
-// one node (one matrix row) lie in one workgroup, in element order, so the
-// blocks of the row's CSR entries are summed inside the workgroup: in round
-// b every lane puts its block (a, b) in LDS and owner lanes add the
-// contributions listed for their chunk of a CSR entry (built on the host in
-// element order); a leader combines the chunks of an entry in double,
-// applies bc3LHS and stores the block once.  The owner of a diagonal entry
-// also sums the node's residual and its BDiag.  No atomics: every output is
-// written once, in a fixed order.
-//
-// T: tangent arithmetic and stored block type; BH: blocks formed per pass
-// (4 in float; 2 in double, two passes, to keep the accumulators in
-// registers).
-struct RowArgs {
-  const int* code;          // kRowLanes per workgroup: e * 4 + a, -1 idle
-  const int4* task;         // kRowLanes per workgroup: contribution-list
-                            // offset, CSR entry, count (0: no task), flags
-                            // (bit 0 leader, bit 1 diagonal, bits 2..7 chunks
-                            // of the entry, bits 8.. element count of the row)
-  const uint16_t* list;     // (b << 8) | lane of each contribution
-  const int* ien;           // 4 planes of nel
-  const uint8_t* bcm;
-  const void* qps;
-  const void* geo;
-  const double* resid;      // (nel * 4) x 5
-  double* res;              // 5 per node
-  double* bdiag;            // 25 per node
-  void* lhs;                // one padded block per CSR entry
-  long long nel, nwg;
-  int lhs_on, prec_on;
-  double mu, lam, kap, cfac;
-};
-
-template <typename T, int BH>
-__global__ void __launch_bounds__(kRowLanes) k_asm_rows(RowArgs A) {
-  constexpr int SS = aos_stride<T>();  // 16-byte aligned block slots
-  __shared__ T slot[kRowLanes * SS];
-  __shared__ double rst[kRowLanes * 5];
-  // consecutive workgroups go round robin over the 8 XCDs: give each XCD a
-  // contiguous range of rows (their elements' point states share its L2)
-  const long long per = (gridDim.x + 7) / 8;
-  const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (wg >= A.nwg) return;  // the whole workgroup
-  const int t = threadIdx.x;
-  const long long base = wg * kRowLanes;
-  const int code = A.code[base + t];
-  const int4 tk = A.task[base + t];
-  const bool act = code >= 0;
-  const long long e = act ? (code >> 2) : 0;
-  const int a = act ? (code & 3) : 0;
-  using S = T;  // point states are stored in the tangent's type
-  const S* go = reinterpret_cast<const S*>(A.geo) + (size_t)e * rec_stride<S>();
-  const S* qrec = reinterpret_cast<const S*>(A.qps) + (size_t)e * 4 * rec_stride<S>();
-  T sg[4][3];
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) sg[k][c] = go[3 * k + c];
-  const T W = go[12];
-  const T sA[3] = {sel4t(sg[0][0], sg[1][0], sg[2][0], sg[3][0], a),
-                   sel4t(sg[0][1], sg[1][1], sg[2][1], sg[3][1], a),
-                   sel4t(sg[0][2], sg[1][2], sg[2][2], sg[3][2], a)};
-  const T muR = (T)A.mu, lamR = (T)A.lam, kapR = (T)A.kap, cfacR = (T)A.cfac;
-  T own[25];
-#pragma unroll
-  for (int i = 0; i < 25; ++i) own[i] = (T)0;
-#pragma unroll 1
-  for (int h = 0; h < 4 / BH; ++h) {
-    T acc[BH][25];
-#pragma unroll
-    for (int bb = 0; bb < BH; ++bb)
-#pragma unroll
-      for (int i = 0; i < 25; ++i) acc[bb][i] = (T)0;
-    T ub[3] = {(T)0, (T)0, (T)0};
-    if (act) {
-#pragma unroll 1
-      for (int q = 0; q < 4; ++q) {
-        S o[16];
-        load_rec<S>(qrec + q * rec_stride<S>(), o);
-        const T u0 = o[0], u1 = o[1], u2 = o[2];
-        const T uq[3] = {u0, u1, u2};
-        ub[0] += u0;
-        ub[1] += u1;
-        ub[2] += u2;
-        const T rho = o[3], drdp = o[4], drdT = o[5];
-        const T e1p = o[6], e2p = o[7], e3p = o[8], e4p = o[9];
-        const T tau0 = o[10], tau1 = o[11], tau2 = o[12];
-        const T Na = W * (T)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
-        const T una = u0 * sA[0] + u1 * sA[1] + u2 * sA[2];
-        // P = A(s_a) diag(tau) + N_a I (rows m, columns k); the stored tau
-        // is WdetJ tau and N_a carries WdetJ, so P comes out times WdetJ
-        T P[5][5];
-        P[0][0] = drdp * una * tau0 + Na;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) P[0][1 + n] = rho * sA[n] * tau1;
-        P[0][4] = drdT * una * tau2;
-#pragma unroll
-        for (int mm = 0; mm < 3; ++mm) {
-          P[1 + mm][0] = (drdp * uq[mm] * una + sA[mm]) * tau0;
-#pragma unroll
-          for (int n = 0; n < 3; ++n)
-            P[1 + mm][1 + n] =
-                rho * (uq[mm] * sA[n] + (mm == n ? una : (T)0)) * tau1 + (mm == n ? Na : (T)0);
-          P[1 + mm][4] = drdT * uq[mm] * una * tau2;
-        }
-        P[4][0] = e2p * una * tau0;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) P[4][1 + n] = (e3p * sA[n] + rho * uq[n] * una) * tau1;
-        P[4][4] = e4p * una * tau2 + Na;
-        // t_m = P(m,0) + sum_k P(m,1+k) u_k: the part of row m shared by
-        // Q's columns through its rank-one terms
-        T tm[5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) tm[m] = P[m][0] + (P[m][1] * u0 + P[m][2] * u1 + P[m][3] * u2);
-#pragma unroll
-        for (int bb = 0; bb < BH; ++bb) {
-          const int b = h * BH + bb;
-          const T rR[3] = {sel4t(sg[0][0], sg[1][0], sg[2][0], sg[3][0], b),
-                           sel4t(sg[0][1], sg[1][1], sg[2][1], sg[3][1], b),
-                           sel4t(sg[0][2], sg[1][2], sg[2][2], sg[3][2], b)};
-          const T Nb = (T)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
-          const T unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
-          const T w = unb + cfacR * Nb;
-          // Q = A(s_b) + c N_b A0: column 0 = drdp w u + r, columns 1..3
-          // = rho (u r^T + w I) on the momentum rows, column 4 = drdT w u
-          const T q00 = drdp * w, q04 = drdT * w, rw = rho * w;
-          T q0[3], q4[3];
-#pragma unroll
-          for (int n = 0; n < 3; ++n) {
-            q0[n] = rho * rR[n];
-            q4[n] = e3p * rR[n] + rw * uq[n];
-          }
-          const T q40 = e2p * unb + cfacR * Nb * e1p, q44 = e4p * w;
-#pragma unroll
-          for (int m = 0; m < 5; ++m) {
-            T& c0 = acc[bb][m];
-            c0 += tm[m] * q00;
-            c0 += P[m][1] * rR[0];
-            c0 += P[m][2] * rR[1];
-            c0 += P[m][3] * rR[2];
-            c0 += P[m][4] * q40;
-#pragma unroll
-            for (int n = 0; n < 3; ++n) {
-              T& cn = acc[bb][m + 5 * (1 + n)];
-              cn += tm[m] * q0[n];
-              cn += rw * P[m][1 + n];
-              cn += P[m][4] * q4[n];
-            }
-            T& c4 = acc[bb][m + 20];
-            c4 += tm[m] * q04;
-            c4 += P[m][4] * q44;
-          }
-        }
-      }
-      // viscous part sum_q WdetJ K_ab: the momentum rows and the conduction
-      // term do not depend on the point, the energy row only through the
-      // velocity, linearly (sum over the points = ub)
-#pragma unroll
-      for (int bb = 0; bb < BH; ++bb) {
-        const int b = h * BH + bb;
-        const T rR[3] = {sel4t(sg[0][0], sg[1][0], sg[2][0], sg[3][0], b),
-                         sel4t(sg[0][1], sg[1][1], sg[2][1], sg[3][1], b),
-                         sel4t(sg[0][2], sg[1][2], sg[2][2], sg[3][2], b)};
-        const T st = sA[0] * rR[0] + sA[1] * rR[1] + sA[2] * rR[2];
-        const T W4 = (T)4 * W;
-        const T ubs = ub[0] * sA[0] + ub[1] * sA[1] + ub[2] * sA[2];
-        const T ubr = ub[0] * rR[0] + ub[1] * rR[1] + ub[2] * rR[2];
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-          for (int n = 0; n < 3; ++n)
-            acc[bb][(1 + m) + 5 * (1 + n)] +=
-                W4 * (muR * ((m == n ? st : (T)0) + sA[n] * rR[m]) + lamR * sA[m] * rR[n]);
-#pragma unroll
-        for (int n = 0; n < 3; ++n)
-          acc[bb][4 + 5 * (1 + n)] += W * (muR * (st * ub[n] + sA[n] * ubr) + lamR * ubs * rR[n]);
-        acc[bb][24] += W4 * kapR * st;
-      }
-    }
-    // rounds: block b of every lane through LDS to the owners of its entry
-#pragma unroll
-    for (int bb = 0; bb < BH; ++bb) {
-      const int b = h * BH + bb;
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 25; ++i) slot[t * SS + i] = acc[bb][i];
-      __syncthreads();
-      for (int c = 0; c < tk.z; ++c) {
-        const int v = A.list[tk.x + c];
-        if ((v >> 8) != b) continue;
-        const T* src = slot + (v & 255) * SS;
-#pragma unroll
-        for (int i = 0; i < 25; ++i) own[i] += src[i];
-      }
-    }
-  }
-  // the node's residual: the element residuals of the diagonal entry's
-  // contributions (lane m's own element and node)
-  const bool diag = (tk.w >> 1) & 1;
-  double rown[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (diag) {
-    for (int c = 0; c < tk.z; ++c) {
-      const int mcode = A.code[base + (A.list[tk.x + c] & 255)];
-      const double* ro = A.resid + (size_t)mcode * 5;
-#pragma unroll
-      for (int f = 0; f < 5; ++f) rown[f] += ro[f];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 25; ++i) slot[t * SS + i] = own[i];
-#pragma unroll
-  for (int f = 0; f < 5; ++f) rst[t * 5 + f] = rown[f];
-  __syncthreads();
-  if (!(tk.w & 1) || tk.z == 0) return;
-  // leader: the chunks of the entry are the next nch lanes' tasks
-  const int nch = (tk.w >> 2) & 63;
-  const int total = tk.w >> 8;
-  double sum[25];
-#pragma unroll
-  for (int i = 0; i < 25; ++i) sum[i] = (double)slot[t * SS + i];
-  for (int cc = 1; cc < nch; ++cc)
-#pragma unroll
-    for (int i = 0; i < 25; ++i) sum[i] += (double)slot[(t + cc) * SS + i];
-  const long long k = tk.y;
-  const int mcode0 = A.code[base + (A.list[tk.x] & 255)];
-  const long long e0 = mcode0 >> 2;
-  const int a0 = mcode0 & 3, b0 = (A.list[tk.x] >> 8) & 3;
-  const long long nel = A.nel;
-  const int v = A.ien[a0 * nel + e0], wcol = A.ien[b0 * nel + e0];
-  if (diag) {
-    double racc[5];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) racc[f] = rst[t * 5 + f];
-    for (int cc = 1; cc < nch; ++cc)
-#pragma unroll
-      for (int f = 0; f < 5; ++f) racc[f] += rst[(t + cc) * 5 + f];
-#pragma unroll
-    for (int f = 0; f < 5; ++f) A.res[5 * (size_t)v + f] = racc[f];
-    // block diagonal before the Dirichlet rows/columns (asigmr.f:92-103)
-    if (A.prec_on) {
-#pragma unroll
-      for (int i = 0; i < 25; ++i) A.bdiag[25 * (size_t)v + i] = sum[i];
-    }
-  }
-  if (!A.lhs_on) return;
-  // bc3LHS on every element block: constrained rows of node a and columns of
-  // node b zeroed, a unit diagonal per element block of the node
-  const uint8_t ma = A.bcm[v], mb = A.bcm[wcol];
-  if (ma | mb) {
-#pragma unroll
-    for (int j = 0; j < 5; ++j)
-#pragma unroll
-      for (int i = 0; i < 5; ++i) {
-        if (((ma >> i) & 1) || ((mb >> j) & 1)) sum[i + 5 * j] = 0.0;
-        if (diag && i == j && ((ma >> i) & 1)) sum[i + 5 * j] = (double)total;
-      }
-  }
-  T* dst = reinterpret_cast<T*>(A.lhs) + (size_t)aos_stride<T>() * k;
-  if constexpr (sizeof(T) == 4) {
-    float4* vv = reinterpret_cast<float4*>(dst);
-#pragma unroll
-    for (int kk = 0; kk < 6; ++kk)
-      vv[kk] = make_float4((float)sum[4 * kk], (float)sum[4 * kk + 1], (float)sum[4 * kk + 2],
-                           (float)sum[4 * kk + 3]);
-    vv[6] = make_float4((float)sum[24], 0.f, 0.f, 0.f);
-  } else {
-    double2* vv = reinterpret_cast<double2*>(dst);
-#pragma unroll
-    for (int kk = 0; kk < 12; ++kk) vv[kk] = make_double2(sum[2 * kk], sum[2 * kk + 1]);
-    vv[12] = make_double2(sum[24], 0.0);
-  }
-}
-
 // Wall faces: floating convective, pressure, viscous and heat fluxes
 // (iBCB = 0), residual only (iLHScond = 0).  Nodes 1..3 lie on the face,
 // ordered so (x2-x1) x (x3-x1) points outward.
@@ -1648,39 +1369,11 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.lam = -(2.0 / 3.0) * ph.mu;
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
-  if (c->rows_pass) {
-    RowArgs R{};
-    R.code = c->rl_code;
-    R.task = c->rl_task;
-    R.list = c->rl_list;
-    R.ien = c->ienc;
-    R.bcm = c->bcm;
-    R.qps = Q.qps;
-    R.geo = Q.geo;
-    R.resid = c->resid;
-    R.res = c->acc_res;
-    R.bdiag = c->acc_bd;
-    R.lhs = B.lhs;
-    R.nel = c->nel;
-    R.nwg = c->rl_nwg;
-    R.lhs_on = lhs;
-    R.prec_on = prec;
-    R.mu = B.mu;
-    R.lam = B.lam;
-    R.kap = B.kap;
-    R.cfac = B.cfac;
-    const int g3 = (int)((c->rl_nwg + 7) / 8 * 8);  // whole XCD rounds
-    if (mixed)
-      k_asm_rows<float, 4><<<g3, kRowLanes, 0, c->stream>>>(R);
-    else
-      k_asm_rows<double, 1><<<g3, kRowLanes, 0, c->stream>>>(R);
-  } else {
-    const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
-    if (mixed)
-      k_asm_blocks<float, float><<<g2, 64, 0, c->stream>>>(B);
-    else
-      k_asm_blocks<double, double><<<g2, 64, 0, c->stream>>>(B);
-  }
+  const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
+  if (mixed)
+    k_asm_blocks<float, float><<<g2, 64, 0, c->stream>>>(B);
+  else
+    k_asm_blocks<double, double><<<g2, 64, 0, c->stream>>>(B);
   PHG_HIP(hipGetLastError());
   PHG_HIP(hipEventRecord(c->aev[2], c->stream));
   // wall faces: residual only, added colour by colour

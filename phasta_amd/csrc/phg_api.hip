@@ -79,146 +79,6 @@ bool bc_code_supported(int ibc) {
   return v == 0 || v == 7;
 }
 
-// Row lanes of the block pass (k_asm_rows): one lane per (element, local
-// node) incidence; the lanes of a node (one matrix row) are consecutive, in
-// element order, and never split across workgroups of kRowLanes lanes.
-// Per workgroup the owner tasks of its rows' CSR entries: each entry's
-// contributions (lane, b) in element order, cut into chunks (tasks) of at
-// most `chunk` contributions, the chunks of one entry on consecutive lanes.
-// The chunk size doubles for a workgroup whose tasks would outnumber its
-// lanes.
-struct RowLayout {
-  long long nwg = 0;
-  std::vector<int> code;       // nwg * kRowLanes: e * 4 + a, -1 idle
-  std::vector<int4> task;      // nwg * kRowLanes
-  std::vector<uint16_t> list;  // (b << 8) | lane
-};
-
-int build_row_layout(long long nshg, long long nel, const std::vector<int>& rows,
-                     const std::vector<int>& incp, const std::vector<int>& incl,
-                     const std::vector<uint32_t>& bmap, const int* colm, const int* rowp,
-                     RowLayout& L) {
-  // pack rows (in the given order) into workgroups: a workgroup holds at
-  // most kRowLanes incidences (its lanes) and at most kRowLanes CSR entries
-  // (so one owner task per entry always fits).  Rows of periodic edge and
-  // corner nodes have more entries than incident elements, so the second
-  // bound is needed where such rows cluster.
-  std::vector<long long> wg_row0(1, 0);
-  {
-    int used = 0, used_ent = 0;
-    for (long long ri = 0; ri < nshg; ++ri) {
-      const int n = incp[rows[ri] + 1] - incp[rows[ri]];
-      const int ne = colm[rows[ri] + 1] - colm[rows[ri]];
-      if (n > kRowLanes || n == 0 || ne > kRowLanes) {
-        phg_set_error("node %d belongs to %d elements and %d CSR entries (1..%d supported)",
-                      rows[ri] + 1, n, ne, kRowLanes);
-        return PHG_EINVAL;
-      }
-      if (used + n > kRowLanes || used_ent + ne > kRowLanes) {
-        wg_row0.push_back(ri);
-        used = used_ent = 0;
-      }
-      used += n;
-      used_ent += ne;
-    }
-    wg_row0.push_back(nshg);
-  }
-  const long long nwg = (long long)wg_row0.size() - 1;
-  L.nwg = nwg;
-  L.code.assign(nwg * kRowLanes, -1);
-  L.task.assign(nwg * kRowLanes, make_int4(0, 0, 0, 0));
-  std::vector<std::vector<uint16_t>> lists(nwg);
-  int bad = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(| : bad)
-  for (long long g = 0; g < nwg; ++g) {
-    int* code = L.code.data() + g * kRowLanes;
-    int4* task = L.task.data() + g * kRowLanes;
-    int m = 0;
-    for (long long ri = wg_row0[g]; ri < wg_row0[g + 1]; ++ri)
-      for (int j = incp[rows[ri]]; j < incp[rows[ri] + 1]; ++j) code[m++] = incl[j];
-    // contributions of every entry of the workgroup's rows, element order
-    std::vector<std::vector<uint16_t>> contrib;
-    std::vector<long long> ent;  // CSR entry of each contrib list
-    std::vector<int> ent_row;    // node of the entry's row
-    m = 0;
-    for (long long ri = wg_row0[g]; ri < wg_row0[g + 1]; ++ri) {
-      const int row = rows[ri];
-      const long long k0 = colm[row] - 1, nk = colm[row + 1] - colm[row];
-      const size_t first = contrib.size();
-      for (long long k = 0; k < nk; ++k) {
-        contrib.emplace_back();
-        ent.push_back(k0 + k);
-        ent_row.push_back(row);
-      }
-      for (int j = incp[row]; j < incp[row + 1]; ++j, ++m) {
-        const long long e = code[m] >> 2;
-        const int a = code[m] & 3;
-        for (int b = 0; b < 4; ++b) {
-          const long long kk = bmap[(4 * a + b) * nel + e];
-          if (kk < k0 || kk >= k0 + nk) {
-            bad = 1;
-            continue;
-          }
-          contrib[first + (kk - k0)].push_back((uint16_t)((b << 8) | m));
-        }
-      }
-    }
-    for (const auto& cl : contrib)
-      if (cl.empty()) bad = 1;
-    if (bad) continue;
-    // chunk the entries; double the chunk size until the tasks fit the lanes
-    size_t maxc = 0;
-    for (const auto& cl : contrib) maxc = std::max(maxc, cl.size());
-    for (int chunk = 4;; chunk *= 2) {
-      long long ntask = 0;
-      for (const auto& cl : contrib) ntask += ((long long)cl.size() + chunk - 1) / chunk;
-      if (ntask > kRowLanes) {
-        if ((size_t)chunk >= maxc) {  // one task per entry does not fit either
-          bad = 2;
-          break;
-        }
-        continue;
-      }
-      int t = 0;
-      auto& lst = lists[g];
-      for (size_t q = 0; q < contrib.size(); ++q) {
-        const auto& cl = contrib[q];
-        const int n = (int)cl.size();
-        const int nch = (n + chunk - 1) / chunk;
-        const bool dg = (rowp[ent[q]] - 1) == ent_row[q];
-        for (int ch = 0; ch < nch; ++ch, ++t) {
-          const int c0 = n * ch / nch, c1 = n * (ch + 1) / nch;
-          int flags = (dg ? 2 : 0) | (ch == 0 ? 1 : 0);
-          if (ch == 0) flags |= (nch << 2) | (n << 8);
-          task[t] = make_int4((int)lst.size(), (int)ent[q], c1 - c0, flags);
-          for (int c = c0; c < c1; ++c) lst.push_back(cl[c]);
-        }
-      }
-      break;
-    }
-  }
-  if (bad) {
-    phg_set_error(bad & 1 ? "element adjacency missing from colm/rowp (row-lane layout)"
-                          : "more CSR entries than lanes in a row-lane workgroup");
-    return PHG_EINVAL;
-  }
-  // concatenate the per-workgroup lists
-  std::vector<long long> off(nwg + 1, 0);
-  for (long long g = 0; g < nwg; ++g) off[g + 1] = off[g] + (long long)lists[g].size();
-  if (off[nwg] >= (1LL << 31)) {
-    phg_set_error("row-lane contribution list too long");
-    return PHG_EINVAL;
-  }
-  L.list.resize(off[nwg]);
-#pragma omp parallel for schedule(static)
-  for (long long g = 0; g < nwg; ++g) {
-    std::copy(lists[g].begin(), lists[g].end(), L.list.begin() + off[g]);
-    for (int t = 0; t < kRowLanes; ++t)
-      if (L.task[g * kRowLanes + t].z > 0) L.task[g * kRowLanes + t].x += (int)off[g];
-  }
-  return 0;
-}
-
 int setup(phg_ctx* c, const phg_mesh* m) {
   const long long nshg = m->nshg, nel = m->nel, nelb = m->nelb, nnz = m->nnz_tot;
   // ---- validation ----
@@ -466,9 +326,6 @@ int setup(phg_ctx* c, const phg_mesh* m) {
       for (int j = 0; j < incp[n + 1] - incp[n]; ++j) incs[goff[g] + 64LL * j + l] = incl[incp[n] + j];
     }
   }
-  RowLayout rl;
-  PHG_TRY(build_row_layout(nshg, nel, rows, incp, incl, bmap, m->colm, m->rowp, rl));
-  c->rl_nwg = rl.nwg;
   incl.clear();
   incl.shrink_to_fit();
   std::vector<int> ienbc(4 * std::max<long long>(nelb, 1));
@@ -508,9 +365,6 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->ienc, 4 * nel));
   PHG_TRY(dev_alloc(c, &c->lanes, c->nlanes));
   PHG_TRY(dev_alloc(c, &c->blocks, 16 * nel));
-  PHG_TRY(dev_alloc(c, &c->rl_code, rl.code.size()));
-  PHG_TRY(dev_alloc(c, &c->rl_task, rl.task.size()));
-  PHG_TRY(dev_alloc(c, &c->rl_list, rl.list.size()));
   // point-state and geometry records: 13 values padded to 16 floats or 14
   // doubles (16-byte loads)
   if (mixed) {
@@ -570,9 +424,6 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->inc, incs.data(), incs.size()));
   PHG_TRY(upload(c, c->lanes, lanes.data(), lanes.size()));
   PHG_TRY(upload(c, c->blocks, blk_lane.data(), blk_lane.size()));
-  PHG_TRY(upload(c, c->rl_code, rl.code.data(), rl.code.size()));
-  PHG_TRY(upload(c, c->rl_task, rl.task.data(), rl.task.size()));
-  PHG_TRY(upload(c, c->rl_list, rl.list.data(), rl.list.size()));
   PHG_TRY(upload(c, c->ienbc, ienbc.data(), ienbc.size()));
   PHG_TRY(upload(c, c->bcm, bcm.data(), bcm.size()));
   PHG_TRY(upload(c, c->iper, iper0.data(), iper0.size()));
@@ -1021,7 +872,6 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
   c->ndofBC = mesh->ndofBC;
   c->precision = precision;
   c->prm = *prm;
-  c->rows_pass = std::getenv("PHG_OWNER_LANES") == nullptr;  // TEMP A/B
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     phg_set_error("hipStreamCreate: %s", hipGetErrorString(e));
@@ -1068,8 +918,7 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
                   c->acold,    c->dy,       c->krylov, c->red_part, c->scal,
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
-                  c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode,
-                  c->rl_code,  c->rl_task,  c->rl_list};
+                  c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);

@@ -102,10 +102,6 @@ __device__ inline void store_slot(double* blk, int r, const A a[25]) {
 // Largest number of element blocks one assembly lane sums (the blocks of
 // one CSR entry are split into chunks of at most this size).
 constexpr int kChunk = 6;
-// Lanes per workgroup of the row-lane block pass (three wavefronts: eight
-// interior nodes of 24 elements each).
-constexpr int kRowLanes = 192;
-
 // Physical constants and time-integration factors used in the kernels.
 struct PhysParams {
   double Rgas, gamma, gamma1, pr, mu;
@@ -198,12 +194,6 @@ struct phg_ctx {
   long long nlanes = 0;
   int4* lanes = nullptr;        // block-list offset, CSR entry, count, flags
   int* blocks = nullptr;        // e*16 + a*4 + b, grouped per owner lane
-  // row lanes of the block pass (k_asm_rows), kRowLanes per workgroup
-  long long rl_nwg = 0;
-  int* rl_code = nullptr;       // e * 4 + a per lane, -1 idle
-  int4* rl_task = nullptr;      // owner task per lane
-  uint16_t* rl_list = nullptr;  // (b << 8) | lane per contribution
-  bool rows_pass = true;        // block pass: row lanes (else owner lanes)
   double* qps = nullptr;        // (nel*4) x 14 point states (13 + pad, FP64 path)
   float* qps32 = nullptr;       // (nel*4) x 16, mixed path
   double* geo = nullptr;        // nel x 14: shape gradients, WdetJ (+ pad)
