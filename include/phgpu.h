@@ -174,6 +174,16 @@ int phg_sparse_ap_t(phg_ctx* ctx, const double* lhsK, double* p);
 int phg_i3lu(phg_ctx* ctx, double* Diag, double* r, const char* code);
 /* Spsi3pre(BDiag,lhsK,col,row)                      spsi3pre.f:1     */
 int phg_spsi3pre(phg_ctx* ctx, const double* BDiag, double* lhsK);
+/* phg_spsi3pre without allocation (no counterpart in the reference): the
+ * same result as phg_spsi3pre, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  lhsK is staged through a static 64 MiB device array
+ * instead of a buffer allocated on first use, so nnz_tot is limited to
+ * 335,544 blocks (25 doubles each); larger contexts get PHG_EINVAL.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (two
+ * host threads, two processes' contexts are separate) race.  FP64
+ * contexts only, as phg_spsi3pre. */
+int phg_spsi3pre_noalloc(phg_ctx* ctx, const double* BDiag, double* lhsK);
 /* bc3per(iBC,res,iper,ilwork,nQs)                   bc3per.f:1       */
 int phg_bc3per(phg_ctx* ctx, double* res, int nQs);
 

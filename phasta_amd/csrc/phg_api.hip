@@ -11,9 +11,43 @@
 
 #include "phg_internal.h"
 
+// Staging memory of phg_spsi3pre_noalloc (lhsK in the reference layout on
+// its way in and out), so that entry point calls no allocator.  Every GPU
+// the library runs on holds one copy, from the module load to the end of
+// the process.  Host code never passes `scratch` itself (its host-side name
+// is not a device address): set-up looks up each device's address with
+// hipGetSymbolAddress and the kernels get that pointer.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
 namespace {
 
 char g_err[1024] = "";
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch, looked up once per device.
+int scratch_address(void** out) {
+  *out = nullptr;
+  int dev = -1;
+  PHG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    phg_set_error("device %d outside the scratch table (%d devices)", dev, kMaxDevices);
+    return PHG_EINVAL;
+  }
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    PHG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+      phg_set_error("hipGetSymbolAddress(scratch) returned no address");
+      return PHG_EHIP;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
 
 template <typename T>
 int dev_alloc(phg_ctx* c, T** p, size_t count) {
@@ -888,6 +922,7 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
     if (hipEventCreate(&ev) != hipSuccess) ev = nullptr;
   }
   int rc = setup(c, mesh);
+  if (rc == 0) rc = scratch_address(&c->scratch);
   if (rc == 0) {
     for (auto ev : c->ev)
       if (!ev) rc = PHG_EHIP;
@@ -1142,6 +1177,31 @@ extern "C" int phg_spsi3pre(phg_ctx* c, const double* BDiag, double* lhsK) {
   PHG_TRY(launch_spsi3pre(c));
   PHG_TRY(launch_slots_to_csr(c, c->lhs64, c->tmp));
   PHG_TRY(download(c, lhsK, c->tmp, 25 * c->nnz));
+  return sync(c);
+}
+
+extern "C" int phg_spsi3pre_noalloc(phg_ctx* c, const double* BDiag, double* lhsK) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(require_fp64(c, "phg_spsi3pre_noalloc"));
+  const size_t need = sizeof(double) * 25 * (size_t)c->nnz;
+  if (need > kScratchBytes) {
+    phg_set_error("phg_spsi3pre_noalloc: lhsK of %lld blocks needs %zu bytes of staging, "
+                  "above the %zu-byte scratch (use phg_spsi3pre)",
+                  c->nnz, need, kScratchBytes);
+    return PHG_EINVAL;
+  }
+  if (!c->scratch) {
+    phg_set_error("phg_spsi3pre_noalloc: scratch address not set up");
+    return PHG_EINVAL;
+  }
+  // the same steps as phg_spsi3pre with the staging in the device's scratch
+  double* stage = static_cast<double*>(c->scratch);
+  PHG_TRY(upload(c, c->bdiag, BDiag, 25 * c->nshg));
+  PHG_TRY(upload(c, stage, lhsK, 25 * c->nnz));
+  PHG_TRY(launch_ref_to_aos(c, stage));
+  PHG_TRY(launch_spsi3pre(c));
+  PHG_TRY(launch_slots_to_csr(c, c->lhs64, stage));
+  PHG_TRY(download(c, lhsK, stage, 25 * c->nnz));
   return sync(c);
 }
 

@@ -194,6 +194,8 @@ struct phg_ctx {
   long long nlanes = 0;
   int4* lanes = nullptr;        // block-list offset, CSR entry, count, flags
   int* blocks = nullptr;        // e*16 + a*4 + b, grouped per owner lane
+  void* scratch = nullptr;      // this device's address of the library's
+                                // static staging array (phg_spsi3pre_noalloc)
   double* qps = nullptr;        // (nel*4) x 14 point states (13 + pad, FP64 path)
   float* qps32 = nullptr;       // (nel*4) x 16, mixed path
   double* geo = nullptr;        // nel x 14: shape gradients, WdetJ (+ pad)

@@ -75,7 +75,7 @@ EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_
            "phg_state_download_old", "phg_state_upload_cur", "phg_predict",
            "phg_newton_iteration", "phg_update", "phg_asiq", "phg_sparse_ap", "phg_sparse_ap_t",
            "phg_tangent_ap",
-           "phg_i3lu", "phg_spsi3pre", "phg_bc3per",
+           "phg_i3lu", "phg_spsi3pre", "phg_spsi3pre_noalloc", "phg_bc3per",
            "phg_bench_spmv", "phg_bench_assembly", "phg_bench_copy", "phg_sync",
            "phg_diagnostics",
            "phg_mpnn_param_count", "phg_mpnn_init", "phg_mpnn_set_weights",
