@@ -164,7 +164,10 @@ int phg_sparse_ap(phg_ctx* ctx, const double* lhsK, double* p);
  * by element without forming or storing lhsK: the unscaled operator of
  * SparseAp + bc3per.  The mixed-precision path forms its refinement
  * residual with it.  Matrix-free SolMFG (solmfg.f, au1mfg.f:56-78) instead
- * differences the residual at y + eps u; this is the exact product. */
+ * differences the residual at y + eps u; this is the exact product.  With
+ * iDC != 0 the DC term uses the DC factors of the context's last assembly
+ * (phg_elmgmrs / phg_solgmrs / phg_newton_iteration at the same state),
+ * as the tangent that assembly formed does; PHG_EINVAL before any. */
 int phg_tangent_ap(phg_ctx* ctx, const double* y, double* p);
 /* Transpose of SparseAp's operator, p <- K^T p with K p = bc3per(A p(iper))
  * (no counterpart in the reference: the learned preconditioner's training

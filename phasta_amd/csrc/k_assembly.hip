@@ -87,8 +87,11 @@ struct QpArgs {
   void* qps;               // (nel*4) x 13 point states (S)
   void* geo;               // nel x 13 (S)
   double* resid;           // (nel*4) x 5 residual of element e at its node a
+  double* dcw;             // nel*4 discontinuity-capturing factor per point (iDC != 0)
   long long nel;
   PhysParams ph;
+  double epsM;             // /precis/ epsM, regularises the DC norm
+  int idc;                 // iDC 1, 2, 3 (only read when the kernel has kDC)
 };
 
 __device__ inline double sel4(double v0, double v1, double v2, double v3, int i) {
@@ -172,11 +175,11 @@ __device__ inline void tet_shg(const double ix[3][3], double shg[4][3]) {
 }
 
 // metric tensor G = dxi/dx^T M dxi/dx with the tet invariance correction
-// (e3tau.f:1442-1474), its trace and G:G
-__device__ inline void tet_metric_tensor(const double ix[3][3], double G[6], double& trG,
-                                         double& GG) {
-  const double c1 = 1.259921049894873e+00, c2 = 6.299605249474365e-01;
-  double t[3][3];  // t[k][j] = sum_l M_kl ix[l][j], M = c2 + (c1-c2) I
+// (e3tau.f:1442-1474), its trace and G:G.  G's order: 11, 12, 22, 13, 23, 33.
+template <typename T>
+__device__ inline void tet_metric_tensor(const T ix[3][3], T G[6], T& trG, T& GG) {
+  const T c1 = (T)1.259921049894873e+00, c2 = (T)6.299605249474365e-01;
+  T t[3][3];  // t[k][j] = sum_l M_kl ix[l][j], M = c2 + (c1-c2) I
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     t[0][j] = c1 * ix[0][j] + c2 * (ix[1][j] + ix[2][j]);
@@ -190,7 +193,27 @@ __device__ inline void tet_metric_tensor(const double ix[3][3], double G[6], dou
   G[4] = ix[0][1] * t[0][2] + ix[1][1] * t[1][2] + ix[2][1] * t[2][2];
   G[5] = ix[0][2] * t[0][2] + ix[1][2] * t[1][2] + ix[2][2] * t[2][2];
   trG = G[0] + G[2] + G[5];
-  GG = G[0] * G[0] + G[2] * G[2] + G[5] * G[5] + 2.0 * (G[1] * G[1] + G[3] * G[3] + G[4] * G[4]);
+  GG = G[0] * G[0] + G[2] * G[2] + G[5] * G[5] + (T)2 * (G[1] * G[1] + G[3] * G[3] + G[4] * G[4]);
+}
+
+// giju, the inverse of the metric tensor G by cofactors (e3tau.f:248-268),
+// in the order 11, 22, 33, 12, 13, 23
+template <typename T>
+__device__ inline void metric_inverse(const T G[6], T gu[6]) {
+  const T m1 = G[0], m2 = G[2], m3 = G[5], m4 = G[1], m5 = G[3], m6 = G[4];
+  const T detI = (T)1 / (m1 * m2 * m3 - m1 * m6 * m6 - m4 * m4 * m3 + m4 * m5 * m6 * (T)2 -
+                         m5 * m5 * m2);
+  gu[0] = detI * (m2 * m3 - m6 * m6);
+  gu[1] = detI * (m1 * m3 - m5 * m5);
+  gu[2] = detI * (m1 * m2 - m4 * m4);
+  gu[3] = detI * (m5 * m6 - m4 * m3);
+  gu[4] = detI * (m4 * m6 - m5 * m2);
+  gu[5] = detI * (m4 * m5 - m1 * m6);
+}
+
+// giju row i as (column 0, 1, 2) indices into gu
+__device__ inline int gu_idx(int i, int j) {
+  return i == j ? i : (i + j == 1 ? 3 : (i + j == 2 ? 4 : 5));
 }
 
 // Thermodynamic state at a point from Y = (p, u, v, w, T) (getthm.f:106-172
@@ -226,12 +249,101 @@ __device__ inline void point_thermo(const double Y[5], const PhysParams& ph, con
   t.tau[2] = t.tau[1] / cv * ph.temper;
 }
 
+// Discontinuity-capturing factor at a point (iDC 1-3, ipord = 1): e3mtrx.f's
+// A0DC, A0inv and dV/dY of the state (ideal gas, ipress = 0), e3tau.f's rTLS
+// = L.(dV/dY tau L) over its upper triangle and raLS = L.A0inv L
+// (e3tau.f:198-247), and e3dc.f's norm of the state gradient in the metric
+// giju with A0DC (e3dc.f:118-260).  Lt: the least-squares residual before
+// tau, TL: after it, g: the Y gradients, gu: giju.
+__device__ inline double dc_factor(int idc, double epsM, const Thermo& t, double pres,
+                                   double Tq, double rk, double cp, const double Lt[5],
+                                   const double TL[5], const double g[3][5], const double gu[6]) {
+  const double rho = t.rho, u1 = t.u[0], u2 = t.u[1], u3 = t.u[2];
+  const double alfap = 1.0 / Tq, betaT = 1.0 / pres;
+  const double h = Tq * cp;
+  const double s1 = 1.0 / (rho * rho * betaT * Tq);
+  const double cvl = cp - (alfap * alfap * Tq / rho / betaT);
+  const double D0 = (rho * betaT) * (rho * betaT) * s1;
+  const double D1 = -rho * alfap * rho * betaT * s1;
+  const double D2 = rho / Tq;
+  const double D3 = (-rho * alfap) * (-rho * alfap) * s1 + (rho * cvl / (Tq * Tq));
+  // A0inv, symmetric: 11 22 33 44 55 12 13 14 15 23 24 25 34 35 45
+  const double fact1 = 1.0 / (rho * cvl * (Tq * Tq));
+  const double d = alfap * Tq / rho / betaT;
+  const double e1bar = h - rk, e2bar = e1bar - d, e3bar = e2bar - cvl * Tq;
+  const double e5bar = e1bar * e1bar - 2.0 * e1bar * d + 2.0 * rk * cvl * Tq + cp * Tq / rho / betaT;
+  double Ai[15];
+  Ai[0] = e5bar * fact1;
+  Ai[1] = (u1 * u1 + cvl * Tq) * fact1;
+  Ai[2] = (u2 * u2 + cvl * Tq) * fact1;
+  Ai[3] = (u3 * u3 + cvl * Tq) * fact1;
+  Ai[4] = fact1;
+  Ai[5] = u1 * e3bar * fact1;
+  Ai[6] = u2 * e3bar * fact1;
+  Ai[7] = u3 * e3bar * fact1;
+  Ai[8] = -e2bar * fact1;
+  Ai[9] = u1 * u2 * fact1;
+  Ai[10] = u3 * u1 * fact1;
+  Ai[11] = -u1 * fact1;
+  Ai[12] = u2 * u3 * fact1;
+  Ai[13] = -u2 * fact1;
+  Ai[14] = -u3 * fact1;
+  // dV/dY, upper triangle by columns: 11 12 22 13 23 33 14 24 34 44 15 25 35 45 55
+  const double f1 = 1.0 / Tq, f2 = f1 / Tq;
+  const double V0 = f1 / rho, V1 = -f1 * u1, V2 = f1, V3 = -f1 * u2, V5 = f1, V6 = -f1 * u3;
+  const double V9 = f1, V10 = -(h - rk) * f2, V11 = -f2 * u1, V12 = -f2 * u2, V13 = -f2 * u3,
+               V14 = f2;
+  const double* L = Lt;
+  const double* r = TL;
+  const double rTLS = L[0] * (r[0] * V0 + V1 * r[1] + V3 * r[2] + r[3] * V6 + V10 * r[4]) +
+                      L[1] * (r[1] * V2 + V11 * r[4]) + L[2] * (r[2] * V5 + V12 * r[4]) +
+                      L[3] * (r[3] * V9 + V13 * r[4]) + L[4] * (V14 * r[4]);
+  const double raLS = 2.0 * L[3] * L[4] * Ai[14] + 2.0 * L[2] * L[4] * Ai[13] +
+                      2.0 * L[0] * L[1] * Ai[5] + 2.0 * L[1] * L[2] * Ai[9] +
+                      2.0 * L[1] * L[3] * Ai[10] + 2.0 * L[0] * L[2] * Ai[6] +
+                      2.0 * L[2] * L[3] * Ai[12] + 2.0 * L[1] * L[4] * Ai[11] +
+                      2.0 * L[0] * L[3] * Ai[7] + 2.0 * L[0] * L[4] * Ai[8] + L[0] * L[0] * Ai[0] +
+                      L[1] * L[1] * Ai[1] + L[2] * L[2] * Ai[2] + L[3] * L[3] * Ai[3] +
+                      L[4] * L[4] * Ai[4];
+  // Y_,i A0DC Y_,j for (ij) = 11 22 33 12 13 23
+  double yAy[6];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double* a = g[j];
+    yAy[j] = D0 * (a[0] * a[0]) + 2.0 * a[0] * D1 * a[4] + D2 * (a[1] * a[1]) +
+             D2 * (a[2] * a[2]) + D2 * (a[3] * a[3]) + D3 * (a[4] * a[4]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double* a = g[k == 2 ? 1 : 0];
+    const double* b = g[k == 0 ? 1 : 2];
+    yAy[3 + k] = a[0] * D0 * b[0] + a[0] * D1 * b[4] + a[1] * D2 * b[1] + a[2] * D2 * b[2] +
+                 a[3] * D2 * b[3] + a[4] * D1 * b[0] + a[4] * D3 * b[4];
+  }
+  const double gnorm = 1.0 / (gu[0] * yAy[0] + 2.0 * gu[3] * yAy[3] + 2.0 * gu[4] * yAy[4] +
+                              gu[1] * yAy[1] + 2.0 * gu[5] * yAy[5] + gu[2] * yAy[2] + epsM);
+  if (idc == 1) return fmax(0.0, sqrt(raLS * gnorm) - rTLS * gnorm);
+  if (idc == 2) return 2.0 * rTLS * gnorm;
+  return fmin(fmax(0.0, sqrt(raLS * gnorm) - rTLS * gnorm), 2.0 * rTLS * gnorm);
+}
+
+// DC flux of direction i: DC sum_j giju(i,j) A0 v_,j (e3dc.f:262-300 with
+// ires = 1); A0g[j] = A0 v_,j
+__device__ inline void dc_flux(double DC, const double gu[6], const double A0g[3][5], int i,
+                               double out[5]) {
+  const double w0 = gu[gu_idx(i, 0)], w1 = gu[gu_idx(i, 1)], w2 = gu[gu_idx(i, 2)];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) out[f] = DC * (w0 * A0g[0][f] + w1 * A0g[1][f] + w2 * A0g[2][f]);
+}
+
 // ---------------- pass 1: point states and residual ----------------
 // S: storage type of the point states and geometry (double, or float for
 // the mixed-precision path whose tangent is formed in float).  kDiff: the
 // divergence of the reconstructed diffusive flux is subtracted from the
 // least-squares residual (idiff 1 or 3; e3ivar.f:361-395, e3ls.f:148-155).
-template <typename S, bool kDiff>
+// kDC: discontinuity capturing (iDC 1-3, e3.f:217-222): the DC flux joins
+// the diffusive flux and the point's factor is kept for the tangent pass.
+template <typename S, bool kDiff, bool kDC>
 __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   const long long e_raw = blockIdx.x * 64LL + threadIdx.x / 4;
   const int q = threadIdx.x & 3;
@@ -361,12 +473,22 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
       L[1 + m] -= dq;
     }
   }
+  double DC = 0.0, gu[6], A0g[3][5];
+  if constexpr (kDC) {
+    metric_inverse(G, gu);
+    const double TL[5] = {t.tau[0] * L[0], t.tau[1] * L[1], t.tau[1] * L[2], t.tau[1] * L[3],
+                          t.tau[2] * L[4]};
+    DC = dc_factor(A.idc, A.epsM, t, pres, Y[4], rk, cp, L, TL, g, gu);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A0_times(t, g[j], A0g[j]);
+    if (valid) A.dcw[(size_t)e * 4 + q] = DC;
+  }
   L[0] *= t.tau[0];
   L[1] *= t.tau[1];
   L[2] *= t.tau[1];
   L[3] *= t.tau[1];
   L[4] *= t.tau[2];
-  // per direction i: F_i = -F_i(conv) + F^v_i + A_i tau L
+  // per direction i: F_i = -F_i(conv) + F^v_i + A_i tau L (+ the DC flux)
   const double E = ei + rk;
   double F[3][5];
 #pragma unroll
@@ -384,6 +506,11 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
     Ai_times(t, i, L, tmp);
 #pragma unroll
     for (int f = 0; f < 5; ++f) F[i][f] += tmp[f];
+    if constexpr (kDC) {
+      dc_flux(DC, gu, A0g, i, tmp);
+#pragma unroll
+      for (int f = 0; f < 5; ++f) F[i][f] += tmp[f];
+    }
   }
   // contributions of point q to node a, summed over the element's four
   // lanes (q = 0..3); lane q keeps node a = q
@@ -429,11 +556,15 @@ struct JvArgs {
   const int* iper;         // 0-based master
   const uint8_t* bcm;
   double* out;             // (nel*4) x 5: element e's product at its node a
+  const double* dcw;       // nel*4 DC factor per point of the last assembly (kDC)
   long long nel, nshg;
   PhysParams ph;
   double cfac;             // almi/(gami alfi) Dtgl
 };
 
+// kDC: with discontinuity capturing the product adds the DC flux of v,
+// DC sum_j giju(i,j) A0 v_,j, with the point's factor of the last assembly.
+template <bool kDC>
 __global__ void __launch_bounds__(256) k_jv(JvArgs A) {
   const long long e_raw = blockIdx.x * 64LL + threadIdx.x / 4;
   const int q = threadIdx.x & 3;
@@ -516,6 +647,19 @@ __global__ void __launch_bounds__(256) k_jv(JvArgs A) {
     }
     F[i][4] += t.u[0] * tv[0] + t.u[1] * tv[1] + t.u[2] * tv[2] + kap * gv[i][4];
   }
+  if constexpr (kDC) {
+    const double DC = A.dcw[(size_t)e * 4 + q];
+    double gu[6], A0g[3][5], dv[5];
+    metric_inverse(G, gu);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A0_times(t, gv[j], A0g[j]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      dc_flux(DC, gu, A0g, i, dv);
+#pragma unroll
+      for (int f = 0; f < 5; ++f) F[i][f] += dv[f];
+    }
+  }
   double mine[5];
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
@@ -586,6 +730,7 @@ struct BlkArgs {
   const void* qps;
   const void* geo;
   const double* resid;
+  const double* dcw;       // nel*4 DC factor per point (kDC)
   double* res;             // 5 per node
   double* bdiag;           // 25 per node
   void* lhs;               // one padded block per CSR entry (T), lhsK order
@@ -596,8 +741,10 @@ struct BlkArgs {
 
 // T: stored block type and arithmetic type of the per-point tangent (float
 // on the mixed path); S: storage type of the point states.  Sums over
-// points and elements are taken in double.
-template <typename T, typename S>
+// points and elements are taken in double.  kDC: discontinuity capturing
+// adds WdetJ DC (s_a . giju s_b) A0 per point (e3dc.f:410-446 through
+// e3wmlt), with giju recomputed from the stored shape gradients.
+template <typename T, typename S, bool kDC>
 __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   // chunk partials of the 64 lanes: tangent sums in the tangent's type;
   // residual sums (double) in their own array on the float path, through
@@ -642,6 +789,13 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     // scaled here, so P's rows come out multiplied by WdetJ
     auto add = [&](int idx, R val) { acc[idx] += val; };
     R ub[3] = {(R)0, (R)0, (R)0};  // sum of the point velocities
+    // kDC: sum_q DC_q A0(q) through its distinct entries: drdp, drdT,
+    // drdp u (3), drdT u (3), rho, rho u (3), e1p, e4p
+    R dA[14];
+    if constexpr (kDC) {
+#pragma unroll
+      for (int i = 0; i < 14; ++i) dA[i] = (R)0;
+    }
     const S* qrec = qps + (size_t)e * 4 * rec_stride<S>();
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
@@ -654,6 +808,21 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
       ub[2] += u2;
       const R rho = (R)o[3], drdp = (R)o[4], drdT = (R)o[5];
       const R e1p = (R)o[6], e2p = (R)o[7], e3p = (R)o[8], e4p = (R)o[9];
+      if constexpr (kDC) {
+        const R dq = (R)A.dcw[(size_t)e * 4 + q];
+        const R dp = dq * drdp, dT = dq * drdT, dr = dq * rho;
+        dA[0] += dp;
+        dA[1] += dT;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+          dA[2 + n] += dp * uq[n];
+          dA[5 + n] += dT * uq[n];
+          dA[9 + n] += dr * uq[n];
+        }
+        dA[8] += dr;
+        dA[12] += dq * e1p;
+        dA[13] += dq * e4p;
+      }
       const R tau0 = (R)o[10], tau1 = (R)o[11], tau2 = (R)o[12];
       const R Na = W * (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
       const R Nb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
@@ -736,7 +905,33 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
         add(4 + 5 * (1 + n), W * (muR * (st * ub[n] + sR[n] * ubr) + lamR * ubs * rR[n]));
       add(24, W4 * kapR * st);
     }
-
+    if constexpr (kDC) {
+      // giju of the element from dxi/dx = 2 (shape gradients of nodes 1..3)
+      R ix[3][3], G[6], trG, GG, gu[6];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ix[k][c] = (R)2 * (R)go[3 * k + c];
+      tet_metric_tensor(ix, G, trG, GG);
+      metric_inverse(G, gu);
+      R sab = (R)0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sab += sR[i] * gu[gu_idx(i, j)] * rR[j];
+      const R c = W * sab;
+      add(0, c * dA[0]);
+      add(20, c * dA[1]);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        add(1 + m, c * dA[2 + m]);
+        add((1 + m) + 5 * (1 + m), c * dA[8]);
+        add((1 + m) + 20, c * dA[5 + m]);
+        add(4 + 5 * (1 + m), c * dA[9 + m]);
+      }
+      add(4, c * dA[12]);
+      add(24, c * dA[13]);
+    }
   }
   // the node's residual: element residuals of the diagonal blocks' elements
   if (diag) {
@@ -1336,16 +1531,35 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   Q.resid = c->resid;
   Q.nel = c->nel;
   Q.ph = ph;
+  const bool dc = c->prm.idc != 0;
+  if (dc && !c->dcw) {
+    phg_set_error("iDC %d: the DC factor buffer is not allocated", c->prm.idc);
+    return PHG_EINVAL;
+  }
+  Q.dcw = c->dcw;
+  Q.epsM = c->prm.epsM;
+  Q.idc = c->prm.idc;
   const int g1 = (int)((c->nel + 63) / 64);
   PHG_HIP(hipEventRecord(c->aev[0], c->stream));
-  if (mixed && diff)
-    k_asm_qp<float, true><<<g1, 256, 0, c->stream>>>(Q);
-  else if (mixed)
-    k_asm_qp<float, false><<<g1, 256, 0, c->stream>>>(Q);
-  else if (diff)
-    k_asm_qp<double, true><<<g1, 256, 0, c->stream>>>(Q);
-  else
-    k_asm_qp<double, false><<<g1, 256, 0, c->stream>>>(Q);
+  if (mixed) {
+    if (diff && dc)
+      k_asm_qp<float, true, true><<<g1, 256, 0, c->stream>>>(Q);
+    else if (diff)
+      k_asm_qp<float, true, false><<<g1, 256, 0, c->stream>>>(Q);
+    else if (dc)
+      k_asm_qp<float, false, true><<<g1, 256, 0, c->stream>>>(Q);
+    else
+      k_asm_qp<float, false, false><<<g1, 256, 0, c->stream>>>(Q);
+  } else {
+    if (diff && dc)
+      k_asm_qp<double, true, true><<<g1, 256, 0, c->stream>>>(Q);
+    else if (diff)
+      k_asm_qp<double, true, false><<<g1, 256, 0, c->stream>>>(Q);
+    else if (dc)
+      k_asm_qp<double, false, true><<<g1, 256, 0, c->stream>>>(Q);
+    else
+      k_asm_qp<double, false, false><<<g1, 256, 0, c->stream>>>(Q);
+  }
   PHG_HIP(hipGetLastError());
   PHG_HIP(hipEventRecord(c->aev[1], c->stream));
   // pass 2: block owners (res, BDiag, lhsK written once)
@@ -1357,6 +1571,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.qps = Q.qps;
   B.geo = Q.geo;
   B.resid = c->resid;
+  B.dcw = c->dcw;
   B.res = c->acc_res;
   B.bdiag = c->acc_bd;
   B.lhs = mixed ? (void*)c->lhs_aos32 : (void*)c->lhs_aos;
@@ -1370,10 +1585,14 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
-  if (mixed)
-    k_asm_blocks<float, float><<<g2, 64, 0, c->stream>>>(B);
+  if (mixed && dc)
+    k_asm_blocks<float, float, true><<<g2, 64, 0, c->stream>>>(B);
+  else if (mixed)
+    k_asm_blocks<float, float, false><<<g2, 64, 0, c->stream>>>(B);
+  else if (dc)
+    k_asm_blocks<double, double, true><<<g2, 64, 0, c->stream>>>(B);
   else
-    k_asm_blocks<double, double><<<g2, 64, 0, c->stream>>>(B);
+    k_asm_blocks<double, double, false><<<g2, 64, 0, c->stream>>>(B);
   PHG_HIP(hipGetLastError());
   PHG_HIP(hipEventRecord(c->aev[2], c->stream));
   // wall faces: residual only, added colour by colour
@@ -1405,11 +1624,21 @@ int launch_jv(phg_ctx* c, const double* v, double* w) {
   J.iper = c->iper;
   J.bcm = c->bcm;
   J.out = c->resid;
+  J.dcw = c->dcw;
   J.nel = c->nel;
   J.nshg = c->nshg;
   J.ph = ph;
   J.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
-  k_jv<<<(int)((c->nel + 63) / 64), 256, 0, c->stream>>>(J);
+  if (c->prm.idc != 0) {
+    if (!c->dcw) {
+      phg_set_error("tangent product with iDC %d: no assembly has formed the DC factors",
+                    c->prm.idc);
+      return PHG_EINVAL;
+    }
+    k_jv<true><<<(int)((c->nel + 63) / 64), 256, 0, c->stream>>>(J);
+  } else {
+    k_jv<false><<<(int)((c->nel + 63) / 64), 256, 0, c->stream>>>(J);
+  }
   PHG_HIP(hipGetLastError());
   const long long n = c->nshg;
   k_jv_nodes<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->inc_goff, c->inc, c->resid, v,

@@ -504,10 +504,12 @@ int check_idiff(phg_ctx* c) {
                   "reference never forms qres (elmgmr.f:361)", d);
     return PHG_EINVAL;
   }
-  if (c->prm.idc != 0) {
-    phg_set_error("iDC %d: discontinuity capturing is not built on the GPU path", c->prm.idc);
+  if (c->prm.idc < 0 || c->prm.idc > 3) {
+    // e3dc.f:246-260 defines DC for iDC 1, 2, 3 only
+    phg_set_error("iDC %d: 0 (off), 1, 2 or 3 supported", c->prm.idc);
     return PHG_EINVAL;
   }
+  if (c->prm.idc != 0 && !c->dcw) PHG_TRY(dev_alloc(c, &c->dcw, (size_t)c->nel * 4));
   if (d != 0 && !c->qnode) {
     PHG_TRY(dev_alloc(c, &c->qrec, (size_t)c->nel * 16));
     PHG_TRY(dev_alloc(c, &c->qnode, (size_t)c->nshg * 16));
@@ -953,7 +955,8 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
                   c->acold,    c->dy,       c->krylov, c->red_part, c->scal,
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
-                  c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode};
+                  c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode,
+                  c->dcw};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
