@@ -59,6 +59,10 @@ typedef struct phg_params {
   int idc;                              /* /solpar/ iDC: 0 off, 1 DC-mallet,
                                            2 DC-quadratic, 3 DC-minimum
                                            (input_fform.cc:683-686, e3dc.f) */
+  int itau;                             /* /genpar/ itau: 0 Shakib's diagonal
+                                           tau (e3tau.f:140-176), 10 the
+                                           matrix tau of e3tau_nd
+                                           (e3tau.f:284-744)                */
 } phg_params;
 
 /* Mesh arrays as the Fortran host holds them (readnblk.f, genadj.f);
@@ -243,7 +247,10 @@ int phg_mpnn_train_pair(phg_ctx* ctx, int slot, float lr, double* loss);
  * ||K z(v) - v||^2 / ||v||^2 with K = bc3per o SparseAp the scaled operator
  * of that solve (gradient through K^T, as phg_sparse_ap_t).  mean_loss and
  * grad (the gradient of the last vector's loss, phg_mpnn_param_count()
- * floats) are nullable. */
+ * floats) are nullable.  The vectors are the context's Krylov basis as it
+ * stands: host-array entry points that run in between (phg_sparse_ap,
+ * phg_sparse_ap_t, phg_tangent_ap) stage their arguments in its
+ * first vectors and the matrix they are given replaces K. */
 int phg_mpnn_train_krylov(phg_ctx* ctx, int nvec, float lr, double* mean_loss, float* grad);
 
 /* Diagnostics (no counterpart in the reference): PHG_DIAG_ORTH measures

@@ -358,6 +358,7 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
     if (!bc_supported(m->iBC[n])) return 3;
   if (P->idiff != 0 && P->idiff != 1 && P->idiff != 3) return 5;
   if (P->idc < 0 || P->idc > 3) return 5; /* input_fform.cc:683-686 */
+  if (P->itau != 0 && P->itau != 10) return 5; /* diagonal Shakib or e3tau_nd */
   /* elmgmr.f:361-391 global reconstruction of the diffusive flux */
   double* qres = NULL;
   if (P->idiff != 0) {

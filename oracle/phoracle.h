@@ -44,6 +44,8 @@ typedef struct orc_params {
   int iter;                             /* /timdat/ Newton iteration of the
                                            time step (itrdrv.f:409,434)    */
   int idc;                              /* /solpar/ iDC 0-3 (e3dc.f)       */
+  int itau;                             /* /genpar/ itau: 0 diagonal (Shakib),
+                                           10 matrix tau (e3tau_nd)         */
 } orc_params;
 
 typedef struct orc_stats {

@@ -123,6 +123,225 @@ static inline double rowdot(const double M[5][5], int r, const double v[5]) {
          M[r][4] * v[4];
 }
 
+/* sign(one, x) of the reference (IEEE sign bit of x) */
+static inline double fsign1(double x) { return copysign(ONE, x); }
+
+/* e3eig1.f:1-221 (lcsyst = 1): eigenvectors Q of the Euler Jacobian along
+ * the streamline, in entropy variables; rotates dxidx into the streamline
+ * frame in place (as the reference does, so later uses of dxidx in the same
+ * point see the rotated rows) and returns in a1..a3 the squared column norms
+ * of the rotated dxidx with the tet correction */
+static void e3eig1(const orc_params* P, double rho, double T, double cp, double gamb,
+                   double c, double u1, double u2, double u3, double* a1, double* a2,
+                   double* a3, double eb1, double dxidx[3][3], double u, double Q[5][5]) {
+  const double pt39 = 0.3968502629920498686879264098181;
+  const double sqt2 = 1.4142135623730950488016887242097;
+  double R[3][3], fact;
+  if (u != 0.0) {
+    fact = ONE / u;
+    R[0][0] = u1 * fact;
+    R[0][1] = u2 * fact;
+    R[0][2] = u3 * fact;
+  } else {
+    R[0][0] = ONE;
+    R[0][1] = 0.0;
+    R[0][2] = 0.0;
+  }
+  fact = *a1 * R[0][0] + *a2 * R[0][1] + *a3 * R[0][2];
+  *a1 = *a1 - fact * R[0][0];
+  *a2 = *a2 - fact * R[0][1];
+  *a3 = *a3 - fact * R[0][2];
+  fact = *a1 * *a1 + *a2 * *a2 + *a3 * *a3;
+  if (fact > P->epsM) {
+    R[1][0] = *a1;
+    R[1][1] = *a2;
+    R[1][2] = *a3;
+  } else {
+    const double sg = fsign1(R[0][1] * R[0][2]);
+    R[1][0] = R[0][1] + sg * R[0][2];
+    R[1][1] = -R[0][0] - sg * R[0][2];
+    R[1][2] = sg * (R[0][1] - R[0][0]);
+    fact = R[1][0] * R[1][0] + R[1][1] * R[1][1] + R[1][2] * R[1][2];
+  }
+  fact = ONE / sqrt(fact);
+  R[1][0] = R[1][0] * fact;
+  R[1][1] = R[1][1] * fact;
+  R[1][2] = R[1][2] * fact;
+  R[2][0] = R[0][1] * R[1][2] - R[0][2] * R[1][1];
+  R[2][1] = R[0][2] * R[1][0] - R[0][0] * R[1][2];
+  R[2][2] = R[0][0] * R[1][1] - R[0][1] * R[1][0];
+  for (int k = 0; k < 3; ++k) {
+    const double b1 = R[0][0] * dxidx[k][0] + R[0][1] * dxidx[k][1] + R[0][2] * dxidx[k][2];
+    const double b2 = R[1][0] * dxidx[k][0] + R[1][1] * dxidx[k][1] + R[1][2] * dxidx[k][2];
+    const double b3 = R[2][0] * dxidx[k][0] + R[2][1] * dxidx[k][1] + R[2][2] * dxidx[k][2];
+    dxidx[k][0] = b1;
+    dxidx[k][1] = b2;
+    dxidx[k][2] = b3;
+  }
+  double c1 = dxidx[0][0] * dxidx[0][0] + dxidx[1][0] * dxidx[1][0] + dxidx[2][0] * dxidx[2][0];
+  double c2 = dxidx[0][1] * dxidx[0][1] + dxidx[1][1] * dxidx[1][1] + dxidx[2][1] * dxidx[2][1];
+  double c3 = dxidx[0][2] * dxidx[0][2] + dxidx[1][2] * dxidx[1][2] + dxidx[2][2] * dxidx[2][2];
+  double s1 = dxidx[0][0] + dxidx[1][0] + dxidx[2][0];
+  double s2 = dxidx[0][1] + dxidx[1][1] + dxidx[2][1];
+  double s3 = dxidx[0][2] + dxidx[1][2] + dxidx[2][2];
+  *a1 = (c1 + s1 * s1) * pt39;
+  *a2 = (c2 + s2 * s2) * pt39;
+  *a3 = (c3 + s3 * s3) * pt39;
+  fact = (ONE / sqrt(TWO * rho * T)) / c;
+  Q[0][4] = fact * ((c + u) * c - eb1 * gamb);
+  Q[1][4] = -fact * (c + u * gamb) * R[0][0];
+  Q[2][4] = -fact * (c + u * gamb) * R[0][1];
+  Q[3][4] = -fact * (c + u * gamb) * R[0][2];
+  Q[4][4] = fact * gamb;
+  Q[0][0] = fact * ((c - u) * c - eb1 * gamb);
+  Q[1][0] = fact * (c - u * gamb) * R[0][0];
+  Q[2][0] = fact * (c - u * gamb) * R[0][1];
+  Q[3][0] = fact * (c - u * gamb) * R[0][2];
+  Q[4][0] = fact * gamb;
+  Q[0][2] = 0.0;
+  Q[1][2] = -fact * c * sqt2 * R[1][0];
+  Q[2][2] = -fact * c * sqt2 * R[1][1];
+  Q[3][2] = -fact * c * sqt2 * R[1][2];
+  Q[4][2] = 0.0;
+  Q[0][1] = 0.0;
+  Q[1][1] = fact * c * sqt2 * R[2][0];
+  Q[2][1] = fact * c * sqt2 * R[2][1];
+  Q[3][1] = fact * c * sqt2 * R[2][2];
+  Q[4][1] = 0.0;
+  fact = (ONE / sqrt(cp * rho)) / T;
+  Q[0][3] = fact * eb1;
+  Q[1][3] = fact * u * R[0][0];
+  Q[2][3] = fact * u * R[0][1];
+  Q[3][3] = fact * u * R[0][2];
+  Q[4][3] = -fact;
+}
+
+/* e3eig1.f:224-342 (e3eig2): eigenvalues of the rotated metric-weighted
+ * Jacobian, the (1,5) pair diagonalised by one Jacobi rotation of Q; with
+ * the time term (iremoveStabTimeTerm = 0) */
+static void e3eig2(const orc_params* P, double u, double c, double AR1, double AR2, double AR3,
+                   double rlam[5], double Q[5][5], double* eigmax) {
+  const double offd = (AR2 + AR3) * PT5 * (c * c);
+  rlam[0] = AR1 * ((u + c) * (u + c)) + offd;
+  rlam[1] = AR1 * (u * u) + AR3 * (c * c);
+  rlam[2] = AR1 * (u * u) + AR2 * (c * c);
+  rlam[3] = AR1 * (u * u);
+  rlam[4] = AR1 * ((u - c) * (u - c)) + offd;
+  const double tmp = P->dtsfct * 4.0 * (P->Dtgl * P->Dtgl);
+  for (int i = 0; i < 5; ++i) rlam[i] = rlam[i] + tmp;
+  double t = PT5 * (rlam[0] - rlam[4]) / offd;
+  t = fsign1(t) / (fabs(t) + sqrt(ONE + t * t));
+  const double Rc = ONE / sqrt(ONE + t * t);
+  const double Rs = t * Rc;
+  rlam[0] = rlam[0] - t * offd;
+  rlam[4] = rlam[4] + t * offd;
+  for (int r = 0; r < 5; ++r) {
+    const double tt = Rc * Q[r][0] - Rs * Q[r][4];
+    Q[r][4] = Rs * Q[r][0] + Rc * Q[r][4];
+    Q[r][0] = tt;
+  }
+  double m = rlam[0];
+  for (int i = 1; i < 5; ++i) m = fmax(m, rlam[i]);
+  *eigmax = m;
+}
+
+/* e3visc.f:143-268 (itau >= 10, lcsyst = 1): the compact diffusivity in
+ * local parent coordinates */
+static void e3compk(const double dxidx[3][3], double T, double rmu, double rlm,
+                    double rlm2mu, double con, double u1, double u2, double u3,
+                    double compK[10]) {
+  const double pt39 = 0.3968502629920498686879264098181;
+  const double (*d)[3] = dxidx;
+  double f1 = d[0][0] * d[0][0] + d[1][0] * d[1][0] + d[2][0] * d[2][0];
+  double f2 = d[0][0] * d[0][1] + d[1][0] * d[1][1] + d[2][0] * d[2][1];
+  double f3 = d[0][1] * d[0][1] + d[1][1] * d[1][1] + d[2][1] * d[2][1];
+  double f4 = d[0][0] * d[0][2] + d[1][0] * d[1][2] + d[2][0] * d[2][2];
+  double f5 = d[0][1] * d[0][2] + d[1][1] * d[1][2] + d[2][1] * d[2][2];
+  double f6 = d[0][2] * d[0][2] + d[1][2] * d[1][2] + d[2][2] * d[2][2];
+  const double s1 = d[0][0] + d[1][0] + d[2][0];
+  const double s2 = d[0][1] + d[1][1] + d[2][1];
+  const double s3 = d[0][2] + d[1][2] + d[2][2];
+  f1 = (f1 + s1 * s1) * pt39;
+  f2 = (f2 + s1 * s2) * pt39;
+  f3 = (f3 + s2 * s2) * pt39;
+  f4 = (f4 + s1 * s3) * pt39;
+  f5 = (f5 + s2 * s3) * pt39;
+  f6 = (f6 + s3 * s3) * pt39;
+  compK[0] = f1 * T * rlm2mu + f3 * T * rmu + f6 * T * rmu;
+  compK[1] = f2 * T * (rlm + rmu);
+  compK[2] = f1 * T * rmu + f3 * T * rlm2mu + f6 * T * rmu;
+  compK[3] = f4 * T * (rlm + rmu);
+  compK[4] = f5 * T * (rlm + rmu);
+  compK[5] = f1 * T * rmu + f3 * T * rmu + f6 * T * rlm2mu;
+  compK[6] = f1 * T * rlm2mu * u1 + f2 * T * (rlm + rmu) * u2 + f3 * T * rmu * u1 +
+             f4 * T * (rlm + rmu) * u3 + f6 * T * rmu * u1;
+  compK[7] = f1 * T * rmu * u2 + f2 * T * (rlm + rmu) * u1 + f3 * T * rlm2mu * u2 +
+             f5 * T * (rlm + rmu) * u3 + f6 * T * rmu * u2;
+  compK[8] = f1 * T * rmu * u3 + f3 * T * rmu * u3 + f4 * T * (rlm + rmu) * u1 +
+             f5 * T * (rlm + rmu) * u2 + f6 * T * rlm2mu * u3;
+  const double rk = PT5 * (u1 * u1 + u2 * u2 + u3 * u3);
+  compK[9] = f1 * T * (con * T + TWO * rmu * rk + (rlm + rmu) * (u1 * u1)) +
+             f2 * T * (rlm + rmu) * TWO * u1 * u2 +
+             f3 * T * (con * T + TWO * rmu * rk + (rlm + rmu) * (u2 * u2)) +
+             f4 * T * (rlm + rmu) * TWO * u1 * u3 + f5 * T * (rlm + rmu) * TWO * u2 * u3 +
+             f6 * T * (con * T + TWO * rmu * rk + (rlm + rmu) * (u3 * u3));
+}
+
+/* e3tau.f:284-518 (e3tau_nd, ipord = 1, Navier = 1): the matrix tau in
+ * primitive variables, PTau = dY/dV Q diag(lambda) Q^T; dxidx is rotated in
+ * place by e3eig1 */
+static void e3tau_nd(const orc_params* P, double rho, double cp, double T, double u1,
+                     double u2, double u3, double a1, double a2, double a3,
+                     double dxidx[3][3], const double compK[10], double PTau[5][5]) {
+  const double pt33 = 0.3333333333333333333333333333333;
+  const double gamb = P->gamma1;
+  double c = sqrt((P->gamma * P->Rgas) * T);
+  const double u = sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+  const double eb1 = cp * T - PT5 * (u1 * u1 + u2 * u2 + u3 * u3);
+  double Q[5][5], rlam[5], eigmax;
+  e3eig1(P, rho, T, cp, gamb, c, u1, u2, u3, &a1, &a2, &a3, eb1, dxidx, u, Q);
+  e3eig2(P, u, c, a1, a2, a3, rlam, Q, &eigmax);
+  for (int i = 0; i < 5; ++i) {
+    if (rlam[i] > (P->epsM * P->epsM) * eigmax)
+      rlam[i] = ONE / sqrt(rlam[i]);
+    else
+      rlam[i] = 0.0;
+  }
+  for (int i = 0; i < 5; ++i) {
+    const double q2 = Q[1][i], q3 = Q[2][i], q4 = Q[3][i], q5 = Q[4][i];
+    c = pt33 * (q2 * (compK[0] * q2 + compK[1] * q3 + compK[3] * q4 + compK[6] * q5) +
+                q3 * (compK[1] * q2 + compK[2] * q3 + compK[4] * q4 + compK[7] * q5) +
+                q4 * (compK[3] * q2 + compK[4] * q3 + compK[5] * q4 + compK[8] * q5) +
+                q5 * (compK[6] * q2 + compK[7] * q3 + compK[8] * q4 + compK[9] * q5));
+    const double Pe = ONE / (c * rlam[i]);
+    rlam[i] = rlam[i] * fmin(pt33 * Pe, ONE);
+  }
+  /* STau, symmetric, entropy variables: 11 12 22 13 23 33 14 24 34 44 15 25 35 45 55 */
+  static const int sr[15] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3, 0, 1, 2, 3, 4};
+  static const int sc[15] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4, 4};
+  double S[15];
+  for (int k = 0; k < 15; ++k) {
+    const int r = sr[k], cc = sc[k];
+    S[k] = rlam[0] * Q[r][0] * Q[cc][0] + rlam[1] * Q[r][1] * Q[cc][1] +
+           rlam[2] * Q[r][2] * Q[cc][2] + rlam[3] * Q[r][3] * Q[cc][3] +
+           rlam[4] * Q[r][4] * Q[cc][4];
+  }
+  /* PTau = dY/dV STau (G. Hauke's dY/dV) */
+  double b = cp * T + PT5 * (u1 * u1 + u2 * u2 + u3 * u3);
+  PTau[0][0] = rho * T * (S[0] + u1 * S[1] + u2 * S[3] + u3 * S[6] + b * S[10]);
+  PTau[0][1] = rho * T * (S[1] + u1 * S[2] + u2 * S[4] + u3 * S[7] + b * S[11]);
+  PTau[0][2] = rho * T * (S[3] + u1 * S[4] + u2 * S[5] + u3 * S[8] + b * S[12]);
+  PTau[0][3] = rho * T * (S[6] + u1 * S[7] + u2 * S[8] + u3 * S[9] + b * S[13]);
+  PTau[0][4] = rho * T * (S[10] + u1 * S[11] + u2 * S[12] + u3 * S[13] + b * S[14]);
+  const double uu[3] = {u1, u2, u3};
+  static const int col[3][5] = {{1, 2, 4, 7, 11}, {3, 4, 5, 8, 12}, {6, 7, 8, 9, 13}};
+  static const int last[5] = {10, 11, 12, 13, 14};
+  for (int m = 0; m < 3; ++m)
+    for (int j = 0; j < 5; ++j) PTau[1 + m][j] = T * (S[col[m][j]] + uu[m] * S[last[j]]);
+  b = T * T;
+  for (int j = 0; j < 5; ++j) PTau[4][j] = b * S[last[j]];
+}
+
 void orc_e3_q(const orc_params* P, const double yl[4][5],
               const double acl[4][5], const double xl[4][3],
               const double ql[4][12], double rl[4][5], double EG[20][20],
@@ -465,8 +684,16 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
       rLyi[3] = rLyi[3] - divqi[2];
       rLyi[4] = rLyi[4] - divqi[3];
     }
-    /* ---- e3tau.f:1-283 (itau = 0, ipord = 1) ---- */
+    /* ---- e3tau.f:1-283 (itau = 0, ipord = 1); e3tau_nd (itau >= 10) ---- */
     double gijd[6];
+    double PTau[5][5];
+    if (P->itau >= 10) {
+      /* compK is formed in e3visc from the unrotated dxidx (e3visc.f:143);
+       * e3tau_nd then rotates dxidx, and its DC giju uses the rotated rows */
+      double compK[10];
+      e3compk((const double(*)[3])dxidx, T, rmu, rlm, rlm2mu, con, u1, u2, u3, compK);
+      e3tau_nd(P, rho, cp, T, u1, u2, u3, aci[1], aci[2], aci[3], dxidx, compK, PTau);
+    }
     e3gijd(dxidx, gijd);
     const double fff = 36.0;
     double dts = P->dtsfct * P->Dtgl;
@@ -484,11 +711,20 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
     tau[2] = tau[1] / cv * P->temper;
     double Lt[5]; /* e3tau.f:188: rLyitemp, the LS residual before tau */
     for (int f = 0; f < 5; ++f) Lt[f] = rLyi[f];
-    rLyi[0] = tau[0] * rLyi[0];
-    rLyi[1] = tau[1] * rLyi[1];
-    rLyi[2] = tau[1] * rLyi[2];
-    rLyi[3] = tau[1] * rLyi[3];
-    rLyi[4] = tau[2] * rLyi[4];
+    if (P->itau >= 10) {
+      /* e3tau.f:633-643 rLyi = Tau rLyi */
+      for (int i = 0; i < 5; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < 5; ++j) acc = acc + PTau[i][j] * Lt[j];
+        rLyi[i] = acc;
+      }
+    } else {
+      rLyi[0] = tau[0] * rLyi[0];
+      rLyi[1] = tau[1] * rLyi[1];
+      rLyi[2] = tau[1] * rLyi[2];
+      rLyi[3] = tau[1] * rLyi[3];
+      rLyi[4] = tau[2] * rLyi[4];
+    }
     /* e3tau.f:198-268 (iDC != 0): rTLS = L.(dV/dY tau L), raLS = L.A0inv L
      * and giju, the inverse of the metric tensor */
     double rTLS = 0.0, raLS = 0.0, giju[6] = {0, 0, 0, 0, 0, 0};
@@ -531,12 +767,22 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
       for (int k = 0; k < 3; ++k) {
         const double (*Ak)[5] = (const double (*)[5])Ai[k];
         double Atau[5][5];
-        for (int i = 0; i < 5; ++i) {
-          Atau[i][0] = Ak[i][0] * tau[0];
-          Atau[i][1] = Ak[i][1] * tau[1];
-          Atau[i][2] = Ak[i][2] * tau[1];
-          Atau[i][3] = Ak[i][3] * tau[1];
-          Atau[i][4] = Ak[i][4] * tau[2];
+        if (P->itau >= 10) {
+          /* e3ls.f:477-486 Atau = A_k PTau */
+          for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j) {
+              double acc = 0.0;
+              for (int m = 0; m < 5; ++m) acc = acc + Ak[i][m] * PTau[m][j];
+              Atau[i][j] = acc;
+            }
+        } else {
+          for (int i = 0; i < 5; ++i) {
+            Atau[i][0] = Ak[i][0] * tau[0];
+            Atau[i][1] = Ak[i][1] * tau[1];
+            Atau[i][2] = Ak[i][2] * tau[1];
+            Atau[i][3] = Ak[i][3] * tau[1];
+            Atau[i][4] = Ak[i][4] * tau[2];
+          }
         }
         for (int j = 0; j < 5; ++j)
           for (int i = 0; i < 5; ++i)

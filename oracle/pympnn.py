@@ -43,11 +43,15 @@ def bf(a):
 
 
 def silu(u):
-    return u / (1.0 + np.exp(-u))
+    # exp(-u) overflows float32 for u < -88.7; the quotient then takes its
+    # exact limit (-0), so the overflow is not reported
+    with np.errstate(over="ignore"):
+        return u / (1.0 + np.exp(-u))
 
 
 def dsilu(u):
-    s = 1.0 / (1.0 + np.exp(-u))
+    with np.errstate(over="ignore"):
+        s = 1.0 / (1.0 + np.exp(-u))
     return s * (1.0 + u * (1.0 - s))
 
 

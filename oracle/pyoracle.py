@@ -27,7 +27,7 @@ class OrcParams(ctypes.Structure):
                 ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                  "almi", "alfi", "gami", "Dtgl", "etol", "epsM")] + \
                [(n, ctypes.c_int) for n in ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred",
-                                               "idiff", "iter", "idc")]
+                                               "idiff", "iter", "idc", "itau")]
 
 
 class OrcStats(ctypes.Structure):

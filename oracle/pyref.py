@@ -19,7 +19,8 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 PARAM_ORDER = ("Rgas", "gamma", "gamma1", "pr", "mu", "dtsfct", "taucfct", "temper",
                "almi", "alfi", "gami", "Dtgl", "etol", "epsM")
-IPARAM_ORDER = ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred", "idiff", "idc")
+IPARAM_ORDER = ("Kspace", "nGMRES", "minIters", "lhs", "iprec", "ipred", "idiff", "idc",
+                "itau")
 
 
 def available():

@@ -32,7 +32,7 @@ c-----------------------------------------------------------------------
       real(c_double) :: x_(nshg_,3), BC_(nshg_,ndofBC_), dprm(14)
       integer(c_int) :: ien_(nel_,4), ienb_(nelb_,4), iBC_(nshg_),
      &                  iper_(nshg_), colm_(nshg_+1), rowp_(nnz_),
-     &                  iprm(8)
+     &                  iprm(9)
       integer :: iblk, iel0, nprob, k, q, a, j
       real*8 :: qa4, qb4, qA, qB, qC, rr(4,3), r3(3,3), el(4)
       interface
@@ -89,6 +89,7 @@ c.... physics (/mmatpar/, /matdat/, /genpar/, /outpar/, /timdat/)
       ipred = iprm(6)
       idiff = iprm(7)
       iDC = iprm(8)
+      itau = iprm(9)
       idflx = 0
       if (idiff >= 1) idflx = (nflow-1) * nsd
       isurf = 0

@@ -105,16 +105,19 @@ def time_integration(rhoinf=-1.0, ipred=1):
 
 
 def solver_params(N, mach, kspace, etol=1e-3, min_iters=0, grade=1.0, cfl=CFL,
-                  rhoinf=-1.0, ipred=1, idiff=0, first=0.0, idc=0):
+                  rhoinf=-1.0, ipred=1, idiff=0, first=0.0, idc=0,
+                  itau=0):
     """The COMMON-block scalars the path reads, as a plain dict.  idiff:
     0, 1 (what "Include Viscous Correction in Stabilization: True" sets for
     linear elements, input_fform.cc:706-711) or 3: global reconstruction of
     the diffusive flux with row-sum or special lumping (elmgmr.f:361-391,
     e3q.f:185-205).  idc: discontinuity capturing, 0 off, 1 DC-mallet,
-    2 DC-quadratic, 3 DC-minimum (input_fform.cc:683-686, e3dc.f)."""
+    2 DC-quadratic, 3 DC-minimum (input_fform.cc:683-686, e3dc.f).  itau:
+    0 Shakib's diagonal tau, 10 the matrix tau of e3tau_nd ("Tau Matrix:
+    Matrix-Mallet", input_fform.cc)."""
     dt = time_step(N, mach, grade, cfl, first)
     return dict(Rgas=RGAS, gamma=GAMMA, gamma1=GAMMA1, pr=PRANDTL, mu=MU,
                 dtsfct=1.0, taucfct=1.0, temper=1.0, Dtgl=1.0 / dt,
                 etol=etol, epsM=EPSM, Kspace=kspace, nGMRES=1,
-                minIters=min_iters, lhs=1, iprec=1, idiff=idiff, idc=idc,
+                minIters=min_iters, lhs=1, iprec=1, idiff=idiff, idc=idc, itau=itau,
                 **time_integration(rhoinf, ipred))

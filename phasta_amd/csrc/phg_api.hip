@@ -510,6 +510,11 @@ int check_idiff(phg_ctx* c) {
     return PHG_EINVAL;
   }
   if (c->prm.idc != 0 && !c->dcw) PHG_TRY(dev_alloc(c, &c->dcw, (size_t)c->nel * 4));
+  if (c->prm.itau != 0) {
+    phg_set_error("itau %d: only Shakib's diagonal tau (itau 0) is built on the GPU path",
+                  c->prm.itau);
+    return PHG_EINVAL;
+  }
   if (d != 0 && !c->qnode) {
     PHG_TRY(dev_alloc(c, &c->qrec, (size_t)c->nel * 16));
     PHG_TRY(dev_alloc(c, &c->qnode, (size_t)c->nshg * 16));

@@ -125,7 +125,7 @@ program phg_boxdrv
   prm%Dtgl = 1.0d0 / dt
   prm%etol = 1.0d-3;  prm%epsM = sqrt(epsilon(1.0d0))
   prm%Kspace = kspace;  prm%nGMRES = 1;  prm%minIters = 0
-  prm%lhs = 1;  prm%iprec = 1;  prm%ipred = 1;  prm%idiff = 0;  prm%iter = 0;  prm%idc = 0
+  prm%lhs = 1;  prm%iprec = 1;  prm%ipred = 1;  prm%idiff = 0;  prm%iter = 0;  prm%idc = 0;  prm%itau = 0
 
   allocate(y(nshg,5), ac(nshg,5), yold(nshg,5), acold(nshg,5))
   allocate(res(nshg,5), rmes(nshg,5), BDiag(nshg,5,5), Dy(nshg,5))

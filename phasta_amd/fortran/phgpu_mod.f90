@@ -18,7 +18,7 @@ module phgpu
     real(c_double) :: etol, epsM
     integer(c_int) :: Kspace, nGMRES, minIters
     integer(c_int) :: lhs, iprec
-    integer(c_int) :: ipred, idiff, iter, idc
+    integer(c_int) :: ipred, idiff, iter, idc, itau
   end type
 
   type, bind(C) :: phg_mesh

@@ -112,6 +112,24 @@ def make_dc(N, seed):
     np.savez_compressed(os.path.join(HERE, f"ref_dc_box{N}.npz"), N=N, y=y, ac=ac, **out)
 
 
+def make_mtau(N, seed):
+    """ElmGMRs with the matrix tau of e3tau_nd (itau = 10: e3eig1/e3eig2
+    eigen-decomposition, e3visc's compK diffusion correction, dY/dV) alone
+    and with discontinuity capturing (iDC 1, whose giju then comes from the
+    rotated dxidx), on a jittered box at a perturbed state."""
+    case = mesh.box_case(N, mach=0.5, kspace=10, jitter=0.1)
+    rng = np.random.default_rng(seed)
+    y, ac = pyref.Reference(case).predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    out = {}
+    for tag, extra in (("", {}), ("_dc", {"idc": 1})):
+        prm = dict(case.params, itau=10, **extra)
+        res, rmes, bd, lk = pyref.Reference(case, params=prm).elmgmrs(y, ac)
+        out[f"res{tag}"], out[f"BDiag{tag}"], out[f"lhsK{tag}"] = res, bd, lk
+    np.savez_compressed(os.path.join(HERE, f"ref_mtau_box{N}.npz"), N=N, y=y, ac=ac, **out)
+
+
 if __name__ == "__main__":
     for N, seed in ((2, 11), (3, 12)):
         make(N, seed)
@@ -122,3 +140,5 @@ if __name__ == "__main__":
     print("wrote ref_idiff_box3.npz")
     make_dc(3, 15)
     print("wrote ref_dc_box3.npz")
+    make_mtau(3, 17)
+    print("wrote ref_mtau_box3.npz")

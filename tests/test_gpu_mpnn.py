@@ -1,12 +1,18 @@
 """Learned preconditioner kernels (k_mpnn.hip) against the CPU oracle of the
 network (oracle/pympnn.py), and the flexible-GMRES plumbing.
 
-Bars: the forward correction z - v and the gradients agree with the oracle's
-bf16 model to 3e-2 normwise (bf16 roundings of intermediate values can land
-on different sides of a tie when float sums are taken in another order; the
-oracle's own gradients are pinned by finite differences in float64).  A
-zero network makes FGMRES identical to block-diagonal GMRES.
+Inputs: the committed network on what the solver gives it, the block
+diagonal before factorisation, the preconditioned residual and the scaled
+solution of a GMRES solve on the 8^3 graded box (features of order one, no
+saturated activations).  Bars: the forward correction z - v and the
+gradients agree with the oracle's bf16 model to 1e-2 normwise (bf16
+roundings of intermediate values can land on different sides of a tie when
+float sums are taken in another order; the oracle's own gradients are
+pinned by finite differences in float64).  A zero network makes FGMRES
+identical to block-diagonal GMRES.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -15,7 +21,9 @@ import pyoracle
 from phasta_amd import mesh, solver
 
 pytestmark = pytest.mark.gpu
-TOL_NET = 3e-2
+TOL_NET = 1e-2
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "phasta_amd", "weights", "mpnn_box64.npz")
 
 
 def rel(a, b):
@@ -24,6 +32,30 @@ def rel(a, b):
 
 @pytest.fixture(scope="module")
 def net_case():
+    """The committed network and the pair of a block-diagonal solve."""
+    case = mesh.box_case(8, mach=0.3, kspace=10, grade=1.1)
+    print(f"[mpnn] box8 graded: nshg={case.nshg}", flush=True)
+    g = M.periodic_graph(case)
+    w = np.load(WEIGHTS)
+    p = solver.mpnn_unflatten(w["weights"])
+    dscale = np.asarray(w["dscale"], dtype=np.float32)
+    orc = pyoracle.Oracle(case)
+    y, ac = orc.itr_predict_bc(case.yold)
+    gpu = solver.GpuSolver(case)
+    gpu.mpnn_init(dscale)
+    gpu.mpnn_set_weights(w["weights"])
+    gpu.mpnn_enable(use_pc=0, keep_pair=True)
+    gpu.solgmrs(y, ac)
+    bd, v, t = gpu.mpnn_last_pair()
+    gpu.mpnn_enable(use_pc=0, keep_pair=False)
+    yield case, g, p, bd, v, t, dscale, gpu
+    gpu.close()
+
+
+@pytest.fixture(scope="module")
+def random_case():
+    """A fresh network with random biases and random inputs (the optimiser
+    test starts from a loss well above its minimum)."""
     case = mesh.box_case(4, mach=0.3, kspace=10)
     print(f"[mpnn] box4: nshg={case.nshg}", flush=True)
     g = M.periodic_graph(case)
@@ -72,8 +104,8 @@ def test_gradients_match_oracle(net_case):
         assert e < TOL_NET, k
 
 
-def test_adam_steps_reduce_the_loss(net_case):
-    case, g, p, bd, v, t, dscale, gpu = net_case
+def test_adam_steps_reduce_the_loss(random_case):
+    case, g, p, bd, v, t, dscale, gpu = random_case
     losses = [gpu.mpnn_train_step(bd, v, t, lr=2e-3) for _ in range(30)]
     print(f"[mpnn] loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
     assert losses[-1] < 0.7 * losses[0]
@@ -161,19 +193,22 @@ def test_residual_loss_matches_host_and_training_reduces_it():
     v = b / ||b||, formed on the device, equals the same quantity from the
     host-array entry points; its gradient matches the oracle's backward pass;
     Adam steps on it reduce it."""
-    case = mesh.box_case(4, mach=0.3, kspace=10)
+    case = mesh.box_case(8, mach=0.3, kspace=10, grade=1.1)
     orc = pyoracle.Oracle(case)
     y, ac = orc.itr_predict_bc(case.yold)
-    p = M.init_params(7)
-    p["Wout"] = (0.3 * np.random.default_rng(3).standard_normal(p["Wout"].shape)).astype(np.float32)
+    w = np.load(WEIGHTS)
+    p = solver.mpnn_unflatten(w["weights"])
+    dscale = np.asarray(w["dscale"], dtype=np.float32)
     gpu = solver.GpuSolver(case)
     try:
-        gpu.mpnn_init(np.ones(25, dtype=np.float32))
-        gpu.mpnn_set_weights(solver.mpnn_flatten(p))
+        gpu.mpnn_init(dscale)
+        gpu.mpnn_set_weights(w["weights"])
         gpu.mpnn_enable(use_pc=0, keep_pair=True)
         out = gpu.solgmrs(y, ac, want_lhs=True)      # lhsK: the scaled matrix K
         bd, b, _ = gpu.mpnn_last_pair()
-        loss_dev = gpu.mpnn_train_krylov(1, lr=0.0)
+        # loss and gradient on the solve's first Krylov vector, before the
+        # host-array entry points below overwrite the context's Krylov basis
+        loss_dev, grad = gpu.mpnn_train_krylov(1, lr=0.0, want_grad=True)
         v = b / np.linalg.norm(b)
         z = gpu.mpnn_apply(bd, v)
         kz = gpu.bc3per(gpu.sparse_ap(out["lhsK"], z))  # SparseAp, then bc3per
@@ -181,9 +216,8 @@ def test_residual_loss_matches_host_and_training_reduces_it():
         print(f"[mpnn] residual loss device {loss_dev:.6e} host {loss_host:.6e}", flush=True)
         assert abs(loss_dev - loss_host) <= 1e-6 * loss_host
         # its gradient: the oracle's backward pass from dL/dy = -2 K^T (v - K z) / s
-        _, grad = gpu.mpnn_train_krylov(1, lr=0.0, want_grad=True)
         g = M.periodic_graph(case)
-        x, s = M.features(bd, v, np.ones(25), g.active)
+        x, s = M.features(bd, v, dscale, g.active)
         dz = gpu.sparse_ap_t(out["lhsK"], v - kz)      # K^T (v - K z)
         dy = np.where(g.active[:, None], -2.0 * dz / s, 0.0)
         y_o, _ = M.forward(p, g, x)
@@ -195,11 +229,12 @@ def test_residual_loss_matches_host_and_training_reduces_it():
             e = rel(gg[k], gr[k])
             print(f"[mpnn] residual-loss grad {k} rel err {e:.2e}", flush=True)
             assert e < TOL_NET, k
-        # training on the Krylov vectors of this solve
+        # training on the Krylov vectors of this solve, at the step size the
+        # committed network was trained with
         gpu.solgmrs(y, ac)
         l0 = gpu.mpnn_train_krylov(out["iKs"] + 1, lr=0.0)
         for _ in range(20):
-            gpu.mpnn_train_krylov(out["iKs"] + 1, lr=2e-3)
+            gpu.mpnn_train_krylov(out["iKs"] + 1, lr=2e-5)
         l1 = gpu.mpnn_train_krylov(out["iKs"] + 1, lr=0.0)
         print(f"[mpnn] residual training {l0:.4e} -> {l1:.4e}", flush=True)
         assert l1 < l0
@@ -217,9 +252,11 @@ def test_trained_network_against_block_diagonal_gmres():
     The numbers are printed (the gain may be negative: the network trained
     on 64^3 boxes has not beaten block-diagonal GMRES here).  Asserted: both
     solves converge to the same tolerance (the learned one tested on its true
-    residual), and the fallback bounds the cost: at most one cycle of
-    Kspace directions more than block-diagonal GMRES."""
-    import os
+    residual), and what the acceptance test guarantees: a network whose first
+    direction is rejected (pc_fallback 3) costs no Krylov iteration over
+    block-diagonal GMRES, and an accepted one at most one cycle of Kspace
+    directions (the cycle is discarded when it does not reduce the true
+    residual)."""
     import time
     from phasta_amd import config
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -259,4 +296,7 @@ def test_trained_network_against_block_diagonal_gmres():
         gpu.close()
     for sb, sl in rows:
         assert sb["converged"] and sl["converged"]
-        assert sl["iKs"] <= sb["iKs"] + 20
+        if sl["pc_fallback"] == 3:
+            assert sl["iKs"] == sb["iKs"]
+        else:
+            assert sl["iKs"] <= sb["iKs"] + 20

@@ -293,6 +293,48 @@ def test_restart_file_resumes_the_device_state(tmp_path):
     assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
 
 
+def test_case_from_phasta_files_against_oracle(tmp_path):
+    """A case read from PHASTA files (a geombc and a restart written in the
+    phastaIO POSIX format, top wall moving at 10 m/s, 8^3 graded box),
+    stepped on the GPU (two time steps of two Newton iterations) against
+    the oracle on the same files: every update and the final state at
+    1e-12 with the same Krylov counts; the GPU's restart file matches the
+    oracle's under checkphasta's rule."""
+    from phasta_amd import phastaio
+    case = mesh.box_case(8, mach=0.3, kspace=10, grade=1.1)
+    top = np.isclose(case.x[:, 1], 1.0)
+    case.BC[top, 2] = 10.0
+    geo = tmp_path / phastaio.geombc_name()
+    rst = tmp_path / phastaio.restart_name(0)
+    phastaio.write_geombc(geo, case)
+    phastaio.write_restart(rst, case.yold, np.zeros_like(case.yold), 0)
+    c2 = phastaio.case_from_geombc(geo, case.params, restart=rst)
+    q, a, _ = phastaio.read_restart(rst)
+    yo_ref, aco_ref, hist = pyoracle.Oracle(c2).time_steps(q, a, steps=2, newton=2)
+    gpu = solver.GpuSolver(c2)
+    its = []
+    try:
+        gpu.upload_state(q, a)
+        for _ in range(2):
+            gpu.predict()
+            for _ in range(2):
+                its.append(gpu.newton_iteration()["iKs"])
+            gpu.update()
+        yo, aco = gpu.download_old()
+    finally:
+        gpu.close()
+    assert its == [h["iKs"] for h in hist]
+    assert rel(yo, yo_ref) < TOL and rel(aco, aco_ref) < TOL
+    assert np.allclose(yo[top, 0], 10.0)
+    r_gpu, r_orc = tmp_path / "gpu" / phastaio.restart_name(2), tmp_path / "orc" / phastaio.restart_name(2)
+    r_gpu.parent.mkdir()
+    r_orc.parent.mkdir()
+    phastaio.write_restart(r_gpu, yo, aco, 2)
+    phastaio.write_restart(r_orc, yo_ref, aco_ref, 2)
+    worst, ok = phastaio.compare_restarts([r_gpu], [r_orc])
+    assert ok and worst <= 1e-10 * np.max(np.abs(yo_ref))
+
+
 @pytest.mark.parametrize("N", [2, 3])
 def test_gpu_against_reference_golden(N):
     """GPU vs the reference's own compiled routines (committed fixtures)."""

@@ -495,3 +495,40 @@ def test_divergence_check_skips_the_first_iteration_of_a_time_step(box4):
     o2.reslast = quiet
     yo, aco, hist = o2.time_steps(box4.yold, acold, steps=1, newton=2)
     assert len(hist) == 2
+
+
+# ------------------------------------------------ matrix tau (itau = 10)
+def test_matrix_tau_elmgmrs_equals_reference_golden():
+    """ElmGMRs with the matrix tau of e3tau_nd (e3tau.f:284-744 with
+    e3eig1/e3eig2 and e3visc's compK), alone and with iDC 1, bitwise against
+    the reference's compiled routines (fixture tests/golden/ref_mtau_box3.npz);
+    the matrix tau changes the residual and the tangent; itau values the
+    build does not implement are refused."""
+    g = np.load(os.path.join(GOLD, "ref_mtau_box3.npz"))
+    case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+    y, ac = np.asfortranarray(g["y"]), np.asfortranarray(g["ac"])
+    base = pyoracle.Oracle(case).elmgmrs(y, ac)
+    for tag, extra in (("", {}), ("_dc", {"idc": 1})):
+        prm = dict(case.params, itau=10, **extra)
+        res, rmes, bd, lk = pyoracle.Oracle(case, params=prm).elmgmrs(y, ac)
+        assert np.array_equal(res, g[f"res{tag}"]), tag
+        assert np.array_equal(bd, g[f"BDiag{tag}"]), tag
+        assert np.array_equal(lk, g[f"lhsK{tag}"]), tag
+    assert rel(g["res"], base[0]) > 1e-2 and rel(g["lhsK"], base[3]) > 1e-2
+    for itau in (1, 11):
+        with pytest.raises(RuntimeError):
+            pyoracle.Oracle(case, params=dict(case.params, itau=itau)).elmgmrs(y, ac)
+
+
+@pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
+def test_matrix_tau_elmgmrs_bitwise_equals_live_reference():
+    case = mesh.box_case(4, mach=0.3, kspace=10, jitter=0.1, grade=1.1)
+    rng = np.random.default_rng(21)
+    y, ac = pyoracle.Oracle(case).itr_predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    for extra in ({}, {"idc": 2}, {"idiff": 1}):
+        prm = dict(case.params, itau=10, **extra)
+        for a, b in zip(pyoracle.Oracle(case, params=prm).elmgmrs(y, ac),
+                        pyref.Reference(case, params=prm).elmgmrs(y, ac)):
+            assert np.array_equal(a, b), extra
