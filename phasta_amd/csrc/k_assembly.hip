@@ -88,6 +88,7 @@ struct QpArgs {
   void* geo;               // nel x 13 (S)
   double* resid;           // (nel*4) x 5 residual of element e at its node a
   double* dcw;             // nel*4 discontinuity-capturing factor per point (iDC != 0)
+  void* mtau;              // nel*4 x mt_stride W PTau per point (itau = 10), S
   long long nel;
   PhysParams ph;
   double epsM;             // /precis/ epsM, regularises the DC norm
@@ -336,6 +337,195 @@ __device__ inline void dc_flux(double DC, const double gu[6], const double A0g[3
   for (int f = 0; f < 5; ++f) out[f] = DC * (w0 * A0g[0][f] + w1 * A0g[1][f] + w2 * A0g[2][f]);
 }
 
+// Matrix tau (itau = 10, e3tau.f:284-518 "e3tau_nd" with e3eig1.f's
+// e3eig1/e3eig2 and e3visc.f:143-268's compK, ipord = 1, Navier = 1, the
+// stabilisation time term kept): PTau = dY/dV Q diag(lambda) Q^T in the
+// primitive variables (p,u,v,w,T).  Q are the eigenvectors of the Euler
+// Jacobian in the streamline frame (entropy variables; the frame's second
+// axis is the acceleration projected off the velocity), lambda^-1/2 of the
+// metric-weighted eigenvalues (one Jacobi rotation diagonalises the
+// acoustic pair) reduced by the mode's Peclet number.  ix is rotated into
+// the streamline frame in place, as the reference rotates dxidx.
+__device__ inline void matrix_tau(const PhysParams& ph, double epsM, double rho, double cp,
+                                  double T, double u1, double u2, double u3, double a1,
+                                  double a2, double a3, double ix[3][3], const double compK[10],
+                                  double PT[5][5]) {
+  const double pt39 = 0.3968502629920498686879264098181;
+  const double sqt2 = 1.4142135623730950488016887242097;
+  const double pt33 = 0.3333333333333333333333333333333;
+  const double gamb = ph.gamma1;
+  double c = sqrt((ph.gamma * ph.Rgas) * T);
+  const double u = sqrt(u1 * u1 + u2 * u2 + u3 * u3);
+  const double eb1 = cp * T - 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+  // e3eig1: streamline frame R, rotated metric column norms AR, eigenvectors Q
+  double R[3][3], fact;
+  if (u != 0.0) {
+    fact = 1.0 / u;
+    R[0][0] = u1 * fact;
+    R[0][1] = u2 * fact;
+    R[0][2] = u3 * fact;
+  } else {
+    R[0][0] = 1.0;
+    R[0][1] = 0.0;
+    R[0][2] = 0.0;
+  }
+  fact = a1 * R[0][0] + a2 * R[0][1] + a3 * R[0][2];
+  a1 = a1 - fact * R[0][0];
+  a2 = a2 - fact * R[0][1];
+  a3 = a3 - fact * R[0][2];
+  fact = a1 * a1 + a2 * a2 + a3 * a3;
+  if (fact > epsM) {
+    R[1][0] = a1;
+    R[1][1] = a2;
+    R[1][2] = a3;
+  } else {
+    const double sg = copysign(1.0, R[0][1] * R[0][2]);
+    R[1][0] = R[0][1] + sg * R[0][2];
+    R[1][1] = -R[0][0] - sg * R[0][2];
+    R[1][2] = sg * (R[0][1] - R[0][0]);
+    fact = R[1][0] * R[1][0] + R[1][1] * R[1][1] + R[1][2] * R[1][2];
+  }
+  fact = 1.0 / sqrt(fact);
+  R[1][0] *= fact;
+  R[1][1] *= fact;
+  R[1][2] *= fact;
+  R[2][0] = R[0][1] * R[1][2] - R[0][2] * R[1][1];
+  R[2][1] = R[0][2] * R[1][0] - R[0][0] * R[1][2];
+  R[2][2] = R[0][0] * R[1][1] - R[0][1] * R[1][0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double b1 = R[0][0] * ix[k][0] + R[0][1] * ix[k][1] + R[0][2] * ix[k][2];
+    const double b2 = R[1][0] * ix[k][0] + R[1][1] * ix[k][1] + R[1][2] * ix[k][2];
+    const double b3 = R[2][0] * ix[k][0] + R[2][1] * ix[k][1] + R[2][2] * ix[k][2];
+    ix[k][0] = b1;
+    ix[k][1] = b2;
+    ix[k][2] = b3;
+  }
+  double AR[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double sq = ix[0][j] * ix[0][j] + ix[1][j] * ix[1][j] + ix[2][j] * ix[2][j];
+    const double sm = ix[0][j] + ix[1][j] + ix[2][j];
+    AR[j] = (sq + sm * sm) * pt39;
+  }
+  double Q[5][5];
+  fact = (1.0 / sqrt(2.0 * rho * T)) / c;
+  Q[0][4] = fact * ((c + u) * c - eb1 * gamb);
+  Q[0][0] = fact * ((c - u) * c - eb1 * gamb);
+  Q[4][4] = fact * gamb;
+  Q[4][0] = fact * gamb;
+  Q[0][1] = Q[0][2] = Q[4][1] = Q[4][2] = 0.0;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    Q[1 + n][4] = -fact * (c + u * gamb) * R[0][n];
+    Q[1 + n][0] = fact * (c - u * gamb) * R[0][n];
+    Q[1 + n][2] = -fact * c * sqt2 * R[1][n];
+    Q[1 + n][1] = fact * c * sqt2 * R[2][n];
+  }
+  fact = (1.0 / sqrt(cp * rho)) / T;
+  Q[0][3] = fact * eb1;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) Q[1 + n][3] = fact * u * R[0][n];
+  Q[4][3] = -fact;
+  // e3eig2
+  double lam[5];
+  const double offd = (AR[1] + AR[2]) * 0.5 * (c * c);
+  lam[0] = AR[0] * ((u + c) * (u + c)) + offd;
+  lam[1] = AR[0] * (u * u) + AR[2] * (c * c);
+  lam[2] = AR[0] * (u * u) + AR[1] * (c * c);
+  lam[3] = AR[0] * (u * u);
+  lam[4] = AR[0] * ((u - c) * (u - c)) + offd;
+  const double tt = ph.dtsfct * 4.0 * (ph.Dtgl * ph.Dtgl);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) lam[i] += tt;
+  double t = 0.5 * (lam[0] - lam[4]) / offd;
+  t = copysign(1.0, t) / (fabs(t) + sqrt(1.0 + t * t));
+  const double Rc = 1.0 / sqrt(1.0 + t * t), Rs = t * Rc;
+  lam[0] = lam[0] - t * offd;
+  lam[4] = lam[4] + t * offd;
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+    const double q0 = Rc * Q[r][0] - Rs * Q[r][4];
+    Q[r][4] = Rs * Q[r][0] + Rc * Q[r][4];
+    Q[r][0] = q0;
+  }
+  double emax = lam[0];
+#pragma unroll
+  for (int i = 1; i < 5; ++i) emax = fmax(emax, lam[i]);
+  // lambda^-1/2, then the diffusive (Peclet) reduction of every mode
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    lam[i] = lam[i] > (epsM * epsM) * emax ? 1.0 / sqrt(lam[i]) : 0.0;
+    const double q2 = Q[1][i], q3 = Q[2][i], q4 = Q[3][i], q5 = Q[4][i];
+    c = pt33 * (q2 * (compK[0] * q2 + compK[1] * q3 + compK[3] * q4 + compK[6] * q5) +
+                q3 * (compK[1] * q2 + compK[2] * q3 + compK[4] * q4 + compK[7] * q5) +
+                q4 * (compK[3] * q2 + compK[4] * q3 + compK[5] * q4 + compK[8] * q5) +
+                q5 * (compK[6] * q2 + compK[7] * q3 + compK[8] * q4 + compK[9] * q5));
+    const double Pe = 1.0 / (c * lam[i]);
+    lam[i] = lam[i] * fmin(pt33 * Pe, 1.0);
+  }
+  // STau = Q diag(lambda) Q^T (entropy variables), then dY/dV STau
+  double S[5][5];
+#pragma unroll
+  for (int r = 0; r < 5; ++r)
+#pragma unroll
+    for (int cc = r; cc < 5; ++cc) {
+      const double v = lam[0] * Q[r][0] * Q[cc][0] + lam[1] * Q[r][1] * Q[cc][1] +
+                       lam[2] * Q[r][2] * Q[cc][2] + lam[3] * Q[r][3] * Q[cc][3] +
+                       lam[4] * Q[r][4] * Q[cc][4];
+      S[r][cc] = v;
+      S[cc][r] = v;
+    }
+  const double b = cp * T + 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+  const double uu[3] = {u1, u2, u3};
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    PT[0][j] = rho * T * (S[0][j] + u1 * S[1][j] + u2 * S[2][j] + u3 * S[3][j] + b * S[4][j]);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) PT[1 + m][j] = T * (S[1 + m][j] + uu[m] * S[4][j]);
+    PT[4][j] = (T * T) * S[4][j];
+  }
+}
+
+// e3visc.f:143-268 (itau >= 10, tets): compact diffusivity in parent
+// coordinates from the (unrotated) inverse Jacobian ix
+__device__ inline void compact_k(const double ix[3][3], double T, double mu, double lam,
+                                 double kap, double u1, double u2, double u3, double K[10]) {
+  const double pt39 = 0.3968502629920498686879264098181;
+  double f[6];
+  const int pa[6] = {0, 0, 1, 0, 1, 2}, pb[6] = {0, 1, 1, 2, 2, 2};
+  double sm[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) sm[j] = ix[0][j] + ix[1][j] + ix[2][j];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int a = pa[k], bb = pb[k];
+    f[k] = (ix[0][a] * ix[0][bb] + ix[1][a] * ix[1][bb] + ix[2][a] * ix[2][bb] + sm[a] * sm[bb]) *
+           pt39;
+  }
+  const double l2m = lam + 2.0 * mu, lm = lam + mu;
+  K[0] = f[0] * T * l2m + f[2] * T * mu + f[5] * T * mu;
+  K[1] = f[1] * T * lm;
+  K[2] = f[0] * T * mu + f[2] * T * l2m + f[5] * T * mu;
+  K[3] = f[3] * T * lm;
+  K[4] = f[4] * T * lm;
+  K[5] = f[0] * T * mu + f[2] * T * mu + f[5] * T * l2m;
+  K[6] = f[0] * T * l2m * u1 + f[1] * T * lm * u2 + f[2] * T * mu * u1 + f[3] * T * lm * u3 +
+         f[5] * T * mu * u1;
+  K[7] = f[0] * T * mu * u2 + f[1] * T * lm * u1 + f[2] * T * l2m * u2 + f[4] * T * lm * u3 +
+         f[5] * T * mu * u2;
+  K[8] = f[0] * T * mu * u3 + f[2] * T * mu * u3 + f[3] * T * lm * u1 + f[4] * T * lm * u2 +
+         f[5] * T * l2m * u3;
+  const double rk = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+  K[9] = f[0] * T * (kap * T + 2.0 * mu * rk + lm * (u1 * u1)) + f[1] * T * lm * 2.0 * u1 * u2 +
+         f[2] * T * (kap * T + 2.0 * mu * rk + lm * (u2 * u2)) + f[3] * T * lm * 2.0 * u1 * u3 +
+         f[4] * T * lm * 2.0 * u2 * u3 + f[5] * T * (kap * T + 2.0 * mu * rk + lm * (u3 * u3));
+}
+
+// W PTau per point for the tangent pass: 25 values padded to 16-byte loads
+template <typename S>
+constexpr int mt_stride() { return sizeof(S) == 8 ? 26 : 28; }
+
 // ---------------- pass 1: point states and residual ----------------
 // S: storage type of the point states and geometry (double, or float for
 // the mixed-precision path whose tangent is formed in float).  kDiff: the
@@ -343,7 +533,9 @@ __device__ inline void dc_flux(double DC, const double gu[6], const double A0g[3
 // least-squares residual (idiff 1 or 3; e3ivar.f:361-395, e3ls.f:148-155).
 // kDC: discontinuity capturing (iDC 1-3, e3.f:217-222): the DC flux joins
 // the diffusive flux and the point's factor is kept for the tangent pass.
-template <typename S, bool kDiff, bool kDC>
+// kMT: the matrix tau of e3tau_nd (itau = 10) replaces Shakib's diagonal
+// tau; W PTau is kept for the tangent pass.
+template <typename S, bool kDiff, bool kDC, bool kMT>
 __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   const long long e_raw = blockIdx.x * 64LL + threadIdx.x / 4;
   const int q = threadIdx.x & 3;
@@ -437,6 +629,27 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
     o[11] = (S)(WdetJ * t.tau[1]);
     o[12] = (S)(WdetJ * t.tau[2]);
   }
+  double PT[5][5];
+  if constexpr (kMT) {
+    // e3visc's compK from the unrotated ix, then e3tau_nd (rotates a copy)
+    double K[10], ixr[3][3];
+    compact_k(ix, Y[4], mu, lam, kap, t.u[0], t.u[1], t.u[2], K);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ixr[i][j] = ix[i][j];
+    matrix_tau(ph, A.epsM, t.rho, cp, Y[4], t.u[0], t.u[1], t.u[2], aci[1], aci[2], aci[3], ixr,
+               K, PT);
+    if (valid) {
+      S* mo = reinterpret_cast<S*>(A.mtau) + ((size_t)e * 4 + q) * mt_stride<S>();
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) mo[5 * i + j] = (S)(WdetJ * PT[i][j]);
+#pragma unroll
+      for (int k = 25; k < mt_stride<S>(); ++k) mo[k] = (S)0;
+    }
+  }
 
   // time term A0 aci and the least-squares residual L = A_i Y,i + A0 Y,t
   double m0[5], L[5], tmp[5];
@@ -483,11 +696,22 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
     for (int j = 0; j < 3; ++j) A0_times(t, g[j], A0g[j]);
     if (valid) A.dcw[(size_t)e * 4 + q] = DC;
   }
-  L[0] *= t.tau[0];
-  L[1] *= t.tau[1];
-  L[2] *= t.tau[1];
-  L[3] *= t.tau[1];
-  L[4] *= t.tau[2];
+  if constexpr (kMT) {
+    // e3tau.f:633-643: tau L with the full matrix
+    double TL[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+      TL[i] = PT[i][0] * L[0] + PT[i][1] * L[1] + PT[i][2] * L[2] + PT[i][3] * L[3] +
+              PT[i][4] * L[4];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) L[i] = TL[i];
+  } else {
+    L[0] *= t.tau[0];
+    L[1] *= t.tau[1];
+    L[2] *= t.tau[1];
+    L[3] *= t.tau[1];
+    L[4] *= t.tau[2];
+  }
   // per direction i: F_i = -F_i(conv) + F^v_i + A_i tau L (+ the DC flux)
   const double E = ei + rk;
   double F[3][5];
@@ -560,11 +784,13 @@ struct JvArgs {
   long long nel, nshg;
   PhysParams ph;
   double cfac;             // almi/(gami alfi) Dtgl
+  double epsM;             // /precis/ epsM (kMT)
 };
 
 // kDC: with discontinuity capturing the product adds the DC flux of v,
 // DC sum_j giju(i,j) A0 v_,j, with the point's factor of the last assembly.
-template <bool kDC>
+// kMT: the matrix tau of the point (itau = 10) in place of the diagonal one.
+template <bool kDC, bool kMT>
 __global__ void __launch_bounds__(256) k_jv(JvArgs A) {
   const long long e_raw = blockIdx.x * 64LL + threadIdx.x / 4;
   const int q = threadIdx.x & 3;
@@ -575,7 +801,7 @@ __global__ void __launch_bounds__(256) k_jv(JvArgs A) {
   int nd[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) nd[k] = A.ien[k * nel + e];
-  double xl[4][3], yl[4][5];
+  double xl[4][3], yl[4][5], al[4][3];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const double2* v = reinterpret_cast<const double2*>(A.node + 16 * (size_t)nd[k]);
@@ -588,6 +814,12 @@ __global__ void __launch_bounds__(256) k_jv(JvArgs A) {
     xl[k][0] = x01.x;
     xl[k][1] = x01.y;
     xl[k][2] = x2.x;
+    if constexpr (kMT) {  // velocity accelerations: ac in local order at 5..9
+      const double2 a12 = v[3], a3 = v[4];
+      al[k][0] = a12.x;
+      al[k][1] = a12.y;
+      al[k][2] = a3.x;
+    }
   }
   double ix[3][3], det;
   tet_inverse(xl, ix, det);
@@ -634,6 +866,25 @@ __global__ void __launch_bounds__(256) k_jv(JvArgs A) {
   for (int f = 0; f < 5; ++f) L[f] += A.cfac * m0[f];
   double TL[5] = {t.tau[0] * L[0], t.tau[1] * L[1], t.tau[1] * L[2], t.tau[1] * L[3],
                   t.tau[2] * L[4]};
+  if constexpr (kMT) {
+    double acq[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) acq[c3] += Nq[k] * al[k][c3];
+    double K[10], ixr[3][3], PT[5][5];
+    compact_k(ix, Y[4], mu, lam, kap, t.u[0], t.u[1], t.u[2], K);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) ixr[i][j] = ix[i][j];
+    matrix_tau(ph, A.epsM, t.rho, cp, Y[4], t.u[0], t.u[1], t.u[2], acq[0], acq[1], acq[2], ixr, K,
+               PT);
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+      TL[i] = PT[i][0] * L[0] + PT[i][1] * L[1] + PT[i][2] * L[2] + PT[i][3] * L[3] +
+              PT[i][4] * L[4];
+  }
   const double divv = gv[0][1] + gv[1][2] + gv[2][3];
   double F[3][5];
 #pragma unroll
@@ -731,6 +982,7 @@ struct BlkArgs {
   const void* geo;
   const double* resid;
   const double* dcw;       // nel*4 DC factor per point (kDC)
+  const void* mtau;        // nel*4 x mt_stride W PTau per point (kMT), S
   double* res;             // 5 per node
   double* bdiag;           // 25 per node
   void* lhs;               // one padded block per CSR entry (T), lhsK order
@@ -743,8 +995,10 @@ struct BlkArgs {
 // on the mixed path); S: storage type of the point states.  Sums over
 // points and elements are taken in double.  kDC: discontinuity capturing
 // adds WdetJ DC (s_a . giju s_b) A0 per point (e3dc.f:410-446 through
-// e3wmlt), with giju recomputed from the stored shape gradients.
-template <typename T, typename S, bool kDC>
+// e3wmlt), with giju recomputed from the stored shape gradients.  kMT: the
+// matrix tau (itau = 10), P = A(s_a) (W PTau) + N_a I with the stored
+// W PTau (e3ls.f:477-486 Atau = A_i PTau).
+template <typename T, typename S, bool kDC, bool kMT>
 __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   // chunk partials of the 64 lanes: tangent sums in the tangent's type;
   // residual sums (double) in their own array on the float path, through
@@ -827,6 +1081,12 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
       const R Na = W * (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
       const R Nb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
       const R una = u0 * sR[0] + u1 * sR[1] + u2 * sR[2];
+      R WPT[25];
+      if constexpr (kMT) {
+        const S* mp = reinterpret_cast<const S*>(A.mtau) + ((size_t)e * 4 + q) * mt_stride<S>();
+#pragma unroll
+        for (int k = 0; k < 25; ++k) WPT[k] = (R)mp[k];
+      }
       const R unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
       const R w = unb + cfacR * Nb;
       // Q = A(s_b) + c N_b A0, held in factored form: its rows 1..3 are
@@ -846,7 +1106,31 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
 #pragma unroll
       for (int m = 0; m < 5; ++m) {
         R Pr[5];
-        if (m == 0) {
+        if constexpr (kMT) {
+          // row m of A(s_a), then P(m, :) = A(s_a)(m, :) W PTau + N_a e_m
+          R Am[5];
+          if (m == 0) {
+            Am[0] = drdp * una;
+#pragma unroll
+            for (int n = 0; n < 3; ++n) Am[1 + n] = rho * sR[n];
+            Am[4] = drdT * una;
+          } else if (m < 4) {
+            const int mm = m - 1;
+            Am[0] = drdp * uq[mm] * una + sR[mm];
+#pragma unroll
+            for (int n = 0; n < 3; ++n) Am[1 + n] = rho * (uq[mm] * sR[n] + (mm == n ? una : (R)0));
+            Am[4] = drdT * uq[mm] * una;
+          } else {
+            Am[0] = e2p * una;
+#pragma unroll
+            for (int n = 0; n < 3; ++n) Am[1 + n] = e3p * sR[n] + rho * uq[n] * una;
+            Am[4] = e4p * una;
+          }
+#pragma unroll
+          for (int j = 0; j < 5; ++j)
+            Pr[j] = Am[0] * WPT[j] + Am[1] * WPT[5 + j] + Am[2] * WPT[10 + j] +
+                    Am[3] * WPT[15 + j] + Am[4] * WPT[20 + j] + (m == j ? Na : (R)0);
+        } else if (m == 0) {
           Pr[0] = drdp * una * tau0 + Na;
 #pragma unroll
           for (int n = 0; n < 3; ++n) Pr[1 + n] = rho * sR[n] * tau1;
@@ -1472,6 +1756,24 @@ PhysParams phys_of(const phg_params& p) {
 
 }  // namespace
 
+// point pass for the flag set: diffusive-flux correction, DC, matrix tau
+// (matrix tau with DC is refused by the caller)
+template <typename S>
+void launch_qp(int g1, hipStream_t st, const QpArgs& Q, bool diff, bool dc, bool mt) {
+  if (mt && diff)
+    k_asm_qp<S, true, false, true><<<g1, 256, 0, st>>>(Q);
+  else if (mt)
+    k_asm_qp<S, false, false, true><<<g1, 256, 0, st>>>(Q);
+  else if (diff && dc)
+    k_asm_qp<S, true, true, false><<<g1, 256, 0, st>>>(Q);
+  else if (diff)
+    k_asm_qp<S, true, false, false><<<g1, 256, 0, st>>>(Q);
+  else if (dc)
+    k_asm_qp<S, false, true, false><<<g1, 256, 0, st>>>(Q);
+  else
+    k_asm_qp<S, false, false, false><<<g1, 256, 0, st>>>(Q);
+}
+
 int launch_fill(phg_ctx* c, void* p, size_t bytes) {
   size_t n = bytes / 4;
   int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
@@ -1536,30 +1838,22 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
     phg_set_error("iDC %d: the DC factor buffer is not allocated", c->prm.idc);
     return PHG_EINVAL;
   }
+  const bool mt = c->prm.itau == 10;
+  if ((mt && !c->mtau) || (mt && dc)) {
+    phg_set_error("itau %d with iDC %d: matrix tau needs its buffer and no DC", c->prm.itau,
+                  c->prm.idc);
+    return PHG_EINVAL;
+  }
   Q.dcw = c->dcw;
+  Q.mtau = c->mtau;
   Q.epsM = c->prm.epsM;
   Q.idc = c->prm.idc;
   const int g1 = (int)((c->nel + 63) / 64);
   PHG_HIP(hipEventRecord(c->aev[0], c->stream));
-  if (mixed) {
-    if (diff && dc)
-      k_asm_qp<float, true, true><<<g1, 256, 0, c->stream>>>(Q);
-    else if (diff)
-      k_asm_qp<float, true, false><<<g1, 256, 0, c->stream>>>(Q);
-    else if (dc)
-      k_asm_qp<float, false, true><<<g1, 256, 0, c->stream>>>(Q);
-    else
-      k_asm_qp<float, false, false><<<g1, 256, 0, c->stream>>>(Q);
-  } else {
-    if (diff && dc)
-      k_asm_qp<double, true, true><<<g1, 256, 0, c->stream>>>(Q);
-    else if (diff)
-      k_asm_qp<double, true, false><<<g1, 256, 0, c->stream>>>(Q);
-    else if (dc)
-      k_asm_qp<double, false, true><<<g1, 256, 0, c->stream>>>(Q);
-    else
-      k_asm_qp<double, false, false><<<g1, 256, 0, c->stream>>>(Q);
-  }
+  if (mixed)
+    launch_qp<float>(g1, c->stream, Q, diff, dc, mt);
+  else
+    launch_qp<double>(g1, c->stream, Q, diff, dc, mt);
   PHG_HIP(hipGetLastError());
   PHG_HIP(hipEventRecord(c->aev[1], c->stream));
   // pass 2: block owners (res, BDiag, lhsK written once)
@@ -1572,6 +1866,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.geo = Q.geo;
   B.resid = c->resid;
   B.dcw = c->dcw;
+  B.mtau = c->mtau;
   B.res = c->acc_res;
   B.bdiag = c->acc_bd;
   B.lhs = mixed ? (void*)c->lhs_aos32 : (void*)c->lhs_aos;
@@ -1585,14 +1880,18 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
-  if (mixed && dc)
-    k_asm_blocks<float, float, true><<<g2, 64, 0, c->stream>>>(B);
+  if (mixed && mt)
+    k_asm_blocks<float, float, false, true><<<g2, 64, 0, c->stream>>>(B);
+  else if (mixed && dc)
+    k_asm_blocks<float, float, true, false><<<g2, 64, 0, c->stream>>>(B);
   else if (mixed)
-    k_asm_blocks<float, float, false><<<g2, 64, 0, c->stream>>>(B);
+    k_asm_blocks<float, float, false, false><<<g2, 64, 0, c->stream>>>(B);
+  else if (mt)
+    k_asm_blocks<double, double, false, true><<<g2, 64, 0, c->stream>>>(B);
   else if (dc)
-    k_asm_blocks<double, double, true><<<g2, 64, 0, c->stream>>>(B);
+    k_asm_blocks<double, double, true, false><<<g2, 64, 0, c->stream>>>(B);
   else
-    k_asm_blocks<double, double, false><<<g2, 64, 0, c->stream>>>(B);
+    k_asm_blocks<double, double, false, false><<<g2, 64, 0, c->stream>>>(B);
   PHG_HIP(hipGetLastError());
   PHG_HIP(hipEventRecord(c->aev[2], c->stream));
   // wall faces: residual only, added colour by colour
@@ -1625,19 +1924,27 @@ int launch_jv(phg_ctx* c, const double* v, double* w) {
   J.bcm = c->bcm;
   J.out = c->resid;
   J.dcw = c->dcw;
+  J.epsM = c->prm.epsM;
   J.nel = c->nel;
   J.nshg = c->nshg;
   J.ph = ph;
   J.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
+  const int gj = (int)((c->nel + 63) / 64);
   if (c->prm.idc != 0) {
     if (!c->dcw) {
       phg_set_error("tangent product with iDC %d: no assembly has formed the DC factors",
                     c->prm.idc);
       return PHG_EINVAL;
     }
-    k_jv<true><<<(int)((c->nel + 63) / 64), 256, 0, c->stream>>>(J);
+    if (c->prm.itau == 10) {
+      phg_set_error("tangent product: matrix tau with DC is not built");
+      return PHG_EINVAL;
+    }
+    k_jv<true, false><<<gj, 256, 0, c->stream>>>(J);
+  } else if (c->prm.itau == 10) {
+    k_jv<false, true><<<gj, 256, 0, c->stream>>>(J);
   } else {
-    k_jv<false><<<(int)((c->nel + 63) / 64), 256, 0, c->stream>>>(J);
+    k_jv<false, false><<<gj, 256, 0, c->stream>>>(J);
   }
   PHG_HIP(hipGetLastError());
   const long long n = c->nshg;

@@ -510,10 +510,22 @@ int check_idiff(phg_ctx* c) {
     return PHG_EINVAL;
   }
   if (c->prm.idc != 0 && !c->dcw) PHG_TRY(dev_alloc(c, &c->dcw, (size_t)c->nel * 4));
-  if (c->prm.itau != 0) {
-    phg_set_error("itau %d: only Shakib's diagonal tau (itau 0) is built on the GPU path",
-                  c->prm.itau);
+  if (c->prm.itau != 0 && c->prm.itau != 10) {
+    phg_set_error("itau %d: Shakib's diagonal tau (0) and the matrix tau of e3tau_nd (10) "
+                  "are built", c->prm.itau);
     return PHG_EINVAL;
+  }
+  if (c->prm.itau == 10 && c->prm.idc != 0) {
+    // e3tau_nd rotates dxidx before e3dc forms giju from it, per point
+    phg_set_error("itau 10 with iDC %d is not built on the GPU path", c->prm.idc);
+    return PHG_EINVAL;
+  }
+  if (c->prm.itau == 10 && !c->mtau) {
+    // W PTau per point: 25 values padded to 28 floats or 26 doubles
+    const size_t rec = c->precision == PHG_MIXED ? 28 * sizeof(float) : 26 * sizeof(double);
+    uint8_t* p = nullptr;
+    PHG_TRY(dev_alloc(c, &p, (size_t)c->nel * 4 * rec));
+    c->mtau = p;
   }
   if (d != 0 && !c->qnode) {
     PHG_TRY(dev_alloc(c, &c->qrec, (size_t)c->nel * 16));
@@ -961,7 +973,7 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->acold,    c->dy,       c->krylov, c->red_part, c->scal,
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
                   c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode,
-                  c->dcw};
+                  c->dcw,      c->mtau};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);

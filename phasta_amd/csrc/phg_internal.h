@@ -202,6 +202,7 @@ struct phg_ctx {
   float* geo32 = nullptr;       // nel x 16
   double* resid = nullptr;      // (nel*4) x 5 element residual per node
   double* dcw = nullptr;        // nel*4 DC factor per point (allocated when iDC != 0)
+  void* mtau = nullptr;         // nel*4 W PTau per point (itau = 10), the tangent's type
   int* ienbc = nullptr;         // 4 planes of nelb, colour order, 0-based
   uint8_t* bcm = nullptr;       // constraint mask per node
   int* iper = nullptr;          // 0-based master

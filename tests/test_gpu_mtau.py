@@ -1,0 +1,135 @@
+"""Matrix tau (itau = 10: e3tau_nd with e3eig1/e3eig2's streamline
+eigen-decomposition, e3visc's compK Peclet reduction and dY/dV, e3tau.f:
+284-518; Atau = A_i PTau in the least-squares tangent, e3ls.f:477-486) on
+the GPU against the CPU oracle (bit for bit equal to the reference's
+compiled routines) and the reference fixture.
+
+* ElmGMRs res, BDiag, lhsK on 4^3, 16^3 and the 8^3 graded box at a
+  perturbed state, 1e-12; the matrix tau moves them by far more;
+* against the fixture of the reference's compiled ElmGMRs with itau = 10
+  (tests/golden/ref_mtau_box3.npz), 1e-12;
+* SolGMRs with the matrix tau: Dy at 1e-12 and the same Krylov count;
+* the FP64 tangent product (the mixed path's refinement operator) against
+  the oracle's SparseAp + bc3per on its assembled lhsK, 1e-12;
+* the mixed-precision path against FP64: residual 1e-13, tangent 2e-6;
+* itau 11 and the matrix tau combined with DC are refused on the GPU.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+from phasta_amd import mesh, solver
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module", params=[(4, 1.0), (16, 1.0), (8, 1.1)],
+                ids=["box4", "box16", "box8_graded"])
+def state(request):
+    N, grade = request.param
+    case = mesh.box_case(N, mach=0.5, kspace=10, jitter=0.1, grade=grade)
+    print(f"[mtau] box{N} grade {grade}: nshg={case.nshg}", flush=True)
+    rng = np.random.default_rng(100 + N)
+    y, ac = pyoracle.Oracle(case).itr_predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    return case, y, ac
+
+
+def test_mtau_elmgmrs_against_oracle(state):
+    case, y, ac = state
+    prm = dict(case.params, itau=10)
+    ref = pyoracle.Oracle(case, params=prm).elmgmrs(y, ac)
+    base = pyoracle.Oracle(case).elmgmrs(y, ac)
+    gpu = solver.GpuSolver(case, params=prm)
+    try:
+        out = gpu.elmgmrs(y, ac)
+    finally:
+        gpu.close()
+    for name, k in (("res", 0), ("BDiag", 2), ("lhsK", 3)):
+        e = rel(out[k], ref[k])
+        print(f"[mtau] {name}: GPU vs oracle {e:.2e}, matrix vs diagonal tau "
+              f"{rel(ref[k], base[k]):.1e}", flush=True)
+        assert e < TOL, name
+        assert rel(ref[k], base[k]) > 1e3 * TOL, name
+
+
+def test_mtau_elmgmrs_against_reference_golden():
+    g = np.load(os.path.join(GOLD, "ref_mtau_box3.npz"))
+    case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+    gpu = solver.GpuSolver(case, params=dict(case.params, itau=10))
+    try:
+        res, _, bd, lk = gpu.elmgmrs(np.asfortranarray(g["y"]), np.asfortranarray(g["ac"]))
+    finally:
+        gpu.close()
+    assert rel(res, g["res"]) < TOL
+    assert rel(bd, g["BDiag"]) < TOL
+    assert rel(lk, g["lhsK"]) < TOL
+
+
+def test_mtau_solgmrs_against_oracle(state):
+    case, y, ac = state
+    prm = dict(case.params, itau=10)
+    ref = pyoracle.Oracle(case, params=prm).solgmrs(y, ac)
+    gpu = solver.GpuSolver(case, params=prm)
+    try:
+        out = gpu.solgmrs(y, ac)
+    finally:
+        gpu.close()
+    assert out["iKs"] == ref["iKs"]
+    for k in ("res", "BDiag", "Dy"):
+        assert rel(out[k], ref[k]) < TOL, k
+
+
+@pytest.mark.parametrize("prec", [solver.PHG_FP64, solver.PHG_MIXED], ids=["fp64", "mixed"])
+def test_mtau_tangent_product_against_oracle(state, prec):
+    case, y, ac = state
+    prm = dict(case.params, itau=10)
+    _, _, _, lk = pyoracle.Oracle(case, params=prm).elmgmrs(y, ac)
+    orc = pyoracle.Oracle(case, params=prm)
+    gpu = solver.GpuSolver(case, params=prm, precision=prec)
+    try:
+        # the product reads the state's accelerations from the context
+        gpu.elmgmrs(y, ac)
+        p = np.asfortranarray(np.random.default_rng(3).standard_normal((case.nshg, 5)))
+        q = p.copy(order="F")
+        orc.sparse_ap(lk, q)
+        orc.bc3per(q)
+        assert rel(gpu.tangent_ap(y, p), q) < TOL
+    finally:
+        gpu.close()
+
+
+def test_mtau_mixed_against_fp64(state):
+    case, y, ac = state
+    prm = dict(case.params, itau=10)
+    g64 = solver.GpuSolver(case, params=prm)
+    gmx = solver.GpuSolver(case, params=prm, precision=solver.PHG_MIXED)
+    try:
+        r64, _, bd64, lk64 = g64.elmgmrs(y, ac)
+        rmx, _, bdmx, lkmx = gmx.elmgmrs(y, ac)
+    finally:
+        g64.close()
+        gmx.close()
+    assert rel(rmx, r64) < 1e-13
+    assert rel(bdmx, bd64) < 2e-6 and rel(lkmx, lk64) < 2e-6
+
+
+def test_mtau_flags_outside_the_build_are_refused(state):
+    case, y, ac = state
+    for extra in ({"itau": 11}, {"itau": 10, "idc": 1}):
+        gpu = solver.GpuSolver(case, params=dict(case.params, **extra))
+        try:
+            with pytest.raises(solver.PhgError) as ei:
+                gpu.elmgmrs(y, ac)
+            assert ei.value.code == 1
+        finally:
+            gpu.close()
