@@ -629,10 +629,10 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
     o[11] = (S)(WdetJ * t.tau[1]);
     o[12] = (S)(WdetJ * t.tau[2]);
   }
-  double PT[5][5];
+  double PT[5][5], ixr[3][3];
   if constexpr (kMT) {
     // e3visc's compK from the unrotated ix, then e3tau_nd (rotates a copy)
-    double K[10], ixr[3][3];
+    double K[10];
     compact_k(ix, Y[4], mu, lam, kap, t.u[0], t.u[1], t.u[2], K);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -688,13 +688,39 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   }
   double DC = 0.0, gu[6], A0g[3][5];
   if constexpr (kDC) {
-    metric_inverse(G, gu);
-    const double TL[5] = {t.tau[0] * L[0], t.tau[1] * L[1], t.tau[1] * L[2], t.tau[1] * L[3],
-                          t.tau[2] * L[4]};
+    double TL[5];
+    if constexpr (kMT) {
+      // with the matrix tau, giju comes from the inverse Jacobian rotated
+      // into the point's streamline frame (e3tau.f:683-710 after e3eig1)
+      double Gr[6], trGr, GGr;
+      tet_metric_tensor(ixr, Gr, trGr, GGr);
+      metric_inverse(Gr, gu);
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+        TL[i] = PT[i][0] * L[0] + PT[i][1] * L[1] + PT[i][2] * L[2] + PT[i][3] * L[3] +
+                PT[i][4] * L[4];
+    } else {
+      metric_inverse(G, gu);
+      TL[0] = t.tau[0] * L[0];
+      TL[1] = t.tau[1] * L[1];
+      TL[2] = t.tau[1] * L[2];
+      TL[3] = t.tau[1] * L[3];
+      TL[4] = t.tau[2] * L[4];
+    }
     DC = dc_factor(A.idc, A.epsM, t, pres, Y[4], rk, cp, L, TL, g, gu);
 #pragma unroll
     for (int j = 0; j < 3; ++j) A0_times(t, g[j], A0g[j]);
-    if (valid) A.dcw[(size_t)e * 4 + q] = DC;
+    if (valid) {
+      if constexpr (kMT) {
+        // the point's DC giju (6 values, padded to 8) for the tangent pass
+        double* d = A.dcw + ((size_t)e * 4 + q) * 8;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) d[k] = DC * gu[k];
+        d[6] = d[7] = 0.0;
+      } else {
+        A.dcw[(size_t)e * 4 + q] = DC;
+      }
+    }
   }
   if constexpr (kMT) {
     // e3tau.f:633-643: tau L with the full matrix
@@ -899,9 +925,17 @@ __global__ void __launch_bounds__(256) k_jv(JvArgs A) {
     F[i][4] += t.u[0] * tv[0] + t.u[1] * tv[1] + t.u[2] * tv[2] + kap * gv[i][4];
   }
   if constexpr (kDC) {
-    const double DC = A.dcw[(size_t)e * 4 + q];
-    double gu[6], A0g[3][5], dv[5];
-    metric_inverse(G, gu);
+    double DC, gu[6], A0g[3][5], dv[5];
+    if constexpr (kMT) {
+      // the point's DC giju of the last assembly (DC folded in)
+      const double* d = A.dcw + ((size_t)e * 4 + q) * 8;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) gu[k] = d[k];
+      DC = 1.0;
+    } else {
+      DC = A.dcw[(size_t)e * 4 + q];
+      metric_inverse(G, gu);
+    }
 #pragma unroll
     for (int j = 0; j < 3; ++j) A0_times(t, gv[j], A0g[j]);
 #pragma unroll
@@ -1063,7 +1097,20 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
       const R rho = (R)o[3], drdp = (R)o[4], drdT = (R)o[5];
       const R e1p = (R)o[6], e2p = (R)o[7], e3p = (R)o[8], e4p = (R)o[9];
       if constexpr (kDC) {
-        const R dq = (R)A.dcw[(size_t)e * 4 + q];
+        // weight of the point's A0: DC (the element's giju is applied after
+        // the loop), or with the matrix tau s_a . (DC giju_q) s_b of the
+        // point's rotated giju
+        R dq;
+        if constexpr (kMT) {
+          const double* d = A.dcw + ((size_t)e * 4 + q) * 8;
+          dq = (R)0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) dq += sR[i] * (R)d[gu_idx(i, j)] * rR[j];
+        } else {
+          dq = (R)A.dcw[(size_t)e * 4 + q];
+        }
         const R dp = dq * drdp, dT = dq * drdT, dr = dq * rho;
         dA[0] += dp;
         dA[1] += dT;
@@ -1190,19 +1237,22 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
       add(24, W4 * kapR * st);
     }
     if constexpr (kDC) {
-      // giju of the element from dxi/dx = 2 (shape gradients of nodes 1..3)
-      R ix[3][3], G[6], trG, GG, gu[6];
+      R sab = (R)1;
+      if constexpr (!kMT) {
+        // giju of the element from dxi/dx = 2 (shape gradients of nodes 1..3)
+        R ix[3][3], G[6], trG, GG, gu[6];
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) ix[k][c] = (R)2 * (R)go[3 * k + c];
-      tet_metric_tensor(ix, G, trG, GG);
-      metric_inverse(G, gu);
-      R sab = (R)0;
+          for (int c = 0; c < 3; ++c) ix[k][c] = (R)2 * (R)go[3 * k + c];
+        tet_metric_tensor(ix, G, trG, GG);
+        metric_inverse(G, gu);
+        sab = (R)0;
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) sab += sR[i] * gu[gu_idx(i, j)] * rR[j];
+          for (int j = 0; j < 3; ++j) sab += sR[i] * gu[gu_idx(i, j)] * rR[j];
+      }
       const R c = W * sab;
       add(0, c * dA[0]);
       add(20, c * dA[1]);
@@ -1757,10 +1807,13 @@ PhysParams phys_of(const phg_params& p) {
 }  // namespace
 
 // point pass for the flag set: diffusive-flux correction, DC, matrix tau
-// (matrix tau with DC is refused by the caller)
 template <typename S>
 void launch_qp(int g1, hipStream_t st, const QpArgs& Q, bool diff, bool dc, bool mt) {
-  if (mt && diff)
+  if (mt && diff && dc)
+    k_asm_qp<S, true, true, true><<<g1, 256, 0, st>>>(Q);
+  else if (mt && dc)
+    k_asm_qp<S, false, true, true><<<g1, 256, 0, st>>>(Q);
+  else if (mt && diff)
     k_asm_qp<S, true, false, true><<<g1, 256, 0, st>>>(Q);
   else if (mt)
     k_asm_qp<S, false, false, true><<<g1, 256, 0, st>>>(Q);
@@ -1839,9 +1892,8 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
     return PHG_EINVAL;
   }
   const bool mt = c->prm.itau == 10;
-  if ((mt && !c->mtau) || (mt && dc)) {
-    phg_set_error("itau %d with iDC %d: matrix tau needs its buffer and no DC", c->prm.itau,
-                  c->prm.idc);
+  if (mt && !c->mtau) {
+    phg_set_error("itau %d: the matrix tau buffer is not allocated", c->prm.itau);
     return PHG_EINVAL;
   }
   Q.dcw = c->dcw;
@@ -1880,12 +1932,16 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
-  if (mixed && mt)
+  if (mixed && mt && dc)
+    k_asm_blocks<float, float, true, true><<<g2, 64, 0, c->stream>>>(B);
+  else if (mixed && mt)
     k_asm_blocks<float, float, false, true><<<g2, 64, 0, c->stream>>>(B);
   else if (mixed && dc)
     k_asm_blocks<float, float, true, false><<<g2, 64, 0, c->stream>>>(B);
   else if (mixed)
     k_asm_blocks<float, float, false, false><<<g2, 64, 0, c->stream>>>(B);
+  else if (mt && dc)
+    k_asm_blocks<double, double, true, true><<<g2, 64, 0, c->stream>>>(B);
   else if (mt)
     k_asm_blocks<double, double, false, true><<<g2, 64, 0, c->stream>>>(B);
   else if (dc)
@@ -1936,11 +1992,10 @@ int launch_jv(phg_ctx* c, const double* v, double* w) {
                     c->prm.idc);
       return PHG_EINVAL;
     }
-    if (c->prm.itau == 10) {
-      phg_set_error("tangent product: matrix tau with DC is not built");
-      return PHG_EINVAL;
-    }
-    k_jv<true, false><<<gj, 256, 0, c->stream>>>(J);
+    if (c->prm.itau == 10)
+      k_jv<true, true><<<gj, 256, 0, c->stream>>>(J);
+    else
+      k_jv<true, false><<<gj, 256, 0, c->stream>>>(J);
   } else if (c->prm.itau == 10) {
     k_jv<false, true><<<gj, 256, 0, c->stream>>>(J);
   } else {

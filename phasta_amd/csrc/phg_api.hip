@@ -509,15 +509,23 @@ int check_idiff(phg_ctx* c) {
     phg_set_error("iDC %d: 0 (off), 1, 2 or 3 supported", c->prm.idc);
     return PHG_EINVAL;
   }
-  if (c->prm.idc != 0 && !c->dcw) PHG_TRY(dev_alloc(c, &c->dcw, (size_t)c->nel * 4));
+  if (c->prm.idc != 0) {
+    // DC factor per point; with the matrix tau DC giju per point (8 values)
+    const size_t need = (size_t)c->nel * 4 * (c->prm.itau == 10 ? 8 : 1);
+    if (c->dcw_len < need) {
+      if (c->dcw) {
+        PHG_HIP(hipStreamSynchronize(c->stream));
+        PHG_HIP(hipFree(c->dcw));
+        c->dev_bytes -= (long long)(sizeof(double) * c->dcw_len);
+        c->dcw = nullptr;
+      }
+      PHG_TRY(dev_alloc(c, &c->dcw, need));
+      c->dcw_len = need;
+    }
+  }
   if (c->prm.itau != 0 && c->prm.itau != 10) {
     phg_set_error("itau %d: Shakib's diagonal tau (0) and the matrix tau of e3tau_nd (10) "
                   "are built", c->prm.itau);
-    return PHG_EINVAL;
-  }
-  if (c->prm.itau == 10 && c->prm.idc != 0) {
-    // e3tau_nd rotates dxidx before e3dc forms giju from it, per point
-    phg_set_error("itau 10 with iDC %d is not built on the GPU path", c->prm.idc);
     return PHG_EINVAL;
   }
   if (c->prm.itau == 10 && !c->mtau) {

@@ -201,7 +201,9 @@ struct phg_ctx {
   double* geo = nullptr;        // nel x 14: shape gradients, WdetJ (+ pad)
   float* geo32 = nullptr;       // nel x 16
   double* resid = nullptr;      // (nel*4) x 5 element residual per node
-  double* dcw = nullptr;        // nel*4 DC factor per point (allocated when iDC != 0)
+  double* dcw = nullptr;        // DC per point (iDC != 0): nel*4 factors, or nel*4 x 8
+                                // DC giju with the matrix tau
+  size_t dcw_len = 0;           // doubles allocated in dcw
   void* mtau = nullptr;         // nel*4 W PTau per point (itau = 10), the tangent's type
   int* ienbc = nullptr;         // 4 planes of nelb, colour order, 0-based
   uint8_t* bcm = nullptr;       // constraint mask per node

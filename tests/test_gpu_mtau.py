@@ -12,7 +12,10 @@ compiled routines) and the reference fixture.
 * the FP64 tangent product (the mixed path's refinement operator) against
   the oracle's SparseAp + bc3per on its assembled lhsK, 1e-12;
 * the mixed-precision path against FP64: residual 1e-13, tangent 2e-6;
-* itau 11 and the matrix tau combined with DC are refused on the GPU.
+* the matrix tau with discontinuity capturing (iDC 1, 3), whose DC metric
+  giju comes from each point's streamline-rotated inverse Jacobian: against
+  the oracle and the reference fixture, 1e-12, and its tangent product;
+* itau 11 is refused on the GPU.
 """
 import os
 
@@ -65,14 +68,36 @@ def test_mtau_elmgmrs_against_oracle(state):
 def test_mtau_elmgmrs_against_reference_golden():
     g = np.load(os.path.join(GOLD, "ref_mtau_box3.npz"))
     case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
-    gpu = solver.GpuSolver(case, params=dict(case.params, itau=10))
+    for tag, extra in (("", {}), ("_dc", {"idc": 1})):
+        gpu = solver.GpuSolver(case, params=dict(case.params, itau=10, **extra))
+        try:
+            res, _, bd, lk = gpu.elmgmrs(np.asfortranarray(g["y"]), np.asfortranarray(g["ac"]))
+        finally:
+            gpu.close()
+        assert rel(res, g[f"res{tag}"]) < TOL, tag
+        assert rel(bd, g[f"BDiag{tag}"]) < TOL, tag
+        assert rel(lk, g[f"lhsK{tag}"]) < TOL, tag
+
+
+@pytest.mark.parametrize("idc", [1, 3])
+def test_mtau_with_dc_against_oracle(state, idc):
+    case, y, ac = state
+    prm = dict(case.params, itau=10, idc=idc)
+    orc = pyoracle.Oracle(case, params=prm)
+    ref = orc.elmgmrs(y, ac)
+    gpu = solver.GpuSolver(case, params=prm)
     try:
-        res, _, bd, lk = gpu.elmgmrs(np.asfortranarray(g["y"]), np.asfortranarray(g["ac"]))
+        out = gpu.elmgmrs(y, ac)
+        p = np.asfortranarray(np.random.default_rng(4).standard_normal((case.nshg, 5)))
+        q = p.copy(order="F")
+        orc.sparse_ap(ref[3], q)
+        orc.bc3per(q)
+        e_ap = rel(gpu.tangent_ap(y, p), q)
     finally:
         gpu.close()
-    assert rel(res, g["res"]) < TOL
-    assert rel(bd, g["BDiag"]) < TOL
-    assert rel(lk, g["lhsK"]) < TOL
+    for name, k in (("res", 0), ("BDiag", 2), ("lhsK", 3)):
+        assert rel(out[k], ref[k]) < TOL, name
+    assert e_ap < TOL
 
 
 def test_mtau_solgmrs_against_oracle(state):
@@ -125,7 +150,7 @@ def test_mtau_mixed_against_fp64(state):
 
 def test_mtau_flags_outside_the_build_are_refused(state):
     case, y, ac = state
-    for extra in ({"itau": 11}, {"itau": 10, "idc": 1}):
+    for extra in ({"itau": 11}, {"itau": 1}):
         gpu = solver.GpuSolver(case, params=dict(case.params, **extra))
         try:
             with pytest.raises(solver.PhgError) as ei:
