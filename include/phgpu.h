@@ -155,6 +155,50 @@ int phg_newton_iteration(phg_ctx* ctx, phg_stats* st);
  *                                   itrPC.f:176-208, itrdrv.f:651-652 */
 int phg_update(phg_ctx* ctx);
 
+/* Matrix-free GMRES (Solver Type: GMRES Matrix Free, impl = 20; selected at
+ * itrdrv.f:489-506 with lhs = 0, iprec from LHSupd).  PHG_FP64 contexts,
+ * lhs = 0, iDC = 0 and itau = 0 (PHG_EINVAL otherwise); idiff 0, 1, 3.
+ *
+ * ElmMFG(y,ac,x,shp,shgl,iBC,BC,shpb,shglb,res,rmes,BDiag,iper,ilwork,rerr)
+ *                                                     elmmfg.f:1
+ * res as ElmGMRs forms it, rmes the modified residual (e3 with ires = 3),
+ * BDiag (iprec != 0) the block diagonal of e3bdg.f; no lhsK is formed. */
+int phg_elmmfg(phg_ctx* ctx, const double* y, const double* ac, double* res,
+               double* rmes, double* BDiag);
+/* ItrRes(yp,yc,x,shp,shgl,iBC,BC,shpb,shglb,rmes,iper,ilwork,ac)
+ *                                                     itrres.f:1
+ * rmes (in/out, accumulated as the reference does) += the modified residual
+ * of the variables yp (nshg,5, global order) at the state yc, interior
+ * elements only, then bc3Res; iabres = 1 takes |.| per element node
+ * (asires.f:90, the /precis/ flag itrFDI sets). */
+int phg_itrres(phg_ctx* ctx, const double* yp, const double* yc,
+               const double* ac, double* rmes, int iabres);
+/* Au1MFG(ypre,y,ac,x,rmes,res,uBrg,BDiag,iBC,BC,iper,ilwork,shp,shgl,shpb,
+ *        shglb)                                       au1mfg.f:1
+ * u <- (L^-1 Rmod(itrBC(old2new(U^-1 (ypre + eGMRES u)))) - rmes) / eGMRES:
+ * the one-sided finite-difference product of SolMFG's Krylov loop, at the
+ * state y.  ypre (nshg,5) = U new2old(y) and u in the equation order, rmes
+ * forward-reduced, BDiag LU-factored (as SolMFG holds them). */
+int phg_au1mfg(phg_ctx* ctx, const double* ypre, const double* y,
+               const double* ac, const double* rmes, const double* BDiag,
+               double eGMRES, double* u);
+/* SolMFG(y,ac,yold,acold,x,iBC,BC,res,BDiag,HBrg,eBrg,yBrg,Rcos,Rsin,iper,
+ *        ilwork,shp,shgl,shpb,shglb,Dy,rerr)          solmfg.f:1
+ * ElmMFG, then GMRES on finite-difference products of the modified residual
+ * (Au1MFG, au1mfg.f; Au2MFG's two-sided difference for the restarts,
+ * au2mfg.f).  eGMRES: the /itrpar/ finite-difference interval, in/out; with
+ * fdi = 1 itrFDI recomputes it first (itrfdi.f; the reference does so when
+ * iter = 1 and mod(istep, 20) = 0, solmfg.f:163).  Returns res
+ * preconditioned, rmes forward-reduced, BDiag LU-factored, Dy the increment
+ * consumed by itrCorrect; st->iKs is the last cycle's Krylov count. */
+int phg_solmfg(phg_ctx* ctx, const double* y, const double* ac, double* res,
+               double* rmes, double* BDiag, double* Dy, double* eGMRES, int fdi,
+               phg_stats* st);
+/* SolMFG + itrCorrect + itrBC on the resident state (itrdrv.f:489-599);
+ * eGMRES as phg_solmfg (NULL: the context's interval). */
+int phg_newton_iteration_mfg(phg_ctx* ctx, double* eGMRES, int fdi,
+                             phg_stats* st);
+
 /* Fine-grained entry points (unit parity; host arrays in/out). */
 /* AsIq over the interior elements + qpbc, the global reconstruction of the
  * diffusive flux for idiff 1 or 3          elmgmr.f:361-391, asiq.f:1,
@@ -177,7 +221,8 @@ int phg_tangent_ap(phg_ctx* ctx, const double* y, double* p);
  * (no counterpart in the reference: the learned preconditioner's training
  * differentiates through the matvec). */
 int phg_sparse_ap_t(phg_ctx* ctx, const double* lhsK, double* p);
-/* i3LU(Diag,r,code), code 'LU_Fact ','forward ','backward' i3lu.f:1   */
+/* i3LU(Diag,r,code), code 'LU_Fact ','forward ','backward','product '
+ *                                                   i3lu.f:1         */
 int phg_i3lu(phg_ctx* ctx, double* Diag, double* r, const char* code);
 /* Spsi3pre(BDiag,lhsK,col,row)                      spsi3pre.f:1     */
 int phg_spsi3pre(phg_ctx* ctx, const double* BDiag, double* lhsK);

@@ -864,6 +864,477 @@ int orc_solgmrs(const orc_mesh* m, const orc_params* P, const double* y,
   return 0;
 }
 
+/* ---------------- matrix-free GMRES: SolMFG and its helpers ---------------- */
+
+/* bc3res.f:25-163 on one residual-type array */
+static void mfg_bc3res(const orc_mesh* m, double* res) {
+  const long long nshg = m->nshg;
+  for (long long n = 0; n < nshg; ++n) {
+    int ibc = m->iBC[n];
+    if (ibc & 4) res[n] = 0.0;
+    if (vel_code(ibc) == 7) {
+      res[n + nshg] = 0.0;
+      res[n + 2 * nshg] = 0.0;
+      res[n + 3 * nshg] = 0.0;
+    }
+    if (ibc & 2) res[n + 4 * nshg] = 0.0;
+  }
+  for (long long j = 0; j < nshg; ++j)
+    if (m->iBC[j] & 1024) {
+      long long i = m->iper[j] - 1;
+      for (int f = 0; f < 5; ++f) {
+        res[i + nshg * f] = res[i + nshg * f] + res[j + nshg * f];
+        res[j + nshg * f] = 0.0;
+      }
+    }
+}
+
+/* bc3bdg.f:25-352 */
+static void mfg_bc3bdg(const orc_mesh* m, double* BDiag) {
+  const long long nshg = m->nshg;
+  for (long long n = 0; n < nshg; ++n) {
+    int ibc = m->iBC[n];
+    int cons[5] = {(ibc & 4) != 0, vel_code(ibc) == 7, vel_code(ibc) == 7, vel_code(ibc) == 7,
+                   (ibc & 2) != 0};
+    for (int d = 0; d < 5; ++d) {
+      if (!cons[d]) continue;
+      for (int o = 0; o < 5; ++o) {
+        BDiag[n + nshg * (d + 5 * o)] = 0.0;
+        BDiag[n + nshg * (o + 5 * d)] = 0.0;
+      }
+      BDiag[n + nshg * (d + 5 * d)] = ONE;
+    }
+  }
+  for (long long j = 0; j < nshg; ++j)
+    if (m->iBC[j] & 1024) {
+      long long i = m->iper[j] - 1;
+      for (int jk = 0; jk < 25; ++jk) BDiag[i + nshg * jk] = BDiag[i + nshg * jk] + BDiag[j + nshg * jk];
+    }
+  for (long long j = 0; j < nshg; ++j)
+    if (m->iBC[j] & 1024) {
+      long long i = m->iper[j] - 1;
+      for (int jk = 0; jk < 25; ++jk) BDiag[j + nshg * jk] = BDiag[i + nshg * jk];
+    }
+}
+
+/* localy.f:53-73 gather of node g in the local order (p,u,v,w,T) */
+static void gather_local5(const double* y, long long nshg, long long g, double v[5]) {
+  v[1] = y[g];
+  v[2] = y[g + nshg];
+  v[3] = y[g + 2 * nshg];
+  v[0] = y[g + 3 * nshg];
+  v[4] = y[g + 4 * nshg];
+}
+
+/* AsIMFG (asimfg.f:43-110, ires = 3) or AsIRes (asires.f:29-94, ires = 2)
+ * over the interior elements, ibksiz at a time, scattered in the
+ * reference's order (local.f:67-74).  yp: e3's yl, y: e3's ycl (the same
+ * array for AsIMFG).  res / BDiag only for ires = 3, rmes always; iabres:
+ * |rml| per element before the scatter (asires.f:90, itrFDI's cancellation
+ * estimate). */
+static int mfg_interior(const orc_mesh* m, const orc_params* P, const double* yp,
+                        const double* y, const double* ac, const double* qres, int ires,
+                        int iabres, double* res, double* rmes, double* BDiag) {
+  const long long nshg = m->nshg, nel = m->nel;
+  const int bk = m->ibksiz > 0 ? m->ibksiz : 64;
+  const int bdg = ires == 3 && P->iprec == 1 && P->lhs != 1; /* e3.f:261 */
+  enum { kSuper = 64 };
+  const long long cap = (long long)kSuper * bk;
+  double(*rl)[4][5] = malloc(sizeof(double) * 20 * cap);
+  double(*rml)[4][5] = malloc(sizeof(double) * 20 * cap);
+  double(*BDl)[4][25] = bdg ? malloc(sizeof(double) * 100 * cap) : NULL;
+  int(*nod)[4] = malloc(sizeof(int) * 4 * cap);
+  if (!rl || !rml || (bdg && !BDl) || !nod) {
+    free(rl);
+    free(rml);
+    free(BDl);
+    free(nod);
+    return 4;
+  }
+  for (long long s0 = 0; s0 < nel; s0 += cap) {
+    const long long ns = (nel - s0) < cap ? (nel - s0) : cap;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long e = 0; e < ns; ++e) {
+      double ylp[4][5], yl[4][5], acl[4][5], xl[4][3], qll[4][12];
+      for (int a = 0; a < 4; ++a) {
+        long long g = m->ien[(s0 + e) + nel * a] - 1;
+        nod[e][a] = (int)g;
+        gather_local5(yp, nshg, g, ylp[a]);
+        gather_local5(y, nshg, g, yl[a]);
+        gather_local5(ac, nshg, g, acl[a]);
+        for (int c = 0; c < 3; ++c) xl[a][c] = m->x[g + nshg * c];
+        if (qres)
+          for (int k = 0; k < 12; ++k) qll[a][k] = qres[g + nshg * k];
+      }
+      orc_e3_mfg(P, (const double(*)[5])ylp, (const double(*)[5])yl, (const double(*)[5])acl,
+                 (const double(*)[3])xl, qres ? (const double(*)[12])qll : NULL, ires, rl[e],
+                 rml[e], bdg ? BDl[e] : NULL);
+      if (iabres)
+        for (int a = 0; a < 4; ++a)
+          for (int f = 0; f < 5; ++f) rml[e][a][f] = fabs(rml[e][a][f]);
+    }
+    for (long long e0 = 0; e0 < ns; e0 += bk) {
+      const int npro = (int)((ns - e0) < bk ? (ns - e0) : bk);
+      if (ires == 3)
+        for (int f = 0; f < 5; ++f)
+          for (int i = 0; i < 4; ++i)
+            for (int e = (int)e0; e < e0 + npro; ++e)
+              res[nod[e][i] + nshg * f] = res[nod[e][i] + nshg * f] + rl[e][i][f];
+      for (int f = 0; f < 5; ++f)
+        for (int i = 0; i < 4; ++i)
+          for (int e = (int)e0; e < e0 + npro; ++e)
+            rmes[nod[e][i] + nshg * f] = rmes[nod[e][i] + nshg * f] + rml[e][i][f];
+      if (bdg)
+        for (int jk = 0; jk < 25; ++jk)
+          for (int i = 0; i < 4; ++i)
+            for (int e = (int)e0; e < e0 + npro; ++e)
+              BDiag[nod[e][i] + nshg * jk] = BDiag[nod[e][i] + nshg * jk] + BDl[e][i][jk];
+    }
+  }
+  free(rl);
+  free(rml);
+  free(BDl);
+  free(nod);
+  return 0;
+}
+
+static int mfg_flags(const orc_mesh* m, const orc_params* P) {
+  for (long long n = 0; n < m->nshg; ++n)
+    if (!bc_supported(m->iBC[n])) return 3;
+  if (P->idiff != 0 && P->idiff != 1 && P->idiff != 3) return 5;
+  /* e3dc's ires = 2 branch reads giju, rTLS and raLS, which e3tau forms only
+   * for ires 1 and 3 (e3tau.f:185-262); e3bdg_nd (itau >= 10) is not
+   * restated */
+  if (P->idc != 0 || P->itau != 0) return 5;
+  return 0;
+}
+
+int orc_elmmfg(const orc_mesh* m, const orc_params* P, const double* y, const double* ac,
+               double* res, double* rmes, double* BDiag) {
+  const long long nshg = m->nshg, nelb = m->nelb;
+  int rc = mfg_flags(m, P);
+  if (rc) return rc;
+  double* qres = NULL;
+  if (P->idiff != 0) { /* elmmfg.f:68-124 */
+    qres = malloc(sizeof(double) * 13 * nshg);
+    if (!qres) return 4;
+    rc = orc_asiq(m, P, y, qres, qres + 12 * nshg);
+    if (rc) {
+      free(qres);
+      return rc;
+    }
+  }
+  /* elmmfg.f:128-132 */
+  memset(res, 0, sizeof(double) * nshg * 5);
+  memset(rmes, 0, sizeof(double) * nshg * 5);
+  if (P->iprec != 0) memset(BDiag, 0, sizeof(double) * 25 * nshg);
+  rc = mfg_interior(m, P, y, y, ac, qres, 3, 0, res, rmes, BDiag);
+  free(qres);
+  if (rc) return rc;
+  /* AsBMFG (elmmfg.f:178-226): the wall flux enters res only, e3b adds to
+   * rmes only with Jactyp = 1 (e3b.f:296-297) and itrSetup sets Jactyp = 0
+   * (itrPC.f:29) */
+  const int bk = m->ibksiz > 0 ? m->ibksiz : 64;
+  double(*rl)[4][5] = malloc(sizeof(double) * 20 * bk);
+  int(*nod)[4] = malloc(sizeof(int) * 4 * bk);
+  if (!rl || !nod) {
+    free(rl);
+    free(nod);
+    return 4;
+  }
+  for (long long e0 = 0; e0 < nelb; e0 += bk) {
+    int npro = (int)((nelb - e0) < bk ? (nelb - e0) : bk);
+    for (int e = 0; e < npro; ++e) {
+      double yl[4][5], xl[4][3];
+      for (int a = 0; a < 4; ++a) {
+        long long g = m->ienb[(e0 + e) + nelb * a] - 1;
+        nod[e][a] = (int)g;
+        gather_local5(y, nshg, g, yl[a]);
+        for (int c = 0; c < 3; ++c) xl[a][c] = m->x[g + nshg * c];
+      }
+      orc_e3b(P, (const double(*)[5])yl, (const double(*)[3])xl, rl[e]);
+    }
+    for (int f = 0; f < 5; ++f)
+      for (int i = 0; i < 4; ++i)
+        for (int e = 0; e < npro; ++e)
+          res[nod[e][i] + nshg * f] = res[nod[e][i] + nshg * f] + rl[e][i][f];
+  }
+  free(rl);
+  free(nod);
+  /* elmmfg.f:247-256 */
+  mfg_bc3res(m, res);
+  mfg_bc3res(m, rmes);
+  if (P->iprec != 0) mfg_bc3bdg(m, BDiag);
+  return 0;
+}
+
+int orc_itrres(const orc_mesh* m, const orc_params* P, const double* yp, const double* yc,
+               const double* ac, double* rmes, int iabres) {
+  int rc = mfg_flags(m, P);
+  if (rc) return rc;
+  /* itrres.f:40-81: interior elements only (the boundary loop needs
+   * Jactyp != 0, itrres.f:86), accumulated into rmes, then bc3Res */
+  rc = mfg_interior(m, P, yp, yc, ac, NULL, 2, iabres, NULL, rmes, NULL);
+  if (rc) return rc;
+  mfg_bc3res(m, rmes);
+  return 0;
+}
+
+/* shuffle.f:1-30: (p,u,v,w,T) <-> (u,v,w,p,T) of an (nshg,5) array */
+void orc_yshuffle(long long nshg, double* r, int new2old) {
+  for (long long n = 0; n < nshg; ++n) {
+    double t[5];
+    for (int f = 0; f < 5; ++f) t[f] = r[n + nshg * f];
+    if (new2old) {
+      r[n] = t[3];
+      r[n + nshg] = t[0];
+      r[n + 2 * nshg] = t[1];
+      r[n + 3 * nshg] = t[2];
+    } else {
+      r[n] = t[1];
+      r[n + nshg] = t[2];
+      r[n + 2 * nshg] = t[3];
+      r[n + 3 * nshg] = t[0];
+    }
+  }
+}
+
+/* y-variables of the direction: old2new(U^-1 (ypre + s d)) (au1mfg.f:64-68,
+ * itrfdi.f:61-65) */
+static void mfg_point(long long nshg, const double* BDiag, const double* ypre, double s,
+                      const double* d, double* out) {
+  const long long V = 5 * nshg;
+  if (d)
+    for (long long i = 0; i < V; ++i) out[i] = ypre[i] + s * d[i];
+  else
+    memcpy(out, ypre, sizeof(double) * V);
+  orc_i3lu(nshg, (double*)BDiag, out, 2);
+  orc_yshuffle(nshg, out, 0);
+}
+
+int orc_itrfdi(const orc_mesh* m, const orc_params* P, const double* ypre, const double* y,
+               const double* ac, const double* rmes, const double* u, const double* BDiag,
+               double* eGMRES) {
+  const long long nshg = m->nshg, V = 5 * nshg;
+  double* yt = malloc(sizeof(double) * V);
+  double* rt = calloc(V, sizeof(double));
+  if (!yt || !rt) {
+    free(yt);
+    free(rt);
+    return 4;
+  }
+  int rc;
+  /* itrfdi.f:56-82 cancellation error epsA */
+  mfg_point(nshg, BDiag, ypre, 0.0, NULL, yt);
+  if ((rc = orc_itrres(m, P, yt, y, ac, rt, 1))) goto out;
+  orc_i3lu(nshg, (double*)BDiag, rt, 1);
+  const double epsA = (P->epsM * P->epsM) * sqrt(sumsq_prod(V, rt, rt));
+  /* itrfdi.f:86-130 second difference at epsSD = sqrt(epsM) along u */
+  const double epsSD = sqrt(P->epsM);
+  memset(rt, 0, sizeof(double) * V);
+  mfg_point(nshg, BDiag, ypre, epsSD, u, yt);
+  if ((rc = orc_itrres(m, P, yt, y, ac, rt, 0))) goto out;
+  mfg_point(nshg, BDiag, ypre, -epsSD, u, yt);
+  if ((rc = orc_itrres(m, P, yt, y, ac, rt, 0))) goto out;
+  orc_i3lu(nshg, (double*)BDiag, rt, 1);
+  for (long long i = 0; i < V; ++i) {
+    double d = (rt[i] - TWO * rmes[i]) / P->epsM;
+    rt[i] = d * d;
+  }
+  {
+    double SDnrm = 0.0;
+    for (long long i = 0; i < V; ++i) SDnrm = SDnrm + rt[i];
+    SDnrm = sqrt(SDnrm);
+    *eGMRES = TWO * sqrt(epsA / SDnrm); /* itrfdi.f:139 */
+  }
+out:
+  free(yt);
+  free(rt);
+  return rc;
+}
+
+int orc_au1mfg(const orc_mesh* m, const orc_params* P, const double* ypre, const double* y,
+               const double* ac, const double* rmes, const double* BDiag, double eGMRES,
+               double* u) {
+  const long long nshg = m->nshg, V = 5 * nshg;
+  double* ub = calloc(V, sizeof(double));
+  if (!ub) return 4;
+  /* au1mfg.f:62-83 */
+  for (long long i = 0; i < V; ++i) u[i] = ypre[i] + eGMRES * u[i];
+  orc_i3lu(nshg, (double*)BDiag, u, 2);
+  orc_yshuffle(nshg, u, 0);
+  int rc = orc_itr_bc(m, u, NULL); /* ires = 2: y only (itrbc.f:137-139) */
+  if (!rc) rc = orc_itrres(m, P, u, y, ac, ub, 0);
+  if (!rc) {
+    orc_i3lu(nshg, (double*)BDiag, ub, 1);
+    for (long long i = 0; i < V; ++i) u[i] = (ub[i] - rmes[i]) / eGMRES;
+  }
+  free(ub);
+  return rc;
+}
+
+int orc_au2mfg(const orc_mesh* m, const orc_params* P, const double* ypre, const double* y,
+               const double* ac, const double* res, const double* Dy, const double* BDiag,
+               double* u) {
+  const long long nshg = m->nshg, V = 5 * nshg;
+  double* u1 = calloc(V, sizeof(double));
+  double* u2 = calloc(V, sizeof(double));
+  double* w = malloc(sizeof(double) * V);
+  int rc = 4;
+  if (!u1 || !u2 || !w) goto out;
+  /* au2mfg.f:59-108 */
+  const double eps = pow(P->epsM, PT66) / sqrt(sumsq_prod(V, Dy, Dy));
+  for (long long i = 0; i < V; ++i) w[i] = ypre[i] + eps * Dy[i];
+  orc_i3lu(nshg, (double*)BDiag, w, 2);
+  orc_yshuffle(nshg, w, 0);
+  if ((rc = orc_itr_bc(m, w, NULL))) goto out;
+  if ((rc = orc_itrres(m, P, w, y, ac, u1, 0))) goto out;
+  orc_i3lu(nshg, (double*)BDiag, u1, 1);
+  for (long long i = 0; i < V; ++i) w[i] = ypre[i] - eps * Dy[i];
+  orc_i3lu(nshg, (double*)BDiag, w, 2);
+  orc_yshuffle(nshg, w, 0);
+  if ((rc = orc_itr_bc(m, w, NULL))) goto out;
+  if ((rc = orc_itrres(m, P, w, y, ac, u2, 0))) goto out;
+  orc_i3lu(nshg, (double*)BDiag, u2, 1);
+  for (long long i = 0; i < V; ++i) u[i] = res[i] - (u1[i] - u2[i]) / (TWO * eps);
+out:
+  free(u1);
+  free(u2);
+  free(w);
+  return rc;
+}
+
+int orc_solmfg(const orc_mesh* m, const orc_params* P, const double* y, const double* ac,
+               double* res, double* rmes, double* BDiag, double* Dy, double* eGMRES, int fdi,
+               orc_stats* st) {
+  const long long nshg = m->nshg, V = 5 * nshg;
+  const int K = P->Kspace;
+  int rc = orc_elmmfg(m, P, y, ac, res, rmes, BDiag);
+  if (rc) return rc;
+  /* solmfg.f:101-118 */
+  if (P->iprec != 0) orc_i3lu(nshg, BDiag, NULL, 0);
+  orc_i3lu(nshg, BDiag, res, 1);
+  orc_i3lu(nshg, BDiag, rmes, 1);
+  double* u = malloc(sizeof(double) * V * (K + 1));
+  double* ypre = malloc(sizeof(double) * V);
+  double* H = calloc((size_t)(K + 1) * K, sizeof(double));
+  double* e = calloc(K + 1, sizeof(double));
+  double* yB = calloc(K + 1, sizeof(double));
+  double* Rc = calloc(K + 1, sizeof(double));
+  double* Rs = calloc(K + 1, sizeof(double));
+  if (!u || !ypre || !H || !e || !yB || !Rc || !Rs) {
+    rc = 4;
+    goto out;
+  }
+#define HB(a, b) H[((a)-1) + (K + 1) * ((b)-1)]
+#define UB(k) (u + V * ((k)-1))
+  memcpy(UB(1), res, sizeof(double) * V);
+  memset(Dy, 0, sizeof(double) * V);
+  /* solmfg.f:124-133 ypre = U new2old(y) */
+  memcpy(ypre, y, sizeof(double) * V);
+  orc_yshuffle(nshg, ypre, 1);
+  orc_i3lu(nshg, BDiag, ypre, 3);
+  double unorm = sqrt(sumsq_prod(V, res, res));
+  int iKs = 0;
+  double epsnrm = 0.0, echeck = 0.0;
+  st->lGMRES = 0;
+  st->unorm0 = unorm;
+  if (!(unorm < 100. * P->epsM * P->epsM)) {
+    epsnrm = P->etol * unorm;
+    /* solmfg.f:163-170 (the caller's iter = 1 and mod(istep, 20) = 0) */
+    if (fdi && (rc = orc_itrfdi(m, P, ypre, y, ac, rmes, res, BDiag, eGMRES))) goto out;
+    for (int mG = 1; mG <= P->nGMRES; ++mG) {
+      st->lGMRES = mG - 1;
+      if (mG > 1) { /* solmfg.f:181-202 R - A Dy by the two-sided difference */
+        if ((rc = orc_au2mfg(m, P, ypre, y, ac, res, Dy, BDiag, UB(1)))) goto out;
+        unorm = sqrt(sumsq_prod(V, UB(1), UB(1)));
+      }
+      memset(e, 0, sizeof(double) * (K + 1));
+      e[0] = unorm;
+      for (long long i = 0; i < V; ++i) UB(1)[i] = UB(1)[i] / unorm;
+      for (int iK = 1; iK <= K; ++iK) {
+        iKs = iK;
+        double* w = UB(iK + 1);
+        memcpy(w, UB(iK), sizeof(double) * V);
+        if ((rc = orc_au1mfg(m, P, ypre, y, ac, rmes, BDiag, *eGMRES, w))) goto out;
+        double beta = 0.0;
+        for (int jK = 1; jK <= iK + 1; ++jK) {
+          if (jK == 1) {
+            beta = sumsq_prod(V, w, UB(1));
+          } else {
+            const double* um = UB(jK - 1);
+            for (long long i = 0; i < V; ++i) w[i] = w[i] - beta * um[i];
+            beta = sumsq_prod(V, w, UB(jK));
+          }
+          HB(jK, iK) = beta;
+        }
+        unorm = sqrt(beta);
+        HB(iK + 1, iK) = unorm;
+        for (long long i = 0; i < V; ++i) w[i] = w[i] / unorm;
+        for (int jK = 1; jK <= iK - 1; ++jK) {
+          double tmp = Rc[jK] * HB(jK, iK) + Rs[jK] * HB(jK + 1, iK);
+          HB(jK + 1, iK) = -Rs[jK] * HB(jK, iK) + Rc[jK] * HB(jK + 1, iK);
+          HB(jK, iK) = tmp;
+        }
+        double tmp = sqrt(HB(iK, iK) * HB(iK, iK) + HB(iK + 1, iK) * HB(iK + 1, iK));
+        Rc[iK] = HB(iK, iK) / tmp;
+        Rs[iK] = HB(iK + 1, iK) / tmp;
+        HB(iK, iK) = tmp;
+        HB(iK + 1, iK) = 0.0;
+        tmp = Rc[iK] * e[iK - 1] + Rs[iK] * e[iK];
+        e[iK] = -Rs[iK] * e[iK - 1] + Rc[iK] * e[iK];
+        e[iK - 1] = tmp;
+        st->ntotGM++;
+        echeck = fabs(e[iK]);
+        if (echeck <= epsnrm && iK >= P->minIters) break;
+      }
+      for (int jK = iKs; jK >= 1; --jK) {
+        yB[jK] = e[jK - 1] / HB(jK, jK);
+        for (int lK = 1; lK <= jK - 1; ++lK) e[lK - 1] = e[lK - 1] - yB[jK] * HB(lK, jK);
+      }
+      for (int jK = 1; jK <= iKs; ++jK) {
+        const double* uj = UB(jK);
+        for (long long i = 0; i < V; ++i) Dy[i] = Dy[i] + yB[jK] * uj[i];
+      }
+      echeck = fabs(e[iKs]);
+      if (echeck <= epsnrm) break;
+    }
+  }
+#undef HB
+#undef UB
+  /* solmfg.f:364 */
+  orc_i3lu(nshg, BDiag, Dy, 2);
+  st->iKs = iKs;
+  st->epsnrm = epsnrm;
+  st->echeck = echeck;
+  st->converged = echeck <= epsnrm;
+  {
+    /* rstat.f:95-116 on res; solmfg.f:368 passes no second array, so the
+     * second norm here is the preconditioned modified residual's */
+    double s1 = 0.0, s2 = 0.0;
+    for (long long n = 0; n < nshg; ++n) {
+      double a = 0.0, b = 0.0;
+      for (int f = 0; f < 5; ++f) {
+        a = a + res[n + nshg * f] * res[n + nshg * f];
+        b = b + rmes[n + nshg * f] * rmes[n + nshg * f];
+      }
+      s1 = s1 + a;
+      s2 = s2 + b;
+    }
+    st->totres[0] = sqrt(s1) / (double)(float)nshg;
+    st->totres[1] = sqrt(s2) / (double)(float)nshg;
+  }
+out:
+  free(u);
+  free(ypre);
+  free(H);
+  free(e);
+  free(yB);
+  free(Rc);
+  free(Rs);
+  return rc;
+}
+
 /* itrPC.f:56-73, the same-velocity predictor.  ipred 2-4 (itrPC.f:75-119)
  * pass undeclared, implicitly typed scalars for iBC, BC, iper and ilwork to
  * itrBC, so the reference defines no result for them: rejected (1). */

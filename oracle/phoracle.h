@@ -83,6 +83,14 @@ void orc_e3_q(const orc_params* pr, const double yl[4][5],
               const double ql[4][12], double rl[4][5], double EGmass[20][20],
               int want_lhs);
 
+/* e3 in its residual modes ires (1 res, 2 modified residual, 3 both;
+ * e3.f:95-300).  ylp: e3's yl (point values and gradients), yl: e3's ycl
+ * (the state); the same array except in ItrRes.  rl / rml / BDl (e3bdg's
+ * BDiagl(a,i,k) at [a][i+5k], or NULL) are overwritten. */
+void orc_e3_mfg(const orc_params* pr, const double ylp[4][5], const double yl[4][5],
+                const double acl[4][5], const double xl[4][3], const double ql[4][12],
+                int ires, double rl[4][5], double rml[4][5], double BDl[4][25]);
+
 /* AsIq over all interior elements + qpbc (elmgmr.f:361-391, asiq.f,
  * e3q.f, e3qvar.f, qpbc.f): qres (nshg,12) the reconstructed diffusive
  * flux, rmass (nshg) the inverted lumped mass.  idiff 1 (row-sum lumping)
@@ -117,6 +125,41 @@ void orc_bc3per(long long nshg, const int* iBC, const int* iper, double* res,
 int orc_solgmrs(const orc_mesh* m, const orc_params* pr, const double* y,
                 const double* ac, double* res, double* rmes, double* BDiag,
                 double* lhsK, double* Dy, orc_stats* st);
+
+/* Matrix-free GMRES (impl = 20, itrdrv.f:489-506).  The frozen flags plus
+ * idiff 0, 1, 3; iDC != 0 and itau != 0 return 5.
+ * ElmMFG (elmmfg.f): res, the modified residual rmes and (iprec) e3bdg's
+ * BDiag, after bc3Res / bc3BDg. */
+int orc_elmmfg(const orc_mesh* m, const orc_params* pr, const double* y,
+               const double* ac, double* res, double* rmes, double* BDiag);
+/* ItrRes (itrres.f): rmes += the modified residual of the variables yp
+ * (global order) at the state yc, interior elements only, then bc3Res;
+ * iabres: element |rml| (itrFDI's cancellation estimate). */
+int orc_itrres(const orc_mesh* m, const orc_params* pr, const double* yp,
+               const double* yc, const double* ac, double* rmes, int iabres);
+/* yshuffle (shuffle.f): new2old (u,v,w,p,T) -> (p,u,v,w,T), else back */
+void orc_yshuffle(long long nshg, double* r, int new2old);
+/* itrFDI (itrfdi.f:1-141): the finite-difference interval eGMRES from the
+ * preconditioned base point ypre, rmes (forward-reduced) and direction u */
+int orc_itrfdi(const orc_mesh* m, const orc_params* pr, const double* ypre,
+               const double* y, const double* ac, const double* rmes,
+               const double* u, const double* BDiag, double* eGMRES);
+/* Au1MFG (au1mfg.f): u <- (L^-1 Rmod(y(ypre + eGMRES u)) - rmes) / eGMRES */
+int orc_au1mfg(const orc_mesh* m, const orc_params* pr, const double* ypre,
+               const double* y, const double* ac, const double* rmes,
+               const double* BDiag, double eGMRES, double* u);
+/* Au2MFG (au2mfg.f): u <- res - the two-sided difference along Dy */
+int orc_au2mfg(const orc_mesh* m, const orc_params* pr, const double* ypre,
+               const double* y, const double* ac, const double* res,
+               const double* Dy, const double* BDiag, double* u);
+/* SolMFG (solmfg.f:1-381) including ElmMFG.  eGMRES: the /itrpar/ interval
+ * (in/out); fdi: recompute it by itrFDI first (the reference does so when
+ * iter = 1 and mod(istep, 20) = 0, solmfg.f:163).  res returns
+ * preconditioned, rmes forward-reduced, BDiag LU-factored (iprec), Dy the
+ * increment consumed by itrCorrect. */
+int orc_solmfg(const orc_mesh* m, const orc_params* pr, const double* y,
+               const double* ac, double* res, double* rmes, double* BDiag,
+               double* Dy, double* eGMRES, int fdi, orc_stats* st);
 
 /* itrPredict / itrCorrect / itrBC (itrPC.f:56-148, itrbc.f:1-198);
  * y, ac, yold, acold are (nshg,5) in the global order (u,v,w,p,T).

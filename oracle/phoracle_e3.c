@@ -342,13 +342,124 @@ static void e3tau_nd(const orc_params* P, double rho, double cp, double T, doubl
   for (int j = 0; j < 5; ++j) PTau[4][j] = b * S[last[j]];
 }
 
-void orc_e3_q(const orc_params* P, const double yl[4][5],
-              const double acl[4][5], const double xl[4][3],
-              const double ql[4][12], double rl[4][5], double EG[20][20],
-              int want_lhs) {
+/* e3bdg.f:1-2596 (itau < 10, ngauss = 4, Weak Form SUPG: ivart = 2, the
+ * input default input.config:226), the block diagonal of the matrix-free
+ * solver at one point, added to BDl[a][i + 5 k] = BDiagl(a, i, k):
+ *   N_a A_i N_a,i                                     (e3bdg.f:34-189)
+ *   N_a^2 WdetJ Dtgl almi/(gami alfi) A0               (e3bdg.f:216-246)
+ *   N_a WdetJ almi/(gami alfi) Dtgl N_a,i A_i tau A0   (e3bdg.f:251-342)
+ *   N_a,k N_a,l WdetJ (A_k tau A_l + K_kl)             (e3bdg.f:344-2595)
+ * The reference writes each product out entry by entry and leaves out the
+ * terms in which an A-matrix entry is zero by construction (e3mtrx.f); the
+ * full sums here add those terms as exact zeros, in the same grouping and
+ * order (tau class by tau class, A_k-first terms before A_l-first terms),
+ * so the result is the same to the bit.  The stiffness blocks come in the
+ * reference's order 11, 22, 33, 12+21, 13+31, 23+32, each with its
+ * viscous K (e3bdg.f:602-610, 862-870, 1123-1131, 1609-1614, 2092-2097,
+ * 2575-2580). */
+static void e3bdg(const orc_params* P, const double shape[4], const double shg[4][3],
+                  double WdetJ, const double A1[5][5], const double A2[5][5],
+                  const double A3[5][5], const double A0[5][5], const double tau[3],
+                  double u1, double u2, double u3, double rmu, double rlm2mu, double con,
+                  double BDl[4][25]) {
+  const double(*Am[3])[5] = {A1, A2, A3};
+  for (int j = 0; j < 4; ++j) {
+    const double tmp = shape[j] * WdetJ;
+    for (int k = 0; k < 5; ++k)
+      for (int i = 0; i < 5; ++i)
+        BDl[j][i + 5 * k] =
+            BDl[j][i + 5 * k] +
+            tmp * (shg[j][0] * A1[i][k] + shg[j][1] * A2[i][k] + shg[j][2] * A3[i][k]);
+  }
+  {
+    const double ff = P->almi / P->gami / P->alfi;
+    const double tmp = WdetJ * (P->Dtgl * ff + 0.0);
+    for (int j = 0; j < 4; ++j) {
+      const double tmp2 = (shape[j] * shape[j]) * tmp;
+      for (int k = 0; k < 5; ++k)
+        for (int i = 0; i < 5; ++i) BDl[j][i + 5 * k] = BDl[j][i + 5 * k] + tmp2 * A0[i][k];
+    }
+  }
+  /* A_k tau A0 with the diagonal tau (tau1 continuity, tau2 momentum, tau3
+   * energy columns) */
+  const double tc[5] = {tau[0], tau[1], tau[1], tau[1], tau[2]};
+  double AtA0[3][5][5];
+  for (int d = 0; d < 3; ++d) {
+    double At[5][5];
+    for (int i = 0; i < 5; ++i)
+      for (int m = 0; m < 5; ++m) At[i][m] = Am[d][i][m] * tc[m];
+    for (int c = 0; c < 5; ++c)
+      for (int i = 0; i < 5; ++i)
+        AtA0[d][i][c] = At[i][0] * A0[0][c] + At[i][1] * A0[1][c] + At[i][2] * A0[2][c] +
+                        At[i][3] * A0[3][c] + At[i][4] * A0[4][c];
+  }
+  for (int a = 0; a < 4; ++a) {
+    const double tmp1 = shape[a] * WdetJ * P->almi / P->gami / P->alfi * P->Dtgl;
+    for (int i = 0; i < 5; ++i)
+      for (int c = 0; c < 5; ++c)
+        BDl[a][i + 5 * c] = BDl[a][i + 5 * c] +
+                            tmp1 * (shg[a][0] * AtA0[0][i][c] + shg[a][1] * AtA0[1][i][c] +
+                                    shg[a][2] * AtA0[2][i][c]);
+  }
+  /* stiffness blocks (k,l): diagonal ones tau1 A(i,1)A(1,j) + tau2 (...) +
+   * tau3 A(i,5)A(5,j); off-diagonal ones the sum of both orders per class */
+  static const int kl[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  for (int b = 0; b < 6; ++b) {
+    const int k = kl[b][0], l = kl[b][1];
+    const double(*Ak)[5] = Am[k];
+    const double(*Al)[5] = Am[l];
+    double M[5][5];
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) {
+        if (k == l) {
+          M[i][j] = tau[0] * Ak[i][0] * Ak[0][j] +
+                    tau[1] * (Ak[i][1] * Ak[1][j] + Ak[i][2] * Ak[2][j] + Ak[i][3] * Ak[3][j]) +
+                    tau[2] * Ak[i][4] * Ak[4][j];
+        } else {
+          M[i][j] = tau[0] * (Ak[i][0] * Al[0][j] + Al[i][0] * Ak[0][j]) +
+                    tau[1] * (Ak[i][1] * Al[1][j] + Ak[i][2] * Al[2][j] + Ak[i][3] * Al[3][j] +
+                              Al[i][1] * Ak[1][j] + Al[i][2] * Ak[2][j] + Al[i][3] * Ak[3][j]) +
+                    tau[2] * (Ak[i][4] * Al[4][j] + Al[i][4] * Ak[4][j]);
+        }
+      }
+    const double uu[3] = {u1, u2, u3};
+    if (k == l) {
+      for (int m = 0; m < 3; ++m) M[1 + m][1 + m] = M[1 + m][1 + m] + (m == k ? rlm2mu : rmu);
+      for (int m = 0; m < 3; ++m) M[4][1 + m] = M[4][1 + m] + (m == k ? rlm2mu : rmu) * uu[m];
+      M[4][4] = M[4][4] + con;
+    } else {
+      M[1 + k][1 + l] = M[1 + k][1 + l] + rlm2mu - rmu;
+      M[1 + l][1 + k] = M[1 + l][1 + k] + rlm2mu - rmu;
+      M[4][1 + k] = M[4][1 + k] + (rlm2mu - rmu) * uu[l];
+      M[4][1 + l] = M[4][1 + l] + (rlm2mu - rmu) * uu[k];
+    }
+    for (int a = 0; a < 4; ++a) {
+      const double tmp = WdetJ * shg[a][k] * shg[a][l];
+      for (int c = 0; c < 5; ++c)
+        for (int i = 0; i < 5; ++i) BDl[a][i + 5 * c] = BDl[a][i + 5 * c] + tmp * M[i][c];
+    }
+  }
+}
+
+/* e3 of one element (e3.f:95-300) in the reference's three residual modes
+ * (common /genpar/ ires): 1 the residual rl, 2 the modified residual rml of
+ * the matrix-free solver, 3 both.  ylp is e3's yl: the variables whose point
+ * values dui and gradients enter (e3ivar.f:96-101,259-334); yl is e3's ycl:
+ * the state (pressure, velocity, temperature) at the points
+ * (e3ivar.f:150-155).  ElmGMRs and ElmMFG pass the same array for both;
+ * ItrRes/AsIRes pass the perturbed variables as ylp (asires.f:56-59,83-86).
+ * BDl (non-NULL) receives e3bdg's block diagonal (e3.f:261-272, with
+ * iprec = 1 and lhs != 1). */
+static void e3_body(const orc_params* P, const double ylp[4][5], const double yl[4][5],
+                    const double acl[4][5], const double xl[4][3], const double ql[4][12],
+                    int ires, double rl[4][5], double rml[4][5], double EG[20][20],
+                    int want_lhs, double BDl[4][25]) {
   const quad_t* Q = quad();
   const int lhs = want_lhs;
-  memset(rl, 0, sizeof(double) * 20);
+  const int want_r = ires == 1 || ires == 3, want_m = ires != 1;
+  if (want_r) memset(rl, 0, sizeof(double) * 20);
+  if (want_m) memset(rml, 0, sizeof(double) * 20);
+  if (BDl) memset(BDl, 0, sizeof(double) * 100);
   if (lhs) memset(EG, 0, sizeof(double) * 400);
   const double Rgas = P->Rgas, gamma = P->gamma, gamma1 = P->gamma1;
   for (int q = 0; q < 4; ++q) {
@@ -375,6 +486,22 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
     }
     for (int n = 0; n < 4; ++n)
       for (int f = 0; f < 5; ++f) aci[f] = aci[f] + shape[n] * acl[n][f];
+    /* e3ivar.f:94-129 dui: the conservative variables (rho, rho u, rho E)
+     * of e3's yl at the point (getthm ithm = 6; used by the modified
+     * residual only) */
+    double dui[5] = {0, 0, 0, 0, 0};
+    if (want_m) {
+      for (int n = 0; n < 4; ++n)
+        for (int f = 0; f < 5; ++f) dui[f] = dui[f] + shape[n] * ylp[n][f];
+      const double rkp = PT5 * (dui[1] * dui[1] + dui[2] * dui[2] + dui[3] * dui[3]);
+      const double rhop = dui[0] / (Rgas * dui[4]);
+      const double eip = dui[4] * (Rgas / gamma1);
+      dui[0] = rhop;
+      dui[1] = rhop * dui[1];
+      dui[2] = rhop * dui[2];
+      dui[3] = rhop * dui[3];
+      dui[4] = rhop * (eip + rkp);
+    }
     /* e3ivar.f:203 */
     double rk = PT5 * (u1 * u1 + u2 * u2 + u3 * u3);
     /* getthm.f:130-160, ithm = 7 */
@@ -403,12 +530,12 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
       g1[f] = 0.0;
       g2[f] = 0.0;
       g3[f] = 0.0;
-      g1[f] = g1[f] + shg[0][0] * yl[0][f] + shg[1][0] * yl[1][f] +
-              shg[2][0] * yl[2][f] + shg[3][0] * yl[3][f];
-      g2[f] = g2[f] + shg[0][1] * yl[0][f] + shg[1][1] * yl[1][f] +
-              shg[2][1] * yl[2][f] + shg[3][1] * yl[3][f];
-      g3[f] = g3[f] + shg[0][2] * yl[0][f] + shg[1][2] * yl[1][f] +
-              shg[2][2] * yl[2][f] + shg[3][2] * yl[3][f];
+      g1[f] = g1[f] + shg[0][0] * ylp[0][f] + shg[1][0] * ylp[1][f] +
+              shg[2][0] * ylp[2][f] + shg[3][0] * ylp[3][f];
+      g2[f] = g2[f] + shg[0][1] * ylp[0][f] + shg[1][1] * ylp[1][f] +
+              shg[2][1] * ylp[2][f] + shg[3][1] * ylp[3][f];
+      g3[f] = g3[f] + shg[0][2] * ylp[0][f] + shg[1][2] * ylp[1][f] +
+              shg[2][2] * ylp[2][f] + shg[3][2] * ylp[3][f];
     }
 
     /* ---- e3mtrx.f:87-224 ---- */
@@ -553,7 +680,8 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
       dVdY[14] = f2;
     }
 
-    /* ---- e3conv.f:72-96 Euler flux (integrated by parts) ---- */
+    /* ---- e3conv.f:72-96 Euler flux (integrated by parts), ires 1 or 3 ---- */
+    if (want_r) {
     ri[0] = (-u1) * rho;
     ri[1] = (-u1) * rho * u1 - pres;
     ri[2] = (-u1) * rho * u2;
@@ -569,6 +697,7 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
     ri[12] = (-u3) * rho * u2;
     ri[13] = (-u3) * rho * u3 - pres;
     ri[14] = (-u3) * rho * (ei + rk) - u3 * pres;
+    }
     /* e3conv.f:100-179 rLyi = A_i Y,i */
     double rLyi[5];
     for (int i = 0; i < 5; ++i)
@@ -577,6 +706,10 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
                 A2[i][1] * g2[1] + A2[i][2] * g2[2] + A2[i][3] * g2[3] +
                 A2[i][4] * g2[4] + A3[i][0] * g3[0] + A3[i][1] * g3[1] +
                 A3[i][2] * g3[2] + A3[i][3] * g3[3] + A3[i][4] * g3[4];
+    /* e3conv.f:183-184: the modified residual takes the convective term in
+     * non-integrated form */
+    if (want_m)
+      for (int i = 0; i < 5; ++i) rmi[15 + i] = rLyi[i];
     /* e3conv.f:188-345 LHS: EGmass(a,b) += N_a (WdetJ N_b,i A_i) */
     if (lhs) {
       for (int j = 0; j < 4; ++j) {
@@ -667,13 +800,20 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
               rlm2mu * u3 * g3[3] + con * g3[4];
     for (int k = 11; k <= 14; ++k) ri[k] = ri[k] + rmi[k];
 
+    /* ---- e3ls.f:98-103 the modified LS residual: A_i Y,i + fct1 Y ---- */
+    double rLymi[5] = {0, 0, 0, 0, 0};
+    if (want_m) {
+      const double fct1 = P->almi / P->gami / P->alfi * P->Dtgl;
+      for (int i = 0; i < 5; ++i) rLymi[i] = rLyi[i] + fct1 * dui[i];
+    }
     /* ---- e3ls.f:105-143 add the time term to the LS residual ---- */
+    if (want_r)
     for (int i = 0; i < 5; ++i)
       rLyi[i] = rLyi[i] + A0[i][0] * aci[0] + A0[i][1] * aci[1] +
                 A0[i][2] * aci[2] + A0[i][3] * aci[3] + A0[i][4] * aci[4];
     /* ---- e3ivar.f:361-395 divergence of the reconstructed diffusive flux,
      * e3ls.f:148-155 subtracted from the LS residual (isurf = 0) ---- */
-    if (ql) {
+    if (ql && want_r) {
       double divqi[4] = {0.0, 0.0, 0.0, 0.0};
       for (int n = 0; n < 4; ++n)
         for (int k = 0; k < 4; ++k)
@@ -711,7 +851,9 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
     tau[2] = tau[1] / cv * P->temper;
     double Lt[5]; /* e3tau.f:188: rLyitemp, the LS residual before tau */
     for (int f = 0; f < 5; ++f) Lt[f] = rLyi[f];
-    if (P->itau >= 10) {
+    if (!want_r) {
+      /* e3tau.f:185 (and :638): rLyi keeps A_i Y,i for ires = 2 */
+    } else if (P->itau >= 10) {
       /* e3tau.f:633-643 rLyi = Tau rLyi */
       for (int i = 0; i < 5; ++i) {
         double acc = 0.0;
@@ -725,10 +867,28 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
       rLyi[3] = tau[1] * rLyi[3];
       rLyi[4] = tau[2] * rLyi[4];
     }
+    /* e3tau.f:263-271 and :723-732: tau times the modified LS residual */
+    if (want_m) {
+      if (P->itau >= 10) {
+        double em[5];
+        for (int j = 0; j < 5; ++j) em[j] = rLymi[j];
+        for (int i = 0; i < 5; ++i) {
+          double acc = 0.0;
+          for (int j = 0; j < 5; ++j) acc = acc + PTau[i][j] * em[j];
+          rLymi[i] = acc;
+        }
+      } else {
+        rLymi[0] = tau[0] * rLymi[0];
+        rLymi[1] = tau[1] * rLymi[1];
+        rLymi[2] = tau[1] * rLymi[2];
+        rLymi[3] = tau[1] * rLymi[3];
+        rLymi[4] = tau[2] * rLymi[4];
+      }
+    }
     /* e3tau.f:198-268 (iDC != 0): rTLS = L.(dV/dY tau L), raLS = L.A0inv L
      * and giju, the inverse of the metric tensor */
     double rTLS = 0.0, raLS = 0.0, giju[6] = {0, 0, 0, 0, 0, 0};
-    if (P->idc != 0) {
+    if (P->idc != 0 && want_r) {
       const double* L = Lt;
       const double* V = dVdY;
       const double* r = rLyi;
@@ -756,10 +916,18 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
       giju[4] = detI * (m4 * m6 - m5 * m2);
       giju[5] = detI * (m4 * m5 - m1 * m6);
     }
+    /* e3ls.f:231-348 modified residual rmi += A_i (tau Lm) */
+    if (want_m) {
+      for (int i = 0; i < 5; ++i) rmi[i] = rowdot(A1, i, rLymi) + rmi[i];
+      for (int i = 0; i < 5; ++i) rmi[5 + i] = rowdot(A2, i, rLymi) + rmi[5 + i];
+      for (int i = 0; i < 5; ++i) rmi[10 + i] = rowdot(A3, i, rLymi) + rmi[10 + i];
+    }
     /* e3ls.f:351-458 RHS ri += A_i (tau L) */
+    if (want_r) {
     for (int i = 0; i < 5; ++i) ri[i] = rowdot(A1, i, rLyi) + ri[i];
     for (int i = 0; i < 5; ++i) ri[5 + i] = rowdot(A2, i, rLyi) + ri[5 + i];
     for (int i = 0; i < 5; ++i) ri[10 + i] = rowdot(A3, i, rLyi) + ri[10 + i];
+    }
     /* e3ls.f:464-766 LHS */
     if (lhs) {
       const double* Ai[3] = {&A1[0][0], &A2[0][0], &A3[0][0]};
@@ -816,8 +984,10 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
         }
       }
     }
-    /* ---- e3dc.f:44-451 discontinuity capturing (e3.f:217-222) ---- */
-    if (P->idc != 0) {
+    /* ---- e3dc.f:44-451 discontinuity capturing (e3.f:217-222); the
+     * ires = 2 branch (e3dc.f:380-409) is not restated: the drivers of the
+     * modified residual refuse iDC != 0 ---- */
+    if (P->idc != 0 && want_r) {
       const double* gy[3] = {g1, g2, g3};
       double A0g[15];
       for (int j = 0; j < 3; ++j)
@@ -878,9 +1048,14 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
       }
     }
     /* ---- e3massr.f:34-72 ---- */
+    if (want_r)
     for (int i = 0; i < 5; ++i)
       ri[15 + i] = ri[15 + i] + A0[i][0] * aci[0] + A0[i][1] * aci[1] +
                    A0[i][2] * aci[2] + A0[i][3] * aci[3] + A0[i][4] * aci[4];
+    if (want_m) {
+      const double fct1 = P->almi / P->gami / P->alfi * P->Dtgl;
+      for (int i = 0; i < 5; ++i) rmi[15 + i] = rmi[15 + i] + fct1 * dui[i];
+    }
     /* ---- e3massl.f:29-71 (bcool = 0) ---- */
     if (lhs) {
       double temp = WdetJ * P->almi / P->gami / P->alfi * P->Dtgl;
@@ -898,11 +1073,23 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
         }
       }
     }
+    /* ---- e3.f:261-272 the block diagonal of the matrix-free solver ---- */
+    if (BDl) e3bdg(P, shape, (const double(*)[3])shg, WdetJ, A1, A2, A3, A0, tau, u1, u2, u3,
+                   rmu, rlm2mu, con, BDl);
     /* ---- e3wmlt.f:72-143 RHS ---- */
+    if (want_r) {
     for (int i = 0; i < 4; ++i)
       for (int f = 0; f < 5; ++f)
         rl[i][f] = rl[i][f] + WdetJ * (shg[i][0] * ri[f] + shg[i][1] * ri[5 + f] +
                                        shg[i][2] * ri[10 + f]);
+    }
+    if (want_m)
+      for (int i = 0; i < 4; ++i)
+        for (int f = 0; f < 5; ++f)
+          rml[i][f] = rml[i][f] + WdetJ * (shg[i][0] * rmi[f] + shg[i][1] * rmi[5 + f] +
+                                           shg[i][2] * rmi[10 + f] + shape[i] * rmi[15 + f]);
+    /* e3wmlt.f:124-143 runs for every ires; with ires = 2 ri(16:20) is zero */
+    if (want_r)
     for (int i = 0; i < 4; ++i)
       for (int f = 0; f < 5; ++f)
         rl[i][f] = rl[i][f] + shape[i] * WdetJ * ri[15 + f];
@@ -930,6 +1117,19 @@ void orc_e3_q(const orc_params* P, const double yl[4][5],
       }
     }
   }
+}
+
+void orc_e3_q(const orc_params* P, const double yl[4][5],
+              const double acl[4][5], const double xl[4][3],
+              const double ql[4][12], double rl[4][5], double EG[20][20],
+              int want_lhs) {
+  e3_body(P, yl, yl, acl, xl, ql, 1, rl, NULL, EG, want_lhs, NULL);
+}
+
+void orc_e3_mfg(const orc_params* P, const double ylp[4][5], const double yl[4][5],
+                const double acl[4][5], const double xl[4][3], const double ql[4][12],
+                int ires, double rl[4][5], double rml[4][5], double BDl[4][25]) {
+  e3_body(P, ylp, yl, acl, xl, ql, ires, rl, rml, NULL, 0, BDl);
 }
 
 void orc_e3(const orc_params* P, const double yl[4][5],

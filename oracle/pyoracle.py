@@ -169,6 +169,91 @@ class Oracle:
                     ntotGM=st.ntotGM, converged=bool(st.converged), unorm0=st.unorm0,
                     epsnrm=st.epsnrm, echeck=st.echeck, totres=tuple(st.totres))
 
+    # --- matrix-free GMRES (SolMFG) ------------------------------------
+    def _chk(self, rc, name):
+        if rc:
+            raise RuntimeError(f"{name} rc={rc}")
+
+    def elmmfg(self, y, ac, bd=None, **over):
+        """ElmMFG: (res, rmes, BDiag) after bc3Res / bc3BDg; bd, when given,
+        is the caller's persistent BDiag (kept when iprec = 0)."""
+        n = self.case.nshg
+        res = np.zeros((n, 5), order="F")
+        rmes = np.zeros((n, 5), order="F")
+        if bd is None:
+            bd = np.zeros((n, 5, 5), order="F")
+        self._chk(lib().orc_elmmfg(ctypes.byref(self.m), ctypes.byref(self.p(**over)),
+                                   _p(np.asfortranarray(y)), _p(np.asfortranarray(ac)),
+                                   _p(res), _p(rmes), _p(bd)), "orc_elmmfg")
+        return res, rmes, bd
+
+    def itrres(self, yp, yc, ac, rmes=None, iabres=False, **over):
+        """ItrRes: rmes (accumulated; zeros if None) += the modified residual
+        of the variables yp at the state yc."""
+        n = self.case.nshg
+        rmes = np.zeros((n, 5), order="F") if rmes is None else rmes
+        assert rmes.flags.f_contiguous
+        self._chk(lib().orc_itrres(ctypes.byref(self.m), ctypes.byref(self.p(**over)),
+                                   _p(np.asfortranarray(yp)), _p(np.asfortranarray(yc)),
+                                   _p(np.asfortranarray(ac)), _p(rmes),
+                                   ctypes.c_int(1 if iabres else 0)), "orc_itrres")
+        return rmes
+
+    def yshuffle(self, r, code):
+        r = np.array(r, order="F", dtype=np.float64)
+        lib().orc_yshuffle(_i64(r.shape[0]), _p(r), ctypes.c_int(1 if code == "new2old " else 0))
+        return r
+
+    def itrfdi(self, ypre, y, ac, rmes, u, bd_lu, **over):
+        eg = ctypes.c_double(0.0)
+        self._chk(lib().orc_itrfdi(ctypes.byref(self.m), ctypes.byref(self.p(**over)),
+                                   _p(np.asfortranarray(ypre)), _p(np.asfortranarray(y)),
+                                   _p(np.asfortranarray(ac)), _p(np.asfortranarray(rmes)),
+                                   _p(np.asfortranarray(u)), _p(np.asfortranarray(bd_lu)),
+                                   ctypes.byref(eg)), "orc_itrfdi")
+        return eg.value
+
+    def au1mfg(self, ypre, y, ac, rmes, bd_lu, egmres, u, **over):
+        u = np.array(u, order="F", dtype=np.float64)
+        self._chk(lib().orc_au1mfg(ctypes.byref(self.m), ctypes.byref(self.p(**over)),
+                                   _p(np.asfortranarray(ypre)), _p(np.asfortranarray(y)),
+                                   _p(np.asfortranarray(ac)), _p(np.asfortranarray(rmes)),
+                                   _p(np.asfortranarray(bd_lu)), ctypes.c_double(egmres),
+                                   _p(u)), "orc_au1mfg")
+        return u
+
+    def au2mfg(self, ypre, y, ac, res, dy, bd_lu, **over):
+        u = np.zeros((self.case.nshg, 5), order="F")
+        self._chk(lib().orc_au2mfg(ctypes.byref(self.m), ctypes.byref(self.p(**over)),
+                                   _p(np.asfortranarray(ypre)), _p(np.asfortranarray(y)),
+                                   _p(np.asfortranarray(ac)), _p(np.asfortranarray(res)),
+                                   _p(np.asfortranarray(dy)), _p(np.asfortranarray(bd_lu)),
+                                   _p(u)), "orc_au2mfg")
+        return u
+
+    def solmfg(self, y, ac, egmres=0.0, fdi=True, bd=None, **over):
+        """SolMFG.  egmres: the interval held from an earlier call (used when
+        fdi is False); returns the dict of solgmrs plus eGMRES."""
+        n = self.case.nshg
+        res = np.zeros((n, 5), order="F")
+        rmes = np.zeros((n, 5), order="F")
+        if bd is None:
+            bd = np.zeros((n, 5, 5), order="F")
+        dy = np.zeros((n, 5), order="F")
+        eg = ctypes.c_double(egmres)
+        st = OrcStats()
+        st.ntotGM = self.ntotGM
+        self._chk(lib().orc_solmfg(ctypes.byref(self.m), ctypes.byref(self.p(**over)),
+                                   _p(np.asfortranarray(y)), _p(np.asfortranarray(ac)),
+                                   _p(res), _p(rmes), _p(bd), _p(dy), ctypes.byref(eg),
+                                   ctypes.c_int(1 if fdi else 0), ctypes.byref(st)),
+                  "orc_solmfg")
+        self.ntotGM = st.ntotGM
+        return dict(res=res, rmes=rmes, BDiag=bd, Dy=dy, iKs=st.iKs, lGMRES=st.lGMRES,
+                    ntotGM=st.ntotGM, converged=bool(st.converged), unorm0=st.unorm0,
+                    epsnrm=st.epsnrm, echeck=st.echeck, totres=tuple(st.totres),
+                    eGMRES=eg.value)
+
     def itr_correct_bc(self, y, ac, yold, acold, dy):
         n = self.case.nshg
         y, ac = y.copy(order="F"), ac.copy(order="F")

@@ -1,6 +1,8 @@
 """ctypes bindings of oracle/_ref/libphref.so: the reference's own compiled
-ElmGMRs, e3, SparseAp, Spsi3pre, bc3per, i3LU and itrPredict/itrCorrect/
-itrBC behind the test harness oracle/ref/phref_harness.f.  Test infrastructure only.
+ElmGMRs, e3, SparseAp, Spsi3pre, bc3per, i3LU, itrPredict/itrCorrect/
+itrBC and the matrix-free ElmMFG, ItrRes, itrFDI, Au1MFG, Au2MFG and
+yshuffle behind the test harness oracle/ref/phref_harness.f.  Test
+infrastructure only.
 
 The library keeps its state in the reference's COMMON blocks, so one case
 is active at a time (phref_setup replaces it).
@@ -122,9 +124,49 @@ class Reference:
         return rl[0], eg[0]
 
     def i3lu(self, diag, r, code):
-        codes = {"LU_Fact ": 0, "forward ": 1, "backward": 2}
+        codes = {"LU_Fact ": 0, "forward ": 1, "backward": 2, "product ": 3}
         n = self.case.nshg
         diag = np.array(diag, order="F")
         r = np.zeros((n, 5), order="F") if r is None else np.array(r, order="F")
         lib().phref_i3lu(_p(diag), _p(r), ctypes.c_int(codes[code]))
         return diag, r
+
+    # --- matrix-free GMRES pieces ---------------------------------------
+    def elmmfg(self, y, ac, bd=None):
+        n = self.case.nshg
+        res = np.zeros((n, 5), order="F")
+        rmes = np.zeros((n, 5), order="F")
+        bd = np.zeros((n, 5, 5), order="F") if bd is None else np.array(bd, order="F")
+        lib().phref_elmmfg(_p(np.asfortranarray(y)), _p(np.asfortranarray(ac)), _p(res),
+                           _p(rmes), _p(bd))
+        return res, rmes, bd
+
+    def itrres(self, yp, yc, ac, rmes=None, iabres=False):
+        n = self.case.nshg
+        rmes = np.zeros((n, 5), order="F") if rmes is None else np.array(rmes, order="F")
+        lib().phref_itrres(_p(np.asfortranarray(yp)), _p(np.asfortranarray(yc)),
+                           _p(np.asfortranarray(ac)), _p(rmes), ctypes.c_int(int(iabres)))
+        return rmes
+
+    def itrfdi(self, ypre, y, ac, rmes, u, bd_lu):
+        eg = ctypes.c_double(0.0)
+        lib().phref_itrfdi(*(_p(np.asfortranarray(a)) for a in (ypre, y, ac, rmes, u, bd_lu)),
+                           ctypes.byref(eg))
+        return eg.value
+
+    def au1mfg(self, ypre, y, ac, rmes, res, bd_lu, egmres, u):
+        u = np.array(u, order="F", dtype=np.float64)
+        lib().phref_au1mfg(*(_p(np.asfortranarray(a)) for a in (ypre, y, ac, rmes, res)),
+                           _p(u), _p(np.asfortranarray(bd_lu)), ctypes.c_double(egmres))
+        return u
+
+    def au2mfg(self, ypre, y, ac, rmes, res, dy, bd_lu):
+        u = np.zeros((self.case.nshg, 5), order="F")
+        lib().phref_au2mfg(*(_p(np.asfortranarray(a)) for a in (ypre, y, ac, rmes, res, dy)),
+                           _p(u), _p(np.asfortranarray(bd_lu)))
+        return u
+
+    def yshuffle(self, r, code):
+        r = np.array(r, order="F", dtype=np.float64)
+        lib().phref_yshuffle(_p(r), ctypes.c_int(1 if code == "new2old " else 0))
+        return r

@@ -4,8 +4,11 @@ c  oracle/ref/Makefile from the reference sources into oracle/_ref/).
 c  It fills the COMMON blocks of common.h and the element blocks of
 c  module pointer_data the way readnblk/genblk/genbkb/genint/genshp
 c  would for a linear-tet mesh, and exposes ElmGMRs, e3, SparseAp,
-c  Spsi3pre, bc3per, i3LU, itrPredict, itrCorrect and itrBC to C through
-c  bind(C) wrappers.  SolGMRs itself is not built: its history print
+c  Spsi3pre, bc3per, i3LU, itrPredict, itrCorrect and itrBC, and the
+c  matrix-free solver's ElmMFG, ItrRes, itrFDI, Au1MFG, Au2MFG and
+c  yshuffle, to C through bind(C) wrappers.  SolMFG itself is not built
+c  either: it calls rstat with two arguments where rstat takes three
+c  (solmfg.f:368, rstat.f:1), so its last array is undefined.  SolGMRs itself is not built: its history print
 c  (rstat.f:115) needs TMRC from tmrc.c, whose header includes the
 c  CMake-generated FCMangle.h.
 c  Test infrastructure only: the product never links or calls this.
@@ -95,6 +98,10 @@ c.... physics (/mmatpar/, /matdat/, /genpar/, /outpar/, /timdat/)
       isurf = 0
       rhoinf(1) = -1.0d0
       Jactyp = 0
+c.... Weak Form: SUPG, the input default (input.config:226,
+c     input_fform.cc:645-648); read only by e3bdg
+      ivart = 2
+      iabres = 0
       itseq = 1
       iter = 1
       nitr = 0
@@ -297,7 +304,110 @@ c.... boundary element blocks (genbkb.f)
       if (kcode .eq. 0) code = 'LU_Fact '
       if (kcode .eq. 1) code = 'forward '
       if (kcode .eq. 2) code = 'backward'
+      if (kcode .eq. 3) code = 'product '
       call i3LU(Diag, r, code)
+      end subroutine
+
+c.... matrix-free GMRES pieces.  ItrRes sets iprec = 0 and ires = 2 in
+c     COMMON (itrres.f:38-40); the wrappers restore both so that later
+c     calls see the caller's flags.
+      subroutine phref_elmmfg(y, ac, res, rmes, BDiag)
+     &           bind(C, name='phref_elmmfg')
+      use iso_c_binding
+      use phref_data
+      include "common.h"
+      real(c_double) :: y(nshg,5), ac(nshg,5), res(nshg,5),
+     &                  rmes(nshg,5), BDiag(nshg,5,5)
+      integer :: isav(2)
+      isav = (/ iprec, ires /)
+      call ElmMFG(y, ac, hx, hshp, hshgl, hiBC, hBC, hshpb, hshglb,
+     &            res, rmes, BDiag, hiper, hilwork, hrerr)
+      iprec = isav(1)
+      ires = isav(2)
+      end subroutine
+
+      subroutine phref_itrres(yp, yc, ac, rmes, iab)
+     &           bind(C, name='phref_itrres')
+      use iso_c_binding
+      use phref_data
+      include "common.h"
+      real(c_double) :: yp(nshg,5), yc(nshg,5), ac(nshg,5),
+     &                  rmes(nshg,5)
+      integer(c_int), value :: iab
+      integer :: isav(2)
+      isav = (/ iprec, ires /)
+      iabres = iab
+      call ItrRes(yp, yc, hx, hshp, hshgl, hiBC, hBC, hshpb, hshglb,
+     &            rmes, hiper, hilwork, ac)
+      iabres = 0
+      iprec = isav(1)
+      ires = isav(2)
+      end subroutine
+
+      subroutine phref_itrfdi(ypre, y, ac, rmes, u, BDiag, eg)
+     &           bind(C, name='phref_itrfdi')
+      use iso_c_binding
+      use phref_data
+      include "common.h"
+      real(c_double) :: ypre(nshg,5), y(nshg,5), ac(nshg,5),
+     &                  rmes(nshg,5), u(nshg,5), BDiag(nshg,5,5), eg
+      integer :: isav(2)
+      isav = (/ iprec, ires /)
+      ires = 3
+      call itrFDI(ypre, y, ac, hx, rmes, u, BDiag, hiBC, hBC, hiper,
+     &            hilwork, hshp, hshgl, hshpb, hshglb)
+      eg = eGMRES
+      iprec = isav(1)
+      ires = isav(2)
+      end subroutine
+
+c.... SolMFG sets ires = 2 before its Krylov loop (solmfg.f:171), which
+c     keeps itrBC from touching its second (aliased) argument
+      subroutine phref_au1mfg(ypre, y, ac, rmes, res, u, BDiag, eg)
+     &           bind(C, name='phref_au1mfg')
+      use iso_c_binding
+      use phref_data
+      include "common.h"
+      real(c_double) :: ypre(nshg,5), y(nshg,5), ac(nshg,5),
+     &                  rmes(nshg,5), res(nshg,5), u(nshg,5),
+     &                  BDiag(nshg,5,5)
+      real(c_double), value :: eg
+      integer :: isav(2)
+      isav = (/ iprec, ires /)
+      eGMRES = eg
+      ires = 2
+      call Au1MFG(ypre, y, ac, hx, rmes, res, u, BDiag, hiBC, hBC,
+     &            hiper, hilwork, hshp, hshgl, hshpb, hshglb)
+      iprec = isav(1)
+      ires = isav(2)
+      end subroutine
+
+      subroutine phref_au2mfg(ypre, y, ac, rmes, res, Dy, u, BDiag)
+     &           bind(C, name='phref_au2mfg')
+      use iso_c_binding
+      use phref_data
+      include "common.h"
+      real(c_double) :: ypre(nshg,5), y(nshg,5), ac(nshg,5),
+     &                  rmes(nshg,5), res(nshg,5), Dy(nshg,5),
+     &                  u(nshg,5), BDiag(nshg,5,5)
+      integer :: isav(2)
+      isav = (/ iprec, ires /)
+      ires = 2
+      call Au2MFG(ypre, y, ac, hx, rmes, res, Dy, u, BDiag, hiBC, hBC,
+     &            hiper, hilwork, hshp, hshgl, hshpb, hshglb)
+      iprec = isav(1)
+      ires = isav(2)
+      end subroutine
+
+      subroutine phref_yshuffle(r, kcode) bind(C, name='phref_yshuffle')
+      use iso_c_binding
+      include "common.h"
+      real(c_double) :: r(nshg,5)
+      integer(c_int), value :: kcode
+      character*8 code
+      code = 'old2new '
+      if (kcode .eq. 1) code = 'new2old '
+      call yshuffle(r, code)
       end subroutine
 
       subroutine phref_predict_bc(yold, acold, y, ac)

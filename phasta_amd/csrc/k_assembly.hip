@@ -1753,6 +1753,262 @@ __global__ void k_fill(T* p, size_t n) {
 
 bool g_quad_ready = false;
 
+// ---------------- matrix-free GMRES (SolMFG) ----------------
+// The modified residual of the matrix-free solver (e3 with ires = 2, and the
+// rmes of ElmMFG with ires = 3; e3.f:95-300) of the variables yp at the
+// state y, per element and node.  Lane q = quadrature point q.  The state
+// (pressure, velocity, temperature and everything derived from them: the
+// A matrices, tau, viscosity) comes from y (e3ivar.f:150-155); the point
+// values and gradients from yp (e3ivar.f:96-101,259-334).  With
+//     L  = A_i yp_,i                        (e3conv.f:100-184, non-integrated)
+//     M  = L + c U(yp),  c = almi/(gami alfi) Dtgl, U the conservative
+//          variables of yp at the point    (e3ivar.f:108-129, e3ls.f:103,
+//                                            e3massr.f:71-80)
+// the point contributes WdetJ [ s_a,i (F^v_i(yp) + A_i tau M) + N_a M ] to
+// node a (e3visc.f:278-343, e3tau.f:263-271, e3ls.f:231-348,
+// e3wmlt.f:97-122): no Euler flux and no acceleration.  The four lanes of
+// an element sum their contributions as the residual pass does.  iabs
+// takes |.| of each element-node value (asires.f:90, itrFDI's estimate of
+// the cancellation error).
+struct MresArgs {
+  const int* ien;       // 4 planes of nel
+  const double* node;   // 16 per node: y (local order), ac, x
+  const double* yp;     // variables in the local order, yp_stride doubles per node
+  double* resid;        // (nel*4) x 5
+  long long nel;
+  PhysParams ph;
+  double fct1;          // almi/gami/alfi*Dtgl
+  int yp_stride;        // 16: yp is y itself (ElmMFG), 8: a packed perturbed state
+  int iabs;
+};
+
+__global__ void __launch_bounds__(256) k_mres(MresArgs A) {
+  const long long e_raw = blockIdx.x * 64LL + threadIdx.x / 4;
+  const int q = threadIdx.x & 3;
+  const bool valid = e_raw < A.nel;
+  const long long e = valid ? e_raw : A.nel - 1;  // idle lanes load a valid element
+  const long long nel = A.nel;
+  const PhysParams& ph = A.ph;
+  int nd[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) nd[k] = A.ien[k * nel + e];
+  double xl[4][3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2* v = reinterpret_cast<const double2*>(A.node + 16 * (size_t)nd[k]);
+    const double2 x01 = v[5], x2 = v[6];
+    xl[k][0] = x01.x;
+    xl[k][1] = x01.y;
+    xl[k][2] = x2.x;
+  }
+  double ix[3][3], det;
+  tet_inverse(xl, ix, det);
+  const double WdetJ = cQ.Qwt * det;
+  double shg[4][3];
+  tet_shg(ix, shg);
+  double G[6], trG, GG;
+  tet_metric_tensor(ix, G, trG, GG);
+  double Nq[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) Nq[k] = sel4(cQ.N[0][k], cQ.N[1][k], cQ.N[2][k], cQ.N[3][k], q);
+  double Y[5], Yp[5], g[3][5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    Y[f] = 0.0;
+    Yp[f] = 0.0;
+    g[0][f] = g[1][f] = g[2][f] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2* vy = reinterpret_cast<const double2*>(A.node + 16 * (size_t)nd[k]);
+    const double2* vp = reinterpret_cast<const double2*>(A.yp + (size_t)A.yp_stride * nd[k]);
+    double y6[6], p6[6];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const double2 a = vy[t], b = vp[t];
+      y6[2 * t] = a.x;
+      y6[2 * t + 1] = a.y;
+      p6[2 * t] = b.x;
+      p6[2 * t + 1] = b.y;
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      Y[f] += Nq[k] * y6[f];
+      Yp[f] += Nq[k] * p6[f];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) g[i][f] += shg[k][i] * p6[f];
+    }
+  }
+  const double cp = ph.Rgas * ph.gamma / ph.gamma1;
+  const double mu = ph.mu, lam = -(2.0 / 3.0) * mu, kap = mu * cp / ph.pr;
+  Thermo t;
+  double ei, rk;
+  point_thermo(Y, ph, G, trG, GG, t, ei, rk);
+  // L = A_i yp_,i and M = L + c U(yp)
+  double M[5], tmp[5];
+  Ai_times(t, 0, g[0], M);
+#pragma unroll
+  for (int i = 1; i < 3; ++i) {
+    Ai_times(t, i, g[i], tmp);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) M[f] += tmp[f];
+  }
+  {
+    const double rkp = 0.5 * (Yp[1] * Yp[1] + Yp[2] * Yp[2] + Yp[3] * Yp[3]);
+    const double rhop = Yp[0] / (ph.Rgas * Yp[4]);
+    const double eip = Yp[4] * (ph.Rgas / ph.gamma1);
+    M[0] += A.fct1 * rhop;
+    M[1] += A.fct1 * (rhop * Yp[1]);
+    M[2] += A.fct1 * (rhop * Yp[2]);
+    M[3] += A.fct1 * (rhop * Yp[3]);
+    M[4] += A.fct1 * (rhop * (eip + rkp));
+  }
+  double TM[5];
+  TM[0] = t.tau[0] * M[0];
+  TM[1] = t.tau[1] * M[1];
+  TM[2] = t.tau[1] * M[2];
+  TM[3] = t.tau[1] * M[3];
+  TM[4] = t.tau[2] * M[4];
+  const double divu = g[0][1] + g[1][2] + g[2][3];
+  double F[3][5];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Ai_times(t, i, TM, F[i]);
+    double vf[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      vf[m] = mu * (g[i][1 + m] + g[m][1 + i]) + (m == i ? lam * divu : 0.0);
+      F[i][1 + m] += vf[m];
+    }
+    F[i][4] += t.u[0] * vf[0] + t.u[1] * vf[1] + t.u[2] * vf[2] + kap * g[i][4];
+  }
+  double mine[5];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double r5[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      r5[f] = WdetJ * (shg[a][0] * F[0][f] + shg[a][1] * F[1][f] + shg[a][2] * F[2][f] +
+                       Nq[a] * M[f]);
+      r5[f] += __shfl_xor(r5[f], 1, 64);
+      r5[f] += __shfl_xor(r5[f], 2, 64);
+      if (a == q) mine[f] = r5[f];
+    }
+  }
+  if (valid) {
+    double* ro = A.resid + ((size_t)e * 4 + q) * 5;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) ro[f] = A.iabs ? fabs(mine[f]) : mine[f];
+  }
+}
+
+// out (+)= the sum of the node's element contributions in element order
+// (the scatter of asires.f:91 / asimfg.f:96), then bc3Res's Dirichlet rows
+// (bc3res.f:25-153); the periodic fold follows (k_fold_res_bdiag)
+__global__ void __launch_bounds__(256) k_mres_nodes(const long long* __restrict__ goff,
+                                                    const int* __restrict__ inc,
+                                                    const double* __restrict__ resid,
+                                                    const uint8_t* __restrict__ bcm,
+                                                    double* __restrict__ out, long long nshg,
+                                                    int accumulate) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= nshg) return;
+  const long long g = n >> 6;
+  const int lane = (int)(n & 63);
+  const long long o0 = goff[g];
+  const int deg = (int)((goff[g + 1] - o0) >> 6);
+  double acc[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) acc[f] = accumulate ? out[n + f * nshg] : 0.0;
+  for (int j = 0; j < deg; ++j) {
+    const int code = inc[o0 + 64LL * j + lane];
+    if (code < 0) continue;
+    const double* r = resid + (size_t)code * 5;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) acc[f] += r[f];
+  }
+  const uint8_t m = bcm[n];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) out[n + f * nshg] = ((m >> f) & 1) ? 0.0 : acc[f];
+}
+
+// ypack(k) = the local-order variables U^-1 (ypre + s d) of the
+// preconditioned point (au1mfg.f:64-68, au2mfg.f:67-69, itrfdi.f:61-65;
+// the local order is yshuffle's old order, so no shuffle is needed).  With
+// bc, itrBC (itrbc.f:60-139, ires = 2: y only): the Dirichlet values of the
+// node's periodic master, whose values every slave copies.
+__global__ void k_mfg_point(const double* __restrict__ ypre, const double* __restrict__ d,
+                            double s, const double* __restrict__ D,  // nullptr: no U^-1
+                            const uint8_t* __restrict__ bcm, const double* __restrict__ BC,
+                            const int* __restrict__ iper, double* __restrict__ ypack,
+                            long long n, int bc) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const long long m = bc ? (long long)iper[k] : k;
+  double v[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) v[f] = d ? ypre[f * n + m] + s * d[f * n + m] : ypre[f * n + m];
+  // i3LU 'backward' (i3lu.f:119-139) with the node's factors
+  if (D) {
+#pragma unroll
+    for (int i = 4; i >= 0; --i) {
+      double acc = v[i];
+#pragma unroll
+      for (int c = 4; c > i; --c) acc -= v[c] * D[(i + 5 * c) * n + m];
+      v[i] = D[(i + 5 * i) * n + m] * acc;
+    }
+  }
+  if (bc) {
+    const uint8_t msk = bcm[m];
+    if (msk & 16) v[4] = BC[m + n];
+    if (msk & 2) {
+      v[1] = BC[m + 2 * n];
+      v[2] = BC[m + 3 * n];
+      v[3] = BC[m + 4 * n];
+    }
+    if (msk & 1) v[0] = BC[m];
+  }
+  double2* o = reinterpret_cast<double2*>(ypack + 8 * k);
+  o[0] = make_double2(v[0], v[1]);
+  o[1] = make_double2(v[2], v[3]);
+  o[2] = make_double2(v[4], 0.0);
+  o[3] = make_double2(0.0, 0.0);
+}
+
+// w = (L^-1 t - rmes) / e (au1mfg.f:76-83: i3LU 'forward', then the one-sided
+// difference quotient)
+__global__ void k_mfg_fwd_diff(const double* __restrict__ D, const double* __restrict__ t,
+                               const double* __restrict__ rmes, double e,
+                               double* __restrict__ w, long long n) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double v[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) v[i] = t[i * n + k];
+#pragma unroll
+  for (int i = 1; i < 5; ++i) {
+    double acc = v[i];
+#pragma unroll
+    for (int c = 0; c < i; ++c) acc -= D[(i + 5 * c) * n + k] * v[c];
+    v[i] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) w[i * n + k] = (v[i] - rmes[i * n + k]) / e;
+}
+
+// u = res - (u1 - u2) / (2 eps) (au2mfg.f:112), or with mode 1 itrFDI's
+// second difference d = (u1 - 2 rmes) / epsM (itrfdi.f:134), stored in u
+__global__ void k_mfg_combine(const double* __restrict__ a, const double* __restrict__ u1,
+                              const double* __restrict__ u2, double h, double* __restrict__ u,
+                              long long n5, int mode) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n5) return;
+  if (mode == 0)
+    u[k] = a[k] - (u1[k] - u2[k]) / h;
+  else
+    u[k] = (u1[k] - 2.0 * a[k]) / h;
+}
+
 int upload_quadrature() {
   if (g_quad_ready) return 0;
   Quad h{};
@@ -2019,5 +2275,60 @@ int launch_bc_res_bdiag(phg_ctx* c, bool prec) {
         c->res, c->bdiag, c->fold_master, c->fold_ptr, c->fold_slave, c->nfold, n, prec);
     PHG_HIP(hipGetLastError());
   }
+  return 0;
+}
+
+// The modified residual of the variables yp (stride 8, packed by
+// launch_mfg_point; nullptr: y itself, ElmMFG) at the resident state,
+// summed into out (accumulate: added to out's values), Dirichlet rows
+// zeroed and the periodic slaves folded (bc3res.f)
+int launch_mres(phg_ctx* c, const double* yp, double* out, bool accumulate, bool iabs) {
+  PHG_TRY(upload_quadrature());
+  const PhysParams ph = phys_of(c->prm);
+  MresArgs M{};
+  M.ien = c->ienc;
+  M.node = c->nodepack;
+  M.yp = yp ? yp : c->nodepack;
+  M.yp_stride = yp ? 8 : 16;
+  M.resid = c->resid;
+  M.nel = c->nel;
+  M.ph = ph;
+  M.fct1 = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
+  M.iabs = iabs ? 1 : 0;
+  k_mres<<<(int)((c->nel + 63) / 64), 256, 0, c->stream>>>(M);
+  PHG_HIP(hipGetLastError());
+  const long long n = c->nshg;
+  k_mres_nodes<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->inc_goff, c->inc, c->resid,
+                                                              c->bcm, out, n, accumulate ? 1 : 0);
+  PHG_HIP(hipGetLastError());
+  if (c->nfold > 0) {
+    k_fold_res_bdiag<<<(c->nfold + 255) / 256, 256, 0, c->stream>>>(
+        out, nullptr, c->fold_master, c->fold_ptr, c->fold_slave, c->nfold, n, 0);
+    PHG_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+int launch_mfg_point(phg_ctx* c, const double* ypre, const double* d, double s, double* ypack,
+                     bool bc, bool lu) {
+  const long long n = c->nshg;
+  k_mfg_point<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(
+      ypre, d, s, lu ? c->bdiag : nullptr, c->bcm, c->BCv, c->iper, ypack, n, bc ? 1 : 0);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_mfg_fwd_diff(phg_ctx* c, const double* t, const double* rmes, double e, double* w) {
+  const long long n = c->nshg;
+  k_mfg_fwd_diff<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(c->bdiag, t, rmes, e, w, n);
+  PHG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_mfg_combine(phg_ctx* c, const double* a, const double* u1, const double* u2, double h,
+                       double* u, int mode) {
+  const long long n5 = 5 * c->nshg;
+  k_mfg_combine<<<(int)((n5 + 255) / 256), 256, 0, c->stream>>>(a, u1, u2, h, u, n5, mode);
+  PHG_HIP(hipGetLastError());
   return 0;
 }

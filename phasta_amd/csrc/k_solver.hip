@@ -91,6 +91,48 @@ __global__ void k_lu_backward(const double* D, double* r, long long n) {
   for (int i = 0; i < 5; ++i) r[i * n + k] = v[i];
 }
 
+// i3LU 'product' (i3lu.f:150-167): r <- U r with U's diagonal stored
+// inverted (so r_i / D_ii), rows in ascending order (each row reads the
+// entries below it before they change)
+__global__ void k_lu_product(const double* D, double* r, long long n) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double v[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) v[i] = r[i * n + k];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    double s = v[i] / D[(i + 5 * i) * n + k];
+#pragma unroll
+    for (int m = i + 1; m < 5; ++m) s += v[m] * D[(i + 5 * m) * n + k];
+    v[i] = s;
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) r[i * n + k] = v[i];
+}
+
+// yshuffle (shuffle.f:1-30) from src into dst: new2old (u,v,w,p,T) ->
+// (p,u,v,w,T), else back
+__global__ void k_yshuffle(const double* src, double* dst, long long n, int new2old) {
+  long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double v[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) v[i] = src[i * n + k];
+  if (new2old) {
+    dst[k] = v[3];
+    dst[n + k] = v[0];
+    dst[2 * n + k] = v[1];
+    dst[3 * n + k] = v[2];
+  } else {
+    dst[k] = v[1];
+    dst[n + k] = v[2];
+    dst[2 * n + k] = v[3];
+    dst[3 * n + k] = v[0];
+  }
+  dst[4 * n + k] = v[4];
+}
+
 // ---------------- two-sided block scaling (spsi3pre.f) ----------------
 // lhsK(i,j) <- L_i^{-1} lhsK(i,j) U_j^{-1} with the factors of BDiag.  Reads
 // the assembled blocks (reference order, one padded record per block, 16-byte
@@ -665,6 +707,8 @@ int launch_i3lu(phg_ctx* c, double* diag, double* r, int code) {
     k_lu_forward<<<g, 256, 0, c->stream>>>(diag, r, n);
   else if (code == 2)
     k_lu_backward<<<g, 256, 0, c->stream>>>(diag, r, n);
+  else if (code == 3)
+    k_lu_product<<<g, 256, 0, c->stream>>>(diag, r, n);
   else {
     phg_set_error("i3LU: unsupported code %d", code);
     return PHG_EINVAL;
@@ -919,5 +963,12 @@ int launch_itrbc(phg_ctx* c, double* y, double* ac) {
                                                               c->nshg);
     PHG_HIP(hipGetLastError());
   }
+  return 0;
+}
+
+int launch_yshuffle(phg_ctx* c, const double* src, double* dst, bool new2old) {
+  const long long n = c->nshg;
+  k_yshuffle<<<grid_for(n, 256), 256, 0, c->stream>>>(src, dst, n, new2old ? 1 : 0);
+  PHG_HIP(hipGetLastError());
   return 0;
 }

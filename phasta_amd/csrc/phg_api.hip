@@ -553,6 +553,110 @@ int elm_core(phg_ctx* c) {
   return 0;
 }
 
+// ---------------- matrix-free GMRES (SolMFG, solmfg.f) ----------------
+int mfg_alloc(phg_ctx* c) {
+  const long long n = c->nshg;
+  if (!c->mfg.ypre) PHG_TRY(dev_alloc(c, &c->mfg.ypre, 5 * n));
+  if (!c->mfg.t1) PHG_TRY(dev_alloc(c, &c->mfg.t1, 5 * n));
+  if (!c->mfg.t2) PHG_TRY(dev_alloc(c, &c->mfg.t2, 5 * n));
+  if (!c->mfg.pack) PHG_TRY(dev_alloc(c, &c->mfg.pack, 8 * n));
+  return 0;
+}
+
+// Au1MFG (au1mfg.f:56-83): w = (L^-1 Rmod(itrBC(U^-1 (ypre + e v))) - rmes) / e
+// with the forward-reduced modified residual rmes of the base point
+int mfg_apply(phg_ctx* c, const double* v, double* w) {
+  PHG_TRY(launch_mfg_point(c, c->mfg.ypre, v, c->mfg.eg, c->mfg.pack, true, true));
+  PHG_TRY(launch_mres(c, c->mfg.pack, c->mfg.t1, false, false));
+  return launch_mfg_fwd_diff(c, c->mfg.t1, c->rmes, c->mfg.eg, w);
+}
+
+// Au2MFG (au2mfg.f:54-112): out = res - (L^-1 Rmod(y+) - L^-1 Rmod(y-)) / (2 eps),
+// y+- = itrBC(U^-1 (ypre +- eps Dy)), eps = epsM^(2/3) / ||Dy||
+int mfg_au2(phg_ctx* c, double* out) {
+  const long long V = 5 * c->nshg;
+  double* S = c->scal + 200;
+  PHG_TRY(launch_dot(c, c->dy, c->dy, V, S));
+  PHG_TRY(download(c, c->h_scal + 200, S, 1));
+  PHG_TRY(sync(c));
+  const double eps = std::pow(c->prm.epsM, 0.6666666666666666666666666666667) /
+                     std::sqrt(c->h_scal[200]);
+  PHG_TRY(launch_mfg_point(c, c->mfg.ypre, c->dy, eps, c->mfg.pack, true, true));
+  PHG_TRY(launch_mres(c, c->mfg.pack, c->mfg.t1, false, false));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->mfg.t1, 1));
+  PHG_TRY(launch_mfg_point(c, c->mfg.ypre, c->dy, -eps, c->mfg.pack, true, true));
+  PHG_TRY(launch_mres(c, c->mfg.pack, c->mfg.t2, false, false));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->mfg.t2, 1));
+  return launch_mfg_combine(c, c->res, c->mfg.t1, c->mfg.t2, 2.0 * eps, out, 0);
+}
+
+// itrFDI (itrfdi.f:38-141): eGMRES = 2 sqrt(epsA / SDnrm) with the
+// cancellation error epsA = epsM^2 ||L^-1 |Rmod|(ypre)|| and the second
+// difference SDnrm = ||(L^-1 (Rmod(ypre + h u) + Rmod(ypre - h u)) -
+// 2 rmes) / epsM||, h = sqrt(epsM); no itrBC on these points
+int mfg_fdi(phg_ctx* c, const double* u, double* eg) {
+  const long long V = 5 * c->nshg;
+  const double epsM = c->prm.epsM;
+  double* S = c->scal + 200;
+  PHG_TRY(launch_mfg_point(c, c->mfg.ypre, nullptr, 0.0, c->mfg.pack, false, true));
+  PHG_TRY(launch_mres(c, c->mfg.pack, c->mfg.t1, false, true));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->mfg.t1, 1));
+  PHG_TRY(launch_dot(c, c->mfg.t1, c->mfg.t1, V, S));
+  const double h = std::sqrt(epsM);
+  PHG_TRY(launch_mfg_point(c, c->mfg.ypre, u, h, c->mfg.pack, false, true));
+  PHG_TRY(launch_mres(c, c->mfg.pack, c->mfg.t1, false, false));
+  PHG_TRY(launch_mfg_point(c, c->mfg.ypre, u, -h, c->mfg.pack, false, true));
+  PHG_TRY(launch_mres(c, c->mfg.pack, c->mfg.t1, true, false));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->mfg.t1, 1));
+  PHG_TRY(launch_mfg_combine(c, c->rmes, c->mfg.t1, nullptr, epsM, c->mfg.t2, 1));
+  PHG_TRY(launch_dot(c, c->mfg.t2, c->mfg.t2, V, S + 1));
+  PHG_TRY(download(c, c->h_scal + 200, S, 2));
+  PHG_TRY(sync(c));
+  const double epsA = (epsM * epsM) * std::sqrt(c->h_scal[200]);
+  const double SDnrm = std::sqrt(c->h_scal[201]);
+  *eg = 2.0 * std::sqrt(epsA / SDnrm);
+  return 0;
+}
+
+// Flags the matrix-free path restates: the frozen set, idiff 0/1/3, lhs = 0
+// (itrdrv.f:494); iDC and itau != 0 are refused (e3dc's ires = 2 branch
+// reads giju, rTLS and raLS that e3tau forms only for ires 1 and 3,
+// e3tau.f:185-262; e3bdg_nd is not built)
+int mfg_check(phg_ctx* c) {
+  if (c->precision != PHG_FP64) {
+    phg_set_error("SolMFG: finite-difference products need a PHG_FP64 context");
+    return PHG_EINVAL;
+  }
+  if (c->prm.lhs != 0) {
+    phg_set_error("SolMFG: lhs must be 0 (itrdrv.f:494); got %d", c->prm.lhs);
+    return PHG_EINVAL;
+  }
+  if (c->prm.idc != 0 || c->prm.itau != 0) {
+    phg_set_error("SolMFG: iDC %d / itau %d not supported by the matrix-free path", c->prm.idc,
+                  c->prm.itau);
+    return PHG_EINVAL;
+  }
+  const int K = c->prm.Kspace;
+  if (K < 1 || K > c->kcap || K > 60 || c->prm.nGMRES < 1) {
+    phg_set_error("SolMFG: Kspace %d outside 1..%d or nGMRES %d < 1", K, std::min(c->kcap, 60),
+                  c->prm.nGMRES);
+    return PHG_EINVAL;
+  }
+  return check_idiff(c);
+}
+
+// ElmMFG (elmmfg.f:128-256) on the resident y/ac: res and BDiag as the
+// assembly forms them without lhsK (the tangent's diagonal blocks, which
+// e3bdg.f writes out term by term), and the modified residual rmes
+int elm_mfg(phg_ctx* c) {
+  const bool prec = c->prm.iprec != 0;
+  PHG_TRY(mfg_check(c));
+  PHG_TRY(mfg_alloc(c));
+  PHG_TRY(launch_assembly(c, false, prec));
+  PHG_TRY(launch_bc_res_bdiag(c, prec));
+  return launch_mres(c, nullptr, c->rmes, false, false);
+}
+
 // One GMRES cycle on the scaled system (solgmr.f:522-692): kvec(0) holds the
 // normalised start vector, unorm its norm.  Adds sum_k y_k u_k to c->dy.
 struct CycleOut {
@@ -633,9 +737,13 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
     }
     if (!have_w) {
       PHG_HIP(hipEventRecord(c->kev[0], c->stream));
-      PHG_TRY(launch_spmv(c, dir, w));
+      if (c->mfg.on) {
+        PHG_TRY(mfg_apply(c, dir, w));  // Au1MFG: bc3Res folds inside
+      } else {
+        PHG_TRY(launch_spmv(c, dir, w));
+      }
       PHG_HIP(hipEventRecord(c->kev[1], c->stream));
-      PHG_TRY(launch_bc3per(c, w, 5));
+      if (!c->mfg.on) PHG_TRY(launch_bc3per(c, w, 5));
     }
     PHG_HIP(hipEventRecord(c->kev[2], c->stream));
     if (c->precision == PHG_MIXED && iK <= kCgsVectors) {
@@ -709,6 +817,98 @@ int gmres_cycle(phg_ctx* c, double unorm, double epsnrm, phg_stats* st, CycleOut
   PHG_TRY(launch_combine(c, c->dy, flexible ? c->mp.zbasis : c->krylov, iKs, yB.data() + 1, V));
   out->iKs = iKs;
   out->echeck = std::fabs(e[iKs + 1]);
+  return 0;
+}
+
+// SolMFG (solmfg.f:84-381) after ElmMFG.  fdi: recompute the interval
+// (the reference does when iter = 1 and mod(istep, 20) = 0); otherwise the
+// context's interval (phg_solmfg's eGMRES) is used.  Leaves Dy in c->dy.
+int mfg_core(phg_ctx* c, int fdi, phg_stats* st) {
+  const long long n = c->nshg, V = 5 * n;
+  PHG_HIP(hipEventRecord(c->ev[0], c->stream));
+  PHG_TRY(elm_mfg(c));
+  PHG_HIP(hipEventRecord(c->ev[1], c->stream));
+  double* S = c->scal;  // [0] sum res^2, [3] unorm, [4] sum rmes^2
+  if (c->prm.iprec != 0) PHG_TRY(launch_i3lu(c, c->bdiag, nullptr, 0));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->res, 1));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->rmes, 1));
+  PHG_TRY(launch_dot(c, c->res, c->res, V, S));
+  PHG_TRY(launch_dot(c, c->rmes, c->rmes, V, S + 4));
+  PHG_TRY(launch_fill(c, c->dy, sizeof(double) * V));
+  PHG_TRY(launch_scale(c, kvec(c, 0), c->res, V, S, S + 3));
+  // ypre = U new2old(y) (solmfg.f:124-133)
+  PHG_TRY(launch_yshuffle(c, c->y, c->mfg.ypre, true));
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->mfg.ypre, 3));
+  PHG_TRY(download(c, c->h_scal, S, 8));
+  PHG_TRY(sync(c));
+  const double sumsq = c->h_scal[0], sumsq_m = c->h_scal[4];
+  double unorm = c->h_scal[3];
+  st->unorm0 = unorm;
+  st->ms_spmv = st->ms_mgs = 0.0;
+  st->n_spmv = st->n_mgs = 0;
+  st->orth_loss = 0.0;
+  st->refine_cycles = 0;
+  st->pc_fallback = 0;
+  st->nnan = st->nlarge = 0;
+  st->lGMRES = 0;
+  int iKs = 0;
+  double echeck = 0.0, epsnrm = 0.0;
+  if (!(unorm < 100. * c->prm.epsM * c->prm.epsM)) {
+    epsnrm = c->prm.etol * unorm;
+    if (fdi) PHG_TRY(mfg_fdi(c, c->res, &c->mfg.eg));
+    if (!(c->mfg.eg > 0.0) || !std::isfinite(c->mfg.eg)) {
+      phg_set_error("SolMFG: finite-difference interval eGMRES = %g (compute it with fdi = 1)",
+                    c->mfg.eg);
+      return PHG_EINVAL;
+    }
+    c->mfg.on = true;
+    int rc = 0;
+    for (int mG = 1; mG <= c->prm.nGMRES && rc == 0; ++mG) {
+      st->lGMRES = mG - 1;
+      if (mG > 1) {  // solmfg.f:181-202: R - A Dy, normalised into kvec(0)
+        rc = mfg_au2(c, kvec(c, 1));
+        if (!rc) rc = launch_dot(c, kvec(c, 1), kvec(c, 1), V, S + 5);
+        if (!rc) rc = launch_scale(c, kvec(c, 0), kvec(c, 1), V, S + 5, S + 6);
+        if (!rc) rc = download(c, c->h_scal + 6, S + 6, 1);
+        if (!rc) rc = sync(c);
+        if (rc) break;
+        unorm = c->h_scal[6];
+      }
+      CycleOut co;
+      rc = gmres_cycle(c, unorm, epsnrm, st, &co, false);
+      if (rc) break;
+      iKs = co.iKs;
+      echeck = co.echeck;
+      st->refine_cycles += 1;
+      if (echeck <= epsnrm) break;
+    }
+    c->mfg.on = false;
+    if (rc) return rc;
+  }
+  PHG_TRY(launch_i3lu(c, c->bdiag, c->dy, 2));
+  PHG_HIP(hipEventRecord(c->ev[2], c->stream));
+  PHG_TRY(sync(c));
+  // rstat.f:95-116 on res; solmfg.f:368 passes rstat no second array, so
+  // totres[1] reports the forward-reduced modified residual
+  st->totres[0] = std::sqrt(sumsq) / (double)(float)n;
+  st->totres[1] = std::sqrt(sumsq_m) / (double)(float)n;
+  st->iKs = iKs;
+  st->ntotGM = c->ntotGM;
+  st->epsnrm = epsnrm;
+  st->echeck = echeck;
+  st->converged = echeck <= epsnrm;
+  st->reslast = c->reslast;
+  float ms1 = 0, ms2 = 0;
+  PHG_HIP(hipEventElapsedTime(&ms1, c->ev[0], c->ev[1]));
+  PHG_HIP(hipEventElapsedTime(&ms2, c->ev[1], c->ev[2]));
+  st->ms_assembly = ms1;
+  st->ms_solve = ms2;
+  float ma = 0, mb = 0;
+  PHG_HIP(hipEventElapsedTime(&ma, c->aev[0], c->aev[1]));
+  PHG_HIP(hipEventElapsedTime(&mb, c->aev[1], c->aev[2]));
+  st->ms_point = ma;
+  st->ms_blocks = mb;
+  st->ms_total = ms1 + ms2;
   return 0;
 }
 
@@ -981,7 +1181,7 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->acold,    c->dy,       c->krylov, c->red_part, c->scal,
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
                   c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode,
-                  c->dcw,      c->mtau};
+                  c->dcw,      c->mtau,     c->mfg.ypre, c->mfg.t1, c->mfg.t2,  c->mfg.pack};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -1133,6 +1333,94 @@ extern "C" int phg_update(phg_ctx* c) {
   return 0;
 }
 
+// ---------------- matrix-free GMRES ----------------
+extern "C" int phg_elmmfg(phg_ctx* c, const double* y, const double* ac, double* res, double* rmes,
+                          double* BDiag) {
+  PHG_TRY(check_ctx(c));
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->y, y, 5 * n));
+  PHG_TRY(upload(c, c->ac, ac, 5 * n));
+  PHG_TRY(elm_mfg(c));
+  if (res) PHG_TRY(download(c, res, c->res, 5 * n));
+  if (rmes) PHG_TRY(download(c, rmes, c->rmes, 5 * n));
+  if (BDiag && c->prm.iprec != 0) PHG_TRY(download(c, BDiag, c->bdiag, 25 * n));
+  return sync(c);
+}
+
+extern "C" int phg_itrres(phg_ctx* c, const double* yp, const double* yc, const double* ac,
+                          double* rmes, int iabres) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(mfg_check(c));
+  PHG_TRY(mfg_alloc(c));
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->y, yc, 5 * n));
+  PHG_TRY(upload(c, c->ac, ac, 5 * n));
+  PHG_TRY(launch_pack_nodes(c));
+  PHG_TRY(upload(c, c->mfg.t2, yp, 5 * n));
+  PHG_TRY(launch_yshuffle(c, c->mfg.t2, c->mfg.t2, true));
+  PHG_TRY(launch_mfg_point(c, c->mfg.t2, nullptr, 0.0, c->mfg.pack, false, false));
+  PHG_TRY(upload(c, c->mfg.t1, rmes, 5 * n));
+  PHG_TRY(launch_mres(c, c->mfg.pack, c->mfg.t1, true, iabres != 0));
+  PHG_TRY(download(c, rmes, c->mfg.t1, 5 * n));
+  return sync(c);
+}
+
+extern "C" int phg_au1mfg(phg_ctx* c, const double* ypre, const double* y, const double* ac,
+                          const double* rmes, const double* BDiag, double eGMRES, double* u) {
+  PHG_TRY(check_ctx(c));
+  PHG_TRY(mfg_check(c));
+  PHG_TRY(mfg_alloc(c));
+  if (!(eGMRES != 0.0) || !std::isfinite(eGMRES)) {
+    phg_set_error("Au1MFG: eGMRES = %g", eGMRES);
+    return PHG_EINVAL;
+  }
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->y, y, 5 * n));
+  PHG_TRY(upload(c, c->ac, ac, 5 * n));
+  PHG_TRY(launch_pack_nodes(c));
+  PHG_TRY(upload(c, c->bdiag, BDiag, 25 * n));
+  PHG_TRY(upload(c, c->mfg.ypre, ypre, 5 * n));
+  PHG_TRY(upload(c, c->rmes, rmes, 5 * n));
+  PHG_TRY(upload(c, c->mfg.t2, u, 5 * n));
+  const double keep = c->mfg.eg;
+  c->mfg.eg = eGMRES;
+  const int rc = mfg_apply(c, c->mfg.t2, c->mfg.t2);
+  c->mfg.eg = keep;
+  PHG_TRY(rc);
+  PHG_TRY(download(c, u, c->mfg.t2, 5 * n));
+  return sync(c);
+}
+
+extern "C" int phg_solmfg(phg_ctx* c, const double* y, const double* ac, double* res, double* rmes,
+                          double* BDiag, double* Dy, double* eGMRES, int fdi, phg_stats* st) {
+  PHG_TRY(check_ctx(c));
+  phg_stats local{};
+  if (!st) st = &local;
+  const long long n = c->nshg;
+  PHG_TRY(upload(c, c->y, y, 5 * n));
+  PHG_TRY(upload(c, c->ac, ac, 5 * n));
+  if (eGMRES) c->mfg.eg = *eGMRES;
+  PHG_TRY(mfg_core(c, fdi, st));
+  if (eGMRES) *eGMRES = c->mfg.eg;
+  if (res) PHG_TRY(download(c, res, c->res, 5 * n));
+  if (rmes) PHG_TRY(download(c, rmes, c->rmes, 5 * n));
+  if (BDiag) PHG_TRY(download(c, BDiag, c->bdiag, 25 * n));
+  if (Dy) PHG_TRY(download(c, Dy, c->dy, 5 * n));
+  return sync(c);
+}
+
+extern "C" int phg_newton_iteration_mfg(phg_ctx* c, double* eGMRES, int fdi, phg_stats* st) {
+  PHG_TRY(check_ctx(c));
+  phg_stats local{};
+  if (!st) st = &local;
+  c->prm.iter += 1;  // itrdrv.f:434
+  if (eGMRES) c->mfg.eg = *eGMRES;
+  PHG_TRY(mfg_core(c, fdi, st));
+  if (eGMRES) *eGMRES = c->mfg.eg;
+  PHG_TRY(launch_correct(c));
+  return 0;
+}
+
 extern "C" int phg_sparse_ap(phg_ctx* c, const double* lhsK, double* p) {
   PHG_TRY(check_ctx(c));
   PHG_TRY(require_fp64(c, "phg_sparse_ap"));
@@ -1182,6 +1470,7 @@ extern "C" int phg_i3lu(phg_ctx* c, double* Diag, double* r, const char* code) {
   if (code && !std::strncmp(code, "LU_Fact ", 8)) k = 0;
   if (code && !std::strncmp(code, "forward ", 8)) k = 1;
   if (code && !std::strncmp(code, "backward", 8)) k = 2;
+  if (code && !std::strncmp(code, "product ", 8)) k = 3;
   if (k < 0) {
     phg_set_error("i3LU: unknown code");  // i3lu.f:175 calls error()
     return PHG_EINVAL;

@@ -271,6 +271,16 @@ struct phg_ctx {
 
   MpNet mp;
 
+  // matrix-free GMRES (SolMFG): allocated on first use
+  struct {
+    bool on = false;         // gmres_cycle's product is Au1MFG
+    double eg = 0.0;         // the finite-difference interval of the solve
+    double* ypre = nullptr;  // U new2old(y), 5 planes
+    double* t1 = nullptr;    // 5 planes
+    double* t2 = nullptr;    // 5 planes
+    double* pack = nullptr;  // 8 per node: the perturbed variables
+  } mfg;
+
   // timing events on the library stream, created once
   hipEvent_t ev[3] = {};
   hipEvent_t kev[4] = {};
@@ -287,6 +297,20 @@ int launch_bc_res_bdiag(phg_ctx* c, bool prec);
 // the operator of SparseAp + bc3per before the two-sided scaling.  Uses
 // c->resid as scratch, so it runs after the assembly's block pass.
 int launch_jv(phg_ctx* c, const double* v, double* w);
+
+// matrix-free GMRES (SolMFG): the modified residual of the packed variables
+// yp (8 per node; nullptr: y itself) at the resident state into out
+// (+= with accumulate), after bc3Res; c->resid is its scratch
+int launch_mres(phg_ctx* c, const double* yp, double* out, bool accumulate, bool iabs);
+// ypack = U^-1 (ypre + s d) per node in the local order (d may be nullptr;
+// without lu no U^-1), with itrBC's Dirichlet values and periodic copy when bc
+int launch_mfg_point(phg_ctx* c, const double* ypre, const double* d, double s, double* ypack,
+                     bool bc, bool lu);
+// w = (L^-1 t - rmes) / e
+int launch_mfg_fwd_diff(phg_ctx* c, const double* t, const double* rmes, double e, double* w);
+// mode 0: u = a - (u1 - u2) / h; mode 1: u = (u1 - 2 a) / h
+int launch_mfg_combine(phg_ctx* c, const double* a, const double* u1, const double* u2, double h,
+                       double* u, int mode);
 
 // ---- learned preconditioner (k_mpnn.hip) ----
 int mp_param_count();
@@ -349,3 +373,5 @@ int launch_predict(phg_ctx* c);
 int launch_correct(phg_ctx* c);
 int launch_update(phg_ctx* c);
 int launch_itrbc(phg_ctx* c, double* y, double* ac);
+// yshuffle of 5-plane vectors (src and dst may be the same)
+int launch_yshuffle(phg_ctx* c, const double* src, double* dst, bool new2old);

@@ -73,7 +73,9 @@ class PhgStats(ctypes.Structure):
 EXPORTS = ("phg_init", "phg_finalize", "phg_set_params", "phg_last_error", "phg_info",
            "phg_elmgmrs", "phg_solgmrs", "phg_state_upload", "phg_state_download",
            "phg_state_download_old", "phg_state_upload_cur", "phg_predict",
-           "phg_newton_iteration", "phg_update", "phg_asiq", "phg_sparse_ap", "phg_sparse_ap_t",
+           "phg_newton_iteration", "phg_update", "phg_elmmfg", "phg_itrres", "phg_au1mfg",
+           "phg_solmfg",
+           "phg_newton_iteration_mfg", "phg_asiq", "phg_sparse_ap", "phg_sparse_ap_t",
            "phg_tangent_ap",
            "phg_i3lu", "phg_spsi3pre", "phg_spsi3pre_noalloc", "phg_bc3per",
            "phg_bench_spmv", "phg_bench_assembly", "phg_bench_copy", "phg_sync",
@@ -195,6 +197,65 @@ class GpuSolver:
                                  _p(dy), ctypes.byref(st)))
         out = st.as_dict()
         out.update(res=res, rmes=rmes, BDiag=bd, lhsK=lk, Dy=dy)
+        return out
+
+    # ---- matrix-free GMRES (SolMFG) ----
+    def elmmfg(self, y, ac):
+        """ElmMFG: (res, rmes, BDiag); BDiag only with iprec != 0."""
+        n = self.case.nshg
+        y, ac = _f64(y, (n, 5)), _f64(ac, (n, 5))
+        res = np.zeros((n, 5), order="F")
+        rmes = np.zeros((n, 5), order="F")
+        bd = np.zeros((n, 5, 5), order="F")
+        _check(lib().phg_elmmfg(self.h, _p(y), _p(ac), _p(res), _p(rmes), _p(bd)))
+        return res, rmes, bd
+
+    def itrres(self, yp, yc, ac, rmes=None, iabres=False):
+        """ItrRes: rmes (zeros if None) += the modified residual of yp at
+        the state yc."""
+        n = self.case.nshg
+        rmes = np.zeros((n, 5), order="F") if rmes is None else _f64(rmes, (n, 5)).copy(order="F")
+        _check(lib().phg_itrres(self.h, _p(_f64(yp, (n, 5))), _p(_f64(yc, (n, 5))),
+                                _p(_f64(ac, (n, 5))), _p(rmes), int(bool(iabres))))
+        return rmes
+
+    def au1mfg(self, ypre, y, ac, rmes, bd_lu, egmres, u):
+        """Au1MFG: the one-sided difference product of u at the state y."""
+        n = self.case.nshg
+        u = _f64(u, (n, 5)).copy(order="F")
+        _check(lib().phg_au1mfg(self.h, _p(_f64(ypre, (n, 5))), _p(_f64(y, (n, 5))),
+                                _p(_f64(ac, (n, 5))), _p(_f64(rmes, (n, 5))),
+                                _p(_f64(bd_lu, (n, 5, 5))), ctypes.c_double(egmres), _p(u)))
+        return u
+
+    def solmfg(self, y, ac, egmres=0.0, fdi=True):
+        """SolMFG on host arrays; returns the stats dict with res, rmes,
+        BDiag, Dy and the interval eGMRES."""
+        n = self.case.nshg
+        y, ac = _f64(y, (n, 5)), _f64(ac, (n, 5))
+        res = np.zeros((n, 5), order="F")
+        rmes = np.zeros((n, 5), order="F")
+        bd = np.zeros((n, 5, 5), order="F")
+        dy = np.zeros((n, 5), order="F")
+        eg = ctypes.c_double(egmres)
+        st = PhgStats()
+        _check(lib().phg_solmfg(self.h, _p(y), _p(ac), _p(res), _p(rmes), _p(bd), _p(dy),
+                                ctypes.byref(eg), int(bool(fdi)), ctypes.byref(st)))
+        out = st.as_dict()
+        out.update(res=res, rmes=rmes, BDiag=bd, Dy=dy, eGMRES=eg.value)
+        return out
+
+    def newton_iteration_mfg(self, egmres=None, fdi=False):
+        """SolMFG + itrCorrect + itrBC on the resident state.  egmres: the
+        interval to use (None: the context's); returns the stats dict with
+        the interval used."""
+        eg = ctypes.c_double(0.0 if egmres is None else egmres)
+        st = PhgStats()
+        _check(lib().phg_newton_iteration_mfg(self.h, None if egmres is None else ctypes.byref(eg),
+                                              int(bool(fdi)), ctypes.byref(st)))
+        out = st.as_dict()
+        if egmres is not None:
+            out["eGMRES"] = eg.value
         return out
 
     def asiq(self, y):

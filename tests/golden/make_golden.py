@@ -130,15 +130,52 @@ def make_mtau(N, seed):
     np.savez_compressed(os.path.join(HERE, f"ref_mtau_box{N}.npz"), N=N, y=y, ac=ac, **out)
 
 
+def make_mfg(N, seed):
+    """The matrix-free solver's pieces (lhs = 0, iprec = 1) at a perturbed
+    state on a jittered box: ElmMFG (res, the modified residual rmes, e3bdg's
+    BDiag), ItrRes of perturbed variables (plain and with iabres), the
+    preconditioned base point ypre = U new2old(y) (i3LU 'product'), itrFDI's
+    interval eGMRES, Au1MFG's one-sided and Au2MFG's two-sided products; and
+    ElmMFG with idiff = 1."""
+    case = mesh.box_case(N, mach=0.5, kspace=10, jitter=0.1)
+    rng = np.random.default_rng(seed)
+    y, ac = pyref.Reference(case).predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    prm = dict(case.params, lhs=0, iprec=1)
+    R = pyref.Reference(case, params=prm)
+    res, rmes, bd = R.elmmfg(y, ac)
+    yp = np.asfortranarray(y * (1 + 0.01 * rng.standard_normal(y.shape)))
+    rm_p = R.itrres(yp, y, ac)
+    rm_abs = R.itrres(yp, y, ac, iabres=True)
+    bd_lu, _ = R.i3lu(bd, None, "LU_Fact ")
+    _, res_f = R.i3lu(bd_lu, res, "forward ")
+    _, rmes_f = R.i3lu(bd_lu, rmes, "forward ")
+    _, ypre = R.i3lu(bd_lu, R.yshuffle(y, "new2old "), "product ")
+    eg = R.itrfdi(ypre, y, ac, rmes_f, res_f, bd_lu)
+    u = np.asfortranarray(rng.standard_normal(y.shape))
+    au1 = R.au1mfg(ypre, y, ac, rmes_f, res_f, bd_lu, eg, u)
+    dy = np.asfortranarray(rng.standard_normal(y.shape))
+    au2 = R.au2mfg(ypre, y, ac, rmes_f, res_f, dy, bd_lu)
+    res1, rmes1, bd1 = pyref.Reference(case, params=dict(prm, idiff=1)).elmmfg(y, ac)
+    np.savez_compressed(os.path.join(HERE, f"ref_mfg_box{N}.npz"), N=N, y=y, ac=ac, res=res,
+                        rmes=rmes, BDiag=bd, yp=yp, rmes_p=rm_p, rmes_abs=rm_abs,
+                        BDiag_lu=bd_lu, res_f=res_f, rmes_f=rmes_f, ypre=ypre, eGMRES=eg,
+                        u=u, au1=au1, dy=dy, au2=au2, res_idiff=res1, rmes_idiff=rmes1,
+                        BDiag_idiff=bd1)
+
+
+MAKERS = {
+    "box": lambda: [make(N, seed) for N, seed in ((2, 11), (3, 12))],
+    "alpha": lambda: make_alpha(3, 13),
+    "idiff": lambda: make_idiff(3, 14),
+    "dc": lambda: make_dc(3, 15),
+    "mtau": lambda: make_mtau(3, 17),
+    "mfg": lambda: make_mfg(3, 19),
+}
+
 if __name__ == "__main__":
-    for N, seed in ((2, 11), (3, 12)):
-        make(N, seed)
-        print("wrote", f"ref_box{N}.npz")
-    make_alpha(3, 13)
-    print("wrote ref_alpha_box3.npz")
-    make_idiff(3, 14)
-    print("wrote ref_idiff_box3.npz")
-    make_dc(3, 15)
-    print("wrote ref_dc_box3.npz")
-    make_mtau(3, 17)
-    print("wrote ref_mtau_box3.npz")
+    # python tests/golden/make_golden.py [name ...]: all fixtures, or the named ones
+    for name in sys.argv[1:] or list(MAKERS):
+        MAKERS[name]()
+        print("wrote", name)

@@ -1,0 +1,220 @@
+"""Pins of the CPU oracle's matrix-free GMRES (SolMFG, oracle/phoracle.c).
+
+1. Bitwise against the reference's own compiled Fortran: the committed
+   fixture tests/golden/ref_mfg_box3.npz (ElmMFG's res, modified residual
+   rmes and e3bdg's BDiag; ItrRes plain and with iabres; i3LU 'product';
+   itrFDI's interval; Au1MFG's and Au2MFG's products; ElmMFG with idiff 1)
+   and, where oracle/_ref is built, live calls on other states.
+2. What the mathematics fixes:
+   * e3bdg writes out the diagonal blocks of the assembled tangent term by
+     term: the oracle's e3bdg BDiag equals ElmGMRs's diagonal blocks to
+     rounding;
+   * the modified residual of yp at the state y is the residual linearised
+     with frozen coefficients, so its derivative along u is the assembled
+     tangent: Au1MFG's difference quotient equals SolGMRs' scaled matvec
+     L^-1 K U^-1 u (SparseAp + bc3per on the Spsi3pre-scaled lhsK) to the
+     O(eGMRES) error of a one-sided difference, and Au2MFG's two-sided one
+     equals res - K Dy to O(eps^2);
+   * the ordering shuffle and the block product invert each other;
+   * SolMFG's Newton increment agrees with SolGMRs' on the same state to
+     the accuracy of the Krylov tolerance.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+import pyref
+from phasta_amd import mesh
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def perturbed(N, seed, grade=1.0, bc=False):
+    """A jittered box at a perturbed predicted state; bc: with itrBC applied
+    again, so the Dirichlet values hold (the fixtures' states do not)."""
+    case = mesh.box_case(N, mach=0.5, kspace=10, jitter=0.1, grade=grade)
+    rng = np.random.default_rng(seed)
+    o = pyoracle.Oracle(case)
+    y, ac = o.itr_predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    if bc:
+        assert pyoracle.lib().orc_itr_bc(ctypes.byref(o.m), pyoracle._p(y), pyoracle._p(ac)) == 0
+    return case, y, ac, rng
+
+
+def mfg_params(case, **over):
+    """The flags itrdrv sets for SolMFG (lhs = 0, iprec = 1; itrdrv.f:494-495)."""
+    prm = dict(case.params, lhs=0, iprec=1)
+    prm.update(over)
+    return prm
+
+
+def preconditioned(o, res, rmes, bd, y):
+    """SolMFG's set-up (solmfg.f:101-133): factored BDiag, forward-reduced
+    res and rmes, ypre = U new2old(y)."""
+    lu = bd.copy(order="F")
+    o.i3lu(lu, None, "LU_Fact ")
+    r, m = res.copy(order="F"), rmes.copy(order="F")
+    o.i3lu(lu, r, "forward ")
+    o.i3lu(lu, m, "forward ")
+    ypre = o.yshuffle(y, "new2old ")
+    o.i3lu(lu, ypre, "product ")
+    return lu, r, m, ypre
+
+
+# ---------------------------------------------------------------- golden
+def test_mfg_pieces_bitwise_equal_reference_golden():
+    g = np.load(os.path.join(GOLD, "ref_mfg_box3.npz"))
+    case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+    o = pyoracle.Oracle(case, params=mfg_params(case))
+    y, ac = np.asfortranarray(g["y"]), np.asfortranarray(g["ac"])
+    res, rmes, bd = o.elmmfg(y, ac)
+    assert np.array_equal(res, g["res"])
+    assert np.array_equal(rmes, g["rmes"])
+    assert np.array_equal(bd, g["BDiag"])
+    assert np.array_equal(o.itrres(g["yp"], y, ac), g["rmes_p"])
+    assert np.array_equal(o.itrres(g["yp"], y, ac, iabres=True), g["rmes_abs"])
+    lu, r, m, ypre = preconditioned(o, res, rmes, bd, y)
+    assert np.array_equal(lu, g["BDiag_lu"])
+    assert np.array_equal(r, g["res_f"]) and np.array_equal(m, g["rmes_f"])
+    assert np.array_equal(ypre, g["ypre"])
+    eg = o.itrfdi(ypre, y, ac, m, r, lu)
+    assert eg == float(g["eGMRES"])
+    assert np.array_equal(o.au1mfg(ypre, y, ac, m, lu, eg, g["u"]), g["au1"])
+    assert np.array_equal(o.au2mfg(ypre, y, ac, r, g["dy"], lu), g["au2"])
+    o1 = pyoracle.Oracle(case, params=mfg_params(case, idiff=1))
+    res1, rmes1, bd1 = o1.elmmfg(y, ac)
+    assert np.array_equal(res1, g["res_idiff"])
+    assert np.array_equal(rmes1, g["rmes_idiff"])
+    assert np.array_equal(bd1, g["BDiag_idiff"])
+
+
+@pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("N,grade,idiff", [(4, 1.1, 0), (3, 1.0, 3)])
+def test_mfg_pieces_bitwise_equal_live_reference(N, grade, idiff):
+    case, y, ac, rng = perturbed(N, 40 + N, grade)
+    prm = mfg_params(case, idiff=idiff)
+    o, R = pyoracle.Oracle(case, params=prm), pyref.Reference(case, params=prm)
+    res, rmes, bd = o.elmmfg(y, ac)
+    r2, m2, b2 = R.elmmfg(y, ac)
+    assert np.array_equal(res, r2) and np.array_equal(rmes, m2) and np.array_equal(bd, b2)
+    yp = np.asfortranarray(y * (1 + 0.02 * rng.standard_normal(y.shape)))
+    acc = np.asfortranarray(rng.standard_normal(y.shape))
+    # ItrRes accumulates into its output
+    assert np.array_equal(o.itrres(yp, y, ac, rmes=acc.copy(order="F")),
+                          R.itrres(yp, y, ac, rmes=acc))
+    lu, r, m, ypre = preconditioned(o, res, rmes, bd, y)
+    eg = o.itrfdi(ypre, y, ac, m, r, lu)
+    assert eg == R.itrfdi(ypre, y, ac, m, r, lu)
+    u = np.asfortranarray(rng.standard_normal(y.shape))
+    assert np.array_equal(o.au1mfg(ypre, y, ac, m, lu, eg, u),
+                          R.au1mfg(ypre, y, ac, m, r, lu, eg, u))
+
+
+# ---------------------------------------------------------------- mathematics
+def test_e3bdg_is_the_tangent_diagonal():
+    case, y, ac, _ = perturbed(4, 3, grade=1.1)
+    _, _, bd = pyoracle.Oracle(case, params=mfg_params(case)).elmmfg(y, ac)
+    _, _, bd_t, _ = pyoracle.Oracle(case).elmgmrs(y, ac)
+    assert rel(bd, bd_t) < 1e-13
+
+
+def test_shuffle_and_block_product_invert():
+    case, y, ac, _ = perturbed(3, 5)
+    o = pyoracle.Oracle(case, params=mfg_params(case))
+    _, _, bd = o.elmmfg(y, ac)
+    lu = bd.copy(order="F")
+    o.i3lu(lu, None, "LU_Fact ")
+    v = o.yshuffle(y, "new2old ")
+    assert np.array_equal(v[:, 0], y[:, 3]) and np.array_equal(v[:, 1:4], y[:, 0:3])
+    assert np.array_equal(o.yshuffle(v, "old2new "), y)
+    w = v.copy(order="F")
+    o.i3lu(lu, w, "product ")
+    o.i3lu(lu, w, "backward")
+    assert rel(w, v) < 1e-13
+
+
+def scaled_matvec(o, y, ac, u):
+    """SolGMRs' operator on the same state: L^-1 bc3per(K (U^-1 u)(iper))
+    with K the assembled tangent, through the Spsi3pre-scaled lhsK."""
+    _, _, bd, lk = o.elmgmrs(y, ac)
+    lu = bd.copy(order="F")
+    o.i3lu(lu, None, "LU_Fact ")
+    ls = lk.copy(order="F")
+    o.spsi3pre(lu, ls)
+    p = u.copy(order="F")
+    o.sparse_ap(ls, p)
+    o.bc3per(p)
+    return p
+
+
+@pytest.mark.parametrize("N,grade", [(3, 1.0), (4, 1.1)])
+def test_fd_product_is_the_scaled_tangent(N, grade):
+    case, y, ac, rng = perturbed(N, 7 + N, grade, bc=True)
+    o = pyoracle.Oracle(case, params=mfg_params(case))
+    res, rmes, bd = o.elmmfg(y, ac)
+    lu, r, m, ypre = preconditioned(o, res, rmes, bd, y)
+    eg = o.itrfdi(ypre, y, ac, m, r, lu)
+    # a direction in the Krylov space's range: zero on the constrained
+    # equations and on periodic slaves, as res is
+    u = np.asfortranarray(rng.standard_normal(y.shape))
+    u[r == 0.0] = 0.0
+    kv = scaled_matvec(pyoracle.Oracle(case), y, ac, u)
+
+    def err(f):
+        return rel(o.au1mfg(ypre, y, ac, m, lu, f * eg, u), kv)
+
+    # measured 5e-8 at the interval itrFDI chooses
+    assert err(1.0) < 1e-6
+    # below it the cancellation error grows as 1/interval, far above it the
+    # truncation error of the one-sided difference as the interval: a term
+    # missing from the modified residual or the tangent would leave an O(1)
+    # difference at every interval
+    assert 7 < err(0.01) / err(0.1) < 13
+    assert 9 < err(1e5) / err(1e4) < 11
+    # Au2MFG: res - K Dy by a two-sided difference (second order)
+    dy = u
+    kd = scaled_matvec(pyoracle.Oracle(case), y, ac, dy)
+    assert rel(o.au2mfg(ypre, y, ac, r, dy, lu), r - kd) < 1e-6
+
+
+@pytest.mark.parametrize("N,grade", [(4, 1.0), (8, 1.1)])
+def test_solmfg_increment_matches_solgmrs(N, grade):
+    case = mesh.box_case(N, mach=0.5, kspace=20, jitter=0.1, grade=grade)
+    o = pyoracle.Oracle(case)
+    y, ac = o.itr_predict_bc(case.yold)
+    out = o.solmfg(y, ac, fdi=True, lhs=0, iprec=1)
+    g = pyoracle.Oracle(case).solgmrs(y, ac)
+    assert out["converged"] and g["converged"]
+    assert abs(out["iKs"] - g["iKs"]) <= 1
+    # both stop at ||r|| <= 1e-3 ||r0||: the increments agree to about that
+    assert rel(out["Dy"], g["Dy"]) < 5e-3
+    assert 0 < out["eGMRES"] < 1e-3
+
+
+def test_restart_cycles_use_the_two_sided_residual():
+    """nGMRES = 3 with Kspace 2: later cycles restart from Au2MFG's
+    res - A Dy, and the cycles drive the residual down."""
+    case = mesh.box_case(4, mach=0.5, kspace=2, jitter=0.1)
+    o = pyoracle.Oracle(case)
+    y, ac = o.itr_predict_bc(case.yold)
+    one = o.solmfg(y, ac, fdi=True, lhs=0, iprec=1, nGMRES=1)
+    three = pyoracle.Oracle(case).solmfg(y, ac, fdi=True, lhs=0, iprec=1, nGMRES=3)
+    assert three["lGMRES"] >= 1
+    assert three["echeck"] < one["echeck"]
+
+
+def test_mfg_refuses_flags_it_does_not_restate():
+    case, y, ac, _ = perturbed(3, 9)
+    for over in ({"idc": 1}, {"itau": 10}, {"idiff": 2}):
+        with pytest.raises(RuntimeError, match="rc=5"):
+            pyoracle.Oracle(case, params=mfg_params(case, **over)).elmmfg(y, ac)
