@@ -9,11 +9,13 @@ predictor restarts the time step from the same yold, so each timed step
 repeats a real Newton iteration of the workload.
 
 The default workload is the 128^3 y-graded box the metric is quoted on.
-Three arms run on the same state: plain FP64 block-diagonal GMRES ("fp64"),
+The arms run on the same state: plain FP64 block-diagonal GMRES ("fp64"),
 the mixed-precision form (float tangent and matrix, double Krylov/MGS/Givens
 and refinement residual) with block-diagonal GMRES ("mixed"), and the same
 with the learned preconditioner on the first direction of every flexible
-GMRES cycle ("learned", weights phasta_amd/weights/mpnn_box64.npz).
+GMRES cycle ("learned", weights phasta_amd/weights/mpnn_box64.npz); a
+fourth arm runs the matrix-free solver SolMFG in FP64 ("mfg": no lhsK, each
+Krylov product one modified-residual pass over the elements).
 value = interior tets per second of the learned arm, the form the metric
 names (nel / time per Newton iteration); every arm's time, its ratio to
 FP64 and its update's error against FP64 are reported beside it.  Inputs
@@ -88,12 +90,18 @@ def cpu_newton(case, threads):
     return dt, used, out
 
 
-def run_arm(case, precision, newton, steps, warmup, weights=None, idiff=0):
+def run_arm(case, precision, newton, steps, warmup, weights=None, idiff=0, mfg=False):
     """Warm-up, then `steps` timed Newton iterations; returns timings, the
-    per-step statistics and the update of the first Newton iteration."""
+    per-step statistics and the update of the first Newton iteration.
+    mfg: matrix-free GMRES (SolMFG, lhs = 0), its interval eGMRES computed by
+    itrFDI at the first Newton iteration only (the reference recomputes it
+    once every 20 time steps, solmfg.f:163) and held by the context."""
     import numpy as np
     from phasta_amd import solver
-    gpu = solver.GpuSolver(case, params=dict(case.params, idiff=idiff), precision=precision)
+    prm = dict(case.params, idiff=idiff)
+    if mfg:
+        prm.update(lhs=0, iprec=1)
+    gpu = solver.GpuSolver(case, params=prm, precision=precision)
     if weights is not None:
         w = np.load(weights)
         gpu.mpnn_init(w["dscale"])
@@ -102,7 +110,7 @@ def run_arm(case, precision, newton, steps, warmup, weights=None, idiff=0):
     acold = np.zeros((case.nshg, 5), order="F")
     gpu.upload_state(case.yold, acold)
     gpu.predict()
-    first = gpu.newton_iteration()
+    first = gpu.newton_iteration_mfg(egmres=0.0, fdi=True) if mfg else gpu.newton_iteration()
     _, _, dy_first = gpu.download_state()
     gpu.upload_state(case.yold, acold)
     it = [0]
@@ -110,7 +118,7 @@ def run_arm(case, precision, newton, steps, warmup, weights=None, idiff=0):
     def step():
         if it[0] % newton == 0:
             gpu.predict()
-        st = gpu.newton_iteration()
+        st = gpu.newton_iteration_mfg() if mfg else gpu.newton_iteration()
         it[0] += 1
         return st
 
@@ -128,7 +136,7 @@ def run_arm(case, precision, newton, steps, warmup, weights=None, idiff=0):
     # between two HIP events on the library stream: the kernel's own time,
     # without the launch gaps that the per-launch events inside GMRES see
     # (the host waits for each Hessenberg column before the next launch)
-    ms_mv, by_mv = gpu.bench_spmv(20)
+    ms_mv, by_mv = gpu.bench_spmv(20) if not mfg else (float("nan"), float("nan"))
     info = gpu.info()
     gpu.close()
     return dict(ms=(t1 - t0) * 1e3 / steps, stats=stats, first=first, dy=dy_first,
@@ -166,8 +174,8 @@ def main():
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--case", default="box128")
-    ap.add_argument("--arms", default="fp64,mixed,learned",
-                    help="comma-separated subset of fp64, mixed, learned")
+    ap.add_argument("--arms", default="fp64,mixed,learned,mfg",
+                    help="comma-separated subset of fp64, mixed, learned, mfg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--idiff", type=int, default=0, choices=(0, 1, 3),
                     help="global reconstruction of the diffusive flux (0: off, the "
@@ -184,16 +192,19 @@ def main():
     spec = dict(config.CASES[args.case])
     case = mesh.named_case(args.case)
     nel = case.nel
-    arms = [a for a in ("fp64", "mixed", "learned") if a in args.arms.split(",")]
+    arms = [a for a in ("fp64", "mixed", "learned", "mfg") if a in args.arms.split(",")]
     if not arms:
-        ap.error("--arms names none of fp64, mixed, learned")
+        ap.error("--arms names none of fp64, mixed, learned, mfg")
     res = {}
     for a in arms:
-        prec = solver.PHG_FP64 if a == "fp64" else solver.PHG_MIXED
+        prec = solver.PHG_FP64 if a in ("fp64", "mfg") else solver.PHG_MIXED
         wts = WEIGHTS if a == "learned" else None
         print(f"[bench] {args.case} arm {a}", file=sys.stderr, flush=True)
-        res[a] = run_arm(case, prec, spec["newton"], args.steps, args.warmup, wts, args.idiff)
+        res[a] = run_arm(case, prec, spec["newton"], args.steps, args.warmup, wts, args.idiff,
+                         mfg=a == "mfg")
     head_name = "learned" if "learned" in res else ("mixed" if "mixed" in res else "fp64")
+    if head_name not in res:
+        ap.error("the mfg arm is reported beside one of fp64, mixed, learned")
     head = res[head_name]
     value = nel / (head["ms"] / 1e3)
 

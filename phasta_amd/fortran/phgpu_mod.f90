@@ -129,6 +129,30 @@ module phgpu
       type(c_ptr), value :: ctx
       type(phg_stats), intent(out) :: st
     end function
+
+    ! SolMFG(y,ac,yold,acold,x,iBC,BC,res,BDiag,HBrg,eBrg,yBrg,Rcos,Rsin,
+    !        iper,ilwork,shp,shgl,shpb,shglb,Dy,rerr): eGMRES is the
+    ! /itrpar/ interval (in/out), fdi = 1 when iter = 1 and
+    ! mod(istep, 20) = 0 (solmfg.f:163)
+    integer(c_int) function phg_solmfg(ctx, y, ac, res, rmes, BDiag, Dy, eGMRES, fdi, st) &
+        bind(C, name='phg_solmfg')
+      import :: c_int, c_ptr, c_double, phg_stats
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: y(*), ac(*)
+      real(c_double), intent(out) :: res(*), rmes(*), BDiag(*), Dy(*)
+      real(c_double), intent(inout) :: eGMRES
+      integer(c_int), value :: fdi
+      type(phg_stats), intent(out) :: st
+    end function
+
+    integer(c_int) function phg_newton_iteration_mfg(ctx, eGMRES, fdi, st) &
+        bind(C, name='phg_newton_iteration_mfg')
+      import :: c_int, c_ptr, c_double, phg_stats
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: eGMRES
+      integer(c_int), value :: fdi
+      type(phg_stats), intent(out) :: st
+    end function
   end interface
 
 contains
