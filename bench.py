@@ -197,7 +197,7 @@ def main():
         ap.error("--arms names none of fp64, mixed, learned, mfg")
     res = {}
     for a in arms:
-        prec = solver.PHG_FP64 if a in ("fp64", "mfg") else solver.PHG_MIXED
+        prec = {"fp64": solver.PHG_FP64, "mfg": solver.PHG_FP64_MATRIX_FREE}.get(a, solver.PHG_MIXED)
         wts = WEIGHTS if a == "learned" else None
         print(f"[bench] {args.case} arm {a}", file=sys.stderr, flush=True)
         res[a] = run_arm(case, prec, spec["newton"], args.steps, args.warmup, wts, args.idiff,
@@ -216,8 +216,8 @@ def main():
     ms_bl = sum(s["ms_blocks"] for s in st) / len(st)
     asm_tf = E3_FLOPS * nel / ((ms_pt + ms_bl) * 1e-3) / 1e12
     tdt = "f64" if head_name == "fp64" else "f32"
-    kasm = (f"k_asm_qp<{'double' if tdt == 'f64' else 'float'}, false> + "
-            f"k_asm_blocks<{'double, double' if tdt == 'f64' else 'float, float'}>")
+    kasm = (f"k_asm_qp<{'double' if tdt == 'f64' else 'float'}, false, false, false> + "
+            f"k_asm_blocks<{'double, double' if tdt == 'f64' else 'float, float'}, false, false>")
     roof_asm = {"kernel": f"{kasm} (ElmGMRs interior assembly: e3 residual + tangent, "
                           f"res/BDiag/lhsK written once)",
                 "bound": "mfma", "compute": f"VALU {tdt} (no MFMA in this kernel; the peak "

@@ -35,7 +35,12 @@ enum {
   PHG_ECOLOR = 12     /* element colouring needs more than 64 colours      */
 };
 
-enum { PHG_FP64 = 0, PHG_MIXED = 1 };
+/* PHG_FP64_MATRIX_FREE: an FP64 context for SolMFG alone.  As itrdrv.f:163-185
+ * allocates lhsK, rowp and colm only for the sparse solver (impl 1x) and
+ * BDiag alone for the matrix-free one (impl 2x), it holds no lhsK, no matvec
+ * layout and no owner lanes of the off-diagonal entries (about 13 GB less at
+ * 12.6 M elements); the entry points that need lhsK return PHG_EINVAL. */
+enum { PHG_FP64 = 0, PHG_MIXED = 1, PHG_FP64_MATRIX_FREE = 2 };
 
 /* COMMON-block scalars read on the path (common.h:121-259); same field
  * order as the CPU oracle's parameter block. */
@@ -111,7 +116,8 @@ typedef struct phg_ctx phg_ctx;
 
 /* Set-up: copies mesh and BC data to the GPU, colours the elements, builds
  * the device block-CSR layout and the element-to-block map that replaces
- * sparseloc (fillsparse.f:243-270).  precision: PHG_FP64 or PHG_MIXED. */
+ * sparseloc (fillsparse.f:243-270).  precision: PHG_FP64, PHG_MIXED or
+ * PHG_FP64_MATRIX_FREE. */
 int phg_init(const phg_mesh* mesh, const phg_params* prm, int precision,
              phg_ctx** out);
 int phg_finalize(phg_ctx* ctx);
@@ -156,7 +162,8 @@ int phg_newton_iteration(phg_ctx* ctx, phg_stats* st);
 int phg_update(phg_ctx* ctx);
 
 /* Matrix-free GMRES (Solver Type: GMRES Matrix Free, impl = 20; selected at
- * itrdrv.f:489-506 with lhs = 0, iprec from LHSupd).  PHG_FP64 contexts,
+ * itrdrv.f:489-506 with lhs = 0, iprec from LHSupd).  PHG_FP64 and
+ * PHG_FP64_MATRIX_FREE contexts,
  * lhs = 0, iDC = 0 and itau = 0 (PHG_EINVAL otherwise); idiff 0, 1, 3.
  *
  * ElmMFG(y,ac,x,shp,shgl,iBC,BC,shpb,shglb,res,rmes,BDiag,iper,ilwork,rerr)

@@ -51,11 +51,43 @@ struct Quad {
 
 __constant__ Quad cQ;
 
-// Records of 13 values (point state: u(3) rho drdp drdT e1p..e4p WdetJ tau(3);
-// geometry: shape gradients (4 x 3) and WdetJ), padded so that a record is
-// read with 16-byte loads: 14 doubles or 16 floats.
+// Geometry records of 13 values (shape gradients (4 x 3) and WdetJ), padded
+// so that a record is read with 16-byte loads: 14 doubles or 16 floats.
 template <typename S>
 constexpr int rec_stride() { return sizeof(S) == 8 ? 14 : 16; }
+
+// Point records of 12 values, what the tangent pass needs at a point:
+//   0..2 u, 3 rho, 4 drdp, 5 drdT, 6 e1p, 7 e3p, 8 e4p (e2p = e1p + 1),
+//   9..11 WdetJ tau (the tangent pass builds P scaled by WdetJ)
+// read with three 16-byte loads in float (six in double).  The pass is
+// bound by the vector-memory address path (one tag lookup per lane and
+// load instruction), so the record is kept to the values it needs.
+template <typename S>
+constexpr int pt_stride() { return 12; }
+enum PtField { kPtRho = 3, kPtDrdp, kPtDrdT, kPtE1p, kPtE3p, kPtE4p, kPtT0, kPtT1, kPtT2 };
+
+template <typename S>
+__device__ inline void load_pt(const S* p, S v[12]) {
+  if constexpr (sizeof(S) == 8) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const double2 t = q[i];
+      v[2 * i] = t.x;
+      v[2 * i + 1] = t.y;
+    }
+  } else {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float4 t = q[i];
+      v[4 * i] = t.x;
+      v[4 * i + 1] = t.y;
+      v[4 * i + 2] = t.z;
+      v[4 * i + 3] = t.w;
+    }
+  }
+}
 
 template <typename S>
 __device__ inline void load_rec(const S* p, S v[16]) {
@@ -614,20 +646,29 @@ __global__ void __launch_bounds__(256) k_asm_qp(QpArgs A) {
   double ei, rk;
   point_thermo(Y, ph, G, trG, GG, t, ei, rk);
   if (valid) {
-    S* o = reinterpret_cast<S*>(A.qps) + ((size_t)e * 4 + q) * rec_stride<S>();
-    o[0] = (S)t.u[0];
-    o[1] = (S)t.u[1];
-    o[2] = (S)t.u[2];
-    o[3] = (S)t.rho;
-    o[4] = (S)t.drdp;
-    o[5] = (S)t.drdT;
-    o[6] = (S)t.e1p;
-    o[7] = (S)t.e2p;
-    o[8] = (S)t.e3p;
-    o[9] = (S)t.e4p;
-    o[10] = (S)(WdetJ * t.tau[0]);  // WdetJ tau: the tangent pass scales P by WdetJ
-    o[11] = (S)(WdetJ * t.tau[1]);
-    o[12] = (S)(WdetJ * t.tau[2]);
+    S v[12];
+    v[0] = (S)t.u[0];
+    v[1] = (S)t.u[1];
+    v[2] = (S)t.u[2];
+    v[kPtRho] = (S)t.rho;
+    v[kPtDrdp] = (S)t.drdp;
+    v[kPtDrdT] = (S)t.drdT;
+    v[kPtE1p] = (S)t.e1p;
+    v[kPtE3p] = (S)t.e3p;
+    v[kPtE4p] = (S)t.e4p;
+    v[kPtT0] = (S)(WdetJ * t.tau[0]);
+    v[kPtT1] = (S)(WdetJ * t.tau[1]);
+    v[kPtT2] = (S)(WdetJ * t.tau[2]);
+    S* o = reinterpret_cast<S*>(A.qps) + ((size_t)e * 4 + q) * pt_stride<S>();
+    if constexpr (sizeof(S) == 8) {
+      double2* o2 = reinterpret_cast<double2*>(o);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) o2[i] = make_double2(v[2 * i], v[2 * i + 1]);
+    } else {
+      float4* o4 = reinterpret_cast<float4*>(o);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) o4[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    }
   }
   double PT[5][5], ixr[3][3];
   if constexpr (kMT) {
@@ -1034,13 +1075,7 @@ struct BlkArgs {
 // W PTau (e3ls.f:477-486 Atau = A_i PTau).
 template <typename T, typename S, bool kDC, bool kMT>
 __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
-  // chunk partials of the 64 lanes: tangent sums in the tangent's type;
-  // residual sums (double) in their own array on the float path, through
-  // the same array after the tangent on the FP64 path (6.4 + 2.5 KB or
-  // 12.8 KB of LDS per workgroup, so LDS does not cap the occupancy)
   using R = T;
-  __shared__ R stage[64][25];
-  __shared__ double rstage[sizeof(R) == 4 ? 64 : 1][5];
   // consecutive workgroups run on different XCDs (round robin over 8): give
   // each XCD a contiguous range of lanes so its L2 sees neighbouring blocks
   const long long per = (gridDim.x + 7) / 8;
@@ -1051,7 +1086,7 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   const bool diag = (L.w >> 1) & 1;
   const int cnt = (A.lhs_on || diag) ? L.z : 0;
   // block sums in the tangent's type (float accumulation on the mixed
-  // path); the chunks of one block are combined in double
+  // path); the chunks of one block are combined in that type too
   R acc[25];
 #pragma unroll
   for (int i = 0; i < 25; ++i) acc[i] = (R)0;
@@ -1064,14 +1099,16 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     const long long e = code >> 4;
     const int a = (code >> 2) & 3, b = code & 3;
     // the two shape gradients and WdetJ of the element's geometry record
-    const S* go = geo + (size_t)e * rec_stride<S>();
+    // the element's geometry record with 16-byte loads, then the two rows
+    S g16[16];
+    load_rec<S>(geo + (size_t)e * rec_stride<S>(), g16);
     R sR[3], rR[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      sR[c] = (R)go[3 * a + c];
-      rR[c] = (R)go[3 * b + c];
+      sR[c] = (R)sel4t(g16[c], g16[3 + c], g16[6 + c], g16[9 + c], a);
+      rR[c] = (R)sel4t(g16[c], g16[3 + c], g16[6 + c], g16[9 + c], b);
     }
-    const R W = (R)go[12];
+    const R W = (R)g16[12];
     const R st = sR[0] * rR[0] + sR[1] * rR[1] + sR[2] * rR[2];
     // acc += WdetJ [P Q] per point: the stored tau is WdetJ tau and N_a is
     // scaled here, so P's rows come out multiplied by WdetJ
@@ -1084,18 +1121,21 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
 #pragma unroll
       for (int i = 0; i < 14; ++i) dA[i] = (R)0;
     }
-    const S* qrec = qps + (size_t)e * 4 * rec_stride<S>();
-#pragma unroll 1
+    const S* qrec = qps + (size_t)e * 4 * pt_stride<S>();
+    // float: unrolled so the four points' record loads issue together
+    // (2.6% at 128^3); double: the unrolled body spills
+    constexpr int kPtUnroll = sizeof(S) == 4 ? 4 : 1;
+#pragma unroll kPtUnroll
     for (int q = 0; q < 4; ++q) {
-      S o[16];
-      load_rec<S>(qrec + q * rec_stride<S>(), o);
+      S o[12];
+      load_pt<S>(qrec + q * pt_stride<S>(), o);
       const R u0 = (R)o[0], u1 = (R)o[1], u2 = (R)o[2];
       const R uq[3] = {u0, u1, u2};
       ub[0] += u0;
       ub[1] += u1;
       ub[2] += u2;
-      const R rho = (R)o[3], drdp = (R)o[4], drdT = (R)o[5];
-      const R e1p = (R)o[6], e2p = (R)o[7], e3p = (R)o[8], e4p = (R)o[9];
+      const R rho = (R)o[kPtRho], drdp = (R)o[kPtDrdp], drdT = (R)o[kPtDrdT];
+      const R e1p = (R)o[kPtE1p], e2p = e1p + (R)1, e3p = (R)o[kPtE3p], e4p = (R)o[kPtE4p];
       if constexpr (kDC) {
         // weight of the point's A0: DC (the element's giju is applied after
         // the loop), or with the matrix tau s_a . (DC giju_q) s_b of the
@@ -1124,23 +1164,19 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
         dA[12] += dq * e1p;
         dA[13] += dq * e4p;
       }
-      const R tau0 = (R)o[10], tau1 = (R)o[11], tau2 = (R)o[12];
       const R Na = W * (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
       const R Nb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
       const R una = u0 * sR[0] + u1 * sR[1] + u2 * sR[2];
-      R WPT[25];
-      if constexpr (kMT) {
-        const S* mp = reinterpret_cast<const S*>(A.mtau) + ((size_t)e * 4 + q) * mt_stride<S>();
-#pragma unroll
-        for (int k = 0; k < 25; ++k) WPT[k] = (R)mp[k];
-      }
       const R unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
       const R w = unb + cfacR * Nb;
       // Q = A(s_b) + c N_b A0, held in factored form: its rows 1..3 are
       // drdp w u_k + r_k (column 0), rho (u_k r^T + w e_k^T) (columns 1..3)
-      // and drdT w u_k (column 4), so with pu = sum_k P(m,k) u_k they enter
-      // the product as q00 pu + sum_k P(m,k) r_k, rho r_n pu + rho w P(m,1+n)
-      // and q04 pu; only rows 0 and 4 and rho w are formed
+      // and drdT w u_k (column 4), so with pu = sum_k P(m,k) u_k and
+      // Sm = P(m,0) + pu they enter row m of the product as
+      //   column 0:    Sm q00 + sum_k P(m,k) r_k + P(m,4) q40
+      //   column 1+n:  Sm q0n + rho w P(m,1+n) + P(m,4) q4n
+      //   column 4:    Sm q04 + P(m,4) q44
+      // and only rows 0 and 4 of Q and rho w are formed
       const R q00 = drdp * w, q04 = drdT * w, rw = rho * w;
       R q0[3], q4[3];
 #pragma unroll
@@ -1149,11 +1185,14 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
         q4[n] = e3p * rR[n] + rw * uq[n];
       }
       const R q40 = e2p * unb + cfacR * Nb * e1p, q44 = e4p * w;
-      // rows of P = A(s_a) diag(tau) + N_a I
+      if constexpr (kMT) {
+        // rows of P = A(s_a) W PTau + N_a I from the point's W PTau
+        R WPT[25];
+        const S* mp = reinterpret_cast<const S*>(A.mtau) + ((size_t)e * 4 + q) * mt_stride<S>();
 #pragma unroll
-      for (int m = 0; m < 5; ++m) {
-        R Pr[5];
-        if constexpr (kMT) {
+        for (int k = 0; k < 25; ++k) WPT[k] = (R)mp[k];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
           // row m of A(s_a), then P(m, :) = A(s_a)(m, :) W PTau + N_a e_m
           R Am[5];
           if (m == 0) {
@@ -1173,49 +1212,84 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
             for (int n = 0; n < 3; ++n) Am[1 + n] = e3p * sR[n] + rho * uq[n] * una;
             Am[4] = e4p * una;
           }
+          R Pr[5];
 #pragma unroll
           for (int j = 0; j < 5; ++j)
             Pr[j] = Am[0] * WPT[j] + Am[1] * WPT[5 + j] + Am[2] * WPT[10 + j] +
                     Am[3] * WPT[15 + j] + Am[4] * WPT[20 + j] + (m == j ? Na : (R)0);
-        } else if (m == 0) {
-          Pr[0] = drdp * una * tau0 + Na;
+          const R Sm = Pr[0] + (Pr[1] * u0 + Pr[2] * u1 + Pr[3] * u2);
+          R& c0 = acc[m];
+          c0 += Sm * q00;
+          c0 += Pr[1] * rR[0];
+          c0 += Pr[2] * rR[1];
+          c0 += Pr[3] * rR[2];
+          c0 += Pr[4] * q40;
 #pragma unroll
-          for (int n = 0; n < 3; ++n) Pr[1 + n] = rho * sR[n] * tau1;
-          Pr[4] = drdT * una * tau2;
-        } else if (m < 4) {
-          const int mm = m - 1;
-          Pr[0] = (drdp * uq[mm] * una + sR[mm]) * tau0;
-#pragma unroll
-          for (int n = 0; n < 3; ++n)
-            Pr[1 + n] =
-                rho * (uq[mm] * sR[n] + (mm == n ? una : (R)0)) * tau1 + (mm == n ? Na : (R)0);
-          Pr[4] = drdT * uq[mm] * una * tau2;
-        } else {
-          Pr[0] = e2p * una * tau0;
-#pragma unroll
-          for (int n = 0; n < 3; ++n) Pr[1 + n] = (e3p * sR[n] + rho * uq[n] * una) * tau1;
-          Pr[4] = e4p * una * tau2 + Na;
+          for (int n = 0; n < 3; ++n) {
+            R& cn = acc[m + 5 * (1 + n)];
+            cn += Sm * q0[n];
+            cn += rw * Pr[1 + n];
+            cn += Pr[4] * q4[n];
+          }
+          R& c4 = acc[m + 20];
+          c4 += Sm * q04;
+          c4 += Pr[4] * q44;
         }
-        const R pu = Pr[1] * u0 + Pr[2] * u1 + Pr[3] * u2;
-        R& c0 = acc[m];
-        c0 += Pr[0] * q00;
-        c0 += pu * q00;
-        c0 += Pr[1] * rR[0];
-        c0 += Pr[2] * rR[1];
-        c0 += Pr[3] * rR[2];
-        c0 += Pr[4] * q40;
+      } else {
+        // Shakib's diagonal tau.  With the products of the state with W tau,
+        // P's row 0 is v + N_a e_0, v = (W t0 drdp una, W t1 rho s, W t2
+        // drdT una); rows 1+k are u_k v + (W t0 s_k) e_0 + dg e_{1+k}, dg =
+        // W t1 rho una + N_a; row 4 is (W t0 e2p una, W t1 e3p s + (W t1 rho
+        // una) u, W t2 e4p una + N_a).  Q's rows 1+k are u_k Q0 + r_k e_0 +
+        // rho w e_{1+k} (Q0 its row 0), so with vQ = v Q
+        //   (P Q)(0)   = vQ + N_a Q0
+        //   (P Q)(1+k) = u_k (vQ + dg Q0) + (W t0 s_k) Q0 + dg r_k e_0
+        //                + dg rho w e_{1+k}
+        // and row 4 through the same factored Q.
+        const R t0 = (R)o[kPtT0], wt1 = (R)o[kPtT1], wt2 = (R)o[kPtT2];
+        const R rt1 = wt1 * rho, pt0 = t0 * drdp, tt2 = wt2 * drdT;
+        const R et0 = t0 * e2p, et1 = wt1 * e3p, et2 = wt2 * e4p;
+        const R a0 = pt0 * una, r1 = rt1 * una, p04 = tt2 * una, dg = r1 + Na;
+        const R v1[3] = {rt1 * sR[0], rt1 * sR[1], rt1 * sR[2]};
+        const R Q0[5] = {q00, q0[0], q0[1], q0[2], q04};
+        // vQ
+        const R sg = a0 + (v1[0] * u0 + v1[1] * u1 + v1[2] * u2);
+        const R rg = v1[0] * rR[0] + v1[1] * rR[1] + v1[2] * rR[2];
+        R vQ[5];
+        vQ[0] = sg * q00 + rg + p04 * q40;
 #pragma unroll
-        for (int n = 0; n < 3; ++n) {
-          R& cn = acc[m + 5 * (1 + n)];
-          cn += Pr[0] * q0[n];
-          cn += pu * q0[n];
-          cn += rw * Pr[1 + n];
-          cn += Pr[4] * q4[n];
+        for (int n = 0; n < 3; ++n) vQ[1 + n] = sg * q0[n] + rw * v1[n] + p04 * q4[n];
+        vQ[4] = sg * q04 + p04 * q44;
+        // row 0
+#pragma unroll
+        for (int c = 0; c < 5; ++c) acc[5 * c] += vQ[c] + Na * Q0[c];
+        // rows 1..3
+        R Z[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) Z[c] = vQ[c] + dg * Q0[c];
+        const R dgrw = dg * rw;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const R ts = t0 * sR[k];
+#pragma unroll
+          for (int c = 0; c < 5; ++c) {
+            R& ck = acc[(1 + k) + 5 * c];
+            ck += uq[k] * Z[c];
+            ck += ts * Q0[c];
+          }
+          acc[1 + k] += dg * rR[k];
+          acc[(1 + k) + 5 * (1 + k)] += dgrw;
         }
-        R& c4 = acc[m + 20];
-        c4 += Pr[0] * q04;
-        c4 += pu * q04;
-        c4 += Pr[4] * q44;
+        // row 4
+        const R p40 = et0 * una, p44 = et2 * una + Na;
+        R p4[3];
+#pragma unroll
+        for (int n = 0; n < 3; ++n) p4[n] = et1 * sR[n] + r1 * uq[n];
+        const R s4 = p40 + (p4[0] * u0 + p4[1] * u1 + p4[2] * u2);
+        acc[4] += s4 * q00 + (p4[0] * rR[0] + p4[1] * rR[1] + p4[2] * rR[2]) + p44 * q40;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) acc[4 + 5 * (1 + n)] += s4 * q0[n] + rw * p4[n] + p44 * q4[n];
+        acc[24] += s4 * q04 + p44 * q44;
       }
     }
     // viscous part sum_q WdetJ K_ab: the momentum rows and the conduction
@@ -1244,7 +1318,7 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-          for (int c = 0; c < 3; ++c) ix[k][c] = (R)2 * (R)go[3 * k + c];
+          for (int c = 0; c < 3; ++c) ix[k][c] = (R)2 * (R)g16[3 * k + c];
         tet_metric_tensor(ix, G, trG, GG);
         metric_inverse(G, gu);
         sab = (R)0;
@@ -1276,38 +1350,37 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
       for (int f = 0; f < 5; ++f) racc[f] += ro[f];
     }
   }
-  // combine the chunks of one block in lane order
+  // combine the chunks of one block in lane order: the leader adds the
+  // partials of the next nch - 1 lanes (the entry's chunks are consecutive
+  // lanes of the wavefront) through cross-lane reads, so the pass holds no
+  // LDS and its occupancy is set by its registers
   const bool leader = (L.w & 1) && l < A.nlanes;
   const int nch = (L.w >> 8) & 0xff;
   const int total = L.w >> 16;
+  int mx = nch;
 #pragma unroll
-  for (int i = 0; i < 25; ++i) stage[threadIdx.x][i] = acc[i];
-  if constexpr (sizeof(R) == 4) {
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+  // the chunk sums stay in the tangent's type (a few float additions more
+  // on the mixed path), which keeps the pass under 64 VGPRs in float
+  R csum[25];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) rstage[threadIdx.x][f] = racc[f];
-  }
-  __syncthreads();
-  double sum[25];
-  if (leader) {
+  for (int i = 0; i < 25; ++i) csum[i] = acc[i];
+  for (int c = 1; c < mx; ++c) {
 #pragma unroll
-    for (int i = 0; i < 25; ++i) sum[i] = (double)stage[threadIdx.x][i];
-    for (int c = 1; c < nch; ++c) {
+    for (int i = 0; i < 25; ++i) {
+      const R v = __shfl_down(acc[i], c, 64);
+      if (c < nch) csum[i] += v;
+    }
 #pragma unroll
-      for (int i = 0; i < 25; ++i) sum[i] += (double)stage[threadIdx.x + c][i];
+    for (int f = 0; f < 5; ++f) {
+      const double v = __shfl_down(racc[f], c, 64);
+      if (c < nch) racc[f] += v;
     }
   }
-  if constexpr (sizeof(R) == 8) {
-    __syncthreads();
-#pragma unroll
-    for (int f = 0; f < 5; ++f) stage[threadIdx.x][f] = racc[f];
-    __syncthreads();
-  }
   if (!leader) return;
-  for (int c = 1; c < nch; ++c) {
+  double sum[25];
 #pragma unroll
-    for (int f = 0; f < 5; ++f)
-      racc[f] += sizeof(R) == 4 ? rstage[threadIdx.x + c][f] : (double)stage[threadIdx.x + c][f];
-  }
+  for (int i = 0; i < 25; ++i) sum[i] = (double)csum[i];
   const int code0 = A.blocks[L.x];
   const long long e0 = code0 >> 4;
   const int a0 = (code0 >> 2) & 3, b0 = code0 & 3;
@@ -2126,6 +2199,7 @@ int launch_diffusive_flux(phg_ctx* c) {
 }
 
 int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
+  if (lhs) PHG_TRY(require_matrix(c, "ElmGMRs with lhs = 1"));
   PHG_TRY(upload_quadrature());
   PHG_TRY(launch_pack_nodes(c));
   const bool mixed = c->precision == PHG_MIXED;
@@ -2166,7 +2240,9 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   PHG_HIP(hipEventRecord(c->aev[1], c->stream));
   // pass 2: block owners (res, BDiag, lhsK written once)
   BlkArgs B{};
-  B.lanes = c->lanes;
+  // without lhsK only the diagonal entries' owners have work (the node's
+  // residual and BDiag): run their lanes alone
+  B.lanes = lhs ? c->lanes : c->lanes_diag;
   B.blocks = c->blocks;
   B.ien = c->ienc;
   B.bcm = c->bcm;
@@ -2179,7 +2255,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.bdiag = c->acc_bd;
   B.lhs = mixed ? (void*)c->lhs_aos32 : (void*)c->lhs_aos;
   B.nel = c->nel;
-  B.nlanes = c->nlanes;
+  B.nlanes = lhs ? c->nlanes : c->nlanes_diag;
   B.lhs_on = lhs;
   B.prec_on = prec;
   const double cp = ph.Rgas * ph.gamma / ph.gamma1;
@@ -2187,7 +2263,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.lam = -(2.0 / 3.0) * ph.mu;
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
-  const int g2 = (int)(((c->nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
+  const int g2 = (int)(((B.nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
   if (mixed && mt && dc)
     k_asm_blocks<float, float, true, true><<<g2, 64, 0, c->stream>>>(B);
   else if (mixed && mt)

@@ -718,6 +718,7 @@ int launch_i3lu(phg_ctx* c, double* diag, double* r, int code) {
 }
 
 int launch_spsi3pre(phg_ctx* c) {
+  PHG_TRY(require_matrix(c, "Spsi3pre"));
   const int g = (int)c->ngroups, nt = kPreWaves * kGroup;
   if (c->precision == PHG_MIXED)
     k_spsi3pre<float, float><<<g, nt, 0, c->stream>>>(
@@ -730,6 +731,7 @@ int launch_spsi3pre(phg_ctx* c) {
 }
 
 int launch_spmv(phg_ctx* c, const double* p, double* q) {
+  PHG_TRY(require_matrix(c, "SparseAp"));
   const int g = (int)((c->ngroups + 7) / 8 * 8);  // one workgroup per group, XCD-strided
   const int nt = kSpmvWaves * kGroup;
   if (c->precision == PHG_MIXED)

@@ -277,6 +277,10 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   }
   std::vector<int4> lanes;
   lanes.reserve(nnz + 4 * nshg);
+  // the same chunks for the diagonal entries alone: the pass without lhsK
+  // (residual and BDiag only) runs these lanes
+  std::vector<int4> lanes_diag;
+  lanes_diag.reserve(4 * nshg);
   std::vector<int> blk_lane;  // blocks in lane order
   blk_lane.reserve(16 * nel);
   // rows in Morton order of their node coordinates, so the lanes that run
@@ -323,16 +327,21 @@ int setup(phg_ctx* c, const phg_mesh* m) {
       if ((long long)(lanes.size() % 64) + nch > 64)
         while (lanes.size() % 64) lanes.push_back(make_int4(0, 0, 0, 0));
       const bool dg = (m->rowp[k] - 1) == row;
+      if (dg && (long long)(lanes_diag.size() % 64) + nch > 64)
+        while (lanes_diag.size() % 64) lanes_diag.push_back(make_int4(0, 0, 0, 0));
       for (int ch = 0; ch < nch; ++ch) {
         const int b0 = (int)((long long)n * ch / nch), b1 = (int)((long long)n * (ch + 1) / nch);
         int flags = (dg ? 2 : 0) | (ch == 0 ? 1 : 0);
         if (ch == 0) flags |= (nch << 8) | (n << 16);
-        lanes.push_back(make_int4((int)blk_lane.size(), (int)k, b1 - b0, flags));
+        const int4 ln = make_int4((int)blk_lane.size(), (int)k, b1 - b0, flags);
+        lanes.push_back(ln);
+        if (dg) lanes_diag.push_back(ln);
         for (int b = b0; b < b1; ++b) blk_lane.push_back(blocks[bcount[k] + b]);
       }
     }
   }
   c->nlanes = (long long)lanes.size();
+  c->nlanes_diag = (long long)lanes_diag.size();
   blocks.clear();
   blocks.shrink_to_fit();
   // ---- node -> (element, local node) incidence, element order per node
@@ -397,15 +406,20 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   const bool mixed = c->precision == PHG_MIXED;
   PHG_TRY(dev_alloc(c, &c->x, 3 * nshg));
   PHG_TRY(dev_alloc(c, &c->ienc, 4 * nel));
-  PHG_TRY(dev_alloc(c, &c->lanes, c->nlanes));
+  // a matrix-free context keeps no lhsK, no matvec layout and no owner
+  // lanes of the off-diagonal entries (itrdrv.f:180-185 allocates BDiag
+  // alone for SolMFG)
+  const bool mat = !c->matrix_free;
+  if (mat) PHG_TRY(dev_alloc(c, &c->lanes, c->nlanes));
+  PHG_TRY(dev_alloc(c, &c->lanes_diag, c->nlanes_diag));
   PHG_TRY(dev_alloc(c, &c->blocks, 16 * nel));
-  // point-state and geometry records: 13 values padded to 16 floats or 14
-  // doubles (16-byte loads)
+  // point records: 12 values per point; geometry: 13 values padded to 16
+  // floats or 14 doubles (16-byte loads)
   if (mixed) {
-    PHG_TRY(dev_alloc(c, &c->qps32, (size_t)nel * 4 * 16));
+    PHG_TRY(dev_alloc(c, &c->qps32, (size_t)nel * 4 * 12));
     PHG_TRY(dev_alloc(c, &c->geo32, (size_t)nel * 16));
   } else {
-    PHG_TRY(dev_alloc(c, &c->qps, (size_t)nel * 4 * 14));
+    PHG_TRY(dev_alloc(c, &c->qps, (size_t)nel * 4 * 12));
     PHG_TRY(dev_alloc(c, &c->geo, (size_t)nel * 14));
   }
   PHG_TRY(dev_alloc(c, &c->resid, (size_t)nel * 4 * 5));
@@ -417,23 +431,25 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(dev_alloc(c, &c->fold_ptr, c->nfold + 1));
   PHG_TRY(dev_alloc(c, &c->fold_slave, std::max<size_t>(1, fsl.size())));
   PHG_TRY(dev_alloc(c, &c->grp_slot, c->ngroups + 1));
-  PHG_TRY(dev_alloc(c, &c->scol, c->nslots * kGroup));
-  PHG_TRY(dev_alloc(c, &c->scolp, c->nslots * kGroup));
-  PHG_TRY(dev_alloc(c, &c->tslot, c->nslots * kGroup));
-  PHG_TRY(dev_alloc(c, &c->csr2slot, nnz));
-  PHG_TRY(dev_alloc(c, &c->slot2csr, c->nslots * kGroup));
+  if (mat) {
+    PHG_TRY(dev_alloc(c, &c->scol, c->nslots * kGroup));
+    PHG_TRY(dev_alloc(c, &c->scolp, c->nslots * kGroup));
+    PHG_TRY(dev_alloc(c, &c->tslot, c->nslots * kGroup));
+    PHG_TRY(dev_alloc(c, &c->csr2slot, nnz));
+    PHG_TRY(dev_alloc(c, &c->slot2csr, c->nslots * kGroup));
+  }
   PHG_TRY(dev_alloc(c, &c->acc_res, 5 * nshg));
-  if (mixed)
+  if (mat && mixed)
     PHG_TRY(dev_alloc(c, &c->lhs_aos32, (size_t)aos_stride<float>() * nnz));
-  else
+  else if (mat)
     PHG_TRY(dev_alloc(c, &c->lhs_aos, (size_t)aos_stride<double>() * nnz));
   PHG_TRY(dev_alloc(c, &c->acc_bd, 25 * nshg));
   PHG_TRY(dev_alloc(c, &c->nodepack, 16 * nshg));
   PHG_TRY(dev_alloc(c, &c->inc_goff, goff.size()));
   PHG_TRY(dev_alloc(c, &c->inc, incs.size()));
-  if (mixed)
+  if (mat && mixed)
     PHG_TRY(dev_alloc(c, &c->lhs32, c->nslots * kGroup * 25));
-  else
+  else if (mat)
     PHG_TRY(dev_alloc(c, &c->lhs64, c->nslots * kGroup * 25));
   PHG_TRY(dev_alloc(c, &c->res, 5 * nshg));
   PHG_TRY(dev_alloc(c, &c->rmes, 5 * nshg));
@@ -456,7 +472,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->ienc, ienc.data(), ienc.size()));
   PHG_TRY(upload(c, c->inc_goff, goff.data(), goff.size()));
   PHG_TRY(upload(c, c->inc, incs.data(), incs.size()));
-  PHG_TRY(upload(c, c->lanes, lanes.data(), lanes.size()));
+  if (mat) PHG_TRY(upload(c, c->lanes, lanes.data(), lanes.size()));
+  PHG_TRY(upload(c, c->lanes_diag, lanes_diag.data(), lanes_diag.size()));
   PHG_TRY(upload(c, c->blocks, blk_lane.data(), blk_lane.size()));
   PHG_TRY(upload(c, c->ienbc, ienbc.data(), ienbc.size()));
   PHG_TRY(upload(c, c->bcm, bcm.data(), bcm.size()));
@@ -468,15 +485,17 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   }
   PHG_TRY(upload(c, c->fold_ptr, fptr.data(), fptr.size()));
   PHG_TRY(upload(c, c->grp_slot, c->h_grp_slot.data(), c->h_grp_slot.size()));
-  PHG_TRY(upload(c, c->scol, scol.data(), scol.size()));
-  PHG_TRY(upload(c, c->scolp, scolp.data(), scolp.size()));
-  PHG_TRY(upload(c, c->tslot, tsl.data(), tsl.size()));
-  PHG_TRY(upload(c, c->csr2slot, c2s.data(), c2s.size()));
-  PHG_TRY(upload(c, c->slot2csr, s2c.data(), s2c.size()));
-  if (mixed)
-    PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
-  else
-    PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
+  if (mat) {
+    PHG_TRY(upload(c, c->scol, scol.data(), scol.size()));
+    PHG_TRY(upload(c, c->scolp, scolp.data(), scolp.size()));
+    PHG_TRY(upload(c, c->tslot, tsl.data(), tsl.size()));
+    PHG_TRY(upload(c, c->csr2slot, c2s.data(), c2s.size()));
+    PHG_TRY(upload(c, c->slot2csr, s2c.data(), s2c.size()));
+    if (mixed)
+      PHG_TRY(launch_fill(c, c->lhs32, sizeof(float) * 25 * kGroup * c->nslots));
+    else
+      PHG_TRY(launch_fill(c, c->lhs64, sizeof(double) * 25 * kGroup * c->nslots));
+  }
   PHG_TRY(launch_fill(c, c->yold, sizeof(double) * 5 * nshg));
   PHG_TRY(launch_fill(c, c->acold, sizeof(double) * 5 * nshg));
   PHG_TRY(sync(c));
@@ -1092,6 +1111,18 @@ int check_ctx(phg_ctx* c) {
   return 0;
 }
 
+}  // namespace
+
+int require_matrix(phg_ctx* c, const char* who) {
+  if (c->matrix_free) {
+    phg_set_error("%s: a PHG_FP64_MATRIX_FREE context holds no lhsK (SolMFG only)", who);
+    return PHG_EINVAL;
+  }
+  return 0;
+}
+
+namespace {
+
 int require_fp64(phg_ctx* c, const char* who) {
   if (c->precision != PHG_FP64) {
     phg_set_error("%s: host-array matrix entry points need a PHG_FP64 context", who);
@@ -1117,7 +1148,7 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
     return PHG_EINVAL;
   }
   *out = nullptr;
-  if (precision != PHG_FP64 && precision != PHG_MIXED) {
+  if (precision != PHG_FP64 && precision != PHG_MIXED && precision != PHG_FP64_MATRIX_FREE) {
     phg_set_error("phg_init: unknown precision %d", precision);
     return PHG_EINVAL;
   }
@@ -1131,7 +1162,8 @@ extern "C" int phg_init(const phg_mesh* mesh, const phg_params* prm, int precisi
   c->nelb = mesh->nelb;
   c->nnz = mesh->nnz_tot;
   c->ndofBC = mesh->ndofBC;
-  c->precision = precision;
+  c->matrix_free = precision == PHG_FP64_MATRIX_FREE;
+  c->precision = c->matrix_free ? PHG_FP64 : precision;
   c->prm = *prm;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -1173,7 +1205,7 @@ extern "C" int phg_finalize(phg_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   mp_free(c);
-  void* ptrs[] = {c->x,        c->ienc,     c->lanes,  c->blocks, c->qps, c->qps32, c->geo,
+  void* ptrs[] = {c->x,        c->ienc,     c->lanes,  c->lanes_diag, c->blocks, c->qps, c->qps32, c->geo,
                   c->geo32,    c->resid,    c->tslot,    c->ienbc,  c->bcm,       c->iper,
                   c->BCv,      c->fold_master, c->fold_ptr, c->fold_slave, c->grp_slot,
                   c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,
@@ -1256,6 +1288,7 @@ extern "C" int phg_elmgmrs(phg_ctx* c, const double* y, const double* ac, double
 extern "C" int phg_solgmrs(phg_ctx* c, const double* y, const double* ac, double* res, double* rmes,
                            double* BDiag, double* lhsK, double* Dy, phg_stats* st) {
   PHG_TRY(check_ctx(c));
+  PHG_TRY(require_matrix(c, "phg_solgmrs"));
   phg_stats local{};
   if (!st) st = &local;
   const long long n = c->nshg;
@@ -1317,6 +1350,7 @@ extern "C" int phg_predict(phg_ctx* c) {
 
 extern "C" int phg_newton_iteration(phg_ctx* c, phg_stats* st) {
   PHG_TRY(check_ctx(c));
+  PHG_TRY(require_matrix(c, "phg_newton_iteration"));
   phg_stats local{};
   if (!st) st = &local;
   c->prm.iter += 1;  // itrdrv.f:434
@@ -1423,6 +1457,7 @@ extern "C" int phg_newton_iteration_mfg(phg_ctx* c, double* eGMRES, int fdi, phg
 
 extern "C" int phg_sparse_ap(phg_ctx* c, const double* lhsK, double* p) {
   PHG_TRY(check_ctx(c));
+  PHG_TRY(require_matrix(c, "phg_sparse_ap"));
   PHG_TRY(require_fp64(c, "phg_sparse_ap"));
   const long long n = c->nshg;
   if (lhsK) {
@@ -1450,6 +1485,7 @@ extern "C" int phg_tangent_ap(phg_ctx* c, const double* y, double* p) {
 
 extern "C" int phg_sparse_ap_t(phg_ctx* c, const double* lhsK, double* p) {
   PHG_TRY(check_ctx(c));
+  PHG_TRY(require_matrix(c, "phg_sparse_ap_t"));
   PHG_TRY(require_fp64(c, "phg_sparse_ap_t"));
   const long long n = c->nshg;
   if (lhsK) {
@@ -1486,6 +1522,7 @@ extern "C" int phg_i3lu(phg_ctx* c, double* Diag, double* r, const char* code) {
 
 extern "C" int phg_spsi3pre(phg_ctx* c, const double* BDiag, double* lhsK) {
   PHG_TRY(check_ctx(c));
+  PHG_TRY(require_matrix(c, "phg_spsi3pre"));
   PHG_TRY(require_fp64(c, "phg_spsi3pre"));
   PHG_TRY(upload(c, c->bdiag, BDiag, 25 * c->nshg));
   PHG_TRY(ensure_tmp(c, sizeof(double) * 25 * c->nnz));
@@ -1499,6 +1536,7 @@ extern "C" int phg_spsi3pre(phg_ctx* c, const double* BDiag, double* lhsK) {
 
 extern "C" int phg_spsi3pre_noalloc(phg_ctx* c, const double* BDiag, double* lhsK) {
   PHG_TRY(check_ctx(c));
+  PHG_TRY(require_matrix(c, "phg_spsi3pre_noalloc"));
   PHG_TRY(require_fp64(c, "phg_spsi3pre_noalloc"));
   const size_t need = sizeof(double) * 25 * (size_t)c->nnz;
   if (need > kScratchBytes) {
@@ -1538,6 +1576,7 @@ extern "C" int phg_bc3per(phg_ctx* c, double* res, int nQs) {
 
 extern "C" int phg_bench_spmv(phg_ctx* c, int reps, double* ms_per_launch, double* bytes) {
   PHG_TRY(check_ctx(c));
+  PHG_TRY(require_matrix(c, "phg_bench_spmv"));
   PHG_TRY(launch_spmv(c, kvec(c, 0), kvec(c, 1)));  // warm
   PHG_HIP(hipEventRecord(c->ev[0], c->stream));
   for (int i = 0; i < reps; ++i) PHG_TRY(launch_spmv(c, kvec(c, i & 1), kvec(c, (i + 1) & 1)));
@@ -1580,6 +1619,7 @@ extern "C" int phg_mpnn_param_count(void) { return mp_param_count(); }
 
 extern "C" int phg_mpnn_init(phg_ctx* c, const phg_mesh* mesh, const float* dscale) {
   PHG_TRY(check_ctx(c));
+  PHG_TRY(require_matrix(c, "phg_mpnn_init"));
   if (!mesh || !dscale || mesh->nshg != c->nshg || mesh->nnz_tot != c->nnz) {
     phg_set_error("phg_mpnn_init: mesh does not match the context");
     return PHG_EINVAL;

@@ -11,6 +11,8 @@
 #include "phgpu.h"
 
 void phg_set_error(const char* fmt, ...);
+// PHG_EINVAL with a message when the context is matrix-free (no lhsK)
+int require_matrix(phg_ctx* c, const char* who);
 
 #define PHG_HIP(call)                                                     \
   do {                                                                    \
@@ -183,6 +185,7 @@ struct phg_ctx {
   long long nshg = 0, nel = 0, nelb = 0, nnz = 0;
   int ndofBC = 0;
   int precision = PHG_FP64;
+  bool matrix_free = false;     // PHG_FP64_MATRIX_FREE: no lhsK, SolMFG only
   phg_params prm{};
   hipStream_t stream = nullptr;
   long long dev_bytes = 0;
@@ -193,11 +196,13 @@ struct phg_ctx {
   // assembly: point states of the elements and the block-owner lanes
   long long nlanes = 0;
   int4* lanes = nullptr;        // block-list offset, CSR entry, count, flags
+  long long nlanes_diag = 0;
+  int4* lanes_diag = nullptr;   // the lanes of the diagonal entries alone
   int* blocks = nullptr;        // e*16 + a*4 + b, grouped per owner lane
   void* scratch = nullptr;      // this device's address of the library's
                                 // static staging array (phg_spsi3pre_noalloc)
-  double* qps = nullptr;        // (nel*4) x 14 point states (13 + pad, FP64 path)
-  float* qps32 = nullptr;       // (nel*4) x 16, mixed path
+  double* qps = nullptr;        // (nel*4) x 12 point records (FP64 path)
+  float* qps32 = nullptr;       // (nel*4) x 12, mixed path
   double* geo = nullptr;        // nel x 14: shape gradients, WdetJ (+ pad)
   float* geo32 = nullptr;       // nel x 16
   double* resid = nullptr;      // (nel*4) x 5 element residual per node

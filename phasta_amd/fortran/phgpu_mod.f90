@@ -9,7 +9,7 @@ module phgpu
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: PHG_FP64 = 0, PHG_MIXED = 1
+  integer(c_int), parameter :: PHG_FP64 = 0, PHG_MIXED = 1, PHG_FP64_MATRIX_FREE = 2
 
   type, bind(C) :: phg_params
     real(c_double) :: Rgas, gamma, gamma1, pr, mu

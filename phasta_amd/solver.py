@@ -25,7 +25,7 @@ _i64 = ctypes.c_longlong
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
-PHG_FP64, PHG_MIXED = 0, 1
+PHG_FP64, PHG_MIXED, PHG_FP64_MATRIX_FREE = 0, 1, 2
 
 
 class PhgError(RuntimeError):

@@ -14,7 +14,11 @@ Bars:
   interval is estimated along the unnormalised residual), Dy at 1e4 times
   the interval within 1e-5 of the oracle's and of SolGMRs' Dy;
 * the interval itself (itrFDI) within 1e-5;
-* iDC, itau != 0, lhs = 1 and the mixed-precision context are refused.
+* iDC, itau != 0, lhs = 1 and the mixed-precision context are refused;
+* a PHG_FP64_MATRIX_FREE context (no lhsK, as itrdrv.f:163-185 allocates
+  for impl 2x) runs the same kernels as a PHG_FP64 one: its SolMFG results
+  are bitwise equal, it holds at least the matrix's bytes less, and the
+  entry points that need lhsK refuse it.
 """
 import ctypes
 import os
@@ -327,5 +331,62 @@ def test_mixed_precision_context_is_refused(state):
         with pytest.raises(solver.PhgError) as ei:
             gpu.elmmfg(y, ac)
         assert ei.value.code == 1
+    finally:
+        gpu.close()
+
+
+def test_matrix_free_context_equals_the_fp64_context(state):
+    """Same kernels, same order: res, rmes, BDiag, eGMRES and Dy bitwise
+    equal; two resident Newton iterations too.  The context holds no lhsK
+    (26 doubles per CSR entry in the element-block layout, 25 in the matvec
+    layout) and no off-diagonal owner lanes."""
+    case, y, ac = state
+    prm = mfg_params(case)
+    outs, infos = [], []
+    for prec in (solver.PHG_FP64, solver.PHG_FP64_MATRIX_FREE):
+        gpu = solver.GpuSolver(case, params=prm, precision=prec)
+        try:
+            o = gpu.solmfg(y, ac, egmres=0.0, fdi=True)
+            gpu.upload_state(case.yold, np.zeros((case.nshg, 5), order="F"))
+            gpu.predict()
+            its = [gpu.newton_iteration_mfg(egmres=0.0, fdi=True)["iKs"],
+                   gpu.newton_iteration_mfg(fdi=False)["iKs"]]
+            o["resident"] = gpu.download_state()
+            o["its"] = its
+            outs.append(o)
+            infos.append(gpu.info()["dev_bytes"])
+        finally:
+            gpu.close()
+    a, b = outs
+    for k in ("res", "rmes", "BDiag", "Dy"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["eGMRES"] == b["eGMRES"] and a["iKs"] == b["iKs"] and a["its"] == b["its"]
+    for u, v in zip(a["resident"], b["resident"]):
+        assert np.array_equal(u, v)
+    saved = infos[0] - infos[1]
+    print(f"[mfg] device bytes {infos[0]} -> {infos[1]} ({saved} less)", flush=True)
+    assert saved >= 8 * 51 * case.nnz_tot
+
+
+def test_matrix_free_context_refuses_matrix_entry_points(state):
+    case, y, ac = state
+    n, nnz = case.nshg, case.nnz_tot
+    gpu = solver.GpuSolver(case, params=mfg_params(case), precision=solver.PHG_FP64_MATRIX_FREE)
+    try:
+        calls = [lambda: gpu.solgmrs(y, ac),
+                 lambda: gpu.sparse_ap(np.zeros((25, nnz), order="F"), np.zeros((n, 5), order="F")),
+                 lambda: gpu.spsi3pre(np.zeros((n, 5, 5), order="F"), np.zeros((25, nnz), order="F"))]
+        for call in calls:
+            with pytest.raises(solver.PhgError) as ei:
+                call()
+            assert ei.value.code == 1
+            assert "MATRIX_FREE" in str(ei.value)
+        gpu.set_params(lhs=1)
+        with pytest.raises(solver.PhgError) as ei:
+            gpu.elmgmrs(y, ac)
+        assert ei.value.code == 1
+        # the matrix-free path still runs after the refusals
+        gpu.set_params(lhs=0)
+        assert gpu.solmfg(y, ac)["iKs"] >= 1
     finally:
         gpu.close()
