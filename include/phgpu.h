@@ -164,12 +164,15 @@ int phg_update(phg_ctx* ctx);
 /* Matrix-free GMRES (Solver Type: GMRES Matrix Free, impl = 20; selected at
  * itrdrv.f:489-506 with lhs = 0, iprec from LHSupd).  PHG_FP64 and
  * PHG_FP64_MATRIX_FREE contexts,
- * lhs = 0, iDC = 0 and itau = 0 (PHG_EINVAL otherwise); idiff 0, 1, 3.
+ * lhs = 0, iDC = 0 (PHG_EINVAL otherwise); idiff 0, 1, 3; itau 0 or 10
+ * (BDiag from e3bdg.f or e3bdg_nd.f).
  *
  * ElmMFG(y,ac,x,shp,shgl,iBC,BC,shpb,shglb,res,rmes,BDiag,iper,ilwork,rerr)
  *                                                     elmmfg.f:1
  * res as ElmGMRs forms it, rmes the modified residual (e3 with ires = 3),
- * BDiag (iprec != 0) the block diagonal of e3bdg.f; no lhsK is formed. */
+ * BDiag (iprec != 0) the block diagonal of e3bdg.f (itau = 10: of
+ * e3bdg_nd.f, whose eight mis-indexed statements it reproduces); no lhsK is
+ * formed. */
 int phg_elmmfg(phg_ctx* ctx, const double* y, const double* ac, double* res,
                double* rmes, double* BDiag);
 /* ItrRes(yp,yc,x,shp,shgl,iBC,BC,shpb,shglb,rmes,iper,ilwork,ac)

@@ -1003,9 +1003,9 @@ static int mfg_flags(const orc_mesh* m, const orc_params* P) {
     if (!bc_supported(m->iBC[n])) return 3;
   if (P->idiff != 0 && P->idiff != 1 && P->idiff != 3) return 5;
   /* e3dc's ires = 2 branch reads giju, rTLS and raLS, which e3tau forms only
-   * for ires 1 and 3 (e3tau.f:185-262); e3bdg_nd (itau >= 10) is not
-   * restated */
-  if (P->idc != 0 || P->itau != 0) return 5;
+   * for ires 1 and 3 (e3tau.f:185-262) */
+  if (P->idc != 0) return 5;
+  if (P->itau != 0 && P->itau < 10) return 5;
   return 0;
 }
 

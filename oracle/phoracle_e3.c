@@ -357,12 +357,10 @@ static void e3tau_nd(const orc_params* P, double rho, double cp, double T, doubl
  * reference's order 11, 22, 33, 12+21, 13+31, 23+32, each with its
  * viscous K (e3bdg.f:602-610, 862-870, 1123-1131, 1609-1614, 2092-2097,
  * 2575-2580). */
-static void e3bdg(const orc_params* P, const double shape[4], const double shg[4][3],
-                  double WdetJ, const double A1[5][5], const double A2[5][5],
-                  const double A3[5][5], const double A0[5][5], const double tau[3],
-                  double u1, double u2, double u3, double rmu, double rlm2mu, double con,
-                  double BDl[4][25]) {
-  const double(*Am[3])[5] = {A1, A2, A3};
+/* e3bdg.f:34-246 = e3bdg_nd.f:34-247: the convective and mass terms */
+static void e3bdg_head(const orc_params* P, const double shape[4], const double shg[4][3],
+                       double WdetJ, const double A1[5][5], const double A2[5][5],
+                       const double A3[5][5], const double A0[5][5], double BDl[4][25]) {
   for (int j = 0; j < 4; ++j) {
     const double tmp = shape[j] * WdetJ;
     for (int k = 0; k < 5; ++k)
@@ -371,15 +369,39 @@ static void e3bdg(const orc_params* P, const double shape[4], const double shg[4
             BDl[j][i + 5 * k] +
             tmp * (shg[j][0] * A1[i][k] + shg[j][1] * A2[i][k] + shg[j][2] * A3[i][k]);
   }
-  {
-    const double ff = P->almi / P->gami / P->alfi;
-    const double tmp = WdetJ * (P->Dtgl * ff + 0.0);
-    for (int j = 0; j < 4; ++j) {
-      const double tmp2 = (shape[j] * shape[j]) * tmp;
-      for (int k = 0; k < 5; ++k)
-        for (int i = 0; i < 5; ++i) BDl[j][i + 5 * k] = BDl[j][i + 5 * k] + tmp2 * A0[i][k];
-    }
+  const double ff = P->almi / P->gami / P->alfi;
+  const double tmp = WdetJ * (P->Dtgl * ff + 0.0);
+  for (int j = 0; j < 4; ++j) {
+    const double tmp2 = (shape[j] * shape[j]) * tmp;
+    for (int k = 0; k < 5; ++k)
+      for (int i = 0; i < 5; ++i) BDl[j][i + 5 * k] = BDl[j][i + 5 * k] + tmp2 * A0[i][k];
   }
+}
+
+/* the viscous K of stiffness block (k,l) (e3bdg.f:602-610, 1609-1614 and
+ * the like; e3bdg_nd.f:565-573, 1341-1346 and the like) */
+static void e3bdg_visc(int k, int l, double u1, double u2, double u3, double rmu,
+                       double rlm2mu, double con, double M[5][5]) {
+  const double uu[3] = {u1, u2, u3};
+  if (k == l) {
+    for (int m = 0; m < 3; ++m) M[1 + m][1 + m] = M[1 + m][1 + m] + (m == k ? rlm2mu : rmu);
+    for (int m = 0; m < 3; ++m) M[4][1 + m] = M[4][1 + m] + (m == k ? rlm2mu : rmu) * uu[m];
+    M[4][4] = M[4][4] + con;
+  } else {
+    M[1 + k][1 + l] = M[1 + k][1 + l] + rlm2mu - rmu;
+    M[1 + l][1 + k] = M[1 + l][1 + k] + rlm2mu - rmu;
+    M[4][1 + k] = M[4][1 + k] + (rlm2mu - rmu) * uu[l];
+    M[4][1 + l] = M[4][1 + l] + (rlm2mu - rmu) * uu[k];
+  }
+}
+
+static void e3bdg(const orc_params* P, const double shape[4], const double shg[4][3],
+                  double WdetJ, const double A1[5][5], const double A2[5][5],
+                  const double A3[5][5], const double A0[5][5], const double tau[3],
+                  double u1, double u2, double u3, double rmu, double rlm2mu, double con,
+                  double BDl[4][25]) {
+  const double(*Am[3])[5] = {A1, A2, A3};
+  e3bdg_head(P, shape, shg, WdetJ, A1, A2, A3, A0, BDl);
   /* A_k tau A0 with the diagonal tau (tau1 continuity, tau2 momentum, tau3
    * energy columns) */
   const double tc[5] = {tau[0], tau[1], tau[1], tau[1], tau[2]};
@@ -422,17 +444,96 @@ static void e3bdg(const orc_params* P, const double shape[4], const double shg[4
                     tau[2] * (Ak[i][4] * Al[4][j] + Al[i][4] * Ak[4][j]);
         }
       }
-    const double uu[3] = {u1, u2, u3};
-    if (k == l) {
-      for (int m = 0; m < 3; ++m) M[1 + m][1 + m] = M[1 + m][1 + m] + (m == k ? rlm2mu : rmu);
-      for (int m = 0; m < 3; ++m) M[4][1 + m] = M[4][1 + m] + (m == k ? rlm2mu : rmu) * uu[m];
-      M[4][4] = M[4][4] + con;
-    } else {
-      M[1 + k][1 + l] = M[1 + k][1 + l] + rlm2mu - rmu;
-      M[1 + l][1 + k] = M[1 + l][1 + k] + rlm2mu - rmu;
-      M[4][1 + k] = M[4][1 + k] + (rlm2mu - rmu) * uu[l];
-      M[4][1 + l] = M[4][1 + l] + (rlm2mu - rmu) * uu[k];
+    e3bdg_visc(k, l, u1, u2, u3, rmu, rlm2mu, con, M);
+    for (int a = 0; a < 4; ++a) {
+      const double tmp = WdetJ * shg[a][k] * shg[a][l];
+      for (int c = 0; c < 5; ++c)
+        for (int i = 0; i < 5; ++i) BDl[a][i + 5 * c] = BDl[a][i + 5 * c] + tmp * M[i][c];
     }
+  }
+}
+
+/* e3bdg_nd.f:1-2048 (itau >= 10, ivart = 2): e3bdg with the matrix tau.
+ * Atau_k = A_k PTau (e3bdg_nd.f:252-314, accumulated from zero over the
+ * inner index), A_k tau A0 and the least-squares time term as in e3bdg, and
+ * the stiffness blocks (e3bdg_nd.f:354-2044) in the order 11, 22, 33, 12,
+ * 13, 23: entry (i,j) of block (k,l) is sum_m Atau_k(i,m) A_l(m,j), followed
+ * for k != l by sum_m Atau_l(i,m) A_k(m,j), left to right; the reference
+ * leaves out the terms whose A entry is zero by construction, which the
+ * full sums add as exact zeros.  Eight of the reference's statements read
+ * another row of Atau_k than the entry's, and are restated as written
+ * (1-based rows/columns):
+ *   block 11, entry (5,3): Atau1(3,5) for Atau1(5,5)        (e3bdg_nd.f:539)
+ *   block 22, entries (3,3), (4,3), (5,3): Atau2(2,m) for m = 1, 2, 4
+ *     (e3bdg_nd.f:678, 716, 754), and in (5,3) Atau2(3,5) for Atau2(5,5)
+ *   block 33, entries (3,4), (4,4), (5,4): Atau3(2,m) for m = 1, 2, 3
+ *     (e3bdg_nd.f:902, 940, 978); entry (5,3): Atau3(3,5) for Atau3(5,5)
+ *     (e3bdg_nd.f:971)
+ * so this block diagonal is the tangent's only outside those entries. */
+static void e3bdg_nd(const orc_params* P, const double shape[4], const double shg[4][3],
+                     double WdetJ, const double A1[5][5], const double A2[5][5],
+                     const double A3[5][5], const double A0[5][5], const double PTau[5][5],
+                     double u1, double u2, double u3, double rmu, double rlm2mu, double con,
+                     double BDl[4][25]) {
+  const double(*Am[3])[5] = {A1, A2, A3};
+  e3bdg_head(P, shape, shg, WdetJ, A1, A2, A3, A0, BDl);
+  double Atau[3][5][5], AtA0[3][5][5];
+  for (int d = 0; d < 3; ++d) {
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) {
+        double acc = 0.0;
+        for (int m = 0; m < 5; ++m) acc = acc + Am[d][i][m] * PTau[m][j];
+        Atau[d][i][j] = acc;
+      }
+    for (int c = 0; c < 5; ++c)
+      for (int i = 0; i < 5; ++i)
+        AtA0[d][i][c] = Atau[d][i][0] * A0[0][c] + Atau[d][i][1] * A0[1][c] +
+                        Atau[d][i][2] * A0[2][c] + Atau[d][i][3] * A0[3][c] +
+                        Atau[d][i][4] * A0[4][c];
+  }
+  for (int a = 0; a < 4; ++a) {
+    const double tmp1 = shape[a] * WdetJ * P->almi / P->gami / P->alfi * P->Dtgl;
+    for (int i = 0; i < 5; ++i)
+      for (int c = 0; c < 5; ++c)
+        BDl[a][i + 5 * c] = BDl[a][i + 5 * c] +
+                            tmp1 * (shg[a][0] * AtA0[0][i][c] + shg[a][1] * AtA0[1][i][c] +
+                                    shg[a][2] * AtA0[2][i][c]);
+  }
+  static const int kl[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+  for (int b = 0; b < 6; ++b) {
+    const int k = kl[b][0], l = kl[b][1];
+    double M[5][5];
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) {
+        /* row of Atau_k read for each m (the entry's own row but for the
+         * eight statements above) */
+        double X[5];
+        for (int m = 0; m < 5; ++m) X[m] = Atau[k][i][m];
+        if (k == l) {
+          if (k == 0 && i == 4 && j == 2) X[4] = Atau[0][2][4];
+          if (k == 1 && j == 2 && i >= 2) {
+            X[0] = Atau[1][1][0];
+            X[1] = Atau[1][1][1];
+            X[3] = Atau[1][1][3];
+            if (i == 4) X[4] = Atau[1][2][4];
+          }
+          if (k == 2 && j == 3 && i >= 2) {
+            X[0] = Atau[2][1][0];
+            X[1] = Atau[2][1][1];
+            X[2] = Atau[2][1][2];
+          }
+          if (k == 2 && i == 4 && j == 2) X[4] = Atau[2][2][4];
+        }
+        const double(*Al)[5] = Am[l];
+        double acc = X[0] * Al[0][j];
+        for (int m = 1; m < 5; ++m) acc = acc + X[m] * Al[m][j];
+        if (k != l) {
+          const double(*Ak)[5] = Am[k];
+          for (int m = 0; m < 5; ++m) acc = acc + Atau[l][i][m] * Ak[m][j];
+        }
+        M[i][j] = acc;
+      }
+    e3bdg_visc(k, l, u1, u2, u3, rmu, rlm2mu, con, M);
     for (int a = 0; a < 4; ++a) {
       const double tmp = WdetJ * shg[a][k] * shg[a][l];
       for (int c = 0; c < 5; ++c)
@@ -1074,8 +1175,12 @@ static void e3_body(const orc_params* P, const double ylp[4][5], const double yl
       }
     }
     /* ---- e3.f:261-272 the block diagonal of the matrix-free solver ---- */
-    if (BDl) e3bdg(P, shape, (const double(*)[3])shg, WdetJ, A1, A2, A3, A0, tau, u1, u2, u3,
-                   rmu, rlm2mu, con, BDl);
+    if (BDl && P->itau >= 10)
+      e3bdg_nd(P, shape, (const double(*)[3])shg, WdetJ, A1, A2, A3, A0,
+               (const double(*)[5])PTau, u1, u2, u3, rmu, rlm2mu, con, BDl);
+    else if (BDl)
+      e3bdg(P, shape, (const double(*)[3])shg, WdetJ, A1, A2, A3, A0, tau, u1, u2, u3, rmu,
+            rlm2mu, con, BDl);
     /* ---- e3wmlt.f:72-143 RHS ---- */
     if (want_r) {
     for (int i = 0; i < 4; ++i)

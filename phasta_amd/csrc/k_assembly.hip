@@ -1072,8 +1072,13 @@ struct BlkArgs {
 // adds WdetJ DC (s_a . giju s_b) A0 per point (e3dc.f:410-446 through
 // e3wmlt), with giju recomputed from the stored shape gradients.  kMT: the
 // matrix tau (itau = 10), P = A(s_a) (W PTau) + N_a I with the stored
-// W PTau (e3ls.f:477-486 Atau = A_i PTau).
-template <typename T, typename S, bool kDC, bool kMT>
+// W PTau (e3ls.f:477-486 Atau = A_i PTau).  kNd (with kMT, the pass
+// without lhsK for the matrix-free solver): the block diagonal is e3bdg_nd's
+// (e3.f:273-283), which is the tangent's but for eight of e3bdg_nd.f's
+// statements that read another row of Atau_k = A_k PTau than the entry's
+// (e3bdg_nd.f:539, 678, 716, 754, 902, 940, 971, 978); the lane adds their
+// difference, W s_k^2 (slip - own), per point.
+template <typename T, typename S, bool kDC, bool kMT, bool kNd = false>
 __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   using R = T;
   // consecutive workgroups run on different XCDs (round robin over 8): give
@@ -1234,6 +1239,81 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
           R& c4 = acc[m + 20];
           c4 += Sm * q04;
           c4 += Pr[4] * q44;
+        }
+        if constexpr (kNd) {
+          // row m of A_k (the flux Jacobian along x_k, pressure-primitive)
+          auto arow = [&](int k, int m, R r[5]) {
+            const R uk = uq[k];
+            if (m == 0) {
+              r[0] = drdp * uk;
+#pragma unroll
+              for (int n = 0; n < 3; ++n) r[1 + n] = n == k ? rho : (R)0;
+              r[4] = drdT * uk;
+            } else if (m < 4) {
+              const int mm = m - 1;
+              r[0] = drdp * uq[mm] * uk + (mm == k ? (R)1 : (R)0);
+#pragma unroll
+              for (int n = 0; n < 3; ++n)
+                r[1 + n] = rho * ((n == k ? uq[mm] : (R)0) + (mm == n ? uk : (R)0));
+              r[4] = drdT * uq[mm] * uk;
+            } else {
+              r[0] = e2p * uk;
+#pragma unroll
+              for (int n = 0; n < 3; ++n) r[1 + n] = (n == k ? e3p : (R)0) + rho * uq[n] * uk;
+              r[4] = e4p * uk;
+            }
+          };
+          // W Atau_k rows 1..4: row i of A_k times W PTau
+          auto wat = [&](int k, R W4[4][5]) {
+#pragma unroll
+            for (int i = 1; i < 5; ++i) {
+              R r[5];
+              arow(k, i, r);
+#pragma unroll
+              for (int j = 0; j < 5; ++j)
+                W4[i - 1][j] = r[0] * WPT[j] + r[1] * WPT[5 + j] + r[2] * WPT[10 + j] +
+                               r[3] * WPT[15 + j] + r[4] * WPT[20 + j];
+            }
+          };
+          R Wt[4][5], col[5], r4[5];
+          // block 11, entry (5,3): Atau1(3,5) for Atau1(5,5)
+          wat(0, Wt);
+          arow(0, 4, r4);
+          acc[4 + 10] += sR[0] * sR[0] * ((Wt[1][4] - Wt[3][4]) * r4[2]);
+          // block 22, entries (3,3), (4,3), (5,3): row 2 of Atau2 for m = 1, 2, 4;
+          // (5,3) also Atau2(3,5) for Atau2(5,5)
+          wat(1, Wt);
+#pragma unroll
+          for (int m = 0; m < 5; ++m) {
+            R r[5];
+            arow(1, m, r);
+            col[m] = r[2];
+          }
+          const R s1 = sR[1] * sR[1];
+#pragma unroll
+          for (int i = 2; i < 5; ++i) {
+            R d = (Wt[0][0] - Wt[i - 1][0]) * col[0] + (Wt[0][1] - Wt[i - 1][1]) * col[1] +
+                  (Wt[0][3] - Wt[i - 1][3]) * col[3];
+            if (i == 4) d += (Wt[1][4] - Wt[3][4]) * col[4];
+            acc[i + 10] += s1 * d;
+          }
+          // block 33, entries (3,4), (4,4), (5,4): row 2 of Atau3 for m = 1, 2, 3;
+          // entry (5,3): Atau3(3,5) for Atau3(5,5)
+          wat(2, Wt);
+#pragma unroll
+          for (int m = 0; m < 5; ++m) {
+            R r[5];
+            arow(2, m, r);
+            col[m] = r[3];
+          }
+          const R s2 = sR[2] * sR[2];
+#pragma unroll
+          for (int i = 2; i < 5; ++i)
+            acc[i + 15] += s2 * ((Wt[0][0] - Wt[i - 1][0]) * col[0] +
+                                 (Wt[0][1] - Wt[i - 1][1]) * col[1] +
+                                 (Wt[0][2] - Wt[i - 1][2]) * col[2]);
+          arow(2, 4, r4);
+          acc[4 + 10] += s2 * ((Wt[1][4] - Wt[3][4]) * r4[2]);
         }
       } else {
         // Shakib's diagonal tau.  With the products of the state with W tau,
@@ -1842,7 +1922,11 @@ bool g_quad_ready = false;
 // e3wmlt.f:97-122): no Euler flux and no acceleration.  The four lanes of
 // an element sum their contributions as the residual pass does.  iabs
 // takes |.| of each element-node value (asires.f:90, itrFDI's estimate of
-// the cancellation error).
+// the cancellation error).  With the matrix tau (itau = 10) tau M is
+// PTau M (e3tau.f:723-732), PTau the point's matrix tau at the state, read
+// from the W PTau records of the last point pass at that state (it depends
+// on the state alone: e3ls.f:198-214 forms it from the state's point values,
+// acceleration and metrics).
 struct MresArgs {
   const int* ien;       // 4 planes of nel
   const double* node;   // 16 per node: y (local order), ac, x
@@ -1853,6 +1937,7 @@ struct MresArgs {
   double fct1;          // almi/gami/alfi*Dtgl
   int yp_stride;        // 16: yp is y itself (ElmMFG), 8: a packed perturbed state
   int iabs;
+  const double* mtau;   // itau = 10: nel*4 x mt_stride<double>() W PTau; else null
 };
 
 __global__ void __launch_bounds__(256) k_mres(MresArgs A) {
@@ -1937,11 +2022,20 @@ __global__ void __launch_bounds__(256) k_mres(MresArgs A) {
     M[4] += A.fct1 * (rhop * (eip + rkp));
   }
   double TM[5];
-  TM[0] = t.tau[0] * M[0];
-  TM[1] = t.tau[1] * M[1];
-  TM[2] = t.tau[1] * M[2];
-  TM[3] = t.tau[1] * M[3];
-  TM[4] = t.tau[2] * M[4];
+  if (A.mtau) {
+    const double* wp = A.mtau + ((size_t)e * 4 + q) * mt_stride<double>();
+    const double iW = 1.0 / WdetJ;
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+      TM[i] = (wp[5 * i] * M[0] + wp[5 * i + 1] * M[1] + wp[5 * i + 2] * M[2] +
+               wp[5 * i + 3] * M[3] + wp[5 * i + 4] * M[4]) * iW;
+  } else {
+    TM[0] = t.tau[0] * M[0];
+    TM[1] = t.tau[1] * M[1];
+    TM[2] = t.tau[1] * M[2];
+    TM[3] = t.tau[1] * M[3];
+    TM[4] = t.tau[2] * M[4];
+  }
   const double divu = g[0][1] + g[1][2] + g[2][3];
   double F[3][5];
 #pragma unroll
@@ -2274,6 +2368,8 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
     k_asm_blocks<float, float, false, false><<<g2, 64, 0, c->stream>>>(B);
   else if (mt && dc)
     k_asm_blocks<double, double, true, true><<<g2, 64, 0, c->stream>>>(B);
+  else if (mt && !lhs && prec)  // the matrix-free solver's e3bdg_nd
+    k_asm_blocks<double, double, false, true, true><<<g2, 64, 0, c->stream>>>(B);
   else if (mt)
     k_asm_blocks<double, double, false, true><<<g2, 64, 0, c->stream>>>(B);
   else if (dc)
@@ -2371,6 +2467,11 @@ int launch_mres(phg_ctx* c, const double* yp, double* out, bool accumulate, bool
   M.ph = ph;
   M.fct1 = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   M.iabs = iabs ? 1 : 0;
+  M.mtau = c->prm.itau == 10 ? reinterpret_cast<const double*>(c->mtau) : nullptr;
+  if (c->prm.itau == 10 && (!c->mtau || c->precision != PHG_FP64)) {
+    phg_set_error("modified residual: the matrix tau needs the FP64 W PTau records");
+    return PHG_EINVAL;
+  }
   k_mres<<<(int)((c->nel + 63) / 64), 256, 0, c->stream>>>(M);
   PHG_HIP(hipGetLastError());
   const long long n = c->nshg;

@@ -638,9 +638,9 @@ int mfg_fdi(phg_ctx* c, const double* u, double* eg) {
 }
 
 // Flags the matrix-free path restates: the frozen set, idiff 0/1/3, lhs = 0
-// (itrdrv.f:494); iDC and itau != 0 are refused (e3dc's ires = 2 branch
-// reads giju, rTLS and raLS that e3tau forms only for ires 1 and 3,
-// e3tau.f:185-262; e3bdg_nd is not built)
+// (itrdrv.f:494), itau 0 or 10 (e3bdg or e3bdg_nd for BDiag); iDC is
+// refused (e3dc's ires = 2 branch reads giju, rTLS and raLS that e3tau forms
+// only for ires 1 and 3, e3tau.f:185-262)
 int mfg_check(phg_ctx* c) {
   if (c->precision != PHG_FP64) {
     phg_set_error("SolMFG: finite-difference products need a PHG_FP64 context");
@@ -650,9 +650,8 @@ int mfg_check(phg_ctx* c) {
     phg_set_error("SolMFG: lhs must be 0 (itrdrv.f:494); got %d", c->prm.lhs);
     return PHG_EINVAL;
   }
-  if (c->prm.idc != 0 || c->prm.itau != 0) {
-    phg_set_error("SolMFG: iDC %d / itau %d not supported by the matrix-free path", c->prm.idc,
-                  c->prm.itau);
+  if (c->prm.idc != 0) {
+    phg_set_error("SolMFG: iDC %d not supported by the matrix-free path", c->prm.idc);
     return PHG_EINVAL;
   }
   const int K = c->prm.Kspace;
@@ -666,7 +665,10 @@ int mfg_check(phg_ctx* c) {
 
 // ElmMFG (elmmfg.f:128-256) on the resident y/ac: res and BDiag as the
 // assembly forms them without lhsK (the tangent's diagonal blocks, which
-// e3bdg.f writes out term by term), and the modified residual rmes
+// e3bdg.f writes out term by term; with the matrix tau e3bdg_nd's, the
+// block pass adding its mis-indexed entries), and the modified residual
+// rmes.  The point pass also leaves the state's W PTau records that the
+// modified residual reads with the matrix tau.
 int elm_mfg(phg_ctx* c) {
   const bool prec = c->prm.iprec != 0;
   PHG_TRY(mfg_check(c));
@@ -1390,6 +1392,8 @@ extern "C" int phg_itrres(phg_ctx* c, const double* yp, const double* yc, const 
   PHG_TRY(upload(c, c->y, yc, 5 * n));
   PHG_TRY(upload(c, c->ac, ac, 5 * n));
   PHG_TRY(launch_pack_nodes(c));
+  // the matrix tau at the state: the point pass's W PTau records
+  if (c->prm.itau == 10) PHG_TRY(launch_assembly(c, false, false));
   PHG_TRY(upload(c, c->mfg.t2, yp, 5 * n));
   PHG_TRY(launch_yshuffle(c, c->mfg.t2, c->mfg.t2, true));
   PHG_TRY(launch_mfg_point(c, c->mfg.t2, nullptr, 0.0, c->mfg.pack, false, false));
@@ -1412,6 +1416,7 @@ extern "C" int phg_au1mfg(phg_ctx* c, const double* ypre, const double* y, const
   PHG_TRY(upload(c, c->y, y, 5 * n));
   PHG_TRY(upload(c, c->ac, ac, 5 * n));
   PHG_TRY(launch_pack_nodes(c));
+  if (c->prm.itau == 10) PHG_TRY(launch_assembly(c, false, false));
   PHG_TRY(upload(c, c->bdiag, BDiag, 25 * n));
   PHG_TRY(upload(c, c->mfg.ypre, ypre, 5 * n));
   PHG_TRY(upload(c, c->rmes, rmes, 5 * n));

@@ -165,6 +165,32 @@ def make_mfg(N, seed):
                         BDiag_idiff=bd1)
 
 
+def make_mfg_mtau(N, seed):
+    """The matrix-free solver's pieces with the matrix tau (itau = 10): ElmMFG
+    (res, rmes, and the block diagonal of e3bdg_nd), ItrRes of perturbed
+    variables, itrFDI's interval and Au1MFG's product."""
+    case = mesh.box_case(N, mach=0.5, kspace=10, jitter=0.1)
+    rng = np.random.default_rng(seed)
+    y, ac = pyref.Reference(case).predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    prm = dict(case.params, lhs=0, iprec=1, itau=10)
+    R = pyref.Reference(case, params=prm)
+    res, rmes, bd = R.elmmfg(y, ac)
+    yp = np.asfortranarray(y * (1 + 0.01 * rng.standard_normal(y.shape)))
+    rm_p = R.itrres(yp, y, ac)
+    bd_lu, _ = R.i3lu(bd, None, "LU_Fact ")
+    _, res_f = R.i3lu(bd_lu, res, "forward ")
+    _, rmes_f = R.i3lu(bd_lu, rmes, "forward ")
+    _, ypre = R.i3lu(bd_lu, R.yshuffle(y, "new2old "), "product ")
+    eg = R.itrfdi(ypre, y, ac, rmes_f, res_f, bd_lu)
+    u = np.asfortranarray(rng.standard_normal(y.shape))
+    au1 = R.au1mfg(ypre, y, ac, rmes_f, res_f, bd_lu, eg, u)
+    np.savez_compressed(os.path.join(HERE, f"ref_mfg_mtau_box{N}.npz"), N=N, y=y, ac=ac,
+                        res=res, rmes=rmes, BDiag=bd, yp=yp, rmes_p=rm_p, eGMRES=eg, u=u,
+                        au1=au1)
+
+
 MAKERS = {
     "box": lambda: [make(N, seed) for N, seed in ((2, 11), (3, 12))],
     "alpha": lambda: make_alpha(3, 13),
@@ -172,6 +198,7 @@ MAKERS = {
     "dc": lambda: make_dc(3, 15),
     "mtau": lambda: make_mtau(3, 17),
     "mfg": lambda: make_mfg(3, 19),
+    "mfg_mtau": lambda: make_mfg_mtau(3, 21),
 }
 
 if __name__ == "__main__":

@@ -14,7 +14,11 @@ Bars:
   interval is estimated along the unnormalised residual), Dy at 1e4 times
   the interval within 1e-5 of the oracle's and of SolGMRs' Dy;
 * the interval itself (itrFDI) within 1e-5;
-* iDC, itau != 0, lhs = 1 and the mixed-precision context are refused;
+* with the matrix tau (itau = 10) the same bars: ElmMFG's BDiag is then
+  e3bdg_nd's, mis-indexed entries included (against the oracle and the
+  reference fixture ref_mfg_mtau_box3.npz);
+* iDC, itau other than 0 and 10, lhs = 1 and the mixed-precision context
+  are refused;
 * a PHG_FP64_MATRIX_FREE context (no lhsK, as itrdrv.f:163-185 allocates
   for impl 2x) runs the same kernels as a PHG_FP64 one: its SolMFG results
   are bitwise equal, it holds at least the matrix's bytes less, and the
@@ -66,10 +70,10 @@ def state(request):
     return case, y, ac
 
 
-@pytest.mark.parametrize("idiff", [0, 1])
-def test_elmmfg_against_oracle(state, idiff):
+@pytest.mark.parametrize("idiff,itau", [(0, 0), (1, 0), (0, 10), (1, 10)])
+def test_elmmfg_against_oracle(state, idiff, itau):
     case, y, ac = state
-    prm = mfg_params(case, idiff=idiff)
+    prm = mfg_params(case, idiff=idiff, itau=itau)
     ref = pyoracle.Oracle(case, params=prm).elmmfg(y, ac)
     gpu = solver.GpuSolver(case, params=prm)
     try:
@@ -78,7 +82,7 @@ def test_elmmfg_against_oracle(state, idiff):
         gpu.close()
     for name, a, b in zip(("res", "rmes", "BDiag"), out, ref):
         e = rel(a, b)
-        print(f"[mfg] idiff {idiff} ElmMFG {name}: {e:.2e}", flush=True)
+        print(f"[mfg] idiff {idiff} itau {itau} ElmMFG {name}: {e:.2e}", flush=True)
         assert e < TOL, name
 
 
@@ -106,9 +110,25 @@ def test_mfg_pieces_against_reference_golden():
         gpu.close()
 
 
-def test_itrres_against_oracle_accumulates(state):
+def test_mfg_matrix_tau_pieces_against_reference_golden():
+    g = np.load(os.path.join(GOLD, "ref_mfg_mtau_box3.npz"))
+    case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+    y, ac = np.asfortranarray(g["y"]), np.asfortranarray(g["ac"])
+    gpu = solver.GpuSolver(case, params=mfg_params(case, itau=10))
+    try:
+        res, rmes, bd = gpu.elmmfg(y, ac)
+        assert rel(res, g["res"]) < TOL
+        assert rel(rmes, g["rmes"]) < TOL
+        assert rel(bd, g["BDiag"]) < TOL
+        assert rel(gpu.itrres(g["yp"], y, ac), g["rmes_p"]) < TOL
+    finally:
+        gpu.close()
+
+
+@pytest.mark.parametrize("itau", [0, 10])
+def test_itrres_against_oracle_accumulates(state, itau):
     case, y, ac = state
-    prm = mfg_params(case)
+    prm = mfg_params(case, itau=itau)
     rng = np.random.default_rng(5)
     yp = np.asfortranarray(y * (1 + 0.02 * rng.standard_normal(y.shape)))
     acc = np.asfortranarray(rng.standard_normal(y.shape))
@@ -148,14 +168,15 @@ def base_point(o, y, ac):
     return lu, r, m, ypre, o.itrfdi(ypre, y, ac, m, r, lu)
 
 
+@pytest.mark.parametrize("itau", [0, 10])
 @pytest.mark.parametrize("scale", [1.0, 100.0], ids=["itrfdi_interval", "interval_x100"])
-def test_au1mfg_against_oracle(state, scale):
+def test_au1mfg_against_oracle(state, scale, itau):
     """The one-sided product of a random direction (zero where res is: the
     constrained equations and periodic slaves): within ten times the
     oracle's own rounding sensitivity, measured 1e-8..1e-7 at itrFDI's
     interval and 1e-10..1e-9 at a hundred times it."""
     case, y, ac = state
-    prm = mfg_params(case)
+    prm = mfg_params(case, itau=itau)
     o = pyoracle.Oracle(case, params=prm)
     lu, r, m, ypre, eg = base_point(o, y, ac)
     rng = np.random.default_rng(11)
@@ -192,20 +213,27 @@ def test_au1mfg_against_reference_golden():
     assert e < 1e-6
 
 
+@pytest.mark.parametrize("itau", [0, 10])
 @pytest.mark.parametrize("fdi", [False, True], ids=["given_interval", "itrfdi"])
-def test_solmfg_against_oracle(state, fdi):
+def test_solmfg_against_oracle(state, fdi, itau):
     """At itrFDI's interval, which the reference estimates along the
     unnormalised residual (solmfg.f:165-168) and applies to unit Krylov
     vectors, the products of Krylov directions keep two or three digits and
     the oracle's own Dy moves by about 1 % when y moves by one ulp: the GPU's
-    Dy is held to five times that measured sensitivity, and the interval
-    itself (a ratio of smooth differences) to 1e-5."""
+    Dy is held to five times that measured sensitivity.  The interval
+    itself is a ratio of second differences taken at sqrt(epsM): the
+    oracle's moves by 2e-7..1.3e-5 over three one-ulp moves of y (the upper
+    end with the matrix tau on the 4^3 box); the GPU's is held to five times
+    the largest of those and to 1e-4."""
     case, y, ac = state
-    prm = mfg_params(case)
+    prm = mfg_params(case, itau=itau)
     ref = pyoracle.Oracle(case, params=prm).solmfg(y, ac, fdi=True)
     twin = pyoracle.Oracle(case, params=prm).solmfg(ulp_twin(case, y, ac), ac, fdi=False,
                                                    egmres=ref["eGMRES"])
     noise = rel(twin["Dy"], ref["Dy"])
+    eg_noise = max(abs(pyoracle.Oracle(case, params=prm).solmfg(
+        ulp_twin(case, y, ac, seed), ac, fdi=True)["eGMRES"] - ref["eGMRES"]) / ref["eGMRES"]
+        for seed in (99, 7, 3))
     gpu = solver.GpuSolver(case, params=prm)
     try:
         out = gpu.solmfg(y, ac, egmres=0.0 if fdi else ref["eGMRES"], fdi=fdi)
@@ -214,8 +242,9 @@ def test_solmfg_against_oracle(state, fdi):
     e_eg = abs(out["eGMRES"] - ref["eGMRES"]) / ref["eGMRES"]
     e_dy = rel(out["Dy"], ref["Dy"])
     print(f"[mfg] SolMFG fdi={fdi}: eGMRES {out['eGMRES']:.6e} ({e_eg:.1e}), iKs "
-          f"{out['iKs']}/{ref['iKs']}, Dy {e_dy:.2e} (oracle rounding {noise:.2e})", flush=True)
-    assert e_eg < 1e-5
+          f"{out['iKs']}/{ref['iKs']}, Dy {e_dy:.2e} (oracle rounding {noise:.2e}, interval "
+          f"{eg_noise:.1e})", flush=True)
+    assert e_eg < 5 * eg_noise + 1e-9 and e_eg < 1e-4
     assert abs(out["iKs"] - ref["iKs"]) <= 1
     for k in ("res", "rmes", "BDiag"):
         assert rel(out[k], ref[k]) < TOL, k
@@ -245,6 +274,24 @@ def test_solmfg_at_a_wide_interval_against_oracle_and_solgmrs(state):
     print(f"[mfg] SolMFG at 1e4 x eGMRES: vs oracle {e:.2e}, vs SolGMRs {eg_:.2e}", flush=True)
     assert out["iKs"] == ref["iKs"]
     assert e < 1e-5 and eg_ < 1e-5
+
+
+def test_solmfg_matrix_tau_at_a_wide_interval_against_oracle(state):
+    """The matrix tau at 1e4 times itrFDI's interval: Dy within 1e-5 of the
+    oracle's, the same Krylov count."""
+    case, y, ac = state
+    prm = mfg_params(case, Kspace=10, itau=10)
+    eg = 1e4 * pyoracle.Oracle(case, params=prm).solmfg(y, ac, fdi=True)["eGMRES"]
+    ref = pyoracle.Oracle(case, params=prm).solmfg(y, ac, fdi=False, egmres=eg)
+    gpu = solver.GpuSolver(case, params=prm)
+    try:
+        out = gpu.solmfg(y, ac, egmres=eg, fdi=False)
+    finally:
+        gpu.close()
+    e = rel(out["Dy"], ref["Dy"])
+    print(f"[mfg] SolMFG itau 10 at 1e4 x eGMRES: vs oracle {e:.2e}", flush=True)
+    assert out["iKs"] == ref["iKs"]
+    assert e < 1e-5
 
 
 def test_restart_cycles_against_oracle():
@@ -312,7 +359,7 @@ def test_resident_newton_iterations_against_oracle():
         gpu.close()
 
 
-@pytest.mark.parametrize("over", [{"idc": 1}, {"itau": 10}, {"lhs": 1}], ids=str)
+@pytest.mark.parametrize("over", [{"idc": 1}, {"itau": 1}, {"lhs": 1}], ids=str)
 def test_flags_outside_the_matrix_free_path_are_refused(state, over):
     case, y, ac = state
     gpu = solver.GpuSolver(case, params=mfg_params(case, **over))

@@ -4,11 +4,15 @@
    fixture tests/golden/ref_mfg_box3.npz (ElmMFG's res, modified residual
    rmes and e3bdg's BDiag; ItrRes plain and with iabres; i3LU 'product';
    itrFDI's interval; Au1MFG's and Au2MFG's products; ElmMFG with idiff 1)
-   and, where oracle/_ref is built, live calls on other states.
+   and, where oracle/_ref is built, live calls on other states; the same
+   with the matrix tau (itau = 10, fixture ref_mfg_mtau_box3.npz), whose
+   block diagonal comes from e3bdg_nd.f.
 2. What the mathematics fixes:
    * e3bdg writes out the diagonal blocks of the assembled tangent term by
      term: the oracle's e3bdg BDiag equals ElmGMRs's diagonal blocks to
-     rounding;
+     rounding; e3bdg_nd does so for the matrix tau except in the six entries
+     its eight mis-indexed statements reach, which the compiled reference
+     fixes to the bit;
    * the modified residual of yp at the state y is the residual linearised
      with frozen coefficients, so its derivative along u is the assembled
      tangent: Au1MFG's difference quotient equals SolGMRs' scaled matvec
@@ -98,11 +102,28 @@ def test_mfg_pieces_bitwise_equal_reference_golden():
     assert np.array_equal(bd1, g["BDiag_idiff"])
 
 
+def test_mfg_matrix_tau_pieces_bitwise_equal_reference_golden():
+    g = np.load(os.path.join(GOLD, "ref_mfg_mtau_box3.npz"))
+    case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+    o = pyoracle.Oracle(case, params=mfg_params(case, itau=10))
+    y, ac = np.asfortranarray(g["y"]), np.asfortranarray(g["ac"])
+    res, rmes, bd = o.elmmfg(y, ac)
+    assert np.array_equal(res, g["res"])
+    assert np.array_equal(rmes, g["rmes"])
+    assert np.array_equal(bd, g["BDiag"])
+    assert np.array_equal(o.itrres(g["yp"], y, ac), g["rmes_p"])
+    lu, r, m, ypre = preconditioned(o, res, rmes, bd, y)
+    eg = o.itrfdi(ypre, y, ac, m, r, lu)
+    assert eg == float(g["eGMRES"])
+    assert np.array_equal(o.au1mfg(ypre, y, ac, m, lu, eg, g["u"]), g["au1"])
+
+
 @pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
-@pytest.mark.parametrize("N,grade,idiff", [(4, 1.1, 0), (3, 1.0, 3)])
-def test_mfg_pieces_bitwise_equal_live_reference(N, grade, idiff):
+@pytest.mark.parametrize("N,grade,idiff,itau", [(4, 1.1, 0, 0), (3, 1.0, 3, 0), (4, 1.1, 0, 10),
+                                                (3, 1.0, 1, 10)])
+def test_mfg_pieces_bitwise_equal_live_reference(N, grade, idiff, itau):
     case, y, ac, rng = perturbed(N, 40 + N, grade)
-    prm = mfg_params(case, idiff=idiff)
+    prm = mfg_params(case, idiff=idiff, itau=itau)
     o, R = pyoracle.Oracle(case, params=prm), pyref.Reference(case, params=prm)
     res, rmes, bd = o.elmmfg(y, ac)
     r2, m2, b2 = R.elmmfg(y, ac)
@@ -128,6 +149,26 @@ def test_e3bdg_is_the_tangent_diagonal():
     assert rel(bd, bd_t) < 1e-13
 
 
+# the entries (row, column; 0-based) of a node's block that e3bdg_nd.f's
+# mis-indexed statements write: (5,3) of the 11 block, (3,3), (4,3), (5,3) of
+# the 22 block, (3,4), (4,4), (5,4), (5,3) of the 33 block (1-based)
+SLIPS = {(2, 2), (3, 2), (4, 2), (2, 3), (3, 3), (4, 3)}
+
+
+def test_e3bdg_nd_is_the_tangent_diagonal_but_for_its_slips():
+    case, y, ac, _ = perturbed(4, 3, grade=1.1)
+    _, _, bd = pyoracle.Oracle(case, params=mfg_params(case, itau=10)).elmmfg(y, ac)
+    _, _, bd_t, _ = pyoracle.Oracle(case, params=dict(case.params, itau=10)).elmgmrs(y, ac)
+    for i in range(5):
+        for j in range(5):
+            e = rel(bd[:, i, j], bd_t[:, i, j])
+            if (i, j) in SLIPS:
+                # the slips move the entry by 0.1-2 % of its scale
+                assert e > 1e-4, (i, j, e)
+            else:
+                assert e < 1e-13, (i, j, e)
+
+
 def test_shuffle_and_block_product_invert():
     case, y, ac, _ = perturbed(3, 5)
     o = pyoracle.Oracle(case, params=mfg_params(case))
@@ -143,12 +184,14 @@ def test_shuffle_and_block_product_invert():
     assert rel(w, v) < 1e-13
 
 
-def scaled_matvec(o, y, ac, u):
+def scaled_matvec(o, y, ac, u, lu=None):
     """SolGMRs' operator on the same state: L^-1 bc3per(K (U^-1 u)(iper))
-    with K the assembled tangent, through the Spsi3pre-scaled lhsK."""
+    with K the assembled tangent, through the Spsi3pre-scaled lhsK; lu: the
+    factored block diagonal to scale with (default: the tangent's)."""
     _, _, bd, lk = o.elmgmrs(y, ac)
-    lu = bd.copy(order="F")
-    o.i3lu(lu, None, "LU_Fact ")
+    if lu is None:
+        lu = bd.copy(order="F")
+        o.i3lu(lu, None, "LU_Fact ")
     ls = lk.copy(order="F")
     o.spsi3pre(lu, ls)
     p = u.copy(order="F")
@@ -157,10 +200,13 @@ def scaled_matvec(o, y, ac, u):
     return p
 
 
-@pytest.mark.parametrize("N,grade", [(3, 1.0), (4, 1.1)])
-def test_fd_product_is_the_scaled_tangent(N, grade):
+@pytest.mark.parametrize("N,grade,itau", [(3, 1.0, 0), (4, 1.1, 0), (4, 1.1, 10)])
+def test_fd_product_is_the_scaled_tangent(N, grade, itau):
+    """With the matrix tau the modified residual's derivative is the matrix-
+    tau tangent, scaled here with SolMFG's own factored e3bdg_nd block
+    diagonal."""
     case, y, ac, rng = perturbed(N, 7 + N, grade, bc=True)
-    o = pyoracle.Oracle(case, params=mfg_params(case))
+    o = pyoracle.Oracle(case, params=mfg_params(case, itau=itau))
     res, rmes, bd = o.elmmfg(y, ac)
     lu, r, m, ypre = preconditioned(o, res, rmes, bd, y)
     eg = o.itrfdi(ypre, y, ac, m, r, lu)
@@ -168,7 +214,8 @@ def test_fd_product_is_the_scaled_tangent(N, grade):
     # equations and on periodic slaves, as res is
     u = np.asfortranarray(rng.standard_normal(y.shape))
     u[r == 0.0] = 0.0
-    kv = scaled_matvec(pyoracle.Oracle(case), y, ac, u)
+    tangent = pyoracle.Oracle(case, params=dict(case.params, itau=itau))
+    kv = scaled_matvec(tangent, y, ac, u, lu)
 
     def err(f):
         return rel(o.au1mfg(ypre, y, ac, m, lu, f * eg, u), kv)
@@ -183,7 +230,7 @@ def test_fd_product_is_the_scaled_tangent(N, grade):
     assert 9 < err(1e5) / err(1e4) < 11
     # Au2MFG: res - K Dy by a two-sided difference (second order)
     dy = u
-    kd = scaled_matvec(pyoracle.Oracle(case), y, ac, dy)
+    kd = scaled_matvec(tangent, y, ac, dy, lu)
     assert rel(o.au2mfg(ypre, y, ac, r, dy, lu), r - kd) < 1e-6
 
 
@@ -215,6 +262,6 @@ def test_restart_cycles_use_the_two_sided_residual():
 
 def test_mfg_refuses_flags_it_does_not_restate():
     case, y, ac, _ = perturbed(3, 9)
-    for over in ({"idc": 1}, {"itau": 10}, {"idiff": 2}):
+    for over in ({"idc": 1}, {"itau": 1}, {"idiff": 2}):
         with pytest.raises(RuntimeError, match="rc=5"):
             pyoracle.Oracle(case, params=mfg_params(case, **over)).elmmfg(y, ac)
