@@ -64,10 +64,15 @@ typedef struct phg_params {
   int idc;                              /* /solpar/ iDC: 0 off, 1 DC-mallet,
                                            2 DC-quadratic, 3 DC-minimum
                                            (input_fform.cc:683-686, e3dc.f) */
-  int itau;                             /* /genpar/ itau: 0 Shakib's diagonal
-                                           tau (e3tau.f:140-176), 10 the
+  int itau;                             /* /genpar/ itau (input_fform.cc:
+                                           663-674): 0 Shakib's diagonal tau
+                                           (e3tau.f:140-176), 1 Franca's
+                                           (e3tau.f:54-139), 10 and 11 the
                                            matrix tau of e3tau_nd
-                                           (e3tau.f:284-744)                */
+                                           (e3tau.f:284-744; it reads no
+                                           itau); 2 and 3, which the
+                                           compressible e3tau leaves unset,
+                                           are refused                      */
 } phg_params;
 
 /* Mesh arrays as the Fortran host holds them (readnblk.f, genadj.f);
@@ -164,13 +169,13 @@ int phg_update(phg_ctx* ctx);
 /* Matrix-free GMRES (Solver Type: GMRES Matrix Free, impl = 20; selected at
  * itrdrv.f:489-506 with lhs = 0, iprec from LHSupd).  PHG_FP64 and
  * PHG_FP64_MATRIX_FREE contexts,
- * lhs = 0, iDC = 0 (PHG_EINVAL otherwise); idiff 0, 1, 3; itau 0 or 10
- * (BDiag from e3bdg.f or e3bdg_nd.f).
+ * lhs = 0, iDC = 0 (PHG_EINVAL otherwise); idiff 0, 1, 3; itau 0, 1 (BDiag
+ * from e3bdg.f), 10, 11 (from e3bdg_nd.f).
  *
  * ElmMFG(y,ac,x,shp,shgl,iBC,BC,shpb,shglb,res,rmes,BDiag,iper,ilwork,rerr)
  *                                                     elmmfg.f:1
  * res as ElmGMRs forms it, rmes the modified residual (e3 with ires = 3),
- * BDiag (iprec != 0) the block diagonal of e3bdg.f (itau = 10: of
+ * BDiag (iprec != 0) the block diagonal of e3bdg.f (itau >= 10: of
  * e3bdg_nd.f, whose eight mis-indexed statements it reproduces); no lhsK is
  * formed. */
 int phg_elmmfg(phg_ctx* ctx, const double* y, const double* ac, double* res,

@@ -22,7 +22,7 @@
  *   phSolver/compressible/e3conv.f:1-352     e3conv
  *   phSolver/compressible/e3visc.f:1-357     e3visc
  *   phSolver/compressible/e3ls.f:1-771       e3LS
- *   phSolver/compressible/e3tau.f:1-283,1397-1507  e3tau (itau=0), e3gijd
+ *   phSolver/compressible/e3tau.f:1-283,1397-1507  e3tau (itau 0 and 1), e3gijd
  *   phSolver/compressible/e3massr.f, e3massl.f, e3wmlt.f
  *   phSolver/compressible/asiq.f, e3q.f, e3qvar.f, phSolver/common/qpbc.f
  *                                            diffusive flux (idiff 1, 3)
@@ -358,7 +358,9 @@ int orc_elmgmrs(const orc_mesh* m, const orc_params* P, const double* y,
     if (!bc_supported(m->iBC[n])) return 3;
   if (P->idiff != 0 && P->idiff != 1 && P->idiff != 3) return 5;
   if (P->idc < 0 || P->idc > 3) return 5; /* input_fform.cc:683-686 */
-  if (P->itau != 0 && P->itau != 10) return 5; /* diagonal Shakib or e3tau_nd */
+  /* Shakib's or Franca's diagonal tau, or e3tau_nd; itau 2 and 3 have no
+   * branch in the compressible e3tau (tau is left unset) */
+  if (P->itau != 0 && P->itau != 1 && P->itau != 10 && P->itau != 11) return 5;
   /* elmgmr.f:361-391 global reconstruction of the diffusive flux */
   double* qres = NULL;
   if (P->idiff != 0) {
@@ -1005,7 +1007,7 @@ static int mfg_flags(const orc_mesh* m, const orc_params* P) {
   /* e3dc's ires = 2 branch reads giju, rTLS and raLS, which e3tau forms only
    * for ires 1 and 3 (e3tau.f:185-262) */
   if (P->idc != 0) return 5;
-  if (P->itau != 0 && P->itau < 10) return 5;
+  if (P->itau != 0 && P->itau != 1 && P->itau != 10 && P->itau != 11) return 5;
   return 0;
 }
 

@@ -925,7 +925,9 @@ static void e3_body(const orc_params* P, const double ylp[4][5], const double yl
       rLyi[3] = rLyi[3] - divqi[2];
       rLyi[4] = rLyi[4] - divqi[3];
     }
-    /* ---- e3tau.f:1-283 (itau = 0, ipord = 1); e3tau_nd (itau >= 10) ---- */
+    /* ---- e3tau.f:1-283 (itau = 0 or 1, ipord = 1); e3tau_nd (itau 10 and
+     * 11: e3ls.f:188-214 calls it for every itau >= 10, and it reads no
+     * itau) ---- */
     double gijd[6];
     double PTau[5][5];
     if (P->itau >= 10) {
@@ -950,6 +952,31 @@ static void e3_body(const orc_params* P, const double ylp[4][5], const double yl
     tau[0] = PT125 * fct / (rho * (gijd[0] + gijd[2] + gijd[5])) * P->taucfct;
     tau[1] = ONE / fct;
     tau[2] = tau[1] / cv * P->temper;
+    if (P->itau == 1) {
+      /* e3tau.f:54-139 (Diagonal-Franca; ipord = 1: celt = pt66; the
+       * reference's 1.0e-15 is a default-kind real literal) */
+      const double g1 = gijd[0], g2 = gijd[1], g3 = gijd[2], g4 = gijd[3], g5 = gijd[4],
+                   g6 = gijd[5];
+      double fa = g1 * g3 * g6 - g1 * g5 * g5 - g3 * g4 * g4 - g6 * g2 * g2 + g2 * g4 * g5 * TWO;
+      double uh1 = u1 * u1 * (g3 * g6 - g5 * g5) + u2 * u2 * (g1 * g6 - g4 * g4) +
+                   u3 * u3 * (g1 * g3 - g2 * g2) +
+                   TWO * (u1 * u2 * (g4 * g5 - g2 * g6) + u1 * u3 * (g2 * g5 - g4 * g3) +
+                          u2 * u3 * (g4 * g2 - g1 * g5));
+      uh1 = TWO * sqrt(uh1 / fa);
+      double h2o2u = u1 * u1 * g1 + u2 * u2 * g3 + u3 * u3 * g6 +
+                     (u1 * u2 * g2 + u1 * u3 * g4 + u2 * u3 * g5) * TWO + (double)1.0e-15f;
+      h2o2u = ONE / sqrt(h2o2u);
+      fa = PT5 * rho * uh1 / rmu;
+      tau[0] = PT5 * uh1 * fmin(ONE, fa) * P->taucfct;
+      /* iremoveStabTimeTerm = 0 */
+      const double dtsn = ONE / (P->Dtgl * P->dtsfct);
+      tau[1] = fmin(dtsn, h2o2u);
+      tau[1] = tau[1] / rho;
+      tau[2] = tau[1] / cv;
+      fa = h2o2u * h2o2u * rk * PT66 / rmu;
+      tau[1] = fmin(tau[1], fa);
+      tau[2] = fmin(tau[2], fa * rmu / con) * P->temper;
+    }
     double Lt[5]; /* e3tau.f:188: rLyitemp, the LS residual before tau */
     for (int f = 0; f < 5; ++f) Lt[f] = rLyi[f];
     if (!want_r) {

@@ -250,8 +250,9 @@ __device__ inline int gu_idx(int i, int j) {
 }
 
 // Thermodynamic state at a point from Y = (p, u, v, w, T) (getthm.f:106-172
-// with ipress = 0; e3mtrx.f's A0/A_i entries) and Shakib's diagonal tau
-// (itau = 0, e3tau.f:140-176).  ei: internal energy, rk: kinetic energy.
+// with ipress = 0; e3mtrx.f's A0/A_i entries) and the diagonal tau: Shakib's
+// (itau = 0, e3tau.f:140-176) or Franca's (itau = 1, e3tau.f:54-139 with
+// ipord = 1).  ei: internal energy, rk: kinetic energy.
 __device__ inline void point_thermo(const double Y[5], const PhysParams& ph, const double G[6],
                                     double trG, double GG, Thermo& t, double& ei, double& rk) {
   const double cv = ph.Rgas / ph.gamma1;
@@ -280,6 +281,31 @@ __device__ inline void point_thermo(const double Y[5], const PhysParams& ph, con
   t.tau[0] = 0.125 * sq / (t.rho * trG) * ph.taucfct;
   t.tau[1] = 1.0 / sq;
   t.tau[2] = t.tau[1] / cv * ph.temper;
+  if (ph.itau == 1) {
+    // the diffusive length |u| h1 from the inverse metric, the advective
+    // h2 / (2|u|), the cell Reynolds number limiting tau_c, the time-step
+    // and viscous limits of tau_m and tau_e (celt = 2/3); the reference's
+    // 1.0e-15 is a single-precision literal
+    const double g1 = G[0], g2 = G[1], g3 = G[2], g4 = G[3], g5 = G[4], g6 = G[5];
+    const double u1 = t.u[0], u2 = t.u[1], u3 = t.u[2];
+    const double det = g1 * g3 * g6 - g1 * g5 * g5 - g3 * g4 * g4 - g6 * g2 * g2 + g2 * g4 * g5 * 2.0;
+    const double adj = u1 * u1 * (g3 * g6 - g5 * g5) + u2 * u2 * (g1 * g6 - g4 * g4) +
+                       u3 * u3 * (g1 * g3 - g2 * g2) +
+                       2.0 * (u1 * u2 * (g4 * g5 - g2 * g6) + u1 * u3 * (g2 * g5 - g4 * g3) +
+                              u2 * u3 * (g4 * g2 - g1 * g5));
+    const double uh1 = 2.0 * sqrt(adj / det);
+    const double h2 = 1.0 / sqrt(u1 * u1 * g1 + u2 * u2 * g3 + u3 * u3 * g6 +
+                                 (u1 * u2 * g2 + u1 * u3 * g4 + u2 * u3 * g5) * 2.0 +
+                                 (double)1.0e-15f);
+    const double re = 0.5 * t.rho * uh1 / mu;
+    t.tau[0] = 0.5 * uh1 * fmin(1.0, re) * ph.taucfct;
+    double tm = fmin(1.0 / (ph.Dtgl * ph.dtsfct), h2) / t.rho;
+    double te = tm / cv;
+    const double lim = h2 * h2 * rk * (2.0 / 3.0) / mu;
+    const double con = mu * cp / ph.pr;
+    t.tau[1] = fmin(tm, lim);
+    t.tau[2] = fmin(te, lim * mu / con) * ph.temper;
+  }
 }
 
 // Discontinuity-capturing factor at a point (iDC 1-3, ipord = 1): e3mtrx.f's
@@ -2223,8 +2249,8 @@ int upload_quadrature() {
 }
 
 PhysParams phys_of(const phg_params& p) {
-  return PhysParams{p.Rgas, p.gamma,   p.gamma1, p.pr,   p.mu,   p.dtsfct,
-                    p.taucfct, p.temper, p.almi, p.alfi, p.gami, p.Dtgl};
+  return PhysParams{p.Rgas,   p.gamma,  p.gamma1, p.pr,   p.mu,   p.dtsfct, p.taucfct,
+                    p.temper, p.almi,   p.alfi,   p.gami, p.Dtgl, p.itau};
 }
 
 }  // namespace
@@ -2315,7 +2341,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
     phg_set_error("iDC %d: the DC factor buffer is not allocated", c->prm.idc);
     return PHG_EINVAL;
   }
-  const bool mt = c->prm.itau == 10;
+  const bool mt = c->prm.itau >= 10;
   if (mt && !c->mtau) {
     phg_set_error("itau %d: the matrix tau buffer is not allocated", c->prm.itau);
     return PHG_EINVAL;
@@ -2420,11 +2446,11 @@ int launch_jv(phg_ctx* c, const double* v, double* w) {
                     c->prm.idc);
       return PHG_EINVAL;
     }
-    if (c->prm.itau == 10)
+    if (c->prm.itau >= 10)
       k_jv<true, true><<<gj, 256, 0, c->stream>>>(J);
     else
       k_jv<true, false><<<gj, 256, 0, c->stream>>>(J);
-  } else if (c->prm.itau == 10) {
+  } else if (c->prm.itau >= 10) {
     k_jv<false, true><<<gj, 256, 0, c->stream>>>(J);
   } else {
     k_jv<false, false><<<gj, 256, 0, c->stream>>>(J);
@@ -2467,8 +2493,8 @@ int launch_mres(phg_ctx* c, const double* yp, double* out, bool accumulate, bool
   M.ph = ph;
   M.fct1 = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   M.iabs = iabs ? 1 : 0;
-  M.mtau = c->prm.itau == 10 ? reinterpret_cast<const double*>(c->mtau) : nullptr;
-  if (c->prm.itau == 10 && (!c->mtau || c->precision != PHG_FP64)) {
+  M.mtau = c->prm.itau >= 10 ? reinterpret_cast<const double*>(c->mtau) : nullptr;
+  if (c->prm.itau >= 10 && (!c->mtau || c->precision != PHG_FP64)) {
     phg_set_error("modified residual: the matrix tau needs the FP64 W PTau records");
     return PHG_EINVAL;
   }

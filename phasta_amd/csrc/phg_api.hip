@@ -530,7 +530,7 @@ int check_idiff(phg_ctx* c) {
   }
   if (c->prm.idc != 0) {
     // DC factor per point; with the matrix tau DC giju per point (8 values)
-    const size_t need = (size_t)c->nel * 4 * (c->prm.itau == 10 ? 8 : 1);
+    const size_t need = (size_t)c->nel * 4 * (c->prm.itau >= 10 ? 8 : 1);
     if (c->dcw_len < need) {
       if (c->dcw) {
         PHG_HIP(hipStreamSynchronize(c->stream));
@@ -542,12 +542,14 @@ int check_idiff(phg_ctx* c) {
       c->dcw_len = need;
     }
   }
-  if (c->prm.itau != 0 && c->prm.itau != 10) {
-    phg_set_error("itau %d: Shakib's diagonal tau (0) and the matrix tau of e3tau_nd (10) "
-                  "are built", c->prm.itau);
+  // itau 2 and 3 have no branch in the compressible e3tau (tau left unset);
+  // 11 (Matrix-Modal) reaches the same e3tau_nd as 10 (e3ls.f:188-214)
+  if (c->prm.itau != 0 && c->prm.itau != 1 && c->prm.itau != 10 && c->prm.itau != 11) {
+    phg_set_error("itau %d: the diagonal taus of Shakib (0) and Franca (1) and the matrix "
+                  "tau of e3tau_nd (10, 11) are built", c->prm.itau);
     return PHG_EINVAL;
   }
-  if (c->prm.itau == 10 && !c->mtau) {
+  if (c->prm.itau >= 10 && !c->mtau) {
     // W PTau per point: 25 values padded to 28 floats or 26 doubles
     const size_t rec = c->precision == PHG_MIXED ? 28 * sizeof(float) : 26 * sizeof(double);
     uint8_t* p = nullptr;
@@ -1393,7 +1395,7 @@ extern "C" int phg_itrres(phg_ctx* c, const double* yp, const double* yc, const 
   PHG_TRY(upload(c, c->ac, ac, 5 * n));
   PHG_TRY(launch_pack_nodes(c));
   // the matrix tau at the state: the point pass's W PTau records
-  if (c->prm.itau == 10) PHG_TRY(launch_assembly(c, false, false));
+  if (c->prm.itau >= 10) PHG_TRY(launch_assembly(c, false, false));
   PHG_TRY(upload(c, c->mfg.t2, yp, 5 * n));
   PHG_TRY(launch_yshuffle(c, c->mfg.t2, c->mfg.t2, true));
   PHG_TRY(launch_mfg_point(c, c->mfg.t2, nullptr, 0.0, c->mfg.pack, false, false));
@@ -1416,7 +1418,7 @@ extern "C" int phg_au1mfg(phg_ctx* c, const double* ypre, const double* y, const
   PHG_TRY(upload(c, c->y, y, 5 * n));
   PHG_TRY(upload(c, c->ac, ac, 5 * n));
   PHG_TRY(launch_pack_nodes(c));
-  if (c->prm.itau == 10) PHG_TRY(launch_assembly(c, false, false));
+  if (c->prm.itau >= 10) PHG_TRY(launch_assembly(c, false, false));
   PHG_TRY(upload(c, c->bdiag, BDiag, 25 * n));
   PHG_TRY(upload(c, c->mfg.ypre, ypre, 5 * n));
   PHG_TRY(upload(c, c->rmes, rmes, 5 * n));

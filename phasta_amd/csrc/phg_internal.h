@@ -109,6 +109,7 @@ struct PhysParams {
   double Rgas, gamma, gamma1, pr, mu;
   double dtsfct, taucfct, temper;
   double almi, alfi, gami, Dtgl;
+  int itau;  // 0 Shakib's diagonal tau, 1 Franca's; 10 and 11 the matrix tau
 };
 
 // Per-node Dirichlet mask: bit f set = equation/dof f (local order

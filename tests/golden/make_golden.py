@@ -191,6 +191,26 @@ def make_mfg_mtau(N, seed):
                         au1=au1)
 
 
+def make_franca(N, seed):
+    """ElmGMRs with Franca's diagonal tau (itau = 1, e3tau.f:54-139) alone,
+    with discontinuity capturing (iDC 1) and with the flux reconstruction
+    (idiff 1), and ElmMFG's pieces with it, at a perturbed state."""
+    case = mesh.box_case(N, mach=0.5, kspace=10, jitter=0.1)
+    rng = np.random.default_rng(seed)
+    y, ac = pyref.Reference(case).predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    out = {}
+    for tag, extra in (("", {}), ("_dc", {"idc": 1}), ("_idiff", {"idiff": 1})):
+        prm = dict(case.params, itau=1, **extra)
+        res, rmes, bd, lk = pyref.Reference(case, params=prm).elmgmrs(y, ac)
+        out[f"res{tag}"], out[f"BDiag{tag}"], out[f"lhsK{tag}"] = res, bd, lk
+    res, rmes, bd = pyref.Reference(case, params=dict(case.params, itau=1, lhs=0,
+                                                      iprec=1)).elmmfg(y, ac)
+    out.update(mfg_res=res, mfg_rmes=rmes, mfg_BDiag=bd)
+    np.savez_compressed(os.path.join(HERE, f"ref_franca_box{N}.npz"), N=N, y=y, ac=ac, **out)
+
+
 MAKERS = {
     "box": lambda: [make(N, seed) for N, seed in ((2, 11), (3, 12))],
     "alpha": lambda: make_alpha(3, 13),
@@ -199,6 +219,7 @@ MAKERS = {
     "mtau": lambda: make_mtau(3, 17),
     "mfg": lambda: make_mfg(3, 19),
     "mfg_mtau": lambda: make_mfg_mtau(3, 21),
+    "franca": lambda: make_franca(3, 23),
 }
 
 if __name__ == "__main__":

@@ -17,8 +17,7 @@ Bars:
 * with the matrix tau (itau = 10) the same bars: ElmMFG's BDiag is then
   e3bdg_nd's, mis-indexed entries included (against the oracle and the
   reference fixture ref_mfg_mtau_box3.npz);
-* iDC, itau other than 0 and 10, lhs = 1 and the mixed-precision context
-  are refused;
+* iDC, itau 2 and 3, lhs = 1 and the mixed-precision context are refused;
 * a PHG_FP64_MATRIX_FREE context (no lhsK, as itrdrv.f:163-185 allocates
   for impl 2x) runs the same kernels as a PHG_FP64 one: its SolMFG results
   are bitwise equal, it holds at least the matrix's bytes less, and the
@@ -359,7 +358,7 @@ def test_resident_newton_iterations_against_oracle():
         gpu.close()
 
 
-@pytest.mark.parametrize("over", [{"idc": 1}, {"itau": 1}, {"lhs": 1}], ids=str)
+@pytest.mark.parametrize("over", [{"idc": 1}, {"itau": 2}, {"lhs": 1}], ids=str)
 def test_flags_outside_the_matrix_free_path_are_refused(state, over):
     case, y, ac = state
     gpu = solver.GpuSolver(case, params=mfg_params(case, **over))

@@ -15,7 +15,7 @@ compiled routines) and the reference fixture.
 * the matrix tau with discontinuity capturing (iDC 1, 3), whose DC metric
   giju comes from each point's streamline-rotated inverse Jacobian: against
   the oracle and the reference fixture, 1e-12, and its tangent product;
-* itau 11 is refused on the GPU.
+* itau 2 and 3 (no branch in the compressible e3tau) are refused.
 """
 import os
 
@@ -150,7 +150,7 @@ def test_mtau_mixed_against_fp64(state):
 
 def test_mtau_flags_outside_the_build_are_refused(state):
     case, y, ac = state
-    for extra in ({"itau": 11}, {"itau": 1}):
+    for extra in ({"itau": 2}, {"itau": 3}):
         gpu = solver.GpuSolver(case, params=dict(case.params, **extra))
         try:
             with pytest.raises(solver.PhgError) as ei:

@@ -262,6 +262,6 @@ def test_restart_cycles_use_the_two_sided_residual():
 
 def test_mfg_refuses_flags_it_does_not_restate():
     case, y, ac, _ = perturbed(3, 9)
-    for over in ({"idc": 1}, {"itau": 1}, {"idiff": 2}):
+    for over in ({"idc": 1}, {"itau": 2}, {"idiff": 2}):
         with pytest.raises(RuntimeError, match="rc=5"):
             pyoracle.Oracle(case, params=mfg_params(case, **over)).elmmfg(y, ac)

@@ -515,9 +515,60 @@ def test_matrix_tau_elmgmrs_equals_reference_golden():
         assert np.array_equal(bd, g[f"BDiag{tag}"]), tag
         assert np.array_equal(lk, g[f"lhsK{tag}"]), tag
     assert rel(g["res"], base[0]) > 1e-2 and rel(g["lhsK"], base[3]) > 1e-2
-    for itau in (1, 11):
+    # itau 2 and 3 have no branch in the compressible e3tau (tau left unset)
+    for itau in (2, 3):
         with pytest.raises(RuntimeError):
             pyoracle.Oracle(case, params=dict(case.params, itau=itau)).elmgmrs(y, ac)
+
+
+# ------------------------------------------------ Franca's tau (itau = 1), itau = 11
+def test_franca_tau_equals_reference_golden():
+    """ElmGMRs with Franca's diagonal tau (e3tau.f:54-139) alone, with iDC 1
+    and with idiff 1, and ElmMFG with it, bitwise against the reference's
+    compiled routines (fixture tests/golden/ref_franca_box3.npz); the tau
+    changes res and the tangent by O(1) against Shakib's."""
+    g = np.load(os.path.join(GOLD, "ref_franca_box3.npz"))
+    case = mesh.box_case(3, mach=0.5, kspace=10, jitter=0.1)
+    y, ac = np.asfortranarray(g["y"]), np.asfortranarray(g["ac"])
+    for tag, extra in (("", {}), ("_dc", {"idc": 1}), ("_idiff", {"idiff": 1})):
+        prm = dict(case.params, itau=1, **extra)
+        res, rmes, bd, lk = pyoracle.Oracle(case, params=prm).elmgmrs(y, ac)
+        assert np.array_equal(res, g[f"res{tag}"]), tag
+        assert np.array_equal(bd, g[f"BDiag{tag}"]), tag
+        assert np.array_equal(lk, g[f"lhsK{tag}"]), tag
+    res, rmes, bd = pyoracle.Oracle(case, params=dict(case.params, itau=1, lhs=0,
+                                                      iprec=1)).elmmfg(y, ac)
+    assert np.array_equal(res, g["mfg_res"]) and np.array_equal(rmes, g["mfg_rmes"])
+    assert np.array_equal(bd, g["mfg_BDiag"])
+    base = pyoracle.Oracle(case).elmgmrs(y, ac)
+    assert rel(g["res"], base[0]) > 1e-1 and rel(g["lhsK"], base[3]) > 1e-1
+
+
+def test_modal_matrix_tau_is_the_mallet_one():
+    """itau = 11 (Matrix-Modal) reaches the same e3tau_nd as 10: e3ls.f:188
+    branches on itau < 10 only and e3tau_nd reads no itau."""
+    case = mesh.box_case(4, mach=0.3, kspace=10, jitter=0.1, grade=1.1)
+    y, ac = pyoracle.Oracle(case).itr_predict_bc(case.yold)
+    a = pyoracle.Oracle(case, params=dict(case.params, itau=10)).elmgmrs(y, ac)
+    b = pyoracle.Oracle(case, params=dict(case.params, itau=11)).elmgmrs(y, ac)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+
+
+@pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("itau,extra", [(1, {}), (1, {"idc": 2}), (1, {"idiff": 3}),
+                                        (11, {}), (11, {"idc": 1})], ids=str)
+def test_tau_variants_bitwise_equal_live_reference(itau, extra):
+    case = mesh.box_case(4, mach=0.3, kspace=10, jitter=0.1, grade=1.1)
+    rng = np.random.default_rng(31)
+    y, ac = pyoracle.Oracle(case).itr_predict_bc(case.yold)
+    y = np.asfortranarray(y * (1 + 0.05 * rng.standard_normal(y.shape)))
+    ac = np.asfortranarray(ac + 1e3 * rng.standard_normal(ac.shape))
+    prm = dict(case.params, itau=itau, **extra)
+    a = pyoracle.Oracle(case, params=prm).elmgmrs(y, ac)
+    b = pyref.Reference(case, params=prm).elmgmrs(y, ac)
+    for k in (0, 2, 3):
+        assert np.array_equal(a[k], b[k]), k
 
 
 @pytest.mark.skipif(not pyref.available(), reason="oracle/_ref not built")
