@@ -217,7 +217,7 @@ def main():
     asm_tf = E3_FLOPS * nel / ((ms_pt + ms_bl) * 1e-3) / 1e12
     tdt = "f64" if head_name == "fp64" else "f32"
     kasm = (f"k_asm_qp<{'double' if tdt == 'f64' else 'float'}, false, false, false> + "
-            f"k_asm_blocks<{'double, double' if tdt == 'f64' else 'float, float'}, false, false>")
+            f"k_asm_pairs<{'double, double' if tdt == 'f64' else 'float, float'}>")
     roof_asm = {"kernel": f"{kasm} (ElmGMRs interior assembly: e3 residual + tangent, "
                           f"res/BDiag/lhsK written once)",
                 "bound": "mfma", "compute": f"VALU {tdt} (no MFMA in this kernel; the peak "

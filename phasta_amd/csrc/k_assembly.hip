@@ -1072,10 +1072,107 @@ __global__ void __launch_bounds__(256) k_jv_nodes(const long long* __restrict__ 
 }
 
 // ---------------- pass 2: block owners ----------------
+// One point's WdetJ P Q with Shakib's diagonal tau, added to ac (row + 5
+// column): P = A(sP) diag(W tau) + NaP I (NaP = WdetJ N_a), Q = A(sQ) +
+// c NbQ A0 (NbQ = N_b).  With the products of the state with W tau, P's
+// row 0 is v + N_a e_0, v = (W t0 drdp una, W t1 rho s, W t2 drdT una);
+// rows 1+k are u_k v + (W t0 s_k) e_0 + dg e_{1+k}, dg = W t1 rho una + N_a;
+// row 4 is (W t0 e2p una, W t1 e3p s + (W t1 rho una) u, W t2 e4p una +
+// N_a).  Q's rows 1+k are u_k Q0 + r_k e_0 + rho w e_{1+k} (Q0 its row 0,
+// r = sQ, w = u.r + c N_b), so with vQ = v Q
+//   (P Q)(0)   = vQ + N_a Q0
+//   (P Q)(1+k) = u_k (vQ + dg Q0) + (W t0 s_k) Q0 + dg r_k e_0 + dg rho w e_{1+k}
+// and row 4 through the same factored Q.
+template <typename R>
+__device__ __forceinline__ void shakib_pq(const R uq[3], R rho, R drdp, R drdT, R e1p, R e3p,
+                                          R e4p, R t0, R wt1, R wt2, R cfacR, const R sP[3],
+                                          const R sQ[3], R NaP, R NbQ, R ac[25]) {
+  const R u0 = uq[0], u1 = uq[1], u2 = uq[2];
+  const R e2p = e1p + (R)1;
+  const R una = u0 * sP[0] + u1 * sP[1] + u2 * sP[2];
+  const R unb = u0 * sQ[0] + u1 * sQ[1] + u2 * sQ[2];
+  const R w = unb + cfacR * NbQ;
+  const R q00 = drdp * w, q04 = drdT * w, rw = rho * w;
+  R q0[3], q4[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    q0[n] = rho * sQ[n];
+    q4[n] = e3p * sQ[n] + rw * uq[n];
+  }
+  const R q40 = e2p * unb + cfacR * NbQ * e1p, q44 = e4p * w;
+  const R rt1 = wt1 * rho, pt0 = t0 * drdp, tt2 = wt2 * drdT;
+  const R et0 = t0 * e2p, et1 = wt1 * e3p, et2 = wt2 * e4p;
+  const R a0 = pt0 * una, r1 = rt1 * una, p04 = tt2 * una, dg = r1 + NaP;
+  const R v1[3] = {rt1 * sP[0], rt1 * sP[1], rt1 * sP[2]};
+  const R Q0[5] = {q00, q0[0], q0[1], q0[2], q04};
+  const R sg = a0 + (v1[0] * u0 + v1[1] * u1 + v1[2] * u2);
+  const R rg = v1[0] * sQ[0] + v1[1] * sQ[1] + v1[2] * sQ[2];
+  R vQ[5];
+  vQ[0] = sg * q00 + rg + p04 * q40;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) vQ[1 + n] = sg * q0[n] + rw * v1[n] + p04 * q4[n];
+  vQ[4] = sg * q04 + p04 * q44;
+  // row 0
+#pragma unroll
+  for (int c = 0; c < 5; ++c) ac[5 * c] += vQ[c] + NaP * Q0[c];
+  // rows 1..3
+  R Z[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) Z[c] = vQ[c] + dg * Q0[c];
+  const R dgrw = dg * rw;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const R ts = t0 * sP[k];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      R& ck = ac[(1 + k) + 5 * c];
+      ck += uq[k] * Z[c];
+      ck += ts * Q0[c];
+    }
+    ac[1 + k] += dg * sQ[k];
+    ac[(1 + k) + 5 * (1 + k)] += dgrw;
+  }
+  // row 4
+  const R p40 = et0 * una, p44 = et2 * una + NaP;
+  R p4[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) p4[n] = et1 * sP[n] + r1 * uq[n];
+  const R s4 = p40 + (p4[0] * u0 + p4[1] * u1 + p4[2] * u2);
+  ac[4] += s4 * q00 + (p4[0] * sQ[0] + p4[1] * sQ[1] + p4[2] * sQ[2]) + p44 * q40;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) ac[4 + 5 * (1 + n)] += s4 * q0[n] + rw * p4[n] + p44 * q4[n];
+  ac[24] += s4 * q04 + p44 * q44;
+}
+
+// sum_q WdetJ K_ab of one element block (e3visc.f's stiffness through
+// e3wmlt): the momentum rows and the conduction term do not depend on the
+// point (4 equal weights), the energy row only through the velocity,
+// linearly (ub = the sum of the point velocities)
+template <typename R>
+__device__ __forceinline__ void visc_add(R W, const R sP[3], const R sQ[3], const R ub[3],
+                                         R muR, R lamR, R kapR, R ac[25]) {
+  const R W4 = (R)4 * W;
+  const R st = sP[0] * sQ[0] + sP[1] * sQ[1] + sP[2] * sQ[2];
+  const R ubs = ub[0] * sP[0] + ub[1] * sP[1] + ub[2] * sP[2];
+  const R ubr = ub[0] * sQ[0] + ub[1] * sQ[1] + ub[2] * sQ[2];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+      ac[(1 + m) + 5 * (1 + n)] +=
+          W4 * (muR * ((m == n ? st : (R)0) + sP[n] * sQ[m]) + lamR * sP[m] * sQ[n]);
+#pragma unroll
+  for (int n = 0; n < 3; ++n)
+    ac[4 + 5 * (1 + n)] += W * (muR * (st * ub[n] + sP[n] * ubr) + lamR * ubs * sQ[n]);
+  ac[24] += W4 * kapR * st;
+}
+
 struct BlkArgs {
   const int4* lanes;       // per lane: block-list offset, CSR entry, count,
                            // flags (bit 0 leader, bit 1 diagonal, bits 8..15
                            // chunks of the entry, bits 16.. total count)
+  const int* lanes_k2;     // k_asm_pairs: the transposed entry of the lane's
+                           // (off-diagonal) entry, -1 for diagonal lanes
   const int* blocks;       // e * 16 + a * 4 + b, grouped per lane
   const int* ien;          // 4 planes of nel
   const uint8_t* bcm;
@@ -1140,7 +1237,6 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
       rR[c] = (R)sel4t(g16[c], g16[3 + c], g16[6 + c], g16[9 + c], b);
     }
     const R W = (R)g16[12];
-    const R st = sR[0] * rR[0] + sR[1] * rR[1] + sR[2] * rR[2];
     // acc += WdetJ [P Q] per point: the stored tau is WdetJ tau and N_a is
     // scaled here, so P's rows come out multiplied by WdetJ
     auto add = [&](int idx, R val) { acc[idx] += val; };
@@ -1342,80 +1438,12 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
           acc[4 + 10] += s2 * ((Wt[1][4] - Wt[3][4]) * r4[2]);
         }
       } else {
-        // Shakib's diagonal tau.  With the products of the state with W tau,
-        // P's row 0 is v + N_a e_0, v = (W t0 drdp una, W t1 rho s, W t2
-        // drdT una); rows 1+k are u_k v + (W t0 s_k) e_0 + dg e_{1+k}, dg =
-        // W t1 rho una + N_a; row 4 is (W t0 e2p una, W t1 e3p s + (W t1 rho
-        // una) u, W t2 e4p una + N_a).  Q's rows 1+k are u_k Q0 + r_k e_0 +
-        // rho w e_{1+k} (Q0 its row 0), so with vQ = v Q
-        //   (P Q)(0)   = vQ + N_a Q0
-        //   (P Q)(1+k) = u_k (vQ + dg Q0) + (W t0 s_k) Q0 + dg r_k e_0
-        //                + dg rho w e_{1+k}
-        // and row 4 through the same factored Q.
-        const R t0 = (R)o[kPtT0], wt1 = (R)o[kPtT1], wt2 = (R)o[kPtT2];
-        const R rt1 = wt1 * rho, pt0 = t0 * drdp, tt2 = wt2 * drdT;
-        const R et0 = t0 * e2p, et1 = wt1 * e3p, et2 = wt2 * e4p;
-        const R a0 = pt0 * una, r1 = rt1 * una, p04 = tt2 * una, dg = r1 + Na;
-        const R v1[3] = {rt1 * sR[0], rt1 * sR[1], rt1 * sR[2]};
-        const R Q0[5] = {q00, q0[0], q0[1], q0[2], q04};
-        // vQ
-        const R sg = a0 + (v1[0] * u0 + v1[1] * u1 + v1[2] * u2);
-        const R rg = v1[0] * rR[0] + v1[1] * rR[1] + v1[2] * rR[2];
-        R vQ[5];
-        vQ[0] = sg * q00 + rg + p04 * q40;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) vQ[1 + n] = sg * q0[n] + rw * v1[n] + p04 * q4[n];
-        vQ[4] = sg * q04 + p04 * q44;
-        // row 0
-#pragma unroll
-        for (int c = 0; c < 5; ++c) acc[5 * c] += vQ[c] + Na * Q0[c];
-        // rows 1..3
-        R Z[5];
-#pragma unroll
-        for (int c = 0; c < 5; ++c) Z[c] = vQ[c] + dg * Q0[c];
-        const R dgrw = dg * rw;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const R ts = t0 * sR[k];
-#pragma unroll
-          for (int c = 0; c < 5; ++c) {
-            R& ck = acc[(1 + k) + 5 * c];
-            ck += uq[k] * Z[c];
-            ck += ts * Q0[c];
-          }
-          acc[1 + k] += dg * rR[k];
-          acc[(1 + k) + 5 * (1 + k)] += dgrw;
-        }
-        // row 4
-        const R p40 = et0 * una, p44 = et2 * una + Na;
-        R p4[3];
-#pragma unroll
-        for (int n = 0; n < 3; ++n) p4[n] = et1 * sR[n] + r1 * uq[n];
-        const R s4 = p40 + (p4[0] * u0 + p4[1] * u1 + p4[2] * u2);
-        acc[4] += s4 * q00 + (p4[0] * rR[0] + p4[1] * rR[1] + p4[2] * rR[2]) + p44 * q40;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) acc[4 + 5 * (1 + n)] += s4 * q0[n] + rw * p4[n] + p44 * q4[n];
-        acc[24] += s4 * q04 + p44 * q44;
+        shakib_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, (R)o[kPtT0], (R)o[kPtT1],
+                     (R)o[kPtT2], cfacR, sR, rR, Na, Nb, acc);
       }
     }
-    // viscous part sum_q WdetJ K_ab: the momentum rows and the conduction
-    // term do not depend on the point (4 equal weights), the energy row only
-    // through the velocity, linearly (sum over the points = ub)
-    {
-      const R W4 = (R)4 * W;
-      const R ubs = ub[0] * sR[0] + ub[1] * sR[1] + ub[2] * sR[2];
-      const R ubr = ub[0] * rR[0] + ub[1] * rR[1] + ub[2] * rR[2];
-#pragma unroll
-      for (int m = 0; m < 3; ++m)
-#pragma unroll
-        for (int n = 0; n < 3; ++n)
-          add((1 + m) + 5 * (1 + n),
-              W4 * (muR * ((m == n ? st : (R)0) + sR[n] * rR[m]) + lamR * sR[m] * rR[n]));
-#pragma unroll
-      for (int n = 0; n < 3; ++n)
-        add(4 + 5 * (1 + n), W * (muR * (st * ub[n] + sR[n] * ubr) + lamR * ubs * rR[n]));
-      add(24, W4 * kapR * st);
-    }
+    // viscous part sum_q WdetJ K_ab
+    visc_add<R>(W, sR, rR, ub, muR, lamR, kapR, acc);
     if constexpr (kDC) {
       R sab = (R)1;
       if constexpr (!kMT) {
@@ -1527,6 +1555,175 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = make_double2(sum[2 * k], sum[2 * k + 1]);
     v[12] = make_double2(sum[24], 0.0);
+  }
+}
+
+// The block pass with lhsK for Shakib's diagonal tau, without DC: the
+// entries (A,B) and (B,A) come from the same element blocks with a and b
+// exchanged, so one lane forms both from one read of the point records
+// (the pass is bound by those per-lane reads).  Diagonal entries run in
+// wavefronts of their own, so every lane of a wavefront forms the same
+// number of blocks.  Same arithmetic per block as k_asm_blocks.
+template <typename T, typename S>
+__global__ void __launch_bounds__(64) k_asm_pairs(BlkArgs A) {
+  using R = T;
+  const long long per = (gridDim.x + 7) / 8;
+  const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  const long long l = wg * 64 + threadIdx.x;
+  int4 L = make_int4(0, 0, 0, 0);
+  int k2 = -1;
+  if (l < A.nlanes) {
+    L = A.lanes[l];
+    k2 = A.lanes_k2[l];
+  }
+  const bool diag = (L.w >> 1) & 1;
+  const bool pair = k2 >= 0;
+  const int cnt = L.z;
+  R acc[25], acc2[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) acc[i] = acc2[i] = (R)0;
+  double racc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const S* qps = reinterpret_cast<const S*>(A.qps);
+  const S* geo = reinterpret_cast<const S*>(A.geo);
+  const R muR = (R)A.mu, lamR = (R)A.lam, kapR = (R)A.kap, cfacR = (R)A.cfac;
+  for (int k = 0; k < cnt; ++k) {
+    const int code = A.blocks[L.x + k];
+    const long long e = code >> 4;
+    const int a = (code >> 2) & 3, b = code & 3;
+    S g16[16];
+    load_rec<S>(geo + (size_t)e * rec_stride<S>(), g16);
+    R sR[3], rR[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sR[c] = (R)sel4t(g16[c], g16[3 + c], g16[6 + c], g16[9 + c], a);
+      rR[c] = (R)sel4t(g16[c], g16[3 + c], g16[6 + c], g16[9 + c], b);
+    }
+    const R W = (R)g16[12];
+    R ub[3] = {(R)0, (R)0, (R)0};
+    const S* qrec = qps + (size_t)e * 4 * pt_stride<S>();
+    // float: two points' record loads in flight (136 VGPRs, three
+    // wavefronts per SIMD: 8.23 ms at 128^3 against 8.51 not unrolled);
+    // double: the unrolled body exceeds the register file
+    constexpr int kPairUnroll = sizeof(S) == 4 ? 2 : 1;
+#pragma unroll kPairUnroll
+    for (int q = 0; q < 4; ++q) {
+      S o[12];
+      load_pt<S>(qrec + q * pt_stride<S>(), o);
+      const R uq[3] = {(R)o[0], (R)o[1], (R)o[2]};
+      ub[0] += uq[0];
+      ub[1] += uq[1];
+      ub[2] += uq[2];
+      const R rho = (R)o[kPtRho], drdp = (R)o[kPtDrdp], drdT = (R)o[kPtDrdT];
+      const R e1p = (R)o[kPtE1p], e3p = (R)o[kPtE3p], e4p = (R)o[kPtE4p];
+      const R t0 = (R)o[kPtT0], wt1 = (R)o[kPtT1], wt2 = (R)o[kPtT2];
+      const R Nqa = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
+      const R Nqb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
+      shakib_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, t0, wt1, wt2, cfacR, sR, rR, W * Nqa,
+                   Nqb, acc);
+      if (pair)
+        shakib_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, t0, wt1, wt2, cfacR, rR, sR,
+                     W * Nqb, Nqa, acc2);
+    }
+    visc_add<R>(W, sR, rR, ub, muR, lamR, kapR, acc);
+    if (pair) visc_add<R>(W, rR, sR, ub, muR, lamR, kapR, acc2);
+  }
+  if (diag) {
+    for (int k = 0; k < cnt; ++k) {
+      const int code = A.blocks[L.x + k];
+      const double* ro = A.resid + ((size_t)(code >> 4) * 4 + ((code >> 2) & 3)) * 5;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) racc[f] += ro[f];
+    }
+  }
+  // chunk combine as in k_asm_blocks, one block at a time (the second one's
+  // partials move only in wavefronts of pairs, wavefront-uniform: padding
+  // lanes hold zeros), each stored by the leader before the next
+  const bool leader = (L.w & 1) && l < A.nlanes;
+  const int nch = (L.w >> 8) & 0xff;
+  const int total = L.w >> 16;
+  int mx = nch, wp = pair ? 1 : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = max(mx, __shfl_xor(mx, o, 64));
+    wp = max(wp, __shfl_xor(wp, o, 64));
+  }
+  R csum[25];
+  auto combine = [&](const R* ac) {
+#pragma unroll
+    for (int i = 0; i < 25; ++i) csum[i] = ac[i];
+    for (int c = 1; c < mx; ++c) {
+#pragma unroll
+      for (int i = 0; i < 25; ++i) {
+        const R v = __shfl_down(ac[i], c, 64);
+        if (c < nch) csum[i] += v;
+      }
+    }
+  };
+  if (!wp) {
+    for (int c = 1; c < mx; ++c) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        const double v = __shfl_down(racc[f], c, 64);
+        if (c < nch) racc[f] += v;
+      }
+    }
+  }
+  int v = 0, wcol = 0;
+  uint8_t ma = 0, mb = 0;
+  if (leader) {
+    const int code0 = A.blocks[L.x];
+    const long long e0 = code0 >> 4;
+    const int a0 = (code0 >> 2) & 3, b0 = code0 & 3;
+    const long long nel = A.nel;
+    v = A.ien[a0 * nel + e0];
+    wcol = A.ien[b0 * nel + e0];
+    ma = A.bcm[v];
+    mb = A.bcm[wcol];
+  }
+  // bc3LHS and the store of one block: rows masked by mr, columns by mc
+  auto store = [&](long long entry, uint8_t mr, uint8_t mc, bool dg) {
+    double sum[25];
+#pragma unroll
+    for (int i = 0; i < 25; ++i) sum[i] = (double)csum[i];
+    if (dg && A.prec_on) {
+#pragma unroll
+      for (int i = 0; i < 25; ++i) A.bdiag[25 * (size_t)v + i] = sum[i];
+    }
+    if (mr | mc) {
+#pragma unroll
+      for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+          if (((mr >> i) & 1) || ((mc >> j) & 1)) sum[i + 5 * j] = 0.0;
+          if (dg && i == j && ((mr >> i) & 1)) sum[i + 5 * j] = (double)total;
+        }
+    }
+    T* dst = reinterpret_cast<T*>(A.lhs) + (size_t)aos_stride<T>() * entry;
+    if constexpr (sizeof(T) == 4) {
+      float4* d4 = reinterpret_cast<float4*>(dst);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        d4[k] = make_float4((float)sum[4 * k], (float)sum[4 * k + 1], (float)sum[4 * k + 2],
+                            (float)sum[4 * k + 3]);
+      d4[6] = make_float4((float)sum[24], 0.f, 0.f, 0.f);
+    } else {
+      double2* d2 = reinterpret_cast<double2*>(dst);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) d2[k] = make_double2(sum[2 * k], sum[2 * k + 1]);
+      d2[12] = make_double2(sum[24], 0.0);
+    }
+  };
+  combine(acc);
+  if (leader) {
+    if (diag) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) A.res[5 * (size_t)v + f] = racc[f];
+    }
+    store(L.y, ma, mb, diag);
+  }
+  if (wp) {
+    combine(acc2);
+    if (leader && pair) store(k2, mb, ma, false);
   }
 }
 
@@ -2361,8 +2558,11 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   // pass 2: block owners (res, BDiag, lhsK written once)
   BlkArgs B{};
   // without lhsK only the diagonal entries' owners have work (the node's
-  // residual and BDiag): run their lanes alone
-  B.lanes = lhs ? c->lanes : c->lanes_diag;
+  // residual and BDiag): run their lanes alone; with lhsK, Shakib's tau and
+  // no DC the paired lanes of k_asm_pairs
+  const bool paired = lhs && !dc && !mt;
+  B.lanes = paired ? c->lanes_pair : lhs ? c->lanes : c->lanes_diag;
+  B.lanes_k2 = c->lanes_k2;
   B.blocks = c->blocks;
   B.ien = c->ienc;
   B.bcm = c->bcm;
@@ -2375,7 +2575,7 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.bdiag = c->acc_bd;
   B.lhs = mixed ? (void*)c->lhs_aos32 : (void*)c->lhs_aos;
   B.nel = c->nel;
-  B.nlanes = lhs ? c->nlanes : c->nlanes_diag;
+  B.nlanes = paired ? c->nlanes_pair : lhs ? c->nlanes : c->nlanes_diag;
   B.lhs_on = lhs;
   B.prec_on = prec;
   const double cp = ph.Rgas * ph.gamma / ph.gamma1;
@@ -2384,7 +2584,11 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   const int g2 = (int)(((B.nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
-  if (mixed && mt && dc)
+  if (paired && mixed)
+    k_asm_pairs<float, float><<<g2, 64, 0, c->stream>>>(B);
+  else if (paired)
+    k_asm_pairs<double, double><<<g2, 64, 0, c->stream>>>(B);
+  else if (mixed && mt && dc)
     k_asm_blocks<float, float, true, true><<<g2, 64, 0, c->stream>>>(B);
   else if (mixed && mt)
     k_asm_blocks<float, float, false, true><<<g2, 64, 0, c->stream>>>(B);

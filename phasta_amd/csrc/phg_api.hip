@@ -281,6 +281,30 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   // (residual and BDiag only) runs these lanes
   std::vector<int4> lanes_diag;
   lanes_diag.reserve(4 * nshg);
+  // the pass of k_asm_pairs: each off-diagonal pair (A,B), (B,A) once, on
+  // the lanes of the entry whose row comes first in the visiting order,
+  // with the transposed entry beside.  Pair lanes and diagonal lanes fill
+  // 64-lane wavefronts of their own (pw, dw), each appended to the list
+  // when full, so both kinds follow the row order together and share the
+  // point states in L2
+  std::vector<int4> lanes_pair, pw, dw;
+  std::vector<int> lanes_k2, pk;
+  lanes_pair.reserve(nnz / 2 + 4 * nshg);
+  lanes_k2.reserve(nnz / 2 + 4 * nshg);
+  auto flush = [&](std::vector<int4>& wv, std::vector<int>* k2v) {
+    if (wv.empty()) return;
+    while (wv.size() % 64) {
+      wv.push_back(make_int4(0, 0, 0, 0));
+      if (k2v) k2v->push_back(-1);
+    }
+    lanes_pair.insert(lanes_pair.end(), wv.begin(), wv.end());
+    if (k2v)
+      lanes_k2.insert(lanes_k2.end(), k2v->begin(), k2v->end());
+    else
+      lanes_k2.resize(lanes_pair.size(), -1);
+    wv.clear();
+    if (k2v) k2v->clear();
+  };
   std::vector<int> blk_lane;  // blocks in lane order
   blk_lane.reserve(16 * nel);
   // rows in Morton order of their node coordinates, so the lanes that run
@@ -311,6 +335,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
     for (long long r = 0; r < nshg; ++r) rows[r] = (int)r;
     std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return key[a] < key[b]; });
   }
+  std::vector<int> rank(nshg);
+  for (long long ri = 0; ri < nshg; ++ri) rank[rows[ri]] = (int)ri;
   for (long long ri = 0; ri < nshg; ++ri) {
     const long long row = rows[ri];
     for (long long k = m->colm[row] - 1; k < m->colm[row + 1] - 1; ++k) {
@@ -329,6 +355,21 @@ int setup(phg_ctx* c, const phg_mesh* m) {
       const bool dg = (m->rowp[k] - 1) == row;
       if (dg && (long long)(lanes_diag.size() % 64) + nch > 64)
         while (lanes_diag.size() % 64) lanes_diag.push_back(make_int4(0, 0, 0, 0));
+      const int col = m->rowp[k] - 1;
+      const bool first = !dg && rank[col] > rank[row];
+      int kt = -1;
+      if (first) {
+        const int* lst = m->rowp + (m->colm[col] - 1);
+        const int len = m->colm[col + 1] - m->colm[col];
+        const int* it = std::lower_bound(lst, lst + len, (int)row + 1);
+        if (it == lst + len || *it != (int)row + 1) {
+          phg_set_error("colm/rowp is not structurally symmetric");
+          return PHG_EINVAL;
+        }
+        kt = (m->colm[col] - 1) + (int)(it - lst);
+        if ((long long)pw.size() + nch > 64) flush(pw, &pk);
+      }
+      if (dg && (long long)dw.size() + nch > 64) flush(dw, nullptr);
       for (int ch = 0; ch < nch; ++ch) {
         const int b0 = (int)((long long)n * ch / nch), b1 = (int)((long long)n * (ch + 1) / nch);
         int flags = (dg ? 2 : 0) | (ch == 0 ? 1 : 0);
@@ -336,12 +377,20 @@ int setup(phg_ctx* c, const phg_mesh* m) {
         const int4 ln = make_int4((int)blk_lane.size(), (int)k, b1 - b0, flags);
         lanes.push_back(ln);
         if (dg) lanes_diag.push_back(ln);
+        if (first) {
+          pw.push_back(ln);
+          pk.push_back(kt);
+        }
+        if (dg) dw.push_back(ln);
         for (int b = b0; b < b1; ++b) blk_lane.push_back(blocks[bcount[k] + b]);
       }
     }
   }
   c->nlanes = (long long)lanes.size();
   c->nlanes_diag = (long long)lanes_diag.size();
+  flush(pw, &pk);
+  flush(dw, nullptr);
+  c->nlanes_pair = (long long)lanes_pair.size();
   blocks.clear();
   blocks.shrink_to_fit();
   // ---- node -> (element, local node) incidence, element order per node
@@ -411,6 +460,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   // alone for SolMFG)
   const bool mat = !c->matrix_free;
   if (mat) PHG_TRY(dev_alloc(c, &c->lanes, c->nlanes));
+  if (mat) PHG_TRY(dev_alloc(c, &c->lanes_pair, c->nlanes_pair));
+  if (mat) PHG_TRY(dev_alloc(c, &c->lanes_k2, c->nlanes_pair));
   PHG_TRY(dev_alloc(c, &c->lanes_diag, c->nlanes_diag));
   PHG_TRY(dev_alloc(c, &c->blocks, 16 * nel));
   // point records: 12 values per point; geometry: 13 values padded to 16
@@ -473,6 +524,8 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->inc_goff, goff.data(), goff.size()));
   PHG_TRY(upload(c, c->inc, incs.data(), incs.size()));
   if (mat) PHG_TRY(upload(c, c->lanes, lanes.data(), lanes.size()));
+  if (mat) PHG_TRY(upload(c, c->lanes_pair, lanes_pair.data(), lanes_pair.size()));
+  if (mat) PHG_TRY(upload(c, c->lanes_k2, lanes_k2.data(), lanes_k2.size()));
   PHG_TRY(upload(c, c->lanes_diag, lanes_diag.data(), lanes_diag.size()));
   PHG_TRY(upload(c, c->blocks, blk_lane.data(), blk_lane.size()));
   PHG_TRY(upload(c, c->ienbc, ienbc.data(), ienbc.size()));
@@ -1209,7 +1262,8 @@ extern "C" int phg_finalize(phg_ctx* c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   mp_free(c);
-  void* ptrs[] = {c->x,        c->ienc,     c->lanes,  c->lanes_diag, c->blocks, c->qps, c->qps32, c->geo,
+  void* ptrs[] = {c->x,        c->ienc,     c->lanes,  c->lanes_diag, c->lanes_pair,
+                  c->lanes_k2, c->blocks, c->qps, c->qps32, c->geo,
                   c->geo32,    c->resid,    c->tslot,    c->ienbc,  c->bcm,       c->iper,
                   c->BCv,      c->fold_master, c->fold_ptr, c->fold_slave, c->grp_slot,
                   c->scol,     c->scolp,    c->csr2slot, c->lhs64, c->lhs32,     c->res,

@@ -199,6 +199,9 @@ struct phg_ctx {
   int4* lanes = nullptr;        // block-list offset, CSR entry, count, flags
   long long nlanes_diag = 0;
   int4* lanes_diag = nullptr;   // the lanes of the diagonal entries alone
+  long long nlanes_pair = 0;
+  int4* lanes_pair = nullptr;   // k_asm_pairs: pair wavefronts, then diagonal ones
+  int* lanes_k2 = nullptr;      // per lane of lanes_pair: transposed entry or -1
   int* blocks = nullptr;        // e*16 + a*4 + b, grouped per owner lane
   void* scratch = nullptr;      // this device's address of the library's
                                 // static staging array (phg_spsi3pre_noalloc)
