@@ -872,7 +872,9 @@ struct JvArgs {
   const double* v;         // 5 planes, local order
   const int* iper;         // 0-based master
   const uint8_t* bcm;
-  double* out;             // (nel*4) x 5: element e's product at its node a
+  double* out;             // element e's product at its node a, 5 doubles at
+                           // slot eslot[4 e + a] of the node incidence layout
+  const int* eslot;
   const double* dcw;       // nel*4 DC factor per point of the last assembly (kDC)
   long long nel, nshg;
   PhysParams ph;
@@ -1027,14 +1029,15 @@ __global__ void __launch_bounds__(256) k_jv(JvArgs A) {
   if (valid) {
     const int nq = sel4i(nd[0], nd[1], nd[2], nd[3], q);
     const uint8_t rm = A.bcm[nq];
-    double* o = A.out + ((size_t)e * 4 + q) * 5;
+    double* o = A.out + (size_t)A.eslot[e * 4 + q] * 5;
 #pragma unroll
     for (int f = 0; f < 5; ++f) o[f] = ((rm >> f) & 1) ? 0.0 : mine[f];
   }
 }
 
 // w_n = sum of the node's element products in element order (the node's
-// incidence list, sliced by 64-node groups); a constrained dof f of node n
+// incidence list, sliced by 64-node groups; k_jv writes each product to its
+// slot there, so a wavefront reads whole rows); a constrained dof f of node n
 // takes the unit diagonals of its element blocks, cnt(n) v(iper(n))_f
 // (bc3lhs.f:55-235: one per element containing the node).
 __global__ void __launch_bounds__(256) k_jv_nodes(const long long* __restrict__ goff,
@@ -1053,10 +1056,10 @@ __global__ void __launch_bounds__(256) k_jv_nodes(const long long* __restrict__ 
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int cnt = 0;
   for (int j = 0; j < deg; ++j) {
-    const int code = inc[o0 + 64LL * j + lane];
-    if (code < 0) continue;
+    const long long slot = o0 + 64LL * j + lane;
+    if (inc[slot] < 0) continue;
     ++cnt;
-    const double* r = out + (size_t)code * 5;
+    const double* r = out + slot * 5;
 #pragma unroll
     for (int f = 0; f < 5; ++f) acc[f] += r[f];
   }
@@ -2154,7 +2157,8 @@ struct MresArgs {
   const int* ien;       // 4 planes of nel
   const double* node;   // 16 per node: y (local order), ac, x
   const double* yp;     // variables in the local order, yp_stride doubles per node
-  double* resid;        // (nel*4) x 5
+  double* resid;        // 5 per (e, a) at slot eslot[4 e + a]
+  const int* eslot;
   long long nel;
   PhysParams ph;
   double fct1;          // almi/gami/alfi*Dtgl
@@ -2286,14 +2290,15 @@ __global__ void __launch_bounds__(256) k_mres(MresArgs A) {
     }
   }
   if (valid) {
-    double* ro = A.resid + ((size_t)e * 4 + q) * 5;
+    double* ro = A.resid + (size_t)A.eslot[e * 4 + q] * 5;
 #pragma unroll
     for (int f = 0; f < 5; ++f) ro[f] = A.iabs ? fabs(mine[f]) : mine[f];
   }
 }
 
 // out (+)= the sum of the node's element contributions in element order
-// (the scatter of asires.f:91 / asimfg.f:96), then bc3Res's Dirichlet rows
+// (the scatter of asires.f:91 / asimfg.f:96; read from the incidence slots
+// k_mres writes), then bc3Res's Dirichlet rows
 // (bc3res.f:25-153); the periodic fold follows (k_fold_res_bdiag)
 __global__ void __launch_bounds__(256) k_mres_nodes(const long long* __restrict__ goff,
                                                     const int* __restrict__ inc,
@@ -2311,9 +2316,9 @@ __global__ void __launch_bounds__(256) k_mres_nodes(const long long* __restrict_
 #pragma unroll
   for (int f = 0; f < 5; ++f) acc[f] = accumulate ? out[n + f * nshg] : 0.0;
   for (int j = 0; j < deg; ++j) {
-    const int code = inc[o0 + 64LL * j + lane];
-    if (code < 0) continue;
-    const double* r = resid + (size_t)code * 5;
+    const long long slot = o0 + 64LL * j + lane;
+    if (inc[slot] < 0) continue;
+    const double* r = resid + slot * 5;
 #pragma unroll
     for (int f = 0; f < 5; ++f) acc[f] += r[f];
   }
@@ -2637,6 +2642,7 @@ int launch_jv(phg_ctx* c, const double* v, double* w) {
   J.iper = c->iper;
   J.bcm = c->bcm;
   J.out = c->resid;
+  J.eslot = c->eslot;
   J.dcw = c->dcw;
   J.epsM = c->prm.epsM;
   J.nel = c->nel;
@@ -2693,6 +2699,7 @@ int launch_mres(phg_ctx* c, const double* yp, double* out, bool accumulate, bool
   M.yp = yp ? yp : c->nodepack;
   M.yp_stride = yp ? 8 : 16;
   M.resid = c->resid;
+  M.eslot = c->eslot;
   M.nel = c->nel;
   M.ph = ph;
   M.fct1 = ph.almi / ph.gami / ph.alfi * ph.Dtgl;

@@ -420,6 +420,16 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   }
   incl.clear();
   incl.shrink_to_fit();
+  // the slot of each (element, node) in that layout: the per-element
+  // products of the tangent and of the modified residual are written there,
+  // so the node sums read whole rows
+  if (goff[ngq] >= (1LL << 31)) {
+    phg_set_error("too many node incidences for 32-bit slots");
+    return PHG_EINVAL;
+  }
+  std::vector<int> eslot(4 * nel, -1);
+  for (long long p = 0; p < goff[ngq]; ++p)
+    if (incs[p] >= 0) eslot[incs[p]] = (int)p;
   std::vector<int> ienbc(4 * std::max<long long>(nelb, 1));
   for (long long p = 0; p < nelb; ++p)
     for (int a = 0; a < 4; ++a) ienbc[a * nelb + p] = m->ienb[orderb[p] + nelb * a] - 1;
@@ -473,7 +483,10 @@ int setup(phg_ctx* c, const phg_mesh* m) {
     PHG_TRY(dev_alloc(c, &c->qps, (size_t)nel * 4 * 12));
     PHG_TRY(dev_alloc(c, &c->geo, (size_t)nel * 14));
   }
-  PHG_TRY(dev_alloc(c, &c->resid, (size_t)nel * 4 * 5));
+  // element residuals (element order), then the element products in the
+  // incidence slots (k_jv, k_mres)
+  PHG_TRY(dev_alloc(c, &c->resid, (size_t)std::max<long long>(nel * 4, goff[ngq]) * 5));
+  PHG_TRY(dev_alloc(c, &c->eslot, (size_t)nel * 4));
   PHG_TRY(dev_alloc(c, &c->ienbc, ienbc.size()));
   PHG_TRY(dev_alloc(c, &c->bcm, nshg));
   PHG_TRY(dev_alloc(c, &c->iper, nshg));
@@ -523,6 +536,7 @@ int setup(phg_ctx* c, const phg_mesh* m) {
   PHG_TRY(upload(c, c->ienc, ienc.data(), ienc.size()));
   PHG_TRY(upload(c, c->inc_goff, goff.data(), goff.size()));
   PHG_TRY(upload(c, c->inc, incs.data(), incs.size()));
+  PHG_TRY(upload(c, c->eslot, eslot.data(), eslot.size()));
   if (mat) PHG_TRY(upload(c, c->lanes, lanes.data(), lanes.size()));
   if (mat) PHG_TRY(upload(c, c->lanes_pair, lanes_pair.data(), lanes_pair.size()));
   if (mat) PHG_TRY(upload(c, c->lanes_k2, lanes_k2.data(), lanes_k2.size()));
@@ -1270,7 +1284,7 @@ extern "C" int phg_finalize(phg_ctx* c) {
                   c->rmes,     c->bdiag,    c->bdiag32, c->y,      c->ac,        c->yold,
                   c->acold,    c->dy,       c->krylov, c->red_part, c->scal,
                   c->tmp,      c->slot2csr, c->lhs_aos, c->lhs_aos32, c->acc_res, c->acc_bd,
-                  c->nodepack, c->inc_goff, c->inc,   c->qrec,    c->qnode,
+                  c->nodepack, c->inc_goff, c->inc,   c->eslot,   c->qrec,    c->qnode,
                   c->dcw,      c->mtau,     c->mfg.ypre, c->mfg.t1, c->mfg.t2,  c->mfg.pack};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -249,6 +249,7 @@ struct phg_ctx {
   long long* inc_goff = nullptr;   // per 64-node group: offset of its sliced entries
   int* inc = nullptr;              // e*4 + a per (node, incident element), sliced by
                                    // 64-node groups, element order per node, -1 padding
+  int* eslot = nullptr;            // (nel*4): slot of (e, a) in the incidence layout
   double* qrec = nullptr;          // nel x 16 element records (allocated on first use)
   double* qnode = nullptr;         // 16 per node: q (12), 1/lumped mass
 
