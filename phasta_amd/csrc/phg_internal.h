@@ -102,8 +102,12 @@ __device__ inline void store_slot(double* blk, int r, const A a[25]) {
 }
 
 // Largest number of element blocks one assembly lane sums (the blocks of
-// one CSR entry are split into chunks of at most this size).
-constexpr int kChunk = 6;
+// one CSR entry are split into chunks of at most this size).  Measured at
+// 128^3 (float / FP64 block pass): 2: 7.90 / 12.74 ms, 3: 7.81 / 12.41,
+// 4: 8.07 / 12.77, 6: 8.28 / 12.92, 9: 8.51 / 13.23; shorter chunks give
+// more lanes and even loop trips for the 24 blocks of a diagonal entry and
+// the 4 to 6 of an off-diagonal one.
+constexpr int kChunk = 3;
 // Physical constants and time-integration factors used in the kernels.
 struct PhysParams {
   double Rgas, gamma, gamma1, pr, mu;
