@@ -7,7 +7,7 @@
 //
 //  k_asm_qp      four lanes per element, lane q = quadrature point q: the
 //                element metrics, the interpolated state, thermodynamics and
-//                tau at q (13 values, kept for the tangent pass) and q's
+//                tau at q (12 values, kept for the tangent pass) and q's
 //                residual contribution to the element's four nodes, summed
 //                over q across the four lanes and written per (element,
 //                node).
@@ -19,6 +19,9 @@
 //                order, Dirichlet rows/columns are applied once per block,
 //                and the block is stored.  The owner of a diagonal block
 //                also sums the node's residual and its BDiag.
+//  k_asm_pairs   the same with one lane forming both blocks (A,B) and (B,A)
+//                of an off-diagonal pair from one read of the element
+//                records (Shakib's or Franca's tau without DC).
 //
 // No read-modify-write of the matrix, no atomics, no element colouring, and
 // the summation order is fixed, so the result is deterministic.
@@ -116,8 +119,8 @@ struct QpArgs {
   const int* ien;          // 4 planes of nel (0-based)
   const double* node;      // 16 per node: y (p,u,v,w,T), ac (same order), x
   const double* qnode;     // 16 per node: reconstructed diffusive flux (12)
-  void* qps;               // (nel*4) x 13 point states (S)
-  void* geo;               // nel x 13 (S)
+  void* qps;               // (nel*4) x 12 point states (S)
+  void* geo;               // nel x 13 (S), padded to 16 floats / 14 doubles
   double* resid;           // (nel*4) x 5 residual of element e at its node a
   double* dcw;             // nel*4 discontinuity-capturing factor per point (iDC != 0)
   void* mtau;              // nel*4 x mt_stride W PTau per point (itau = 10), S

@@ -213,7 +213,9 @@ struct phg_ctx {
   float* qps32 = nullptr;       // (nel*4) x 12, mixed path
   double* geo = nullptr;        // nel x 14: shape gradients, WdetJ (+ pad)
   float* geo32 = nullptr;       // nel x 16
-  double* resid = nullptr;      // (nel*4) x 5 element residual per node
+  double* resid = nullptr;      // (nel*4) x 5 element residual per node (element
+                                // order), then the element products of k_jv and
+                                // k_mres in the incidence slots
   double* dcw = nullptr;        // DC per point (iDC != 0): nel*4 factors, or nel*4 x 8
                                 // DC giju with the matrix tau
   size_t dcw_len = 0;           // doubles allocated in dcw
