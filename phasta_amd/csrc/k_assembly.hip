@@ -2151,15 +2151,20 @@ bool g_quad_ready = false;
 // e3wmlt.f:97-122): no Euler flux and no acceleration.  The four lanes of
 // an element sum their contributions as the residual pass does.  iabs
 // takes |.| of each element-node value (asires.f:90, itrFDI's estimate of
-// the cancellation error).  With the matrix tau (itau = 10) tau M is
-// PTau M (e3tau.f:723-732), PTau the point's matrix tau at the state, read
-// from the W PTau records of the last point pass at that state (it depends
-// on the state alone: e3ls.f:198-214 forms it from the state's point values,
-// acceleration and metrics).
+// the cancellation error).  Everything that comes from the state is read
+// from the records the point pass stored at that state (the point values,
+// thermodynamics and W tau, the shape gradients and WdetJ): the solver's
+// products all run at one state, so only yp's values and gradients are
+// formed here.  With the matrix tau (itau >= 10) tau M is PTau M
+// (e3tau.f:723-732), PTau the point's matrix tau at the state, from the
+// W PTau records of the same point pass (it depends on the state alone:
+// e3ls.f:198-214 forms it from the state's point values, acceleration and
+// metrics).
 struct MresArgs {
   const int* ien;       // 4 planes of nel
   const double* node;   // 16 per node: y (local order), ac, x
   const double* yp;     // variables in the local order, yp_stride doubles per node
+                        // (the node lines themselves for yp = y)
   double* resid;        // 5 per (e, a) at slot eslot[4 e + a]
   const int* eslot;
   long long nel;
@@ -2168,6 +2173,8 @@ struct MresArgs {
   int yp_stride;        // 16: yp is y itself (ElmMFG), 8: a packed perturbed state
   int iabs;
   const double* mtau;   // itau = 10: nel*4 x mt_stride<double>() W PTau; else null
+  const double* qps;    // the point pass's FP64 point records at the state
+  const double* geo;    // and its geometry records
 };
 
 __global__ void __launch_bounds__(256) k_mres(MresArgs A) {
@@ -2180,48 +2187,52 @@ __global__ void __launch_bounds__(256) k_mres(MresArgs A) {
   int nd[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) nd[k] = A.ien[k * nel + e];
-  double xl[4][3];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const double2* v = reinterpret_cast<const double2*>(A.node + 16 * (size_t)nd[k]);
-    const double2 x01 = v[5], x2 = v[6];
-    xl[k][0] = x01.x;
-    xl[k][1] = x01.y;
-    xl[k][2] = x2.x;
-  }
-  double ix[3][3], det;
-  tet_inverse(xl, ix, det);
-  const double WdetJ = cQ.Qwt * det;
+  // the state's point values, thermodynamics and W tau, and the element's
+  // shape gradients and WdetJ, as the point pass stored them at this state
+  double rec[12], g16[16];
+  load_pt<double>(A.qps + ((size_t)e * 4 + q) * pt_stride<double>(), rec);
+  load_rec<double>(A.geo + (size_t)e * rec_stride<double>(), g16);
+  const double WdetJ = g16[12];
   double shg[4][3];
-  tet_shg(ix, shg);
-  double G[6], trG, GG;
-  tet_metric_tensor(ix, G, trG, GG);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) shg[k][c] = g16[3 * k + c];
+  Thermo t;
+  t.u[0] = rec[0];
+  t.u[1] = rec[1];
+  t.u[2] = rec[2];
+  t.rho = rec[kPtRho];
+  t.drdp = rec[kPtDrdp];
+  t.drdT = rec[kPtDrdT];
+  t.e1p = rec[kPtE1p];
+  t.e2p = t.e1p + 1.0;
+  t.e3p = rec[kPtE3p];
+  t.e4p = rec[kPtE4p];
+  t.tau[0] = rec[kPtT0] / WdetJ;
+  t.tau[1] = rec[kPtT1] / WdetJ;
+  t.tau[2] = rec[kPtT2] / WdetJ;
   double Nq[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) Nq[k] = sel4(cQ.N[0][k], cQ.N[1][k], cQ.N[2][k], cQ.N[3][k], q);
-  double Y[5], Yp[5], g[3][5];
+  double Yp[5], g[3][5];
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
-    Y[f] = 0.0;
     Yp[f] = 0.0;
     g[0][f] = g[1][f] = g[2][f] = 0.0;
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const double2* vy = reinterpret_cast<const double2*>(A.node + 16 * (size_t)nd[k]);
     const double2* vp = reinterpret_cast<const double2*>(A.yp + (size_t)A.yp_stride * nd[k]);
-    double y6[6], p6[6];
+    double p6[6];
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      const double2 a = vy[t], b = vp[t];
-      y6[2 * t] = a.x;
-      y6[2 * t + 1] = a.y;
-      p6[2 * t] = b.x;
-      p6[2 * t + 1] = b.y;
+    for (int c = 0; c < 3; ++c) {
+      const double2 b = vp[c];
+      p6[2 * c] = b.x;
+      p6[2 * c + 1] = b.y;
     }
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
-      Y[f] += Nq[k] * y6[f];
       Yp[f] += Nq[k] * p6[f];
 #pragma unroll
       for (int i = 0; i < 3; ++i) g[i][f] += shg[k][i] * p6[f];
@@ -2229,9 +2240,6 @@ __global__ void __launch_bounds__(256) k_mres(MresArgs A) {
   }
   const double cp = ph.Rgas * ph.gamma / ph.gamma1;
   const double mu = ph.mu, lam = -(2.0 / 3.0) * mu, kap = mu * cp / ph.pr;
-  Thermo t;
-  double ei, rk;
-  point_thermo(Y, ph, G, trG, GG, t, ei, rk);
   // L = A_i yp_,i and M = L + c U(yp)
   double M[5], tmp[5];
   Ai_times(t, 0, g[0], M);
@@ -2708,6 +2716,12 @@ int launch_mres(phg_ctx* c, const double* yp, double* out, bool accumulate, bool
   M.fct1 = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   M.iabs = iabs ? 1 : 0;
   M.mtau = c->prm.itau >= 10 ? reinterpret_cast<const double*>(c->mtau) : nullptr;
+  if (!c->qps || !c->geo) {
+    phg_set_error("modified residual: the FP64 point records are not allocated");
+    return PHG_EINVAL;
+  }
+  M.qps = c->qps;
+  M.geo = c->geo;
   if (c->prm.itau >= 10 && (!c->mtau || c->precision != PHG_FP64)) {
     phg_set_error("modified residual: the matrix tau needs the FP64 W PTau records");
     return PHG_EINVAL;

@@ -1462,8 +1462,9 @@ extern "C" int phg_itrres(phg_ctx* c, const double* yp, const double* yc, const 
   PHG_TRY(upload(c, c->y, yc, 5 * n));
   PHG_TRY(upload(c, c->ac, ac, 5 * n));
   PHG_TRY(launch_pack_nodes(c));
-  // the matrix tau at the state: the point pass's W PTau records
-  if (c->prm.itau >= 10) PHG_TRY(launch_assembly(c, false, false));
+  // the state's point records (and W PTau with the matrix tau), which the
+  // modified residual reads
+  PHG_TRY(launch_assembly(c, false, false));
   PHG_TRY(upload(c, c->mfg.t2, yp, 5 * n));
   PHG_TRY(launch_yshuffle(c, c->mfg.t2, c->mfg.t2, true));
   PHG_TRY(launch_mfg_point(c, c->mfg.t2, nullptr, 0.0, c->mfg.pack, false, false));
@@ -1486,7 +1487,7 @@ extern "C" int phg_au1mfg(phg_ctx* c, const double* ypre, const double* y, const
   PHG_TRY(upload(c, c->y, y, 5 * n));
   PHG_TRY(upload(c, c->ac, ac, 5 * n));
   PHG_TRY(launch_pack_nodes(c));
-  if (c->prm.itau >= 10) PHG_TRY(launch_assembly(c, false, false));
+  PHG_TRY(launch_assembly(c, false, false));  // the state's point records
   PHG_TRY(upload(c, c->bdiag, BDiag, 25 * n));
   PHG_TRY(upload(c, c->mfg.ypre, ypre, 5 * n));
   PHG_TRY(upload(c, c->rmes, rmes, 5 * n));
