@@ -1173,6 +1173,112 @@ __device__ __forceinline__ void visc_add(R W, const R sP[3], const R sQ[3], cons
   ac[24] += W4 * kapR * st;
 }
 
+// One point's WdetJ P Q with the matrix tau, added to ac (row + 5
+// column): P = A(sP) (W PTau) + NaP I, Q = A(sQ) + c NbQ A0 in the factored
+// form of shakib_pq (Q's rows 0 and 4 and rho w; with pu = sum_k P(m,k) u_k
+// and Sm = P(m,0) + pu, row m of P Q is
+//   column 0:    Sm q00 + sum_k P(m,k) r_k + P(m,4) q40
+//   column 1+n:  Sm q0n + rho w P(m,1+n) + P(m,4) q4n
+//   column 4:    Sm q04 + P(m,4) q44).
+template <typename R>
+__device__ __forceinline__ void mtau_pq(const R uq[3], R rho, R drdp, R drdT, R e1p, R e3p,
+                                        R e4p, const R WPT[25], R cfacR, const R sP[3],
+                                        const R sQ[3], R NaP, R NbQ, R ac[25]) {
+  const R u0 = uq[0], u1 = uq[1], u2 = uq[2];
+  const R e2p = e1p + (R)1;
+  const R una = u0 * sP[0] + u1 * sP[1] + u2 * sP[2];
+  const R unb = u0 * sQ[0] + u1 * sQ[1] + u2 * sQ[2];
+  const R w = unb + cfacR * NbQ;
+  const R q00 = drdp * w, q04 = drdT * w, rw = rho * w;
+  R q0[3], q4[3];
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    q0[n] = rho * sQ[n];
+    q4[n] = e3p * sQ[n] + rw * uq[n];
+  }
+  const R q40 = e2p * unb + cfacR * NbQ * e1p, q44 = e4p * w;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    // row m of A(sP), then P(m, :) = A(sP)(m, :) W PTau + N_a e_m
+    R Am[5];
+    if (m == 0) {
+      Am[0] = drdp * una;
+#pragma unroll
+      for (int n = 0; n < 3; ++n) Am[1 + n] = rho * sP[n];
+      Am[4] = drdT * una;
+    } else if (m < 4) {
+      const int mm = m - 1;
+      Am[0] = drdp * uq[mm] * una + sP[mm];
+#pragma unroll
+      for (int n = 0; n < 3; ++n) Am[1 + n] = rho * (uq[mm] * sP[n] + (mm == n ? una : (R)0));
+      Am[4] = drdT * uq[mm] * una;
+    } else {
+      Am[0] = e2p * una;
+#pragma unroll
+      for (int n = 0; n < 3; ++n) Am[1 + n] = e3p * sP[n] + rho * uq[n] * una;
+      Am[4] = e4p * una;
+    }
+    R Pr[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      Pr[j] = Am[0] * WPT[j] + Am[1] * WPT[5 + j] + Am[2] * WPT[10 + j] + Am[3] * WPT[15 + j] +
+              Am[4] * WPT[20 + j] + (m == j ? NaP : (R)0);
+    const R Sm = Pr[0] + (Pr[1] * u0 + Pr[2] * u1 + Pr[3] * u2);
+    R& c0 = ac[m];
+    c0 += Sm * q00;
+    c0 += Pr[1] * sQ[0];
+    c0 += Pr[2] * sQ[1];
+    c0 += Pr[3] * sQ[2];
+    c0 += Pr[4] * q40;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      R& cn = ac[m + 5 * (1 + n)];
+      cn += Sm * q0[n];
+      cn += rw * Pr[1 + n];
+      cn += Pr[4] * q4[n];
+    }
+    R& c4 = ac[m + 20];
+    c4 += Sm * q04;
+    c4 += Pr[4] * q44;
+  }
+}
+
+// Discontinuity capturing in the tangent (e3dc.f:410-446 through e3wmlt):
+// per point, sum_q dq A0(q) through A0's distinct entries (dA: drdp, drdT,
+// drdp u (3), drdT u (3), rho, rho u (3), e1p, e4p), dq the point's DC
+// weight; per element block, c (sum_q dq A0(q)) with c = WdetJ s_a.giju s_b
+// (or WdetJ when the weight already holds the metric).
+template <typename R>
+__device__ __forceinline__ void dc_point(R dq, R drdp, R drdT, R rho, const R uq[3], R e1p,
+                                         R e4p, R dA[14]) {
+  const R dp = dq * drdp, dT = dq * drdT, dr = dq * rho;
+  dA[0] += dp;
+  dA[1] += dT;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    dA[2 + n] += dp * uq[n];
+    dA[5 + n] += dT * uq[n];
+    dA[9 + n] += dr * uq[n];
+  }
+  dA[8] += dr;
+  dA[12] += dq * e1p;
+  dA[13] += dq * e4p;
+}
+template <typename R>
+__device__ __forceinline__ void dc_block(R c, const R dA[14], R ac[25]) {
+  ac[0] += c * dA[0];
+  ac[20] += c * dA[1];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    ac[1 + m] += c * dA[2 + m];
+    ac[(1 + m) + 5 * (1 + m)] += c * dA[8];
+    ac[(1 + m) + 20] += c * dA[5 + m];
+    ac[4 + 5 * (1 + m)] += c * dA[9 + m];
+  }
+  ac[4] += c * dA[12];
+  ac[24] += c * dA[13];
+}
+
 struct BlkArgs {
   const int4* lanes;       // per lane: block-list offset, CSR entry, count,
                            // flags (bit 0 leader, bit 1 diagonal, bits 8..15
@@ -1245,7 +1351,6 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
     const R W = (R)g16[12];
     // acc += WdetJ [P Q] per point: the stored tau is WdetJ tau and N_a is
     // scaled here, so P's rows come out multiplied by WdetJ
-    auto add = [&](int idx, R val) { acc[idx] += val; };
     R ub[3] = {(R)0, (R)0, (R)0};  // sum of the point velocities
     // kDC: sum_q DC_q A0(q) through its distinct entries: drdp, drdT,
     // drdp u (3), drdT u (3), rho, rho u (3), e1p, e4p
@@ -1284,90 +1389,17 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
         } else {
           dq = (R)A.dcw[(size_t)e * 4 + q];
         }
-        const R dp = dq * drdp, dT = dq * drdT, dr = dq * rho;
-        dA[0] += dp;
-        dA[1] += dT;
-#pragma unroll
-        for (int n = 0; n < 3; ++n) {
-          dA[2 + n] += dp * uq[n];
-          dA[5 + n] += dT * uq[n];
-          dA[9 + n] += dr * uq[n];
-        }
-        dA[8] += dr;
-        dA[12] += dq * e1p;
-        dA[13] += dq * e4p;
+        dc_point<R>(dq, drdp, drdT, rho, uq, e1p, e4p, dA);
       }
       const R Na = W * (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
       const R Nb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
-      const R una = u0 * sR[0] + u1 * sR[1] + u2 * sR[2];
-      const R unb = u0 * rR[0] + u1 * rR[1] + u2 * rR[2];
-      const R w = unb + cfacR * Nb;
-      // Q = A(s_b) + c N_b A0, held in factored form: its rows 1..3 are
-      // drdp w u_k + r_k (column 0), rho (u_k r^T + w e_k^T) (columns 1..3)
-      // and drdT w u_k (column 4), so with pu = sum_k P(m,k) u_k and
-      // Sm = P(m,0) + pu they enter row m of the product as
-      //   column 0:    Sm q00 + sum_k P(m,k) r_k + P(m,4) q40
-      //   column 1+n:  Sm q0n + rho w P(m,1+n) + P(m,4) q4n
-      //   column 4:    Sm q04 + P(m,4) q44
-      // and only rows 0 and 4 of Q and rho w are formed
-      const R q00 = drdp * w, q04 = drdT * w, rw = rho * w;
-      R q0[3], q4[3];
-#pragma unroll
-      for (int n = 0; n < 3; ++n) {
-        q0[n] = rho * rR[n];
-        q4[n] = e3p * rR[n] + rw * uq[n];
-      }
-      const R q40 = e2p * unb + cfacR * Nb * e1p, q44 = e4p * w;
       if constexpr (kMT) {
         // rows of P = A(s_a) W PTau + N_a I from the point's W PTau
         R WPT[25];
         const S* mp = reinterpret_cast<const S*>(A.mtau) + ((size_t)e * 4 + q) * mt_stride<S>();
 #pragma unroll
         for (int k = 0; k < 25; ++k) WPT[k] = (R)mp[k];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          // row m of A(s_a), then P(m, :) = A(s_a)(m, :) W PTau + N_a e_m
-          R Am[5];
-          if (m == 0) {
-            Am[0] = drdp * una;
-#pragma unroll
-            for (int n = 0; n < 3; ++n) Am[1 + n] = rho * sR[n];
-            Am[4] = drdT * una;
-          } else if (m < 4) {
-            const int mm = m - 1;
-            Am[0] = drdp * uq[mm] * una + sR[mm];
-#pragma unroll
-            for (int n = 0; n < 3; ++n) Am[1 + n] = rho * (uq[mm] * sR[n] + (mm == n ? una : (R)0));
-            Am[4] = drdT * uq[mm] * una;
-          } else {
-            Am[0] = e2p * una;
-#pragma unroll
-            for (int n = 0; n < 3; ++n) Am[1 + n] = e3p * sR[n] + rho * uq[n] * una;
-            Am[4] = e4p * una;
-          }
-          R Pr[5];
-#pragma unroll
-          for (int j = 0; j < 5; ++j)
-            Pr[j] = Am[0] * WPT[j] + Am[1] * WPT[5 + j] + Am[2] * WPT[10 + j] +
-                    Am[3] * WPT[15 + j] + Am[4] * WPT[20 + j] + (m == j ? Na : (R)0);
-          const R Sm = Pr[0] + (Pr[1] * u0 + Pr[2] * u1 + Pr[3] * u2);
-          R& c0 = acc[m];
-          c0 += Sm * q00;
-          c0 += Pr[1] * rR[0];
-          c0 += Pr[2] * rR[1];
-          c0 += Pr[3] * rR[2];
-          c0 += Pr[4] * q40;
-#pragma unroll
-          for (int n = 0; n < 3; ++n) {
-            R& cn = acc[m + 5 * (1 + n)];
-            cn += Sm * q0[n];
-            cn += rw * Pr[1 + n];
-            cn += Pr[4] * q4[n];
-          }
-          R& c4 = acc[m + 20];
-          c4 += Sm * q04;
-          c4 += Pr[4] * q44;
-        }
+        mtau_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, WPT, cfacR, sR, rR, Na, Nb, acc);
         if constexpr (kNd) {
           // row m of A_k (the flux Jacobian along x_k, pressure-primitive)
           auto arow = [&](int k, int m, R r[5]) {
@@ -1467,18 +1499,7 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
 #pragma unroll
           for (int j = 0; j < 3; ++j) sab += sR[i] * gu[gu_idx(i, j)] * rR[j];
       }
-      const R c = W * sab;
-      add(0, c * dA[0]);
-      add(20, c * dA[1]);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        add(1 + m, c * dA[2 + m]);
-        add((1 + m) + 5 * (1 + m), c * dA[8]);
-        add((1 + m) + 20, c * dA[5 + m]);
-        add(4 + 5 * (1 + m), c * dA[9 + m]);
-      }
-      add(4, c * dA[12]);
-      add(24, c * dA[13]);
+      dc_block<R>(W * sab, dA, acc);
     }
   }
   // the node's residual: element residuals of the diagonal blocks' elements
@@ -1564,13 +1585,16 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
   }
 }
 
-// The block pass with lhsK for Shakib's diagonal tau, without DC: the
-// entries (A,B) and (B,A) come from the same element blocks with a and b
-// exchanged, so one lane forms both from one read of the point records
-// (the pass is bound by those per-lane reads).  Diagonal entries run in
-// wavefronts of their own, so every lane of a wavefront forms the same
-// number of blocks.  Same arithmetic per block as k_asm_blocks.
-template <typename T, typename S>
+// The block pass with lhsK: the entries (A,B) and (B,A) come from the same
+// element blocks with a and b exchanged, so one lane forms both from one
+// read of the point records (the pass is bound by those per-lane reads).
+// Diagonal entries run in wavefronts of their own, so every lane of a
+// wavefront forms the same number of blocks.  Same arithmetic per block as
+// k_asm_blocks; with DC (kDC) the point weights and s_a.giju s_b are
+// symmetric in a and b and serve both blocks; the matrix tau (kMT) through
+// mtau_pq.  Used for every flag set in float, for Shakib's or Franca's tau
+// without DC in double (the other double variants exceed the registers).
+template <typename T, typename S, bool kDC = false, bool kMT = false>
 __global__ void __launch_bounds__(64) k_asm_pairs(BlkArgs A) {
   using R = T;
   const long long per = (gridDim.x + 7) / 8;
@@ -1606,11 +1630,16 @@ __global__ void __launch_bounds__(64) k_asm_pairs(BlkArgs A) {
     }
     const R W = (R)g16[12];
     R ub[3] = {(R)0, (R)0, (R)0};
+    R dA[14];
+    if constexpr (kDC) {
+#pragma unroll
+      for (int i = 0; i < 14; ++i) dA[i] = (R)0;
+    }
     const S* qrec = qps + (size_t)e * 4 * pt_stride<S>();
     // float: two points' record loads in flight (136 VGPRs, three
     // wavefronts per SIMD: 8.23 ms at 128^3 against 8.51 not unrolled);
-    // double: the unrolled body exceeds the register file
-    constexpr int kPairUnroll = sizeof(S) == 4 ? 2 : 1;
+    // double and the DC / matrix-tau variants: not unrolled (registers)
+    constexpr int kPairUnroll = (sizeof(S) == 4 && !kDC && !kMT) ? 2 : 1;
 #pragma unroll kPairUnroll
     for (int q = 0; q < 4; ++q) {
       S o[12];
@@ -1624,14 +1653,60 @@ __global__ void __launch_bounds__(64) k_asm_pairs(BlkArgs A) {
       const R t0 = (R)o[kPtT0], wt1 = (R)o[kPtT1], wt2 = (R)o[kPtT2];
       const R Nqa = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], a);
       const R Nqb = (R)sel4(cQ.N[q][0], cQ.N[q][1], cQ.N[q][2], cQ.N[q][3], b);
-      shakib_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, t0, wt1, wt2, cfacR, sR, rR, W * Nqa,
-                   Nqb, acc);
-      if (pair)
-        shakib_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, t0, wt1, wt2, cfacR, rR, sR,
-                     W * Nqb, Nqa, acc2);
+      if constexpr (kDC) {
+        R dq;
+        if constexpr (kMT) {
+          // s_a . (DC giju_q) s_b of the point's rotated giju
+          const double* d = A.dcw + ((size_t)e * 4 + q) * 8;
+          dq = (R)0;
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) dq += sR[i] * (R)d[gu_idx(i, j)] * rR[j];
+        } else {
+          dq = (R)A.dcw[(size_t)e * 4 + q];
+        }
+        dc_point<R>(dq, drdp, drdT, rho, uq, e1p, e4p, dA);
+      }
+      if constexpr (kMT) {
+        R WPT[25];
+        const S* mp = reinterpret_cast<const S*>(A.mtau) + ((size_t)e * 4 + q) * mt_stride<S>();
+#pragma unroll
+        for (int k = 0; k < 25; ++k) WPT[k] = (R)mp[k];
+        mtau_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, WPT, cfacR, sR, rR, W * Nqa, Nqb, acc);
+        if (pair)
+          mtau_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, WPT, cfacR, rR, sR, W * Nqb, Nqa,
+                     acc2);
+      } else {
+        shakib_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, t0, wt1, wt2, cfacR, sR, rR, W * Nqa,
+                     Nqb, acc);
+        if (pair)
+          shakib_pq<R>(uq, rho, drdp, drdT, e1p, e3p, e4p, t0, wt1, wt2, cfacR, rR, sR,
+                       W * Nqb, Nqa, acc2);
+      }
     }
     visc_add<R>(W, sR, rR, ub, muR, lamR, kapR, acc);
     if (pair) visc_add<R>(W, rR, sR, ub, muR, lamR, kapR, acc2);
+    if constexpr (kDC) {
+      R sab = (R)1;
+      if constexpr (!kMT) {
+        // giju of the element from dxi/dx = 2 (shape gradients of nodes 1..3)
+        R ix[3][3], G[6], trG, GG, gu[6];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) ix[kk][c] = (R)2 * (R)g16[3 * kk + c];
+        tet_metric_tensor(ix, G, trG, GG);
+        metric_inverse(G, gu);
+        sab = (R)0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) sab += sR[i] * gu[gu_idx(i, j)] * rR[j];
+      }
+      dc_block<R>(W * sab, dA, acc);
+      if (pair) dc_block<R>(W * sab, dA, acc2);
+    }
   }
   if (diag) {
     for (int k = 0; k < cnt; ++k) {
@@ -2574,9 +2649,9 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   // pass 2: block owners (res, BDiag, lhsK written once)
   BlkArgs B{};
   // without lhsK only the diagonal entries' owners have work (the node's
-  // residual and BDiag): run their lanes alone; with lhsK, Shakib's tau and
-  // no DC the paired lanes of k_asm_pairs
-  const bool paired = lhs && !dc && !mt;
+  // residual and BDiag): run their lanes alone; with lhsK the paired lanes
+  // of k_asm_pairs (in double for Shakib's or Franca's tau without DC)
+  const bool paired = lhs && (mixed || (!dc && !mt));
   B.lanes = paired ? c->lanes_pair : lhs ? c->lanes : c->lanes_diag;
   B.lanes_k2 = c->lanes_k2;
   B.blocks = c->blocks;
@@ -2600,7 +2675,13 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.kap = ph.mu * cp / ph.pr;
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   const int g2 = (int)(((B.nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
-  if (paired && mixed)
+  if (paired && mixed && dc && mt)
+    k_asm_pairs<float, float, true, true><<<g2, 64, 0, c->stream>>>(B);
+  else if (paired && mixed && dc)
+    k_asm_pairs<float, float, true, false><<<g2, 64, 0, c->stream>>>(B);
+  else if (paired && mixed && mt)
+    k_asm_pairs<float, float, false, true><<<g2, 64, 0, c->stream>>>(B);
+  else if (paired && mixed)
     k_asm_pairs<float, float><<<g2, 64, 0, c->stream>>>(B);
   else if (paired)
     k_asm_pairs<double, double><<<g2, 64, 0, c->stream>>>(B);

@@ -117,9 +117,10 @@ def test_dc_tangent_product_against_oracle(state, prec):
         gpu.close()
 
 
-def test_dc_mixed_against_fp64(state):
+@pytest.mark.parametrize("idc", [1, 2, 3])
+def test_dc_mixed_against_fp64(state, idc):
     case, y, ac = state
-    prm = dict(case.params, idc=1)
+    prm = dict(case.params, idc=idc)
     g64 = solver.GpuSolver(case, params=prm)
     gmx = solver.GpuSolver(case, params=prm, precision=solver.PHG_MIXED)
     try:

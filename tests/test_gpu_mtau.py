@@ -133,9 +133,10 @@ def test_mtau_tangent_product_against_oracle(state, prec):
         gpu.close()
 
 
-def test_mtau_mixed_against_fp64(state):
+@pytest.mark.parametrize("idc", [0, 1, 3])
+def test_mtau_mixed_against_fp64(state, idc):
     case, y, ac = state
-    prm = dict(case.params, itau=10)
+    prm = dict(case.params, itau=10, idc=idc)
     g64 = solver.GpuSolver(case, params=prm)
     gmx = solver.GpuSolver(case, params=prm, precision=solver.PHG_MIXED)
     try:
