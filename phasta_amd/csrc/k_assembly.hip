@@ -1594,8 +1594,8 @@ __global__ void __launch_bounds__(64) k_asm_blocks(BlkArgs A) {
 // symmetric in a and b and serve both blocks; the matrix tau (kMT) through
 // mtau_pq.  Used for every flag set in float, for Shakib's or Franca's tau
 // without DC in double (the other double variants exceed the registers).
-template <typename T, typename S, bool kDC = false, bool kMT = false>
-__global__ void __launch_bounds__(64) k_asm_pairs(BlkArgs A) {
+template <typename T, typename S, bool kDC, bool kMT>
+__device__ __forceinline__ void asm_pairs(const BlkArgs& A) {
   using R = T;
   const long long per = (gridDim.x + 7) / 8;
   const long long wg = (blockIdx.x % 8) * per + blockIdx.x / 8;
@@ -1806,6 +1806,18 @@ __global__ void __launch_bounds__(64) k_asm_pairs(BlkArgs A) {
     combine(acc2);
     if (leader && pair) store(k2, mb, ma, false);
   }
+}
+
+// Shakib's or Franca's tau without DC, float or double
+template <typename T, typename S>
+__global__ void __launch_bounds__(64) k_asm_pairs(BlkArgs A) {
+  asm_pairs<T, S, false, false>(A);
+}
+
+// DC and / or the matrix tau, mixed precision
+template <bool kDC, bool kMT>
+__global__ void __launch_bounds__(64) k_asm_pairs_dcmt(BlkArgs A) {
+  asm_pairs<float, float, kDC, kMT>(A);
 }
 
 // Wall faces: floating convective, pressure, viscous and heat fluxes
@@ -2650,7 +2662,8 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   BlkArgs B{};
   // without lhsK only the diagonal entries' owners have work (the node's
   // residual and BDiag): run their lanes alone; with lhsK the paired lanes
-  // of k_asm_pairs (in double for Shakib's or Franca's tau without DC)
+  // of k_asm_pairs / k_asm_pairs_dcmt (in double for Shakib's or Franca's
+  // tau without DC)
   const bool paired = lhs && (mixed || (!dc && !mt));
   B.lanes = paired ? c->lanes_pair : lhs ? c->lanes : c->lanes_diag;
   B.lanes_k2 = c->lanes_k2;
@@ -2676,11 +2689,11 @@ int launch_assembly(phg_ctx* c, bool lhs, bool prec) {
   B.cfac = ph.almi / ph.gami / ph.alfi * ph.Dtgl;
   const int g2 = (int)(((B.nlanes + 63) / 64 + 7) / 8 * 8);  // whole XCD rounds
   if (paired && mixed && dc && mt)
-    k_asm_pairs<float, float, true, true><<<g2, 64, 0, c->stream>>>(B);
+    k_asm_pairs_dcmt<true, true><<<g2, 64, 0, c->stream>>>(B);
   else if (paired && mixed && dc)
-    k_asm_pairs<float, float, true, false><<<g2, 64, 0, c->stream>>>(B);
+    k_asm_pairs_dcmt<true, false><<<g2, 64, 0, c->stream>>>(B);
   else if (paired && mixed && mt)
-    k_asm_pairs<float, float, false, true><<<g2, 64, 0, c->stream>>>(B);
+    k_asm_pairs_dcmt<false, true><<<g2, 64, 0, c->stream>>>(B);
   else if (paired && mixed)
     k_asm_pairs<float, float><<<g2, 64, 0, c->stream>>>(B);
   else if (paired)
